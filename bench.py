@@ -1,0 +1,185 @@
+"""Benchmark of the per-timestep simulation loop on one MI355X.
+
+Workload (default): 512 random-tree cells, compartment counts lognormal (median 100) clipped
+to 20-400, HH soma + passive dendrites, somatic current steps, dt 0.025 ms, 10,001 recorded
+rows (250 ms). A bench step = one full simulation of that workload from the initial state,
+inputs already resident in HBM (the library's reset rewinds state on the device).
+
+Prints one JSON line: compartment-updates/s (rows x compartments / s), the roofline of the
+dominant kernel (algorithmic bytes per launch over its HIP-event-timed duration), the
+oracle's CPU throughput on a bounded sample (cpu_baseline) and a trace-tolerance check.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "compartment-updates/s vs EDEN CPU at trace tol; adjoint grads/s vs forward"
+HBM_PEAK = 8.0e12
+TOL_MV = 1e-3
+
+# algorithmic bytes per compartment update (fp32 state), per unit of work:
+B_PASSIVE = 24   # V r+w 8, C 4, R to parent 4, parent index 4, g_leak*A 4
+B_HH = 56        # 24 + 3 gates r+w 24 + gNa*A, gK*A 8
+B_RECORD = 4     # recorded V per column-row
+
+
+def algorithmic_bytes(model, rows):
+    """Bytes per full run: per compartment-row 24 B (passive) or 56 B (HH), plus 4 B per
+    recorded column-row."""
+    ch_b = model.comp_chan_begin
+    n_chan = np.diff(ch_b)
+    hh = int((n_chan > 1).sum())
+    passive = model.n_comps - hh
+    return rows * (B_PASSIVE * passive + B_HH * hh + B_RECORD * len(model.record_comp))
+
+
+def build_workload(name):
+    from eden_amd import configs
+    if name == "config2":
+        m = configs.config2()
+        desc = ("config2: 512 random-tree cells, compartments lognormal(median 100) clipped 20-400, "
+                "HH soma + passive dendrites, dt 0.025 ms, 250 ms (10,001 rows)")
+    elif name == "config4":
+        m = configs.config4()
+        desc = ("config4: 4,096 random-tree cells, compartments log-uniform 10-2000, HH soma x U(0.5,1.5), "
+                "passive dendrites, dt 0.025 ms, 500 ms (20,001 rows)")
+    elif name == "config1":
+        m = configs.hh_population()
+        desc = "config1: 1,000 one-compartment HH cells + pulse, dt 0.025 ms, 100 ms (4,001 rows)"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return m, desc
+
+
+def cpu_baseline(model, threads, budget_s=10.0):
+    """The oracle (CPU restatement, OpenMP over cells) on a bounded sample of the same
+    workload: all cells, the first R rows, R chosen for about budget_s seconds."""
+    from oracle import oracle
+    total = model.total_rows()
+    probe = min(total, 100)
+    t0 = time.perf_counter()
+    oracle.run(model, n_rows=probe, threads=threads, want_trace=False)
+    dtp = time.perf_counter() - t0
+    rows = int(min(total, max(probe, probe * budget_s / max(dtp, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.run(model, n_rows=rows, threads=threads, want_trace=False)
+    el = time.perf_counter() - t0
+    # the 4 initialisation steps are included in the sample time and not counted as rows
+    return dict(value=model.n_comps * rows / el, unit="compartment-updates/s", cores=threads, kind="port",
+                sample=f"all {model.n_cells} cells, first {rows} of {total} rows ({el:.1f} s, OpenMP static)")
+
+
+def parity_sample(model_name, n_cells=16):
+    """Trace tolerance on a sub-population built the same way (first cells of the workload):
+    GPU vs oracle over all rows."""
+    from eden_amd import configs
+    from eden_amd.model import Simulation
+    from oracle import oracle
+    if model_name == "config2":
+        sizes = configs.config2_sizes()[:n_cells]
+        m = configs.random_trees(n_cells, sizes=sizes, t_final_ms=250.0, seed=2001)
+    elif model_name == "config4":
+        sizes = configs.config4_sizes()[:n_cells]
+        m = configs.random_trees(n_cells, sizes=sizes, t_final_ms=500.0, seed=4001, density_scale=True)
+    else:
+        m = configs.hh_population(n_cells=n_cells)
+    with Simulation(m) as s:
+        tg = s.run_all()
+        eg = s.read_events()
+    tr, ec, er, _ = oracle.run(m)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from parity import compare
+    ev_of = {int(c): i for i, c in enumerate(m.event_comp)}
+    rec2ev = [ev_of.get(int(c), 0) for c in m.record_comp]
+    res = compare(tg, eg, tr, (ec, er), len(m.event_comp), rec2ev)
+    return dict(cells=n_cells, rows=int(res["rows"]), max_abs_mV=res["max_abs_mV"],
+                raster_mismatch_cells=res["raster_mismatch_cols"], spikes=res["spikes_ref"],
+                within_tol=bool(res["max_abs_mV"] <= TOL_MV))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--threads-per-cell", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    from eden_amd.model import Simulation
+
+    print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
+    model, desc = build_workload(args.workload)
+    rows = model.total_rows()
+    sim = Simulation(model, threads_per_cell=args.threads_per_cell, profile=True)
+    tpb, k = sim.launch_shape()
+
+    def one_step():
+        sim.reset()
+        done = 0
+        while done < rows:
+            n = sim.run(-1)
+            if n == 0:
+                break
+            done += n
+        return done
+
+    for w in range(args.warmup):
+        one_step()
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    times, kms, launches = [], [], []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        one_step()
+        el = time.perf_counter() - t0
+        ms, nl, kname = sim.kernel_stats()
+        times.append(el)
+        kms.append(ms)
+        launches.append(nl)
+        print(f"[bench] step {s}: {el * 1e3:.2f} ms wall, {ms:.2f} ms in {nl} launch(es)", file=sys.stderr, flush=True)
+    sim.close()
+    t_step = float(np.mean(times))
+    updates = model.n_comps * rows
+    value = updates / t_step
+    per_launch_ms = float(np.sum(kms) / max(1, np.sum(launches)))
+    launches_per_step = float(np.mean(launches))
+    bytes_per_launch = algorithmic_bytes(model, rows) / launches_per_step
+    achieved = bytes_per_launch / (per_launch_ms * 1e-3)
+    roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / HBM_PEAK, traffic=None, kernel=kname,
+                    bytes_per_launch=bytes_per_launch, launch_ms=per_launch_ms)
+    out = {
+        "metric": METRIC, "value": value, "unit": "compartment-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "cells": model.n_cells, "compartments": model.n_comps, "rows": rows,
+                   "launch_shape": {"threads_per_cell": tpb, "compartments_per_thread": k}},
+        "roofline": roofline,
+    }
+    if not args.no_parity:
+        print("[bench] parity sample", file=sys.stderr, flush=True)
+        out["parity"] = parity_sample(args.workload)
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        cb = cpu_baseline(model, threads, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

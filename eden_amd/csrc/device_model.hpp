@@ -1,0 +1,348 @@
+// device_model.hpp -- device-side tables and the per-compartment mechanism step shared by
+// the point-cell and tree-cell kernels. Built for gfx950 only.
+//
+// The per-compartment math restates the C that EDEN generates per cell type:
+//   gate rate forms            eden/Eden.cpp:5126-5200
+//   cnexp update + clipping     eden/Eden.cpp:5236-5259, 5336-5375
+//   open fraction, density      eden/Eden.cpp:5679-5721, 5805-5898
+//   pulse input                 eden/Eden.cpp:3798-3826
+//   synapse trigger/delay rule  eden/Eden.cpp:3636-3707
+//   expTwoSynapse dynamics      eden/Eden.cpp:2614-2994, LEMS_CoreComponents.inc.xml:145-180
+//   Poisson firing synapse      eden/Eden.cpp:3859-3912, LEMS_CoreComponents.inc.xml:623-672
+//   membrane RHS and diagonal   eden/Eden.cpp:6057-6107
+// with the same operation order; kernels are built with -ffp-contract=off and exp/log are
+// rounded once from double (exp_f, log_f), so results match the CPU path bit for bit
+// except in rare exp/log rounding cases.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace edenhip {
+
+constexpr int NCH = 4;      // channel instances per compartment (register slots)
+constexpr int NQ = 4;       // gate state variables per compartment (register slots)
+constexpr int GATES_PER_TYPE = 4;
+constexpr int MAX_CHAN_TYPES = 64;
+
+enum RateForm : int32_t { RATE_EXP = 0, RATE_EXPLINEAR = 1, RATE_SIGMOID = 2, RATE_FIXED = 3 };
+
+// One gate of a channel type, 32 bytes (two ds_read_b128 from LDS).
+struct alignas(16) DGate {
+    int32_t forms;   // alpha form | beta form << 8 | instances << 16
+    float q10;
+    float ar, am;
+    float as, br;
+    float bm, bs;
+};
+
+// Per expTwoSynapse type, with the waveform factor precomputed on the host in fp32 with
+// the reference's operation order (peakTime, waveformFactor are derived parameters).
+struct alignas(16) DSynType {
+    float gbase, erev, tau_rise, tau_decay;
+    float weight_prop, wf, pad0, pad1;
+};
+
+// Flat pointers to every device table (one struct passed by value to kernels).
+struct DevTables {
+    // compartments
+    float *V;               // [NC] membrane voltage (state)
+    float *q;               // [NC*NQ] gate states (state)
+    const float *C, *A, *invrc, *vthr;
+    const uint32_t *chmeta; // [NC] 4 x 8-bit channel type (0xFF = empty)
+    const uint32_t *gmeta;  // [NC] gate slot map: slot s: bits 4s..4s+3 = (chan slot | gate idx << 2);
+                            //      bits 24..26 = gate count, bits 28..30 = channel count
+    const float *chg, *che; // [NC*NCH] channel Gbase (mS/cm2), Erev (mV)
+    const int32_t *pulse_begin;   // [NC+1]
+    const float *pulse_start, *pulse_end, *pulse_amp;  // start, start+duration, Imax*weight
+    const int32_t *rec_col; // [NC] device trace column or -1
+    const int32_t *ev_col;  // [NC] event column or -1
+    // chemical synapses (CSR over compartments)
+    const int32_t *syn_begin;     // [NC+1]
+    const int32_t *syn_type;
+    const float *syn_weight, *syn_delay;
+    float *syn_next, *syn_A, *syn_B;   // state
+    int32_t *trig[2];                  // trigger flags, double-buffered
+    const int32_t *spk_begin;          // [NC+1]
+    const int32_t *spk_target;
+    // Poisson inputs (CSR over compartments)
+    const int32_t *poi_begin;     // [NC+1]
+    const float *poi_isi_mean;    // averageIsi = 1 / averageRate (host fp32)
+    const float *poi_weight;
+    const int32_t *poi_syn_type, *poi_inst, *poi_roff;
+    float *poi_tsince, *poi_isi, *poi_A, *poi_B;   // state
+    const uint32_t *cell_seed;    // [n_cells]
+    const int32_t *comp_cell;     // [NC] owning cell
+    // time
+    const float *t_step;          // [rows] time_f32 of each recorded step
+    // outputs
+    float *trace;                 // [chunk_rows * n_trace_cols]
+    int32_t n_trace_cols;
+    int64_t *events;              // [max_events] packed (row << 24 | col)
+    uint32_t *n_events;
+    uint32_t max_events;
+    int64_t total_rows;
+};
+
+// fp32 exp and log rounded once from a double evaluation: correctly rounded except when the
+// double result lies within one double ulp of a float rounding boundary (about 2^-28 of
+// arguments). glibc's expf/logf, which the reference's generated C calls, are within
+// 0.502 ulp, so both sides agree except in those rare cases. Building with
+// -DEDENHIP_NATIVE_EXPF selects the device's own expf/logf instead.
+__device__ __forceinline__ float exp_f(float x) {
+#ifdef EDENHIP_NATIVE_EXPF
+    return expf(x);
+#else
+    return (float)exp((double)x);
+#endif
+}
+
+__device__ __forceinline__ float log_f(float x) {
+#ifdef EDENHIP_NATIVE_EXPF
+    return logf(x);
+#else
+    return (float)log((double)x);
+#endif
+}
+
+__device__ __forceinline__ float rate_eval(int form, float r, float mid, float scale, float V) {
+    if (form == RATE_EXP) return r * exp_f((V - mid) / scale);
+    if (form == RATE_EXPLINEAR) {
+        return r * ((V == mid) ? 1.0f : (((V - mid) / scale) / (1.0f - exp_f(-(V - mid) / scale))));
+    }
+    if (form == RATE_SIGMOID) return r / (1.0f + exp_f((mid - V) / scale));
+    return r;
+}
+
+// Thomas Wang's 64-bit integer hash (published 1997); EDEN's RNG mixer
+// (eden/Eden.cpp:3200-3212).
+__device__ __forceinline__ uint64_t hash64shift(uint64_t key) {
+    key = (~key) + (key << 21);
+    key = key ^ (key >> 24);
+    key = (key + (key << 3)) + (key << 8);
+    key = key ^ (key >> 14);
+    key = (key + (key << 2)) + (key << 4);
+    key = key ^ (key >> 28);
+    key = key + (key << 31);
+    return key;
+}
+
+// randof (eden/Eden.cpp:3213-3233): counter-based uniform in [0, x).
+__device__ __forceinline__ float randof(float x, int32_t work_item, int64_t instance, int64_t step, int32_t invocation) {
+    uint64_t hi = (uint64_t)(int64_t)work_item * (1ULL << 24) | ((uint64_t)instance % (1ULL << 24));
+    uint64_t lo = (uint64_t)(int64_t)invocation * (1ULL << 40) | ((uint64_t)step % (1ULL << 40));
+    uint64_t s = hash64shift(hash64shift(lo) ^ hi);
+    float r = ((float)(s % (1ULL << 23))) / ((float)(1 << 23));
+    return x * r;
+}
+
+// Per-compartment constants cached in registers for the whole time loop.
+// gmeta bit 27: the compartment has pulse inputs (the first one is cached here); bit 31: it
+// has more than one (the rest are read from global memory).
+constexpr uint32_t GM_PULSE = 1u << 27;
+constexpr uint32_t GM_MORE_PULSES = 1u << 31;
+
+struct CompConst {
+    float C, A, invrc, vthr;
+    uint32_t chmeta, gmeta;
+    float g[NCH], e[NCH];
+    float p0s, p0e, p0a;   // first pulse: start, start + duration, Imax * weight
+    int32_t rec, ev;
+};
+
+__device__ __forceinline__ void load_comp_const(const DevTables &T, int64_t gidx, CompConst &k) {
+    k.C = T.C[gidx]; k.A = T.A[gidx]; k.invrc = T.invrc[gidx]; k.vthr = T.vthr[gidx];
+    k.chmeta = T.chmeta[gidx]; k.gmeta = T.gmeta[gidx];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) { k.g[c] = T.chg[gidx * NCH + c]; k.e[c] = T.che[gidx * NCH + c]; }
+    const int32_t pb = T.pulse_begin[gidx], pe = T.pulse_begin[gidx + 1];
+    if (pe > pb) {
+        k.p0s = T.pulse_start[pb]; k.p0e = T.pulse_end[pb]; k.p0a = T.pulse_amp[pb];
+        k.gmeta |= GM_PULSE;
+        if (pe > pb + 1) k.gmeta |= GM_MORE_PULSES;
+    } else {
+        k.p0s = 0.f; k.p0e = -1.f; k.p0a = 0.f;
+    }
+    k.rec = T.rec_col[gidx]; k.ev = T.ev_col[gidx];
+}
+
+// Channels: gate updates (in place, after each gate's current value has entered the open
+// fraction, eden/Eden.cpp:5330) and channel current/conductance sums in distribution order.
+__device__ __forceinline__ void channels_step(const DGate *__restrict__ gates, const CompConst &k, float V,
+                                              float (&q)[NQ], bool init, float dt, float &I_ch, float &G_ch) {
+    const int nq = (k.gmeta >> 24) & 7;
+    const int nch = (k.gmeta >> 28) & 7;
+    float fop[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) fop[c] = 1.0f;   // conductance_scaling = 1
+#pragma unroll
+    for (int s = 0; s < NQ; s++) {
+        if (s < nq) {
+            const int cs = (k.gmeta >> (4 * s)) & 3;
+            const int gi = (k.gmeta >> (4 * s + 2)) & 3;
+            const int type = (k.chmeta >> (8 * cs)) & 0xFF;
+            const DGate G = gates[type * GATES_PER_TYPE + gi];
+            const float alpha = rate_eval(G.forms & 0xFF, G.ar, G.am, G.as, V);
+            const float beta = rate_eval((G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
+            const float tau = 1.0f / (alpha + beta);
+            const float inf = alpha / (alpha + beta);
+            const float qs = q[s];
+            float nv;
+            if (init) {
+                nv = inf;
+            } else {
+                const float tau_factor = ((dt * G.q10) / tau);
+                const float blend = exp_f(-tau_factor);
+                nv = (blend) * qs + (1.0f - blend) * inf;
+                if (!(nv > 1e-6f)) nv = 1e-6f;
+                if (!(nv < (float)(1 - 1e-6))) nv = (float)(1 - 1e-6);
+            }
+            const int inst = (G.forms >> 16) & 0xFF;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                if (c == cs) {
+                    float p = fop[c];
+                    for (int i = 0; i < inst; i++) p = p * qs;
+                    fop[c] = p;
+                }
+            }
+            q[s] = nv;
+        }
+    }
+    I_ch = 0.0f; G_ch = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        if (c < nch) {
+            const float Gscaled = k.g[c] * fop[c];
+            const float iDensity = Gscaled * (k.e[c] - V) * 1e-5f;
+            const float gDensity = (Gscaled) * 1e-5f;
+            I_ch += (iDensity * k.A);
+            G_ch += (gDensity * k.A);
+        }
+    }
+}
+
+__device__ __forceinline__ float pulses_current(const DevTables &T, const CompConst &k, int64_t gidx, float t) {
+    float I = 0.0f;
+    if (k.gmeta & GM_PULSE) {
+        float Ip = 0.0f;
+        if (k.p0s <= t && t <= k.p0e) Ip += k.p0a;
+        if (k.gmeta & GM_MORE_PULSES) {
+            const int32_t pb = T.pulse_begin[gidx], pe = T.pulse_begin[gidx + 1];
+            for (int p = pb + 1; p < pe; p++)
+                if (T.pulse_start[p] <= t && t <= T.pulse_end[p]) Ip += T.pulse_amp[p];
+        }
+        I += Ip;
+        I += 0.0f;
+    }
+    return I;
+}
+
+// expTwoSynapse LEMS dynamics for one instance: exposures from the current state, then
+// Euler decay, or on an event the OnEvent assignment that overrides the decay.
+__device__ __forceinline__ void exptwo_step(const DSynType &st, float V, float &A, float &B, bool init, bool ev,
+                                            float dt, float &i_out, float &g_out) {
+    const float g = st.gbase * (B - A);
+    const float i = g * (st.erev - V);
+    float An, Bn;
+    if (init) {
+        An = 0.0f; Bn = 0.0f;
+    } else {
+        const float dA = (-A) / st.tau_rise;
+        An = A + dt * dA;
+        const float dB = (-B) / st.tau_decay;
+        Bn = B + dt * dB;
+        if (ev) {
+            An = A + (st.weight_prop * st.wf);
+            Bn = B + (st.weight_prop * st.wf);
+        }
+    }
+    A = An; B = Bn;
+    i_out = i; g_out = g;
+}
+
+// All chemical synapses of a compartment: trigger read + clear, single-slot delay rule,
+// expTwo dynamics, per-type aggregation in list order (eden/Eden.cpp:3636-3740).
+__device__ __forceinline__ void synapses_step(const DevTables &T, const DSynType *__restrict__ stypes, int64_t gidx,
+                                              float V, bool init, float t, float dt, int now,
+                                              float &I_syn, float &G_syn) {
+    I_syn = 0.0f; G_syn = 0.0f;
+    const int32_t b = T.syn_begin[gidx], e = T.syn_begin[gidx + 1];
+    if (e <= b) return;
+    float I_agg = 0.0f, G_agg = 0.0f;
+    int32_t cur = -1;
+    int32_t *trig_now = T.trig[now];
+    for (int32_t s = b; s < e; s++) {
+        const int32_t ty = T.syn_type[s];
+        if (ty != cur) {
+            if (cur >= 0) { I_syn += I_agg; G_syn += G_agg; }
+            I_agg = 0.0f; G_agg = 0.0f; cur = ty;
+        }
+        const float weight = T.syn_weight[s];
+        bool flag = false;
+        if (!init) {
+            flag = trig_now[s] != 0;
+            if (flag) trig_now[s] = 0;
+        }
+        const float delay = T.syn_delay[s];
+        const float next_spike = T.syn_next[s];
+        float nn = next_spike;
+        bool spike_now = false;
+        if (!init) {
+            if (t <= next_spike && next_spike < t + dt) spike_now = true;
+            if (flag) {
+                const float fresh = t + delay;
+                if (t <= fresh && fresh < t + dt) spike_now = true;
+                if (nn < t + dt && nn < fresh) nn = fresh;
+            }
+        }
+        T.syn_next[s] = nn;
+        const DSynType st = stypes[ty];
+        float A = T.syn_A[s], B = T.syn_B[s], i, g;
+        exptwo_step(st, V, A, B, init, spike_now, dt, i, g);
+        T.syn_A[s] = A; T.syn_B[s] = B;
+        I_agg += i * weight;
+        G_agg += g * weight;
+    }
+    I_syn += I_agg; G_syn += G_agg;
+}
+
+// Poisson firing synapse inputs of a compartment (LEMS OnStart / OnCondition with the hash
+// RNG), each driving its own expTwo synapse in the same step.
+__device__ __forceinline__ void poisson_step(const DevTables &T, const DSynType *__restrict__ stypes, int64_t gidx,
+                                             float V, bool init, int64_t step, float dt, float &I_in, float &G_in) {
+    const int32_t b = T.poi_begin[gidx], e = T.poi_begin[gidx + 1];
+    if (e <= b) return;
+    const int32_t seed = (int32_t)T.cell_seed[T.comp_cell[gidx]];
+    float I_agg = 0.0f, G_agg = 0.0f;
+    for (int32_t p = b; p < e; p++) {
+        const float weight = T.poi_weight[p];
+        const float tsince = T.poi_tsince[p], isi = T.poi_isi[p];
+        const float averageIsi = T.poi_isi_mean[p];
+        const int32_t inst = T.poi_inst[p], roff = T.poi_roff[p];
+        bool fired = false;
+        float tn, in;
+        if (init) {
+            tn = 0.0f;
+            in = (-averageIsi) * log_f(1.0f - randof(1.0f, seed, inst, step, roff + 0));
+        } else {
+            tn = tsince + dt * 1.0f;
+            in = isi;
+            if (tsince > isi) {
+                tn = 0.0f;
+                in = (-averageIsi) * log_f(1.0f - randof(1.0f, seed, inst, step, roff + 1));
+                fired = true;
+            }
+        }
+        T.poi_tsince[p] = tn; T.poi_isi[p] = in;
+        const DSynType st = stypes[T.poi_syn_type[p]];
+        float A = T.poi_A[p], B = T.poi_B[p], i, g;
+        exptwo_step(st, V, A, B, init, fired, dt, i, g);
+        T.poi_A[p] = A; T.poi_B[p] = B;
+        I_agg += i * weight;
+        G_agg += g * weight;
+    }
+    I_in += I_agg;
+    G_in += G_agg;
+}
+
+}  // namespace edenhip

@@ -1,0 +1,670 @@
+// edenhip.hip -- host side of the C-ABI (include/edenhip.h): builds the device tables from the
+// instantiated model, plans launches, runs chunks of steps and returns recorded outputs.
+//
+// Replaces, for the GPU path: RawTables instantiation and the initial-state copy
+// (eden/Eden.cpp:7643-9281, 11093-11150), the OpenMP dispatch of generated callbacks
+// (eden/Eden.cpp:11848-11869) and the step/swap logic of the engine loop
+// (eden/Eden.cpp:11486-11938).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/edenhip.h"
+#include "device_model.hpp"
+#include "kernels.hpp"
+#include "tree_schedule.hpp"
+
+using namespace edenhip;
+
+namespace {
+
+struct Bucket {
+    int tpb = 64;
+    std::vector<int32_t> order;    // tree-cell indices, largest first
+    int32_t nmax = 0;
+    int32_t words_max = 0;
+    size_t lds = 0;
+    int grid = 0;
+    int32_t *d_order = nullptr;
+    int32_t *d_queue = nullptr;
+    TreeCell *d_cells = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+}  // namespace
+
+struct edenhip_ctx {
+    std::string err;
+    edenhip_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float dt = 0;
+    int64_t NC = 0, total_rows = 0, rows_done = 0, next_step = -3;
+    int n_cells = 0;
+    bool network = false;
+    // host mirrors for output mapping
+    std::vector<int32_t> desc_to_dev_col;   // per desc trace column
+    int32_t n_dev_cols = 0;
+    std::vector<std::vector<int64_t>> ev_expand;   // dev event column -> desc columns
+    int64_t chunk_rows = 0, last_row0 = 0, last_rows = 0;
+    DevTables T{};
+    DGate *d_gates = nullptr;
+    int n_gates = 0;
+    DSynType *d_stypes = nullptr;
+    int n_stypes = 0;
+    // point cells
+    int32_t *d_point = nullptr;
+    int32_t n_point = 0;
+    // tree cells
+    uint32_t *d_words = nullptr;
+    int32_t *d_pos = nullptr;
+    float2 *d_u = nullptr;
+    std::vector<Bucket> buckets;
+    hipEvent_t ev_start = nullptr;
+    int n_cu = 256;
+    // initial copies of the state tables (for edenhip_reset)
+    struct Snap { void *dst; void *src; size_t bytes; };
+    std::vector<Snap> snaps;
+    // profiling
+    double kernel_ms = 0;
+    int64_t launches = 0;
+    std::string kernel_name;
+    std::vector<void *> allocs;
+};
+
+#define HIPCHECK(ctx, call)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+            return EDENHIP_E_DEVICE;                                                              \
+        }                                                                                         \
+    } while (0)
+
+template <typename T>
+static int upload(edenhip_ctx *ctx, T **dptr, const std::vector<T> &v) {
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    void *p = nullptr;
+    HIPCHECK(ctx, hipMalloc(&p, bytes));
+    ctx->allocs.push_back(p);
+    if (!v.empty()) HIPCHECK(ctx, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    else HIPCHECK(ctx, hipMemset(p, 0, bytes));
+    *dptr = static_cast<T *>(p);
+    return 0;
+}
+
+template <typename T>
+static int alloc_zero(edenhip_ctx *ctx, T **dptr, size_t n) {
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    void *p = nullptr;
+    HIPCHECK(ctx, hipMalloc(&p, bytes));
+    ctx->allocs.push_back(p);
+    HIPCHECK(ctx, hipMemset(p, 0, bytes));
+    *dptr = static_cast<T *>(p);
+    return 0;
+}
+
+#define TRY(x)                         \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_ != 0) return rc_;      \
+    } while (0)
+
+static int fail(edenhip_ctx *ctx, int code, const std::string &msg) {
+    ctx->err = msg;
+    return code;
+}
+
+// time_f32 of every recorded step: the reference accumulates `time += dt` in double from
+// t_initial = 0 and casts to float per step (eden/Eden.cpp:11930-11932, 3252).
+static std::vector<float> step_times(float dt, double t_final) {
+    std::vector<float> ts;
+    double time = 0;
+    while (time <= t_final) {
+        ts.push_back((float)time);
+        time += (double)dt;
+    }
+    return ts;
+}
+
+static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
+    if (d->abi_version != EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
+    if (!(d->dt > 0) || !(d->t_final >= 0)) return fail(ctx, EDENHIP_E_INVALID, "dt must be > 0 and t_final >= 0");
+    if (d->n_cells < 0 || d->n_comps < 0) return fail(ctx, EDENHIP_E_INVALID, "negative sizes");
+    const int64_t NC = d->n_comps;
+    if (NC >= (int64_t)1 << 31) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 compartments");
+    if (d->cell_comp_begin[0] != 0 || d->cell_comp_begin[d->n_cells] != NC)
+        return fail(ctx, EDENHIP_E_INVALID, "cell_comp_begin must start at 0 and end at n_comps");
+    if (d->n_channel_types > MAX_CHAN_TYPES) return fail(ctx, EDENHIP_E_LIMIT, "more than 64 channel types");
+    if (d->n_syn_types > 64) return fail(ctx, EDENHIP_E_LIMIT, "more than 64 synapse types");
+    ctx->NC = NC;
+    ctx->n_cells = d->n_cells;
+    ctx->dt = d->dt;
+
+    // channel type table
+    std::vector<DGate> gates((size_t)std::max(d->n_channel_types, 1) * GATES_PER_TYPE);
+    std::memset(gates.data(), 0, gates.size() * sizeof(DGate));
+    for (int t = 0; t < d->n_channel_types; t++) {
+        const edenhip_channel_type &ct = d->channel_types[t];
+        if (ct.n_gates < 0 || ct.n_gates > GATES_PER_TYPE) return fail(ctx, EDENHIP_E_LIMIT, "channel type with more than 4 gates");
+        for (int gi = 0; gi < ct.n_gates; gi++) {
+            const edenhip_gate &g = ct.gates[gi];
+            if (g.instances < 0 || g.instances > 255) return fail(ctx, EDENHIP_E_INVALID, "gate instances out of range");
+            for (const edenhip_rate *r : {&g.alpha, &g.beta})
+                if (r->form < 0 || r->form > EDENHIP_RATE_FIXED) return fail(ctx, EDENHIP_E_INVALID, "unknown rate form");
+            DGate &G = gates[(size_t)t * GATES_PER_TYPE + gi];
+            G.forms = g.alpha.form | (g.beta.form << 8) | (g.instances << 16);
+            G.q10 = g.q10;
+            G.ar = g.alpha.rate; G.am = g.alpha.midpoint; G.as = g.alpha.scale;
+            G.br = g.beta.rate; G.bm = g.beta.midpoint; G.bs = g.beta.scale;
+        }
+    }
+    ctx->n_gates = (int)gates.size();
+    TRY(upload(ctx, &ctx->d_gates, gates));
+
+    // synapse types; waveformFactor from the derived parameters with the reference's
+    // expression tree: peakTime = (log(td/tr) * (tr*td)) / (td - tr)
+    std::vector<DSynType> st((size_t)std::max(d->n_syn_types, 1));
+    std::memset(st.data(), 0, st.size() * sizeof(DSynType));
+    for (int t = 0; t < d->n_syn_types; t++) {
+        const edenhip_exptwo_type &s = d->syn_types[t];
+        const float tr = s.tau_rise, td = s.tau_decay;
+        const float peak = (logf(td / tr) * (tr * td)) / (td - tr);
+        const float wf = 1.0f / ((-expf((-peak) / tr)) + expf((-peak) / td));
+        st[t] = DSynType{s.gbase, s.erev, s.tau_rise, s.tau_decay, s.weight_prop, wf, 0.f, 0.f};
+    }
+    ctx->n_stypes = (int)st.size();
+    TRY(upload(ctx, &ctx->d_stypes, st));
+
+    // per-compartment packing
+    std::vector<float> V(NC), C(NC), A(NC), invrc(NC), vthr(NC), chg(NC * NCH, 0.f), che(NC * NCH, 0.f), q(NC * NQ, 0.f);
+    std::vector<uint32_t> chmeta(NC), gmeta(NC);
+    std::vector<int32_t> pulse_begin(NC + 1), syn_begin(NC + 1), spk_begin(NC + 1), poi_begin(NC + 1), comp_cell(NC);
+    for (int c = 0; c < d->n_cells; c++)
+        for (int64_t g = d->cell_comp_begin[c]; g < d->cell_comp_begin[c + 1]; g++) comp_cell[g] = c;
+    const int64_t n_syn = d->comp_syn_begin[NC], n_spk = d->comp_spike_begin[NC], n_poi = d->comp_poisson_begin[NC];
+    const int64_t n_pulse = d->comp_pulse_begin[NC];
+    if (n_syn >= (int64_t)1 << 31 || n_spk >= (int64_t)1 << 31 || n_poi >= (int64_t)1 << 31)
+        return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapses or spike targets");
+    for (int64_t g = 0; g < NC; g++) {
+        V[g] = d->v0[g]; C[g] = d->capacitance[g]; A[g] = d->area[g]; invrc[g] = d->inv_rc_axial[g];
+        vthr[g] = d->v_threshold ? d->v_threshold[g] : NAN;
+        const int64_t cb = d->comp_chan_begin[g], ce = d->comp_chan_begin[g + 1];
+        if (ce - cb > NCH) return fail(ctx, EDENHIP_E_LIMIT, "compartment with more than 4 channel distributions");
+        uint32_t cm = 0xFFFFFFFFu, gm = 0;
+        int nq = 0;
+        for (int64_t ch = cb; ch < ce; ch++) {
+            const int slot = (int)(ch - cb);
+            const int ty = d->chan_type[ch];
+            if (ty < 0 || ty >= d->n_channel_types) return fail(ctx, EDENHIP_E_INVALID, "channel type index out of range");
+            cm = (cm & ~(0xFFu << (8 * slot))) | ((uint32_t)ty << (8 * slot));
+            chg[g * NCH + slot] = d->chan_gbase[ch];
+            che[g * NCH + slot] = d->chan_erev[ch];
+            const edenhip_channel_type &ct = d->channel_types[ty];
+            for (int gi = 0; gi < ct.n_gates; gi++) {
+                if (nq >= NQ) return fail(ctx, EDENHIP_E_LIMIT, "compartment with more than 4 gate variables");
+                gm |= (uint32_t)(slot | (gi << 2)) << (4 * nq);
+                // setup value alpha/(alpha+beta) at V0 (eden/Eden.cpp:5313-5321); the
+                // initialisation steps overwrite it with inf(V0)
+                const double a = edenhip_rate_host(ct.gates[gi].alpha, d->v0[g]);
+                const double b = edenhip_rate_host(ct.gates[gi].beta, d->v0[g]);
+                q[g * NQ + nq] = (float)(a / (a + b));
+                nq++;
+            }
+        }
+        gm |= (uint32_t)nq << 24;
+        gm |= (uint32_t)(ce - cb) << 28;
+        chmeta[g] = cm; gmeta[g] = gm;
+        pulse_begin[g + 1] = (int32_t)d->comp_pulse_begin[g + 1];
+        syn_begin[g + 1] = (int32_t)d->comp_syn_begin[g + 1];
+        spk_begin[g + 1] = (int32_t)d->comp_spike_begin[g + 1];
+        poi_begin[g + 1] = (int32_t)d->comp_poisson_begin[g + 1];
+    }
+    std::vector<float> pstart(n_pulse), pend(n_pulse), pamp(n_pulse);
+    for (int64_t p = 0; p < n_pulse; p++) {
+        pstart[p] = d->pulse_start[p];
+        pend[p] = d->pulse_start[p] + d->pulse_duration[p];       // Start + Duration (fp32)
+        pamp[p] = d->pulse_amplitude[p] * d->pulse_weight[p];     // Imax * Weight (fp32)
+    }
+    std::vector<int32_t> syn_type(n_syn), spk_target(n_spk);
+    std::vector<float> syn_weight(n_syn), syn_delay(n_syn), syn_next(n_syn, -INFINITY);
+    for (int64_t s = 0; s < n_syn; s++) {
+        syn_type[s] = d->syn_type[s];
+        if (syn_type[s] < 0 || syn_type[s] >= d->n_syn_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
+        syn_weight[s] = d->syn_weight[s];
+        syn_delay[s] = d->syn_delay[s];
+    }
+    for (int64_t k = 0; k < n_spk; k++) {
+        if (d->spike_target[k] < 0 || d->spike_target[k] >= n_syn) return fail(ctx, EDENHIP_E_INVALID, "spike target out of range");
+        spk_target[k] = (int32_t)d->spike_target[k];
+    }
+    // synapse lists must be grouped by type in ascending type order per compartment
+    for (int64_t g = 0; g < NC; g++)
+        for (int64_t s = d->comp_syn_begin[g] + 1; s < d->comp_syn_begin[g + 1]; s++)
+            if (d->syn_type[s] < d->syn_type[s - 1]) return fail(ctx, EDENHIP_E_INVALID, "synapses of a compartment must be sorted by type");
+    std::vector<float> poi_mean(n_poi), poi_w(n_poi);
+    std::vector<int32_t> poi_st(n_poi), poi_inst(n_poi), poi_roff(n_poi);
+    for (int64_t p = 0; p < n_poi; p++) {
+        poi_mean[p] = 1.0f / d->poisson_rate[p];
+        poi_w[p] = d->poisson_weight[p];
+        poi_st[p] = d->poisson_syn_type[p];
+        if (poi_st[p] < 0 || poi_st[p] >= d->n_syn_types) return fail(ctx, EDENHIP_E_INVALID, "poisson synapse type out of range");
+        poi_inst[p] = d->poisson_instance[p];
+        poi_roff[p] = d->poisson_rng_offset[p];
+    }
+    std::vector<uint32_t> seeds(std::max(d->n_cells, 1));
+    for (int c = 0; c < d->n_cells; c++) seeds[c] = d->cell_rng_seed ? d->cell_rng_seed[c] : 0u;
+
+    // recording maps
+    std::vector<int32_t> rec_col(NC, -1), ev_col(NC, -1);
+    ctx->desc_to_dev_col.assign(d->n_record, -1);
+    ctx->n_dev_cols = 0;
+    for (int64_t r = 0; r < d->n_record; r++) {
+        int64_t g = d->record_comp[r];
+        if (g < 0 || g >= NC) return fail(ctx, EDENHIP_E_INVALID, "record_comp out of range");
+        if (rec_col[g] < 0) rec_col[g] = ctx->n_dev_cols++;
+        ctx->desc_to_dev_col[r] = rec_col[g];
+    }
+    if (d->n_event_cols >= (int64_t)1 << 24) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^24 event columns");
+    ctx->ev_expand.assign((size_t)d->n_event_cols, {});
+    for (int64_t e = 0; e < d->n_event_cols; e++) {
+        int64_t g = d->event_comp[e];
+        if (g < 0 || g >= NC) return fail(ctx, EDENHIP_E_INVALID, "event_comp out of range");
+        if (ev_col[g] < 0) ev_col[g] = (int32_t)e;
+        ctx->ev_expand[ev_col[g]].push_back(e);
+    }
+
+    // time table
+    std::vector<float> ts = step_times(d->dt, d->t_final);
+    ctx->total_rows = (int64_t)ts.size();
+
+    // cells: point (1 compartment) vs tree
+    std::vector<int32_t> point;
+    std::vector<TreeCell> cells;
+    std::vector<int32_t> cell_tpb;
+    std::vector<uint32_t> words;
+    std::vector<int32_t> posv(NC, 0);
+    std::vector<float2> uv(NC, make_float2(0.f, 0.f));
+    for (int c = 0; c < d->n_cells; c++) {
+        const int64_t c0 = d->cell_comp_begin[c], c1 = d->cell_comp_begin[c + 1];
+        const int32_t n = (int32_t)(c1 - c0);
+        if (n <= 0) return fail(ctx, EDENHIP_E_INVALID, "cell without compartments");
+        if (d->parent[c0] != -1) return fail(ctx, EDENHIP_E_INVALID, "compartment 0 of a cell must be its root");
+        for (int32_t i = 1; i < n; i++)
+            if (d->parent[c0 + i] < 0 || d->parent[c0 + i] >= i) return fail(ctx, EDENHIP_E_INVALID, "parents must precede children");
+        if (n == 1) {
+            point.push_back((int32_t)c0);
+            continue;
+        }
+        if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2048 compartments");
+        for (int64_t g = c0; g < c1; g++)
+            if (d->comp_syn_begin[g + 1] > d->comp_syn_begin[g])
+                return fail(ctx, EDENHIP_E_LIMIT, "chemical synapses on multi-compartment cells are not supported yet");
+        const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell);
+        TreePlan P = plan_tree(n, d->parent + c0, tpb);
+        TreeCell tc{};
+        tc.comp_begin = c0;
+        tc.n = n;
+        tc.rounds_fwd = P.rounds_fwd;
+        tc.rounds_bwd = P.rounds_bwd;
+        tc.words_off = (int32_t)words.size();
+        words.insert(words.end(), P.words.begin(), P.words.end());
+        // edge constants by position, fp32 as computed per step by the reference:
+        // Ui = -dt/(R*C[i]), Uj = -dt/(R*C[j]) with R = R_Axial[max(i, j)] = R_Axial[i]
+        for (int32_t i = 0; i < n; i++) posv[c0 + i] = P.pos[i];
+        for (int32_t i = 1; i < n; i++) {
+            const float R = d->r_axial[c0 + i];
+            const int32_t j = d->parent[c0 + i];
+            const float ui = -d->dt / (R * d->capacitance[c0 + i]);
+            const float uj = -d->dt / (R * d->capacitance[c0 + j]);
+            uv[c0 + P.pos[i]] = make_float2(ui, uj);
+        }
+        cells.push_back(tc);
+        cell_tpb.push_back(tpb);
+    }
+    ctx->n_point = (int32_t)point.size();
+    bool any_targets = n_spk > 0;
+    ctx->network = any_targets;
+
+    // device uploads
+    DevTables &T = ctx->T;
+    TRY(upload(ctx, &T.V, V));
+    TRY(upload(ctx, &T.q, q));
+    TRY(upload(ctx, (float **)&T.C, C));
+    TRY(upload(ctx, (float **)&T.A, A));
+    TRY(upload(ctx, (float **)&T.invrc, invrc));
+    TRY(upload(ctx, (float **)&T.vthr, vthr));
+    TRY(upload(ctx, (uint32_t **)&T.chmeta, chmeta));
+    TRY(upload(ctx, (uint32_t **)&T.gmeta, gmeta));
+    TRY(upload(ctx, (float **)&T.chg, chg));
+    TRY(upload(ctx, (float **)&T.che, che));
+    TRY(upload(ctx, (int32_t **)&T.pulse_begin, pulse_begin));
+    TRY(upload(ctx, (float **)&T.pulse_start, pstart));
+    TRY(upload(ctx, (float **)&T.pulse_end, pend));
+    TRY(upload(ctx, (float **)&T.pulse_amp, pamp));
+    TRY(upload(ctx, (int32_t **)&T.rec_col, rec_col));
+    TRY(upload(ctx, (int32_t **)&T.ev_col, ev_col));
+    TRY(upload(ctx, (int32_t **)&T.syn_begin, syn_begin));
+    TRY(upload(ctx, (int32_t **)&T.syn_type, syn_type));
+    TRY(upload(ctx, (float **)&T.syn_weight, syn_weight));
+    TRY(upload(ctx, (float **)&T.syn_delay, syn_delay));
+    TRY(upload(ctx, &T.syn_next, syn_next));
+    TRY(alloc_zero(ctx, &T.syn_A, (size_t)n_syn));
+    TRY(alloc_zero(ctx, &T.syn_B, (size_t)n_syn));
+    TRY(alloc_zero(ctx, &T.trig[0], (size_t)n_syn));
+    TRY(alloc_zero(ctx, &T.trig[1], (size_t)n_syn));
+    TRY(upload(ctx, (int32_t **)&T.spk_begin, spk_begin));
+    TRY(upload(ctx, (int32_t **)&T.spk_target, spk_target));
+    TRY(upload(ctx, (int32_t **)&T.poi_begin, poi_begin));
+    TRY(upload(ctx, (float **)&T.poi_isi_mean, poi_mean));
+    TRY(upload(ctx, (float **)&T.poi_weight, poi_w));
+    TRY(upload(ctx, (int32_t **)&T.poi_syn_type, poi_st));
+    TRY(upload(ctx, (int32_t **)&T.poi_inst, poi_inst));
+    TRY(upload(ctx, (int32_t **)&T.poi_roff, poi_roff));
+    TRY(alloc_zero(ctx, &T.poi_tsince, (size_t)n_poi));
+    TRY(alloc_zero(ctx, &T.poi_isi, (size_t)n_poi));
+    TRY(alloc_zero(ctx, &T.poi_A, (size_t)n_poi));
+    TRY(alloc_zero(ctx, &T.poi_B, (size_t)n_poi));
+    TRY(upload(ctx, (uint32_t **)&T.cell_seed, seeds));
+    TRY(upload(ctx, (int32_t **)&T.comp_cell, comp_cell));
+    TRY(upload(ctx, (float **)&T.t_step, ts));
+    T.n_trace_cols = ctx->n_dev_cols;
+    T.total_rows = ctx->total_rows;
+
+    // trace chunk: all rows unless that exceeds 1 GiB or the caller asked for less
+    int64_t chunk = ctx->opt.chunk_steps > 0 ? ctx->opt.chunk_steps : ctx->total_rows;
+    const int64_t row_bytes = std::max<int64_t>(1, ctx->n_dev_cols) * 4;
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, ((int64_t)1 << 30) / row_bytes));
+    ctx->chunk_rows = chunk;
+    TRY(alloc_zero(ctx, &T.trace, (size_t)(chunk * std::max<int64_t>(1, ctx->n_dev_cols))));
+    T.max_events = 1u << 24;
+    TRY(alloc_zero(ctx, &T.events, (size_t)T.max_events));
+    TRY(alloc_zero(ctx, &T.n_events, 1));
+
+    // snapshot every state table so edenhip_reset can rewind without re-uploading
+    auto snap = [&](void *dst, size_t bytes) -> int {
+        void *p = nullptr;
+        HIPCHECK(ctx, hipMalloc(&p, std::max<size_t>(bytes, 4)));
+        ctx->allocs.push_back(p);
+        HIPCHECK(ctx, hipMemcpy(p, dst, std::max<size_t>(bytes, 4), hipMemcpyDeviceToDevice));
+        ctx->snaps.push_back({dst, p, std::max<size_t>(bytes, 4)});
+        return 0;
+    };
+    TRY(snap(T.V, NC * sizeof(float)));
+    TRY(snap(T.q, NC * NQ * sizeof(float)));
+    TRY(snap(T.syn_next, n_syn * sizeof(float)));
+    TRY(snap(T.syn_A, n_syn * sizeof(float)));
+    TRY(snap(T.syn_B, n_syn * sizeof(float)));
+    TRY(snap(T.trig[0], n_syn * sizeof(int32_t)));
+    TRY(snap(T.trig[1], n_syn * sizeof(int32_t)));
+    TRY(snap(T.poi_tsince, n_poi * sizeof(float)));
+    TRY(snap(T.poi_isi, n_poi * sizeof(float)));
+    TRY(snap(T.poi_A, n_poi * sizeof(float)));
+    TRY(snap(T.poi_B, n_poi * sizeof(float)));
+
+    TRY(upload(ctx, &ctx->d_point, point));
+    TRY(upload(ctx, &ctx->d_words, words));
+    TRY(upload(ctx, &ctx->d_pos, posv));
+    TRY(upload(ctx, &ctx->d_u, uv));
+
+    // launch plan for tree cells: bucket by launch shape
+    hipDeviceProp_t prop;
+    HIPCHECK(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    ctx->n_cu = prop.multiProcessorCount;
+    for (size_t i = 0; i < cells.size(); i++) {
+        Bucket *b = nullptr;
+        for (auto &bb : ctx->buckets)
+            if (bb.tpb == cell_tpb[i]) b = &bb;
+        if (!b) {
+            ctx->buckets.emplace_back();
+            b = &ctx->buckets.back();
+            b->tpb = cell_tpb[i];
+        }
+        b->order.push_back((int32_t)i);
+        b->nmax = std::max(b->nmax, cells[i].n);
+        b->words_max = std::max(b->words_max, (cells[i].rounds_fwd + cells[i].rounds_bwd) * b->tpb);
+    }
+    std::vector<TreeCell> dcells;
+    for (auto &b : ctx->buckets) {
+        // work queue order: largest cells first
+        std::stable_sort(b.order.begin(), b.order.end(), [&](int32_t x, int32_t y) { return cells[x].n > cells[y].n; });
+        std::vector<TreeCell> bc;
+        for (int32_t i : b.order) bc.push_back(cells[i]);
+        std::vector<int32_t> seq(bc.size());
+        for (size_t i = 0; i < seq.size(); i++) seq[i] = (int32_t)i;
+        b.lds = (size_t)b.nmax * 2 * sizeof(float2) + (size_t)b.words_max * sizeof(uint32_t);
+        const void *kp = tree_kernel_ptr(b.tpb);
+        if (!kp) return fail(ctx, EDENHIP_E_LIMIT, "no tree kernel for the chosen workgroup size");
+        int per_cu = 0;
+        HIPCHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, b.tpb, b.lds));
+        if (per_cu <= 0) return fail(ctx, EDENHIP_E_LIMIT, "tree kernel does not fit on a CU (LDS)");
+        b.grid = (int)std::min<int64_t>((int64_t)bc.size(), (int64_t)per_cu * ctx->n_cu);
+        TRY(upload(ctx, &b.d_cells, bc));
+        TRY(upload(ctx, &b.d_order, seq));
+        TRY(alloc_zero(ctx, &b.d_queue, 2));
+        HIPCHECK(ctx, hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        HIPCHECK(ctx, hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    }
+    HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_start, hipEventDisableTiming));
+    HIPCHECK(ctx, hipDeviceSynchronize());
+    return 0;
+}
+
+static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) {
+    DevTables T = ctx->T;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->opt.profile) {
+        HIPCHECK(ctx, hipEventCreate(&e0));
+        HIPCHECK(ctx, hipEventCreate(&e1));
+        HIPCHECK(ctx, hipEventRecord(e0, ctx->stream));
+    }
+    const bool per_step = ctx->network || ctx->opt.persistent < 0;
+    const int64_t stride = per_step ? 1 : (s1 - s0);
+    int64_t nl = 0;
+    for (int64_t a = s0; a < s1; a += stride) {
+        const int64_t b = std::min(s1, a + stride);
+        if (ctx->n_point > 0) {
+            const int tpb = 256;
+            const int grid = (ctx->n_point + tpb - 1) / tpb;
+            float dt = ctx->dt;
+            int32_t n = ctx->n_point;
+            void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &n,
+                            (void *)&a, (void *)&b, &row0, &dt};
+            HIPCHECK(ctx, hipLaunchKernel(point_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
+            nl++;
+        }
+        // tree buckets run concurrently, each on its own stream, joined back to the main stream
+        if (!ctx->buckets.empty()) HIPCHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+        for (auto &bk : ctx->buckets) {
+            float dt = ctx->dt;
+            int32_t ntask = (int32_t)bk.order.size();
+            void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
+                            &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &bk.nmax,
+                            (void *)&a, (void *)&b, &row0, &dt};
+            HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
+            HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb), dim3(bk.grid), dim3(bk.tpb), args, bk.lds, bk.stream));
+            HIPCHECK(ctx, hipEventRecord(bk.done, bk.stream));
+            HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, bk.done, 0));
+            nl++;
+        }
+    }
+    if (ctx->opt.profile) {
+        HIPCHECK(ctx, hipEventRecord(e1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+        ctx->kernel_ms += ms;
+        ctx->launches += nl;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return 0;
+}
+
+extern "C" {
+
+int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, edenhip_ctx **out) {
+    if (!out) return EDENHIP_E_INVALID;
+    *out = nullptr;
+    edenhip_ctx *ctx = new (std::nothrow) edenhip_ctx();
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (opt) ctx->opt = *opt;
+    if (ctx->opt.persistent == 0) ctx->opt.persistent = 1;
+    ctx->device = ctx->opt.device;
+    *out = ctx;
+    if (!desc) return fail(ctx, EDENHIP_E_INVALID, "null descriptor");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ctx, EDENHIP_E_DEVICE, "no HIP device visible");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    HIPCHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    return build(ctx, desc);
+}
+
+int64_t edenhip_total_rows(const edenhip_ctx *ctx) { return ctx ? ctx->total_rows : -1; }
+
+int edenhip_run(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (!ctx->stream) return fail(ctx, EDENHIP_E_STATE, "context was not created successfully");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int64_t want = std::min(n_rows < 0 ? ctx->total_rows : n_rows, ctx->total_rows - ctx->rows_done);
+    want = std::min(want, ctx->chunk_rows);
+    if (want < 0) want = 0;
+    const int64_t row0 = ctx->rows_done;
+    int64_t s0 = ctx->next_step;           // -3 on the first call: the init steps go first
+    const int64_t s1 = row0 + want + 1;    // step s records row s - 1
+    if (want > 0 || s0 <= 0) {
+        if (want == 0) return fail(ctx, EDENHIP_E_STATE, "no rows left to run");
+        TRY(launch_chunk(ctx, s0, s1, row0));
+        ctx->next_step = s1;
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->rows_done += want;
+    ctx->last_row0 = row0;
+    ctx->last_rows = want;
+    if (rows_done) *rows_done = want;
+    return 0;
+}
+
+int edenhip_reset(edenhip_ctx *ctx) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (!ctx->stream) return fail(ctx, EDENHIP_E_STATE, "context was not created successfully");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    for (auto &s : ctx->snaps) HIPCHECK(ctx, hipMemcpyAsync(s.dst, s.src, s.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemsetAsync(ctx->T.n_events, 0, sizeof(uint32_t), ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->rows_done = 0;
+    ctx->next_step = -3;
+    ctx->last_rows = 0;
+    ctx->last_row0 = 0;
+    ctx->kernel_ms = 0;
+    ctx->launches = 0;
+    return 0;
+}
+
+int edenhip_read_trace(edenhip_ctx *ctx, float *dst, int64_t max_rows, int64_t *rows, int64_t *row0) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    const int64_t nr = std::min(max_rows, ctx->last_rows);
+    const int64_t ncol = (int64_t)ctx->desc_to_dev_col.size();
+    if (rows) *rows = nr;
+    if (row0) *row0 = ctx->last_row0;
+    if (nr <= 0 || ncol == 0 || !dst) return 0;
+    std::vector<float> buf((size_t)(nr * ctx->n_dev_cols));
+    HIPCHECK(ctx, hipMemcpy(buf.data(), ctx->T.trace, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < nr; r++)
+        for (int64_t c = 0; c < ncol; c++) dst[r * ncol + c] = buf[(size_t)(r * ctx->n_dev_cols + ctx->desc_to_dev_col[c])];
+    return 0;
+}
+
+int edenhip_trace_device_ptr(edenhip_ctx *ctx, const float **dptr) {
+    if (!ctx || !dptr) return EDENHIP_E_INVALID;
+    *dptr = ctx->T.trace;
+    return 0;
+}
+
+int edenhip_read_events(edenhip_ctx *ctx, int64_t *col, int64_t *row, int64_t max_events, int64_t *n) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    uint32_t cnt = 0;
+    HIPCHECK(ctx, hipMemcpy(&cnt, ctx->T.n_events, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt > ctx->T.max_events) return fail(ctx, EDENHIP_E_LIMIT, "event buffer overflow");
+    std::vector<int64_t> ev(cnt);
+    if (cnt) HIPCHECK(ctx, hipMemcpy(ev.data(), ctx->T.events, cnt * sizeof(int64_t), hipMemcpyDeviceToHost));
+    std::sort(ev.begin(), ev.end());
+    int64_t k = 0;
+    for (int64_t x : ev) {
+        const int64_t r = x >> 24, c = x & 0xFFFFFF;
+        for (int64_t dc : ctx->ev_expand[c]) {
+            if (k < max_events && col && row) { col[k] = dc; row[k] = r; }
+            k++;
+        }
+    }
+    // columns that share a compartment interleave: restore ascending column order per row
+    if (col && row) {
+        const int64_t m = std::min(k, max_events);
+        std::vector<std::pair<int64_t, int64_t>> pr((size_t)m);
+        for (int64_t i = 0; i < m; i++) pr[i] = {row[i], col[i]};
+        std::sort(pr.begin(), pr.end());
+        for (int64_t i = 0; i < m; i++) { row[i] = pr[i].first; col[i] = pr[i].second; }
+    }
+    if (n) *n = k;
+    return 0;
+}
+
+int edenhip_get_voltages(edenhip_ctx *ctx, float *dst) {
+    if (!ctx || !dst) return EDENHIP_E_INVALID;
+    HIPCHECK(ctx, hipMemcpy(dst, ctx->T.V, ctx->NC * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int edenhip_kernel_stats(const edenhip_ctx *ctx, double *total_ms, int64_t *launches, char *name, int32_t name_len) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (total_ms) *total_ms = ctx->kernel_ms;
+    if (launches) *launches = ctx->launches;
+    if (name && name_len > 0) {
+        std::string nm = ctx->buckets.empty() ? "point_kernel" : "tree_kernel";
+        std::strncpy(name, nm.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return 0;
+}
+
+int edenhip_launch_shape(const edenhip_ctx *ctx, int32_t *threads_per_cell, int32_t *slots_per_thread) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (ctx->buckets.empty()) {
+        if (threads_per_cell) *threads_per_cell = 0;
+        if (slots_per_thread) *slots_per_thread = 0;
+        return 0;
+    }
+    if (threads_per_cell) *threads_per_cell = ctx->buckets[0].tpb;
+    if (slots_per_thread) *slots_per_thread = 4;
+    return 0;
+}
+
+const char *edenhip_last_error(const edenhip_ctx *ctx) {
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+void edenhip_destroy(edenhip_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->stream) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void *p : ctx->allocs) (void)hipFree(p);
+        for (auto &b : ctx->buckets) {
+            if (b.stream) (void)hipStreamDestroy(b.stream);
+            if (b.done) (void)hipEventDestroy(b.done);
+        }
+        if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+}  // extern "C"

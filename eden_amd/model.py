@@ -1,0 +1,294 @@
+"""Instantiated models (the SoA form the C-ABI consumes) and the GPU simulation handle.
+
+``Model`` holds every array of ``edenhip_model_desc`` as numpy arrays, in EDEN's instance
+order; ``Model.desc()`` returns the ctypes descriptor (the arrays stay owned by the Model).
+``Simulation`` drives ``libedenhip.so``: create, run chunks of rows, read traces/events.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+# engine units (eden/NeuroML.cpp:305-425): us, mV, nA, uS, pF, MOhm, mS/cm^2, um, 1/us
+PER_MS = 1e-3   # 1 per_ms in 1/us
+
+
+def rate(form, r_per_ms, midpoint_mV, scale_mV):
+    return N.Rate(form, np.float32(r_per_ms * PER_MS), midpoint_mV, scale_mV)
+
+
+def channel_type(gates):
+    """gates: list of (instances, alpha Rate, beta Rate[, q10])."""
+    ct = N.ChannelType()
+    ct.n_gates = len(gates)
+    for i, g in enumerate(gates):
+        inst, a, b = g[:3]
+        q10 = g[3] if len(g) > 3 else 1.0
+        ct.gates[i] = N.Gate(inst, q10, a, b)
+    return ct
+
+
+def hh_squid_types():
+    """Leak, Na (m^3 h) and K (n^4) of the HH squid axon as in NeuroML's
+    NML2_SingleCompHHCell example (examples/NML2_SingleCompHHCell.nml:14-43)."""
+    leak = channel_type([])
+    na = channel_type([
+        (3, rate(N.RATE_EXPLINEAR, 1.0, -40.0, 10.0), rate(N.RATE_EXP, 4.0, -65.0, -18.0)),
+        (1, rate(N.RATE_EXP, 0.07, -65.0, -20.0), rate(N.RATE_SIGMOID, 1.0, -35.0, 10.0)),
+    ])
+    k = channel_type([
+        (4, rate(N.RATE_EXPLINEAR, 0.1, -55.0, 10.0), rate(N.RATE_EXP, 0.125, -65.0, -80.0)),
+    ])
+    return [leak, na, k]
+
+
+def reverse_bits32(x):
+    x = int(x) & 0xFFFFFFFF
+    x = ((x & 0xFFFF0000) >> 16) | ((x & 0x0000FFFF) << 16)
+    x = ((x & 0xFF00FF00) >> 8) | ((x & 0x00FF00FF) << 8)
+    x = ((x & 0xF0F0F0F0) >> 4) | ((x & 0x0F0F0F0F) << 4)
+    x = ((x & 0xCCCCCCCC) >> 2) | ((x & 0x33333333) << 2)
+    x = ((x & 0xAAAAAAAA) >> 1) | ((x & 0x55555555) << 1)
+    return x & 0xFFFFFFFF
+
+
+def cell_rng_seeds(sim_seed, n_cells):
+    """ReverseBits32(sim_seed) ^ cell_gid (eden/Eden.cpp:7679-7691)."""
+    base = reverse_bits32(sim_seed)
+    return (np.uint32(base) ^ np.arange(n_cells, dtype=np.uint32)).astype(np.uint32)
+
+
+def _csr(counts):
+    b = np.zeros(len(counts) + 1, np.int64)
+    np.cumsum(counts, out=b[1:])
+    return b
+
+
+@dataclass
+class Model:
+    dt: float                          # us
+    t_final: float                     # us
+    cell_comp_begin: np.ndarray        # int64 [n_cells+1]
+    v0: np.ndarray
+    capacitance: np.ndarray
+    area: np.ndarray
+    inv_rc_axial: np.ndarray
+    r_axial: np.ndarray
+    v_threshold: np.ndarray
+    parent: np.ndarray                 # int32 cell-local
+    channel_types: list
+    comp_chan_begin: np.ndarray
+    chan_type: np.ndarray
+    chan_gbase: np.ndarray
+    chan_erev: np.ndarray
+    cell_rng_seed: np.ndarray = None
+    comp_pulse_begin: np.ndarray = None
+    pulse_start: np.ndarray = None
+    pulse_duration: np.ndarray = None
+    pulse_amplitude: np.ndarray = None
+    pulse_weight: np.ndarray = None
+    syn_types: list = field(default_factory=list)
+    comp_syn_begin: np.ndarray = None
+    syn_type: np.ndarray = None
+    syn_weight: np.ndarray = None
+    syn_delay: np.ndarray = None
+    comp_spike_begin: np.ndarray = None
+    spike_target: np.ndarray = None
+    comp_poisson_begin: np.ndarray = None
+    poisson_rate: np.ndarray = None
+    poisson_syn_type: np.ndarray = None
+    poisson_weight: np.ndarray = None
+    poisson_instance: np.ndarray = None
+    poisson_rng_offset: np.ndarray = None
+    record_comp: np.ndarray = None
+    event_comp: np.ndarray = None
+    name: str = "model"
+
+    def __post_init__(self):
+        nc = self.n_comps
+        z = lambda: np.zeros(nc + 1, np.int64)
+        for f in ("comp_pulse_begin", "comp_syn_begin", "comp_spike_begin", "comp_poisson_begin"):
+            if getattr(self, f) is None:
+                setattr(self, f, z())
+        f32 = lambda: np.zeros(0, np.float32)
+        i32 = lambda: np.zeros(0, np.int32)
+        for f in ("pulse_start", "pulse_duration", "pulse_amplitude", "pulse_weight", "syn_weight",
+                  "syn_delay", "poisson_rate", "poisson_weight"):
+            if getattr(self, f) is None:
+                setattr(self, f, f32())
+        for f in ("syn_type", "poisson_syn_type", "poisson_instance", "poisson_rng_offset"):
+            if getattr(self, f) is None:
+                setattr(self, f, i32())
+        if self.spike_target is None:
+            self.spike_target = np.zeros(0, np.int64)
+        if self.record_comp is None:
+            self.record_comp = np.zeros(0, np.int64)
+        if self.event_comp is None:
+            self.event_comp = np.zeros(0, np.int64)
+        if self.cell_rng_seed is None:
+            self.cell_rng_seed = cell_rng_seeds(0, self.n_cells)
+        self._normalise()
+
+    def _normalise(self):
+        conv = {
+            np.int64: ("cell_comp_begin", "comp_chan_begin", "comp_pulse_begin", "comp_syn_begin",
+                       "comp_spike_begin", "comp_poisson_begin", "spike_target", "record_comp", "event_comp"),
+            np.int32: ("parent", "chan_type", "syn_type", "poisson_syn_type", "poisson_instance",
+                       "poisson_rng_offset"),
+            np.float32: ("v0", "capacitance", "area", "inv_rc_axial", "r_axial", "v_threshold", "chan_gbase",
+                         "chan_erev", "pulse_start", "pulse_duration", "pulse_amplitude", "pulse_weight",
+                         "syn_weight", "syn_delay", "poisson_rate", "poisson_weight"),
+            np.uint32: ("cell_rng_seed",),
+        }
+        for dt, names in conv.items():
+            for n in names:
+                setattr(self, n, np.ascontiguousarray(getattr(self, n), dtype=dt))
+
+    @property
+    def n_cells(self):
+        return len(self.cell_comp_begin) - 1
+
+    @property
+    def n_comps(self):
+        return int(self.cell_comp_begin[-1])
+
+    @property
+    def n_synapses(self):
+        return int(self.comp_syn_begin[-1])
+
+    def comps_per_cell(self):
+        return np.diff(self.cell_comp_begin)
+
+    def total_rows(self):
+        """floor(t_final/dt) + 1 with the reference's double accumulation of float dt."""
+        t, n, dt = 0.0, 0, float(np.float32(self.dt))
+        # the accumulation is exact for the dt values used here; count directly then verify the tail
+        n = int(np.floor(self.t_final / dt)) + 1
+        while (n - 1) * dt > self.t_final:
+            n -= 1
+        while n * dt <= self.t_final:
+            n += 1
+        return n
+
+    def desc(self):
+        d = N.ModelDesc()
+        d.abi_version = N.ABI_VERSION
+        d.dt = self.dt
+        d.t_final = self.t_final
+        d.n_cells = self.n_cells
+        d.n_comps = self.n_comps
+        self._ct = (N.ChannelType * max(1, len(self.channel_types)))(*self.channel_types)
+        self._st = (N.ExpTwoType * max(1, len(self.syn_types)))(*self.syn_types)
+        d.n_channel_types = len(self.channel_types)
+        d.channel_types = C.cast(self._ct, C.c_void_p)
+        d.n_syn_types = len(self.syn_types)
+        d.syn_types = C.cast(self._st, C.c_void_p)
+        for name, _ in N.ModelDesc._fields_:
+            if name in ("abi_version", "dt", "t_final", "n_cells", "n_comps", "n_channel_types", "channel_types",
+                        "n_syn_types", "syn_types", "n_record", "n_event_cols"):
+                continue
+            setattr(d, name, N.ptr(getattr(self, name)))
+        d.n_record = len(self.record_comp)
+        d.n_event_cols = len(self.event_comp)
+        return d
+
+
+class EdenHipError(RuntimeError):
+    pass
+
+
+class Simulation:
+    """One model on one GPU. Rows are recorded steps (state at t = row * dt)."""
+
+    def __init__(self, model, threads_per_cell=0, persistent=1, chunk_steps=0, profile=False, device=0):
+        self.lib = N.load_library()
+        self.model = model
+        self._desc = model.desc()
+        opt = N.Options(threads_per_cell, persistent, chunk_steps, 0, 1 if profile else 0, device)
+        h = C.c_void_p()
+        rc = self.lib.edenhip_create(C.byref(self._desc), C.byref(opt), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.edenhip_last_error(h).decode() if h.value else "create failed"
+            self.close()
+            raise EdenHipError(f"edenhip_create: {msg} ({rc})")
+        self.total_rows = int(self.lib.edenhip_total_rows(h))
+        self.n_record = len(model.record_comp)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise EdenHipError(f"{what}: {self.lib.edenhip_last_error(self.h).decode()} ({rc})")
+
+    def run(self, n_rows=-1):
+        done = C.c_int64()
+        self._check(self.lib.edenhip_run(self.h, n_rows, C.byref(done)), "edenhip_run")
+        return done.value
+
+    def reset(self):
+        self._check(self.lib.edenhip_reset(self.h), "edenhip_reset")
+
+    def read_trace(self):
+        rows, row0 = C.c_int64(), C.c_int64()
+        self._check(self.lib.edenhip_read_trace(self.h, None, 0, C.byref(rows), C.byref(row0)), "read_trace")
+        out = np.empty((max(1, self.total_rows), max(1, self.n_record)), np.float32)
+        self._check(self.lib.edenhip_read_trace(self.h, N.ptr(out), out.shape[0], C.byref(rows), C.byref(row0)),
+                    "read_trace")
+        return out[: rows.value, : self.n_record], row0.value
+
+    def run_all(self):
+        """Run every row; return the full trace [rows x n_record]."""
+        parts = []
+        while True:
+            n = self.run(-1)
+            if n == 0:
+                break
+            tr, _ = self.read_trace()
+            parts.append(tr.copy())
+            if sum(p.shape[0] for p in parts) >= self.total_rows:
+                break
+        if not parts:
+            return np.zeros((0, self.n_record), np.float32)
+        return np.concatenate(parts, axis=0)
+
+    def read_events(self):
+        n = C.c_int64()
+        self._check(self.lib.edenhip_read_events(self.h, None, None, 0, C.byref(n)), "read_events")
+        col = np.empty(max(1, n.value), np.int64)
+        row = np.empty(max(1, n.value), np.int64)
+        self._check(self.lib.edenhip_read_events(self.h, N.ptr(col), N.ptr(row), n.value, C.byref(n)), "read_events")
+        return col[: n.value], row[: n.value]
+
+    def voltages(self):
+        v = np.empty(self.model.n_comps, np.float32)
+        self._check(self.lib.edenhip_get_voltages(self.h, N.ptr(v)), "get_voltages")
+        return v
+
+    def kernel_stats(self):
+        ms, nl = C.c_double(), C.c_int64()
+        name = C.create_string_buffer(64)
+        self._check(self.lib.edenhip_kernel_stats(self.h, C.byref(ms), C.byref(nl), name, 64), "kernel_stats")
+        return ms.value, nl.value, name.value.decode()
+
+    def launch_shape(self):
+        t, k = C.c_int32(), C.c_int32()
+        self._check(self.lib.edenhip_launch_shape(self.h, C.byref(t), C.byref(k)), "launch_shape")
+        return t.value, k.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.edenhip_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,217 @@
+/*
+ * edenhip.h -- C-ABI of the MI355X (gfx950) per-timestep simulation loop for EDEN models.
+ *
+ * The library replaces the body of EDEN's engine loop: the per-work-item callback
+ *   typedef void (*IterationCallback)(double time, float dt, ...)      eden/Eden.cpp:53-76
+ * exported by every generated `<Cell>.gen.so` as `doit` (eden/Eden.cpp:3239-3246), the
+ *   #pragma omp parallel for schedule(runtime) ... tabs.callbacks[item](...)  eden/Eden.cpp:11848-11869
+ * dispatch around it, and the step / swap logic of eden/Eden.cpp:11486-11938.
+ * A per-item GPU callback at that call site would mean 100k host calls per step, so the
+ * boundary sits one level up: the caller hands over the whole instantiated model once, in
+ * structure-of-arrays form, and asks for N steps at a time.
+ *
+ * Conventions
+ *  - Engine units are EDEN's native units (eden/NeuroML.cpp:305-425): time us, voltage mV,
+ *    current nA, conductance uS, capacitance pF, resistance MOhm, conductivity mS/cm^2,
+ *    length um, rates 1/us.
+ *  - Every entry point returns 0 on success and a negative EDENHIP_E* code on failure;
+ *    edenhip_last_error() then describes the failure. No C++ exception crosses the ABI.
+ *  - Descriptor arrays are borrowed only for the duration of edenhip_create(); the context
+ *    owns device copies. One host thread per context.
+ *  - Compartment indices inside a cell are EDEN's: parents before children, root = 0.
+ */
+#ifndef EDENHIP_H
+#define EDENHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDENHIP_ABI_VERSION 1
+
+enum {
+    EDENHIP_OK = 0,
+    EDENHIP_E_INVALID = -1,   /* malformed descriptor or argument */
+    EDENHIP_E_DEVICE = -2,    /* HIP runtime error */
+    EDENHIP_E_LIMIT = -3,     /* size beyond a compiled-in limit */
+    EDENHIP_E_STATE = -4      /* call not valid in the context's current state */
+};
+
+/* Gate rate forms of NeuroML's HHExpRate / HHExpLinearRate / HHSigmoidRate and fixed rates,
+ * emitted at eden/Eden.cpp:5126-5200. */
+enum {
+    EDENHIP_RATE_EXP = 0,        /* r * exp((V - mid) / scale) */
+    EDENHIP_RATE_EXPLINEAR = 1,  /* r * (V == mid ? 1 : x / (1 - exp(-x))), x = (V - mid) / scale */
+    EDENHIP_RATE_SIGMOID = 2,    /* r / (1 + exp((mid - V) / scale)) */
+    EDENHIP_RATE_FIXED = 3       /* r */
+};
+
+#define EDENHIP_MAX_GATES 4
+
+typedef struct edenhip_rate {
+    int32_t form;        /* EDENHIP_RATE_* */
+    float rate;          /* 1/us */
+    float midpoint;      /* mV */
+    float scale;         /* mV */
+} edenhip_rate;
+
+/* One gateHHrates gate: alpha/beta form, tau = 1/(alpha+beta), inf = alpha/(alpha+beta),
+ * cnexp update with clipping (eden/Eden.cpp:5236-5259, 5336-5375). */
+typedef struct edenhip_gate {
+    int32_t instances;   /* gate power in the open fraction (eden/Eden.cpp:5679-5694) */
+    float q10;           /* fixed rate multiplier (1 when the gate has no q10Settings) */
+    edenhip_rate alpha;  /* forward rate */
+    edenhip_rate beta;   /* reverse rate */
+} edenhip_gate;
+
+/* An ion channel type. n_gates == 0 is a passive (leak) channel: open fraction 1. */
+typedef struct edenhip_channel_type {
+    int32_t n_gates;
+    edenhip_gate gates[EDENHIP_MAX_GATES];
+} edenhip_channel_type;
+
+/* expTwoSynapse (eden/neuroml/LEMS_CoreComponents.inc.xml:145-180), run by EDEN as a
+ * lemsified component (eden/Eden.cpp:2614-2994). */
+typedef struct edenhip_exptwo_type {
+    float gbase;         /* uS */
+    float erev;          /* mV */
+    float tau_rise;      /* us */
+    float tau_decay;     /* us */
+    float weight_prop;   /* the LEMS `weight` property, default 1 */
+} edenhip_exptwo_type;
+
+/*
+ * The instantiated model. Global compartment index g runs over all cells in cell order;
+ * cell c owns compartments [cell_comp_begin[c], cell_comp_begin[c+1]).
+ * CSR arrays "x_begin" have n_comps + 1 entries and index per-compartment lists.
+ * Lists are in EDEN's instance order (eden/Eden.cpp:8791-9281): summation order of the
+ * per-compartment currents follows it.
+ */
+typedef struct edenhip_model_desc {
+    int32_t abi_version;          /* EDENHIP_ABI_VERSION */
+    float dt;                     /* us, EngineConfig::dt */
+    double t_final;               /* us, EngineConfig::t_final (= <Simulation length>) */
+
+    int32_t n_cells;
+    int64_t n_comps;
+    const int64_t *cell_comp_begin;   /* [n_cells + 1] */
+    const uint32_t *cell_rng_seed;    /* [n_cells] ReverseBits(sim_seed) ^ gid (eden/Eden.cpp:7679-7691) */
+
+    /* per compartment [n_comps] */
+    const float *v0;              /* initial V, mV */
+    const float *capacitance;     /* C, pF */
+    const float *area;            /* membrane area, um^2 */
+    const float *inv_rc_axial;    /* sum over neighbours of 1/(R C), 1/us (eden/Eden.cpp:4488-4501) */
+    const float *r_axial;         /* resistance to the parent, MOhm; unused for a root */
+    const float *v_threshold;     /* spike threshold, mV; NaN = no spike output */
+    const int32_t *parent;        /* cell-local parent index, -1 for the root */
+
+    /* ion channel distributions (channelDensity): CSR over compartments */
+    int32_t n_channel_types;
+    const edenhip_channel_type *channel_types;   /* [n_channel_types] */
+    const int64_t *comp_chan_begin;              /* [n_comps + 1] */
+    const int32_t *chan_type;                    /* [n_chan] */
+    const float *chan_gbase;                     /* [n_chan] mS/cm^2 */
+    const float *chan_erev;                      /* [n_chan] mV */
+
+    /* pulseGenerator inputs: CSR over compartments (eden/Eden.cpp:3798-3826) */
+    const int64_t *comp_pulse_begin;             /* [n_comps + 1] */
+    const float *pulse_start;                    /* us */
+    const float *pulse_duration;                 /* us */
+    const float *pulse_amplitude;                /* nA */
+    const float *pulse_weight;                   /* input weight */
+
+    /* synapse types */
+    int32_t n_syn_types;
+    const edenhip_exptwo_type *syn_types;        /* [n_syn_types] */
+
+    /* chemical synapse instances (spike-triggered, with delay): CSR over compartments
+     * (eden/Eden.cpp:3561-3740) */
+    const int64_t *comp_syn_begin;               /* [n_comps + 1] */
+    const int32_t *syn_type;                     /* [n_syn] */
+    const float *syn_weight;                     /* [n_syn] connection weight */
+    const float *syn_delay;                      /* [n_syn] us */
+
+    /* spike recipients: for every compartment, the synapse instances (global index into the
+     * syn_* arrays) that a spike of it triggers (eden/Eden.cpp:3946-3995, 8029-8046) */
+    const int64_t *comp_spike_begin;             /* [n_comps + 1] */
+    const int64_t *spike_target;                 /* [n_targets] */
+
+    /* poissonFiringSynapse inputs: CSR over compartments, each with its own synapse
+     * (eden/Eden.cpp:3859-3912; LEMS_CoreComponents.inc.xml:623-672) */
+    const int64_t *comp_poisson_begin;           /* [n_comps + 1] */
+    const float *poisson_rate;                   /* 1/us */
+    const int32_t *poisson_syn_type;             /* synapse type driven by the generator */
+    const float *poisson_weight;                 /* input weight */
+    const int32_t *poisson_instance;             /* instance index for the RNG stamp */
+    const int32_t *poisson_rng_offset;           /* rng_offset + random_call_counter base */
+
+    /* recording */
+    int64_t n_record;                            /* trajectory columns */
+    const int64_t *record_comp;                  /* [n_record] compartment whose V is logged */
+    int64_t n_event_cols;                        /* spike-event logger columns */
+    const int64_t *event_comp;                   /* [n_event_cols] compartment spiking */
+} edenhip_model_desc;
+
+/* Launch-shape choices. Zero-initialise for defaults. */
+typedef struct edenhip_options {
+    int32_t threads_per_cell;     /* 0 = autotuned; else 64, 128, 256 or 512 for tree cells */
+    int32_t persistent;           /* 1 (default when 0) = time loop inside one launch per chunk;
+                                     -1 = one launch per step (the plain per-step launch) */
+    int32_t chunk_steps;          /* steps per trace chunk; 0 = all steps */
+    int32_t use_graph;            /* 1 = capture the per-step launches of a chunk in a hipGraph */
+    int32_t profile;              /* 1 = time each launch with HIP events */
+    int32_t device;               /* HIP device ordinal */
+} edenhip_options;
+
+typedef struct edenhip_ctx edenhip_ctx;
+
+/* Build device tables from the descriptor (replaces RawTables instantiation +
+ * the initial state copy, eden/Eden.cpp:7643-9281, 11093-11150). */
+int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, edenhip_ctx **out);
+
+/* Rows the full run records: floor(t_final / dt) + 1 (eden/Eden.cpp:11488). */
+int64_t edenhip_total_rows(const edenhip_ctx *ctx);
+
+/* Run the 4 initialisation steps (if not yet run) and then up to `n_rows` recorded steps
+ * (each: record the state at `time`, compute, time += dt; eden/Eden.cpp:11549-11938).
+ * Recorded V values go to the context's device trace buffer for this chunk; spike events
+ * go to its device event list. Returns the number of rows done in *rows_done. */
+int edenhip_run(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done);
+
+/* Rewind to the state edenhip_create() built (before the initialisation steps), clearing the
+ * event log; device tables stay resident. No counterpart in EDEN, which runs a model once. */
+int edenhip_reset(edenhip_ctx *ctx);
+
+/* Copy the trace rows of the last edenhip_run() call: [rows x n_record] floats (mV),
+ * row r holding the state at time (row0 + r) * dt. dst is host memory. */
+int edenhip_read_trace(edenhip_ctx *ctx, float *dst, int64_t max_rows, int64_t *rows, int64_t *row0);
+
+/* Device pointer of the trace buffer of the last run (rows x n_record floats). */
+int edenhip_trace_device_ptr(edenhip_ctx *ctx, const float **dptr);
+
+/* Spike events logged so far, as (event column, row) pairs in ascending (row, column)
+ * order: the spike detected while computing row r - 1 is logged at row r with timestamp
+ * r * dt (eden/Eden.cpp:11598-11635). Copies up to max_events pairs; *n gets the total. */
+int edenhip_read_events(edenhip_ctx *ctx, int64_t *col, int64_t *row, int64_t max_events, int64_t *n);
+
+/* Copy the membrane voltage of every compartment (n_comps floats) to host memory. */
+int edenhip_get_voltages(edenhip_ctx *ctx, float *dst);
+
+/* Launch statistics of the hot kernel (only with options.profile = 1): the summed duration
+ * of the kernel launches in ms, measured with HIP events on the stream they ran on, and
+ * the launch count. */
+int edenhip_kernel_stats(const edenhip_ctx *ctx, double *total_ms, int64_t *launches, char *name, int32_t name_len);
+
+/* Describe the launch shape chosen for the tree-cell kernel (autotuned or given). */
+int edenhip_launch_shape(const edenhip_ctx *ctx, int32_t *threads_per_cell, int32_t *slots_per_thread);
+
+const char *edenhip_last_error(const edenhip_ctx *ctx);
+void edenhip_destroy(edenhip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDENHIP_H */

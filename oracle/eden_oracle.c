@@ -1,0 +1,507 @@
+/*
+ * eden_oracle.c -- CPU restatement of EDEN's per-timestep simulation loop.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP path: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load or run it, and never
+ * as the thing measured or shipped.
+ *
+ * It consumes the same instantiated model (edenhip_model_desc, include/edenhip.h) as the
+ * GPU library and follows the semantics of the C code EDEN generates per cell type and of
+ * the engine loop that drives it, in EDEN's own execution structure: one work item per
+ * cell, `#pragma omp parallel for schedule(static)`, every synapse instance scanned every
+ * step, serial depth-first Hines elimination, double-buffered state.
+ *
+ * Compiled three times: REAL=float (fp32, as EDEN's `typedef float Real`, eden/NeuroML.h:29),
+ * REAL=float with glibc expf/logf (ORACLE_LIBM), and
+ * REAL=double (the FP64 arbiter; EDEN has no FP64 mode). `substeps` > 1 divides dt for the
+ * dt/10 arbiter run and records every `substeps`-th row.
+ *
+ * Reference files followed (paths relative to the EDEN source tree):
+ *   engine loop, init steps, output-before-compute, swap  eden/Eden.cpp:11486-11938
+ *   hash RNG (hash64shift, hash_128_to_64, randof)        eden/Eden.cpp:3200-3233
+ *   gate rates / q10 / cnexp + clip                       eden/Eden.cpp:5126-5200, 5203-5259, 5336-5375
+ *   open fraction (gate powers), density current          eden/Eden.cpp:5679-5721, 5805-5898
+ *   synapse trigger / single-slot delay rule              eden/Eden.cpp:3636-3707
+ *   expTwoSynapse LEMS dynamics                           eden/Eden.cpp:2614-2994,
+ *                                                         LEMS_CoreComponents.inc.xml:145-180
+ *   pulse input                                           eden/Eden.cpp:3798-3826
+ *   Poisson firing synapse input                          eden/Eden.cpp:3859-3912,
+ *                                                         LEMS_CoreComponents.inc.xml:623-672
+ *   membrane RHS and diagonal                             eden/Eden.cpp:6057-6107
+ *   DFS elimination order                                 eden/Eden.cpp:4070-4078, 4416-4486
+ *   Hines forward elimination / back-substitution         eden/Eden.cpp:6114-6225
+ *   spike detection and atomic-OR send                    eden/Eden.cpp:3946-3995, 6248
+ *   event logger (flag read + clear at output time)       eden/Eden.cpp:11598-11635
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "../include/edenhip.h"
+
+#ifndef REAL
+#define REAL float
+#define SFX f32
+#endif
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(CAT(oracle_, SFX), CAT(_, name))
+
+/* Transcendentals. EDEN's generated C calls expf/logf from libm, built with -Ofast
+ * -march=native (eden/Eden.cpp:7017-7030), so the exact bits depend on the host's libm and
+ * its vector variants; no reference test pins them. The fp32 oracle rounds exp/log once
+ * from a double evaluation (correctly rounded except when the double result lies within a
+ * double ulp of a float rounding boundary), the construction the HIP path uses too. The
+ * ORACLE_LIBM build calls glibc's expf/logf instead, to measure the sensitivity of results
+ * to that choice. */
+#if defined(ORACLE_F64)
+#define EXP exp
+#define LOG log
+#elif defined(ORACLE_LIBM)
+#define EXP expf
+#define LOG logf
+#else
+static inline float exp_cr(float x) { return (float)exp((double)x); }
+static inline float log_cr(float x) { return (float)log((double)x); }
+#define EXP exp_cr
+#define LOG log_cr
+#endif
+
+typedef REAL real;
+
+/* Thomas Wang's 64-bit integer hash ("Integer Hash Function", 1997), the mixer EDEN's
+ * generated code uses for its counter-based RNG (eden/Eden.cpp:3200-3212). */
+static uint64_t hash64shift(uint64_t key) {
+    key = (~key) + (key << 21);
+    key = key ^ (key >> 24);
+    key = (key + (key << 3)) + (key << 8);
+    key = key ^ (key >> 14);
+    key = (key + (key << 2)) + (key << 4);
+    key = key ^ (key >> 28);
+    key = key + (key << 31);
+    return key;
+}
+
+/* randof(): a uniform sample in [0, x) stamped by (work item, instance, invocation, step)
+ * (eden/Eden.cpp:3213-3233). work_item arrives as a sign-extended int, step may be negative
+ * on the initialisation steps: both wrap modulo 2^64 exactly as the C code does. */
+static float randof_f(float x, int32_t work_item, int64_t instance, int64_t step, int32_t invocation) {
+    uint64_t hi = (uint64_t)(int64_t)work_item * (1ULL << 24) | ((uint64_t)instance % (1ULL << 24));
+    uint64_t lo = (uint64_t)(int64_t)invocation * (1ULL << 40) | ((uint64_t)step % (1ULL << 40));
+    uint64_t s = hash64shift(hash64shift(lo) ^ hi);
+    const int scale = 1 << 23;
+    float r = ((float)(s % scale)) / ((float)scale);
+    return x * r;
+}
+
+/* Exported for the RNG known-answer tests. */
+uint64_t FN(hash64shift)(uint64_t key) { return hash64shift(key); }
+float FN(randof)(float x, int32_t work_item, int64_t instance, int64_t step, int32_t invocation) {
+    return randof_f(x, work_item, instance, step, invocation);
+}
+
+static real rate_value(const edenhip_rate *r, real V) {
+    real rate = (real)r->rate, mid = (real)r->midpoint, scale = (real)r->scale;
+    switch (r->form) {
+    case EDENHIP_RATE_EXP:
+        return rate * EXP((V - mid) / scale);
+    case EDENHIP_RATE_EXPLINEAR:
+        return rate * ((V == mid) ? (real)1 : (((V - mid) / scale) / ((real)1 - EXP(-(V - mid) / scale))));
+    case EDENHIP_RATE_SIGMOID:
+        return rate / ((real)1 + EXP((mid - V) / scale));
+    default:
+        return rate;
+    }
+}
+
+/* Exported so the tests can compare the rate forms with a direct evaluation. */
+double FN(rate)(const edenhip_rate *r, double V) { return (double)rate_value(r, (real)V); }
+
+/* Depth-first post-order from compartment 0 over adjacency lists [parent, children
+ * ascending] (eden/Eden.cpp:4070-4078, 4416-4486), iteratively. */
+static void dfs_order(int32_t n, const int32_t *parent, int32_t *order) {
+    int32_t *child_begin = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    int32_t *child = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t *fill = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    int32_t *stack = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t *next = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    for (int32_t i = 1; i < n; i++) child_begin[parent[i] + 1]++;
+    for (int32_t i = 0; i < n; i++) child_begin[i + 1] += child_begin[i];
+    for (int32_t i = 1; i < n; i++) child[child_begin[parent[i]] + fill[parent[i]]++] = i; /* ascending */
+    int32_t sp = 0, k = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        int32_t v = stack[sp - 1];
+        if (next[v] < child_begin[v + 1] - child_begin[v]) {
+            stack[sp++] = child[child_begin[v] + next[v]++];
+        } else {
+            order[k++] = v;
+            sp--;
+        }
+    }
+    free(child_begin); free(child); free(fill); free(stack); free(next);
+}
+
+/* Exported for the ordering tests. */
+void FN(dfs_order)(int32_t n, const int32_t *parent, int32_t *order) { dfs_order(n, parent, order); }
+
+typedef struct {
+    /* double-buffered state */
+    real *V[2];
+    real *q[2];             /* gate states, per channel instance per gate */
+    real *D;                /* Hines scratch diagonal (state table in EDEN, written before read) */
+    int64_t *trig[2];       /* synapse Trigger flags */
+    real *nspk[2];          /* synapse NextSpike */
+    real *sA[2], *sB[2];    /* expTwo A, B per synapse */
+    real *pT[2], *pI[2];    /* Poisson tsince, isi */
+    real *pA[2], *pB[2];    /* Poisson synapse A, B */
+    int64_t *evflag[2];     /* event logger flags */
+    /* derived tables */
+    int64_t *gate_begin;    /* per channel instance */
+    int32_t *order;         /* per compartment, cell-local DFS order, laid out per cell */
+    int64_t *comp_ev_begin; /* compartment -> event columns CSR */
+    int64_t *comp_ev;
+} ostate;
+
+#define ALLOC(p, n, T) do { (p) = (T *)calloc((size_t)((n) > 0 ? (n) : 1), sizeof(T)); } while (0)
+
+static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int now, int nxt, int64_t step,
+                    double time_f64, real dt) {
+    const int initial_state = (step <= 0);
+#if defined(ORACLE_F64)
+    const real time_r = (real)time_f64;
+#else
+    const real time_r = (float)time_f64; /* time_f32 (eden/Eden.cpp:3252) */
+#endif
+    const int64_t c0 = d->cell_comp_begin[cell], c1 = d->cell_comp_begin[cell + 1];
+    const int32_t n = (int32_t)(c1 - c0);
+    const int32_t rng_object_id = (int32_t)d->cell_rng_seed[cell];
+    real *V = s->V[now], *Vn = s->V[nxt];
+
+    for (int64_t g = c0; g < c1; g++) {
+        const real Acomp = (real)d->area[g];
+        const real Vcomp = V[g];
+        real I_channels_total = 0, G_channels_total = 0;
+        /* ion channel distributions, in distribution order */
+        for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
+            const edenhip_channel_type *ct = &d->channel_types[d->chan_type[ch]];
+            real fopen = 1; /* conductance_scaling */
+            int64_t qb = s->gate_begin[ch];
+            for (int gi = 0; gi < ct->n_gates; gi++) {
+                const edenhip_gate *gt = &ct->gates[gi];
+                real q = s->q[now][qb + gi];
+                real q10 = (real)gt->q10;
+                real alpha = rate_value(&gt->alpha, Vcomp);
+                real beta = rate_value(&gt->beta, Vcomp);
+                real tau = (real)1 / (alpha + beta);
+                real inf = alpha / (alpha + beta);
+                real qn;
+                if (initial_state) {
+                    qn = inf;
+                } else {
+                    real tau_factor = ((dt * q10) / tau);
+                    real blend = EXP(-tau_factor);
+                    qn = (blend) * q + ((real)1 - blend) * inf;
+                    if (!(qn > (real)(1e-6f))) qn = (real)1e-6f;
+                    if (!(qn < (real)(1 - 1e-6f))) qn = (real)(1 - 1e-6f);
+                }
+                s->q[nxt][qb + gi] = qn;
+                for (int k = 0; k < gt->instances; k++) fopen = fopen * q;
+            }
+            real Gbase = (real)d->chan_gbase[ch];
+            real Erev = (real)d->chan_erev[ch];
+            real Gscaled = Gbase * fopen;
+            real iDensity = Gscaled * (Erev - Vcomp) * (real)1e-5f;
+            real gDensity = (Gscaled) * (real)1e-5f;
+            real I_chan = (iDensity * Acomp);
+            real g_chan = (gDensity * Acomp);
+            I_channels_total += I_chan;
+            G_channels_total += g_chan;
+        }
+        /* synapses: per type (ascending type order in the list), aggregated per type */
+        real I_synapses_total = 0, G_synapses_total = 0;
+        {
+            real I_agg = 0, G_agg = 0;
+            int32_t cur_type = -1;
+            for (int64_t sy = d->comp_syn_begin[g]; sy < d->comp_syn_begin[g + 1]; sy++) {
+                int32_t ty = d->syn_type[sy];
+                if (ty != cur_type) {
+                    if (cur_type >= 0) { I_synapses_total += I_agg; G_synapses_total += G_agg; }
+                    I_agg = 0; G_agg = 0; cur_type = ty;
+                }
+                const edenhip_exptwo_type *st = &d->syn_types[ty];
+                real weight = (real)d->syn_weight[sy];
+                int spike_in_flag = 0;
+                if (!initial_state) {
+                    spike_in_flag = !!s->trig[now][sy];
+                    s->trig[now][sy] = 0;
+                }
+                real delay = (real)d->syn_delay[sy];
+                real next_spike = s->nspk[now][sy];
+                real next_next_spike = next_spike;
+                int spike_now = 0;
+                if (!initial_state) {
+                    if (time_r <= next_spike && next_spike < time_r + dt) spike_now = 1;
+                    if (spike_in_flag) {
+                        real fresh_spike = time_r + delay;
+                        if (time_r <= fresh_spike && fresh_spike < time_r + dt) spike_now = 1;
+                        if (next_next_spike < time_r + dt && next_next_spike < fresh_spike) next_next_spike = fresh_spike;
+                    }
+                }
+                spike_in_flag = spike_now;
+                s->nspk[nxt][sy] = next_next_spike;
+                /* expTwoSynapse */
+                real A = s->sA[now][sy], B = s->sB[now][sy];
+                real tr = (real)st->tau_rise, td = (real)st->tau_decay;
+                real peakTime = (LOG(td / tr) * (tr * td)) / (td - tr);
+                real wf = (real)1 / ((-EXP((-peakTime) / tr)) + EXP((-peakTime) / td));
+                real gsyn = (real)st->gbase * (B - A);
+                real isyn = gsyn * ((real)st->erev - Vcomp);
+                real An, Bn;
+                if (initial_state) {
+                    An = 0; Bn = 0;
+                } else {
+                    real dA = (-A) / tr;
+                    An = A + dt * dA;
+                    real dB = (-B) / td;
+                    Bn = B + dt * dB;
+                    if (spike_in_flag) {
+                        real lw = (real)st->weight_prop;
+                        An = A + (lw * wf);
+                        Bn = B + (lw * wf);
+                    }
+                }
+                s->sA[nxt][sy] = An; s->sB[nxt][sy] = Bn;
+                I_agg += isyn * weight;
+                G_agg += gsyn * weight;
+            }
+            if (cur_type >= 0) { I_synapses_total += I_agg; G_synapses_total += G_agg; }
+        }
+        /* inputs: core pulse inputs first (negative type ids), then Poisson inputs */
+        real I_input_total = 0, G_input_total = 0;
+        if (d->comp_pulse_begin[g + 1] > d->comp_pulse_begin[g]) {
+            real I_input_pulse = 0;
+            for (int64_t p = d->comp_pulse_begin[g]; p < d->comp_pulse_begin[g + 1]; p++) {
+                real st = (real)d->pulse_start[p], du = (real)d->pulse_duration[p];
+                if (st <= time_r && time_r <= st + du) I_input_pulse += (real)d->pulse_amplitude[p] * (real)d->pulse_weight[p];
+            }
+            I_input_total += I_input_pulse;
+            I_input_total += 0;
+        }
+        if (d->comp_poisson_begin[g + 1] > d->comp_poisson_begin[g]) {
+            real I_agg = 0, G_agg = 0;
+            for (int64_t p = d->comp_poisson_begin[g]; p < d->comp_poisson_begin[g + 1]; p++) {
+                real weight = (real)d->poisson_weight[p];
+                real tsince = s->pT[now][p], isi = s->pI[now][p];
+                real averageIsi = (real)1 / (real)d->poisson_rate[p];
+                int32_t inst = d->poisson_instance[p], roff = d->poisson_rng_offset[p];
+                int fired = 0;
+                if (initial_state) {
+                    real t0 = 0;
+                    real i0 = (-averageIsi) * LOG((real)1 - (real)randof_f(1.0f, rng_object_id, inst, step, roff + 0));
+                    s->pT[nxt][p] = t0; s->pI[nxt][p] = i0;
+                } else {
+                    real deriv = (real)1;
+                    s->pT[nxt][p] = tsince + dt * deriv;
+                    s->pI[nxt][p] = isi;
+                    if (tsince > isi) {
+                        s->pT[nxt][p] = 0;
+                        s->pI[nxt][p] = (-averageIsi) * LOG((real)1 - (real)randof_f(1.0f, rng_object_id, inst, step, roff + 1));
+                        fired = 1;
+                    }
+                }
+                const edenhip_exptwo_type *st = &d->syn_types[d->poisson_syn_type[p]];
+                real A = s->pA[now][p], B = s->pB[now][p];
+                real tr = (real)st->tau_rise, td = (real)st->tau_decay;
+                real peakTime = (LOG(td / tr) * (tr * td)) / (td - tr);
+                real wf = (real)1 / ((-EXP((-peakTime) / tr)) + EXP((-peakTime) / td));
+                real gsyn = (real)st->gbase * (B - A);
+                real isyn = gsyn * ((real)st->erev - Vcomp);
+                real An, Bn;
+                if (initial_state) {
+                    An = 0; Bn = 0;
+                } else {
+                    real dA = (-A) / tr;
+                    An = A + dt * dA;
+                    real dB = (-B) / td;
+                    Bn = B + dt * dB;
+                    if (fired) {
+                        real lw = (real)st->weight_prop;
+                        An = A + (lw * wf);
+                        Bn = B + (lw * wf);
+                    }
+                }
+                s->pA[nxt][p] = An; s->pB[nxt][p] = Bn;
+                I_agg += isyn * weight;
+                G_agg += gsyn * weight;
+            }
+            I_input_total += I_agg;
+            G_input_total += G_agg;
+        }
+        real I_internal = I_channels_total + I_input_total + I_synapses_total;
+        real G_internal = G_channels_total + G_input_total + G_synapses_total;
+        const real Cg = (real)d->capacitance[g];
+        s->D[g] = 1 * (G_internal / Cg) + (real)d->inv_rc_axial[g];
+        if (initial_state) {
+            Vn[g] = V[g];
+        } else {
+            Vn[g] = V[g] + (dt * (I_internal + ((V[g] * G_internal))) / Cg);
+        }
+    }
+
+    /* cable equation: backward Euler over the tree, in DFS elimination order */
+    if (!initial_state) {
+        real *D = s->D + c0;
+        real *X = Vn + c0;
+        const int32_t *Order = s->order + c0;
+        const int32_t *Parent = d->parent + c0;
+        const float *R_Axial = d->r_axial + c0;
+        const float *C = d->capacitance + c0;
+        for (int32_t k = 0; k < n; k++) D[k] = 1 + D[k] * dt;
+        for (int32_t k = 0; k < n - 1; k++) {
+            int32_t i = Order[k];
+            int32_t j = Parent[i];
+            int32_t idx = (i > j) ? i : j;
+            real R = (real)R_Axial[idx];
+            real Ui = -dt / (R * (real)C[i]);
+            real Uj = -dt / (R * (real)C[j]);
+            real Li = Uj;
+            real ratio = Li / D[i];
+            D[j] -= ratio * Ui;
+            X[j] -= ratio * X[i];
+        }
+        int32_t r = Order[n - 1];
+        X[r] = X[r] / D[r];
+        for (int32_t k = n - 2; k >= 0; k--) {
+            int32_t i = Order[k];
+            int32_t j = Parent[i];
+            int32_t idx = (i > j) ? i : j;
+            real R = (real)R_Axial[idx];
+            real Ui = -dt / (R * (real)C[i]);
+            X[i] = (X[i] - Ui * X[j]) / D[i];
+        }
+    }
+
+    /* post-update: spike detection and send to the recipients' next buffers */
+    for (int64_t g = c0; g < c1; g++) {
+        real Vt = (real)d->v_threshold[g];
+        if (V[g] < Vt && Vt < Vn[g]) {
+            for (int64_t k = d->comp_spike_begin[g]; k < d->comp_spike_begin[g + 1]; k++)
+                __atomic_fetch_or(&s->trig[nxt][d->spike_target[k]], (int64_t)1, __ATOMIC_RELAXED);
+            for (int64_t k = s->comp_ev_begin[g]; k < s->comp_ev_begin[g + 1]; k++)
+                __atomic_fetch_or(&s->evflag[nxt][s->comp_ev[k]], (int64_t)1, __ATOMIC_RELAXED);
+        }
+    }
+}
+
+/*
+ * Run the model for up to n_rows recorded rows (all rows when n_rows < 0).
+ *   trace   [n_rows x n_record] floats, may be NULL
+ *   ev_col, ev_row: event log as (column, row) in ascending row then column; rows are in
+ *                   units of the recorded row (fine step index when substeps > 1)
+ * Returns rows recorded, or a negative value on error.
+ */
+int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, int32_t n_threads,
+                float *trace, int64_t *ev_col, int64_t *ev_row, int64_t max_ev, int64_t *n_ev,
+                float *final_v) {
+    if (!d || d->abi_version != EDENHIP_ABI_VERSION || substeps < 1) return -1;
+    const int64_t NC = d->n_comps;
+    const int64_t NCH = d->comp_chan_begin[NC];
+    const int64_t NSY = d->comp_syn_begin[NC];
+    const int64_t NP = d->comp_poisson_begin[NC];
+    ostate s;
+    memset(&s, 0, sizeof(s));
+    ALLOC(s.gate_begin, NCH + 1, int64_t);
+    for (int64_t ch = 0; ch < NCH; ch++) s.gate_begin[ch + 1] = s.gate_begin[ch] + d->channel_types[d->chan_type[ch]].n_gates;
+    const int64_t NQ = s.gate_begin[NCH];
+    for (int b = 0; b < 2; b++) {
+        ALLOC(s.V[b], NC, real); ALLOC(s.q[b], NQ, real);
+        ALLOC(s.trig[b], NSY, int64_t); ALLOC(s.nspk[b], NSY, real);
+        ALLOC(s.sA[b], NSY, real); ALLOC(s.sB[b], NSY, real);
+        ALLOC(s.pT[b], NP, real); ALLOC(s.pI[b], NP, real);
+        ALLOC(s.pA[b], NP, real); ALLOC(s.pB[b], NP, real);
+        ALLOC(s.evflag[b], d->n_event_cols, int64_t);
+    }
+    ALLOC(s.D, NC, real);
+    ALLOC(s.order, NC, int32_t);
+    for (int32_t c = 0; c < d->n_cells; c++) {
+        int64_t c0 = d->cell_comp_begin[c];
+        dfs_order((int32_t)(d->cell_comp_begin[c + 1] - c0), d->parent + c0, s.order + c0);
+    }
+    ALLOC(s.comp_ev_begin, NC + 1, int64_t);
+    ALLOC(s.comp_ev, d->n_event_cols, int64_t);
+    for (int64_t e = 0; e < d->n_event_cols; e++) s.comp_ev_begin[d->event_comp[e] + 1]++;
+    for (int64_t g = 0; g < NC; g++) s.comp_ev_begin[g + 1] += s.comp_ev_begin[g];
+    {
+        int64_t *fill; ALLOC(fill, NC, int64_t);
+        for (int64_t e = 0; e < d->n_event_cols; e++) {
+            int64_t g = d->event_comp[e];
+            s.comp_ev[s.comp_ev_begin[g] + fill[g]++] = e;
+        }
+        free(fill);
+    }
+    /* initial state: V0; gates at alpha/(alpha+beta) of V0 (overwritten by the init steps);
+     * NextSpike = -inf (eden/Eden.cpp:9091); LEMS states 0 */
+    for (int64_t g = 0; g < NC; g++) {
+        s.V[0][g] = (real)d->v0[g];
+        for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
+            const edenhip_channel_type *ct = &d->channel_types[d->chan_type[ch]];
+            for (int gi = 0; gi < ct->n_gates; gi++) {
+                real a = rate_value(&ct->gates[gi].alpha, (real)d->v0[g]);
+                real b = rate_value(&ct->gates[gi].beta, (real)d->v0[g]);
+                s.q[0][s.gate_begin[ch] + gi] = a / (a + b);
+            }
+        }
+    }
+    for (int64_t sy = 0; sy < NSY; sy++) s.nspk[0][sy] = (real)-INFINITY;
+
+    const real dt = (real)d->dt / (real)substeps;
+    const double dt_f64 = (double)((float)d->dt / (float)1) / (double)substeps;
+    int now = 0, nxt = 1;
+    double time = 0;
+    int64_t rows = 0, fine = 0, nev = 0;
+    int64_t step;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    for (step = -3; time <= d->t_final; step++) {
+        const int initializing = step <= 0;
+        if (!initializing) {
+            if (fine % substeps == 0) {
+                if (n_rows >= 0 && rows >= n_rows) break;
+                if (trace)
+                    for (int64_t r = 0; r < d->n_record; r++) trace[rows * d->n_record + r] = (float)s.V[now][d->record_comp[r]];
+                rows++;
+            }
+            for (int64_t e = 0; e < d->n_event_cols; e++) {
+                if (s.evflag[now][e]) {
+                    if (ev_col && nev < max_ev) { ev_col[nev] = e; ev_row[nev] = fine; }
+                    nev++;
+                }
+                s.evflag[now][e] = 0;
+            }
+        }
+#pragma omp parallel for schedule(static)
+        for (int32_t c = 0; c < d->n_cells; c++) do_cell(d, &s, c, now, nxt, step, time, dt);
+        if (!initializing) {
+            time += dt_f64;
+            fine++;
+        }
+        now ^= 1; nxt ^= 1;
+    }
+    if (n_ev) *n_ev = nev;
+    if (final_v)
+        for (int64_t g = 0; g < NC; g++) final_v[g] = (float)s.V[now][g];
+    for (int b = 0; b < 2; b++) {
+        free(s.V[b]); free(s.q[b]); free(s.trig[b]); free(s.nspk[b]); free(s.sA[b]); free(s.sB[b]);
+        free(s.pT[b]); free(s.pI[b]); free(s.pA[b]); free(s.pB[b]); free(s.evflag[b]);
+    }
+    free(s.D); free(s.order); free(s.gate_begin); free(s.comp_ev_begin); free(s.comp_ev);
+    return rows;
+}

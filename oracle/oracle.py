@@ -1,0 +1,84 @@
+"""ctypes binding of the CPU restatement (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py, as the checker. The product path (eden_amd) never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = C.CDLL(LIB)
+        for sfx in ("f32", "f32libm", "f64"):
+            run = getattr(L, f"oracle_{sfx}_run")
+            run.restype = C.c_int64
+            run.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_int64, C.POINTER(C.c_int64), C.c_void_p]
+            h = getattr(L, f"oracle_{sfx}_hash64shift")
+            h.restype = C.c_uint64
+            h.argtypes = [C.c_uint64]
+            r = getattr(L, f"oracle_{sfx}_randof")
+            r.restype = C.c_float
+            r.argtypes = [C.c_float, C.c_int32, C.c_int64, C.c_int64, C.c_int32]
+            rt = getattr(L, f"oracle_{sfx}_rate")
+            rt.restype = C.c_double
+            rt.argtypes = [C.c_void_p, C.c_double]
+            o = getattr(L, f"oracle_{sfx}_dfs_order")
+            o.restype = None
+            o.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_000_000, want_trace=True):
+    """Run the restatement on a Model. Returns (trace [rows x n_record] float32,
+    event columns, event rows, final voltages). With substeps > 1, dt is divided and the
+    trace keeps every substeps-th row; event rows are then in fine steps."""
+    L = lib()
+    d = model.desc()
+    total = model.total_rows() if n_rows < 0 else n_rows
+    nrec = len(model.record_comp)
+    trace = np.zeros((max(total, 1), max(nrec, 1)), np.float32) if want_trace else None
+    ev_col = np.zeros(max_events, np.int64)
+    ev_row = np.zeros(max_events, np.int64)
+    n_ev = C.c_int64()
+    final_v = np.zeros(model.n_comps, np.float32)
+    fn = getattr(L, f"oracle_{precision}_run")
+    rows = fn(C.byref(d), n_rows, substeps, threads,
+              None if trace is None else trace.ctypes.data, ev_col.ctypes.data, ev_row.ctypes.data, max_events,
+              C.byref(n_ev), final_v.ctypes.data)
+    if rows < 0:
+        raise RuntimeError("oracle run failed")
+    ne = min(n_ev.value, max_events)
+    tr = None if trace is None else trace[:rows, :nrec]
+    return tr, ev_col[:ne].copy(), ev_row[:ne].copy(), final_v
+
+
+def hash64shift(x):
+    return int(lib().oracle_f32_hash64shift(x))
+
+
+def randof(x, work_item, instance, step, invocation):
+    return float(lib().oracle_f32_randof(x, work_item, instance, step, invocation))
+
+
+def rate(r, v, precision="f32"):
+    return float(getattr(lib(), f"oracle_{precision}_rate")(C.byref(r), v))
+
+
+def dfs_order(parent):
+    parent = np.ascontiguousarray(parent, np.int32)
+    out = np.zeros(len(parent), np.int32)
+    lib().oracle_f32_dfs_order(len(parent), parent.ctypes.data, out.ctypes.data)
+    return out

@@ -1,0 +1,98 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle on the same seeded inputs.
+
+Float tolerance: rasters equal up to one dt, voltage max-abs <= 1e-3 mV wherever the rasters
+agree (tests/parity.py). Sizes are scaled down so the oracle finishes in seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from eden_amd import configs
+from eden_amd.model import Simulation
+from oracle import oracle
+from parity import TOL_MV, compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_run(model, **kw):
+    with Simulation(model, **kw) as sim:
+        tr = sim.run_all()
+        ev = sim.read_events()
+    return tr, ev
+
+
+def _check(model, network=False, **kw):
+    _say(f"[parity] {model.name} cells={model.n_cells} comps={model.n_comps} opts={kw}")
+    tr_g, ev_g = _gpu_run(model, **kw)
+    tr_r, ec, er, _ = oracle.run(model)
+    assert tr_g.shape == tr_r.shape
+    rec2ev = None
+    if not network and len(model.event_comp):
+        ev_of_comp = {int(c): i for i, c in enumerate(model.event_comp)}
+        rec2ev = [ev_of_comp.get(int(c), 0) for c in model.record_comp]
+    res = compare(tr_g, ev_g, tr_r, (ec, er), len(model.event_comp), rec2ev, network=network)
+    _say(f"[parity] {model.name}: {res}")
+    assert res["max_abs_mV"] <= TOL_MV, res
+    return res, (tr_g, ev_g), (tr_r, (ec, er))
+
+
+def test_hh_population_small():
+    res, g, r = _check(configs.hh_population(n_cells=200, t_final_ms=100.0))
+    assert res["spikes_ref"] > 0
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+def test_random_trees_small():
+    sizes = [20, 35, 64, 65, 100, 128, 129, 200, 256, 257, 400, 31]
+    res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=60.0, seed=7))
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+def test_random_trees_all_compartments_recorded():
+    sizes = [23, 77, 150]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=30.0, seed=11, record_all=True)
+    res, g, r = _check(m)
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+def test_large_trees_workgroup_per_cell():
+    sizes = [513, 1000, 1500, 2000, 700]
+    res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=25.0, seed=13))
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+def test_hh_everywhere_trees():
+    sizes = [100] * 6
+    res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=40.0, seed=5001,
+                                            hh_everywhere=True))
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+@pytest.mark.parametrize("tpb", [64, 128, 256])
+def test_launch_shapes_agree(tpb):
+    sizes = [60, 120, 250]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=20.0, seed=17)
+    _check(m, threads_per_cell=tpb)
+
+
+def test_chunked_run_matches_single_run():
+    m = configs.random_trees(4, sizes=[40, 80, 120, 160], t_final_ms=20.0, seed=19)
+    with Simulation(m) as s:
+        full = s.run_all()
+    with Simulation(m, chunk_steps=97) as s:
+        parts = s.run_all()
+    assert np.array_equal(full, parts)
+
+
+def test_spiking_network_small():
+    m = configs.spiking_network(n_cells=300, in_degree=20, t_final_ms=60.0, poisson_hz=200.0,
+                                weight_mu=2.0, n_record=50)
+    res, g, r = _check(m, network=True)
+    assert res["spikes_ref"] > 0
