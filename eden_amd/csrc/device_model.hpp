@@ -28,7 +28,7 @@ enum RateForm : int32_t { RATE_EXP = 0, RATE_EXPLINEAR = 1, RATE_SIGMOID = 2, RA
 
 // One gate of a channel type, 32 bytes (two ds_read_b128 from LDS).
 struct alignas(16) DGate {
-    int32_t forms;   // alpha form | beta form << 8 | instances << 16
+    int32_t forms;   // alpha form | beta form << 8 | instances << 16 | gates of the type << 24
     float q10;
     float ar, am;
     float as, br;
@@ -73,7 +73,7 @@ struct DevTables {
     const uint32_t *cell_seed;    // [n_cells]
     const int32_t *comp_cell;     // [NC] owning cell
     // time
-    const float *t_step;          // [rows] time_f32 of each recorded step
+    const double *t_row;          // [rows] time (double, as accumulated) at each recorded row
     // outputs
     float *trace;                 // [chunk_rows * n_trace_cols]
     int32_t n_trace_cols;
@@ -88,11 +88,41 @@ struct DevTables {
 // arguments). glibc's expf/logf, which the reference's generated C calls, are within
 // 0.502 ulp, so both sides agree except in those rare cases. Building with
 // -DEDENHIP_NATIVE_EXPF selects the device's own expf/logf instead.
+//
+// exp in double: Cody-Waite reduction x = k ln2 + r with |r| <= ln2/2 (ln2 split in a
+// 53-bit head and a tail), the Taylor series of e^r to degree 13 (truncation error below
+// 4e-18 relative) evaluated by Horner in double, then scaling by 2^k. Total relative error
+// is a few double ulps, far below the 2^-24 spacing the final rounding to float resolves.
+__device__ __forceinline__ double exp_taylor(double x) {
+    const double LOG2E = 1.4426950408889634074;
+    const double LN2_HI = 0x1.62e42ff000000p-1;    // ln2 rounded to 32 significant bits: k * LN2_HI is exact
+    const double LN2_LO = -0x1.718432a1b0e26p-35;  // ln2 - LN2_HI
+    const double k = rint(x * LOG2E);
+    double r = fma(-k, LN2_HI, x);
+    r = fma(-k, LN2_LO, r);
+    double p = 1.0 / 6227020800.0;                      // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
 __device__ __forceinline__ float exp_f(float x) {
 #ifdef EDENHIP_NATIVE_EXPF
     return expf(x);
 #else
-    return (float)exp((double)x);
+    if (!(x > -200.0f && x < 200.0f)) return expf(x);   // saturated, NaN or inf: the float result is exact
+    return (float)exp_taylor((double)x);
 #endif
 }
 
@@ -104,13 +134,22 @@ __device__ __forceinline__ float log_f(float x) {
 #endif
 }
 
+// Rate forms without divergent branches: one exp and at most one extra division per rate,
+// each form's value picked by selects. Every form keeps the reference's operation order:
+//   EXP        r * exp((V - mid) / scale)
+//   EXPLINEAR  r * ((V == mid) ? 1 : ((V - mid) / scale) / (1 - exp(-(V - mid) / scale)))
+//   SIGMOID    r / (1 + exp((mid - V) / scale))
+//   FIXED      r
 __device__ __forceinline__ float rate_eval(int form, float r, float mid, float scale, float V) {
-    if (form == RATE_EXP) return r * exp_f((V - mid) / scale);
-    if (form == RATE_EXPLINEAR) {
-        return r * ((V == mid) ? 1.0f : (((V - mid) / scale) / (1.0f - exp_f(-(V - mid) / scale))));
-    }
-    if (form == RATE_SIGMOID) return r / (1.0f + exp_f((mid - V) / scale));
-    return r;
+    const bool sig = form == RATE_SIGMOID, lin = form == RATE_EXPLINEAR;
+    const float x = (sig ? (mid - V) : (V - mid)) / scale;
+    const float e = exp_f(lin ? -x : x);
+    const float num = lin ? x : r;
+    const float den = lin ? (1.0f - e) : (1.0f + e);
+    const float q = num / den;
+    const float v_exp = r * e;
+    const float v_lin = r * ((V == mid) ? 1.0f : q);
+    return form == RATE_EXP ? v_exp : (lin ? v_lin : (sig ? q : r));
 }
 
 // Thomas Wang's 64-bit integer hash (published 1997); EDEN's RNG mixer
@@ -222,11 +261,13 @@ __device__ __forceinline__ void channels_step(const DGate *__restrict__ gates, c
 }
 
 __device__ __forceinline__ float pulses_current(const DevTables &T, const CompConst &k, int64_t gidx, float t) {
+    // one pulse: 0 + (0 + amplitude) + 0 = amplitude exactly, or 0
+    if (!(k.gmeta & GM_MORE_PULSES)) return (k.p0s <= t && t <= k.p0e) ? k.p0a : 0.0f;
     float I = 0.0f;
-    if (k.gmeta & GM_PULSE) {
+    {
         float Ip = 0.0f;
         if (k.p0s <= t && t <= k.p0e) Ip += k.p0a;
-        if (k.gmeta & GM_MORE_PULSES) {
+        {
             const int32_t pb = T.pulse_begin[gidx], pe = T.pulse_begin[gidx + 1];
             for (int p = pb + 1; p < pe; p++)
                 if (T.pulse_start[p] <= t && t <= T.pulse_end[p]) Ip += T.pulse_amp[p];

@@ -26,8 +26,12 @@ namespace {
 
 struct Bucket {
     int tpb = 64;
+    bool split = false;   // one extra wave runs the channel phase concurrently
     std::vector<int32_t> order;    // tree-cell indices, largest first
     int32_t nmax = 0;
+    int32_t ngmax = 0;
+    int32_t ninmax = 0;
+    int32_t nctmax = 0;
     int32_t words_max = 0;
     size_t lds = 0;
     int grid = 0;
@@ -66,6 +70,8 @@ struct edenhip_ctx {
     uint32_t *d_words = nullptr;
     int32_t *d_pos = nullptr;
     float2 *d_u = nullptr;
+    uint32_t *d_tcm = nullptr, *d_ctask = nullptr, *d_inl = nullptr;
+    float *d_gq = nullptr;
     std::vector<Bucket> buckets;
     hipEvent_t ev_start = nullptr;
     int n_cu = 256;
@@ -122,13 +128,14 @@ static int fail(edenhip_ctx *ctx, int code, const std::string &msg) {
     return code;
 }
 
-// time_f32 of every recorded step: the reference accumulates `time += dt` in double from
-// t_initial = 0 and casts to float per step (eden/Eden.cpp:11930-11932, 3252).
-static std::vector<float> step_times(float dt, double t_final) {
-    std::vector<float> ts;
+// time of every recorded row: the reference accumulates `time += dt` in double from
+// t_initial = 0 while time <= t_final, and casts to float per step (eden/Eden.cpp:11488,
+// 11930-11932, 3252). Kernels continue the same double accumulation from a chunk's first row.
+static std::vector<double> row_times(float dt, double t_final) {
+    std::vector<double> ts;
     double time = 0;
     while (time <= t_final) {
-        ts.push_back((float)time);
+        ts.push_back(time);
         time += (double)dt;
     }
     return ts;
@@ -160,7 +167,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             for (const edenhip_rate *r : {&g.alpha, &g.beta})
                 if (r->form < 0 || r->form > EDENHIP_RATE_FIXED) return fail(ctx, EDENHIP_E_INVALID, "unknown rate form");
             DGate &G = gates[(size_t)t * GATES_PER_TYPE + gi];
-            G.forms = g.alpha.form | (g.beta.form << 8) | (g.instances << 16);
+            G.forms = g.alpha.form | (g.beta.form << 8) | (g.instances << 16) | (ct.n_gates << 24);
             G.q10 = g.q10;
             G.ar = g.alpha.rate; G.am = g.alpha.midpoint; G.as = g.alpha.scale;
             G.br = g.beta.rate; G.bm = g.beta.midpoint; G.bs = g.beta.scale;
@@ -282,7 +289,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     }
 
     // time table
-    std::vector<float> ts = step_times(d->dt, d->t_final);
+    std::vector<double> ts = row_times(d->dt, d->t_final);
     ctx->total_rows = (int64_t)ts.size();
 
     // cells: point (1 compartment) vs tree
@@ -292,6 +299,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     std::vector<uint32_t> words;
     std::vector<int32_t> posv(NC, 0);
     std::vector<float2> uv(NC, make_float2(0.f, 0.f));
+    // tree cells: compartment word (channel count | gated-slot mask << 3 | first channel task
+    // << 16), channel tasks (compartment | channel type << 11 | first gate << 17) and the gate
+    // state, in compartment / distribution / gate order
+    std::vector<uint32_t> tcm(NC, 0), ctask, inl;
+    std::vector<float> gqv;
     for (int c = 0; c < d->n_cells; c++) {
         const int64_t c0 = d->cell_comp_begin[c], c1 = d->cell_comp_begin[c + 1];
         const int32_t n = (int32_t)(c1 - c0);
@@ -303,13 +315,49 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             point.push_back((int32_t)c0);
             continue;
         }
-        if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2048 compartments");
+        if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2047 compartments");
         for (int64_t g = c0; g < c1; g++)
             if (d->comp_syn_begin[g + 1] > d->comp_syn_begin[g])
                 return fail(ctx, EDENHIP_E_LIMIT, "chemical synapses on multi-compartment cells are not supported yet");
         const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell);
         TreePlan P = plan_tree(n, d->parent + c0, tpb);
         TreeCell tc{};
+        tc.gate_off = (int64_t)gqv.size();
+        tc.ctask_off = (int32_t)ctask.size();
+        for (int32_t i = 0; i < n; i++) {
+            const int64_t g = c0 + i;
+            const int32_t first_task = (int32_t)(ctask.size() - tc.ctask_off);
+            uint32_t mask = 0;
+            for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
+                const int slot = (int)(ch - d->comp_chan_begin[g]);
+                const edenhip_channel_type &ct = d->channel_types[d->chan_type[ch]];
+                if (ct.n_gates == 0) continue;
+                const int64_t first_gate = (int64_t)gqv.size() - tc.gate_off;
+                if (first_gate >= (1 << 15)) return fail(ctx, EDENHIP_E_LIMIT, "more than 32767 gate variables in a cell");
+                mask |= 1u << slot;
+                ctask.push_back((uint32_t)i | ((uint32_t)d->chan_type[ch] << 11) | ((uint32_t)first_gate << 17));
+                for (int gi = 0; gi < ct.n_gates; gi++) {
+                    const double a = edenhip_rate_host(ct.gates[gi].alpha, d->v0[g]);
+                    const double b = edenhip_rate_host(ct.gates[gi].beta, d->v0[g]);
+                    gqv.push_back((float)(a / (a + b)));
+                }
+            }
+            const int nch = (int)(d->comp_chan_begin[g + 1] - d->comp_chan_begin[g]);
+            if (first_task > 0xFFFF) return fail(ctx, EDENHIP_E_LIMIT, "more than 65535 gated channels in a cell");
+            tcm[g] = (uint32_t)nch | (mask << 3) | ((uint32_t)first_task << 16);
+        }
+        tc.n_ctask = (int32_t)(ctask.size() - tc.ctask_off);
+        tc.n_gate = (int32_t)(gqv.size() - tc.gate_off);
+        // input list: compartments with Poisson synapse inputs
+        tc.input_off = (int32_t)inl.size();
+        for (int32_t i = 0; i < n; i++) {
+            const int64_t g = c0 + i;
+            if (d->comp_poisson_begin[g + 1] == d->comp_poisson_begin[g]) continue;
+            inl.push_back((uint32_t)i);
+            tcm[g] |= TCM_INPUT;
+        }
+        tc.n_input = (int32_t)(inl.size() - tc.input_off);
+
         tc.comp_begin = c0;
         tc.n = n;
         tc.rounds_fwd = P.rounds_fwd;
@@ -374,7 +422,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     TRY(alloc_zero(ctx, &T.poi_B, (size_t)n_poi));
     TRY(upload(ctx, (uint32_t **)&T.cell_seed, seeds));
     TRY(upload(ctx, (int32_t **)&T.comp_cell, comp_cell));
-    TRY(upload(ctx, (float **)&T.t_step, ts));
+    TRY(upload(ctx, (double **)&T.t_row, ts));
     T.n_trace_cols = ctx->n_dev_cols;
     T.total_rows = ctx->total_rows;
 
@@ -413,6 +461,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     TRY(upload(ctx, &ctx->d_words, words));
     TRY(upload(ctx, &ctx->d_pos, posv));
     TRY(upload(ctx, &ctx->d_u, uv));
+    TRY(upload(ctx, &ctx->d_tcm, tcm));
+    TRY(upload(ctx, &ctx->d_ctask, ctask));
+    TRY(upload(ctx, &ctx->d_gq, gqv));
+    TRY(upload(ctx, &ctx->d_inl, inl));
+    TRY(snap(ctx->d_gq, gqv.size() * sizeof(float)));
 
     // launch plan for tree cells: bucket by launch shape
     hipDeviceProp_t prop;
@@ -429,7 +482,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         }
         b->order.push_back((int32_t)i);
         b->nmax = std::max(b->nmax, cells[i].n);
-        b->words_max = std::max(b->words_max, (cells[i].rounds_fwd + cells[i].rounds_bwd) * b->tpb);
+        b->ngmax = std::max(b->ngmax, cells[i].n_gate);
+        b->ninmax = std::max(b->ninmax, cells[i].n_input);
+        b->nctmax = std::max(b->nctmax, cells[i].n_ctask);
+        b->words_max = std::max(b->words_max, (cells[i].rounds_fwd + cells[i].rounds_bwd + 2) * b->tpb);
     }
     std::vector<TreeCell> dcells;
     for (auto &b : ctx->buckets) {
@@ -439,11 +495,20 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         for (int32_t i : b.order) bc.push_back(cells[i]);
         std::vector<int32_t> seq(bc.size());
         for (size_t i = 0; i < seq.size(); i++) seq[i] = (int32_t)i;
-        b.lds = (size_t)b.nmax * 2 * sizeof(float2) + (size_t)b.words_max * sizeof(uint32_t);
-        const void *kp = tree_kernel_ptr(b.tpb);
+        // sDX and sU with a dummy slot, Poisson input currents, sV, gate values, channel open
+        // fractions and tasks,
+        // input list, and the task words for workgroups of up to 128 threads
+        b.nctmax = std::max(b.nctmax, 1);   // ungated slots read sF[first task], kept in bounds
+        b.split = b.tpb == 64 && ctx->opt.persistent >= 0;
+        b.lds = (size_t)(b.nmax + 1) * 2 * sizeof(float2) +
+                (size_t)b.nmax * sizeof(float2) + (size_t)(b.nmax + 1) * 2 * sizeof(float) +
+                (size_t)b.ngmax * sizeof(float) + (size_t)b.nctmax * 3 * sizeof(uint32_t) +
+                (size_t)b.ninmax * sizeof(uint32_t) +
+                (b.tpb <= 128 ? (size_t)b.words_max * sizeof(uint32_t) : 0);
+        const void *kp = tree_kernel_ptr(b.tpb, b.split);
         if (!kp) return fail(ctx, EDENHIP_E_LIMIT, "no tree kernel for the chosen workgroup size");
         int per_cu = 0;
-        HIPCHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, b.tpb, b.lds));
+        HIPCHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, b.tpb + (b.split ? 64 : 0), b.lds));
         if (per_cu <= 0) return fail(ctx, EDENHIP_E_LIMIT, "tree kernel does not fit on a CU (LDS)");
         b.grid = (int)std::min<int64_t>((int64_t)bc.size(), (int64_t)per_cu * ctx->n_cu);
         TRY(upload(ctx, &b.d_cells, bc));
@@ -471,7 +536,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
     for (int64_t a = s0; a < s1; a += stride) {
         const int64_t b = std::min(s1, a + stride);
         if (ctx->n_point > 0) {
-            const int tpb = 256;
+            const int tpb = 64;
             const int grid = (ctx->n_point + tpb - 1) / tpb;
             float dt = ctx->dt;
             int32_t n = ctx->n_point;
@@ -486,10 +551,12 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             float dt = ctx->dt;
             int32_t ntask = (int32_t)bk.order.size();
             void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
-                            &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &bk.nmax,
+                            &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &ctx->d_tcm,
+                            &ctx->d_ctask, &ctx->d_gq, &ctx->d_inl, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
                             (void *)&a, (void *)&b, &row0, &dt};
             HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
-            HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb), dim3(bk.grid), dim3(bk.tpb), args, bk.lds, bk.stream));
+            HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split), dim3(bk.grid), dim3(bk.tpb + (bk.split ? 64 : 0)),
+                                          args, bk.lds, bk.stream));
             HIPCHECK(ctx, hipEventRecord(bk.done, bk.stream));
             HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, bk.done, 0));
             nl++;

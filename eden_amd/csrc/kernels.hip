@@ -30,7 +30,7 @@ __device__ __forceinline__ void send_spike(const DevTables &T, int64_t gidx, int
     for (int32_t k = b; k < e; k++) trig[T.spk_target[k]] = 1;   // idempotent flag (EDEN: atomic OR 1)
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
 point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const DSynType *__restrict__ stypes_g,
              int n_stypes, const int32_t *__restrict__ comps, int32_t n, int64_t s0, int64_t s1, int64_t row0,
              float dt) {
@@ -52,9 +52,11 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
     const bool has_poi = T.poi_begin[g + 1] > T.poi_begin[g];
     const bool has_spk = T.spk_begin[g + 1] > T.spk_begin[g];
 
+    // time_f64 accumulates `time += dt` in double, as the reference's engine loop does
+    double time = (s0 >= 1) ? T.t_row[s0 - 1] : 0.0;
     for (int64_t s = s0; s < s1; s++) {
         const bool init = s <= 0;
-        const float t = init ? 0.0f : T.t_step[s - 1];
+        const float t = (float)time;
         const int now = (int)((s + 3) & 1), nxt = now ^ 1;
         if (!init && k.rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + k.rec] = V;
         float I_ch, G_ch;
@@ -79,6 +81,7 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
             log_event(T, k.ev, s);
         }
         V = Vn;
+        if (!init) time += (double)dt;
     }
     T.V[g] = V;
 #pragma unroll
@@ -88,37 +91,132 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
 // LDS-only workgroup barrier: waits for this wave's LDS operations, then (for more than one
 // wave) the hardware barrier. Outstanding global loads/stores (the trace write, the next
 // step's time prefetch) are not drained, unlike __syncthreads().
+// A single wave needs no wait at all: LDS executes one wave's instructions in issue order,
+// so a later read sees an earlier write; only the compiler must not reorder across it.
 template <int TPB>
 __device__ __forceinline__ void lds_barrier() {
     if constexpr (TPB <= 64) {
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
     } else {
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }
 }
 
+// Open fraction of every channel of a compartment from its gate values (in gate order,
+// each multiplied in `instances` times, eden/Eden.cpp:5679-5721), then the channel current
+// and conductance sums in distribution order (eden/Eden.cpp:5805-5898).
+__device__ __forceinline__ void tree_channels(const CompConst &k, uint32_t tcm, const float *sF, float V, float &I_ch,
+                                              float &G_ch) {
+    // channel slot c uses the open fraction of its channel task (gated channels, in slot
+    // order from the compartment's first task) or 1 (conductance_scaling alone)
+    const int nch = tcm & 7;
+    const uint32_t mask = (tcm >> 3) & 15;
+    int idx = tcm >> 16;
+    I_ch = 0.0f; G_ch = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        const bool gated = (mask >> c) & 1;
+        const float f = gated ? sF[idx] : 1.0f;
+        idx += gated ? 1 : 0;
+        const float Gscaled = k.g[c] * f;
+        const float iDensity = Gscaled * (k.e[c] - V) * 1e-5f;
+        const float gDensity = (Gscaled) * 1e-5f;
+        I_ch = (c < nch) ? I_ch + (iDensity * k.A) : I_ch;
+        G_ch = (c < nch) ? G_ch + (gDensity * k.A) : G_ch;
+    }
+}
+
+// Open fraction of a channel from its gate values, multiplied in gate order, each gate
+// `instances` times, starting from conductance_scaling = 1 (eden/Eden.cpp:5679-5721).
+__device__ __forceinline__ float channel_fopen(const DGate *gates, int type, const float *q) {
+    const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
+    float f = 1.0f;
+    for (int gi = 0; gi < ng; gi++) {
+        const int inst = (gates[type * GATES_PER_TYPE + gi].forms >> 16) & 0xFF;
+        const float qv = q[gi];
+        for (int i = 0; i < inst; i++) f = f * qv;
+    }
+    return f;
+}
+
+// One gate's update from the current membrane voltage: rates, tau, inf, cnexp with clipping
+// (eden/Eden.cpp:5236-5259, 5336-5375); initialisation steps set the gate to inf.
+__device__ __forceinline__ float gate_update(const DGate &G, float V, float q, bool init, float dt) {
+    const float alpha = rate_eval(G.forms & 0xFF, G.ar, G.am, G.as, V);
+    const float beta = rate_eval((G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
+    const float tau = 1.0f / (alpha + beta);
+    const float inf = alpha / (alpha + beta);
+    if (init) return inf;
+    const float tau_factor = ((dt * G.q10) / tau);
+    const float blend = exp_f(-tau_factor);
+    float nv = (blend) * q + (1.0f - blend) * inf;
+    if (!(nv > 1e-6f)) nv = 1e-6f;
+    if (!(nv < (float)(1 - 1e-6))) nv = (float)(1 - 1e-6);
+    return nv;
+}
+
+// Gated channels of a cell, one per lane of the channel group (lanes `lane` + `stride` k):
+// next gate values from the voltage, then the channel's open fraction for the next step.
+__device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t *sCT, float *sQ, float *sF_out,
+                                              const float *sV_in, int NCT, int lane, int stride, bool init,
+                                              float dt) {
+    for (int i = lane; i < NCT; i += stride) {
+        const uint32_t w = sCT[i];
+        const int type = (w >> 11) & 63;
+        float *q = sQ + (w >> 17);
+        const float Vc = sV_in[w & 0x7FF];
+        const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
+        for (int gi = 0; gi < ng; gi++) q[gi] = gate_update(gates[type * GATES_PER_TYPE + gi], Vc, q[gi], init, dt);
+        sF_out[i] = channel_fopen(gates, type, q);
+    }
+}
+
 // One workgroup per multi-compartment cell, cells popped from a queue in decreasing size.
-// Thread t holds compartments t, t + TPB, t + 2 TPB, t + 3 TPB in registers. The Hines solve
-// works in LDS on breadth-first positions (tree_schedule.hpp): sDX[p] = (diagonal, rhs) and
-// sU[p] = (Ui, Uj) of the edge from p to its parent, driven by one task word per thread per
-// round.
-template <int TPB>
-__global__ void __launch_bounds__(TPB)
+// The cable group (TPB threads) owns compartments t + TPB j (j < 4) in registers and runs,
+// per step:
+//   inputs     per-cell list of compartments with Poisson inputs -> sIn (LDS)
+//   membrane   currents from the channels' open fractions (sF) and inputs, semi-implicit RHS
+//   Hines      in LDS on breadth-first positions (tree_schedule.hpp): sDX[p] = (diagonal,
+//              rhs), sU[p] = (Ui, Uj) of the edge from p to its parent, one task word per
+//              thread per round
+//   readback   new voltages, spike detection
+// The channel phase (next gate values and open fractions from the present voltages) runs
+// either in the cable group after the membrane phase, or, with SPLIT, in one extra wave at
+// the same time as the cable group's step; voltages and open fractions are then
+// double-buffered and the two meet at one workgroup barrier per step.
+template <int TPB, bool SPLIT>
+__global__ void __launch_bounds__(TPB + (SPLIT ? 64 : 0))
 tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const DSynType *__restrict__ stypes_g,
             int n_stypes, const TreeCell *__restrict__ cells, const int32_t *__restrict__ task_order, int32_t n_tasks,
             int32_t *__restrict__ queue, const uint32_t *__restrict__ words_g, const int32_t *__restrict__ pos_g,
-            const float2 *__restrict__ u_g, int32_t nmax, int64_t s0, int64_t s1, int64_t row0, float dt) {
-    constexpr int K = 4;
+            const float2 *__restrict__ u_g, const uint32_t *__restrict__ tcm_g, const uint32_t *__restrict__ ctask_g,
+            float *__restrict__ gq_g, const uint32_t *__restrict__ inl_g, int32_t nmax, int32_t ngmax, int32_t nctmax,
+            int32_t ninmax, int64_t s0, int64_t s1, int64_t row0, float dt) {
+    constexpr int K = TREE_SLOTS;
+    constexpr int NT = TPB + (SPLIT ? 64 : 0);
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ DGate gates[MAX_CHAN_TYPES * GATES_PER_TYPE];
     __shared__ DSynType stypes[64];
     __shared__ int32_t s_task;
+    // dynamic LDS: sDX, sU [nmax+1] float2 | sIn [nmax] float2 | sV [2][nmax+1] | sQ [ngmax] |
+    //              sF [2][nctmax] | sCT [nctmax] | sIL [ninmax] | task words
     float2 *sDX = reinterpret_cast<float2 *>(smem);
-    float2 *sU = sDX + nmax;
-    uint32_t *sW = reinterpret_cast<uint32_t *>(sU + nmax);
+    float2 *sU = sDX + (nmax + 1);
+    float2 *sIn = sU + (nmax + 1);
+    float *sVb = reinterpret_cast<float *>(sIn + nmax);
+    float *sQ = sVb + 2 * (nmax + 1);
+    float *sFb = sQ + ngmax;
+    uint32_t *sCT = reinterpret_cast<uint32_t *>(sFb + 2 * nctmax);
+    uint32_t *sIL = sCT + nctmax;
+    // task words: in LDS for one- and two-wave cable groups; larger groups read them from
+    // global memory (cached), two rounds ahead
+    constexpr bool WORDS_LDS = TPB <= 128;
+    uint32_t *sW = sIL + ninmax;
     const int tid = threadIdx.x;
-    for (int i = tid; i < n_gates; i += TPB) gates[i] = gates_g[i];
-    for (int i = tid; i < n_stypes; i += TPB) stypes[i] = stypes_g[i];
+    const bool cable = !SPLIT || tid < TPB;
+    for (int i = tid; i < n_gates; i += NT) gates[i] = gates_g[i];
+    for (int i = tid; i < n_stypes; i += NT) stypes[i] = stypes_g[i];
 
     for (;;) {
         __syncthreads();
@@ -128,120 +226,190 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         if (task >= n_tasks) break;
         const TreeCell cell = cells[task_order[task]];
         const int32_t N = cell.n;
+        const int32_t NG = cell.n_gate;
+        const int32_t NCT = cell.n_ctask;
+        const int32_t NIN = cell.n_input;
         const int64_t c0 = cell.comp_begin;
+        const int64_t gq0 = cell.gate_off;
         const int RF = cell.rounds_fwd, RB = cell.rounds_bwd;
-        const int nw = (RF + RB) * TPB;
-        for (int i = tid; i < nw; i += TPB) sW[i] = words_g[cell.words_off + i];
-        for (int i = tid; i < N; i += TPB) sU[i] = u_g[c0 + i];
+        const int nw = (RF + RB + 2) * TPB;
+        const uint32_t *Wg = words_g + cell.words_off;
+        if constexpr (WORDS_LDS)
+            for (int i = tid; i < nw; i += NT) sW[i] = Wg[i];
+        const uint32_t *W = WORDS_LDS ? sW : Wg;
+        for (int i = tid; i < N; i += NT) sU[i] = u_g[c0 + i];
+        for (int i = tid; i < NG; i += NT) sQ[i] = gq_g[gq0 + i];
+        for (int i = tid; i < NCT; i += NT) sCT[i] = ctask_g[cell.ctask_off + i];
+        for (int i = tid; i < NIN; i += NT) sIL[i] = inl_g[cell.input_off + i];
+        for (int i = tid; i < N; i += NT) sVb[i] = T.V[c0 + i];
+        if (tid == 0) {
+            // dummy slot for threads without a task: finite values, never read back
+            sU[N] = make_float2(0.0f, 0.0f);
+            sDX[N] = make_float2(1.0f, 0.0f);
+        }
 
         CompConst kc[K];
+        uint32_t tcm[K];
         float V[K];
-        float q[K][NQ];
         int pos[K];
-        bool has_poi[K];
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const int c = tid + TPB * j;
-            has_poi[j] = false;
             V[j] = 0.0f;
             pos[j] = 0;
-#pragma unroll
-            for (int s = 0; s < NQ; s++) q[j][s] = 0.0f;
-            if (c < N) {
+            tcm[j] = 0;
+            if (cable && c < N) {
                 load_comp_const(T, c0 + c, kc[j]);
+                tcm[j] = tcm_g[c0 + c];
                 V[j] = T.V[c0 + c];
                 pos[j] = pos_g[c0 + c];
-#pragma unroll
-                for (int s = 0; s < NQ; s++) q[j][s] = T.q[(c0 + c) * NQ + s];
-                has_poi[j] = T.poi_begin[c0 + c + 1] > T.poi_begin[c0 + c];
             }
         }
         __syncthreads();
+        // open fractions of the gated channels from the present gate values
+        for (int i = tid; i < NCT; i += NT) {
+            const uint32_t w = sCT[i];
+            sFb[i] = channel_fopen(gates, (w >> 11) & 63, sQ + (w >> 17));
+        }
+        __syncthreads();
 
-        float t_next = (s0 >= 1) ? T.t_step[s0 - 1] : 0.0f;
+        // time_f64 accumulates `time += dt` in double, as the reference's engine loop does
+        double time = (s0 >= 1) ? T.t_row[s0 - 1] : 0.0;
+        int cur = 0;
         for (int64_t s = s0; s < s1; s++) {
             const bool init = s <= 0;
-            const float t = t_next;
-            if (s + 1 < s1) t_next = (s + 1 >= 1) ? T.t_step[s] : 0.0f;   // prefetch the next step's time
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int c = tid + TPB * j;
-                if (c < N) {
-                    if (!init && kc[j].rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + kc[j].rec] = V[j];
-                    float I_ch, G_ch;
-                    channels_step(gates, kc[j], V[j], q[j], init, dt, I_ch, G_ch);
-                    float I_in = pulses_current(T, kc[j], c0 + c, t), G_in = 0.0f;
-                    if (has_poi[j]) poisson_step(T, stypes, c0 + c, V[j], init, s, dt, I_in, G_in);
-                    const float I_syn = 0.0f, G_syn = 0.0f;
-                    const float I_int = I_ch + I_in + I_syn;
-                    const float G_int = G_ch + G_in + G_syn;
-                    const float Dd = 1 * (G_int / kc[j].C) + kc[j].invrc;
-                    const float Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int))) / kc[j].C);
-                    sDX[pos[j]] = make_float2(1 + Dd * dt, Vn);
-                }
-            }
-            if (!init) {
-                lds_barrier<TPB>();
-                // forward elimination, rounds of independent gathers
-                uint32_t w = RF > 0 ? sW[tid] : TASK_IDLE;
-                for (int r = 0; r < RF; r++) {
-                    const uint32_t wn = (r + 1 < RF) ? sW[(r + 1) * TPB + tid] : TASK_IDLE;
-                    if (w != TASK_IDLE) {
-                        const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
-                        float2 dx = sDX[p];
-                        for (int i = 0; i < nc; i++) {
-                            const float2 dxc = sDX[fc + i];
-                            const float2 u = sU[fc + i];
-                            const float ratio = u.y / dxc.x;
-                            dx.x -= ratio * u.x;
-                            dx.y -= ratio * dxc.y;
-                        }
-                        sDX[p] = dx;
+            const float t = (float)time;
+            if (!init) time += (double)dt;
+            const int nxt = SPLIT ? cur ^ 1 : cur;
+            const float *sV = sVb + cur * (nmax + 1);
+            float *sVn = sVb + nxt * (nmax + 1);
+            const float *sF = sFb + cur * nctmax;
+            float *sFn = sFb + nxt * nctmax;
+            if (cable) {
+                // Poisson inputs of the compartments that have any
+                if (NIN > 0) {
+                    for (int i = tid; i < NIN; i += TPB) {
+                        const uint32_t e = sIL[i];
+                        const int c = e & 0x7FF;
+                        float Ii = 0.0f, Gi = 0.0f;
+                        poisson_step(T, stypes, c0 + c, sV[c], init, s, dt, Ii, Gi);
+                        sIn[c] = make_float2(Ii, Gi);
                     }
                     lds_barrier<TPB>();
-                    w = wn;
                 }
-                if (tid == 0) {
-                    float2 r0 = sDX[0];
-                    r0.y = r0.y / r0.x;
-                    sDX[0] = r0;
+                // compartments: currents from the present open fractions, semi-implicit RHS
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int c = tid + TPB * j;
+                    if (c < N) {
+                        if (!init && kc[j].rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + kc[j].rec] = V[j];
+                        float I_ch, G_ch;
+                        tree_channels(kc[j], tcm[j], sF, V[j], I_ch, G_ch);
+                        float I_in = pulses_current(T, kc[j], c0 + c, t), G_in = 0.0f;
+                        if (tcm[j] & TCM_INPUT) {
+                            const float2 in = sIn[c];
+                            I_in += in.x;
+                            G_in += in.y;
+                        }
+                        const float I_syn = 0.0f, G_syn = 0.0f;
+                        const float I_int = I_ch + I_in + I_syn;
+                        const float G_int = G_ch + G_in + G_syn;
+                        const float Dd = 1 * (G_int / kc[j].C) + kc[j].invrc;
+                        const float Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int))) / kc[j].C);
+                        sDX[pos[j]] = make_float2(1 + Dd * dt, Vn);
+                    }
                 }
                 lds_barrier<TPB>();
-                // back-substitution, rounds by depth
-                const uint32_t *sWb = sW + RF * TPB;
-                w = RB > 0 ? sWb[tid] : TASK_IDLE;
-                for (int r = 0; r < RB; r++) {
-                    const uint32_t wn = (r + 1 < RB) ? sWb[(r + 1) * TPB + tid] : TASK_IDLE;
-                    if (w != TASK_IDLE) {
+                if constexpr (!SPLIT) channel_phase(gates, sCT, sQ, sFn, sV, NCT, tid, TPB, init, dt);
+                if (!init) {
+                    // forward elimination, rounds of independent gathers. Straight-line code:
+                    // threads without a task work on the dummy slot N (sDX[N] = (1, 0) and
+                    // sU[N] = (0, 0) stay fixed), and a node with one child takes the dummy
+                    // slot as its second child, whose contribution r1 * 0 is exactly zero.
+                    // Nodes with more than two children fold the rest in a loop. The next
+                    // round's word is always read (the schedule ends with an idle round).
+                    uint32_t w = W[tid];
+                    uint32_t wn = W[TPB + tid];
+                    for (int r = 0; r < RF; r++) {
+                        // rounds r + 1 and r + 2 exist: the sweep ends with an idle round and
+                        // the backward words follow
+                        const uint32_t wnn = W[(r + 2) * TPB + tid];
+                        const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
+                        const int c1 = nc >= 2 ? fc + 1 : N;
+                        const float2 dx = sDX[p];
+                        const float2 d0 = sDX[fc], u0 = sU[fc];
+                        const float2 d1 = sDX[c1], u1 = sU[c1];
+                        const float r0 = u0.y / d0.x;
+                        const float r1 = u1.y / d1.x;
+                        float2 o;
+                        o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
+                        o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
+                        if (nc > 2) {
+                            for (int i = 2; i < nc; i++) {
+                                const float2 dxc = sDX[fc + i];
+                                const float2 u = sU[fc + i];
+                                const float ratio = u.y / dxc.x;
+                                o.x -= ratio * u.x;
+                                o.y -= ratio * dxc.y;
+                            }
+                        }
+                        sDX[p] = o;
+                        lds_barrier<TPB>();
+                        w = wn;
+                        wn = wnn;
+                    }
+                    if (tid == 0) {
+                        float2 r0 = sDX[0];
+                        r0.y = r0.y / r0.x;
+                        sDX[0] = r0;
+                    }
+                    lds_barrier<TPB>();
+                    // back-substitution, rounds by depth, straight-line as above
+                    const uint32_t *Wb = W + (RF + 1) * TPB;
+                    w = Wb[tid];
+                    wn = Wb[TPB + tid];
+                    for (int r = 0; r < RB; r++) {
+                        const uint32_t wnn = (r + 2 <= RB) ? Wb[(r + 2) * TPB + tid] : 0u;
                         const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
                         float2 dx = sDX[p];
+                        const float ui = sU[p].x;
                         const float xp = sDX[pp].y;
-                        dx.y = (dx.y - sU[p].x * xp) / dx.x;
+                        dx.y = (dx.y - ui * xp) / dx.x;
                         sDX[p] = dx;
+                        lds_barrier<TPB>();
+                        w = wn;
+                        wn = wnn;
                     }
-                    lds_barrier<TPB>();
-                    w = wn;
                 }
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int c = tid + TPB * j;
+                    if (c < N) {
+                        const float Vn = init ? V[j] : sDX[pos[j]].y;
+                        if (V[j] < kc[j].vthr && kc[j].vthr < Vn) log_event(T, kc[j].ev, s);
+                        V[j] = Vn;
+                        sVn[c] = Vn;
+                    }
+                }
+            } else {
+                channel_phase(gates, sCT, sQ, sFn, sV, NCT, tid - TPB, 64, init, dt);
             }
+            // the next step reads voltages and open fractions written above by other lanes
+            if constexpr (SPLIT) {
+                asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            } else {
+                lds_barrier<TPB>();
+            }
+            cur = nxt;
+        }
+        if (cable) {
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 const int c = tid + TPB * j;
-                if (c < N) {
-                    const float Vn = init ? V[j] : sDX[pos[j]].y;
-                    if (V[j] < kc[j].vthr && kc[j].vthr < Vn) log_event(T, kc[j].ev, s);
-                    V[j] = Vn;
-                }
+                if (c < N) T.V[c0 + c] = V[j];
             }
         }
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int c = tid + TPB * j;
-            if (c < N) {
-                T.V[c0 + c] = V[j];
-#pragma unroll
-                for (int s = 0; s < NQ; s++) T.q[(c0 + c) * NQ + s] = q[j][s];
-            }
-        }
+        for (int i = tid; i < NG; i += NT) gq_g[gq0 + i] = sQ[i];
     }
     // the last workgroup out resets the queue for the next launch
     if (tid == 0) {
@@ -253,21 +421,24 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     }
 }
 
-#define EDENHIP_TREE_INST(TPB)                                                                                     \
-    template __global__ void tree_kernel<TPB>(DevTables, const DGate *, int, const DSynType *, int, const TreeCell *, \
-                                              const int32_t *, int32_t, int32_t *, const uint32_t *, const int32_t *,  \
-                                              const float2 *, int32_t, int64_t, int64_t, int64_t, float);
-EDENHIP_TREE_INST(64)
-EDENHIP_TREE_INST(128)
-EDENHIP_TREE_INST(256)
-EDENHIP_TREE_INST(512)
+#define EDENHIP_TREE_INST(TPB, SPLIT)                                                                              \
+    template __global__ void tree_kernel<TPB, SPLIT>(                                                              \
+        DevTables, const DGate *, int, const DSynType *, int, const TreeCell *, const int32_t *, int32_t, int32_t *,  \
+        const uint32_t *, const int32_t *, const float2 *, const uint32_t *, const uint32_t *, float *,             \
+        const uint32_t *, int32_t, int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
+EDENHIP_TREE_INST(64, true)
+EDENHIP_TREE_INST(64, false)
+EDENHIP_TREE_INST(128, false)
+EDENHIP_TREE_INST(256, false)
+EDENHIP_TREE_INST(512, false)
 
-const void *tree_kernel_ptr(int tpb) {
+const void *tree_kernel_ptr(int tpb, bool split) {
+    if (split) return tpb == 64 ? reinterpret_cast<const void *>(&tree_kernel<64, true>) : nullptr;
     switch (tpb) {
-    case 64: return reinterpret_cast<const void *>(&tree_kernel<64>);
-    case 128: return reinterpret_cast<const void *>(&tree_kernel<128>);
-    case 256: return reinterpret_cast<const void *>(&tree_kernel<256>);
-    case 512: return reinterpret_cast<const void *>(&tree_kernel<512>);
+    case 64: return reinterpret_cast<const void *>(&tree_kernel<64, false>);
+    case 128: return reinterpret_cast<const void *>(&tree_kernel<128, false>);
+    case 256: return reinterpret_cast<const void *>(&tree_kernel<256, false>);
+    case 512: return reinterpret_cast<const void *>(&tree_kernel<512, false>);
     default: return nullptr;
     }
 }

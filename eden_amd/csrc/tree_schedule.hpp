@@ -25,8 +25,7 @@ inline double edenhip_rate_host(const edenhip_rate &r, double V) {
     }
 }
 
-constexpr uint32_t TASK_IDLE = 0xFFFFFFFFu;
-constexpr int32_t MAX_TREE_N = 2048;   // 11-bit positions in task words
+constexpr int32_t MAX_TREE_N = 2047;   // 11-bit positions in task words, plus the dummy slot
 
 // Schedule of one tree for a workgroup of `tpb` threads.
 //
@@ -43,12 +42,15 @@ constexpr int32_t MAX_TREE_N = 2048;   // 11-bit positions in task words
 // Each group is split into rounds of `tpb` tasks, one 32-bit word per thread:
 //   forward:  pos | first_child_pos << 11 | n_children << 22
 //   backward: pos | parent_pos << 11
-// with TASK_IDLE for threads without work in that round.
+// Threads without work in a round get an idle word that points every access at the dummy
+// slot n (forward: node n with one child n; backward: node n with parent n), so the rounds
+// run as straight-line code. Each sweep is followed by one extra idle round, read as the
+// prefetch of its last round.
 struct TreePlan {
     std::vector<int32_t> pos;          // [n] BFS position of each compartment
     std::vector<int32_t> parent_pos;   // [n] by position
     int32_t rounds_fwd = 0, rounds_bwd = 0;
-    std::vector<uint32_t> words;       // (rounds_fwd + rounds_bwd) * tpb
+    std::vector<uint32_t> words;       // (rounds_fwd + 1 + rounds_bwd + 1) * tpb
 };
 
 inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
@@ -70,12 +72,14 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
     const int32_t H = height[0];
     int32_t Dm = 0;
     for (int32_t i = 0; i < n; i++) Dm = std::max(Dm, depth[i]);
-    auto emit = [&](const std::vector<uint32_t> &tasks, int32_t &rounds) {
+    const uint32_t idle_fwd = (uint32_t)n | ((uint32_t)n << 11) | (1u << 22);
+    const uint32_t idle_bwd = (uint32_t)n | ((uint32_t)n << 11);
+    auto emit = [&](const std::vector<uint32_t> &tasks, int32_t &rounds, uint32_t idle) {
         const int32_t r = (int32_t)((tasks.size() + tpb - 1) / tpb);
         for (int32_t k = 0; k < r; k++)
             for (int32_t t = 0; t < tpb; t++) {
                 const size_t idx = (size_t)k * tpb + t;
-                P.words.push_back(idx < tasks.size() ? tasks[idx] : TASK_IDLE);
+                P.words.push_back(idx < tasks.size() ? tasks[idx] : idle);
             }
         rounds += r;
     };
@@ -87,8 +91,9 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
                 const uint32_t fc = (uint32_t)P.pos[ch[i][0]];
                 tasks.push_back(p | (fc << 11) | ((uint32_t)ch[i].size() << 22));
             }
-        emit(tasks, P.rounds_fwd);
+        emit(tasks, P.rounds_fwd, idle_fwd);
     }
+    for (int32_t t = 0; t < tpb; t++) P.words.push_back(idle_fwd);
     std::vector<uint32_t> fwd_words;
     fwd_words.swap(P.words);
     for (int32_t d = 1; d <= Dm; d++) {
@@ -97,8 +102,9 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
             const int32_t i = order[p];
             if (depth[i] == d) tasks.push_back((uint32_t)p | ((uint32_t)P.parent_pos[p] << 11));
         }
-        emit(tasks, P.rounds_bwd);
+        emit(tasks, P.rounds_bwd, idle_bwd);
     }
+    for (int32_t t = 0; t < tpb; t++) P.words.push_back(idle_bwd);
     std::vector<uint32_t> bwd_words;
     bwd_words.swap(P.words);
     P.words = fwd_words;
