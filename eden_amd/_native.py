@@ -60,7 +60,8 @@ class ModelDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("threads_per_cell", C.c_int32), ("persistent", C.c_int32), ("chunk_steps", C.c_int32),
-                ("use_graph", C.c_int32), ("profile", C.c_int32), ("device", C.c_int32)]
+                ("use_graph", C.c_int32), ("profile", C.c_int32), ("device", C.c_int32),
+                ("log_synapse_events", C.c_int32)]
 
 
 EXPORTS = {
@@ -73,6 +74,7 @@ EXPORTS = {
     "edenhip_trace_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "edenhip_read_events": (C.c_int, [C.c_void_p, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "edenhip_get_voltages": (C.c_int, [C.c_void_p, _P]),
+    "edenhip_read_synapse_events": (C.c_int, [C.c_void_p, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "edenhip_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_char_p, C.c_int32]),
     "edenhip_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),

@@ -81,6 +81,11 @@ struct DevTables {
     uint32_t *n_events;
     uint32_t max_events;
     int64_t total_rows;
+    // optional log of synaptic events: (step << 32 | synapse instance)
+    int64_t *syn_events;
+    uint32_t *n_syn_events;
+    uint32_t max_syn_events;
+    int32_t log_syn;
 };
 
 // fp32 exp and log rounded once from a double evaluation: correctly rounded except when the
@@ -304,7 +309,7 @@ __device__ __forceinline__ void exptwo_step(const DSynType &st, float V, float &
 // All chemical synapses of a compartment: trigger read + clear, single-slot delay rule,
 // expTwo dynamics, per-type aggregation in list order (eden/Eden.cpp:3636-3740).
 __device__ __forceinline__ void synapses_step(const DevTables &T, const DSynType *__restrict__ stypes, int64_t gidx,
-                                              float V, bool init, float t, float dt, int now,
+                                              float V, bool init, float t, float dt, int now, int64_t step,
                                               float &I_syn, float &G_syn) {
     I_syn = 0.0f; G_syn = 0.0f;
     const int32_t b = T.syn_begin[gidx], e = T.syn_begin[gidx + 1];
@@ -337,6 +342,10 @@ __device__ __forceinline__ void synapses_step(const DevTables &T, const DSynType
             }
         }
         T.syn_next[s] = nn;
+        if (spike_now && T.log_syn) {
+            const uint32_t slot = atomicAdd(T.n_syn_events, 1u);
+            if (slot < T.max_syn_events) T.syn_events[slot] = (step << 32) | (int64_t)s;
+        }
         const DSynType st = stypes[ty];
         float A = T.syn_A[s], B = T.syn_B[s], i, g;
         exptwo_step(st, V, A, B, init, spike_now, dt, i, g);

@@ -435,6 +435,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     T.max_events = 1u << 24;
     TRY(alloc_zero(ctx, &T.events, (size_t)T.max_events));
     TRY(alloc_zero(ctx, &T.n_events, 1));
+    T.log_syn = ctx->opt.log_synapse_events > 0;
+    T.max_syn_events = T.log_syn ? (uint32_t)std::min<int64_t>(ctx->opt.log_synapse_events, (int64_t)1 << 30) : 1u;
+    TRY(alloc_zero(ctx, &T.syn_events, (size_t)T.max_syn_events));
+    TRY(alloc_zero(ctx, &T.n_syn_events, 1));
 
     // snapshot every state table so edenhip_reset can rewind without re-uploading
     auto snap = [&](void *dst, size_t bytes) -> int {
@@ -625,6 +629,7 @@ int edenhip_reset(edenhip_ctx *ctx) {
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     for (auto &s : ctx->snaps) HIPCHECK(ctx, hipMemcpyAsync(s.dst, s.src, s.bytes, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHECK(ctx, hipMemsetAsync(ctx->T.n_events, 0, sizeof(uint32_t), ctx->stream));
+    HIPCHECK(ctx, hipMemsetAsync(ctx->T.n_syn_events, 0, sizeof(uint32_t), ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->rows_done = 0;
     ctx->next_step = -3;
@@ -680,6 +685,22 @@ int edenhip_read_events(edenhip_ctx *ctx, int64_t *col, int64_t *row, int64_t ma
         for (int64_t i = 0; i < m; i++) { row[i] = pr[i].first; col[i] = pr[i].second; }
     }
     if (n) *n = k;
+    return 0;
+}
+
+int edenhip_read_synapse_events(edenhip_ctx *ctx, int64_t *syn, int64_t *step, int64_t max_events, int64_t *n) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    uint32_t cnt = 0;
+    HIPCHECK(ctx, hipMemcpy(&cnt, ctx->T.n_syn_events, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (ctx->T.log_syn && cnt > ctx->T.max_syn_events) return fail(ctx, EDENHIP_E_LIMIT, "synaptic event log overflow");
+    std::vector<int64_t> ev(cnt);
+    if (cnt) HIPCHECK(ctx, hipMemcpy(ev.data(), ctx->T.syn_events, cnt * sizeof(int64_t), hipMemcpyDeviceToHost));
+    std::sort(ev.begin(), ev.end());
+    for (int64_t k = 0; k < (int64_t)cnt && k < max_events; k++) {
+        if (syn) syn[k] = ev[k] & 0xFFFFFFFF;
+        if (step) step[k] = ev[k] >> 32;
+    }
+    if (n) *n = cnt;
     return 0;
 }
 

@@ -62,7 +62,7 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
         float I_ch, G_ch;
         channels_step(gates, k, V, q, init, dt, I_ch, G_ch);
         float I_syn = 0.0f, G_syn = 0.0f;
-        if (has_syn) synapses_step(T, stypes, g, V, init, t, dt, now, I_syn, G_syn);
+        if (has_syn) synapses_step(T, stypes, g, V, init, t, dt, now, s, I_syn, G_syn);
         float I_in = pulses_current(T, k, g, t), G_in = 0.0f;
         if (has_poi) poisson_step(T, stypes, g, V, init, s, dt, I_in, G_in);
         const float I_int = I_ch + I_in + I_syn;

@@ -202,11 +202,12 @@ class EdenHipError(RuntimeError):
 class Simulation:
     """One model on one GPU. Rows are recorded steps (state at t = row * dt)."""
 
-    def __init__(self, model, threads_per_cell=0, persistent=1, chunk_steps=0, profile=False, device=0):
+    def __init__(self, model, threads_per_cell=0, persistent=1, chunk_steps=0, profile=False, device=0,
+                 log_synapse_events=0):
         self.lib = N.load_library()
         self.model = model
         self._desc = model.desc()
-        opt = N.Options(threads_per_cell, persistent, chunk_steps, 0, 1 if profile else 0, device)
+        opt = N.Options(threads_per_cell, persistent, chunk_steps, 0, 1 if profile else 0, device, log_synapse_events)
         h = C.c_void_p()
         rc = self.lib.edenhip_create(C.byref(self._desc), C.byref(opt), C.byref(h))
         self.h = h
@@ -259,6 +260,16 @@ class Simulation:
         row = np.empty(max(1, n.value), np.int64)
         self._check(self.lib.edenhip_read_events(self.h, N.ptr(col), N.ptr(row), n.value, C.byref(n)), "read_events")
         return col[: n.value], row[: n.value]
+
+    def read_synapse_events(self):
+        """(steps, synapses) of the logged synaptic events, sorted by (step, synapse)."""
+        n = C.c_int64()
+        self._check(self.lib.edenhip_read_synapse_events(self.h, None, None, 0, C.byref(n)), "read_synapse_events")
+        syn = np.empty(max(1, n.value), np.int64)
+        step = np.empty(max(1, n.value), np.int64)
+        self._check(self.lib.edenhip_read_synapse_events(self.h, N.ptr(syn), N.ptr(step), n.value, C.byref(n)),
+                    "read_synapse_events")
+        return step[: n.value], syn[: n.value]
 
     def voltages(self):
         v = np.empty(self.model.n_comps, np.float32)

@@ -164,6 +164,7 @@ typedef struct edenhip_options {
     int32_t use_graph;            /* 1 = capture the per-step launches of a chunk in a hipGraph */
     int32_t profile;              /* 1 = time each launch with HIP events */
     int32_t device;               /* HIP device ordinal */
+    int32_t log_synapse_events;   /* > 0: log up to this many synaptic events (see below) */
 } edenhip_options;
 
 typedef struct edenhip_ctx edenhip_ctx;
@@ -196,6 +197,12 @@ int edenhip_trace_device_ptr(edenhip_ctx *ctx, const float **dptr);
  * order: the spike detected while computing row r - 1 is logged at row r with timestamp
  * r * dt (eden/Eden.cpp:11598-11635). Copies up to max_events pairs; *n gets the total. */
 int edenhip_read_events(edenhip_ctx *ctx, int64_t *col, int64_t *row, int64_t max_events, int64_t *n);
+
+/* Synaptic events logged so far (options.log_synapse_events > 0): every (synapse instance,
+ * step) at which a spike-triggered synapse receives its event, i.e. the step whose time
+ * window [t, t + dt) holds the pending arrival time (eden/Eden.cpp:3649-3685). Sorted by
+ * (step, synapse); steps count from 1 for the first recorded row, as in the event log. */
+int edenhip_read_synapse_events(edenhip_ctx *ctx, int64_t *syn, int64_t *step, int64_t max_events, int64_t *n);
 
 /* Copy the membrane voltage of every compartment (n_comps floats) to host memory. */
 int edenhip_get_voltages(edenhip_ctx *ctx, float *dst);

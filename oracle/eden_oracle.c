@@ -118,6 +118,26 @@ static real rate_value(const edenhip_rate *r, real V) {
     }
 }
 
+/* Optional synaptic event log: every (synapse instance, step) at which a spike-triggered
+ * synapse receives its event (spike_now, eden/Eden.cpp:3649-3685). Set before a run. */
+static int64_t *g_syn_ev = NULL;
+static int64_t g_syn_ev_max = 0;
+static int64_t g_syn_ev_n = 0;
+
+void FN(set_synapse_log)(int64_t *buf, int64_t max) {
+    g_syn_ev = buf;
+    g_syn_ev_max = max;
+    g_syn_ev_n = 0;
+}
+
+int64_t FN(synapse_log_count)(void) { return g_syn_ev_n; }
+
+static void log_syn_event(int64_t syn, int64_t step) {
+    if (!g_syn_ev) return;
+    int64_t k = __atomic_fetch_add(&g_syn_ev_n, 1, __ATOMIC_RELAXED);
+    if (k < g_syn_ev_max) g_syn_ev[k] = (step << 32) | syn;
+}
+
 /* Exported so the tests can compare the rate forms with a direct evaluation. */
 double FN(rate)(const edenhip_rate *r, double V) { return (double)rate_value(r, (real)V); }
 
@@ -253,6 +273,7 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                     }
                 }
                 spike_in_flag = spike_now;
+                if (spike_now) log_syn_event(sy, step);
                 s->nspk[nxt][sy] = next_next_spike;
                 /* expTwoSynapse */
                 real A = s->sA[now][sy], B = s->sB[now][sy];

@@ -34,11 +34,33 @@ def lib():
             rt = getattr(L, f"oracle_{sfx}_rate")
             rt.restype = C.c_double
             rt.argtypes = [C.c_void_p, C.c_double]
+            sl = getattr(L, f"oracle_{sfx}_set_synapse_log")
+            sl.restype = None
+            sl.argtypes = [C.c_void_p, C.c_int64]
+            sc = getattr(L, f"oracle_{sfx}_synapse_log_count")
+            sc.restype = C.c_int64
+            sc.argtypes = []
             o = getattr(L, f"oracle_{sfx}_dfs_order")
             o.restype = None
             o.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
+
+
+def run_with_synapse_log(model, n_rows=-1, precision="f32", max_syn_events=10_000_000, **kw):
+    """run() plus the synaptic event log as sorted (step, synapse) pairs."""
+    L = lib()
+    buf = np.zeros(max_syn_events, np.int64)
+    getattr(L, f"oracle_{precision}_set_synapse_log")(buf.ctypes.data, max_syn_events)
+    try:
+        out = run(model, n_rows=n_rows, precision=precision, **kw)
+        n = int(getattr(L, f"oracle_{precision}_synapse_log_count")())
+    finally:
+        getattr(L, f"oracle_{precision}_set_synapse_log")(None, 0)
+    if n > max_syn_events:
+        raise RuntimeError("synapse log overflow")
+    ev = np.sort(buf[:n])
+    return out, (ev >> 32, ev & 0xFFFFFFFF)
 
 
 def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_000_000, want_trace=True):

@@ -11,7 +11,7 @@ import pytest
 from eden_amd import configs
 from eden_amd.model import Simulation
 from oracle import oracle
-from parity import TOL_MV, compare
+from parity import TOL_MV, compare  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -96,3 +96,49 @@ def test_spiking_network_small():
                                 weight_mu=2.0, n_record=50)
     res, g, r = _check(m, network=True)
     assert res["spikes_ref"] > 0
+
+
+def test_synaptic_events_bit_exact():
+    """Integer event path: every (step, synapse instance) at which a synapse receives its
+    event, GPU against the oracle, exactly."""
+    m = configs.spiking_network(n_cells=400, in_degree=30, t_final_ms=80.0, poisson_hz=150.0,
+                                weight_mu=2.0, n_record=10)
+    _say(f"[parity] synaptic events {m.name} synapses={m.n_synapses}")
+    with Simulation(m, log_synapse_events=5_000_000) as sim:
+        sim.run_all()
+        g_step, g_syn = sim.read_synapse_events()
+        ev_g = sim.read_events()
+    (_, ec, er, _), (o_step, o_syn) = oracle.run_with_synapse_log(m)
+    _say(f"[parity] synaptic events: gpu {len(g_step)} oracle {len(o_step)} spikes {len(er)}")
+    assert len(o_step) > 100
+    assert np.array_equal(ev_g[0], ec) and np.array_equal(ev_g[1], er)
+    assert np.array_equal(g_step, o_step)
+    assert np.array_equal(g_syn, o_syn)
+
+
+@pytest.mark.parametrize("delay_ms", [1.0, 15.0])
+def test_delay_rule_on_gpu(delay_ms):
+    from test_oracle import _two_cell_delay_model, expected_deliveries
+    m = _two_cell_delay_model(delay_ms)
+    with Simulation(m, log_synapse_events=100_000) as sim:
+        sim.run_all()
+        step, syn = sim.read_synapse_events()
+        col, row = sim.read_events()
+    D = int(round(delay_ms / 0.025))
+    want = [k for k in expected_deliveries(list(row), D) if k <= m.total_rows()]
+    assert list(step) == want
+
+
+def test_deviation_from_fp64_dt10_is_the_oracles():
+    """Both fp32 paths (GPU and oracle) against the FP64 run at dt/10: the deviations are
+    reported and, the paths being bit-identical here, equal."""
+    m = configs.hh_population(n_cells=50, t_final_ms=60.0)
+    tr_g, ev_g = _gpu_run(m)
+    tr_o, ec, er, _ = oracle.run(m)
+    tr_64, ec64, er64, _ = oracle.run(m, precision="f64", substeps=10)
+    dev_g = float(np.abs(tr_g.astype(np.float64) - tr_64).max())
+    dev_o = float(np.abs(tr_o.astype(np.float64) - tr_64).max())
+    spikes_64 = len(er64)
+    _say(f"[parity] deviation from FP64 dt/10: gpu {dev_g:.4g} mV, oracle fp32 {dev_o:.4g} mV, "
+         f"spikes fp32 {len(er)} fp64/10 {spikes_64}")
+    assert dev_g == dev_o
