@@ -50,6 +50,10 @@ def build_workload(name):
         m = configs.config4()
         desc = ("config4: 4,096 random-tree cells, compartments log-uniform 10-2000, HH soma x U(0.5,1.5), "
                 "passive dendrites, dt 0.025 ms, 500 ms (20,001 rows)")
+    elif name == "config3":
+        m = configs.spiking_network()
+        desc = ("config3: 100,000 one-compartment HH cells (80% E), in-degree 100, expTwo synapses, lognormal "
+                "weights, delays U(1,5) ms, 5 Hz Poisson drive, dt 0.025 ms, 1 s (40,001 rows)")
     elif name == "config1":
         m = configs.hh_population()
         desc = "config1: 1,000 one-compartment HH cells + pulse, dt 0.025 ms, 100 ms (4,001 rows)"

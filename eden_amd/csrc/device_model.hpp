@@ -56,8 +56,12 @@ struct DevTables {
     const float *pulse_start, *pulse_end, *pulse_amp;  // start, start+duration, Imax*weight
     const int32_t *rec_col; // [NC] device trace column or -1
     const int32_t *ev_col;  // [NC] event column or -1
-    // chemical synapses (CSR over compartments)
-    const int32_t *syn_begin;     // [NC+1]
+    // chemical synapses of one-compartment cells, block-interleaved: the cells of a wave
+    // (64 consecutive point cells) store their k-th synapses in 64 consecutive slots, so
+    // synapse k of lane l sits at syn_base[comp] + 64 k
+    const int32_t *syn_base;      // [NC] first slot, or 0
+    const int32_t *syn_cnt;       // [NC] synapses of the compartment
+    const int32_t *syn_orig;      // [slots] instance index in the descriptor (for the event log)
     const int32_t *syn_type;
     const float *syn_weight, *syn_delay;
     float *syn_next, *syn_A, *syn_B;   // state
@@ -307,51 +311,71 @@ __device__ __forceinline__ void exptwo_step(const DSynType &st, float V, float &
 }
 
 // All chemical synapses of a compartment: trigger read + clear, single-slot delay rule,
-// expTwo dynamics, per-type aggregation in list order (eden/Eden.cpp:3636-3740).
-__device__ __forceinline__ void synapses_step(const DevTables &T, const DSynType *__restrict__ stypes, int64_t gidx,
-                                              float V, bool init, float t, float dt, int now, int64_t step,
-                                              float &I_syn, float &G_syn) {
+// expTwo dynamics, per-type aggregation in list order (eden/Eden.cpp:3636-3740). The fields
+// of SYN_CHUNK synapses are loaded before any of them is computed (independent loads in
+// flight); the accumulation stays sequential in synapse order.
+constexpr int SYN_CHUNK = 8;
+
+__device__ __forceinline__ void synapses_step(const DevTables &T, const DSynType *__restrict__ stypes, int32_t base,
+                                              int32_t cnt, float V, bool init, float t, float dt, int now,
+                                              int64_t step, float &I_syn, float &G_syn) {
     I_syn = 0.0f; G_syn = 0.0f;
-    const int32_t b = T.syn_begin[gidx], e = T.syn_begin[gidx + 1];
-    if (e <= b) return;
+    if (cnt <= 0) return;
     float I_agg = 0.0f, G_agg = 0.0f;
     int32_t cur = -1;
     int32_t *trig_now = T.trig[now];
-    for (int32_t s = b; s < e; s++) {
-        const int32_t ty = T.syn_type[s];
-        if (ty != cur) {
-            if (cur >= 0) { I_syn += I_agg; G_syn += G_agg; }
-            I_agg = 0.0f; G_agg = 0.0f; cur = ty;
+    for (int32_t k0 = 0; k0 < cnt; k0 += SYN_CHUNK) {
+        int32_t ty[SYN_CHUNK], tg[SYN_CHUNK];
+        float w[SYN_CHUNK], dl[SYN_CHUNK], nx[SYN_CHUNK], a[SYN_CHUNK], b[SYN_CHUNK];
+#pragma unroll
+        for (int u = 0; u < SYN_CHUNK; u++) {
+            const int32_t s = base + 64 * min(k0 + u, cnt - 1);
+            ty[u] = T.syn_type[s];
+            tg[u] = trig_now[s];
+            w[u] = T.syn_weight[s];
+            dl[u] = T.syn_delay[s];
+            nx[u] = T.syn_next[s];
+            a[u] = T.syn_A[s];
+            b[u] = T.syn_B[s];
         }
-        const float weight = T.syn_weight[s];
-        bool flag = false;
-        if (!init) {
-            flag = trig_now[s] != 0;
-            if (flag) trig_now[s] = 0;
-        }
-        const float delay = T.syn_delay[s];
-        const float next_spike = T.syn_next[s];
-        float nn = next_spike;
-        bool spike_now = false;
-        if (!init) {
-            if (t <= next_spike && next_spike < t + dt) spike_now = true;
-            if (flag) {
-                const float fresh = t + delay;
-                if (t <= fresh && fresh < t + dt) spike_now = true;
-                if (nn < t + dt && nn < fresh) nn = fresh;
+#pragma unroll
+        for (int u = 0; u < SYN_CHUNK; u++) {
+            if (k0 + u >= cnt) break;
+            const int32_t s = base + 64 * (k0 + u);
+            if (ty[u] != cur) {
+                if (cur >= 0) { I_syn += I_agg; G_syn += G_agg; }
+                I_agg = 0.0f; G_agg = 0.0f; cur = ty[u];
             }
+            const float weight = w[u];
+            bool flag = false;
+            if (!init) {
+                flag = tg[u] != 0;
+                if (flag) trig_now[s] = 0;
+            }
+            const float delay = dl[u];
+            const float next_spike = nx[u];
+            float nn = next_spike;
+            bool spike_now = false;
+            if (!init) {
+                if (t <= next_spike && next_spike < t + dt) spike_now = true;
+                if (flag) {
+                    const float fresh = t + delay;
+                    if (t <= fresh && fresh < t + dt) spike_now = true;
+                    if (nn < t + dt && nn < fresh) nn = fresh;
+                }
+            }
+            if (nn != next_spike) T.syn_next[s] = nn;
+            if (spike_now && T.log_syn) {
+                const uint32_t slot = atomicAdd(T.n_syn_events, 1u);
+                if (slot < T.max_syn_events) T.syn_events[slot] = (step << 32) | (int64_t)T.syn_orig[s];
+            }
+            const DSynType st = stypes[ty[u]];
+            float A = a[u], B = b[u], i, g;
+            exptwo_step(st, V, A, B, init, spike_now, dt, i, g);
+            T.syn_A[s] = A; T.syn_B[s] = B;
+            I_agg += i * weight;
+            G_agg += g * weight;
         }
-        T.syn_next[s] = nn;
-        if (spike_now && T.log_syn) {
-            const uint32_t slot = atomicAdd(T.n_syn_events, 1u);
-            if (slot < T.max_syn_events) T.syn_events[slot] = (step << 32) | (int64_t)s;
-        }
-        const DSynType st = stypes[ty];
-        float A = T.syn_A[s], B = T.syn_B[s], i, g;
-        exptwo_step(st, V, A, B, init, spike_now, dt, i, g);
-        T.syn_A[s] = A; T.syn_B[s] = B;
-        I_agg += i * weight;
-        G_agg += g * weight;
     }
     I_syn += I_agg; G_syn += G_agg;
 }

@@ -193,7 +193,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     // per-compartment packing
     std::vector<float> V(NC), C(NC), A(NC), invrc(NC), vthr(NC), chg(NC * NCH, 0.f), che(NC * NCH, 0.f), q(NC * NQ, 0.f);
     std::vector<uint32_t> chmeta(NC), gmeta(NC);
-    std::vector<int32_t> pulse_begin(NC + 1), syn_begin(NC + 1), spk_begin(NC + 1), poi_begin(NC + 1), comp_cell(NC);
+    std::vector<int32_t> pulse_begin(NC + 1), spk_begin(NC + 1), poi_begin(NC + 1), comp_cell(NC);
     for (int c = 0; c < d->n_cells; c++)
         for (int64_t g = d->cell_comp_begin[c]; g < d->cell_comp_begin[c + 1]; g++) comp_cell[g] = c;
     const int64_t n_syn = d->comp_syn_begin[NC], n_spk = d->comp_spike_begin[NC], n_poi = d->comp_poisson_begin[NC];
@@ -230,7 +230,6 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         gm |= (uint32_t)(ce - cb) << 28;
         chmeta[g] = cm; gmeta[g] = gm;
         pulse_begin[g + 1] = (int32_t)d->comp_pulse_begin[g + 1];
-        syn_begin[g + 1] = (int32_t)d->comp_syn_begin[g + 1];
         spk_begin[g + 1] = (int32_t)d->comp_spike_begin[g + 1];
         poi_begin[g + 1] = (int32_t)d->comp_poisson_begin[g + 1];
     }
@@ -240,22 +239,14 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         pend[p] = d->pulse_start[p] + d->pulse_duration[p];       // Start + Duration (fp32)
         pamp[p] = d->pulse_amplitude[p] * d->pulse_weight[p];     // Imax * Weight (fp32)
     }
-    std::vector<int32_t> syn_type(n_syn), spk_target(n_spk);
-    std::vector<float> syn_weight(n_syn), syn_delay(n_syn), syn_next(n_syn, -INFINITY);
-    for (int64_t s = 0; s < n_syn; s++) {
-        syn_type[s] = d->syn_type[s];
-        if (syn_type[s] < 0 || syn_type[s] >= d->n_syn_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
-        syn_weight[s] = d->syn_weight[s];
-        syn_delay[s] = d->syn_delay[s];
-    }
-    for (int64_t k = 0; k < n_spk; k++) {
-        if (d->spike_target[k] < 0 || d->spike_target[k] >= n_syn) return fail(ctx, EDENHIP_E_INVALID, "spike target out of range");
-        spk_target[k] = (int32_t)d->spike_target[k];
-    }
-    // synapse lists must be grouped by type in ascending type order per compartment
+    // synapse types must be grouped in ascending type order per compartment
     for (int64_t g = 0; g < NC; g++)
         for (int64_t s = d->comp_syn_begin[g] + 1; s < d->comp_syn_begin[g + 1]; s++)
             if (d->syn_type[s] < d->syn_type[s - 1]) return fail(ctx, EDENHIP_E_INVALID, "synapses of a compartment must be sorted by type");
+    for (int64_t s = 0; s < n_syn; s++)
+        if (d->syn_type[s] < 0 || d->syn_type[s] >= d->n_syn_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
+    for (int64_t k = 0; k < n_spk; k++)
+        if (d->spike_target[k] < 0 || d->spike_target[k] >= n_syn) return fail(ctx, EDENHIP_E_INVALID, "spike target out of range");
     std::vector<float> poi_mean(n_poi), poi_w(n_poi);
     std::vector<int32_t> poi_st(n_poi), poi_inst(n_poi), poi_roff(n_poi);
     for (int64_t p = 0; p < n_poi; p++) {
@@ -316,9 +307,6 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             continue;
         }
         if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2047 compartments");
-        for (int64_t g = c0; g < c1; g++)
-            if (d->comp_syn_begin[g + 1] > d->comp_syn_begin[g])
-                return fail(ctx, EDENHIP_E_LIMIT, "chemical synapses on multi-compartment cells are not supported yet");
         const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell);
         TreePlan P = plan_tree(n, d->parent + c0, tpb);
         TreeCell tc{};
@@ -378,6 +366,38 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         cell_tpb.push_back(tpb);
     }
     ctx->n_point = (int32_t)point.size();
+    // block-interleaved synapse slots for the point cells (64 per wave, in point order)
+    std::vector<int32_t> syn_base(NC, 0), syn_cnt(NC, 0);
+    std::vector<int64_t> blocked_of(n_syn, -1);
+    int64_t n_slots = 0;
+    for (size_t b0 = 0; b0 < point.size(); b0 += 64) {
+        int64_t kmax = 0;
+        for (size_t l = 0; l < 64 && b0 + l < point.size(); l++) {
+            const int64_t g = point[b0 + l];
+            kmax = std::max<int64_t>(kmax, d->comp_syn_begin[g + 1] - d->comp_syn_begin[g]);
+        }
+        for (size_t l = 0; l < 64 && b0 + l < point.size(); l++) {
+            const int64_t g = point[b0 + l];
+            const int64_t cnt = d->comp_syn_begin[g + 1] - d->comp_syn_begin[g];
+            syn_base[g] = (int32_t)(n_slots + (int64_t)l);
+            syn_cnt[g] = (int32_t)cnt;
+            for (int64_t k = 0; k < cnt; k++) blocked_of[d->comp_syn_begin[g] + k] = n_slots + 64 * k + (int64_t)l;
+        }
+        n_slots += 64 * kmax;
+        if (n_slots >= ((int64_t)1 << 31)) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapse slots");
+    }
+    for (int64_t s = 0; s < n_syn; s++)
+        if (blocked_of[s] < 0) return fail(ctx, EDENHIP_E_LIMIT, "chemical synapses on multi-compartment cells are not supported yet");
+    std::vector<int32_t> syn_type(n_slots, 0), syn_orig(n_slots, -1), spk_target(n_spk);
+    std::vector<float> syn_weight(n_slots, 0.f), syn_delay(n_slots, 0.f), syn_next(n_slots, -INFINITY);
+    for (int64_t s = 0; s < n_syn; s++) {
+        const int64_t k = blocked_of[s];
+        syn_type[k] = d->syn_type[s];
+        syn_weight[k] = d->syn_weight[s];
+        syn_delay[k] = d->syn_delay[s];
+        syn_orig[k] = (int32_t)s;
+    }
+    for (int64_t k = 0; k < n_spk; k++) spk_target[k] = (int32_t)blocked_of[d->spike_target[k]];
     bool any_targets = n_spk > 0;
     ctx->network = any_targets;
 
@@ -399,15 +419,17 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     TRY(upload(ctx, (float **)&T.pulse_amp, pamp));
     TRY(upload(ctx, (int32_t **)&T.rec_col, rec_col));
     TRY(upload(ctx, (int32_t **)&T.ev_col, ev_col));
-    TRY(upload(ctx, (int32_t **)&T.syn_begin, syn_begin));
+    TRY(upload(ctx, (int32_t **)&T.syn_base, syn_base));
+    TRY(upload(ctx, (int32_t **)&T.syn_cnt, syn_cnt));
+    TRY(upload(ctx, (int32_t **)&T.syn_orig, syn_orig));
     TRY(upload(ctx, (int32_t **)&T.syn_type, syn_type));
     TRY(upload(ctx, (float **)&T.syn_weight, syn_weight));
     TRY(upload(ctx, (float **)&T.syn_delay, syn_delay));
     TRY(upload(ctx, &T.syn_next, syn_next));
-    TRY(alloc_zero(ctx, &T.syn_A, (size_t)n_syn));
-    TRY(alloc_zero(ctx, &T.syn_B, (size_t)n_syn));
-    TRY(alloc_zero(ctx, &T.trig[0], (size_t)n_syn));
-    TRY(alloc_zero(ctx, &T.trig[1], (size_t)n_syn));
+    TRY(alloc_zero(ctx, &T.syn_A, (size_t)n_slots));
+    TRY(alloc_zero(ctx, &T.syn_B, (size_t)n_slots));
+    TRY(alloc_zero(ctx, &T.trig[0], (size_t)n_slots));
+    TRY(alloc_zero(ctx, &T.trig[1], (size_t)n_slots));
     TRY(upload(ctx, (int32_t **)&T.spk_begin, spk_begin));
     TRY(upload(ctx, (int32_t **)&T.spk_target, spk_target));
     TRY(upload(ctx, (int32_t **)&T.poi_begin, poi_begin));
@@ -451,11 +473,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     };
     TRY(snap(T.V, NC * sizeof(float)));
     TRY(snap(T.q, NC * NQ * sizeof(float)));
-    TRY(snap(T.syn_next, n_syn * sizeof(float)));
-    TRY(snap(T.syn_A, n_syn * sizeof(float)));
-    TRY(snap(T.syn_B, n_syn * sizeof(float)));
-    TRY(snap(T.trig[0], n_syn * sizeof(int32_t)));
-    TRY(snap(T.trig[1], n_syn * sizeof(int32_t)));
+    TRY(snap(T.syn_next, n_slots * sizeof(float)));
+    TRY(snap(T.syn_A, n_slots * sizeof(float)));
+    TRY(snap(T.syn_B, n_slots * sizeof(float)));
+    TRY(snap(T.trig[0], n_slots * sizeof(int32_t)));
+    TRY(snap(T.trig[1], n_slots * sizeof(int32_t)));
     TRY(snap(T.poi_tsince, n_poi * sizeof(float)));
     TRY(snap(T.poi_isi, n_poi * sizeof(float)));
     TRY(snap(T.poi_A, n_poi * sizeof(float)));

@@ -48,7 +48,8 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
     float q[NQ];
 #pragma unroll
     for (int s = 0; s < NQ; s++) q[s] = T.q[g * NQ + s];
-    const bool has_syn = T.syn_begin[g + 1] > T.syn_begin[g];
+    const int32_t syn_base = T.syn_base[g], syn_cnt = T.syn_cnt[g];
+    const bool has_syn = syn_cnt > 0;
     const bool has_poi = T.poi_begin[g + 1] > T.poi_begin[g];
     const bool has_spk = T.spk_begin[g + 1] > T.spk_begin[g];
 
@@ -62,7 +63,7 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
         float I_ch, G_ch;
         channels_step(gates, k, V, q, init, dt, I_ch, G_ch);
         float I_syn = 0.0f, G_syn = 0.0f;
-        if (has_syn) synapses_step(T, stypes, g, V, init, t, dt, now, s, I_syn, G_syn);
+        if (has_syn) synapses_step(T, stypes, syn_base, syn_cnt, V, init, t, dt, now, s, I_syn, G_syn);
         float I_in = pulses_current(T, k, g, t), G_in = 0.0f;
         if (has_poi) poisson_step(T, stypes, g, V, init, s, dt, I_in, G_in);
         const float I_int = I_ch + I_in + I_syn;
