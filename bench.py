@@ -1,13 +1,19 @@
 """Benchmark of the per-timestep simulation loop on one MI355X.
 
-Workload (default): 512 random-tree cells, compartment counts lognormal (median 100) clipped
-to 20-400, HH soma + passive dendrites, somatic current steps, dt 0.025 ms, 10,001 recorded
-rows (250 ms). A bench step = one full simulation of that workload from the initial state,
-inputs already resident in HBM (the library's reset rewinds state on the device).
+Workload (default): the batched uneven-instance case, 4,096 independent random-tree cells
+with compartment counts log-uniform over 10-2,000 (1.54M compartments), HH soma with
+per-cell density factors U(0.5, 1.5), passive dendrites, somatic current steps, dt 0.025 ms,
+20,001 recorded rows (500 ms). A bench step = one full simulation of that workload from the
+initial state, inputs already resident in HBM (the library's reset rewinds state on the
+device). Other workloads (--workload config1/config2/config3) are the 1,000-cell HH
+population, the 512-tree cable case and the 100,000-cell spiking network.
 
 Prints one JSON line: compartment-updates/s (rows x compartments / s), the roofline of the
-dominant kernel (algorithmic bytes per launch over its HIP-event-timed duration), the
+dominant kernel (its algorithmic bytes per launch over its HIP-event-timed duration), the
 oracle's CPU throughput on a bounded sample (cpu_baseline) and a trace-tolerance check.
+
+--calibrate-copy MiB times a plain device-to-device copy of that size instead (the
+reference kernel for scaling the FETCH_SIZE / WRITE_SIZE profiler counters to bytes).
 """
 import argparse
 import json
@@ -30,14 +36,39 @@ B_HH = 56        # 24 + 3 gates r+w 24 + gNa*A, gK*A 8
 B_RECORD = 4     # recorded V per column-row
 
 
-def algorithmic_bytes(model, rows):
-    """Bytes per full run: per compartment-row 24 B (passive) or 56 B (HH), plus 4 B per
-    recorded column-row."""
-    ch_b = model.comp_chan_begin
-    n_chan = np.diff(ch_b)
-    hh = int((n_chan > 1).sum())
-    passive = model.n_comps - hh
-    return rows * (B_PASSIVE * passive + B_HH * hh + B_RECORD * len(model.record_comp))
+def algorithmic_bytes(model, rows, cells=None):
+    """Bytes per full run over the given cells (all by default): per compartment-row 24 B
+    (passive) or 56 B (HH), plus 4 B per recorded column-row."""
+    n_chan = np.diff(model.comp_chan_begin)
+    is_hh = n_chan > 1
+    if cells is None:
+        sel = np.ones(model.n_comps, bool)
+    else:
+        cb = model.cell_comp_begin
+        comp_cell = np.repeat(np.arange(model.n_cells), np.diff(cb))
+        sel = np.isin(comp_cell, np.asarray(cells))
+    hh = int((is_hh & sel).sum())
+    passive = int(sel.sum()) - hh
+    rec = int(sel[np.asarray(model.record_comp, np.int64)].sum())
+    return rows * (B_PASSIVE * passive + B_HH * hh + B_RECORD * rec)
+
+
+def calibrate_copy(mib, reps=10):
+    """Plain device-to-device copy of `mib` MiB, `reps` times: the profiler's FETCH_SIZE and
+    WRITE_SIZE per dispatch of this kernel should each read `mib` * 1024 KiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    assert torch.equal(dst, src)
+    print(json.dumps({"copy_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4, "reps": reps,
+                      "ms_per_copy": el * 1e3, "gbs_read_plus_write": 2 * n * 4 / el / 1e9}), flush=True)
 
 
 def build_workload(name):
@@ -113,7 +144,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--workload", default="config4")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB")
     ap.add_argument("--threads-per-cell", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
@@ -123,13 +155,17 @@ def main():
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+
     from eden_amd.model import Simulation
 
     print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
     model, desc = build_workload(args.workload)
     rows = model.total_rows()
     sim = Simulation(model, threads_per_cell=args.threads_per_cell, profile=True)
-    tpb, k = sim.launch_shape()
+    tpb, slots = sim.launch_shape()
 
     def one_step():
         sim.reset()
@@ -144,33 +180,43 @@ def main():
     for w in range(args.warmup):
         one_step()
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
-    times, kms, launches = [], [], []
+    times = []
+    kern = {}
     for s in range(args.steps):
         t0 = time.perf_counter()
         one_step()
         el = time.perf_counter() - t0
-        ms, nl, kname = sim.kernel_stats()
         times.append(el)
-        kms.append(ms)
-        launches.append(nl)
-        print(f"[bench] step {s}: {el * 1e3:.2f} ms wall, {ms:.2f} ms in {nl} launch(es)", file=sys.stderr, flush=True)
+        infos = sim.kernel_infos()
+        for ki in infos:
+            acc = kern.setdefault(ki["name"], dict(ki, total_ms=0.0, launches=0))
+            acc["total_ms"] += ki["total_ms"]
+            acc["launches"] += ki["launches"]
+        msg = ", ".join(f"{ki['name']} {ki['total_ms']:.2f} ms/{ki['launches']}" for ki in infos)
+        print(f"[bench] step {s}: {el * 1e3:.2f} ms wall; {msg}", file=sys.stderr, flush=True)
     sim.close()
     t_step = float(np.mean(times))
     updates = model.n_comps * rows
     value = updates / t_step
-    per_launch_ms = float(np.sum(kms) / max(1, np.sum(launches)))
-    launches_per_step = float(np.mean(launches))
-    bytes_per_launch = algorithmic_bytes(model, rows) / launches_per_step
+    # dominant kernel: the one with the largest summed duration
+    dom = max(kern.values(), key=lambda k: k["total_ms"])
+    per_launch_ms = dom["total_ms"] / max(1, dom["launches"])
+    launches_per_step = dom["launches"] / max(1, args.steps)
+    bytes_per_launch = algorithmic_bytes(model, rows, dom["cell_index"]) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms * 1e-3)
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None, kernel=kname,
-                    bytes_per_launch=bytes_per_launch, launch_ms=per_launch_ms)
+                    frac=achieved / HBM_PEAK, traffic=None, kernel=dom["name"],
+                    kernel_cells=int(dom["cells"]), kernel_comps=int(dom["comps"]),
+                    bytes_per_launch=bytes_per_launch, launch_ms=per_launch_ms,
+                    all_kernels={k["name"]: dict(ms_per_launch=k["total_ms"] / max(1, k["launches"]),
+                                                 cells=int(k["cells"]), comps=int(k["comps"]))
+                                 for k in kern.values()})
     out = {
         "metric": METRIC, "value": value, "unit": "compartment-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "cells": model.n_cells, "compartments": model.n_comps, "rows": rows,
-                   "launch_shape": {"threads_per_cell": tpb, "compartments_per_thread": k}},
+                   "launch_shape": {"threads_per_cell": tpb, "compartments_per_thread": slots}},
         "roofline": roofline,
     }
     if not args.no_parity:

@@ -64,6 +64,12 @@ class Options(C.Structure):
                 ("log_synapse_events", C.c_int32)]
 
 
+class KernelInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("threads_per_block", C.c_int32), ("grid", C.c_int32),
+                ("n_cells", C.c_int32), ("reserved", C.c_int32), ("n_comps", C.c_int64),
+                ("launches", C.c_int64), ("total_ms", C.c_double)]
+
+
 EXPORTS = {
     # name: (restype, argtypes)
     "edenhip_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(Options), C.POINTER(C.c_void_p)]),
@@ -76,6 +82,9 @@ EXPORTS = {
     "edenhip_get_voltages": (C.c_int, [C.c_void_p, _P]),
     "edenhip_read_synapse_events": (C.c_int, [C.c_void_p, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "edenhip_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_char_p, C.c_int32]),
+    "edenhip_kernel_count": (C.c_int32, [C.c_void_p]),
+    "edenhip_get_kernel_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(KernelInfo)]),
+    "edenhip_get_kernel_cells": (C.c_int, [C.c_void_p, C.c_int32, _P, C.c_int32]),
     "edenhip_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),
     "edenhip_destroy": (None, [C.c_void_p]),

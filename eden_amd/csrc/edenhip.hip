@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -33,6 +34,7 @@ struct Bucket {
     int32_t ninmax = 0;
     int32_t nctmax = 0;
     int32_t words_max = 0;
+    int64_t comps = 0;
     size_t lds = 0;
     int grid = 0;
     int32_t *d_order = nullptr;
@@ -40,6 +42,10 @@ struct Bucket {
     TreeCell *d_cells = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    // profiling (options.profile): events around the bucket's launches of a chunk
+    hipEvent_t p0 = nullptr, p1 = nullptr;
+    double ms = 0;
+    int64_t launches = 0;
 };
 
 }  // namespace
@@ -81,6 +87,11 @@ struct edenhip_ctx {
     // profiling
     double kernel_ms = 0;
     int64_t launches = 0;
+    // per-kernel profile of the point kernel, and descriptor cell index of every point / tree cell
+    hipEvent_t pp0 = nullptr, pp1 = nullptr;
+    double point_ms = 0;
+    int64_t point_launches = 0;
+    std::vector<int32_t> point_desc, tree_desc;
     std::string kernel_name;
     std::vector<void *> allocs;
 };
@@ -304,6 +315,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             if (d->parent[c0 + i] < 0 || d->parent[c0 + i] >= i) return fail(ctx, EDENHIP_E_INVALID, "parents must precede children");
         if (n == 1) {
             point.push_back((int32_t)c0);
+            ctx->point_desc.push_back(c);
             continue;
         }
         if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2047 compartments");
@@ -351,7 +363,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         tc.rounds_fwd = P.rounds_fwd;
         tc.rounds_bwd = P.rounds_bwd;
         tc.words_off = (int32_t)words.size();
-        words.insert(words.end(), P.words.begin(), P.words.end());
+        tc.words_len = (int32_t)P.block.size();
+        words.insert(words.end(), P.block.begin(), P.block.end());
         // edge constants by position, fp32 as computed per step by the reference:
         // Ui = -dt/(R*C[i]), Uj = -dt/(R*C[j]) with R = R_Axial[max(i, j)] = R_Axial[i]
         for (int32_t i = 0; i < n; i++) posv[c0 + i] = P.pos[i];
@@ -364,6 +377,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         }
         cells.push_back(tc);
         cell_tpb.push_back(tpb);
+        ctx->tree_desc.push_back(c);
     }
     ctx->n_point = (int32_t)point.size();
     // block-interleaved synapse slots for the point cells (64 per wave, in point order)
@@ -507,11 +521,12 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             b->tpb = cell_tpb[i];
         }
         b->order.push_back((int32_t)i);
+        b->comps += cells[i].n;
         b->nmax = std::max(b->nmax, cells[i].n);
         b->ngmax = std::max(b->ngmax, cells[i].n_gate);
         b->ninmax = std::max(b->ninmax, cells[i].n_input);
         b->nctmax = std::max(b->nctmax, cells[i].n_ctask);
-        b->words_max = std::max(b->words_max, (cells[i].rounds_fwd + cells[i].rounds_bwd + 2) * b->tpb);
+        b->words_max = std::max(b->words_max, cells[i].words_len);
     }
     std::vector<TreeCell> dcells;
     for (auto &b : ctx->buckets) {
@@ -526,11 +541,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // input list, and the task words for workgroups of up to 128 threads
         b.nctmax = std::max(b.nctmax, 1);   // ungated slots read sF[first task], kept in bounds
         b.split = b.tpb == 64 && ctx->opt.persistent >= 0;
-        b.lds = (size_t)(b.nmax + 1) * 2 * sizeof(float2) +
-                (size_t)b.nmax * sizeof(float2) + (size_t)(b.nmax + 1) * 2 * sizeof(float) +
-                (size_t)b.ngmax * sizeof(float) + (size_t)b.nctmax * 3 * sizeof(uint32_t) +
-                (size_t)b.ninmax * sizeof(uint32_t) +
-                (b.tpb <= 128 ? (size_t)b.words_max * sizeof(uint32_t) : 0);
+        b.lds = (size_t)tree_lds(b.nmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.split).total * sizeof(float);
         const void *kp = tree_kernel_ptr(b.tpb, b.split);
         if (!kp) return fail(ctx, EDENHIP_E_LIMIT, "no tree kernel for the chosen workgroup size");
         int per_cu = 0;
@@ -542,6 +553,14 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(alloc_zero(ctx, &b.d_queue, 2));
         HIPCHECK(ctx, hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
         HIPCHECK(ctx, hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+        if (ctx->opt.profile) {
+            HIPCHECK(ctx, hipEventCreate(&b.p0));
+            HIPCHECK(ctx, hipEventCreate(&b.p1));
+        }
+    }
+    if (ctx->opt.profile) {
+        HIPCHECK(ctx, hipEventCreate(&ctx->pp0));
+        HIPCHECK(ctx, hipEventCreate(&ctx->pp1));
     }
     HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_start, hipEventDisableTiming));
     HIPCHECK(ctx, hipDeviceSynchronize());
@@ -559,8 +578,10 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
     const bool per_step = ctx->network || ctx->opt.persistent < 0;
     const int64_t stride = per_step ? 1 : (s1 - s0);
     int64_t nl = 0;
+    const bool prof = ctx->opt.profile != 0;
     for (int64_t a = s0; a < s1; a += stride) {
         const int64_t b = std::min(s1, a + stride);
+        const bool first = a == s0, last = b == s1;
         if (ctx->n_point > 0) {
             const int tpb = 64;
             const int grid = (ctx->n_point + tpb - 1) / tpb;
@@ -568,7 +589,9 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             int32_t n = ctx->n_point;
             void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &n,
                             (void *)&a, (void *)&b, &row0, &dt};
+            if (prof && first) HIPCHECK(ctx, hipEventRecord(ctx->pp0, ctx->stream));
             HIPCHECK(ctx, hipLaunchKernel(point_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
+            if (prof && last) HIPCHECK(ctx, hipEventRecord(ctx->pp1, ctx->stream));
             nl++;
         }
         // tree buckets run concurrently, each on its own stream, joined back to the main stream
@@ -579,22 +602,34 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
                             &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &ctx->d_tcm,
                             &ctx->d_ctask, &ctx->d_gq, &ctx->d_inl, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
-                            (void *)&a, (void *)&b, &row0, &dt};
+                            &bk.words_max, (void *)&a, (void *)&b, &row0, &dt};
             HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
+            if (prof && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));
             HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split), dim3(bk.grid), dim3(bk.tpb + (bk.split ? 64 : 0)),
                                           args, bk.lds, bk.stream));
+            if (prof && last) HIPCHECK(ctx, hipEventRecord(bk.p1, bk.stream));
             HIPCHECK(ctx, hipEventRecord(bk.done, bk.stream));
             HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, bk.done, 0));
             nl++;
+            if (prof) bk.launches++;
         }
     }
-    if (ctx->opt.profile) {
+    if (prof) {
         HIPCHECK(ctx, hipEventRecord(e1, ctx->stream));
         HIPCHECK(ctx, hipEventSynchronize(e1));
         float ms = 0;
         HIPCHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
         ctx->kernel_ms += ms;
         ctx->launches += nl;
+        if (ctx->n_point > 0) {
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->pp0, ctx->pp1));
+            ctx->point_ms += ms;
+            ctx->point_launches += per_step ? (s1 - s0) : 1;
+        }
+        for (auto &bk : ctx->buckets) {
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, bk.p0, bk.p1));
+            bk.ms += ms;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
@@ -659,6 +694,12 @@ int edenhip_reset(edenhip_ctx *ctx) {
     ctx->last_row0 = 0;
     ctx->kernel_ms = 0;
     ctx->launches = 0;
+    ctx->point_ms = 0;
+    ctx->point_launches = 0;
+    for (auto &bk : ctx->buckets) {
+        bk.ms = 0;
+        bk.launches = 0;
+    }
     return 0;
 }
 
@@ -744,6 +785,48 @@ int edenhip_kernel_stats(const edenhip_ctx *ctx, double *total_ms, int64_t *laun
     return 0;
 }
 
+int32_t edenhip_kernel_count(const edenhip_ctx *ctx) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    return (ctx->n_point > 0 ? 1 : 0) + (int32_t)ctx->buckets.size();
+}
+
+int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_info *out) {
+    if (!ctx || !out || i < 0 || i >= edenhip_kernel_count(ctx)) return EDENHIP_E_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    if (ctx->n_point > 0 && i == 0) {
+        std::snprintf(out->name, sizeof(out->name), "point_kernel");
+        out->threads_per_block = 64;
+        out->grid = (ctx->n_point + 63) / 64;
+        out->n_cells = ctx->n_point;
+        out->n_comps = ctx->n_point;
+        out->launches = ctx->point_launches;
+        out->total_ms = ctx->point_ms;
+        return 0;
+    }
+    const Bucket &b = ctx->buckets[(size_t)(i - (ctx->n_point > 0 ? 1 : 0))];
+    std::snprintf(out->name, sizeof(out->name), "tree_kernel<%d, %s>", b.tpb, b.split ? "true" : "false");
+    out->threads_per_block = b.tpb + (b.split ? 64 : 0);
+    out->grid = b.grid;
+    out->n_cells = (int32_t)b.order.size();
+    out->n_comps = b.comps;
+    out->launches = b.launches;
+    out->total_ms = b.ms;
+    return 0;
+}
+
+int edenhip_get_kernel_cells(const edenhip_ctx *ctx, int32_t i, int32_t *cells, int32_t max_cells) {
+    if (!ctx || !cells || i < 0 || i >= edenhip_kernel_count(ctx)) return EDENHIP_E_INVALID;
+    if (ctx->n_point > 0 && i == 0) {
+        if (max_cells < ctx->n_point) return EDENHIP_E_INVALID;
+        std::copy(ctx->point_desc.begin(), ctx->point_desc.end(), cells);
+        return 0;
+    }
+    const Bucket &b = ctx->buckets[(size_t)(i - (ctx->n_point > 0 ? 1 : 0))];
+    if (max_cells < (int32_t)b.order.size()) return EDENHIP_E_INVALID;
+    for (size_t k = 0; k < b.order.size(); k++) cells[k] = ctx->tree_desc[(size_t)b.order[k]];
+    return 0;
+}
+
 int edenhip_launch_shape(const edenhip_ctx *ctx, int32_t *threads_per_cell, int32_t *slots_per_thread) {
     if (!ctx) return EDENHIP_E_INVALID;
     if (ctx->buckets.empty()) {
@@ -770,8 +853,12 @@ void edenhip_destroy(edenhip_ctx *ctx) {
         for (auto &b : ctx->buckets) {
             if (b.stream) (void)hipStreamDestroy(b.stream);
             if (b.done) (void)hipEventDestroy(b.done);
+            if (b.p0) (void)hipEventDestroy(b.p0);
+            if (b.p1) (void)hipEventDestroy(b.p1);
         }
         if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        if (ctx->pp0) (void)hipEventDestroy(ctx->pp0);
+        if (ctx->pp1) (void)hipEventDestroy(ctx->pp1);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -193,27 +193,25 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             int32_t *__restrict__ queue, const uint32_t *__restrict__ words_g, const int32_t *__restrict__ pos_g,
             const float2 *__restrict__ u_g, const uint32_t *__restrict__ tcm_g, const uint32_t *__restrict__ ctask_g,
             float *__restrict__ gq_g, const uint32_t *__restrict__ inl_g, int32_t nmax, int32_t ngmax, int32_t nctmax,
-            int32_t ninmax, int64_t s0, int64_t s1, int64_t row0, float dt) {
+            int32_t ninmax, int32_t words_max, int64_t s0, int64_t s1, int64_t row0, float dt) {
     constexpr int K = TREE_SLOTS;
     constexpr int NT = TPB + (SPLIT ? 64 : 0);
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ DGate gates[MAX_CHAN_TYPES * GATES_PER_TYPE];
     __shared__ DSynType stypes[64];
     __shared__ int32_t s_task;
-    // dynamic LDS: sDX, sU [nmax+1] float2 | sIn [nmax] float2 | sV [2][nmax+1] | sQ [ngmax] |
-    //              sF [2][nctmax] | sCT [nctmax] | sIL [ninmax] | task words
-    float2 *sDX = reinterpret_cast<float2 *>(smem);
-    float2 *sU = sDX + (nmax + 1);
-    float2 *sIn = sU + (nmax + 1);
-    float *sVb = reinterpret_cast<float *>(sIn + nmax);
-    float *sQ = sVb + 2 * (nmax + 1);
-    float *sFb = sQ + ngmax;
-    uint32_t *sCT = reinterpret_cast<uint32_t *>(sFb + 2 * nctmax);
-    uint32_t *sIL = sCT + nctmax;
-    // task words: in LDS for one- and two-wave cable groups; larger groups read them from
-    // global memory (cached), two rounds ahead
-    constexpr bool WORDS_LDS = TPB <= 128;
-    uint32_t *sW = sIL + ninmax;
+    // dynamic LDS, laid out by tree_lds()
+    const TreeLds L = tree_lds(nmax, ngmax, nctmax, ninmax, words_max, SPLIT);
+    float *sm = reinterpret_cast<float *>(smem);
+    float2 *sDX = reinterpret_cast<float2 *>(sm + L.dx);
+    float2 *sU = reinterpret_cast<float2 *>(sm + L.u);
+    float2 *sIn = reinterpret_cast<float2 *>(sm + L.in);
+    float *sVb = sm + L.v;
+    float *sQ = sm + L.q;
+    float *sFb = sm + L.f;
+    uint32_t *sCT = reinterpret_cast<uint32_t *>(sm + L.ct);
+    uint32_t *sIL = reinterpret_cast<uint32_t *>(sm + L.il);
+    uint32_t *sW = reinterpret_cast<uint32_t *>(sm + L.w);   // schedule block: round headers, task words
     const int tid = threadIdx.x;
     const bool cable = !SPLIT || tid < TPB;
     for (int i = tid; i < n_gates; i += NT) gates[i] = gates_g[i];
@@ -233,11 +231,20 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         const int64_t c0 = cell.comp_begin;
         const int64_t gq0 = cell.gate_off;
         const int RF = cell.rounds_fwd, RB = cell.rounds_bwd;
-        const int nw = (RF + RB + 2) * TPB;
-        const uint32_t *Wg = words_g + cell.words_off;
-        if constexpr (WORDS_LDS)
-            for (int i = tid; i < nw; i += NT) sW[i] = Wg[i];
-        const uint32_t *W = WORDS_LDS ? sW : Wg;
+        {
+            const uint32_t *Wg = words_g + cell.words_off;
+            for (int i = tid; i < cell.words_len; i += NT) sW[i] = Wg[i];
+        }
+        // task word of this thread in round r: its word when the round has one for it, else
+        // the idle word (dummy slot N)
+        const uint32_t idle_fwd = (uint32_t)N | ((uint32_t)N << 11) | (1u << 22);
+        const uint32_t idle_bwd = (uint32_t)N | ((uint32_t)N << 11);
+        auto task_word = [&](int r, uint32_t idle) -> uint32_t {
+            const uint32_t h = sW[r];
+            uint32_t w = idle;
+            if ((uint32_t)tid < (h >> 16)) w = sW[(h & 0xFFFFu) + tid];
+            return w;
+        };
         for (int i = tid; i < N; i += NT) sU[i] = u_g[c0 + i];
         for (int i = tid; i < NG; i += NT) sQ[i] = gq_g[gq0 + i];
         for (int i = tid; i < NCT; i += NT) sCT[i] = ctask_g[cell.ctask_off + i];
@@ -329,12 +336,12 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     // slot as its second child, whose contribution r1 * 0 is exactly zero.
                     // Nodes with more than two children fold the rest in a loop. The next
                     // round's word is always read (the schedule ends with an idle round).
-                    uint32_t w = W[tid];
-                    uint32_t wn = W[TPB + tid];
+                    uint32_t w = task_word(0, idle_fwd);
+                    uint32_t wn = task_word(1, idle_fwd);
                     for (int r = 0; r < RF; r++) {
-                        // rounds r + 1 and r + 2 exist: the sweep ends with an idle round and
-                        // the backward words follow
-                        const uint32_t wnn = W[(r + 2) * TPB + tid];
+                        // rounds r + 1 and r + 2 exist: the sweep ends with an empty round and
+                        // the backward rounds follow
+                        const uint32_t wnn = task_word(r + 2, idle_fwd);
                         const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
                         const int c1 = nc >= 2 ? fc + 1 : N;
                         const float2 dx = sDX[p];
@@ -366,11 +373,11 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     }
                     lds_barrier<TPB>();
                     // back-substitution, rounds by depth, straight-line as above
-                    const uint32_t *Wb = W + (RF + 1) * TPB;
-                    w = Wb[tid];
-                    wn = Wb[TPB + tid];
+                    const int rb0 = RF + 1;
+                    w = task_word(rb0, idle_bwd);
+                    wn = task_word(rb0 + (RB > 0 ? 1 : 0), idle_bwd);
                     for (int r = 0; r < RB; r++) {
-                        const uint32_t wnn = (r + 2 <= RB) ? Wb[(r + 2) * TPB + tid] : 0u;
+                        const uint32_t wnn = (r + 2 <= RB) ? task_word(rb0 + r + 2, idle_bwd) : idle_bwd;
                         const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
                         float2 dx = sDX[p];
                         const float ui = sU[p].x;
@@ -426,7 +433,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     template __global__ void tree_kernel<TPB, SPLIT>(                                                              \
         DevTables, const DGate *, int, const DSynType *, int, const TreeCell *, const int32_t *, int32_t, int32_t *,  \
         const uint32_t *, const int32_t *, const float2 *, const uint32_t *, const uint32_t *, float *,             \
-        const uint32_t *, int32_t, int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
+        const uint32_t *, int32_t, int32_t, int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
 EDENHIP_TREE_INST(64, true)
 EDENHIP_TREE_INST(64, false)
 EDENHIP_TREE_INST(128, false)

@@ -1,5 +1,6 @@
 // kernels.hpp -- launch-side declarations shared by the host code and kernels.hip.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace edenhip {
@@ -10,7 +11,8 @@ struct TreeCell {
     int32_t n;            // compartments
     int32_t rounds_fwd;   // forward-elimination rounds
     int32_t rounds_bwd;   // back-substitution rounds
-    int32_t words_off;    // offset of its task words
+    int32_t words_off;    // offset of its schedule block (round headers, then task words)
+    int32_t words_len;    // length of that block in 32-bit words
     int32_t n_gate;       // gate variables (one per gate of every channel distribution)
     int64_t gate_off;     // offset of its gate variables
     int32_t n_input;      // compartments with Poisson inputs
@@ -26,6 +28,31 @@ constexpr uint32_t TCM_INPUT = 1u << 7;
 constexpr int TREE_SLOTS = 4;        // compartments per thread
 
 
+
+// Dynamic LDS of the tree kernel, in 4-byte units: sDX, sU [nmax+1] float2 | sIn [nmax]
+// float2 (only with Poisson inputs) | sV [nbuf][nmax+1] | sQ [ngmax] | sF [nbuf][nctmax] |
+// sCT [nctmax] | sIL [ninmax] | schedule block [words_max]; nbuf = 2 when a separate wave
+// runs the channel phase, else 1.
+struct TreeLds {
+    int32_t dx, u, in, v, q, f, ct, il, w, total;
+};
+
+__host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t nctmax, int32_t ninmax,
+                                            int32_t words_max, bool split) {
+    TreeLds L;
+    const int32_t nbuf = split ? 2 : 1;
+    L.dx = 0;
+    L.u = L.dx + 2 * (nmax + 1);
+    L.in = L.u + 2 * (nmax + 1);
+    L.v = L.in + (ninmax > 0 ? 2 * nmax : 0);
+    L.q = L.v + nbuf * (nmax + 1);
+    L.f = L.q + ngmax;
+    L.ct = L.f + nbuf * nctmax;
+    L.il = L.ct + nctmax;
+    L.w = L.il + ninmax;
+    L.total = L.w + words_max;
+    return L;
+}
 
 const void *tree_kernel_ptr(int tpb, bool split);
 const void *point_kernel_ptr();

@@ -39,19 +39,24 @@ constexpr int32_t MAX_TREE_N = 2047;   // 11-bit positions in task words, plus t
 // diagonal and right-hand side of every node see the same operations in the same order.
 // Back-substitution runs over non-root nodes grouped by depth.
 //
-// Each group is split into rounds of `tpb` tasks, one 32-bit word per thread:
+// Each group is split into rounds of at most `tpb` tasks, one 32-bit word per task:
 //   forward:  pos | first_child_pos << 11 | n_children << 22
 //   backward: pos | parent_pos << 11
-// Threads without work in a round get an idle word that points every access at the dummy
-// slot n (forward: node n with one child n; backward: node n with parent n), so the rounds
-// run as straight-line code. Each sweep is followed by one extra idle round, read as the
-// prefetch of its last round.
+// The schedule of a cell is one block of 32-bit words: the round headers (offset of the
+// round's first word in the block | task count << 16), forward rounds followed by one empty
+// round, backward rounds followed by one empty round, then the task words. Threads at or
+// beyond a round's task count take an idle word that points every access at the dummy slot
+// n (forward: node n with one child n; backward: node n with parent n), so the rounds run
+// as straight-line code; the empty rounds are read as the prefetch past a sweep's end.
 struct TreePlan {
     std::vector<int32_t> pos;          // [n] BFS position of each compartment
     std::vector<int32_t> parent_pos;   // [n] by position
     int32_t rounds_fwd = 0, rounds_bwd = 0;
-    std::vector<uint32_t> words;       // (rounds_fwd + 1 + rounds_bwd + 1) * tpb
+    std::vector<uint32_t> block;       // headers (rounds_fwd + rounds_bwd + 2), then task words
 };
+
+inline uint32_t idle_fwd_word(int32_t n) { return (uint32_t)n | ((uint32_t)n << 11) | (1u << 22); }
+inline uint32_t idle_bwd_word(int32_t n) { return (uint32_t)n | ((uint32_t)n << 11); }
 
 inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
     TreePlan P;
@@ -72,16 +77,15 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
     const int32_t H = height[0];
     int32_t Dm = 0;
     for (int32_t i = 0; i < n; i++) Dm = std::max(Dm, depth[i]);
-    const uint32_t idle_fwd = (uint32_t)n | ((uint32_t)n << 11) | (1u << 22);
-    const uint32_t idle_bwd = (uint32_t)n | ((uint32_t)n << 11);
-    auto emit = [&](const std::vector<uint32_t> &tasks, int32_t &rounds, uint32_t idle) {
-        const int32_t r = (int32_t)((tasks.size() + tpb - 1) / tpb);
-        for (int32_t k = 0; k < r; k++)
-            for (int32_t t = 0; t < tpb; t++) {
-                const size_t idx = (size_t)k * tpb + t;
-                P.words.push_back(idx < tasks.size() ? tasks[idx] : idle);
-            }
-        rounds += r;
+    std::vector<uint32_t> words;
+    std::vector<std::pair<uint32_t, uint32_t>> rounds;   // (first word, count)
+    auto emit = [&](const std::vector<uint32_t> &tasks, int32_t &nr) {
+        for (size_t k = 0; k < tasks.size(); k += (size_t)tpb) {
+            const size_t cnt = std::min(tasks.size() - k, (size_t)tpb);
+            rounds.emplace_back((uint32_t)words.size(), (uint32_t)cnt);
+            words.insert(words.end(), tasks.begin() + (long)k, tasks.begin() + (long)(k + cnt));
+            nr++;
+        }
     };
     for (int32_t h = 1; h <= H; h++) {
         std::vector<uint32_t> tasks;
@@ -91,24 +95,21 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
                 const uint32_t fc = (uint32_t)P.pos[ch[i][0]];
                 tasks.push_back(p | (fc << 11) | ((uint32_t)ch[i].size() << 22));
             }
-        emit(tasks, P.rounds_fwd, idle_fwd);
+        emit(tasks, P.rounds_fwd);
     }
-    for (int32_t t = 0; t < tpb; t++) P.words.push_back(idle_fwd);
-    std::vector<uint32_t> fwd_words;
-    fwd_words.swap(P.words);
+    rounds.emplace_back((uint32_t)words.size(), 0u);
     for (int32_t d = 1; d <= Dm; d++) {
         std::vector<uint32_t> tasks;
         for (int32_t p = 0; p < n; p++) {
             const int32_t i = order[p];
             if (depth[i] == d) tasks.push_back((uint32_t)p | ((uint32_t)P.parent_pos[p] << 11));
         }
-        emit(tasks, P.rounds_bwd, idle_bwd);
+        emit(tasks, P.rounds_bwd);
     }
-    for (int32_t t = 0; t < tpb; t++) P.words.push_back(idle_bwd);
-    std::vector<uint32_t> bwd_words;
-    bwd_words.swap(P.words);
-    P.words = fwd_words;
-    P.words.insert(P.words.end(), bwd_words.begin(), bwd_words.end());
+    rounds.emplace_back((uint32_t)words.size(), 0u);
+    const uint32_t nh = (uint32_t)rounds.size();
+    for (auto &r : rounds) P.block.push_back((nh + r.first) | (r.second << 16));
+    P.block.insert(P.block.end(), words.begin(), words.end());
     return P;
 }
 

@@ -282,6 +282,20 @@ class Simulation:
         self._check(self.lib.edenhip_kernel_stats(self.h, C.byref(ms), C.byref(nl), name, 64), "kernel_stats")
         return ms.value, nl.value, name.value.decode()
 
+    def kernel_infos(self):
+        """Per-kernel launch statistics: list of dicts (name, threads_per_block, grid, cells,
+        comps, launches, total_ms, cell_index array)."""
+        out = []
+        for i in range(self.lib.edenhip_kernel_count(self.h)):
+            ki = N.KernelInfo()
+            self._check(self.lib.edenhip_get_kernel_info(self.h, i, C.byref(ki)), "get_kernel_info")
+            cells = np.empty(max(1, ki.n_cells), np.int32)
+            self._check(self.lib.edenhip_get_kernel_cells(self.h, i, N.ptr(cells), len(cells)), "get_kernel_cells")
+            out.append(dict(name=ki.name.decode(), threads_per_block=ki.threads_per_block, grid=ki.grid,
+                            cells=ki.n_cells, comps=ki.n_comps, launches=ki.launches, total_ms=ki.total_ms,
+                            cell_index=cells[: ki.n_cells]))
+        return out
+
     def launch_shape(self):
         t, k = C.c_int32(), C.c_int32()
         self._check(self.lib.edenhip_launch_shape(self.h, C.byref(t), C.byref(k)), "launch_shape")

@@ -212,6 +212,27 @@ int edenhip_get_voltages(edenhip_ctx *ctx, float *dst);
  * the launch count. */
 int edenhip_kernel_stats(const edenhip_ctx *ctx, double *total_ms, int64_t *launches, char *name, int32_t name_len);
 
+/* Per-kernel launch statistics (options.profile = 1). Kernel 0 is the point-cell kernel when
+ * the model has one-compartment cells; then one entry per tree-cell bucket (cells grouped
+ * by workgroup size, each bucket one kernel on a stream of its own). total_ms sums, over
+ * the chunks run since create or reset, the time from the event recorded before the
+ * kernel's first launch in a chunk to the event after its last, on the kernel's stream. */
+typedef struct edenhip_kernel_info {
+    char name[64];               /* kernel name as a profiler lists it, without arguments */
+    int32_t threads_per_block;
+    int32_t grid;                /* workgroups per launch */
+    int32_t n_cells;             /* cells this kernel advances */
+    int32_t reserved;
+    int64_t n_comps;             /* compartments of those cells */
+    int64_t launches;
+    double total_ms;
+} edenhip_kernel_info;
+
+int32_t edenhip_kernel_count(const edenhip_ctx *ctx);
+int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_info *out);
+/* Descriptor cell indices advanced by kernel i (n_cells of them), in work-queue order. */
+int edenhip_get_kernel_cells(const edenhip_ctx *ctx, int32_t i, int32_t *cells, int32_t max_cells);
+
 /* Describe the launch shape chosen for the tree-cell kernel (autotuned or given). */
 int edenhip_launch_shape(const edenhip_ctx *ctx, int32_t *threads_per_cell, int32_t *slots_per_thread);
 

@@ -1,0 +1,166 @@
+// Host check of the tree kernel's Hines schedule (eden_amd/csrc/tree_schedule.hpp).
+//
+// For random trees of several shapes and every workgroup size, the schedule block is decoded
+// exactly as tree_kernel decodes it (round headers, task words, idle words aimed at the dummy
+// slot n, the two-child straight-line fold plus the loop for further children) and the
+// elimination is replayed round by round in fp32. The result must equal, bit for bit, the
+// reference's serial elimination over the depth-first post-order with adjacency lists
+// [parent, children ascending] (eden/Eden.cpp:4416-4486, 6114-6225). Structural checks:
+// every node with children is folded exactly once, every non-root node is back-substituted
+// exactly once, no round has more tasks than threads.
+//
+// Build: g++ -std=c++17 -O2 -ffp-contract=off -I<repo> sched_check.cpp; prints "ok <trials>".
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <vector>
+
+#include "eden_amd/csrc/tree_schedule.hpp"
+
+using namespace edenhip;
+
+static std::vector<int32_t> make_parents(std::mt19937 &rng, int n, int shape) {
+    std::vector<int32_t> par(n, -1);
+    std::vector<int32_t> open{0}, nch(n, 0);
+    for (int i = 1; i < n; i++) {
+        switch (shape) {
+        case 0: par[i] = (int32_t)(rng() % i); break;              // random recursive (any fan-out)
+        case 1: {                                                    // binary, uniform open node
+            const size_t k = rng() % open.size();
+            par[i] = open[k];
+            if (++nch[open[k]] == 2) {
+                open[k] = open.back();
+                open.pop_back();
+            }
+            open.push_back(i);
+            break;
+        }
+        case 2: par[i] = i - 1; break;                               // unbranched cable
+        case 3: par[i] = (int32_t)(rng() % std::min(i, 3)); break;  // star-like, wide fan-out
+        default: par[i] = i > 8 ? (int32_t)(i - 1 - rng() % 8) : i - 1; break;   // long bushy chains
+        }
+    }
+    return par;
+}
+
+int main(int argc, char **argv) {
+    const int trials = argc > 1 ? std::atoi(argv[1]) : 200;
+    std::mt19937 rng(12345);
+    std::uniform_real_distribution<float> U(0.5f, 2.0f);
+    int bad = 0, runs = 0;
+    for (int trial = 0; trial < trials && bad == 0; trial++) {
+        const int n = 2 + (int)(rng() % (trial % 4 == 0 ? 2046 : 300));
+        const int shape = trial % 5;
+        const std::vector<int32_t> par = make_parents(rng, n, shape);
+        std::vector<float> D0(n), X0(n), ui(n), uj(n);
+        for (int i = 0; i < n; i++) {
+            D0[i] = 1.0f + 3.0f * U(rng);
+            X0[i] = U(rng) - 1.25f;
+            ui[i] = -U(rng) * 0.3f;
+            uj[i] = -U(rng) * 0.3f;
+        }
+        // reference: serial elimination over the DFS post-order, then back-substitution
+        std::vector<std::vector<int>> ch(n);
+        for (int i = 1; i < n; i++) ch[par[i]].push_back(i);
+        std::vector<int> order;
+        std::vector<std::pair<int, size_t>> st{{0, 0}};
+        while (!st.empty()) {
+            auto &[v, k] = st.back();
+            if (k < ch[v].size()) {
+                const int c = ch[v][k++];
+                st.push_back({c, 0});
+            } else {
+                order.push_back(v);
+                st.pop_back();
+            }
+        }
+        std::vector<float> Dr = D0, Xr = X0;
+        for (int k = 0; k < n - 1; k++) {
+            const int i = order[k], j = par[i];
+            const float r = uj[i] / Dr[i];
+            Dr[j] -= r * ui[i];
+            Xr[j] -= r * Xr[i];
+        }
+        Xr[0] = Xr[0] / Dr[0];
+        for (int k = n - 2; k >= 0; k--) {
+            const int i = order[k], j = par[i];
+            Xr[i] = (Xr[i] - ui[i] * Xr[j]) / Dr[i];
+        }
+        for (int tpb : {64, 128, 256, 512}) {
+            runs++;
+            const TreePlan P = plan_tree(n, par.data(), tpb);
+            const uint32_t *B = P.block.data();
+            const int RF = P.rounds_fwd, RB = P.rounds_bwd;
+            // LDS image by position, with the dummy slot n
+            std::vector<float> dx(n + 1), dv(n + 1), su(n + 1), sj(n + 1);
+            for (int i = 0; i < n; i++) {
+                dx[P.pos[i]] = D0[i];
+                dv[P.pos[i]] = X0[i];
+                su[P.pos[i]] = i ? ui[i] : 0.0f;
+                sj[P.pos[i]] = i ? uj[i] : 0.0f;
+            }
+            dx[n] = 1.0f; dv[n] = 0.0f; su[n] = 0.0f; sj[n] = 0.0f;
+            std::vector<int> folded(n + 1, 0), solved(n + 1, 0);
+            auto word = [&](int r, int t, uint32_t idle) {
+                const uint32_t h = B[r];
+                return (uint32_t)t < (h >> 16) ? B[(h & 0xFFFFu) + t] : idle;
+            };
+            for (int r = 0; r < RF + RB + 2; r++)
+                if ((int)(B[r] >> 16) > tpb) bad++;
+            for (int r = 0; r < RF; r++) {
+                // all tasks of a round read before any writes (distinct heights do not mix)
+                std::vector<std::pair<int, std::pair<float, float>>> out;
+                for (int t = 0; t < tpb; t++) {
+                    const uint32_t w = word(r, t, idle_fwd_word(n));
+                    const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = (int)(w >> 22);
+                    const int c1 = nc >= 2 ? fc + 1 : n;
+                    const float r0 = sj[fc] / dx[fc];
+                    const float r1 = sj[c1] / dx[c1];
+                    float ox = (dx[p] - r0 * su[fc]) - r1 * su[c1];
+                    float oy = (dv[p] - r0 * dv[fc]) - r1 * dv[c1];
+                    for (int i = 2; i < nc; i++) {
+                        const float ratio = sj[fc + i] / dx[fc + i];
+                        ox -= ratio * su[fc + i];
+                        oy -= ratio * dv[fc + i];
+                    }
+                    if (p != n) folded[p]++;
+                    out.push_back({p, {ox, oy}});
+                }
+                for (auto &o : out) {
+                    dx[o.first] = o.second.first;
+                    dv[o.first] = o.second.second;
+                }
+            }
+            dv[0] = dv[0] / dx[0];
+            for (int r = 0; r < RB; r++) {
+                std::vector<std::pair<int, float>> out;
+                for (int t = 0; t < tpb; t++) {
+                    const uint32_t w = word(RF + 1 + r, t, idle_bwd_word(n));
+                    const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
+                    if (p != n) solved[p]++;
+                    out.push_back({p, (dv[p] - su[p] * dv[pp]) / dx[p]});
+                }
+                for (auto &o : out) dv[o.first] = o.second;
+            }
+            if (dx[n] != 1.0f || dv[n] != 0.0f) bad++;
+            for (int i = 0; i < n; i++) {
+                const int p = P.pos[i];
+                if (folded[p] != (ch[i].empty() ? 0 : 1)) bad++;
+                if (solved[p] != (i ? 1 : 0)) bad++;
+                if (std::memcmp(&dv[p], &Xr[i], 4) != 0 || std::memcmp(&dx[p], &Dr[i], 4) != 0) {
+                    std::printf("trial %d shape %d n %d tpb %d: node %d gives %.9g, serial %.9g\n", trial, shape, n,
+                                tpb, i, dv[p], Xr[i]);
+                    bad++;
+                    break;
+                }
+            }
+        }
+    }
+    if (bad) {
+        std::printf("FAIL %d\n", bad);
+        return 1;
+    }
+    std::printf("ok %d\n", runs);
+    return 0;
+}
