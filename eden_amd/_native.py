@@ -85,6 +85,7 @@ EXPORTS = {
     "edenhip_kernel_count": (C.c_int32, [C.c_void_p]),
     "edenhip_get_kernel_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(KernelInfo)]),
     "edenhip_get_kernel_cells": (C.c_int, [C.c_void_p, C.c_int32, _P, C.c_int32]),
+    "edenhip_diag_divide": (C.c_int, [C.c_uint64, C.c_int64, _P, C.c_int32, C.POINTER(C.c_int64)]),
     "edenhip_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),
     "edenhip_destroy": (None, [C.c_void_p]),

@@ -92,6 +92,59 @@ struct DevTables {
     int32_t log_syn;
 };
 
+// Correctly rounded fp32 division whose expansion does not go through VCC.
+//
+// hipcc expands n / d into: v_div_scale of d, v_rcp, one Newton step on the reciprocal,
+// v_div_scale of n, the quotient, two residual corrections (the last one by v_div_fmas, an
+// fma that applies the scaling recorded in VCC) and v_div_fixup for special operands.
+// v_div_scale leaves its operand unchanged and VCC clear unless d is zero, denormal or so
+// large that 1/d is denormal, n is zero or has a biased exponent <= 23, the exponent of n
+// exceeds that of d by 96 or more, or n/d is denormal. div_rn_fast requires, with margins,
+// the opposite: d's biased exponent in [2, 250], and n either a zero (v_div_fixup then
+// returns the signed zero whatever its first operand) or with biased exponent in [26, 252]
+// and exponent difference n - d in [-120, 90]. Then the sequence below performs the same
+// operations on the same values as the compiler's and gives the same correctly rounded
+// quotient; `ok` reports whether the operands were in that range. It does not use VCC, so
+// independent divisions can overlap instead of being serialised on that register.
+__device__ __forceinline__ float div_rn_fast(float n, float d, bool &ok) {
+    // range test without short-circuit operators, so it stays branch-free
+    const uint32_t un = __float_as_uint(n), ud = __float_as_uint(d);
+    const int32_t en = (int32_t)((un >> 23) & 0xFFu);
+    const int32_t ed = (int32_t)((ud >> 23) & 0xFFu);
+    const bool d_ok = (uint32_t)(ed - 2) <= 248u;
+    const bool n_zero = (un << 1) == 0u;
+    const bool n_ok = ((uint32_t)(en - 26) <= 226u) & ((uint32_t)(en - ed + 120) <= 210u);
+    ok = d_ok & (n_zero | n_ok);
+    float y = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = n * y;
+    float r = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(r, y, q);
+    return __builtin_amdgcn_div_fixupf(q, d, n);
+}
+
+__device__ __forceinline__ float div_rn(float n, float d) {
+    bool ok;
+    float q = div_rn_fast(n, d, ok);
+    if (__builtin_expect(!ok, 0)) q = n / d;
+    return q;
+}
+
+// Two independent divisions whose fast paths share one basic block (so they interleave),
+// with one fallback branch for both.
+__device__ __forceinline__ void div2_rn(float n0, float d0, float n1, float d1, float &q0, float &q1) {
+    bool ok0, ok1;
+    q0 = div_rn_fast(n0, d0, ok0);
+    q1 = div_rn_fast(n1, d1, ok1);
+    if (__builtin_expect(!(ok0 & ok1), 0)) {
+        if (!ok0) q0 = n0 / d0;
+        if (!ok1) q1 = n1 / d1;
+    }
+}
+
 // fp32 exp and log rounded once from a double evaluation: correctly rounded except when the
 // double result lies within one double ulp of a float rounding boundary (about 2^-28 of
 // arguments). glibc's expf/logf, which the reference's generated C calls, are within
@@ -100,8 +153,10 @@ struct DevTables {
 //
 // exp in double: Cody-Waite reduction x = k ln2 + r with |r| <= ln2/2 (ln2 split in a
 // 53-bit head and a tail), the Taylor series of e^r to degree 13 (truncation error below
-// 4e-18 relative) evaluated by Horner in double, then scaling by 2^k. Total relative error
-// is a few double ulps, far below the 2^-24 spacing the final rounding to float resolves.
+// 4e-18 relative), then scaling by 2^k. The polynomial is evaluated by Estrin's scheme
+// (pairs, then powers r^2, r^4, r^8), four dependent fma levels instead of Horner's
+// thirteen. Total relative error is a few double ulps, far below the 2^-24 spacing the
+// final rounding to float resolves.
 __device__ __forceinline__ double exp_taylor(double x) {
     const double LOG2E = 1.4426950408889634074;
     const double LN2_HI = 0x1.62e42ff000000p-1;    // ln2 rounded to 32 significant bits: k * LN2_HI is exact
@@ -109,20 +164,20 @@ __device__ __forceinline__ double exp_taylor(double x) {
     const double k = rint(x * LOG2E);
     double r = fma(-k, LN2_HI, x);
     r = fma(-k, LN2_LO, r);
-    double p = 1.0 / 6227020800.0;                      // 1/13!
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
+    const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    const double p01 = r + 1.0;
+    const double p23 = fma(r, 1.0 / 6.0, 0.5);
+    const double p45 = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    const double p67 = fma(r, 1.0 / 5040.0, 1.0 / 720.0);
+    const double p89 = fma(r, 1.0 / 362880.0, 1.0 / 40320.0);
+    const double pab = fma(r, 1.0 / 39916800.0, 1.0 / 3628800.0);
+    const double pcd = fma(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);
+    const double q0 = fma(r2, p23, p01);
+    const double q1 = fma(r2, p67, p45);
+    const double q2 = fma(r2, pab, p89);
+    const double s0 = fma(r4, q1, q0);
+    const double s1 = fma(r4, pcd, q2);
+    const double p = fma(r8, s1, s0);
     return ldexp(p, (int)k);
 }
 
@@ -149,16 +204,60 @@ __device__ __forceinline__ float log_f(float x) {
 //   EXPLINEAR  r * ((V == mid) ? 1 : ((V - mid) / scale) / (1 - exp(-(V - mid) / scale)))
 //   SIGMOID    r / (1 + exp((mid - V) / scale))
 //   FIXED      r
-__device__ __forceinline__ float rate_eval(int form, float r, float mid, float scale, float V) {
+// Division policies for the gate math: DivFast takes div_rn_fast and remembers whether all
+// operands were in its range; DivIeee is the compiler's division. Both give the same
+// quotient wherever DivFast reports ok.
+struct DivFast {
+    bool ok = true;
+    __device__ __forceinline__ float operator()(float n, float d) {
+        bool o;
+        const float q = div_rn_fast(n, d, o);
+        ok = ok & o;
+        return q;
+    }
+};
+struct DivIeee {
+    __device__ __forceinline__ float operator()(float n, float d) const { return n / d; }
+};
+
+template <class Div>
+__device__ __forceinline__ float rate_eval_t(Div &dv, int form, float r, float mid, float scale, float V) {
     const bool sig = form == RATE_SIGMOID, lin = form == RATE_EXPLINEAR;
-    const float x = (sig ? (mid - V) : (V - mid)) / scale;
+    const float x = dv(sig ? (mid - V) : (V - mid), scale);
     const float e = exp_f(lin ? -x : x);
     const float num = lin ? x : r;
     const float den = lin ? (1.0f - e) : (1.0f + e);
-    const float q = num / den;
+    const float q = dv(num, den);
     const float v_exp = r * e;
     const float v_lin = r * ((V == mid) ? 1.0f : q);
     return form == RATE_EXP ? v_exp : (lin ? v_lin : (sig ? q : r));
+}
+
+// One gate's next value from the membrane voltage: rates, tau, inf, cnexp with clipping
+// (eden/Eden.cpp:5236-5259, 5336-5375); initialisation steps set the gate to inf.
+template <class Div>
+__device__ __forceinline__ float gate_next_t(Div &dv, const DGate &G, float V, float q, bool init, float dt) {
+    const float alpha = rate_eval_t(dv, G.forms & 0xFF, G.ar, G.am, G.as, V);
+    const float beta = rate_eval_t(dv, (G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
+    const float tau = dv(1.0f, alpha + beta);
+    const float inf = dv(alpha, alpha + beta);
+    if (init) return inf;
+    const float tau_factor = dv(dt * G.q10, tau);
+    const float blend = exp_f(-tau_factor);
+    float nv = (blend) * q + (1.0f - blend) * inf;
+    if (!(nv > 1e-6f)) nv = 1e-6f;
+    if (!(nv < (float)(1 - 1e-6))) nv = (float)(1 - 1e-6);
+    return nv;
+}
+
+__device__ __forceinline__ float gate_next(const DGate &G, float V, float q, bool init, float dt) {
+    DivFast f;
+    float nv = gate_next_t(f, G, V, q, init, dt);
+    if (__builtin_expect(!f.ok, 0)) {
+        DivIeee d;
+        nv = gate_next_t(d, G, V, q, init, dt);
+    }
+    return nv;
 }
 
 // Thomas Wang's 64-bit integer hash (published 1997); EDEN's RNG mixer
@@ -197,6 +296,14 @@ struct CompConst {
     int32_t rec, ev;
 };
 
+// A tree-cell compartment that is not simple: its constants and compartment word, staged
+// per cell (80 bytes, read with 16-byte loads).
+struct alignas(16) TreeComplex {
+    CompConst k;
+    uint32_t tcm;
+};
+static_assert(sizeof(TreeComplex) == 80, "TreeComplex layout");
+
 __device__ __forceinline__ void load_comp_const(const DevTables &T, int64_t gidx, CompConst &k) {
     k.C = T.C[gidx]; k.A = T.A[gidx]; k.invrc = T.invrc[gidx]; k.vthr = T.vthr[gidx];
     k.chmeta = T.chmeta[gidx]; k.gmeta = T.gmeta[gidx];
@@ -229,21 +336,8 @@ __device__ __forceinline__ void channels_step(const DGate *__restrict__ gates, c
             const int gi = (k.gmeta >> (4 * s + 2)) & 3;
             const int type = (k.chmeta >> (8 * cs)) & 0xFF;
             const DGate G = gates[type * GATES_PER_TYPE + gi];
-            const float alpha = rate_eval(G.forms & 0xFF, G.ar, G.am, G.as, V);
-            const float beta = rate_eval((G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
-            const float tau = 1.0f / (alpha + beta);
-            const float inf = alpha / (alpha + beta);
             const float qs = q[s];
-            float nv;
-            if (init) {
-                nv = inf;
-            } else {
-                const float tau_factor = ((dt * G.q10) / tau);
-                const float blend = exp_f(-tau_factor);
-                nv = (blend) * qs + (1.0f - blend) * inf;
-                if (!(nv > 1e-6f)) nv = 1e-6f;
-                if (!(nv < (float)(1 - 1e-6))) nv = (float)(1 - 1e-6);
-            }
+            const float nv = gate_next(G, V, qs, init, dt);
             const int inst = (G.forms >> 16) & 0xFF;
 #pragma unroll
             for (int c = 0; c < NCH; c++) {

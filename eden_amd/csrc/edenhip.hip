@@ -34,6 +34,8 @@ struct Bucket {
     int32_t ninmax = 0;
     int32_t nctmax = 0;
     int32_t words_max = 0;
+    int32_t cxmax = 0;        // largest complex-compartment count of the bucket's cells
+    int32_t cxmax_lds = 0;    // cxmax when the records are staged in LDS, else 0
     int64_t comps = 0;
     size_t lds = 0;
     int grid = 0;
@@ -76,7 +78,8 @@ struct edenhip_ctx {
     uint32_t *d_words = nullptr;
     int32_t *d_pos = nullptr;
     float2 *d_u = nullptr;
-    uint32_t *d_tcm = nullptr, *d_ctask = nullptr, *d_inl = nullptr;
+    uint32_t *d_tcm = nullptr, *d_ctask = nullptr, *d_inl = nullptr, *d_gt = nullptr, *d_cflag = nullptr;
+    TreeComplex *d_cx = nullptr;
     float *d_gq = nullptr;
     std::vector<Bucket> buckets;
     hipEvent_t ev_start = nullptr;
@@ -306,6 +309,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     // state, in compartment / distribution / gate order
     std::vector<uint32_t> tcm(NC, 0), ctask, inl;
     std::vector<float> gqv;
+    std::vector<uint32_t> cflag(NC, 0);
+    std::vector<TreeComplex> cxv;
+    std::vector<uint32_t> gtv;   // per gate variable: its channel task (local to the cell) | gate index << 16
     for (int c = 0; c < d->n_cells; c++) {
         const int64_t c0 = d->cell_comp_begin[c], c1 = d->cell_comp_begin[c + 1];
         const int32_t n = (int32_t)(c1 - c0);
@@ -337,6 +343,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                 mask |= 1u << slot;
                 ctask.push_back((uint32_t)i | ((uint32_t)d->chan_type[ch] << 11) | ((uint32_t)first_gate << 17));
                 for (int gi = 0; gi < ct.n_gates; gi++) {
+                    gtv.push_back((uint32_t)(ctask.size() - 1 - (size_t)tc.ctask_off) | ((uint32_t)gi << 16));
                     const double a = edenhip_rate_host(ct.gates[gi].alpha, d->v0[g]);
                     const double b = edenhip_rate_host(ct.gates[gi].beta, d->v0[g]);
                     gqv.push_back((float)(a / (a + b)));
@@ -357,6 +364,33 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             tcm[g] |= TCM_INPUT;
         }
         tc.n_input = (int32_t)(inl.size() - tc.input_off);
+        // complex compartments: anything but one ungated channel with no pulse, no Poisson
+        // input and no spike event column; their constants go to a per-cell record table
+        tc.cx_off = (int32_t)cxv.size();
+        for (int32_t i = 0; i < n; i++) {
+            const int64_t g = c0 + i;
+            const bool simple = (tcm[g] & 0xFFu) == 1u && pulse_begin[g + 1] == pulse_begin[g] && ev_col[g] < 0;
+            if (simple) continue;
+            TreeComplex x{};
+            x.k.C = C[g]; x.k.A = A[g]; x.k.invrc = invrc[g]; x.k.vthr = vthr[g];
+            x.k.chmeta = chmeta[g]; x.k.gmeta = gmeta[g];
+            for (int k = 0; k < NCH; k++) { x.k.g[k] = chg[g * NCH + k]; x.k.e[k] = che[g * NCH + k]; }
+            const int32_t pb = pulse_begin[g], pe = pulse_begin[g + 1];
+            if (pe > pb) {
+                x.k.p0s = pstart[pb]; x.k.p0e = pend[pb]; x.k.p0a = pamp[pb];
+                x.k.gmeta |= GM_PULSE;
+                if (pe > pb + 1) x.k.gmeta |= GM_MORE_PULSES;
+            } else {
+                x.k.p0s = 0.f; x.k.p0e = -1.f; x.k.p0a = 0.f;
+            }
+            x.k.rec = rec_col[g]; x.k.ev = ev_col[g];
+            x.tcm = tcm[g];
+            const int32_t idx = (int32_t)(cxv.size() - (size_t)tc.cx_off);
+            if (idx > 0xFFFF) return fail(ctx, EDENHIP_E_LIMIT, "more than 65536 complex compartments in a cell");
+            cflag[g] = TCX_COMPLEX | (uint32_t)idx;
+            cxv.push_back(x);
+        }
+        tc.n_cx = (int32_t)(cxv.size() - (size_t)tc.cx_off);
 
         tc.comp_begin = c0;
         tc.n = n;
@@ -504,6 +538,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     TRY(upload(ctx, &ctx->d_tcm, tcm));
     TRY(upload(ctx, &ctx->d_ctask, ctask));
     TRY(upload(ctx, &ctx->d_gq, gqv));
+    TRY(upload(ctx, &ctx->d_gt, gtv));
+    TRY(upload(ctx, &ctx->d_cflag, cflag));
+    TRY(upload(ctx, &ctx->d_cx, cxv));
     TRY(upload(ctx, &ctx->d_inl, inl));
     TRY(snap(ctx->d_gq, gqv.size() * sizeof(float)));
 
@@ -527,6 +564,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         b->ninmax = std::max(b->ninmax, cells[i].n_input);
         b->nctmax = std::max(b->nctmax, cells[i].n_ctask);
         b->words_max = std::max(b->words_max, cells[i].words_len);
+        b->cxmax = std::max(b->cxmax, cells[i].n_cx);
     }
     std::vector<TreeCell> dcells;
     for (auto &b : ctx->buckets) {
@@ -541,7 +579,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // input list, and the task words for workgroups of up to 128 threads
         b.nctmax = std::max(b.nctmax, 1);   // ungated slots read sF[first task], kept in bounds
         b.split = b.tpb == 64 && ctx->opt.persistent >= 0;
-        b.lds = (size_t)tree_lds(b.nmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.split).total * sizeof(float);
+        // complex-compartment records go to LDS when they take at most 20 KiB
+        b.cxmax_lds = b.cxmax <= 256 ? b.cxmax : 0;
+        b.lds = (size_t)tree_lds(b.nmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, b.split).total *
+                sizeof(float);
         const void *kp = tree_kernel_ptr(b.tpb, b.split);
         if (!kp) return fail(ctx, EDENHIP_E_LIMIT, "no tree kernel for the chosen workgroup size");
         int per_cu = 0;
@@ -601,7 +642,8 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             int32_t ntask = (int32_t)bk.order.size();
             void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
                             &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &ctx->d_tcm,
-                            &ctx->d_ctask, &ctx->d_gq, &ctx->d_inl, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
+                            &ctx->d_ctask, &ctx->d_gq, &ctx->d_gt, &ctx->d_inl, &ctx->d_cflag,
+                            &ctx->d_cx, &bk.cxmax_lds, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
                             &bk.words_max, (void *)&a, (void *)&b, &row0, &dt};
             HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
             if (prof && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));

@@ -141,35 +141,23 @@ __device__ __forceinline__ float channel_fopen(const DGate *gates, int type, con
     return f;
 }
 
-// One gate's update from the current membrane voltage: rates, tau, inf, cnexp with clipping
-// (eden/Eden.cpp:5236-5259, 5336-5375); initialisation steps set the gate to inf.
-__device__ __forceinline__ float gate_update(const DGate &G, float V, float q, bool init, float dt) {
-    const float alpha = rate_eval(G.forms & 0xFF, G.ar, G.am, G.as, V);
-    const float beta = rate_eval((G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
-    const float tau = 1.0f / (alpha + beta);
-    const float inf = alpha / (alpha + beta);
-    if (init) return inf;
-    const float tau_factor = ((dt * G.q10) / tau);
-    const float blend = exp_f(-tau_factor);
-    float nv = (blend) * q + (1.0f - blend) * inf;
-    if (!(nv > 1e-6f)) nv = 1e-6f;
-    if (!(nv < (float)(1 - 1e-6))) nv = (float)(1 - 1e-6);
-    return nv;
-}
-
-// Gated channels of a cell, one per lane of the channel group (lanes `lane` + `stride` k):
-// next gate values from the voltage, then the channel's open fraction for the next step.
-__device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t *sCT, float *sQ, float *sF_out,
-                                              const float *sV_in, int NCT, int lane, int stride, bool init,
-                                              float dt) {
+// Channel phase of a cell, on the lanes `lane` + `stride` k of the channel group: every gate
+// variable's next value from its compartment's voltage (one lane per gate), then, after
+// `sync`, each gated channel's open fraction for the next step (one lane per channel).
+template <class Sync>
+__device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t *sCT, const uint32_t *sGT, float *sQ,
+                                              float *sF_out, const float *sV_in, int NG, int NCT, int lane,
+                                              int stride, bool init, float dt, Sync sync) {
+    for (int g = lane; g < NG; g += stride) {
+        const uint32_t gt = sGT[g];
+        const uint32_t w = sCT[gt & 0xFFFFu];
+        const int type = (w >> 11) & 63;
+        sQ[g] = gate_next(gates[type * GATES_PER_TYPE + (gt >> 16)], sV_in[w & 0x7FF], sQ[g], init, dt);
+    }
+    sync();
     for (int i = lane; i < NCT; i += stride) {
         const uint32_t w = sCT[i];
-        const int type = (w >> 11) & 63;
-        float *q = sQ + (w >> 17);
-        const float Vc = sV_in[w & 0x7FF];
-        const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
-        for (int gi = 0; gi < ng; gi++) q[gi] = gate_update(gates[type * GATES_PER_TYPE + gi], Vc, q[gi], init, dt);
-        sF_out[i] = channel_fopen(gates, type, q);
+        sF_out[i] = channel_fopen(gates, (w >> 11) & 63, sQ + (w >> 17));
     }
 }
 
@@ -187,12 +175,13 @@ __device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t
 // the same time as the cable group's step; voltages and open fractions are then
 // double-buffered and the two meet at one workgroup barrier per step.
 template <int TPB, bool SPLIT>
-__global__ void __launch_bounds__(TPB + (SPLIT ? 64 : 0))
+__global__ void __launch_bounds__(TPB + (SPLIT ? 64 : 0)) __attribute__((amdgpu_waves_per_eu(3)))
 tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const DSynType *__restrict__ stypes_g,
             int n_stypes, const TreeCell *__restrict__ cells, const int32_t *__restrict__ task_order, int32_t n_tasks,
             int32_t *__restrict__ queue, const uint32_t *__restrict__ words_g, const int32_t *__restrict__ pos_g,
             const float2 *__restrict__ u_g, const uint32_t *__restrict__ tcm_g, const uint32_t *__restrict__ ctask_g,
-            float *__restrict__ gq_g, const uint32_t *__restrict__ inl_g, int32_t nmax, int32_t ngmax, int32_t nctmax,
+            float *__restrict__ gq_g, const uint32_t *__restrict__ gt_g, const uint32_t *__restrict__ inl_g,
+            const uint32_t *__restrict__ cflag_g, const TreeComplex *__restrict__ cx_g, int32_t cxmax_lds, int32_t nmax, int32_t ngmax, int32_t nctmax,
             int32_t ninmax, int32_t words_max, int64_t s0, int64_t s1, int64_t row0, float dt) {
     constexpr int K = TREE_SLOTS;
     constexpr int NT = TPB + (SPLIT ? 64 : 0);
@@ -201,13 +190,15 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     __shared__ DSynType stypes[64];
     __shared__ int32_t s_task;
     // dynamic LDS, laid out by tree_lds()
-    const TreeLds L = tree_lds(nmax, ngmax, nctmax, ninmax, words_max, SPLIT);
+    const TreeLds L = tree_lds(nmax, ngmax, nctmax, ninmax, words_max, cxmax_lds, SPLIT);
     float *sm = reinterpret_cast<float *>(smem);
+    TreeComplex *sCX = reinterpret_cast<TreeComplex *>(sm + L.cx);
     float2 *sDX = reinterpret_cast<float2 *>(sm + L.dx);
     float2 *sU = reinterpret_cast<float2 *>(sm + L.u);
     float2 *sIn = reinterpret_cast<float2 *>(sm + L.in);
     float *sVb = sm + L.v;
     float *sQ = sm + L.q;
+    uint32_t *sGT = reinterpret_cast<uint32_t *>(sm + L.gt);
     float *sFb = sm + L.f;
     uint32_t *sCT = reinterpret_cast<uint32_t *>(sm + L.ct);
     uint32_t *sIL = reinterpret_cast<uint32_t *>(sm + L.il);
@@ -246,9 +237,21 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             return w;
         };
         for (int i = tid; i < N; i += NT) sU[i] = u_g[c0 + i];
-        for (int i = tid; i < NG; i += NT) sQ[i] = gq_g[gq0 + i];
+        for (int i = tid; i < NG; i += NT) {
+            sQ[i] = gq_g[gq0 + i];
+            sGT[i] = gt_g[gq0 + i];
+        }
         for (int i = tid; i < NCT; i += NT) sCT[i] = ctask_g[cell.ctask_off + i];
         for (int i = tid; i < NIN; i += NT) sIL[i] = inl_g[cell.input_off + i];
+        // complex-compartment records: staged in LDS when the bucket's largest count fits,
+        // else read from global memory
+        const TreeComplex *CX = cx_g + cell.cx_off;
+        if (cxmax_lds > 0) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(cx_g + cell.cx_off);
+            uint4 *dst = reinterpret_cast<uint4 *>(sCX);
+            for (int i = tid; i < cell.n_cx * 5; i += NT) dst[i] = src[i];
+            CX = sCX;
+        }
         for (int i = tid; i < N; i += NT) sVb[i] = T.V[c0 + i];
         if (tid == 0) {
             // dummy slot for threads without a task: finite values, never read back
@@ -256,22 +259,43 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             sDX[N] = make_float2(1.0f, 0.0f);
         }
 
-        CompConst kc[K];
-        uint32_t tcm[K];
-        float V[K];
-        int pos[K];
+        // Per slot registers: voltage, BFS position, flag word, trace column, and for a simple
+        // compartment its channel's Gbase and Erev, area, capacitance, and the conductance sum
+        // and Hines diagonal, which do not change from step to step (computed here with the
+        // per-step operation order). A complex compartment's constants come from its record.
+        // Slots beyond the cell behave as simple compartments with no channel conductance,
+        // C = 1 and the dummy position N, so they compute exactly the dummy slot's (1, 0).
+        float V[K], g0[K], e0[K], Ar[K], Cr[K], Gs[K], Ds[K];
+        int pos[K], rec[K];
+        uint32_t fl[K];
+        bool slot_any[K], slot_simple[K];   // per wave and slot: any valid lane, no complex lane
+        const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const int c = tid + TPB * j;
-            V[j] = 0.0f;
-            pos[j] = 0;
-            tcm[j] = 0;
-            if (cable && c < N) {
-                load_comp_const(T, c0 + c, kc[j]);
-                tcm[j] = tcm_g[c0 + c];
-                V[j] = T.V[c0 + c];
-                pos[j] = pos_g[c0 + c];
+            const bool valid = cable && c < N;
+            V[j] = 0.0f; g0[j] = 0.0f; e0[j] = 0.0f; Ar[j] = 0.0f; Cr[j] = 1.0f; Gs[j] = 0.0f; Ds[j] = 1.0f;
+            pos[j] = N; rec[j] = -1; fl[j] = 0;
+            if (valid) {
+                const int64_t g = c0 + c;
+                fl[j] = cflag_g[g];
+                V[j] = T.V[g];
+                pos[j] = pos_g[g];
+                rec[j] = T.rec_col[g];
+                if (!(fl[j] & TCX_COMPLEX)) {
+                    g0[j] = T.chg[g * NCH];
+                    e0[j] = T.che[g * NCH];
+                    Ar[j] = T.A[g];
+                    Cr[j] = T.C[g];
+                    const float G_ch = 0.0f + (((g0[j] * 1.0f) * 1e-5f) * Ar[j]);
+                    const float G_int = (G_ch + 0.0f) + 0.0f;
+                    const float Dd = 1 * (G_int / Cr[j]) + T.invrc[g];
+                    Gs[j] = G_int;
+                    Ds[j] = 1 + Dd * dt;
+                }
             }
+            slot_any[j] = __builtin_amdgcn_readfirstlane(wbase + TPB * j) < N && cable;
+            slot_simple[j] = __ballot(valid && (fl[j] & TCX_COMPLEX)) == 0ull;
         }
         __syncthreads();
         // open fractions of the gated channels from the present gate values
@@ -305,16 +329,27 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     }
                     lds_barrier<TPB>();
                 }
-                // compartments: currents from the present open fractions, semi-implicit RHS
+                // compartments: currents from the present open fractions, semi-implicit RHS.
+                // Slots without a valid lane in this wave are skipped; a wave whose valid
+                // lanes in a slot are all simple compartments takes the short form of the
+                // same operations (the constant conductance and diagonal computed above).
 #pragma unroll
                 for (int j = 0; j < K; j++) {
+                    if (!slot_any[j]) continue;
                     const int c = tid + TPB * j;
-                    if (c < N) {
-                        if (!init && kc[j].rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + kc[j].rec] = V[j];
+                    // simple form: channel slot 0, Gbase * 1 (ungated) * (E - V) * 1e-5f * A,
+                    // then the empty input and synapse terms, as in the general form below
+                    const float I_ch_s = 0.0f + ((((g0[j] * 1.0f) * (e0[j] - V[j])) * 1e-5f) * Ar[j]);
+                    const float I_int_s = (I_ch_s + 0.0f) + 0.0f;
+                    float Dn = Ds[j];
+                    float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
+                    if (!slot_simple[j] && (fl[j] & TCX_COMPLEX)) {
+                        const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
+                        const CompConst k = cx.k;
                         float I_ch, G_ch;
-                        tree_channels(kc[j], tcm[j], sF, V[j], I_ch, G_ch);
-                        float I_in = pulses_current(T, kc[j], c0 + c, t), G_in = 0.0f;
-                        if (tcm[j] & TCM_INPUT) {
+                        tree_channels(k, cx.tcm, sF, V[j], I_ch, G_ch);
+                        float I_in = pulses_current(T, k, c0 + c, t), G_in = 0.0f;
+                        if (cx.tcm & TCM_INPUT) {
                             const float2 in = sIn[c];
                             I_in += in.x;
                             G_in += in.y;
@@ -322,13 +357,15 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const float I_syn = 0.0f, G_syn = 0.0f;
                         const float I_int = I_ch + I_in + I_syn;
                         const float G_int = G_ch + G_in + G_syn;
-                        const float Dd = 1 * (G_int / kc[j].C) + kc[j].invrc;
-                        const float Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int))) / kc[j].C);
-                        sDX[pos[j]] = make_float2(1 + Dd * dt, Vn);
+                        const float Dd = 1 * (G_int / k.C) + k.invrc;
+                        Dn = 1 + Dd * dt;
+                        Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int)))) / k.C;
                     }
+                    sDX[pos[j]] = make_float2(Dn, Vn);
                 }
                 lds_barrier<TPB>();
-                if constexpr (!SPLIT) channel_phase(gates, sCT, sQ, sFn, sV, NCT, tid, TPB, init, dt);
+                if constexpr (!SPLIT)
+                    channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, tid, TPB, init, dt, [] { lds_barrier<TPB>(); });
                 if (!init) {
                     // forward elimination, rounds of independent gathers. Straight-line code:
                     // threads without a task work on the dummy slot N (sDX[N] = (1, 0) and
@@ -336,35 +373,48 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     // slot as its second child, whose contribution r1 * 0 is exactly zero.
                     // Nodes with more than two children fold the rest in a loop. The next
                     // round's word is always read (the schedule ends with an idle round).
+                    // Task words are prefetched two rounds ahead and round headers three: a
+                    // round's LDS reads of its own operands are issued first, the prefetch
+                    // reads after them, and nothing in the round waits on the prefetch.
+                    // Waves without a task in a round skip its arithmetic (the round's task
+                    // count is uniform, read from its header).
                     uint32_t w = task_word(0, idle_fwd);
                     uint32_t wn = task_word(1, idle_fwd);
+                    uint32_t hc = sW[0], hn = sW[1], h2 = sW[2];
                     for (int r = 0; r < RF; r++) {
-                        // rounds r + 1 and r + 2 exist: the sweep ends with an empty round and
-                        // the backward rounds follow
-                        const uint32_t wnn = task_word(r + 2, idle_fwd);
-                        const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
-                        const int c1 = nc >= 2 ? fc + 1 : N;
-                        const float2 dx = sDX[p];
-                        const float2 d0 = sDX[fc], u0 = sU[fc];
-                        const float2 d1 = sDX[c1], u1 = sU[c1];
-                        const float r0 = u0.y / d0.x;
-                        const float r1 = u1.y / d1.x;
-                        float2 o;
-                        o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
-                        o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
-                        if (nc > 2) {
-                            for (int i = 2; i < nc; i++) {
-                                const float2 dxc = sDX[fc + i];
-                                const float2 u = sU[fc + i];
-                                const float ratio = u.y / dxc.x;
-                                o.x -= ratio * u.x;
-                                o.y -= ratio * dxc.y;
+                        // headers r + 2 and r + 3 exist while r < RF: the forward sweep ends with
+                        // an empty round and at least one backward round and an empty one follow
+                        const uint32_t h3 = sW[r + 3];
+                        uint32_t wnn = idle_fwd;
+                        if ((uint32_t)tid < (h2 >> 16)) wnn = sW[(h2 & 0xFFFFu) + tid];
+                        if (wbase < (int)(__builtin_amdgcn_readfirstlane(hc) >> 16)) {
+                            const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
+                            const int c1 = nc >= 2 ? fc + 1 : N;
+                            const float2 dx = sDX[p];
+                            const float2 d0 = sDX[fc], u0 = sU[fc];
+                            const float2 d1 = sDX[c1], u1 = sU[c1];
+                            const float r0 = u0.y / d0.x;
+                            const float r1 = u1.y / d1.x;
+                            float2 o;
+                            o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
+                            o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
+                            if (nc > 2) {
+                                for (int i = 2; i < nc; i++) {
+                                    const float2 dxc = sDX[fc + i];
+                                    const float2 u = sU[fc + i];
+                                    const float ratio = u.y / dxc.x;
+                                    o.x -= ratio * u.x;
+                                    o.y -= ratio * dxc.y;
+                                }
                             }
+                            sDX[p] = o;
                         }
-                        sDX[p] = o;
                         lds_barrier<TPB>();
                         w = wn;
                         wn = wnn;
+                        hc = hn;
+                        hn = h2;
+                        h2 = h3;
                     }
                     if (tid == 0) {
                         float2 r0 = sDX[0];
@@ -373,34 +423,50 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     }
                     lds_barrier<TPB>();
                     // back-substitution, rounds by depth, straight-line as above
-                    const int rb0 = RF + 1;
+                    const int rb0 = RF + 1, rlast = RF + RB + 1;
                     w = task_word(rb0, idle_bwd);
-                    wn = task_word(rb0 + (RB > 0 ? 1 : 0), idle_bwd);
+                    wn = task_word(min(rb0 + 1, rlast), idle_bwd);
+                    hc = sW[rb0];
+                    hn = sW[min(rb0 + 1, rlast)];
+                    h2 = sW[min(rb0 + 2, rlast)];
                     for (int r = 0; r < RB; r++) {
-                        const uint32_t wnn = (r + 2 <= RB) ? task_word(rb0 + r + 2, idle_bwd) : idle_bwd;
-                        const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
-                        float2 dx = sDX[p];
-                        const float ui = sU[p].x;
-                        const float xp = sDX[pp].y;
-                        dx.y = (dx.y - ui * xp) / dx.x;
-                        sDX[p] = dx;
+                        const uint32_t h3 = sW[min(rb0 + r + 3, rlast)];
+                        uint32_t wnn = idle_bwd;
+                        if ((uint32_t)tid < (h2 >> 16)) wnn = sW[(h2 & 0xFFFFu) + tid];
+                        if (wbase < (int)(__builtin_amdgcn_readfirstlane(hc) >> 16)) {
+                            const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
+                            float2 dx = sDX[p];
+                            const float ui = sU[p].x;
+                            const float xp = sDX[pp].y;
+                            dx.y = (dx.y - ui * xp) / dx.x;
+                            sDX[p] = dx;
+                        }
                         lds_barrier<TPB>();
                         w = wn;
                         wn = wnn;
+                        hc = hn;
+                        hn = h2;
+                        h2 = h3;
                     }
                 }
 #pragma unroll
                 for (int j = 0; j < K; j++) {
                     const int c = tid + TPB * j;
-                    if (c < N) {
+                    if (slot_any[j] && c < N) {
+                        // the trace row of step s holds the voltage the step started from
+                        if (!init && rec[j] >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + rec[j]] = V[j];
                         const float Vn = init ? V[j] : sDX[pos[j]].y;
-                        if (V[j] < kc[j].vthr && kc[j].vthr < Vn) log_event(T, kc[j].ev, s);
+                        if (fl[j] & TCX_COMPLEX) {
+                            const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
+                            const float vthr = cx.k.vthr;
+                            if (V[j] < vthr && vthr < Vn) log_event(T, cx.k.ev, s);
+                        }
                         V[j] = Vn;
                         sVn[c] = Vn;
                     }
                 }
             } else {
-                channel_phase(gates, sCT, sQ, sFn, sV, NCT, tid - TPB, 64, init, dt);
+                channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, tid - TPB, 64, init, dt, [] { lds_barrier<64>(); });
             }
             // the next step reads voltages and open fractions written above by other lanes
             if constexpr (SPLIT) {
@@ -433,7 +499,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     template __global__ void tree_kernel<TPB, SPLIT>(                                                              \
         DevTables, const DGate *, int, const DSynType *, int, const TreeCell *, const int32_t *, int32_t, int32_t *,  \
         const uint32_t *, const int32_t *, const float2 *, const uint32_t *, const uint32_t *, float *,             \
-        const uint32_t *, int32_t, int32_t, int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
+        const uint32_t *, const uint32_t *, const uint32_t *, const TreeComplex *, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
 EDENHIP_TREE_INST(64, true)
 EDENHIP_TREE_INST(64, false)
 EDENHIP_TREE_INST(128, false)

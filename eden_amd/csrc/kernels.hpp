@@ -19,7 +19,14 @@ struct TreeCell {
     int32_t input_off;    // offset of its input-list entries
     int32_t n_ctask;      // gated channel distributions
     int32_t ctask_off;    // offset of its channel tasks
+    int32_t n_cx;         // complex compartments (see TreeComplex)
+    int32_t cx_off;       // offset of their records
 };
+
+// Per-compartment flag word of tree cells: 0 for a simple compartment (one ungated channel,
+// no pulse, no Poisson input, no spike event column), else TCX_COMPLEX | index of its
+// TreeComplex record within the cell.
+constexpr uint32_t TCX_COMPLEX = 1u << 31;
 
 // Compartment word of the tree kernel: channel count (bits 0-2), gated-channel-slot mask
 // (bits 3-6), Poisson-input flag (bit 7), first channel task (bits 16-31).
@@ -29,24 +36,27 @@ constexpr int TREE_SLOTS = 4;        // compartments per thread
 
 
 
-// Dynamic LDS of the tree kernel, in 4-byte units: sDX, sU [nmax+1] float2 | sIn [nmax]
-// float2 (only with Poisson inputs) | sV [nbuf][nmax+1] | sQ [ngmax] | sF [nbuf][nctmax] |
+// Dynamic LDS of the tree kernel, in 4-byte units: sCX [cxmax] TreeComplex (only when the
+// records are staged in LDS) | sDX, sU [nmax+1] float2 | sIn [nmax] float2 (only with
+// Poisson inputs) | sV [nbuf][nmax+1] | sQ [ngmax] | sGT [ngmax] | sF [nbuf][nctmax] |
 // sCT [nctmax] | sIL [ninmax] | schedule block [words_max]; nbuf = 2 when a separate wave
 // runs the channel phase, else 1.
 struct TreeLds {
-    int32_t dx, u, in, v, q, f, ct, il, w, total;
+    int32_t cx, dx, u, in, v, q, gt, f, ct, il, w, total;
 };
 
 __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t nctmax, int32_t ninmax,
-                                            int32_t words_max, bool split) {
+                                            int32_t words_max, int32_t cxmax, bool split) {
     TreeLds L;
     const int32_t nbuf = split ? 2 : 1;
-    L.dx = 0;
+    L.cx = 0;
+    L.dx = L.cx + 20 * cxmax;
     L.u = L.dx + 2 * (nmax + 1);
     L.in = L.u + 2 * (nmax + 1);
     L.v = L.in + (ninmax > 0 ? 2 * nmax : 0);
     L.q = L.v + nbuf * (nmax + 1);
-    L.f = L.q + ngmax;
+    L.gt = L.q + ngmax;
+    L.f = L.gt + ngmax;
     L.ct = L.f + nbuf * nctmax;
     L.il = L.ct + nctmax;
     L.w = L.il + ninmax;
