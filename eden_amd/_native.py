@@ -70,6 +70,11 @@ class KernelInfo(C.Structure):
                 ("launches", C.c_int64), ("total_ms", C.c_double)]
 
 
+class AdjOptions(C.Structure):
+    _fields_ = [("checkpoint_steps", C.c_int32), ("threads_per_cell", C.c_int32), ("device", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 EXPORTS = {
     # name: (restype, argtypes)
     "edenhip_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(Options), C.POINTER(C.c_void_p)]),
@@ -85,6 +90,14 @@ EXPORTS = {
     "edenhip_kernel_count": (C.c_int32, [C.c_void_p]),
     "edenhip_get_kernel_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(KernelInfo)]),
     "edenhip_get_kernel_cells": (C.c_int, [C.c_void_p, C.c_int32, _P, C.c_int32]),
+    "edenhip_adj_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(AdjOptions), C.POINTER(C.c_void_p)]),
+    "edenhip_adj_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64)]),
+    "edenhip_adj_forward": (C.c_int, [C.c_void_p, _P, _P, _P]),
+    "edenhip_adj_backward": (C.c_int, [C.c_void_p, _P, _P, _P]),
+    "edenhip_adj_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "edenhip_adj_last_error": (C.c_char_p, [C.c_void_p]),
+    "edenhip_adj_destroy": (None, [C.c_void_p]),
     "edenhip_diag_divide": (C.c_int, [C.c_uint64, C.c_int64, _P, C.c_int32, C.POINTER(C.c_int64)]),
     "edenhip_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),

@@ -235,3 +235,16 @@ def spiking_network(n_cells=100_000, in_degree=100, t_final_ms=1000.0, seed=3001
         poisson_weight=np.ones(nc), poisson_instance=np.zeros(nc), poisson_rng_offset=np.zeros(nc),
         record_comp=np.arange(min(n_record, nc)), event_comp=np.arange(nc),
         name=f"spiking_network_{n_cells}")
+
+
+def config5(n_cells=256, n_comps=100, t_final_ms=500.0, seed=2001, init_seed=5001):
+    """Config 5 (adjoint fitting): n_cells random trees of n_comps compartments built as in
+    config 2, HH on every compartment, somatic current steps. Returns (model, gbase_init):
+    the model carries the ground-truth densities (per compartment gNa, gK, gLeak in its
+    channel distributions); gbase_init = ground truth x U(0.5, 1.5), independently per
+    channel distribution (numpy default_rng(init_seed))."""
+    m = random_trees(n_cells, sizes=[n_comps] * n_cells, t_final_ms=t_final_ms, seed=seed, hh_everywhere=True)
+    rng = np.random.default_rng(init_seed)
+    g0 = (np.asarray(m.chan_gbase, np.float64) * rng.uniform(0.5, 1.5, len(m.chan_gbase)))
+    m.name = f"config5_{n_cells}x{n_comps}"
+    return m, g0

@@ -1,0 +1,65 @@
+/* edenhip_adjoint.h -- discrete adjoint of the fixed-step integrator for cells without
+ * synapses (HH-style channels, passive leak, pulse inputs, tree cables).
+ *
+ * No counterpart in the reference (EDEN has no gradients). The differentiated map is the
+ * per-step update of eden/Eden.cpp:5236-5259 (gates), 5805-5898 (channel currents), 3798-3826
+ * (pulses), 6057-6107 (membrane) and 6114-6225 (Hines solve), evaluated in double precision
+ * with the reference's operation order (as the FP64 restatement in oracle/ does). Spike
+ * threshold crossings only feed event logs here, so they carry no gradient.
+ *
+ * The forward pass records the voltage trace of the recorded compartments (double) and a
+ * checkpoint of the full state every `checkpoint_steps` steps; the backward pass takes
+ * dLoss/dtrace and, segment by segment from the last checkpoint, recomputes the segment's
+ * states and runs the adjoint recursion backwards, accumulating dLoss/dGbase per channel
+ * distribution and dLoss/dAmplitude per pulse input. Gates clipped to [1e-6, 1 - 1e-6] in a
+ * step pass no gradient through that step's update.
+ *
+ * Pointers marked (device) are device memory of the context's GPU; calls synchronise their
+ * work before returning. */
+#ifndef EDENHIP_ADJOINT_H
+#define EDENHIP_ADJOINT_H
+#include <stdint.h>
+
+#include "edenhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct edenhip_adj edenhip_adj;
+
+typedef struct edenhip_adj_options {
+    int32_t checkpoint_steps;    /* steps between checkpoints (0 = 500) */
+    int32_t threads_per_cell;    /* 0 = by cell size, or 64/128/256 */
+    int32_t device;
+    int32_t reserved;
+} edenhip_adj_options;
+
+/* Build the differentiable model from the same descriptor edenhip_create() takes. The model
+ * must have no synapses, spike recipients or Poisson inputs (EDENHIP_E_INVALID otherwise).
+ * The descriptor's chan_gbase and pulse_amplitude are the default parameter values. */
+int edenhip_adj_create(const edenhip_model_desc *desc, const edenhip_adj_options *opt, edenhip_adj **out);
+
+/* Sizes: rows of the trace, recorded columns, channel distributions, pulse inputs. */
+int edenhip_adj_sizes(const edenhip_adj *adj, int64_t *rows, int64_t *n_record, int64_t *n_chan, int64_t *n_pulse);
+
+/* Forward pass with parameters gbase[n_chan] and amplitude[n_pulse] (device, double; NULL =
+ * the descriptor's values): writes trace[rows x n_record] (device, double, row r = voltage
+ * at time r * dt in mV) and keeps the checkpoints for edenhip_adj_backward. */
+int edenhip_adj_forward(edenhip_adj *adj, const double *gbase, const double *amplitude, double *trace);
+
+/* Backward pass for the parameters of the last forward pass: grad_trace[rows x n_record]
+ * (device, double) is dLoss/dtrace; writes grad_gbase[n_chan] and grad_amplitude[n_pulse]
+ * (device, double; either may be NULL). */
+int edenhip_adj_backward(edenhip_adj *adj, const double *grad_trace, double *grad_gbase, double *grad_amplitude);
+
+/* Milliseconds of device time of the last forward and backward passes (HIP events). */
+int edenhip_adj_timing(const edenhip_adj *adj, double *forward_ms, double *backward_ms);
+
+const char *edenhip_adj_last_error(const edenhip_adj *adj);
+void edenhip_adj_destroy(edenhip_adj *adj);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDENHIP_ADJOINT_H */

@@ -120,6 +120,12 @@ static real rate_value(const edenhip_rate *r, real V) {
 
 /* Optional synaptic event log: every (synapse instance, step) at which a spike-triggered
  * synapse receives its event (spike_now, eden/Eden.cpp:3649-3685). Set before a run. */
+/* Optional double-precision copy of the trace (set before a run; the FP64 flavour's values
+ * before their rounding to the float trace). */
+static double *g_trace64 = NULL;
+
+void FN(set_trace64)(double *buf) { g_trace64 = buf; }
+
 static int64_t *g_syn_ev = NULL;
 static int64_t g_syn_ev_max = 0;
 static int64_t g_syn_ev_n = 0;
@@ -498,6 +504,8 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
                 if (n_rows >= 0 && rows >= n_rows) break;
                 if (trace)
                     for (int64_t r = 0; r < d->n_record; r++) trace[rows * d->n_record + r] = (float)s.V[now][d->record_comp[r]];
+                if (g_trace64)
+                    for (int64_t r = 0; r < d->n_record; r++) g_trace64[rows * d->n_record + r] = (double)s.V[now][d->record_comp[r]];
                 rows++;
             }
             for (int64_t e = 0; e < d->n_event_cols; e++) {

@@ -40,6 +40,9 @@ def lib():
             sc = getattr(L, f"oracle_{sfx}_synapse_log_count")
             sc.restype = C.c_int64
             sc.argtypes = []
+            st = getattr(L, f"oracle_{sfx}_set_trace64")
+            st.restype = None
+            st.argtypes = [C.c_void_p]
             o = getattr(L, f"oracle_{sfx}_dfs_order")
             o.restype = None
             o.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
@@ -85,6 +88,20 @@ def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_
     ne = min(n_ev.value, max_events)
     tr = None if trace is None else trace[:rows, :nrec]
     return tr, ev_col[:ne].copy(), ev_row[:ne].copy(), final_v
+
+
+def run_f64_trace(model, threads=0):
+    """FP64 flavour's trace in double precision ([rows x n_record] float64)."""
+    L = lib()
+    total = model.total_rows()
+    nrec = len(model.record_comp)
+    buf = np.zeros((max(total, 1), max(nrec, 1)), np.float64)
+    L.oracle_f64_set_trace64(buf.ctypes.data)
+    try:
+        run(model, precision="f64", threads=threads, want_trace=True)
+    finally:
+        L.oracle_f64_set_trace64(None)
+    return buf[:total, :nrec]
 
 
 def hash64shift(x):

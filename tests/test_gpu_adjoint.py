@@ -1,0 +1,136 @@
+"""GPU discrete adjoint (include/edenhip_adjoint.h) against finite differences.
+
+* the adjoint's FP64 forward against the oracle's FP64 flavour (same model, same order of
+  operations; only the double exp/log implementations differ);
+* dLoss/dGbase and dLoss/dAmplitude against central finite differences in FP64 of the
+  GPU's own FP64 forward (h = 1e-6 relative): 1e-4 relative;
+* the same gradient against central finite differences of the oracle's FP64 CPU path;
+* the checkpoint interval does not change the gradient (recomputation is deterministic; only
+  the order in which per-segment gradient sums are added differs, so equal to rounding).
+
+Models: config 5's construction (random trees, HH on every compartment, somatic current
+steps) at 3 cells x 24 compartments and 40 ms, so the oracle runs in well under a second.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from eden_amd import configs
+from eden_amd.adjoint import Adjoint, trace
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _model(t_final_ms=40.0):
+    m, g0 = configs.config5(n_cells=3, n_comps=24, t_final_ms=t_final_ms)
+    # float-representable starting parameters, so the oracle (fp32 inputs) sees the same values
+    return m, np.asarray(g0, np.float32).astype(np.float64)
+
+
+def _loss(tr, target):
+    return ((tr - target) ** 2).sum() / tr.numel()
+
+
+def test_adjoint_forward_matches_oracle_f64():
+    m, _ = _model()
+    _say(f"[adjoint] forward vs oracle f64: cells={m.n_cells} comps={m.n_comps} rows={m.total_rows()}")
+    with Adjoint(m, checkpoint_steps=100) as adj:
+        tr = adj.forward().cpu().numpy()
+    ref = oracle.run_f64_trace(m)
+    assert tr.shape == ref.shape
+    err = float(np.max(np.abs(tr - ref)))
+    _say(f"[adjoint] max |V_gpu64 - V_oracle64| = {err:.3g} mV, V range {ref.min():.1f}..{ref.max():.1f}")
+    assert ref.max() > 0.0          # the cells spike
+    assert err < 1e-8
+
+
+def _gradients(adj, g0, target):
+    g = torch.tensor(g0, dtype=torch.float64, device=DEV, requires_grad=True)
+    _, a0 = adj.default_parameters()
+    a = a0.clone().requires_grad_(True)
+    L = _loss(trace(adj, g, a), target)
+    L.backward()
+    return g.grad.detach().cpu().numpy(), a.grad.detach().cpu().numpy(), a0
+
+
+def test_adjoint_gradient_matches_central_fd():
+    m, g0 = _model()
+    with Adjoint(m, checkpoint_steps=37) as adj:
+        gt, a0 = adj.default_parameters()
+        target = adj.forward(gt, a0)
+        gg, ga, a0 = _gradients(adj, g0, target)
+        _say(f"[adjoint] forward {adj.timing()[0]:.2f} ms, backward {adj.timing()[1]:.2f} ms")
+
+        def L(g, a):
+            return float(_loss(adj.forward(torch.tensor(g, dtype=torch.float64, device=DEV), a), target))
+
+        rng = np.random.default_rng(0)
+        big = np.argsort(-np.abs(gg))[:6]
+        pick = np.unique(np.concatenate([big, rng.choice(len(g0), 6, replace=False)]))
+        scale = float(np.abs(gg).max())
+        for i in pick:
+            h = 1e-6 * abs(g0[i])
+            gp, gm = g0.copy(), g0.copy()
+            gp[i] += h
+            gm[i] -= h
+            fd = (L(gp, a0) - L(gm, a0)) / (gp[i] - gm[i])
+            _say(f"[adjoint] gbase[{i}] = {g0[i]:.4g}: adjoint {gg[i]:.9g}  fd {fd:.9g}")
+            assert abs(gg[i] - fd) <= 1e-4 * abs(fd) + 1e-9 * scale
+        for p in range(len(ga)):
+            ap, am = a0.clone(), a0.clone()
+            h = 1e-6 * abs(float(a0[p]))
+            ap[p] += h
+            am[p] -= h
+            fd = (L(g0, ap) - L(g0, am)) / float(ap[p] - am[p])
+            _say(f"[adjoint] amplitude[{p}] = {float(a0[p]):.4g}: adjoint {ga[p]:.9g}  fd {fd:.9g}")
+            assert abs(ga[p] - fd) <= 1e-4 * abs(fd)
+
+
+def test_adjoint_gradient_matches_fd_of_the_oracle():
+    """Central differences of the oracle's FP64 CPU path (fp32 parameter inputs, so the step
+    is the exact float difference)."""
+    m, g0 = _model()
+    target = oracle.run_f64_trace(m)
+    with Adjoint(m) as adj:
+        gg, _, _ = _gradients(adj, g0, torch.tensor(target, device=DEV))
+    base = m.chan_gbase.copy()
+
+    def L(gf32):
+        m.chan_gbase = gf32
+        try:
+            tr = oracle.run_f64_trace(m)
+        finally:
+            m.chan_gbase = base
+        return float(((tr - target) ** 2).sum() / tr.size)
+
+    for i in np.argsort(-np.abs(gg))[:3]:
+        gp = g0.astype(np.float32).copy()
+        gm = gp.copy()
+        gp[i] = np.float32(g0[i] * (1 + 1e-4))
+        gm[i] = np.float32(g0[i] * (1 - 1e-4))
+        fd = (L(gp) - L(gm)) / (float(gp[i]) - float(gm[i]))
+        _say(f"[adjoint] oracle fd gbase[{i}]: adjoint {gg[i]:.9g}  fd {fd:.9g}")
+        assert abs(gg[i] - fd) <= 1e-3 * abs(fd)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    m, g0 = _model(t_final_ms=20.0)
+    res = []
+    for K in (7, 500):
+        with Adjoint(m, checkpoint_steps=K) as adj:
+            gt, a0 = adj.default_parameters()
+            target = adj.forward(gt, a0)
+            gg, ga, _ = _gradients(adj, g0, target)
+            res.append((gg, ga))
+    np.testing.assert_allclose(res[0][0], res[1][0], rtol=1e-11, atol=1e-14 * np.abs(res[1][0]).max())
+    np.testing.assert_allclose(res[0][1], res[1][1], rtol=1e-11, atol=1e-14 * np.abs(res[1][1]).max())
