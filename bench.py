@@ -12,6 +12,13 @@ Prints one JSON line: compartment-updates/s (rows x compartments / s), the roofl
 dominant kernel (its algorithmic bytes per launch over its HIP-event-timed duration), the
 oracle's CPU throughput on a bounded sample (cpu_baseline) and a trace-tolerance check.
 
+The "adjoint" object (skip with --no-adjoint) is measured on the adjoint-fitting case: 256
+random trees of 100 compartments with HH on every compartment, 20,001 rows, trainable
+Gbase per channel distribution initialised at U(0.5, 1.5) x ground truth, MSE on somatic V
+against the ground-truth trace, checkpoints every 500 steps. It reports the FP64
+differentiable forward, the backward (recompute + adjoint) and the bit-exact fp32 forward
+of the same model, and gradient evaluations per second.
+
 --calibrate-copy MiB times a plain device-to-device copy of that size instead (the
 reference kernel for scaling the FETCH_SIZE / WRITE_SIZE profiler counters to bytes).
 """
@@ -139,6 +146,46 @@ def parity_sample(model_name, n_cells=16):
                 within_tol=bool(res["max_abs_mV"] <= TOL_MV))
 
 
+def adjoint_bench(reps=2):
+    """Adjoint-fitting case: fp32 forward time, FP64 forward and backward times."""
+    import torch
+    from eden_amd import configs
+    from eden_amd.adjoint import Adjoint, trace
+    from eden_amd.model import Simulation
+    m, g0 = configs.config5()
+    with Simulation(m, profile=True) as sim:
+        fwd32 = []
+        for _ in range(reps):
+            sim.reset()
+            t0 = time.perf_counter()
+            sim.run_all()
+            fwd32.append(time.perf_counter() - t0)
+    with Adjoint(m, checkpoint_steps=500) as adj:
+        gt, a0 = adj.default_parameters()
+        target = adj.forward(gt, a0)
+        g = torch.tensor(g0, dtype=torch.float64, device=adj.device, requires_grad=True)
+        f64, bwd, tot = [], [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            loss = ((trace(adj, g, a0) - target) ** 2).mean()
+            loss.backward()
+            torch.cuda.synchronize()
+            tot.append(time.perf_counter() - t0)
+            fms, bms = adj.timing()
+            f64.append(fms / 1e3)
+            bwd.append(bms / 1e3)
+            g.grad = None
+    f32 = float(np.min(fwd32))
+    return dict(workload="256 random trees x 100 compartments, HH on every compartment, 20,001 rows, "
+                         "Gbase per channel distribution trainable (76,800 parameters), MSE on somatic V, "
+                         "checkpoints every 500 steps",
+                forward_fp32_s=f32, forward_fp64_s=float(np.min(f64)), backward_s=float(np.min(bwd)),
+                gradient_s=float(np.min(tot)), grads_per_s=1.0 / float(np.min(tot)),
+                backward_over_forward_fp64=float(np.min(bwd)) / float(np.min(f64)),
+                backward_over_forward_fp32=float(np.min(bwd)) / f32,
+                loss=float(loss.detach()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -222,6 +270,9 @@ def main():
     if not args.no_parity:
         print("[bench] parity sample", file=sys.stderr, flush=True)
         out["parity"] = parity_sample(args.workload)
+    if not args.no_adjoint:
+        print("[bench] adjoint (256 x 100-compartment trees)", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)

@@ -98,6 +98,9 @@ EXPORTS = {
     "edenhip_adj_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "edenhip_adj_last_error": (C.c_char_p, [C.c_void_p]),
     "edenhip_adj_destroy": (None, [C.c_void_p]),
+    "edenhip_write_trajectory": (C.c_int, [C.c_char_p, C.c_int, _P, _P, C.c_int64, C.c_int64, _P]),
+    "edenhip_write_events": (C.c_int, [C.c_char_p, C.c_int, C.c_int32, _P, _P, _P, C.c_int64, _P, C.c_int64]),
+    "edenhip_format_number": (C.c_int, [C.c_double, C.c_char_p, C.c_int32]),
     "edenhip_diag_divide": (C.c_int, [C.c_uint64, C.c_int64, _P, C.c_int32, C.POINTER(C.c_int64)]),
     "edenhip_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),

@@ -1,0 +1,110 @@
+// output.cpp -- EDEN's trajectory and event text formats (include/edenhip_output.h).
+//
+// Number format: FixedWidthNumberPrinter with a 16-character column (eden/Eden.cpp:144-189):
+// "%+15.8g" into 15 characters, then the one-character delimiter ' '. Timestamps: engine
+// time (us) times 1e-6 (GetTimestampAscii, eden/Eden.cpp:184-189). Lines are written with a
+// trailing '\n' (LineWriter_File::PutFrameLines, eden/Eden.cpp:10633-10640).
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/edenhip_output.h"
+
+namespace {
+
+constexpr double US_TO_S = 1e-6;
+
+// 16 characters: the number right-aligned in 15, then a space
+inline void fixed16(double v, char *out) {
+    char tmp[64];
+    std::snprintf(tmp, 16, "%+15.8g", v);   // at most 15 characters, as the reference's snprintf bound
+    const size_t n = std::strlen(tmp);
+    std::memcpy(out, tmp, n);
+    out[n] = ' ';
+    out[n + 1] = '\0';
+}
+
+struct File {
+    FILE *f = nullptr;
+    File(const char *path, int append) { f = std::fopen(path, append ? "ab" : "wb"); }
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int edenhip_format_number(double value, char *buf, int32_t buf_len) {
+    if (!buf || buf_len < 17) return -1;
+    fixed16(value, buf);
+    return 0;
+}
+
+int edenhip_write_trajectory(const char *path, int append, const double *time_us, const float *trace, int64_t n_rows,
+                             int64_t n_cols, const double *scale) {
+    if (!path || n_rows < 0 || n_cols < 0 || (n_rows > 0 && (!time_us || (n_cols > 0 && (!trace || !scale))))) return -1;
+    File out(path, append);
+    if (!out.f) return -2;
+    std::string line;
+    char num[32];
+    for (int64_t r = 0; r < n_rows; r++) {
+        line.clear();
+        fixed16(time_us[r] * US_TO_S, num);
+        line += num;
+        for (int64_t c = 0; c < n_cols; c++) {
+            line += ' ';
+            fixed16((double)trace[r * n_cols + c] * scale[c], num);
+            line += num;
+        }
+        line += '\n';
+        if (std::fwrite(line.data(), 1, line.size(), out.f) != line.size()) return -2;
+    }
+    return std::ferror(out.f) ? -2 : 0;
+}
+
+int edenhip_write_events(const char *path, int append, int32_t format, const double *time_us, const int64_t *ev_row,
+                         const int64_t *ev_col, int64_t n_events, const int64_t *ids, int64_t n_ids) {
+    if (!path || n_events < 0 || (n_events > 0 && (!time_us || !ev_row || !ev_col || !ids))) return -1;
+    if (format != EDENHIP_EVENTS_TIME_ID && format != EDENHIP_EVENTS_ID_TIME && format != EDENHIP_EVENTS_ASCII_V0) return -1;
+    for (int64_t e = 0; e < n_events; e++) {
+        if (ev_col[e] < 0 || ev_col[e] >= n_ids || ev_row[e] < 0) return -1;
+        if (e > 0 && (ev_row[e] < ev_row[e - 1] || (ev_row[e] == ev_row[e - 1] && ev_col[e] <= ev_col[e - 1]))) return -1;
+    }
+    File out(path, append);
+    if (!out.f) return -2;
+    std::string line;
+    char num[32];
+    for (int64_t e = 0; e < n_events;) {
+        const int64_t row = ev_row[e];
+        fixed16(time_us[row] * US_TO_S, num);
+        if (format == EDENHIP_EVENTS_ASCII_V0) {
+            line = num;
+            for (; e < n_events && ev_row[e] == row; e++) {
+                line += ' ';
+                line += std::to_string(ids[ev_col[e]]);
+            }
+            line += '\n';
+        } else {
+            // ID_TIME prints the timestamp without its padding
+            std::string t(num);
+            const size_t b = t.find_first_not_of(' ');
+            const size_t l = t.find_last_not_of(' ');
+            const std::string trimmed = b == std::string::npos ? std::string() : t.substr(b, l - b + 1);
+            line.clear();
+            for (; e < n_events && ev_row[e] == row; e++) {
+                const std::string id = std::to_string(ids[ev_col[e]]);
+                line += format == EDENHIP_EVENTS_TIME_ID ? t + " " + id : id + " " + trimmed;
+                line += '\n';
+            }
+        }
+        if (std::fwrite(line.data(), 1, line.size(), out.f) != line.size()) return -2;
+    }
+    return std::ferror(out.f) ? -2 : 0;
+}
+
+}  // extern "C"

@@ -142,3 +142,20 @@ def test_deviation_from_fp64_dt10_is_the_oracles():
     _say(f"[parity] deviation from FP64 dt/10: gpu {dev_g:.4g} mV, oracle fp32 {dev_o:.4g} mV, "
          f"spikes fp32 {len(er)} fp64/10 {spikes_64}")
     assert dev_g == dev_o
+
+
+def test_output_files_from_a_gpu_run(tmp_path):
+    """OutputFile / EventOutputFile written from a chunked GPU run, byte-compared with the
+    files built from the oracle's trace and events."""
+    from eden_amd import output
+    m = configs.hh_population(n_cells=40, t_final_ms=30.0)
+    tp, ep = tmp_path / "v.dat", tmp_path / "spikes.dat"
+    rows, nev = output.run_to_files(m, tp, ep, "TIME_ID", chunk_rows=333)
+    _say(f"[output] rows {rows} events {nev}")
+    tr, ec, er, _ = oracle.run(m)
+    t_us = output.row_times_us(m.dt, rows + 1)
+    output.write_trajectory(tmp_path / "v_ref.dat", t_us, tr)
+    output.write_events(tmp_path / "spikes_ref.dat", t_us, er, ec, np.arange(len(m.event_comp)), "TIME_ID")
+    assert nev > 0
+    assert tp.read_bytes() == (tmp_path / "v_ref.dat").read_bytes()
+    assert ep.read_bytes() == (tmp_path / "spikes_ref.dat").read_bytes()

@@ -1,0 +1,74 @@
+"""EDEN's output text formats (include/edenhip_output.h), byte for byte against the format
+the reference's writers produce: FixedWidthNumberPrinter(16) = "%+15.8g" then a space
+(eden/Eden.cpp:144-189), trajectory lines time ' ' values (eden/Eden.cpp:10856-10870),
+event lines TIME_ID / ID_TIME / ascii_v0 (eden/Eden.cpp:10897-10927), '\\n' after every line
+(eden/Eden.cpp:10633-10640). Python's %-formatting and C's printf both print the correctly
+rounded decimal, so the expected text is built here from the format specification."""
+import numpy as np
+import pytest
+
+from eden_amd import output
+
+
+def fx(v):
+    return "%+15.8g " % v
+
+
+def test_number_format_matches_the_printf_spec():
+    rng = np.random.default_rng(1)
+    vals = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-12, 12, 2000),
+                           [0.0, -0.0, 1e-300, -1e300, 5e-324, np.inf, -np.inf, 1.0, 0.1, 2.5e-5,
+                            -0.0655, 123456789.0, 1.23456785]])
+    for v in vals:
+        assert output.format_number(v) == fx(v)
+
+
+def test_trajectory_file_bytes(tmp_path):
+    t_us = output.row_times_us(25.0, 5)
+    tr = np.array([[-65.0, -64.5], [-64.9, 12.25], [0.0, 40.123456], [-70.0, -1e-3], [1.5, 2.5]], np.float32)
+    p = tmp_path / "v.dat"
+    output.write_trajectory(p, t_us, tr)
+    want = "".join(fx(t_us[r] * 1e-6) + "".join(" " + fx(float(tr[r, c]) * 1e-3) for c in range(2)) + "\n"
+                   for r in range(5))
+    assert p.read_text() == want
+    # appended chunks give the same file as one write
+    q = tmp_path / "w.dat"
+    output.write_trajectory(q, t_us[:2], tr[:2])
+    output.write_trajectory(q, t_us[2:], tr[2:], append=True)
+    assert q.read_text() == want
+    # and it reloads as the reference's loader does (np.fromstring of whitespace-separated values)
+    data = np.array(p.read_text().split(), float).reshape(5, 3)
+    np.testing.assert_allclose(data[:, 1:], tr.astype(float) * 1e-3, rtol=1e-7)
+
+
+@pytest.mark.parametrize("fmt", ["TIME_ID", "ID_TIME", "ascii_v0"])
+def test_event_file_bytes(tmp_path, fmt):
+    t_us = output.row_times_us(25.0, 100)
+    rows = np.array([3, 3, 7, 40, 40, 40])
+    cols = np.array([0, 2, 1, 0, 1, 2])
+    ids = np.array([10, 11, 12])
+    p = tmp_path / "e.dat"
+    output.write_events(p, t_us, rows, cols, ids, fmt)
+    lines = []
+    if fmt == "ascii_v0":
+        for r in sorted(set(rows)):
+            lines.append(fx(t_us[r] * 1e-6) + "".join(" %d" % ids[c] for rr, c in zip(rows, cols) if rr == r))
+    else:
+        for r, c in zip(rows, cols):
+            t = fx(t_us[r] * 1e-6)
+            lines.append(t + " %d" % ids[c] if fmt == "TIME_ID" else "%d " % ids[c] + t.strip())
+    assert p.read_text() == "".join(line + "\n" for line in lines)
+
+
+def test_event_writer_rejects_unsorted_events(tmp_path):
+    t_us = output.row_times_us(25.0, 10)
+    with pytest.raises(OSError):
+        output.write_events(tmp_path / "x.dat", t_us, [5, 4], [0, 0], [0], "TIME_ID")
+
+
+def test_row_times_accumulate_in_double():
+    t = output.row_times_us(25.0, 4001)
+    acc = 0.0
+    for r in range(4001):
+        assert t[r] == acc
+        acc += float(np.float32(25.0))
