@@ -43,21 +43,29 @@ B_HH = 56        # 24 + 3 gates r+w 24 + gNa*A, gK*A 8
 B_RECORD = 4     # recorded V per column-row
 
 
+B_HH_POINT = 40  # one-compartment HH cell: V, m, h, n r+w 32 + per-instance input 4 + recorded V 4
+B_SYNAPSE = 32   # per synapse-step, scan formulation: A, B r+w 16, weight 4, delay 4, pending slot r+w 8
+
+
 def algorithmic_bytes(model, rows, cells=None):
     """Bytes per full run over the given cells (all by default): per compartment-row 24 B
-    (passive) or 56 B (HH), plus 4 B per recorded column-row."""
+    (passive tree compartment), 56 B (HH tree compartment) or 40 B (one-compartment HH
+    cell), plus 4 B per recorded column-row of tree cells, plus 32 B per synapse-row
+    (chemical and Poisson-input synapses)."""
     n_chan = np.diff(model.comp_chan_begin)
     is_hh = n_chan > 1
-    if cells is None:
-        sel = np.ones(model.n_comps, bool)
-    else:
-        cb = model.cell_comp_begin
-        comp_cell = np.repeat(np.arange(model.n_cells), np.diff(cb))
-        sel = np.isin(comp_cell, np.asarray(cells))
-    hh = int((is_hh & sel).sum())
-    passive = int(sel.sum()) - hh
-    rec = int(sel[np.asarray(model.record_comp, np.int64)].sum())
-    return rows * (B_PASSIVE * passive + B_HH * hh + B_RECORD * rec)
+    cb = model.cell_comp_begin
+    comp_cell = np.repeat(np.arange(model.n_cells), np.diff(cb))
+    point = np.diff(cb)[comp_cell] == 1
+    sel = np.ones(model.n_comps, bool) if cells is None else np.isin(comp_cell, np.asarray(cells))
+    hh_point = int((is_hh & point & sel).sum())
+    hh = int((is_hh & ~point & sel).sum())
+    passive = int((sel & ~is_hh).sum())
+    rec_comp = np.asarray(model.record_comp, np.int64)
+    rec = int((sel[rec_comp] & ~point[rec_comp]).sum())
+    n_syn = np.diff(model.comp_syn_begin) + np.diff(model.comp_poisson_begin)
+    syn = int(n_syn[sel].sum())
+    return rows * (B_PASSIVE * passive + B_HH * hh + B_HH_POINT * hh_point + B_RECORD * rec + B_SYNAPSE * syn)
 
 
 def calibrate_copy(mib, reps=10):
@@ -186,6 +194,24 @@ def adjoint_bench(reps=2):
                 loss=float(loss.detach()))
 
 
+def plain_launch_bench(model, rows):
+    """The same workload with one kernel launch per step (state back in HBM between steps)
+    instead of the persistent in-kernel time loop."""
+    from eden_amd.model import Simulation
+    with Simulation(model, persistent=-1, profile=True) as sim:
+        sim.reset()
+        t0 = time.perf_counter()
+        done = 0
+        while done < rows:
+            n = sim.run(-1)
+            if n == 0:
+                break
+            done += n
+        el = time.perf_counter() - t0
+        launches = sum(k["launches"] for k in sim.kernel_infos())
+    return dict(seconds=el, compartment_updates_per_s=model.n_comps * rows / el, launches=int(launches))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +224,7 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -242,6 +269,11 @@ def main():
             acc["launches"] += ki["launches"]
         msg = ", ".join(f"{ki['name']} {ki['total_ms']:.2f} ms/{ki['launches']}" for ki in infos)
         print(f"[bench] step {s}: {el * 1e3:.2f} ms wall; {msg}", file=sys.stderr, flush=True)
+    net = None
+    if model.n_synapses > 0:
+        w, nwin, rerun, deliv = sim.network_stats()
+        net = dict(window_steps=w, windows=nwin, windows_rerun=rerun, synaptic_events=deliv,
+                   synapse_instances=int(model.n_synapses + len(model.poisson_rate)))
     sim.close()
     t_step = float(np.mean(times))
     updates = model.n_comps * rows
@@ -267,9 +299,19 @@ def main():
                    "launch_shape": {"threads_per_cell": tpb, "compartments_per_thread": slots}},
         "roofline": roofline,
     }
+    if net is not None:
+        if net["synaptic_events"] >= 0:
+            net["synaptic_events_per_s"] = net["synaptic_events"] / t_step
+        net["synapse_steps_per_s"] = net["synapse_instances"] * rows / t_step
+        out["network"] = net
     if not args.no_parity:
         print("[bench] parity sample", file=sys.stderr, flush=True)
         out["parity"] = parity_sample(args.workload)
+    if not args.no_plain and args.workload == "config4":
+        print("[bench] plain launch (one launch per step)", file=sys.stderr, flush=True)
+        pl = plain_launch_bench(model, rows)
+        pl["persistent_speedup"] = pl["seconds"] / t_step
+        out["plain_launch"] = pl
     if not args.no_adjoint:
         print("[bench] adjoint (256 x 100-compartment trees)", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_bench()

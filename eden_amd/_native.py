@@ -61,7 +61,7 @@ class ModelDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("threads_per_cell", C.c_int32), ("persistent", C.c_int32), ("chunk_steps", C.c_int32),
                 ("use_graph", C.c_int32), ("profile", C.c_int32), ("device", C.c_int32),
-                ("log_synapse_events", C.c_int32)]
+                ("log_synapse_events", C.c_int32), ("network_window", C.c_int32)]
 
 
 class KernelInfo(C.Structure):
@@ -102,6 +102,8 @@ EXPORTS = {
     "edenhip_write_events": (C.c_int, [C.c_char_p, C.c_int, C.c_int32, _P, _P, _P, C.c_int64, _P, C.c_int64]),
     "edenhip_format_number": (C.c_int, [C.c_double, C.c_char_p, C.c_int32]),
     "edenhip_diag_divide": (C.c_int, [C.c_uint64, C.c_int64, _P, C.c_int32, C.POINTER(C.c_int64)]),
+    "edenhip_network_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "edenhip_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),
     "edenhip_destroy": (None, [C.c_void_p]),

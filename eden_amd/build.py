@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wno-unused-function",
 ]
 
-SOURCES = ["kernels.hip", "edenhip.hip", "cable_setup.cpp", "diag.hip", "adjoint.hip", "output.cpp"]
+SOURCES = ["kernels.hip", "network.hip", "edenhip.hip", "cable_setup.cpp", "diag.hip", "adjoint.hip", "output.cpp"]
 
 
 def _run(cmd, cwd=None):

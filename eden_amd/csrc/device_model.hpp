@@ -476,15 +476,22 @@ __device__ __forceinline__ void synapses_step(const DevTables &T, const DSynType
 
 // Poisson firing synapse inputs of a compartment (LEMS OnStart / OnCondition with the hash
 // RNG), each driving its own expTwo synapse in the same step.
+// Poisson input state arrays (tsince, isi, expTwo A, B), so a caller can read one copy and
+// write another.
+struct PoiState {
+    float *tsince, *isi, *A, *B;
+};
+
 __device__ __forceinline__ void poisson_step(const DevTables &T, const DSynType *__restrict__ stypes, int64_t gidx,
-                                             float V, bool init, int64_t step, float dt, float &I_in, float &G_in) {
+                                             float V, bool init, int64_t step, float dt, float &I_in, float &G_in,
+                                             const PoiState &rd, const PoiState &wr) {
     const int32_t b = T.poi_begin[gidx], e = T.poi_begin[gidx + 1];
     if (e <= b) return;
     const int32_t seed = (int32_t)T.cell_seed[T.comp_cell[gidx]];
     float I_agg = 0.0f, G_agg = 0.0f;
     for (int32_t p = b; p < e; p++) {
         const float weight = T.poi_weight[p];
-        const float tsince = T.poi_tsince[p], isi = T.poi_isi[p];
+        const float tsince = rd.tsince[p], isi = rd.isi[p];
         const float averageIsi = T.poi_isi_mean[p];
         const int32_t inst = T.poi_inst[p], roff = T.poi_roff[p];
         bool fired = false;
@@ -501,11 +508,11 @@ __device__ __forceinline__ void poisson_step(const DevTables &T, const DSynType 
                 fired = true;
             }
         }
-        T.poi_tsince[p] = tn; T.poi_isi[p] = in;
+        wr.tsince[p] = tn; wr.isi[p] = in;
         const DSynType st = stypes[T.poi_syn_type[p]];
-        float A = T.poi_A[p], B = T.poi_B[p], i, g;
+        float A = rd.A[p], B = rd.B[p], i, g;
         exptwo_step(st, V, A, B, init, fired, dt, i, g);
-        T.poi_A[p] = A; T.poi_B[p] = B;
+        wr.A[p] = A; wr.B[p] = B;
         I_agg += i * weight;
         G_agg += g * weight;
     }

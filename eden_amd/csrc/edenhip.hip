@@ -58,7 +58,7 @@ struct edenhip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     float dt = 0;
-    int64_t NC = 0, total_rows = 0, rows_done = 0, next_step = -3;
+    int64_t NC = 0, n_poi = 0, n_slots = 0, total_rows = 0, rows_done = 0, next_step = -3;
     int n_cells = 0;
     bool network = false;
     // host mirrors for output mapping
@@ -95,6 +95,15 @@ struct edenhip_ctx {
     double point_ms = 0;
     int64_t point_launches = 0;
     std::vector<int32_t> point_desc, tree_desc;
+    // windowed network path (networks of one-compartment cells)
+    bool netwin = false;
+    int32_t net_W = 0;
+    int net_cur = 0;
+    NetTables N{};
+    uint32_t *d_ev_save = nullptr;   // event-log counters saved before a window
+    unsigned long long *d_deliv_save = nullptr;
+    int32_t *h_conflict = nullptr;   // pinned host copy of the conflict flag
+    int64_t net_windows = 0, net_redone = 0;
     std::string kernel_name;
     std::vector<void *> allocs;
 };
@@ -155,6 +164,8 @@ static std::vector<double> row_times(float dt, double t_final) {
     return ts;
 }
 
+static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point);
+
 static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     if (d->abi_version != EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
     if (!(d->dt > 0) || !(d->t_final >= 0)) return fail(ctx, EDENHIP_E_INVALID, "dt must be > 0 and t_final >= 0");
@@ -197,8 +208,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     for (int t = 0; t < d->n_syn_types; t++) {
         const edenhip_exptwo_type &s = d->syn_types[t];
         const float tr = s.tau_rise, td = s.tau_decay;
-        const float peak = (logf(td / tr) * (tr * td)) / (td - tr);
-        const float wf = 1.0f / ((-expf((-peak) / tr)) + expf((-peak) / td));
+        // logf/expf rounded once from double, as the kernels' log_f/exp_f
+        const float peak = ((float)std::log((double)(td / tr)) * (tr * td)) / (td - tr);
+        const float wf = 1.0f / ((-(float)std::exp((double)((-peak) / tr))) + (float)std::exp((double)((-peak) / td)));
         st[t] = DSynType{s.gbase, s.erev, s.tau_rise, s.tau_decay, s.weight_prop, wf, 0.f, 0.f};
     }
     ctx->n_stypes = (int)st.size();
@@ -211,6 +223,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     for (int c = 0; c < d->n_cells; c++)
         for (int64_t g = d->cell_comp_begin[c]; g < d->cell_comp_begin[c + 1]; g++) comp_cell[g] = c;
     const int64_t n_syn = d->comp_syn_begin[NC], n_spk = d->comp_spike_begin[NC], n_poi = d->comp_poisson_begin[NC];
+    ctx->n_poi = n_poi;
     const int64_t n_pulse = d->comp_pulse_begin[NC];
     if (n_syn >= (int64_t)1 << 31 || n_spk >= (int64_t)1 << 31 || n_poi >= (int64_t)1 << 31)
         return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapses or spike targets");
@@ -447,6 +460,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     }
     for (int64_t k = 0; k < n_spk; k++) spk_target[k] = (int32_t)blocked_of[d->spike_target[k]];
     bool any_targets = n_spk > 0;
+    ctx->n_slots = n_slots;
     ctx->network = any_targets;
 
     // device uploads
@@ -604,7 +618,199 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         HIPCHECK(ctx, hipEventCreate(&ctx->pp1));
     }
     HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_start, hipEventDisableTiming));
+    TRY(build_netwin(ctx, d, point));
     HIPCHECK(ctx, hipDeviceSynchronize());
+    return 0;
+}
+
+// Windowed network path: eligibility, per-compartment type runs, spike-target ring
+// addresses, second state copies. Models it does not cover keep the per-step path.
+static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point) {
+    if (!ctx->network || !ctx->buckets.empty() || ctx->opt.network_window == -1) return 0;
+    const int64_t NC = d->n_comps;
+    if (d->n_syn_types > 64 || (int64_t)point.size() != NC) return 0;
+    const double dt = (double)d->dt;
+    // window length: every synapse's delay spans at least W + 2 steps
+    double dmin = INFINITY, dmax = 0.0;
+    for (int64_t s = 0; s < d->comp_syn_begin[NC]; s++) {
+        dmin = std::min(dmin, (double)d->syn_delay[s]);
+        dmax = std::max(dmax, (double)d->syn_delay[s]);
+    }
+    if (!(dmin < INFINITY)) return 0;
+    const int64_t Dmin = (int64_t)std::floor(dmin / dt);
+    const int32_t W = (int32_t)std::min<int64_t>(64, Dmin - 2);
+    if (W < 2) return 0;
+    int64_t R = 1;
+    while (R < (int64_t)std::ceil(dmax / dt) + W + 8) R <<= 1;
+    // per compartment: up to four type runs (synapses are sorted by type), at most 256 synapses
+    std::vector<uint32_t> runs(NC * 4, 0u);
+    int64_t maxcnt = 0;
+    for (int64_t g = 0; g < NC; g++) {
+        const int64_t sb = d->comp_syn_begin[g], cnt = d->comp_syn_begin[g + 1] - sb;
+        if (cnt > 256) return 0;
+        maxcnt = std::max(maxcnt, cnt);
+        int nr = 0;
+        for (int64_t kq = 1; kq <= cnt; kq++)
+            if (kq == cnt || d->syn_type[sb + kq] != d->syn_type[sb + kq - 1]) {
+                if (nr >= 4) return 0;
+                runs[g * 4 + nr++] = (uint32_t)kq | ((uint32_t)d->syn_type[sb + kq - 1] << 16);
+            }
+    }
+    const int32_t nw = (int32_t)std::max<int64_t>(1, (maxcnt + 31) / 32);
+    const int32_t n_blocks = (int32_t)((point.size() + 63) / 64);
+    const int64_t stride = (int64_t)n_blocks * nw * 64;
+    if (stride >= ((int64_t)1 << 27)) return 0;
+    // ring address of every spike target: the target synapse's cell (block, lane) and index
+    std::vector<int64_t> point_of(NC, -1);
+    for (size_t p = 0; p < point.size(); p++) point_of[point[p]] = (int64_t)p;
+    std::vector<int64_t> comp_of_syn(d->comp_syn_begin[NC]);
+    for (int64_t g = 0; g < NC; g++)
+        for (int64_t sq = d->comp_syn_begin[g]; sq < d->comp_syn_begin[g + 1]; sq++) comp_of_syn[sq] = g;
+    const int64_t n_spk = d->comp_spike_begin[NC];
+    std::vector<uint32_t> tgt_rw(std::max<int64_t>(n_spk, 1), 0u);
+    for (int64_t kt = 0; kt < n_spk; kt++) {
+        const int64_t sd = d->spike_target[kt], g = comp_of_syn[sd], p = point_of[g];
+        const int64_t kk = sd - d->comp_syn_begin[g];
+        const int64_t word = ((p / 64) * nw + kk / 32) * 64 + (p % 64);
+        tgt_rw[kt] = ((uint32_t)word << 5) | (uint32_t)(kk % 32);
+    }
+    const int64_t n_slots = ctx->n_slots;
+    NetTables &N = ctx->N;
+    N.n_blocks = n_blocks;
+    N.nw = nw;
+    N.ring_mask = (int32_t)(R - 1);
+    N.ring_stride = stride;
+    TRY(upload(ctx, (uint32_t **)&N.runs, runs));
+    TRY(upload(ctx, (uint32_t **)&N.tgt_rw, tgt_rw));
+    N.gA[0] = ctx->T.syn_A;
+    N.gB[0] = ctx->T.syn_B;
+    TRY(alloc_zero(ctx, &N.gA[1], (size_t)n_slots));
+    TRY(alloc_zero(ctx, &N.gB[1], (size_t)n_slots));
+    TRY(alloc_zero(ctx, &N.ring, (size_t)(R * stride)));
+    const int64_t n_poi = d->comp_poisson_begin[NC];
+    N.V[0] = ctx->T.V; N.q[0] = ctx->T.q;
+    N.poi_tsince[0] = ctx->T.poi_tsince; N.poi_isi[0] = ctx->T.poi_isi;
+    N.poi_A[0] = ctx->T.poi_A; N.poi_B[0] = ctx->T.poi_B;
+    TRY(alloc_zero(ctx, &N.V[1], (size_t)NC));
+    TRY(alloc_zero(ctx, &N.q[1], (size_t)(NC * NQ)));
+    TRY(alloc_zero(ctx, &N.poi_tsince[1], (size_t)n_poi));
+    TRY(alloc_zero(ctx, &N.poi_isi[1], (size_t)n_poi));
+    TRY(alloc_zero(ctx, &N.poi_A[1], (size_t)n_poi));
+    TRY(alloc_zero(ctx, &N.poi_B[1], (size_t)n_poi));
+    TRY(alloc_zero(ctx, &N.spike_mask, (size_t)NC));
+    TRY(alloc_zero(ctx, &N.spk_list, (size_t)NC));
+    TRY(alloc_zero(ctx, &N.spk_count, 1));
+    TRY(alloc_zero(ctx, &N.conflict, 1));
+    TRY(alloc_zero(ctx, &N.n_deliv, 1));
+    TRY(alloc_zero(ctx, &ctx->d_deliv_save, 1));
+    TRY(alloc_zero(ctx, &ctx->d_ev_save, 2));
+    HIPCHECK(ctx, hipHostMalloc((void **)&ctx->h_conflict, sizeof(int32_t), hipHostMallocDefault));
+    *ctx->h_conflict = 0;
+    // reset rewinds the ring too (NextSpike and copy 0 of the state are already snapshotted)
+    {
+        void *p = nullptr;
+        const size_t bytes = (size_t)(R * stride) * sizeof(uint32_t);
+        HIPCHECK(ctx, hipMalloc(&p, bytes));
+        ctx->allocs.push_back(p);
+        HIPCHECK(ctx, hipMemset(p, 0, bytes));
+        ctx->snaps.push_back({N.ring, p, bytes});
+    }
+    ctx->net_W = W;
+    ctx->net_cur = 0;
+    ctx->netwin = true;
+    return 0;
+}
+
+// Copy the state copy `from` over copy `to` (compartments, gates, Poisson inputs, synapses).
+static int net_copy_state(edenhip_ctx *ctx, int from, int to) {
+    NetTables &N = ctx->N;
+    const int64_t NC = ctx->NC, n_poi = ctx->n_poi;
+    const int64_t n_slots = ctx->n_slots;
+    const hipStream_t st = ctx->stream;
+    HIPCHECK(ctx, hipMemcpyAsync(N.V[to], N.V[from], NC * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(N.q[to], N.q[from], NC * NQ * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (n_poi > 0) {
+        HIPCHECK(ctx, hipMemcpyAsync(N.poi_tsince[to], N.poi_tsince[from], n_poi * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(ctx, hipMemcpyAsync(N.poi_isi[to], N.poi_isi[from], n_poi * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(ctx, hipMemcpyAsync(N.poi_A[to], N.poi_A[from], n_poi * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(ctx, hipMemcpyAsync(N.poi_B[to], N.poi_B[from], n_poi * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHECK(ctx, hipMemcpyAsync(N.gA[to], N.gA[from], n_slots * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(ctx, hipMemcpyAsync(N.gB[to], N.gB[from], n_slots * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+// One window [a, b) of the network path from state copy ctx->net_cur; with check, the
+// deliver pass first only checks that the window matches step-by-step order. Returns 1 in
+// *conflict when it does not (nothing committed then).
+static int net_window(edenhip_ctx *ctx, int64_t a, int64_t b, int64_t row0, bool check, bool force, int *conflict,
+                      int64_t *nl) {
+    NetTables &N = ctx->N;
+    DevTables T = ctx->T;
+    const hipStream_t st = ctx->stream;
+    float dt = ctx->dt;
+    int cur = ctx->net_cur;
+    int64_t last = ctx->total_rows;
+    HIPCHECK(ctx, hipMemsetAsync(N.spk_count, 0, sizeof(uint32_t), st));
+    int32_t np = ctx->n_point;
+    void *wargs[] = {&T, &N, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &np,
+                     &a, &b, &row0, &dt, &cur};
+    HIPCHECK(ctx, hipLaunchKernel(net_window_kernel_ptr(), dim3(N.n_blocks), dim3(64), wargs,
+                                  (size_t)std::max(1, ctx->n_gates) * sizeof(DGate), st));
+    (*nl)++;
+    const int grid = 1024;
+    int commit = 0;
+    *conflict = 0;
+    if (check) {
+        HIPCHECK(ctx, hipMemsetAsync(N.conflict, 0, sizeof(int32_t), st));
+        void *dargs[] = {&T, &N, &a, &b, &last, &dt, &commit};
+        HIPCHECK(ctx, hipLaunchKernel(net_deliver_kernel_ptr(), dim3(grid), dim3(64), dargs, 0, st));
+        HIPCHECK(ctx, hipMemcpyAsync(ctx->h_conflict, N.conflict, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(ctx, hipStreamSynchronize(st));
+        (*nl)++;
+        if (*ctx->h_conflict || force) {
+            *conflict = 1;
+            return 0;
+        }
+    }
+    commit = 1;
+    void *dargs[] = {&T, &N, &a, &b, &last, &dt, &commit};
+    HIPCHECK(ctx, hipLaunchKernel(net_deliver_kernel_ptr(), dim3(grid), dim3(64), dargs, 0, st));
+    (*nl)++;
+    ctx->net_cur ^= 1;
+    return 0;
+}
+
+// Steps [s0, s1) of a network through windows of up to net_W steps, each checked against
+// step-by-step order and, where it differs, run again one step at a time.
+static int net_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0, int64_t *nl) {
+    const hipStream_t st = ctx->stream;
+    for (int64_t a = s0; a < s1;) {
+        const int64_t b = std::min<int64_t>(s1, a + ctx->net_W);
+        // event-log counters, restored if the window is run again
+        HIPCHECK(ctx, hipMemcpyAsync(ctx->d_ev_save, ctx->T.n_events, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(ctx, hipMemcpyAsync(ctx->d_ev_save + 1, ctx->T.n_syn_events, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(ctx, hipMemcpyAsync(ctx->d_deliv_save, ctx->N.n_deliv, sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+        int conflict = 0;
+        TRY(net_window(ctx, a, b, row0, true, ctx->opt.network_window == -2, &conflict, nl));
+        ctx->net_windows++;
+        if (conflict) {
+            HIPCHECK(ctx, hipMemcpyAsync(ctx->T.n_events, ctx->d_ev_save, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+            HIPCHECK(ctx, hipMemcpyAsync(ctx->T.n_syn_events, ctx->d_ev_save + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+            HIPCHECK(ctx, hipMemcpyAsync(ctx->N.n_deliv, ctx->d_deliv_save, sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+            for (int64_t s = a; s < b; s++) {
+                int c2 = 0;
+                TRY(net_window(ctx, s, s + 1, row0, false, false, &c2, nl));
+            }
+            ctx->net_redone++;
+        }
+        a = b;
+    }
+    // leave the state in copy 0, where the rest of the library reads it
+    if (ctx->net_cur != 0) {
+        TRY(net_copy_state(ctx, 1, 0));
+        ctx->net_cur = 0;
+    }
     return 0;
 }
 
@@ -619,6 +825,25 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
     const bool per_step = ctx->network || ctx->opt.persistent < 0;
     const int64_t stride = per_step ? 1 : (s1 - s0);
     int64_t nl = 0;
+    if (ctx->netwin) {
+        if (ctx->opt.profile) HIPCHECK(ctx, hipEventRecord(ctx->pp0, ctx->stream));
+        TRY(net_chunk(ctx, s0, s1, row0, &nl));
+        if (ctx->opt.profile) {
+            HIPCHECK(ctx, hipEventRecord(ctx->pp1, ctx->stream));
+            HIPCHECK(ctx, hipEventRecord(e1, ctx->stream));
+            HIPCHECK(ctx, hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+            ctx->kernel_ms += ms;
+            ctx->launches += nl;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->pp0, ctx->pp1));
+            ctx->point_ms += ms;
+            ctx->point_launches += nl;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        return 0;
+    }
     const bool prof = ctx->opt.profile != 0;
     for (int64_t a = s0; a < s1; a += stride) {
         const int64_t b = std::min(s1, a + stride);
@@ -729,6 +954,7 @@ int edenhip_reset(edenhip_ctx *ctx) {
     for (auto &s : ctx->snaps) HIPCHECK(ctx, hipMemcpyAsync(s.dst, s.src, s.bytes, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHECK(ctx, hipMemsetAsync(ctx->T.n_events, 0, sizeof(uint32_t), ctx->stream));
     HIPCHECK(ctx, hipMemsetAsync(ctx->T.n_syn_events, 0, sizeof(uint32_t), ctx->stream));
+    if (ctx->netwin) HIPCHECK(ctx, hipMemsetAsync(ctx->N.n_deliv, 0, sizeof(unsigned long long), ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->rows_done = 0;
     ctx->next_step = -3;
@@ -738,6 +964,9 @@ int edenhip_reset(edenhip_ctx *ctx) {
     ctx->launches = 0;
     ctx->point_ms = 0;
     ctx->point_launches = 0;
+    ctx->net_cur = 0;
+    ctx->net_windows = 0;
+    ctx->net_redone = 0;
     for (auto &bk : ctx->buckets) {
         bk.ms = 0;
         bk.launches = 0;
@@ -836,7 +1065,7 @@ int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_in
     if (!ctx || !out || i < 0 || i >= edenhip_kernel_count(ctx)) return EDENHIP_E_INVALID;
     std::memset(out, 0, sizeof(*out));
     if (ctx->n_point > 0 && i == 0) {
-        std::snprintf(out->name, sizeof(out->name), "point_kernel");
+        std::snprintf(out->name, sizeof(out->name), ctx->netwin ? "net_window_kernel" : "point_kernel");
         out->threads_per_block = 64;
         out->grid = (ctx->n_point + 63) / 64;
         out->n_cells = ctx->n_point;
@@ -866,6 +1095,20 @@ int edenhip_get_kernel_cells(const edenhip_ctx *ctx, int32_t i, int32_t *cells, 
     const Bucket &b = ctx->buckets[(size_t)(i - (ctx->n_point > 0 ? 1 : 0))];
     if (max_cells < (int32_t)b.order.size()) return EDENHIP_E_INVALID;
     for (size_t k = 0; k < b.order.size(); k++) cells[k] = ctx->tree_desc[(size_t)b.order[k]];
+    return 0;
+}
+
+int edenhip_network_stats(edenhip_ctx *ctx, int32_t *window_steps, int64_t *windows, int64_t *windows_rerun,
+                          int64_t *synaptic_events) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (window_steps) *window_steps = ctx->netwin ? ctx->net_W : 1;
+    if (windows) *windows = ctx->net_windows;
+    if (windows_rerun) *windows_rerun = ctx->net_redone;
+    if (synaptic_events) {
+        unsigned long long n = 0;
+        if (ctx->netwin) HIPCHECK(ctx, hipMemcpy(&n, ctx->N.n_deliv, sizeof(n), hipMemcpyDeviceToHost));
+        *synaptic_events = ctx->netwin ? (int64_t)n : -1;
+    }
     return 0;
 }
 
@@ -899,6 +1142,7 @@ void edenhip_destroy(edenhip_ctx *ctx) {
             if (b.p1) (void)hipEventDestroy(b.p1);
         }
         if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        if (ctx->h_conflict) (void)hipHostFree(ctx->h_conflict);
         if (ctx->pp0) (void)hipEventDestroy(ctx->pp0);
         if (ctx->pp1) (void)hipEventDestroy(ctx->pp1);
         (void)hipStreamDestroy(ctx->stream);

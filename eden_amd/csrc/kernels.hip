@@ -65,7 +65,10 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
         float I_syn = 0.0f, G_syn = 0.0f;
         if (has_syn) synapses_step(T, stypes, syn_base, syn_cnt, V, init, t, dt, now, s, I_syn, G_syn);
         float I_in = pulses_current(T, k, g, t), G_in = 0.0f;
-        if (has_poi) poisson_step(T, stypes, g, V, init, s, dt, I_in, G_in);
+        if (has_poi) {
+            const PoiState ps{T.poi_tsince, T.poi_isi, T.poi_A, T.poi_B};
+            poisson_step(T, stypes, g, V, init, s, dt, I_in, G_in, ps, ps);
+        }
         const float I_int = I_ch + I_in + I_syn;
         const float G_int = G_ch + G_in + G_syn;
         float D = 1 * (G_int / k.C) + k.invrc;
@@ -324,7 +327,8 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const uint32_t e = sIL[i];
                         const int c = e & 0x7FF;
                         float Ii = 0.0f, Gi = 0.0f;
-                        poisson_step(T, stypes, c0 + c, sV[c], init, s, dt, Ii, Gi);
+                        const PoiState ps{T.poi_tsince, T.poi_isi, T.poi_A, T.poi_B};
+                        poisson_step(T, stypes, c0 + c, sV[c], init, s, dt, Ii, Gi, ps, ps);
                         sIn[c] = make_float2(Ii, Gi);
                     }
                     lds_barrier<TPB>();

@@ -64,7 +64,33 @@ __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t
     return L;
 }
 
+// Tables of the windowed network path for networks of one-compartment cells. Lane l of
+// wave b advances point cell 64 b + l; its synapses use the block-interleaved slots of
+// DevTables (synapse k at syn_base + 64 k). Ring word (step, b, w, l) =
+// (step & ring_mask) * ring_stride + (b * nw + w) * 64 + l, bit j: synapse 32 w + j of that
+// cell receives its event at that step. State arrays come in two copies: a window reads
+// copy `cur` and writes copy `cur ^ 1`, so a window can be run again.
+struct NetTables {
+    int32_t n_blocks;              // waves of 64 point cells
+    int32_t nw;                    // ring words per cell (ceil(max synapses per cell / 32))
+    int32_t ring_mask;
+    int64_t ring_stride;           // words per ring step = n_blocks * nw * 64
+    const uint32_t *runs;          // [NC * 4] per compartment: type run end | type << 16, 0 = none
+    float *gA[2], *gB[2];          // expTwo state per slot
+    uint32_t *ring;                // [R * ring_stride]
+    float *V[2], *q[2];            // compartment voltage and gate state
+    float *poi_tsince[2], *poi_isi[2], *poi_A[2], *poi_B[2];
+    uint64_t *spike_mask;          // [NC] spikes of the window, bit = step - window start
+    int32_t *spk_list;             // compartments that spiked in the window
+    uint32_t *spk_count;
+    const uint32_t *tgt_rw;        // per spike target (CSR spk_begin): ring word offset << 5 | bit
+    int32_t *conflict;             // set when a window's result differs from step-by-step order
+    unsigned long long *n_deliv;   // synaptic events received (spike-triggered), counted
+};
+
 const void *tree_kernel_ptr(int tpb, bool split);
+const void *net_window_kernel_ptr();
+const void *net_deliver_kernel_ptr();
 const void *point_kernel_ptr();
 
 }  // namespace edenhip

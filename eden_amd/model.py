@@ -203,11 +203,12 @@ class Simulation:
     """One model on one GPU. Rows are recorded steps (state at t = row * dt)."""
 
     def __init__(self, model, threads_per_cell=0, persistent=1, chunk_steps=0, profile=False, device=0,
-                 log_synapse_events=0):
+                 log_synapse_events=0, network_window=0):
         self.lib = N.load_library()
         self.model = model
         self._desc = model.desc()
-        opt = N.Options(threads_per_cell, persistent, chunk_steps, 0, 1 if profile else 0, device, log_synapse_events)
+        opt = N.Options(threads_per_cell, persistent, chunk_steps, 0, 1 if profile else 0, device, log_synapse_events,
+                        network_window)
         h = C.c_void_p()
         rc = self.lib.edenhip_create(C.byref(self._desc), C.byref(opt), C.byref(h))
         self.h = h
@@ -295,6 +296,13 @@ class Simulation:
                             cells=ki.n_cells, comps=ki.n_comps, launches=ki.launches, total_ms=ki.total_ms,
                             cell_index=cells[: ki.n_cells]))
         return out
+
+    def network_stats(self):
+        """(steps per window, windows, windows run again step by step, synaptic events received)."""
+        w, n, r, e = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.edenhip_network_stats(self.h, C.byref(w), C.byref(n), C.byref(r), C.byref(e)),
+                    "network_stats")
+        return w.value, n.value, r.value, e.value
 
     def launch_shape(self):
         t, k = C.c_int32(), C.c_int32()

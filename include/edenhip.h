@@ -165,6 +165,10 @@ typedef struct edenhip_options {
     int32_t profile;              /* 1 = time each launch with HIP events */
     int32_t device;               /* HIP device ordinal */
     int32_t log_synapse_events;   /* > 0: log up to this many synaptic events (see below) */
+    int32_t network_window;       /* networks of one-compartment cells: 0 = advance cells several
+                                     steps per launch (up to the minimum delay) with ring-buffer
+                                     delivery; -1 = one launch per step; -2 = windows, each run
+                                     again step by step (exercises the exact fallback) */
 } edenhip_options;
 
 typedef struct edenhip_ctx edenhip_ctx;
@@ -232,6 +236,12 @@ int32_t edenhip_kernel_count(const edenhip_ctx *ctx);
 int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_info *out);
 /* Descriptor cell indices advanced by kernel i (n_cells of them), in work-queue order. */
 int edenhip_get_kernel_cells(const edenhip_ctx *ctx, int32_t i, int32_t *cells, int32_t max_cells);
+
+/* Network path figures since create or reset: steps per window (1 = one launch per step),
+ * windows run, windows run again step by step (see edenhip_options.network_window), and
+ * spike-triggered synaptic events received (-1 when not counted: per-step path). */
+int edenhip_network_stats(edenhip_ctx *ctx, int32_t *window_steps, int64_t *windows, int64_t *windows_rerun,
+                          int64_t *synaptic_events);
 
 /* Describe the launch shape chosen for the tree-cell kernel (autotuned or given). */
 int edenhip_launch_shape(const edenhip_ctx *ctx, int32_t *threads_per_cell, int32_t *slots_per_thread);

@@ -98,16 +98,26 @@ def test_spiking_network_small():
     assert res["spikes_ref"] > 0
 
 
-def test_synaptic_events_bit_exact():
+@pytest.mark.parametrize("window", [0, -1, -2])
+def test_synaptic_events_bit_exact(window):
     """Integer event path: every (step, synapse instance) at which a synapse receives its
-    event, GPU against the oracle, exactly."""
+    event, GPU against the oracle, exactly; for the windowed ring-buffer path (0), the
+    one-launch-per-step path (-1) and windows all run again step by step (-2)."""
     m = configs.spiking_network(n_cells=400, in_degree=30, t_final_ms=80.0, poisson_hz=150.0,
                                 weight_mu=2.0, n_record=10)
-    _say(f"[parity] synaptic events {m.name} synapses={m.n_synapses}")
-    with Simulation(m, log_synapse_events=5_000_000) as sim:
+    _say(f"[parity] synaptic events {m.name} synapses={m.n_synapses} window mode {window}")
+    with Simulation(m, log_synapse_events=5_000_000, network_window=window) as sim:
         sim.run_all()
         g_step, g_syn = sim.read_synapse_events()
         ev_g = sim.read_events()
+        w, nwin, rerun, deliv = sim.network_stats()
+    _say(f"[parity] window {w} steps, {nwin} windows, {rerun} run again, {deliv} events counted")
+    if window == 0:
+        assert w > 1
+    if window == -2:
+        assert rerun == nwin
+    if window != -1:
+        assert deliv == len(g_step)
     (_, ec, er, _), (o_step, o_syn) = oracle.run_with_synapse_log(m)
     _say(f"[parity] synaptic events: gpu {len(g_step)} oracle {len(o_step)} spikes {len(er)}")
     assert len(o_step) > 100
