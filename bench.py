@@ -233,6 +233,10 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
+    if not args.no_adjoint:
+        # torch's device runtime comes up before the library's own HIP use in this process
+        import torch
+        torch.cuda.init()
 
     from eden_amd.model import Simulation
 
@@ -284,8 +288,14 @@ def main():
     launches_per_step = dom["launches"] / max(1, args.steps)
     bytes_per_launch = algorithmic_bytes(model, rows, dom["cell_index"]) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms * 1e-3)
+    # HBM bytes per launch of that kernel on this workload, from the committed PMC summary
+    import glob
+    from eden_amd.profiling import read_traffic
+    traffic = None
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
+        traffic = traffic or read_traffic(f, dom["name"], args.workload)
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None, kernel=dom["name"],
+                    frac=achieved / HBM_PEAK, traffic=traffic, kernel=dom["name"],
                     kernel_cells=int(dom["cells"]), kernel_comps=int(dom["comps"]),
                     bytes_per_launch=bytes_per_launch, launch_ms=per_launch_ms,
                     all_kernels={k["name"]: dict(ms_per_launch=k["total_ms"] / max(1, k["launches"]),

@@ -1,0 +1,86 @@
+"""Summaries of rocprofv3 output for profiles/: per-kernel durations from a --kernel-trace
+--stats run, and HBM traffic per dispatch from --pmc FETCH_SIZE and WRITE_SIZE runs scaled by
+what the same counters read for a plain device copy of known size.
+
+    python -m eden_amd.profiling stats <run_kernel_stats.csv> <title> <command...>
+    python -m eden_amd.profiling traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
+                                         <copy MiB> <kernel-substring> <workload> <command...>
+
+The traffic summary ends with one line `traffic_bytes_per_launch: <kernel> | <workload> |
+<bytes>` that bench.py reads for its roofline's `traffic`.
+"""
+import csv
+import re
+import sys
+
+
+def _short(name):
+    name = re.sub(r"\(.*$", "", name)
+    return name.replace("void ", "").replace("edenhip::", "").strip()
+
+
+def kernel_stats(path, title, command):
+    rows = list(csv.DictReader(open(path)))
+    lines = [f"# {title}", "", "Command:", "", "```", command, "```", "",
+             "| kernel | calls | total ms | average ms | % |", "|---|---|---|---|---|"]
+    for r in rows:
+        lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                     f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(lines) + "\n"
+
+
+def _per_dispatch(path, counter):
+    out = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        k = (r["Kernel_Name"], r["Dispatch_Id"])
+        out[k] = out.get(k, 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def traffic(fetch, write, cfetch, cwrite, copy_mib, kernel_sub, workload, command):
+    nbytes = copy_mib * 1024.0   # KiB the copy reads, and writes
+    f_copy = [v for (n, _), v in _per_dispatch(cfetch, "FETCH_SIZE").items() if "copyBuffer" in n]
+    w_copy = [v for (n, _), v in _per_dispatch(cwrite, "WRITE_SIZE").items() if "copyBuffer" in n]
+    f_copy = max(f_copy)
+    w_copy = max(w_copy)
+    kf, kw = nbytes / f_copy, nbytes / w_copy
+    fk = {n: v for (n, _), v in _per_dispatch(fetch, "FETCH_SIZE").items() if "edenhip" in n}
+    wk = {n: v for (n, _), v in _per_dispatch(write, "WRITE_SIZE").items() if "edenhip" in n}
+    lines = ["# HBM traffic per dispatch, " + workload, "", "Commands (FETCH_SIZE run, then the same with WRITE_SIZE; the "
+             "calibration copy with each counter):", "", "```", command, "```", "",
+             f"Calibration: a plain {copy_mib} MiB device copy reads FETCH_SIZE {f_copy:.0f} KiB and WRITE_SIZE "
+             f"{w_copy:.0f} KiB for {nbytes:.0f} KiB read and written: factors {kf:.3f} (fetch), {kw:.3f} (write).", "",
+             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes (calibrated) |", "|---|---|---|---|"]
+    chosen = None
+    for n in sorted(fk):
+        t = (fk[n] * kf + wk.get(n, 0.0) * kw) * 1024.0
+        lines.append(f"| {_short(n)} | {fk[n]:.0f} | {wk.get(n, 0.0):.0f} | {t:.4g} |")
+        if kernel_sub in n:
+            chosen = (_short(n), t)
+    lines.append("")
+    if chosen:
+        lines.append(f"traffic_bytes_per_launch: {chosen[0]} | {workload} | {chosen[1]:.6g}")
+    return "\n".join(lines) + "\n"
+
+
+def read_traffic(path, kernel, workload):
+    """Calibrated bytes per launch recorded in a traffic summary, or None."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                k, w, v = [x.strip() for x in line.split(":", 1)[1].split("|")]
+                if k == kernel and w == workload:
+                    return float(v)
+    except OSError:
+        return None
+    return None
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        sys.stdout.write(kernel_stats(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:])))
+    elif sys.argv[1] == "traffic":
+        a = sys.argv[2:]
+        sys.stdout.write(traffic(a[0], a[1], a[2], a[3], int(a[4]), a[5], a[6], " ".join(a[7:])))
