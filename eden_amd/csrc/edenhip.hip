@@ -422,6 +422,13 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             const float uj = -d->dt / (R * d->capacitance[c0 + j]);
             uv[c0 + P.pos[i]] = make_float2(ui, uj);
         }
+        // constant tree below the root: no gated channel and no Poisson input on any other
+        // compartment, so their Hines diagonals are the same every step
+        {
+            bool ok = true;
+            for (int32_t i = 1; i < n && ok; i++) ok = (tcm[c0 + i] & 0xF8u) == 0u;
+            tc.const_tree = ok ? 1 : 0;
+        }
         cells.push_back(tc);
         cell_tpb.push_back(tpb);
         ctx->tree_desc.push_back(c);

@@ -225,6 +225,11 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         const int64_t c0 = cell.comp_begin;
         const int64_t gq0 = cell.gate_off;
         const int RF = cell.rounds_fwd, RB = cell.rounds_bwd;
+        // const_tree: every non-root compartment has a constant diagonal (ungated channels,
+        // no conductance input), so below the root the eliminated diagonals D' and the ratios
+        // Uj / D' are the same every step. They are computed once per task below, with the
+        // per-step operations, and the forward sweep then reads the ratio in place of Uj.
+        const bool const_tree = cell.const_tree != 0;
         {
             const uint32_t *Wg = words_g + cell.words_off;
             for (int i = tid; i < cell.words_len; i += NT) sW[i] = Wg[i];
@@ -301,6 +306,57 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             slot_simple[j] = __ballot(valid && (fl[j] & TCX_COMPLEX)) == 0ull;
         }
         __syncthreads();
+        if (const_tree) {
+            // eliminated diagonals of the constant subtrees: the forward sweep's diagonal part
+            // on this step-independent matrix, with the per-step operations
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int c = tid + TPB * j;
+                if (cable && c < N) {
+                    float Dr = Ds[j];
+                    if (fl[j] & TCX_COMPLEX) {
+                        const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
+                        const CompConst k = cx.k;
+                        float I_ch, G_ch;
+                        tree_channels(k, cx.tcm, sFb, V[j], I_ch, G_ch);
+                        const float G_in = 0.0f, G_syn = 0.0f;
+                        const float G_int = G_ch + G_in + G_syn;
+                        const float Dd = 1 * (G_int / k.C) + k.invrc;
+                        Dr = 1 + Dd * dt;
+                    }
+                    sDX[pos[j]] = make_float2(Dr, 0.0f);
+                }
+            }
+            __syncthreads();
+            for (int r = 0; r < RF; r++) {
+                const uint32_t w = task_word(r, idle_fwd);
+                const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
+                const int c1 = nc >= 2 ? fc + 1 : N;
+                const float dx = sDX[p].x;
+                const float d0 = sDX[fc].x, d1 = sDX[c1].x;
+                const float2 u0 = sU[fc], u1 = sU[c1];
+                float o = (dx - (u0.y / d0) * u0.x) - (u1.y / d1) * u1.x;
+                for (int i = 2; i < nc; i++) {
+                    const float2 u = sU[fc + i];
+                    o -= (u.y / sDX[fc + i].x) * u.x;
+                }
+                __syncthreads();
+                if (p != N) sDX[p].x = o;
+                __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int c = tid + TPB * j;
+                if (cable && c < N && pos[j] != 0) Ds[j] = sDX[pos[j]].x;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int c = tid + TPB * j;
+                if (cable && c < N && pos[j] != 0) sU[pos[j]].y = sU[pos[j]].y / Ds[j];
+            }
+        }
+        __syncthreads();
         // open fractions of the gated channels from the present gate values
         for (int i = tid; i < NCT; i += NT) {
             const uint32_t w = sCT[i];
@@ -362,7 +418,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const float I_int = I_ch + I_in + I_syn;
                         const float G_int = G_ch + G_in + G_syn;
                         const float Dd = 1 * (G_int / k.C) + k.invrc;
-                        Dn = 1 + Dd * dt;
+                        Dn = (const_tree && pos[j] != 0) ? Ds[j] : 1 + Dd * dt;
                         Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int)))) / k.C;
                     }
                     sDX[pos[j]] = make_float2(Dn, Vn);
@@ -394,15 +450,41 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         if (wbase < (int)(__builtin_amdgcn_readfirstlane(hc) >> 16)) {
                             const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
                             const int c1 = nc >= 2 ? fc + 1 : N;
-                            const float2 dx = sDX[p];
-                            const float2 d0 = sDX[fc], u0 = sU[fc];
-                            const float2 d1 = sDX[c1], u1 = sU[c1];
-                            const float r0 = u0.y / d0.x;
-                            const float r1 = u1.y / d1.x;
-                            float2 o;
-                            o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
-                            o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
-                            if (nc > 2) {
+                            if (const_tree) {
+                                // sU[c].y holds the child's constant ratio Uj / D', and sDX[c].x
+                                // its constant eliminated diagonal: only right-hand sides change
+                                const float r0 = sU[fc].y, r1 = sU[c1].y;
+                                float oy = (sDX[p].y - r0 * sDX[fc].y) - r1 * sDX[c1].y;
+                                int i = 2;
+                                for (; i + 3 < nc; i += 4) {
+                                    // four children's loads issued together, then folded in order
+                                    float rr[4], xx[4];
+#pragma unroll
+                                    for (int k = 0; k < 4; k++) {
+                                        rr[k] = sU[fc + i + k].y;
+                                        xx[k] = sDX[fc + i + k].y;
+                                    }
+#pragma unroll
+                                    for (int k = 0; k < 4; k++) oy -= rr[k] * xx[k];
+                                }
+                                for (; i < nc; i++) oy -= sU[fc + i].y * sDX[fc + i].y;
+                                if (p == 0) {
+                                    // the root's diagonal changes every step
+                                    float ox = (sDX[0].x - r0 * sU[fc].x) - r1 * sU[c1].x;
+                                    for (int k = 2; k < nc; k++) ox -= sU[fc + k].y * sU[fc + k].x;
+                                    sDX[0] = make_float2(ox, oy);
+                                } else {
+                                    sDX[p].y = oy;
+                                }
+                            } else {
+                                const float2 dx = sDX[p];
+                                const float2 d0 = sDX[fc], u0 = sU[fc];
+                                const float2 d1 = sDX[c1], u1 = sU[c1];
+                                const float r0 = u0.y / d0.x;
+                                const float r1 = u1.y / d1.x;
+                                float2 o;
+                                o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
+                                o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
                                 for (int i = 2; i < nc; i++) {
                                     const float2 dxc = sDX[fc + i];
                                     const float2 u = sU[fc + i];
@@ -410,8 +492,8 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                                     o.x -= ratio * u.x;
                                     o.y -= ratio * dxc.y;
                                 }
+                                sDX[p] = o;
                             }
-                            sDX[p] = o;
                         }
                         lds_barrier<TPB>();
                         w = wn;

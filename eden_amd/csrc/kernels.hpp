@@ -21,6 +21,8 @@ struct TreeCell {
     int32_t ctask_off;    // offset of its channel tasks
     int32_t n_cx;         // complex compartments (see TreeComplex)
     int32_t cx_off;       // offset of their records
+    int32_t const_tree;   // 1: constant diagonals below the root (see tree_kernel)
+    int32_t pad;
 };
 
 // Per-compartment flag word of tree cells: 0 for a simple compartment (one ungated channel,
