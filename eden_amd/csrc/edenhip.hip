@@ -339,7 +339,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         }
         if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2047 compartments");
         const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell);
-        TreePlan P = plan_tree(n, d->parent + c0, tpb);
+        const WavePlan P = plan_tree_waves(n, d->parent + c0, tpb / 64);
         TreeCell tc{};
         tc.gate_off = (int64_t)gqv.size();
         tc.ctask_off = (int32_t)ctask.size();
@@ -407,8 +407,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
 
         tc.comp_begin = c0;
         tc.n = n;
-        tc.rounds_fwd = P.rounds_fwd;
-        tc.rounds_bwd = P.rounds_bwd;
+        tc.longest = P.longest;
+        tc.n_junction = P.n_junction;
         tc.words_off = (int32_t)words.size();
         tc.words_len = (int32_t)P.block.size();
         words.insert(words.end(), P.block.begin(), P.block.end());
@@ -599,7 +599,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // fractions and tasks,
         // input list, and the task words for workgroups of up to 128 threads
         b.nctmax = std::max(b.nctmax, 1);   // ungated slots read sF[first task], kept in bounds
-        b.split = b.tpb == 64 && ctx->opt.persistent >= 0;
+        // a separate channel wave (gates overlapping the cable step) pays off for one-wave
+        // and 8-wave workgroups; for 2 and 4 cable waves the extra wave costs a workgroup
+        // per CU, which those latency-bound sizes need more
+        b.split = (b.tpb == 64 || b.tpb == 512) && ctx->opt.persistent >= 0;
         // complex-compartment records go to LDS when they take at most 20 KiB
         b.cxmax_lds = b.cxmax <= 256 ? b.cxmax : 0;
         b.lds = (size_t)tree_lds(b.nmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, b.split).total *

@@ -224,7 +224,6 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         const int32_t NIN = cell.n_input;
         const int64_t c0 = cell.comp_begin;
         const int64_t gq0 = cell.gate_off;
-        const int RF = cell.rounds_fwd, RB = cell.rounds_bwd;
         // const_tree: every non-root compartment has a constant diagonal (ungated channels,
         // no conductance input), so below the root the eliminated diagonals D' and the ratios
         // Uj / D' are the same every step. They are computed once per task below, with the
@@ -234,15 +233,35 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             const uint32_t *Wg = words_g + cell.words_off;
             for (int i = tid; i < cell.words_len; i += NT) sW[i] = Wg[i];
         }
-        // task word of this thread in round r: its word when the round has one for it, else
-        // the idle word (dummy slot N)
+        // Hines task lists (tree_schedule.hpp, plan_tree_waves): cable wave wv runs the lists
+        // of its subtrees, wave 0 also those of the junction. run_list calls body(word) round
+        // after round with no barrier (a wave's LDS accesses are performed in order); lanes at
+        // or beyond a round's task count take the idle word, which points every access at the
+        // dummy slot N, so a round is straight-line code. The loop is unrolled twice, so that
+        // no register moves between rounds: after a round's body is issued, the words of the
+        // round after next and the header of the round two after that are read into the
+        // registers the round had used (five sentinel headers end the block). Each value is
+        // then complete before its use, with no wait on the loads just issued.
+        constexpr int W = TPB / 64;
+        const int wv = tid >> 6, lane = tid & 63;
         const uint32_t idle_fwd = (uint32_t)N | ((uint32_t)N << 11) | (1u << 22);
         const uint32_t idle_bwd = (uint32_t)N | ((uint32_t)N << 11);
-        auto task_word = [&](int r, uint32_t idle) -> uint32_t {
-            const uint32_t h = sW[r];
-            uint32_t w = idle;
-            if ((uint32_t)tid < (h >> 16)) w = sW[(h & 0xFFFFu) + tid];
-            return w;
+        auto run_list = [&](int dir, uint32_t idle, auto body) {
+            const uint32_t list = __builtin_amdgcn_readfirstlane(sW[dir]);
+            const int first = list & 0xFFFFu, cnt = list >> 16;
+            auto word = [&](uint32_t h) { return lane < (int)(h >> 16) ? sW[(h & 0xFFFFu) + lane] : idle; };
+            uint32_t ha = sW[first + 2], hb = sW[first + 3];
+            uint32_t wa = word(sW[first]), wb = word(sW[first + 1]);
+            for (int r = 0; r < cnt; r += 2) {
+                body(wa);
+                wa = word(ha);
+                ha = sW[first + r + 4];
+                if (r + 1 < cnt) {
+                    body(wb);
+                    wb = word(hb);
+                    hb = sW[first + r + 5];
+                }
+            }
         };
         for (int i = tid; i < N; i += NT) sU[i] = u_g[c0 + i];
         for (int i = tid; i < NG; i += NT) {
@@ -328,8 +347,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 }
             }
             __syncthreads();
-            for (int r = 0; r < RF; r++) {
-                const uint32_t w = task_word(r, idle_fwd);
+            auto diag_task = [&](uint32_t w) {
                 const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
                 const int c1 = nc >= 2 ? fc + 1 : N;
                 const float dx = sDX[p].x;
@@ -340,21 +358,25 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     const float2 u = sU[fc + i];
                     o -= (u.y / sDX[fc + i].x) * u.x;
                 }
-                __syncthreads();
-                if (p != N) sDX[p].x = o;
-                __syncthreads();
-            }
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int c = tid + TPB * j;
-                if (cable && c < N && pos[j] != 0) Ds[j] = sDX[pos[j]].x;
-            }
+                sDX[p].x = o;
+            };
+            if (cable) run_list(2 * wv, idle_fwd, diag_task);
             __syncthreads();
+            if (cable && wv == 0) run_list(2 * W, idle_fwd, diag_task);
+            __syncthreads();
+            // from here on, below the root: sDX[c] = (ratio Uj / D', right-hand side) and
+            // sU[c] = (Ui, D'); the per-step diagonal register Ds holds the ratio
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 const int c = tid + TPB * j;
-                if (cable && c < N && pos[j] != 0) sU[pos[j]].y = sU[pos[j]].y / Ds[j];
+                if (cable && c < N && pos[j] != 0) {
+                    const float Dp = sDX[pos[j]].x;
+                    Ds[j] = sU[pos[j]].y / Dp;
+                    sU[pos[j]].y = Dp;
+                }
             }
+            // dummy slot: ratio 1 (times right-hand side 0), D' = 1 for the idle backward task
+            if (tid == 0) sU[N] = make_float2(0.0f, 1.0f);
         }
         __syncthreads();
         // open fractions of the gated channels from the present gate values
@@ -427,113 +449,86 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 if constexpr (!SPLIT)
                     channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, tid, TPB, init, dt, [] { lds_barrier<TPB>(); });
                 if (!init) {
-                    // forward elimination, rounds of independent gathers. Straight-line code:
-                    // threads without a task work on the dummy slot N (sDX[N] = (1, 0) and
-                    // sU[N] = (0, 0) stay fixed), and a node with one child takes the dummy
-                    // slot as its second child, whose contribution r1 * 0 is exactly zero.
-                    // Nodes with more than two children fold the rest in a loop. The next
-                    // round's word is always read (the schedule ends with an idle round).
-                    // Task words are prefetched two rounds ahead and round headers three: a
-                    // round's LDS reads of its own operands are issued first, the prefetch
-                    // reads after them, and nothing in the round waits on the prefetch.
-                    // Waves without a task in a round skip its arithmetic (the round's task
-                    // count is uniform, read from its header).
-                    uint32_t w = task_word(0, idle_fwd);
-                    uint32_t wn = task_word(1, idle_fwd);
-                    uint32_t hc = sW[0], hn = sW[1], h2 = sW[2];
-                    for (int r = 0; r < RF; r++) {
-                        // headers r + 2 and r + 3 exist while r < RF: the forward sweep ends with
-                        // an empty round and at least one backward round and an empty one follow
-                        const uint32_t h3 = sW[r + 3];
-                        uint32_t wnn = idle_fwd;
-                        if ((uint32_t)tid < (h2 >> 16)) wnn = sW[(h2 & 0xFFFFu) + tid];
-                        if (wbase < (int)(__builtin_amdgcn_readfirstlane(hc) >> 16)) {
-                            const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
-                            const int c1 = nc >= 2 ? fc + 1 : N;
-                            if (const_tree) {
-                                // sU[c].y holds the child's constant ratio Uj / D', and sDX[c].x
-                                // its constant eliminated diagonal: only right-hand sides change
-                                const float r0 = sU[fc].y, r1 = sU[c1].y;
-                                float oy = (sDX[p].y - r0 * sDX[fc].y) - r1 * sDX[c1].y;
-                                int i = 2;
-                                for (; i + 3 < nc; i += 4) {
-                                    // four children's loads issued together, then folded in order
-                                    float rr[4], xx[4];
+                    // forward elimination: each node folds its children in ascending order
+                    // (one child: the dummy slot N stands in as the second; its right-hand
+                    // side and Ui are 0, so its terms are exactly zero)
+                    auto fwd_task = [&](uint32_t w) {
+                        const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
+                        const int c1 = nc >= 2 ? fc + 1 : N;
+                        if (const_tree) {
+                            // below the root sDX[c] = (constant ratio Uj / D', rhs): only
+                            // right-hand sides change, one 8-byte read per child
+                            const float2 a0 = sDX[fc], a1 = sDX[c1];
+                            const float x = sDX[p].y;
+                            float oy = (x - a0.x * a0.y) - a1.x * a1.y;
+                            int i = 2;
+                            for (; i + 3 < nc; i += 4) {
+                                // four children's loads issued together, then folded in order
+                                float2 a[4];
 #pragma unroll
-                                    for (int k = 0; k < 4; k++) {
-                                        rr[k] = sU[fc + i + k].y;
-                                        xx[k] = sDX[fc + i + k].y;
-                                    }
+                                for (int k = 0; k < 4; k++) a[k] = sDX[fc + i + k];
 #pragma unroll
-                                    for (int k = 0; k < 4; k++) oy -= rr[k] * xx[k];
-                                }
-                                for (; i < nc; i++) oy -= sU[fc + i].y * sDX[fc + i].y;
-                                if (p == 0) {
-                                    // the root's diagonal changes every step
-                                    float ox = (sDX[0].x - r0 * sU[fc].x) - r1 * sU[c1].x;
-                                    for (int k = 2; k < nc; k++) ox -= sU[fc + k].y * sU[fc + k].x;
-                                    sDX[0] = make_float2(ox, oy);
-                                } else {
-                                    sDX[p].y = oy;
-                                }
-                            } else {
-                                const float2 dx = sDX[p];
-                                const float2 d0 = sDX[fc], u0 = sU[fc];
-                                const float2 d1 = sDX[c1], u1 = sU[c1];
-                                const float r0 = u0.y / d0.x;
-                                const float r1 = u1.y / d1.x;
-                                float2 o;
-                                o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
-                                o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
-                                for (int i = 2; i < nc; i++) {
-                                    const float2 dxc = sDX[fc + i];
-                                    const float2 u = sU[fc + i];
-                                    const float ratio = u.y / dxc.x;
-                                    o.x -= ratio * u.x;
-                                    o.y -= ratio * dxc.y;
-                                }
-                                sDX[p] = o;
+                                for (int k = 0; k < 4; k++) oy -= a[k].x * a[k].y;
                             }
+                            for (; i < nc; i++) {
+                                const float2 a = sDX[fc + i];
+                                oy -= a.x * a.y;
+                            }
+                            if (p == 0) {
+                                // the root's diagonal changes every step
+                                float ox = (sDX[0].x - a0.x * sU[fc].x) - a1.x * sU[c1].x;
+                                for (int k = 2; k < nc; k++) ox -= sDX[fc + k].x * sU[fc + k].x;
+                                sDX[0] = make_float2(ox, oy);
+                            } else {
+                                sDX[p].y = oy;
+                            }
+                        } else {
+                            const float2 dx = sDX[p];
+                            const float2 d0 = sDX[fc], u0 = sU[fc];
+                            const float2 d1 = sDX[c1], u1 = sU[c1];
+                            const float r0 = u0.y / d0.x;
+                            const float r1 = u1.y / d1.x;
+                            float2 o;
+                            o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
+                            o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
+                            for (int i = 2; i < nc; i++) {
+                                const float2 dxc = sDX[fc + i];
+                                const float2 u = sU[fc + i];
+                                const float ratio = u.y / dxc.x;
+                                o.x -= ratio * u.x;
+                                o.y -= ratio * dxc.y;
+                            }
+                            sDX[p] = o;
                         }
-                        lds_barrier<TPB>();
-                        w = wn;
-                        wn = wnn;
-                        hc = hn;
-                        hn = h2;
-                        h2 = h3;
-                    }
-                    if (tid == 0) {
-                        float2 r0 = sDX[0];
-                        r0.y = r0.y / r0.x;
-                        sDX[0] = r0;
+                    };
+                    // back-substitution of one non-root node from its solved parent
+                    auto bwd_task = [&](uint32_t w) {
+                        const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
+                        const float xp = sDX[pp].y;
+                        if (const_tree) {
+                            // sU[p] = (Ui, D')
+                            const float2 u = sU[p];
+                            sDX[p].y = (sDX[p].y - u.x * xp) / u.y;
+                        } else {
+                            const float2 dx = sDX[p];
+                            const float ui = sU[p].x;
+                            sDX[p].y = (dx.y - ui * xp) / dx.x;
+                        }
+                    };
+                    run_list(2 * wv, idle_fwd, fwd_task);
+                    lds_barrier<TPB>();
+                    if (wv == 0) {
+                        run_list(2 * W, idle_fwd, fwd_task);
+                        if (lane == 0) {
+                            float2 r0 = sDX[0];
+                            r0.y = r0.y / r0.x;
+                            sDX[0] = r0;
+                        }
+                        run_list(2 * W + 1, idle_bwd, bwd_task);
                     }
                     lds_barrier<TPB>();
-                    // back-substitution, rounds by depth, straight-line as above
-                    const int rb0 = RF + 1, rlast = RF + RB + 1;
-                    w = task_word(rb0, idle_bwd);
-                    wn = task_word(min(rb0 + 1, rlast), idle_bwd);
-                    hc = sW[rb0];
-                    hn = sW[min(rb0 + 1, rlast)];
-                    h2 = sW[min(rb0 + 2, rlast)];
-                    for (int r = 0; r < RB; r++) {
-                        const uint32_t h3 = sW[min(rb0 + r + 3, rlast)];
-                        uint32_t wnn = idle_bwd;
-                        if ((uint32_t)tid < (h2 >> 16)) wnn = sW[(h2 & 0xFFFFu) + tid];
-                        if (wbase < (int)(__builtin_amdgcn_readfirstlane(hc) >> 16)) {
-                            const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
-                            float2 dx = sDX[p];
-                            const float ui = sU[p].x;
-                            const float xp = sDX[pp].y;
-                            dx.y = (dx.y - ui * xp) / dx.x;
-                            sDX[p] = dx;
-                        }
-                        lds_barrier<TPB>();
-                        w = wn;
-                        wn = wnn;
-                        hc = hn;
-                        hn = h2;
-                        h2 = h3;
-                    }
+                    run_list(2 * wv + 1, idle_bwd, bwd_task);
+                    lds_barrier<TPB>();
                 }
 #pragma unroll
                 for (int j = 0; j < K; j++) {
@@ -552,7 +547,22 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     }
                 }
             } else {
+                // channel wave: next open fractions from this step's voltages, overlapping the
+                // cable group's step. With several cable waves it takes part in their barriers
+                // (the input and compartment barriers first, then, after its own work, the
+                // three of the Hines solve), so that every wave passes the same barriers.
+                if constexpr (TPB > 64) {
+                    if (NIN > 0) lds_barrier<TPB>();
+                    lds_barrier<TPB>();
+                }
                 channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, tid - TPB, 64, init, dt, [] { lds_barrier<64>(); });
+                if constexpr (TPB > 64) {
+                    if (!init) {
+                        lds_barrier<TPB>();
+                        lds_barrier<TPB>();
+                        lds_barrier<TPB>();
+                    }
+                }
             }
             // the next step reads voltages and open fractions written above by other lanes
             if constexpr (SPLIT) {
@@ -590,10 +600,17 @@ EDENHIP_TREE_INST(64, true)
 EDENHIP_TREE_INST(64, false)
 EDENHIP_TREE_INST(128, false)
 EDENHIP_TREE_INST(256, false)
+EDENHIP_TREE_INST(512, true)
 EDENHIP_TREE_INST(512, false)
 
 const void *tree_kernel_ptr(int tpb, bool split) {
-    if (split) return tpb == 64 ? reinterpret_cast<const void *>(&tree_kernel<64, true>) : nullptr;
+    if (split) {
+        switch (tpb) {
+        case 64: return reinterpret_cast<const void *>(&tree_kernel<64, true>);
+        case 512: return reinterpret_cast<const void *>(&tree_kernel<512, true>);
+        default: return nullptr;
+        }
+    }
     switch (tpb) {
     case 64: return reinterpret_cast<const void *>(&tree_kernel<64, false>);
     case 128: return reinterpret_cast<const void *>(&tree_kernel<128, false>);

@@ -9,9 +9,9 @@ namespace edenhip {
 struct TreeCell {
     int64_t comp_begin;   // first global compartment
     int32_t n;            // compartments
-    int32_t rounds_fwd;   // forward-elimination rounds
-    int32_t rounds_bwd;   // back-substitution rounds
-    int32_t words_off;    // offset of its schedule block (round headers, then task words)
+    int32_t longest;      // Hines rounds on the schedule's longest path (information)
+    int32_t n_junction;   // nodes in the schedule's junction (information)
+    int32_t words_off;    // offset of its schedule block (plan_tree_waves)
     int32_t words_len;    // length of that block in 32-bit words
     int32_t n_gate;       // gate variables (one per gate of every channel distribution)
     int64_t gate_off;     // offset of its gate variables

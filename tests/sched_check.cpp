@@ -1,4 +1,5 @@
-// Host check of the tree kernel's Hines schedule (eden_amd/csrc/tree_schedule.hpp).
+// Host check of the Hines schedules (eden_amd/csrc/tree_schedule.hpp): the wave-partitioned
+// schedule of the tree kernel and the round schedule of the adjoint kernels.
 //
 // For random trees of several shapes and every workgroup size, the schedule block is decoded
 // exactly as tree_kernel decodes it (round headers, task words, idle words aimed at the dummy
@@ -44,6 +45,125 @@ static std::vector<int32_t> make_parents(std::mt19937 &rng, int n, int shape) {
     return par;
 }
 
+// The wave-partitioned schedule (plan_tree_waves) decoded as tree_kernel decodes it:
+// directory, per-wave lists of rounds (1..64 tasks), the junction lists run by wave 0 between
+// the two workgroup barriers. Within a round all tasks read before any writes (one SIMD
+// instruction stream); rounds of a wave follow in order; waves are replayed one after the
+// other in reverse order, and a read of a node that another wave writes in the same phase is
+// an error (so no order of the waves could change the result).
+static int check_waves(int n, const std::vector<int32_t> &par, const std::vector<std::vector<int>> &ch,
+                       const std::vector<float> &D0, const std::vector<float> &X0, const std::vector<float> &ui,
+                       const std::vector<float> &uj, const std::vector<float> &Dr, const std::vector<float> &Xr,
+                       int waves, int trial, int shape) {
+    const WavePlan P = plan_tree_waves(n, par.data(), waves);
+    const std::vector<uint32_t> &B = P.block;
+    int bad = 0;
+    std::vector<float> dx(n + 1), dv(n + 1), su(n + 1), sj(n + 1);
+    for (int i = 0; i < n; i++) {
+        dx[P.pos[i]] = D0[i];
+        dv[P.pos[i]] = X0[i];
+        su[P.pos[i]] = i ? ui[i] : 0.0f;
+        sj[P.pos[i]] = i ? uj[i] : 0.0f;
+    }
+    dx[n] = 1.0f; dv[n] = 0.0f; su[n] = 0.0f; sj[n] = 0.0f;
+    std::vector<int> folded(n + 1, 0), solved(n + 1, 0), writer(n + 1, -1);
+    const int nd = 2 * waves + 2;
+    // one list: returns false on a malformed block
+    auto run = [&](int dir, int wave, bool fwd, int phase_owner_check) -> bool {
+        if (dir >= (int)B.size()) return false;
+        const int first = B[dir] & 0xFFFF, cnt = B[dir] >> 16;
+        if (cnt > 0 && (first < nd || first + cnt >= (int)B.size())) return false;
+        for (int r = 0; r < cnt; r++) {
+            const uint32_t h = B[first + r];
+            const int off = h & 0xFFFF, tc = h >> 16;
+            if (tc < 1 || tc > 64 || off + tc > (int)B.size()) return false;
+            std::vector<std::pair<int, std::pair<float, float>>> out;
+            for (int t = 0; t < tc; t++) {
+                const uint32_t w = B[off + t];
+                auto touch = [&](int x) {
+                    if (phase_owner_check && writer[x] >= 0 && writer[x] != wave) bad++;
+                };
+                if (fwd) {
+                    const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = (int)(w >> 22);
+                    if (p >= n || nc < 1 || fc + nc > n) return false;
+                    const int c1 = nc >= 2 ? fc + 1 : n;
+                    touch(p);
+                    for (int i = 0; i < nc; i++) touch(fc + i);
+                    const float r0 = sj[fc] / dx[fc];
+                    const float r1 = sj[c1] / dx[c1];
+                    float ox = (dx[p] - r0 * su[fc]) - r1 * su[c1];
+                    float oy = (dv[p] - r0 * dv[fc]) - r1 * dv[c1];
+                    for (int i = 2; i < nc; i++) {
+                        const float ratio = sj[fc + i] / dx[fc + i];
+                        ox -= ratio * su[fc + i];
+                        oy -= ratio * dv[fc + i];
+                    }
+                    folded[p]++;
+                    out.push_back({p, {ox, oy}});
+                } else {
+                    const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
+                    if (p >= n || pp >= n) return false;
+                    touch(p);
+                    touch(pp);
+                    if (solved[pp] == 0 && pp != 0) bad++;   // parent not solved yet
+                    solved[p]++;
+                    out.push_back({p, {dx[p], (dv[p] - su[p] * dv[pp]) / dx[p]}});
+                }
+            }
+            for (auto &o : out) {
+                dx[o.first] = o.second.first;
+                dv[o.first] = o.second.second;
+            }
+        }
+        return true;
+    };
+    bool ok = (int)B.size() >= nd + 1;
+    // the nodes each wave writes in a phase, from its list (before any replay)
+    auto owners = [&](int odd) {
+        std::fill(writer.begin(), writer.end(), -1);
+        for (int w = 0; w < waves && ok; w++) {
+            const int first = B[2 * w + odd] & 0xFFFF, cnt = B[2 * w + odd] >> 16;
+            for (int r = 0; r < cnt && ok; r++) {
+                const uint32_t h = B[first + r];
+                for (int t = 0; t < (int)(h >> 16); t++) {
+                    const int p = B[(h & 0xFFFF) + t] & 0x7FF;
+                    if (p > n || (writer[p] >= 0 && writer[p] != w)) bad++;
+                    writer[p] = w;
+                }
+            }
+        }
+    };
+    // phase 1: every wave's forward list
+    owners(0);
+    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w, w, true, 1);
+    // wave 0: junction forward, root, junction backward
+    std::fill(writer.begin(), writer.end(), -1);
+    if (ok) ok = run(2 * waves, 0, true, 0);
+    dv[0] = dv[0] / dx[0];
+    solved[0] = 1;
+    if (ok) ok = run(2 * waves + 1, 0, false, 0);
+    // phase 3: every wave's backward list
+    if (ok) owners(1);
+    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w + 1, w, false, 1);
+    if (!ok) {
+        std::printf("trial %d shape %d n %d waves %d: malformed block\n", trial, shape, n, waves);
+        return bad + 1;
+    }
+    for (int i = 0; i < n; i++) {
+        const int p = P.pos[i];
+        if (folded[p] != (ch[i].empty() ? 0 : 1)) bad++;
+        if (solved[p] != 1) bad++;
+        if (std::memcmp(&dv[p], &Xr[i], 4) != 0 || std::memcmp(&dx[p], &Dr[i], 4) != 0) {
+            std::printf("trial %d shape %d n %d waves %d: node %d gives %.9g, serial %.9g\n", trial, shape, n, waves,
+                        i, dv[p], Xr[i]);
+            bad++;
+            break;
+        }
+    }
+    if (waves == 1 && P.n_junction != 0) bad++;
+    return bad;
+}
+
 int main(int argc, char **argv) {
     const int trials = argc > 1 ? std::atoi(argv[1]) : 200;
     std::mt19937 rng(12345);
@@ -86,6 +206,10 @@ int main(int argc, char **argv) {
         for (int k = n - 2; k >= 0; k--) {
             const int i = order[k], j = par[i];
             Xr[i] = (Xr[i] - ui[i] * Xr[j]) / Dr[i];
+        }
+        for (int waves : {1, 2, 4, 8}) {
+            runs++;
+            bad += check_waves(n, par, ch, D0, X0, ui, uj, Dr, Xr, waves, trial, shape);
         }
         for (int tpb : {64, 128, 256, 512}) {
             runs++;

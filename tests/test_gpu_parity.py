@@ -40,6 +40,9 @@ def _check(model, network=False, **kw):
     res = compare(tr_g, ev_g, tr_r, (ec, er), len(model.event_comp), rec2ev, network=network)
     _say(f"[parity] {model.name}: {res}")
     assert res["max_abs_mV"] <= TOL_MV, res
+    # stronger than the tolerance: the HIP path performs the oracle's float operations in the
+    # same order (exp/log rounded once from double on both sides), so traces are identical
+    assert res["max_abs_mV"] == 0.0, res
     return res, (tr_g, ev_g), (tr_r, (ec, er))
 
 
@@ -75,7 +78,16 @@ def test_hh_everywhere_trees():
     assert res["raster_mismatch_cols"] == 0, res
 
 
-@pytest.mark.parametrize("tpb", [64, 128, 256])
+def test_hh_everywhere_large_trees():
+    """Gated channels on every compartment (no constant subtrees) in 2-, 4- and 8-wave
+    workgroups, the last with its separate channel wave."""
+    sizes = [300, 700, 1100]
+    res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=15.0, seed=5003,
+                                            hh_everywhere=True, record_all=True))
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+@pytest.mark.parametrize("tpb", [64, 128, 256, 512])
 def test_launch_shapes_agree(tpb):
     sizes = [60, 120, 250]
     m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=20.0, seed=17)

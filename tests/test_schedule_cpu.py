@@ -17,4 +17,4 @@ def test_schedule_replay_matches_serial_elimination(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I", ROOT,
                     os.path.join(ROOT, "tests", "sched_check.cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe), "200"], check=True, capture_output=True, text=True).stdout
-    assert out.strip() == "ok 800", out
+    assert out.strip() == "ok 1600", out
