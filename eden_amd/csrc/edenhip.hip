@@ -409,6 +409,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         tc.n = n;
         tc.longest = P.longest;
         tc.n_junction = P.n_junction;
+        for (int32_t i = 1; i < n; i++) tc.root_children += d->parent[c0 + i] == 0 ? 1 : 0;
         tc.words_off = (int32_t)words.size();
         tc.words_len = (int32_t)P.block.size();
         words.insert(words.end(), P.block.begin(), P.block.end());

@@ -237,21 +237,31 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         // of its subtrees, wave 0 also those of the junction. run_list calls body(word) round
         // after round with no barrier (a wave's LDS accesses are performed in order); lanes at
         // or beyond a round's task count take the idle word, which points every access at the
-        // dummy slot N, so a round is straight-line code. The loop is unrolled twice, so that
-        // no register moves between rounds: after a round's body is issued, the words of the
-        // round after next and the header of the round two after that are read into the
-        // registers the round had used (five sentinel headers end the block). Each value is
-        // then complete before its use, with no wait on the loads just issued.
+        // dummy slot N, so a round is straight-line code. Task words hold byte offsets into
+        // sDX, which starts at a constant LDS address. Every lane reads a word from the round
+        // (the block ends with padding for the lanes beyond a round's count). The loop is
+        // unrolled twice, so that no register moves between rounds: after a round's body is
+        // issued, the words of the round after next and the header of the round two after
+        // that are read into the registers the round had used (five sentinel headers end the
+        // block). Each value is then complete before its use, with no wait on the loads just
+        // issued.
         constexpr int W = TPB / 64;
         const int wv = tid >> 6, lane = tid & 63;
-        const uint32_t idle_fwd = (uint32_t)N | ((uint32_t)N << 11) | (1u << 22);
-        const uint32_t idle_bwd = (uint32_t)N | ((uint32_t)N << 11);
-        auto run_list = [&](int dir, uint32_t idle, auto body) {
+        const uint32_t dummy_off = 8u * (uint32_t)N;
+        const uint2 idle_fold = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
+        const uint32_t idle_bwd = dummy_off | (dummy_off << 16);
+        char *const dxb = reinterpret_cast<char *>(sDX);
+        char *const ub = reinterpret_cast<char *>(sU);
+        auto run_list = [&](int dir, auto idle, auto body) {
+            using Word = decltype(idle);
             const uint32_t list = __builtin_amdgcn_readfirstlane(sW[dir]);
             const int first = list & 0xFFFFu, cnt = list >> 16;
-            auto word = [&](uint32_t h) { return lane < (int)(h >> 16) ? sW[(h & 0xFFFFu) + lane] : idle; };
+            auto word = [&](uint32_t h) {
+                const Word w = reinterpret_cast<const Word *>(sW + (h & 0xFFFFu))[lane];
+                return lane < (int)(h >> 16) ? w : idle;
+            };
             uint32_t ha = sW[first + 2], hb = sW[first + 3];
-            uint32_t wa = word(sW[first]), wb = word(sW[first + 1]);
+            Word wa = word(sW[first]), wb = word(sW[first + 1]);
             for (int r = 0; r < cnt; r += 2) {
                 body(wa);
                 wa = word(ha);
@@ -347,22 +357,18 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 }
             }
             __syncthreads();
-            auto diag_task = [&](uint32_t w) {
-                const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
-                const int c1 = nc >= 2 ? fc + 1 : N;
-                const float dx = sDX[p].x;
-                const float d0 = sDX[fc].x, d1 = sDX[c1].x;
-                const float2 u0 = sU[fc], u1 = sU[c1];
-                float o = (dx - (u0.y / d0) * u0.x) - (u1.y / d1) * u1.x;
-                for (int i = 2; i < nc; i++) {
-                    const float2 u = sU[fc + i];
-                    o -= (u.y / sDX[fc + i].x) * u.x;
-                }
-                sDX[p].x = o;
+            auto diag_task = [&](uint2 w) {
+                const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y;
+                const float d = reinterpret_cast<const float2 *>(dxb + po)->x;
+                const float da = reinterpret_cast<const float2 *>(dxb + ao)->x;
+                const float db = reinterpret_cast<const float2 *>(dxb + bo)->x;
+                const float2 ua = *reinterpret_cast<const float2 *>(ub + ao);
+                const float2 ubb = *reinterpret_cast<const float2 *>(ub + bo);
+                reinterpret_cast<float2 *>(dxb + po)->x = (d - (ua.y / da) * ua.x) - (ubb.y / db) * ubb.x;
             };
-            if (cable) run_list(2 * wv, idle_fwd, diag_task);
+            if (cable) run_list(2 * wv, idle_fold, diag_task);
             __syncthreads();
-            if (cable && wv == 0) run_list(2 * W, idle_fwd, diag_task);
+            if (cable && wv == 0) run_list(2 * W, idle_fold, diag_task);
             __syncthreads();
             // from here on, below the root: sDX[c] = (ratio Uj / D', right-hand side) and
             // sU[c] = (Ui, D'); the per-step diagonal register Ds holds the ratio
@@ -449,78 +455,78 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 if constexpr (!SPLIT)
                     channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, tid, TPB, init, dt, [] { lds_barrier<TPB>(); });
                 if (!init) {
-                    // forward elimination: each node folds its children in ascending order
-                    // (one child: the dummy slot N stands in as the second; its right-hand
-                    // side and Ui are 0, so its terms are exactly zero)
-                    auto fwd_task = [&](uint32_t w) {
-                        const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
-                        const int c1 = nc >= 2 ? fc + 1 : N;
+                    // forward elimination: fold tasks of at most two children, applied in
+                    // ascending child order (the dummy slot N, with right-hand side and Ui 0,
+                    // stands in for a missing second child: its terms are exactly zero)
+                    auto fold_task = [&](uint2 w) {
+                        const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y;
                         if (const_tree) {
                             // below the root sDX[c] = (constant ratio Uj / D', rhs): only
-                            // right-hand sides change, one 8-byte read per child
-                            const float2 a0 = sDX[fc], a1 = sDX[c1];
-                            const float x = sDX[p].y;
-                            float oy = (x - a0.x * a0.y) - a1.x * a1.y;
-                            int i = 2;
-                            for (; i + 3 < nc; i += 4) {
-                                // four children's loads issued together, then folded in order
-                                float2 a[4];
-#pragma unroll
-                                for (int k = 0; k < 4; k++) a[k] = sDX[fc + i + k];
-#pragma unroll
-                                for (int k = 0; k < 4; k++) oy -= a[k].x * a[k].y;
-                            }
-                            for (; i < nc; i++) {
-                                const float2 a = sDX[fc + i];
-                                oy -= a.x * a.y;
-                            }
-                            if (p == 0) {
-                                // the root's diagonal changes every step
-                                float ox = (sDX[0].x - a0.x * sU[fc].x) - a1.x * sU[c1].x;
-                                for (int k = 2; k < nc; k++) ox -= sDX[fc + k].x * sU[fc + k].x;
-                                sDX[0] = make_float2(ox, oy);
-                            } else {
-                                sDX[p].y = oy;
-                            }
+                            // right-hand sides change
+                            const float2 a = *reinterpret_cast<const float2 *>(dxb + ao);
+                            const float2 b = *reinterpret_cast<const float2 *>(dxb + bo);
+                            float *xp = reinterpret_cast<float *>(dxb + po) + 1;
+                            *xp = (*xp - a.x * a.y) - b.x * b.y;
                         } else {
-                            const float2 dx = sDX[p];
-                            const float2 d0 = sDX[fc], u0 = sU[fc];
-                            const float2 d1 = sDX[c1], u1 = sU[c1];
-                            const float r0 = u0.y / d0.x;
-                            const float r1 = u1.y / d1.x;
+                            float2 *dp = reinterpret_cast<float2 *>(dxb + po);
+                            const float2 dx = *dp;
+                            const float2 da = *reinterpret_cast<const float2 *>(dxb + ao);
+                            const float2 ua = *reinterpret_cast<const float2 *>(ub + ao);
+                            const float2 db = *reinterpret_cast<const float2 *>(dxb + bo);
+                            const float2 ubb = *reinterpret_cast<const float2 *>(ub + bo);
+                            const float r0 = ua.y / da.x;
+                            const float r1 = ubb.y / db.x;
                             float2 o;
-                            o.x = (dx.x - r0 * u0.x) - r1 * u1.x;
-                            o.y = (dx.y - r0 * d0.y) - r1 * d1.y;
-                            for (int i = 2; i < nc; i++) {
-                                const float2 dxc = sDX[fc + i];
-                                const float2 u = sU[fc + i];
-                                const float ratio = u.y / dxc.x;
-                                o.x -= ratio * u.x;
-                                o.y -= ratio * dxc.y;
-                            }
-                            sDX[p] = o;
+                            o.x = (dx.x - r0 * ua.x) - r1 * ubb.x;
+                            o.y = (dx.y - r0 * da.y) - r1 * db.y;
+                            *dp = o;
                         }
                     };
                     // back-substitution of one non-root node from its solved parent
                     auto bwd_task = [&](uint32_t w) {
-                        const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
-                        const float xp = sDX[pp].y;
+                        const uint32_t po = w & 0xFFFFu, ppo = w >> 16;
+                        const float xpp = reinterpret_cast<const float2 *>(dxb + ppo)->y;
+                        float2 *dp = reinterpret_cast<float2 *>(dxb + po);
                         if (const_tree) {
                             // sU[p] = (Ui, D')
-                            const float2 u = sU[p];
-                            sDX[p].y = (sDX[p].y - u.x * xp) / u.y;
+                            const float2 u = *reinterpret_cast<const float2 *>(ub + po);
+                            dp->y = (dp->y - u.x * xpp) / u.y;
                         } else {
-                            const float2 dx = sDX[p];
-                            const float ui = sU[p].x;
-                            sDX[p].y = (dx.y - ui * xp) / dx.x;
+                            const float2 dx = *dp;
+                            const float ui = reinterpret_cast<const float2 *>(ub + po)->x;
+                            dp->y = (dx.y - ui * xpp) / dx.x;
                         }
                     };
-                    run_list(2 * wv, idle_fwd, fwd_task);
+                    run_list(2 * wv, idle_fold, fold_task);
                     lds_barrier<TPB>();
                     if (wv == 0) {
-                        run_list(2 * W, idle_fwd, fwd_task);
+                        run_list(2 * W, idle_fold, fold_task);
                         if (lane == 0) {
+                            // the root: all its children (positions 1 .. nr) in order, loads
+                            // issued four at a time, then its solve
+                            const int nr = cell.root_children;
                             float2 r0 = sDX[0];
+                            int c = 1;
+                            for (; c + 3 <= nr; c += 4) {
+                                float2 a[4], u[4];
+#pragma unroll
+                                for (int k = 0; k < 4; k++) {
+                                    a[k] = sDX[c + k];
+                                    u[k] = sU[c + k];
+                                }
+#pragma unroll
+                                for (int k = 0; k < 4; k++) {
+                                    const float r = const_tree ? a[k].x : u[k].y / a[k].x;
+                                    r0.x = r0.x - r * u[k].x;
+                                    r0.y = r0.y - r * a[k].y;
+                                }
+                            }
+                            for (; c <= nr; c++) {
+                                const float2 a = sDX[c], u = sU[c];
+                                const float r = const_tree ? a.x : u.y / a.x;
+                                r0.x = r0.x - r * u.x;
+                                r0.y = r0.y - r * a.y;
+                            }
                             r0.y = r0.y / r0.x;
                             sDX[0] = r0;
                         }

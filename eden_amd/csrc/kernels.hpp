@@ -22,7 +22,7 @@ struct TreeCell {
     int32_t n_cx;         // complex compartments (see TreeComplex)
     int32_t cx_off;       // offset of their records
     int32_t const_tree;   // 1: constant diagonals below the root (see tree_kernel)
-    int32_t pad;
+    int32_t root_children;   // children of the root (BFS positions 1 .. root_children)
 };
 
 // Per-compartment flag word of tree cells: 0 for a simple compartment (one ungated channel,
@@ -49,10 +49,11 @@ struct TreeLds {
 
 __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t nctmax, int32_t ninmax,
                                             int32_t words_max, int32_t cxmax, bool split) {
+    // sDX first, at the start of dynamic LDS: its address is then a constant, which the
+    // compiler folds into the offsets of the Hines solve's LDS instructions
     TreeLds L;
     const int32_t nbuf = split ? 2 : 1;
-    L.cx = 0;
-    L.dx = L.cx + 20 * cxmax;
+    L.dx = 0;
     L.u = L.dx + 2 * (nmax + 1);
     L.in = L.u + 2 * (nmax + 1);
     L.v = L.in + (ninmax > 0 ? 2 * nmax : 0);
@@ -61,8 +62,9 @@ __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t
     L.f = L.gt + ngmax;
     L.ct = L.f + nbuf * nctmax;
     L.il = L.ct + nctmax;
-    L.w = L.il + ninmax;
-    L.total = L.w + words_max;
+    L.w = (L.il + ninmax + 1) & ~1;         // 8-byte task words
+    L.cx = (L.w + words_max + 3) & ~3;      // 16-byte records
+    L.total = L.cx + 20 * cxmax;
     return L;
 }
 

@@ -116,36 +116,46 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
 
 // Wave-partitioned schedule of one tree for a workgroup of `waves` cable waves (tree kernel).
 //
-// Positions as in plan_tree (breadth-first, children consecutive in ascending index order),
-// and the same per-node operations: a forward task folds all children of one node in
-// ascending child index, a backward task solves one non-root node from its parent.
+// Positions as in plan_tree (breadth-first, children consecutive in ascending index order).
+// A node with children is eliminated by fold tasks, each folding at most two consecutive
+// children into the node's diagonal and right-hand side: children 2k and 2k + 1 in task k,
+// the tasks of a node in ascending k. The node therefore sees the same operations in the
+// same order as the reference's serial elimination, which applies its children one after
+// the other in ascending index. The root's fold is not scheduled here: the kernel performs
+// it (all root children in order) after every other fold. A backward task solves one
+// non-root node from its parent.
 //
 // The tree is cut into a junction J (the root and, for several waves, a few nodes below it:
 // J is closed towards the root) and the subtrees hanging from J, which are dealt to the
-// waves. A wave runs the forward tasks of its subtrees by height, then, after one workgroup
-// barrier, wave 0 runs the forward tasks of J (by height within J), solves the root and runs
-// the backward tasks of J's non-root nodes (by depth); after a second barrier every wave runs
-// the backward tasks of its subtrees by depth. Within a wave the rounds follow one another
-// with no barrier: a wave's LDS accesses are performed in program order, so a round reads
-// what the previous rounds of the same wave wrote. With one wave, J is empty and the root's
-// forward task is the last of wave 0's forward rounds.
+// waves. A wave runs the fold tasks of its subtrees, then, after one workgroup barrier, wave
+// 0 runs the fold tasks of J's non-root nodes, folds the root, solves it and runs the
+// backward tasks of J's non-root nodes (by depth); after a second barrier every wave runs the
+// backward tasks of its subtrees by depth. A fold task is placed in the round after the last
+// of its inputs (the node's previous fold, the two children's last folds). Within a wave the
+// rounds follow one another with no barrier: a wave's LDS accesses are performed in program
+// order, so a round reads what the previous rounds of the same wave wrote. With one wave, J
+// is empty (the root is folded after wave 0's rounds).
 //
 // Block layout (32-bit words):
-//   directory  [2 w] = wave w's forward list, [2 w + 1] = its backward list (w < waves),
-//              [2 waves] = J's forward list, [2 waves + 1] = J's backward list;
+//   directory  [2 w] = wave w's fold list, [2 w + 1] = its backward list (w < waves),
+//              [2 waves] = J's fold list, [2 waves + 1] = J's backward list;
 //              a list is (index of its first round header | round count << 16)
-//   headers    one per round: (offset of its first task word | task count << 16), task
-//              count 1..64, then five empty sentinel headers
-//   words      forward: pos | first_child_pos << 11 | n_children << 22
-//              backward: pos | parent_pos << 11
+//   headers    one per round: (index of its first task word, even | task count << 16),
+//              task count 1..64, then five empty sentinel headers
+//   words      fold task, two words (even index): (8 pos | 8 first_child << 16), 8 second
+//              child (8 n, the dummy slot, for a task with one child): byte offsets of
+//              8-byte slots; backward task, one word: 8 pos | 8 parent_pos << 16
+//   padding    128 zero words, so that a wave may read 64 task words from any round
 // The five rounds after a list's last round are always headers (of the next lists or the
 // sentinels), so a wave may read up to five headers past its list.
 struct WavePlan {
     std::vector<int32_t> pos;          // [n] BFS position of each compartment
     int32_t n_junction = 0;            // nodes in J
-    int32_t longest = 0;               // rounds on the longest path (max wave + J, both sweeps)
+    int32_t longest = 0;               // rounds on the longest path (waves + J, both sweeps)
     std::vector<uint32_t> block;
 };
+
+constexpr int32_t WAVE_PLAN_PAD = 128;
 
 inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves) {
     WavePlan P;
@@ -158,58 +168,63 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves)
         for (int32_t c : ch[order[h]]) order.push_back(c);
     P.pos.assign(n, -1);
     for (int32_t p = 0; p < n; p++) P.pos[order[p]] = p;
-    std::vector<int32_t> height(n, 0), depth(n, 0), size(n, 1);
-    for (int32_t i = n - 1; i >= 1; i--) {
-        height[parent[i]] = std::max(height[parent[i]], height[i] + 1);
-        size[parent[i]] += size[i];
-    }
+    std::vector<int32_t> depth(n, 0), size(n, 1);
+    for (int32_t i = n - 1; i >= 1; i--) size[parent[i]] += size[i];
     for (int32_t i = 1; i < n; i++) depth[i] = depth[parent[i]] + 1;
     constexpr int32_t LANES = 64;
 
-    // rounds of a group of tasks keyed by level: one round per 64 tasks of a level
+    // Fold levels of the nodes of one list (owner == o): level of fold task k of node i is
+    // 1 + max(level of task k - 1, last level of children 2k and 2k + 1); nodes outside the
+    // list (done in an earlier phase) and leaves count as level 0. Returns the last level of
+    // every node (0 for nodes without folds in the list) and the per-level task counts.
+    auto fold_levels = [&](const std::vector<int32_t> &owner, int32_t o, std::vector<int32_t> &last,
+                           std::vector<int32_t> &count) {
+        for (int32_t k = n - 1; k >= 0; k--) {   // reverse BFS: children before parents
+            const int32_t i = order[k];
+            if (owner[i] != o || i == 0 || ch[i].empty()) continue;
+            int32_t lv = 0;
+            for (size_t c = 0; c < ch[i].size(); c += 2) {
+                int32_t in = lv;
+                for (size_t d = c; d < std::min(c + 2, ch[i].size()); d++)
+                    if (owner[ch[i][d]] == o) in = std::max(in, last[ch[i][d]]);
+                lv = in + 1;
+                if ((int32_t)count.size() <= lv) count.resize(lv + 1, 0);
+                count[lv]++;
+            }
+            last[i] = lv;
+        }
+    };
     auto count_rounds = [&](const std::vector<int32_t> &per_level) {
         int32_t r = 0;
         for (int32_t c : per_level) r += (c + LANES - 1) / LANES;
         return r;
     };
-    // owner[i]: wave of node i, or -1 for J
-    struct Cut {
-        std::vector<int32_t> owner;
-        int32_t cost = 0;
+    // rounds on the longest path: the longest wave fold list, J's folds, the root fold
+    // (counted as one round per two root children), J's backward list, the longest wave
+    // backward list, and the two barriers
+    auto evaluate = [&](const std::vector<int32_t> &owner) {
+        int32_t wf = 0, wb = 0, jf = 0, jb = 0;
+        for (int32_t o = -1; o < waves; o++) {
+            std::vector<int32_t> last(n, 0), cnt, bc(n, 0);
+            fold_levels(owner, o, last, cnt);
+            for (int32_t i = 1; i < n; i++)
+                if (owner[i] == o) bc[depth[i]]++;
+            if (o < 0) {
+                jf = count_rounds(cnt);
+                jb = count_rounds(bc);
+            } else {
+                wf = std::max(wf, count_rounds(cnt));
+                wb = std::max(wb, count_rounds(bc));
+            }
+        }
+        return wf + jf + (int32_t)(ch[0].size() + 1) / 2 + jb + wb + (waves > 1 ? 2 : 0);
     };
-    auto evaluate = [&](std::vector<int32_t> &owner) {
-        const int32_t H = height[0] + 1, Dm = n;
-        std::vector<std::vector<int32_t>> fw(waves, std::vector<int32_t>(H, 0)), bw(waves);
-        int32_t fmax = 0, bmax = 0;
-        for (int32_t w = 0; w < waves; w++) bw[w].assign((size_t)std::min(Dm, n), 0);
-        for (int32_t i = 0; i < n; i++) {
-            if (owner[i] < 0) continue;
-            if (!ch[i].empty()) fw[owner[i]][height[i]]++;
-            if (i > 0) bw[owner[i]][depth[i]]++;
-        }
-        for (int32_t w = 0; w < waves; w++) {
-            fmax = std::max(fmax, count_rounds(fw[w]));
-            bmax = std::max(bmax, count_rounds(bw[w]));
-        }
-        // J: forward levels by height within J, backward by depth
-        std::vector<int32_t> jh(n, 0), jf(n + 1, 0), jb(n + 1, 0);
-        for (int32_t k = n - 1; k >= 0; k--) {
-            const int32_t i = order[k];
-            if (owner[i] >= 0) continue;
-            int32_t m = 0;
-            for (int32_t c : ch[i])
-                if (owner[c] < 0) m = std::max(m, jh[c]);
-            jh[i] = m + 1;
-            if (!ch[i].empty()) jf[jh[i]]++;
-            if (i > 0) jb[depth[i]]++;
-        }
-        // the barriers cost about one round each
-        return fmax + bmax + count_rounds(jf) + count_rounds(jb) + (waves > 1 ? 2 : 0);
-    };
-    Cut best;
+    std::vector<int32_t> best;
+    int32_t best_cost = 0;
     if (waves <= 1) {
-        best.owner.assign(n, 0);
-        best.cost = evaluate(best.owner);
+        best.assign(n, 0);
+        best[0] = -1;
+        best_cost = evaluate(best);
     } else {
         const double share = (double)n / waves;
         for (double f : {0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 1e9}) {
@@ -245,73 +260,86 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves)
                 }
             }
             const int32_t cost = evaluate(owner);
-            if (best.owner.empty() || cost < best.cost) {
-                best.owner = owner;
-                best.cost = cost;
+            if (best.empty() || cost < best_cost) {
+                best = owner;
+                best_cost = cost;
             }
         }
     }
-    P.longest = best.cost;
-    const std::vector<int32_t> &owner = best.owner;
+    P.longest = best_cost;
+    const std::vector<int32_t> &owner = best;
     for (int32_t i = 0; i < n; i++) P.n_junction += owner[i] < 0 ? 1 : 0;
+    if (waves <= 1) P.n_junction = 0;   // the root alone: folded by wave 0 after its rounds
 
-    // task lists: per wave, forward by height and backward by depth; then J
+    // task lists: per wave, folds by level and backward tasks by depth; then J
     std::vector<uint32_t> words;
     std::vector<std::pair<uint32_t, uint32_t>> rounds;   // (first word, count)
     std::vector<std::pair<uint32_t, uint32_t>> lists;    // (first round, round count)
-    auto emit_list = [&](std::vector<std::vector<uint32_t>> &levels) {
+    auto emit_list = [&](std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &levels, bool two) {
         const uint32_t r0 = (uint32_t)rounds.size();
         for (auto &tasks : levels)
             for (size_t k = 0; k < tasks.size(); k += LANES) {
                 const size_t cnt = std::min(tasks.size() - k, (size_t)LANES);
+                if (words.size() & 1) words.push_back(0);   // every round starts at an even index
                 rounds.emplace_back((uint32_t)words.size(), (uint32_t)cnt);
-                words.insert(words.end(), tasks.begin() + (long)k, tasks.begin() + (long)(k + cnt));
+                for (size_t t = k; t < k + cnt; t++) {
+                    words.push_back(tasks[t].first);
+                    if (two) words.push_back(tasks[t].second);
+                }
             }
         lists.emplace_back(r0, (uint32_t)rounds.size() - r0);
     };
-    auto fwd_word = [&](int32_t i) {
-        return (uint32_t)P.pos[i] | ((uint32_t)P.pos[ch[i][0]] << 11) | ((uint32_t)ch[i].size() << 22);
-    };
-    auto bwd_word = [&](int32_t i) { return (uint32_t)P.pos[i] | ((uint32_t)P.pos[parent[i]] << 11); };
-    const int32_t H = height[0];
-    int32_t Dm = 0;
-    for (int32_t i = 0; i < n; i++) Dm = std::max(Dm, depth[i]);
-    for (int32_t w = 0; w < waves; w++) {
-        std::vector<std::vector<uint32_t>> f(H + 1), b(Dm + 1);
-        for (int32_t p = 0; p < n; p++) {   // position order within a level
-            const int32_t i = order[p];
-            if (owner[i] != w) continue;
-            if (!ch[i].empty()) f[height[i]].push_back(fwd_word(i));
-            if (i > 0) b[depth[i]].push_back(bwd_word(i));
-        }
-        emit_list(f);
-        emit_list(b);
-    }
-    {
-        std::vector<int32_t> jh(n, 0);
+    const uint32_t dummy = 8u * (uint32_t)n;
+    auto add_folds = [&](int32_t o, std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &f) {
+        std::vector<int32_t> last(n, 0), cnt;
+        fold_levels(owner, o, last, cnt);
+        f.assign(cnt.size() + 1, {});
+        std::vector<int32_t> lv(n, 0);
         for (int32_t k = n - 1; k >= 0; k--) {
             const int32_t i = order[k];
-            if (owner[i] >= 0) continue;
-            int32_t m = 0;
-            for (int32_t c : ch[i])
-                if (owner[c] < 0) m = std::max(m, jh[c]);
-            jh[i] = m + 1;
+            if (owner[i] != o || i == 0 || ch[i].empty()) continue;
+            int32_t l = 0;
+            for (size_t c = 0; c < ch[i].size(); c += 2) {
+                int32_t in = l;
+                for (size_t d = c; d < std::min(c + 2, ch[i].size()); d++)
+                    if (owner[ch[i][d]] == o) in = std::max(in, lv[ch[i][d]]);
+                l = in + 1;
+                const uint32_t a = 8u * (uint32_t)P.pos[ch[i][c]];
+                const uint32_t b = c + 1 < ch[i].size() ? 8u * (uint32_t)P.pos[ch[i][c + 1]] : dummy;
+                f[l].push_back({8u * (uint32_t)P.pos[i] | (a << 16), b});
+            }
+            lv[i] = l;
         }
-        std::vector<std::vector<uint32_t>> f(H + 2), b(Dm + 1);
+        // position order within a level
+        for (auto &t : f) std::stable_sort(t.begin(), t.end(), [](auto &x, auto &y) { return (x.first & 0xFFFFu) < (y.first & 0xFFFFu); });
+    };
+    auto add_bwd = [&](int32_t o, std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &b) {
+        int32_t Dm = 0;
+        for (int32_t i = 0; i < n; i++) Dm = std::max(Dm, depth[i]);
+        b.assign(Dm + 1, {});
         for (int32_t p = 0; p < n; p++) {
             const int32_t i = order[p];
-            if (owner[i] >= 0) continue;
-            if (!ch[i].empty()) f[jh[i]].push_back(fwd_word(i));
-            if (i > 0) b[depth[i]].push_back(bwd_word(i));
+            if (owner[i] == o && i > 0) b[depth[i]].push_back({8u * (uint32_t)p | ((8u * (uint32_t)P.pos[parent[i]]) << 16), 0u});
         }
-        emit_list(f);
-        emit_list(b);
+    };
+    for (int32_t w = 0; w <= waves; w++) {
+        const int32_t o = w < waves ? w : -1;
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> f, b;
+        add_folds(o, f);
+        emit_list(f, true);
+        add_bwd(o, b);
+        emit_list(b, false);
     }
+    if (words.size() & 1) words.push_back(0);
     for (int k = 0; k < 5; k++) rounds.emplace_back((uint32_t)words.size(), 0u);   // sentinels
-    const uint32_t nd = 2 * (uint32_t)waves + 2, nh = (uint32_t)rounds.size();
+    const uint32_t nd = 2 * (uint32_t)waves + 2;
+    uint32_t nh = (uint32_t)rounds.size();
+    if ((nd + nh) & 1) nh++;   // task words start at an even index
     for (auto &l : lists) P.block.push_back((nd + l.first) | (l.second << 16));
     for (auto &r : rounds) P.block.push_back((nd + nh + r.first) | (r.second << 16));
+    if (P.block.size() < nd + nh) P.block.push_back(0u);
     P.block.insert(P.block.end(), words.begin(), words.end());
+    P.block.insert(P.block.end(), WAVE_PLAN_PAD, 0u);
     return P;
 }
 

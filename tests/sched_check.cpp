@@ -46,11 +46,14 @@ static std::vector<int32_t> make_parents(std::mt19937 &rng, int n, int shape) {
 }
 
 // The wave-partitioned schedule (plan_tree_waves) decoded as tree_kernel decodes it:
-// directory, per-wave lists of rounds (1..64 tasks), the junction lists run by wave 0 between
-// the two workgroup barriers. Within a round all tasks read before any writes (one SIMD
+// directory, per-wave lists of rounds (1..64 tasks), fold tasks of at most two children given
+// by byte offsets (the dummy slot n as the missing second child), the junction lists run by
+// wave 0 between the two workgroup barriers with the root fold (all root children in order)
+// and the root solve in between. Within a round all tasks read before any writes (one SIMD
 // instruction stream); rounds of a wave follow in order; waves are replayed one after the
 // other in reverse order, and a read of a node that another wave writes in the same phase is
-// an error (so no order of the waves could change the result).
+// an error (so no order of the waves could change the result). Every node must receive its
+// children in ascending order, each exactly once.
 static int check_waves(int n, const std::vector<int32_t> &par, const std::vector<std::vector<int>> &ch,
                        const std::vector<float> &D0, const std::vector<float> &X0, const std::vector<float> &ui,
                        const std::vector<float> &uj, const std::vector<float> &Dr, const std::vector<float> &Xr,
@@ -66,58 +69,77 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
         sj[P.pos[i]] = i ? uj[i] : 0.0f;
     }
     dx[n] = 1.0f; dv[n] = 0.0f; su[n] = 0.0f; sj[n] = 0.0f;
-    std::vector<int> folded(n + 1, 0), solved(n + 1, 0), writer(n + 1, -1);
+    std::vector<std::vector<int>> got(n + 1);   // children folded into each position, in order
+    std::vector<int> solved(n + 1, 0), writer(n + 1, -1);
     const int nd = 2 * waves + 2;
-    // one list: returns false on a malformed block
-    auto run = [&](int dir, int wave, bool fwd, int phase_owner_check) -> bool {
+    auto slot = [&](uint32_t off, bool &ok) {
+        if (off % 8 != 0 || off / 8 > (uint32_t)n) ok = false;
+        return ok ? (int)(off / 8) : n;
+    };
+    auto fold = [&](int p, int c) {   // one child into p, the reference's operations
+        const float r = sj[c] / dx[c];
+        dx[p] = dx[p] - r * su[c];
+        dv[p] = dv[p] - r * dv[c];
+        if (c != n) got[p].push_back(c);
+    };
+    auto run = [&](int dir, int wave, bool fwd, bool owner_check) -> bool {
         if (dir >= (int)B.size()) return false;
         const int first = B[dir] & 0xFFFF, cnt = B[dir] >> 16;
-        if (cnt > 0 && (first < nd || first + cnt >= (int)B.size())) return false;
+        if (cnt > 0 && (first < nd || first + cnt + 5 > (int)B.size())) return false;
         for (int r = 0; r < cnt; r++) {
             const uint32_t h = B[first + r];
             const int off = h & 0xFFFF, tc = h >> 16;
-            if (tc < 1 || tc > 64 || off + tc > (int)B.size()) return false;
-            std::vector<std::pair<int, std::pair<float, float>>> out;
+            // every lane reads a (two-word) task word from the round: within the block
+            if (tc < 1 || tc > 64 || off + 128 > (int)B.size() || (off & 1)) return false;
+            // reads of the whole round first
+            struct T { int p, a, b; float ox, oy; };
+            std::vector<T> tasks;
+            bool ok = true;
             for (int t = 0; t < tc; t++) {
-                const uint32_t w = B[off + t];
                 auto touch = [&](int x) {
-                    if (phase_owner_check && writer[x] >= 0 && writer[x] != wave) bad++;
+                    if (owner_check && x != n && writer[x] >= 0 && writer[x] != wave) bad++;
                 };
                 if (fwd) {
-                    const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = (int)(w >> 22);
-                    if (p >= n || nc < 1 || fc + nc > n) return false;
-                    const int c1 = nc >= 2 ? fc + 1 : n;
-                    touch(p);
-                    for (int i = 0; i < nc; i++) touch(fc + i);
-                    const float r0 = sj[fc] / dx[fc];
-                    const float r1 = sj[c1] / dx[c1];
-                    float ox = (dx[p] - r0 * su[fc]) - r1 * su[c1];
-                    float oy = (dv[p] - r0 * dv[fc]) - r1 * dv[c1];
-                    for (int i = 2; i < nc; i++) {
-                        const float ratio = sj[fc + i] / dx[fc + i];
-                        ox -= ratio * su[fc + i];
-                        oy -= ratio * dv[fc + i];
-                    }
-                    folded[p]++;
-                    out.push_back({p, {ox, oy}});
+                    const uint32_t w0 = B[off + 2 * t], w1 = B[off + 2 * t + 1];
+                    const int p = slot(w0 & 0xFFFF, ok), a = slot(w0 >> 16, ok), b = slot(w1, ok);
+                    if (!ok || p == n || a == n || p == 0) return false;
+                    touch(p); touch(a); touch(b);
+                    // the kernel's two-child form: ((D - r0 Ui0) - r1 Ui1), dummy b gives 0
+                    const float r0 = sj[a] / dx[a], r1 = sj[b] / dx[b];
+                    tasks.push_back({p, a, b, (dx[p] - r0 * su[a]) - r1 * su[b], (dv[p] - r0 * dv[a]) - r1 * dv[b]});
                 } else {
-                    const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
-                    if (p >= n || pp >= n) return false;
-                    touch(p);
-                    touch(pp);
-                    if (solved[pp] == 0 && pp != 0) bad++;   // parent not solved yet
-                    solved[p]++;
-                    out.push_back({p, {dx[p], (dv[p] - su[p] * dv[pp]) / dx[p]}});
+                    const uint32_t w = B[off + t];
+                    const int p = slot(w & 0xFFFF, ok), pp = slot(w >> 16, ok);
+                    if (!ok || p == n || pp == n || p == 0) return false;
+                    touch(p); touch(pp);
+                    if (solved[pp] == 0) bad++;   // parent not solved yet
+                    tasks.push_back({p, pp, -1, dx[p], (dv[p] - su[p] * dv[pp]) / dx[p]});
                 }
             }
-            for (auto &o : out) {
-                dx[o.first] = o.second.first;
-                dv[o.first] = o.second.second;
+            for (auto &t : tasks) {
+                dx[t.p] = t.ox;
+                dv[t.p] = t.oy;
+                if (fwd) {
+                    got[t.p].push_back(t.a);
+                    if (t.b != n) got[t.p].push_back(t.b);
+                } else {
+                    solved[t.p]++;
+                }
             }
         }
         return true;
     };
     bool ok = (int)B.size() >= nd + 1;
+    // every header a wave may read (its lists' and five past them) points at even offsets
+    // with 128 words of block after them
+    for (int d = 0; d < nd && ok; d++) {
+        const int first = B[d] & 0xFFFF, cnt = B[d] >> 16;
+        for (int r = 0; r < cnt + 5 && ok; r++) {
+            if (first + r >= (int)B.size()) { ok = false; break; }
+            const int off = B[first + r] & 0xFFFF;
+            if ((off & 1) || off + 128 > (int)B.size()) ok = false;
+        }
+    }
     // the nodes each wave writes in a phase, from its list (before any replay)
     auto owners = [&](int odd) {
         std::fill(writer.begin(), writer.end(), -1);
@@ -126,32 +148,37 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
             for (int r = 0; r < cnt && ok; r++) {
                 const uint32_t h = B[first + r];
                 for (int t = 0; t < (int)(h >> 16); t++) {
-                    const int p = B[(h & 0xFFFF) + t] & 0x7FF;
+                    const int p = (int)((B[(h & 0xFFFF) + (odd ? t : 2 * t)] & 0xFFFF) / 8);
                     if (p > n || (writer[p] >= 0 && writer[p] != w)) bad++;
                     writer[p] = w;
                 }
             }
         }
     };
-    // phase 1: every wave's forward list
     owners(0);
-    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w, w, true, 1);
-    // wave 0: junction forward, root, junction backward
+    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w, w, true, true);
     std::fill(writer.begin(), writer.end(), -1);
-    if (ok) ok = run(2 * waves, 0, true, 0);
+    if (ok) ok = run(2 * waves, 0, true, false);
+    // the root fold: every child of the root (positions 1 .. nc) in order, then the solve
+    for (size_t k = 0; k < ch[0].size() && ok; k++) fold(0, P.pos[ch[0][k]]);
     dv[0] = dv[0] / dx[0];
     solved[0] = 1;
-    if (ok) ok = run(2 * waves + 1, 0, false, 0);
-    // phase 3: every wave's backward list
+    if (ok) ok = run(2 * waves + 1, 0, false, false);
     if (ok) owners(1);
-    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w + 1, w, false, 1);
+    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w + 1, w, false, true);
     if (!ok) {
         std::printf("trial %d shape %d n %d waves %d: malformed block\n", trial, shape, n, waves);
         return bad + 1;
     }
     for (int i = 0; i < n; i++) {
         const int p = P.pos[i];
-        if (folded[p] != (ch[i].empty() ? 0 : 1)) bad++;
+        std::vector<int> want;
+        for (int c : ch[i]) want.push_back(P.pos[c]);
+        if (got[p] != want) {
+            std::printf("trial %d shape %d n %d waves %d: node %d children folded out of order\n", trial, shape, n,
+                        waves, i);
+            bad++;
+        }
         if (solved[p] != 1) bad++;
         if (std::memcmp(&dv[p], &Xr[i], 4) != 0 || std::memcmp(&dx[p], &Dr[i], 4) != 0) {
             std::printf("trial %d shape %d n %d waves %d: node %d gives %.9g, serial %.9g\n", trial, shape, n, waves,
@@ -160,6 +187,7 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
             break;
         }
     }
+    if (dx[n] != 1.0f || dv[n] != 0.0f) bad++;
     if (waves == 1 && P.n_junction != 0) bad++;
     return bad;
 }
