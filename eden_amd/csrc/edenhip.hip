@@ -339,7 +339,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         }
         if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2047 compartments");
         const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell);
-        const WavePlan P = plan_tree_waves(n, d->parent + c0, tpb / 64);
+        // wave 0 updates the root compartment during the first Hines phase; 2- and 4-wave
+        // cells also update their gates on the last cable wave (they have no channel wave)
+        const bool gates_on_last = tpb == 128 || tpb == 256;
+        const WavePlan P = plan_tree_waves(n, d->parent + c0, tpb / 64, 0.5, gates_on_last ? 2.0 : 0.0);
         TreeCell tc{};
         tc.gate_off = (int64_t)gqv.size();
         tc.ctask_off = (int32_t)ctask.size();

@@ -332,7 +332,8 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 }
             }
             slot_any[j] = __builtin_amdgcn_readfirstlane(wbase + TPB * j) < N && cable;
-            slot_simple[j] = __ballot(valid && (fl[j] & TCX_COMPLEX)) == 0ull;
+            // the root (thread 0, slot 0) is updated separately, after wave 0's Hines rounds
+            slot_simple[j] = __ballot(valid && c != 0 && (fl[j] & TCX_COMPLEX)) == 0ull;
         }
         __syncthreads();
         if (const_tree) {
@@ -392,9 +393,28 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         }
         __syncthreads();
 
+        // diagonal and right-hand side of a complex compartment (slot j, cell-local index c)
+        auto complex_update = [&](int j, int c, const float *sF, float t, bool init, float &Dn, float &Vn) {
+            const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
+            const CompConst k = cx.k;
+            float I_ch, G_ch;
+            tree_channels(k, cx.tcm, sF, V[j], I_ch, G_ch);
+            float I_in = pulses_current(T, k, c0 + c, t), G_in = 0.0f;
+            if (cx.tcm & TCM_INPUT) {
+                const float2 in = sIn[c];
+                I_in += in.x;
+                G_in += in.y;
+            }
+            const float I_syn = 0.0f, G_syn = 0.0f;
+            const float I_int = I_ch + I_in + I_syn;
+            const float G_int = G_ch + G_in + G_syn;
+            const float Dd = 1 * (G_int / k.C) + k.invrc;
+            Dn = (const_tree && pos[j] != 0) ? Ds[j] : 1 + Dd * dt;
+            Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int)))) / k.C;
+        };
         // time_f64 accumulates `time += dt` in double, as the reference's engine loop does
         double time = (s0 >= 1) ? T.t_row[s0 - 1] : 0.0;
-        int cur = 0;
+        int cur = 0, fcur = 0;
         for (int64_t s = s0; s < s1; s++) {
             const bool init = s <= 0;
             const float t = (float)time;
@@ -402,8 +422,10 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             const int nxt = SPLIT ? cur ^ 1 : cur;
             const float *sV = sVb + cur * (nmax + 1);
             float *sVn = sVb + nxt * (nmax + 1);
-            const float *sF = sFb + cur * nctmax;
-            float *sFn = sFb + nxt * nctmax;
+            // open fractions double-buffered in every mode: the root compartment's update, which
+            // reads them, runs after the gate updates may have written the next ones
+            const float *sF = sFb + fcur * nctmax;
+            float *sFn = sFb + (fcur ^ 1) * nctmax;
             if (cable) {
                 // Poisson inputs of the compartments that have any
                 if (NIN > 0) {
@@ -431,29 +453,17 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     const float I_int_s = (I_ch_s + 0.0f) + 0.0f;
                     float Dn = Ds[j];
                     float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
-                    if (!slot_simple[j] && (fl[j] & TCX_COMPLEX)) {
-                        const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
-                        const CompConst k = cx.k;
-                        float I_ch, G_ch;
-                        tree_channels(k, cx.tcm, sF, V[j], I_ch, G_ch);
-                        float I_in = pulses_current(T, k, c0 + c, t), G_in = 0.0f;
-                        if (cx.tcm & TCM_INPUT) {
-                            const float2 in = sIn[c];
-                            I_in += in.x;
-                            G_in += in.y;
-                        }
-                        const float I_syn = 0.0f, G_syn = 0.0f;
-                        const float I_int = I_ch + I_in + I_syn;
-                        const float G_int = G_ch + G_in + G_syn;
-                        const float Dd = 1 * (G_int / k.C) + k.invrc;
-                        Dn = (const_tree && pos[j] != 0) ? Ds[j] : 1 + Dd * dt;
-                        Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int)))) / k.C;
-                    }
+                    if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, Dn, Vn);
                     sDX[pos[j]] = make_float2(Dn, Vn);
                 }
                 lds_barrier<TPB>();
-                if constexpr (!SPLIT)
-                    channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, tid, TPB, init, dt, [] { lds_barrier<TPB>(); });
+                if constexpr (!SPLIT) {
+                    // gates on the last cable wave (lanes of that wave, wave-local sync), before
+                    // its Hines rounds; the schedule gives that wave a lighter share of the tree.
+                    // The open fractions are read again only after the step's last barrier.
+                    if (wv == W - 1)
+                        channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, lane, 64, init, dt, [] { lds_barrier<64>(); });
+                }
                 if (!init) {
                     // forward elimination: fold tasks of at most two children, applied in
                     // ascending child order (the dummy slot N, with right-hand side and Ui 0,
@@ -498,6 +508,13 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         }
                     };
                     run_list(2 * wv, idle_fold, fold_task);
+                    if (tid == 0 && (fl[0] & TCX_COMPLEX)) {
+                        // the root compartment's update, left out of the compartment phase: only
+                        // the root fold reads the root's diagonal and right-hand side
+                        float Dn = Ds[0], Vn = V[0];
+                        complex_update(0, 0, sF, t, false, Dn, Vn);
+                        sDX[0] = make_float2(Dn, Vn);
+                    }
                     lds_barrier<TPB>();
                     if (wv == 0) {
                         run_list(2 * W, idle_fold, fold_task);
@@ -577,6 +594,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 lds_barrier<TPB>();
             }
             cur = nxt;
+            fcur ^= 1;
         }
         if (cable) {
 #pragma unroll

@@ -60,7 +60,7 @@ __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t
     L.q = L.v + nbuf * (nmax + 1);
     L.gt = L.q + ngmax;
     L.f = L.gt + ngmax;
-    L.ct = L.f + nbuf * nctmax;
+    L.ct = L.f + 2 * nctmax;                // open fractions, double-buffered
     L.il = L.ct + nctmax;
     L.w = (L.il + ninmax + 1) & ~1;         // 8-byte task words
     L.cx = (L.w + words_max + 3) & ~3;      // 16-byte records

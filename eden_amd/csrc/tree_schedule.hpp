@@ -157,7 +157,11 @@ struct WavePlan {
 
 constexpr int32_t WAVE_PLAN_PAD = 128;
 
-inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves) {
+// first_bias, last_bias: initial loads of wave 0 and of the last wave, in shares of n / waves,
+// for waves with other work in the same phase (the tree kernel's root compartment update on
+// wave 0, gate updates on the last wave).
+inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves, double first_bias = 0.0,
+                                double last_bias = 0.0) {
     WavePlan P;
     std::vector<std::vector<int32_t>> ch(n);
     for (int32_t i = 1; i < n; i++) ch[parent[i]].push_back(i);   // ascending
@@ -248,6 +252,8 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves)
             // deal the subtrees, largest first, to the least loaded wave
             std::stable_sort(cand.begin(), cand.end(), [&](int32_t a, int32_t b) { return size[a] > size[b]; });
             std::vector<int64_t> load(waves, 0);
+            load[0] += (int64_t)(first_bias * share);
+            load[waves - 1] += (int64_t)(last_bias * share);
             for (int32_t c : cand) {
                 const int32_t w = (int32_t)(std::min_element(load.begin(), load.end()) - load.begin());
                 load[w] += size[c];
