@@ -249,7 +249,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         const int wv = tid >> 6, lane = tid & 63;
         const uint32_t dummy_off = 8u * (uint32_t)N;
         const uint2 idle_fold = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
-        const uint32_t idle_bwd = dummy_off | (dummy_off << 16);
+        const uint2 idle_bwd = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
         char *const dxb = reinterpret_cast<char *>(sDX);
         char *const ub = reinterpret_cast<char *>(sU);
         auto run_list = [&](int dir, auto idle, auto body) {
@@ -492,20 +492,29 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                             *dp = o;
                         }
                     };
-                    // back-substitution of one non-root node from its solved parent
-                    auto bwd_task = [&](uint32_t w) {
-                        const uint32_t po = w & 0xFFFFu, ppo = w >> 16;
-                        const float xpp = reinterpret_cast<const float2 *>(dxb + ppo)->y;
-                        float2 *dp = reinterpret_cast<float2 *>(dxb + po);
+                    // back-substitution of one non-root node from its solved parent, or from its
+                    // solved grandparent through the parent (solved again, the same operations,
+                    // when the parent's own task shares the round); mid = dummy slot N for the
+                    // first form, whose extra solve is discarded
+                    auto bwd_task = [&](uint2 w) {
+                        const uint32_t po = w.x & 0xFFFFu, so = w.x >> 16, mo = w.y;
+                        const float xs = reinterpret_cast<const float2 *>(dxb + so)->y;
+                        const float2 dm = *reinterpret_cast<const float2 *>(dxb + mo);
+                        const float2 um = *reinterpret_cast<const float2 *>(ub + mo);
+                        const float2 dp = *reinterpret_cast<const float2 *>(dxb + po);
+                        const float2 up = *reinterpret_cast<const float2 *>(ub + po);
+                        float xm, x;
                         if (const_tree) {
-                            // sU[p] = (Ui, D')
-                            const float2 u = *reinterpret_cast<const float2 *>(ub + po);
-                            dp->y = (dp->y - u.x * xpp) / u.y;
+                            // sU[c] = (Ui, D')
+                            xm = (dm.y - um.x * xs) / um.y;
+                            xm = mo == dummy_off ? xs : xm;
+                            x = (dp.y - up.x * xm) / up.y;
                         } else {
-                            const float2 dx = *dp;
-                            const float ui = reinterpret_cast<const float2 *>(ub + po)->x;
-                            dp->y = (dx.y - ui * xpp) / dx.x;
+                            xm = (dm.y - um.x * xs) / dm.x;
+                            xm = mo == dummy_off ? xs : xm;
+                            x = (dp.y - up.x * xm) / dp.x;
                         }
+                        reinterpret_cast<float2 *>(dxb + po)->y = x;
                     };
                     run_list(2 * wv, idle_fold, fold_task);
                     if (tid == 0 && (fl[0] & TCX_COMPLEX)) {

@@ -123,7 +123,9 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
 // same order as the reference's serial elimination, which applies its children one after
 // the other in ascending index. The root's fold is not scheduled here: the kernel performs
 // it (all root children in order) after every other fold. A backward task solves one
-// non-root node from its parent.
+// non-root node from its solved parent, or, when it shares a round with its parent's task,
+// solves the parent again from the solved grandparent and then itself: the same operations
+// on the same values, so two depths take one round.
 //
 // The tree is cut into a junction J (the root and, for several waves, a few nodes below it:
 // J is closed towards the root) and the subtrees hanging from J, which are dealt to the
@@ -144,7 +146,9 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
 //              task count 1..64, then five empty sentinel headers
 //   words      fold task, two words (even index): (8 pos | 8 first_child << 16), 8 second
 //              child (8 n, the dummy slot, for a task with one child): byte offsets of
-//              8-byte slots; backward task, one word: 8 pos | 8 parent_pos << 16
+//              8-byte slots; backward task, two words: (8 pos | 8 solved_pos << 16), 8 mid
+//              with solved_pos the parent and mid = 8 n, or solved_pos the grandparent and
+//              mid the parent
 //   padding    128 zero words, so that a wave may read 64 task words from any round
 // The five rounds after a list's last round are always headers (of the next lists or the
 // sentinels), so a wave may read up to five headers past its list.
@@ -203,6 +207,20 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
         for (int32_t c : per_level) r += (c + LANES - 1) / LANES;
         return r;
     };
+    // backward rounds: two consecutive non-empty depths with at most 64 tasks share one
+    auto bwd_rounds = [&](const std::vector<int32_t> &per_depth) {
+        int32_t r = 0;
+        for (size_t d = 0; d < per_depth.size(); d++) {
+            if (per_depth[d] == 0) continue;
+            if (d + 1 < per_depth.size() && per_depth[d + 1] > 0 && per_depth[d] + per_depth[d + 1] <= LANES) {
+                r++;
+                d++;
+            } else {
+                r += (per_depth[d] + LANES - 1) / LANES;
+            }
+        }
+        return r;
+    };
     // rounds on the longest path: the longest wave fold list, J's folds, the root fold
     // (counted as one round per two root children), J's backward list, the longest wave
     // backward list, and the two barriers
@@ -215,10 +233,10 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
                 if (owner[i] == o) bc[depth[i]]++;
             if (o < 0) {
                 jf = count_rounds(cnt);
-                jb = count_rounds(bc);
+                jb = bwd_rounds(bc);
             } else {
                 wf = std::max(wf, count_rounds(cnt));
-                wb = std::max(wb, count_rounds(bc));
+                wb = std::max(wb, bwd_rounds(bc));
             }
         }
         return wf + jf + (int32_t)(ch[0].size() + 1) / 2 + jb + wb + (waves > 1 ? 2 : 0);
@@ -319,13 +337,38 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
         // position order within a level
         for (auto &t : f) std::stable_sort(t.begin(), t.end(), [](auto &x, auto &y) { return (x.first & 0xFFFFu) < (y.first & 0xFFFFu); });
     };
+    // backward tasks by depth; two consecutive depths share a round when they have at most 64
+    // tasks together: a node of the second depth whose parent is in the first solves that
+    // parent again from the grandparent (the same operations on the same values) before
+    // itself, so the round reads only values solved in earlier rounds
     auto add_bwd = [&](int32_t o, std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &b) {
         int32_t Dm = 0;
         for (int32_t i = 0; i < n; i++) Dm = std::max(Dm, depth[i]);
-        b.assign(Dm + 1, {});
+        std::vector<std::vector<int32_t>> lvl(Dm + 2);
         for (int32_t p = 0; p < n; p++) {
             const int32_t i = order[p];
-            if (owner[i] == o && i > 0) b[depth[i]].push_back({8u * (uint32_t)p | ((8u * (uint32_t)P.pos[parent[i]]) << 16), 0u});
+            if (owner[i] == o && i > 0) lvl[depth[i]].push_back(i);
+        }
+        b.clear();
+        auto one = [&](int32_t i) {
+            return std::make_pair(8u * (uint32_t)P.pos[i] | ((8u * (uint32_t)P.pos[parent[i]]) << 16), dummy);
+        };
+        for (int32_t d = 1; d <= Dm; d++) {
+            if (lvl[d].empty()) continue;
+            std::vector<std::pair<uint32_t, uint32_t>> t;
+            for (int32_t i : lvl[d]) t.push_back(one(i));
+            if (!lvl[d + 1].empty() && lvl[d].size() + lvl[d + 1].size() <= (size_t)LANES) {
+                for (int32_t i : lvl[d + 1]) {
+                    const int32_t q = parent[i];
+                    if (owner[q] == o)
+                        t.push_back({8u * (uint32_t)P.pos[i] | ((8u * (uint32_t)P.pos[parent[q]]) << 16),
+                                     8u * (uint32_t)P.pos[q]});
+                    else
+                        t.push_back(one(i));
+                }
+                d++;
+            }
+            b.push_back(t);
         }
     };
     for (int32_t w = 0; w <= waves; w++) {
@@ -334,7 +377,7 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
         add_folds(o, f);
         emit_list(f, true);
         add_bwd(o, b);
-        emit_list(b, false);
+        emit_list(b, true);
     }
     if (words.size() & 1) words.push_back(0);
     for (int k = 0; k < 5; k++) rounds.emplace_back((uint32_t)words.size(), 0u);   // sentinels

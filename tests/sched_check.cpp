@@ -94,6 +94,7 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
             // reads of the whole round first
             struct T { int p, a, b; float ox, oy; };
             std::vector<T> tasks;
+            std::vector<int> two_step;
             bool ok = true;
             for (int t = 0; t < tc; t++) {
                 auto touch = [&](int x) {
@@ -108,12 +109,20 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
                     const float r0 = sj[a] / dx[a], r1 = sj[b] / dx[b];
                     tasks.push_back({p, a, b, (dx[p] - r0 * su[a]) - r1 * su[b], (dv[p] - r0 * dv[a]) - r1 * dv[b]});
                 } else {
-                    const uint32_t w = B[off + t];
-                    const int p = slot(w & 0xFFFF, ok), pp = slot(w >> 16, ok);
-                    if (!ok || p == n || pp == n || p == 0) return false;
-                    touch(p); touch(pp);
-                    if (solved[pp] == 0) bad++;   // parent not solved yet
-                    tasks.push_back({p, pp, -1, dx[p], (dv[p] - su[p] * dv[pp]) / dx[p]});
+                    const uint32_t w0 = B[off + 2 * t], w1 = B[off + 2 * t + 1];
+                    const int p = slot(w0 & 0xFFFF, ok), sp = slot(w0 >> 16, ok), m = slot(w1, ok);
+                    if (!ok || p == n || sp == n || p == 0) return false;
+                    touch(p); touch(sp); touch(m);
+                    if (solved[sp] == 0) bad++;   // the solved node it starts from is not solved yet
+                    float xm = dv[sp];
+                    if (m != n) {
+                        // the parent solved again: it must be unsolved at the round's start, and
+                        // its own task in this round (checked after the round)
+                        if (solved[m] != 0) bad++;
+                        two_step.push_back(m);
+                        xm = (dv[m] - su[m] * dv[sp]) / dx[m];
+                    }
+                    tasks.push_back({p, sp, -1, dx[p], (dv[p] - su[p] * xm) / dx[p]});
                 }
             }
             for (auto &t : tasks) {
@@ -126,6 +135,8 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
                     solved[t.p]++;
                 }
             }
+            for (int m : two_step)
+                if (solved[m] != 1) bad++;   // the parent's own task was not in the round
         }
         return true;
     };
@@ -148,7 +159,7 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
             for (int r = 0; r < cnt && ok; r++) {
                 const uint32_t h = B[first + r];
                 for (int t = 0; t < (int)(h >> 16); t++) {
-                    const int p = (int)((B[(h & 0xFFFF) + (odd ? t : 2 * t)] & 0xFFFF) / 8);
+                    const int p = (int)((B[(h & 0xFFFF) + 2 * t] & 0xFFFF) / 8);
                     if (p > n || (writer[p] >= 0 && writer[p] != w)) bad++;
                     writer[p] = w;
                 }
