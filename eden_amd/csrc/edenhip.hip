@@ -591,6 +591,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         b->words_max = std::max(b->words_max, cells[i].words_len);
         b->cxmax = std::max(b->cxmax, cells[i].n_cx);
     }
+    // launch order: workgroups with the most waves first, so that they find whole CUs
+    std::stable_sort(ctx->buckets.begin(), ctx->buckets.end(), [](const Bucket &x, const Bucket &y) { return x.tpb > y.tpb; });
     std::vector<TreeCell> dcells;
     for (auto &b : ctx->buckets) {
         // work queue order: largest cells first

@@ -154,7 +154,8 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
 // sentinels), so a wave may read up to five headers past its list.
 struct WavePlan {
     std::vector<int32_t> pos;          // [n] BFS position of each compartment
-    int32_t n_junction = 0;            // nodes in J
+    int32_t n_junction = 0;            // nodes in J (the root included)
+    int32_t hines_waves = 1;           // waves with non-empty lists
     int32_t longest = 0;               // rounds on the longest path (waves + J, both sweeps)
     std::vector<uint32_t> block;
 };
@@ -239,16 +240,25 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
                 wb = std::max(wb, bwd_rounds(bc));
             }
         }
-        return wf + jf + (int32_t)(ch[0].size() + 1) / 2 + jb + wb + (waves > 1 ? 2 : 0);
+        return wf + jf + (int32_t)(ch[0].size() + 1) / 2 + jb + wb;
     };
+    // The Hines work goes to the first hw waves (hw = 1, 2, 4, .. waves, the cheapest); the
+    // others have empty lists. With one Hines wave J is the root alone.
     std::vector<int32_t> best;
-    int32_t best_cost = 0;
-    if (waves <= 1) {
-        best.assign(n, 0);
-        best[0] = -1;
-        best_cost = evaluate(best);
-    } else {
-        const double share = (double)n / waves;
+    int32_t best_cost = 0, best_hw = 1;
+    for (int32_t hw = 1; hw <= waves; hw *= 2) {
+        if (hw == 1) {
+            std::vector<int32_t> owner(n, 0);
+            owner[0] = -1;
+            const int32_t cost = evaluate(owner);
+            if (best.empty() || cost < best_cost) {
+                best = owner;
+                best_cost = cost;
+                best_hw = 1;
+            }
+            continue;
+        }
+        const double share = (double)n / hw;
         for (double f : {0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0, 1e9}) {
             const int32_t T = (int32_t)std::max(1.0, f * share);
             std::vector<int32_t> owner(n, 0);
@@ -269,9 +279,9 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
             for (; !heap.empty(); heap.pop()) cand.push_back(-heap.top().second);
             // deal the subtrees, largest first, to the least loaded wave
             std::stable_sort(cand.begin(), cand.end(), [&](int32_t a, int32_t b) { return size[a] > size[b]; });
-            std::vector<int64_t> load(waves, 0);
+            std::vector<int64_t> load(hw, 0);
             load[0] += (int64_t)(first_bias * share);
-            load[waves - 1] += (int64_t)(last_bias * share);
+            if (hw == waves) load[hw - 1] += (int64_t)(last_bias * share);
             for (int32_t c : cand) {
                 const int32_t w = (int32_t)(std::min_element(load.begin(), load.end()) - load.begin());
                 load[w] += size[c];
@@ -283,17 +293,18 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
                     for (int32_t x : ch[v]) st.push_back(x);
                 }
             }
-            const int32_t cost = evaluate(owner);
-            if (best.empty() || cost < best_cost) {
+            const int32_t cost = evaluate(owner) + 2;   // two workgroup barriers
+            if (cost < best_cost) {
                 best = owner;
                 best_cost = cost;
+                best_hw = hw;
             }
         }
     }
+    P.hines_waves = best_hw;
     P.longest = best_cost;
     const std::vector<int32_t> &owner = best;
     for (int32_t i = 0; i < n; i++) P.n_junction += owner[i] < 0 ? 1 : 0;
-    if (waves <= 1) P.n_junction = 0;   // the root alone: folded by wave 0 after its rounds
 
     // task lists: per wave, folds by level and backward tasks by depth; then J
     std::vector<uint32_t> words;

@@ -199,7 +199,10 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
         }
     }
     if (dx[n] != 1.0f || dv[n] != 0.0f) bad++;
-    if (waves == 1 && P.n_junction != 0) bad++;
+    // waves beyond the Hines waves have empty lists
+    if (P.hines_waves < 1 || P.hines_waves > waves) bad++;
+    for (int w = P.hines_waves; w < waves; w++)
+        if ((B[2 * w] >> 16) != 0 || (B[2 * w + 1] >> 16) != 0) bad++;
     return bad;
 }
 
