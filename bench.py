@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip eden_amd.autotune (tree-cell size limits by timed short runs) before the timed run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -243,7 +245,15 @@ def main():
     print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
     model, desc = build_workload(args.workload)
     rows = model.total_rows()
-    sim = Simulation(model, threads_per_cell=args.threads_per_cell, profile=True)
+    limits, tune = None, None
+    if not args.no_autotune and args.threads_per_cell == 0 and model.n_comps > model.n_cells:
+        from eden_amd import autotune
+        t0 = time.perf_counter()
+        limits, report = autotune.autotune(model, steps=400, repeats=2, verbose=True)
+        tune = dict(size_limits=list(limits), seconds=time.perf_counter() - t0,
+                    timings=[dict(size_limits=list(l), ms_per_400_steps=t * 1e3) for l, t in report])
+        print(f"[bench] autotuned size limits {limits}", file=sys.stderr, flush=True)
+    sim = Simulation(model, threads_per_cell=args.threads_per_cell, profile=True, size_limits=limits)
     tpb, slots = sim.launch_shape()
 
     def one_step():
@@ -309,6 +319,8 @@ def main():
                    "launch_shape": {"threads_per_cell": tpb, "compartments_per_thread": slots}},
         "roofline": roofline,
     }
+    if tune is not None:
+        out["autotune"] = tune
     if net is not None:
         if net["synaptic_events"] >= 0:
             net["synaptic_events_per_s"] = net["synaptic_events"] / t_step

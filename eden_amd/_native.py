@@ -61,7 +61,8 @@ class ModelDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("threads_per_cell", C.c_int32), ("persistent", C.c_int32), ("chunk_steps", C.c_int32),
                 ("use_graph", C.c_int32), ("profile", C.c_int32), ("device", C.c_int32),
-                ("log_synapse_events", C.c_int32), ("network_window", C.c_int32)]
+                ("log_synapse_events", C.c_int32), ("network_window", C.c_int32),
+                ("size_limits", C.c_int32 * 4)]
 
 
 class KernelInfo(C.Structure):

@@ -9,7 +9,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -325,6 +327,32 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     std::vector<uint32_t> cflag(NC, 0);
     std::vector<TreeComplex> cxv;
     std::vector<uint32_t> gtv;   // per gate variable: its channel task (local to the cell) | gate index << 16
+    // Hines schedules of the tree cells, planned in parallel (independent per cell); cells
+    // with malformed parents are left unplanned here and rejected in the loop below
+    std::vector<WavePlan> plans(d->n_cells);
+    {
+        auto plan_one = [&](int c) {
+            const int64_t c0 = d->cell_comp_begin[c];
+            const int32_t n = (int32_t)(d->cell_comp_begin[c + 1] - c0);
+            if (n <= 1 || n > MAX_TREE_N || d->parent[c0] != -1) return;
+            for (int32_t i = 1; i < n; i++)
+                if (d->parent[c0 + i] < 0 || d->parent[c0 + i] >= i) return;
+            const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell, ctx->opt.size_limits);
+            // wave 0 updates the root compartment during the first Hines phase; 2- and 4-wave
+            // cells also update their gates on the last cable wave (they have no channel wave)
+            const bool gates_on_last = tpb == 128 || tpb == 256;
+            plans[c] = plan_tree_waves(n, d->parent + c0, tpb / 64, 0.5, gates_on_last ? 2.0 : 0.0);
+        };
+        const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            for (int c; (c = next.fetch_add(1)) < d->n_cells;) plan_one(c);
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nthr; t++) pool.emplace_back(worker);
+        worker();
+        for (auto &th : pool) th.join();
+    }
     for (int c = 0; c < d->n_cells; c++) {
         const int64_t c0 = d->cell_comp_begin[c], c1 = d->cell_comp_begin[c + 1];
         const int32_t n = (int32_t)(c1 - c0);
@@ -338,11 +366,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             continue;
         }
         if (n > MAX_TREE_N) return fail(ctx, EDENHIP_E_LIMIT, "cell with more than 2047 compartments");
-        const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell);
-        // wave 0 updates the root compartment during the first Hines phase; 2- and 4-wave
-        // cells also update their gates on the last cable wave (they have no channel wave)
-        const bool gates_on_last = tpb == 128 || tpb == 256;
-        const WavePlan P = plan_tree_waves(n, d->parent + c0, tpb / 64, 0.5, gates_on_last ? 2.0 : 0.0);
+        const int32_t tpb = choose_tpb(n, ctx->opt.threads_per_cell, ctx->opt.size_limits);
+        const WavePlan &P = plans[c];
         TreeCell tc{};
         tc.gate_off = (int64_t)gqv.size();
         tc.ctask_off = (int32_t)ctask.size();
@@ -931,6 +956,11 @@ int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, e
     ctx->device = ctx->opt.device;
     *out = ctx;
     if (!desc) return fail(ctx, EDENHIP_E_INVALID, "null descriptor");
+    for (int k = 0; k < 4; k++) {
+        const int32_t L = ctx->opt.size_limits[k];
+        if (L < 0 || L > 4 * (64 << k) || (k > 0 && L != 0 && L < ctx->opt.size_limits[k - 1]))
+            return fail(ctx, EDENHIP_E_INVALID, "size_limits: each in [0, 4 x threads], nondecreasing");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ctx, EDENHIP_E_DEVICE, "no HIP device visible");
     HIPCHECK(ctx, hipSetDevice(ctx->device));

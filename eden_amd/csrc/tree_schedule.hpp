@@ -404,12 +404,17 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
 }
 
 // Workgroup size for a cell of n compartments: each thread holds up to 4 compartments.
-inline int32_t choose_tpb(int32_t n, int32_t forced) {
+// limits (may be null): largest cell for 64, 128, 256 and 512 threads.
+inline int32_t choose_tpb(int32_t n, int32_t forced, const int32_t *limits = nullptr) {
     if (forced == 64 || forced == 128 || forced == 256 || forced == 512)
         if (n <= forced * 4) return forced;
-    if (n <= 256) return 64;
-    if (n <= 512) return 128;
-    if (n <= 1024) return 256;
+    static const int32_t dflt[4] = {256, 512, 1024, 2047};
+    const bool custom = limits && (limits[0] | limits[1] | limits[2] | limits[3]) != 0;
+    const int32_t *L = custom ? limits : dflt;
+    for (int k = 0; k < 4; k++) {
+        const int32_t tpb = 64 << k;
+        if (n <= L[k] && n <= 4 * tpb) return tpb;
+    }
     return 512;
 }
 

@@ -157,11 +157,12 @@ typedef struct edenhip_model_desc {
 
 /* Launch-shape choices. Zero-initialise for defaults. */
 typedef struct edenhip_options {
-    int32_t threads_per_cell;     /* 0 = autotuned; else 64, 128, 256 or 512 for tree cells */
+    int32_t threads_per_cell;     /* 0 = by size (size_limits); else 64, 128, 256 or 512 for
+                                     every tree cell */
     int32_t persistent;           /* 1 (default when 0) = time loop inside one launch per chunk;
                                      -1 = one launch per step (the plain per-step launch) */
     int32_t chunk_steps;          /* steps per trace chunk; 0 = all steps */
-    int32_t use_graph;            /* 1 = capture the per-step launches of a chunk in a hipGraph */
+    int32_t use_graph;            /* reserved, 0 */
     int32_t profile;              /* 1 = time each launch with HIP events */
     int32_t device;               /* HIP device ordinal */
     int32_t log_synapse_events;   /* > 0: log up to this many synaptic events (see below) */
@@ -169,6 +170,11 @@ typedef struct edenhip_options {
                                      steps per launch (up to the minimum delay) with ring-buffer
                                      delivery; -1 = one launch per step; -2 = windows, each run
                                      again step by step (exercises the exact fallback) */
+    int32_t size_limits[4];       /* largest tree cell (compartments) given 64, 128, 256 and
+                                     512 threads; all 0 = 256, 512, 1024, 2047. A cell takes
+                                     the first size whose limit holds it (each limit at most 4
+                                     compartments per thread; the 512 limit must be 2047).
+                                     eden_amd.autotune picks them by timing short runs. */
 } edenhip_options;
 
 typedef struct edenhip_ctx edenhip_ctx;

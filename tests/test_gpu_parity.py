@@ -87,6 +87,20 @@ def test_hh_everywhere_large_trees():
     assert res["raster_mismatch_cols"] == 0, res
 
 
+def test_size_limits_agree():
+    """Cells moved to larger workgroups by size limits (other schedules, same operations)."""
+    sizes = [60, 150, 300, 500, 900, 1300]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=15.0, seed=23)
+    res, g, r = _check(m, size_limits=(128, 384, 768, 2047))
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+def test_size_limits_rejected_when_invalid():
+    m = configs.random_trees(2, sizes=[30, 40], t_final_ms=1.0, seed=3)
+    with pytest.raises(Exception, match="size_limits"):
+        Simulation(m, size_limits=(300, 0, 0, 0))
+
+
 @pytest.mark.parametrize("tpb", [64, 128, 256, 512])
 def test_launch_shapes_agree(tpb):
     sizes = [60, 120, 250]
