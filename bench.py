@@ -196,9 +196,11 @@ def adjoint_bench(reps=2):
 
 def plain_launch_bench(model, rows):
     """The same workload with one kernel launch per step (state back in HBM between steps)
-    instead of the persistent in-kernel time loop."""
+    instead of the persistent in-kernel time loop: launched one by one, and captured in
+    hipGraphs of 256 steps (timed on the second run, which replays the graphs)."""
     from eden_amd.model import Simulation
-    with Simulation(model, persistent=-1, profile=True) as sim:
+
+    def timed(sim):
         sim.reset()
         t0 = time.perf_counter()
         done = 0
@@ -207,9 +209,16 @@ def plain_launch_bench(model, rows):
             if n == 0:
                 break
             done += n
-        el = time.perf_counter() - t0
+        return time.perf_counter() - t0
+
+    with Simulation(model, persistent=-1, profile=True) as sim:
+        el = timed(sim)
         launches = sum(k["launches"] for k in sim.kernel_infos())
-    return dict(seconds=el, compartment_updates_per_s=model.n_comps * rows / el, launches=int(launches))
+    with Simulation(model, persistent=-1, use_graph=True) as sim:
+        capture = timed(sim)    # captures and instantiates the graphs, then runs them
+        replay = timed(sim)
+    return dict(seconds=el, compartment_updates_per_s=model.n_comps * rows / el, launches=int(launches),
+                graph_seconds=replay, graph_first_run_seconds=capture)
 
 
 def main():
@@ -333,6 +342,7 @@ def main():
         print("[bench] plain launch (one launch per step)", file=sys.stderr, flush=True)
         pl = plain_launch_bench(model, rows)
         pl["persistent_speedup"] = pl["seconds"] / t_step
+        pl["persistent_speedup_over_graph"] = pl["graph_seconds"] / t_step
         out["plain_launch"] = pl
     if not args.no_adjoint:
         print("[bench] adjoint (256 x 100-compartment trees)", file=sys.stderr, flush=True)

@@ -85,6 +85,12 @@ struct edenhip_ctx {
     float *d_gq = nullptr;
     std::vector<Bucket> buckets;
     hipEvent_t ev_start = nullptr;
+    // use_graph: executable graphs of per-step launch ranges, by (first step, end step, row0)
+    struct GraphEntry {
+        int64_t a, b, row0;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
     int n_cu = 256;
     // initial copies of the state tables (for edenhip_reset)
     struct Snap { void *dst; void *src; size_t bytes; };
@@ -885,42 +891,92 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
         }
         return 0;
     }
-    const bool prof = ctx->opt.profile != 0;
-    for (int64_t a = s0; a < s1; a += stride) {
-        const int64_t b = std::min(s1, a + stride);
-        const bool first = a == s0, last = b == s1;
-        if (ctx->n_point > 0) {
-            const int tpb = 64;
-            const int grid = (ctx->n_point + tpb - 1) / tpb;
-            float dt = ctx->dt;
-            int32_t n = ctx->n_point;
-            void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &n,
-                            (void *)&a, (void *)&b, &row0, &dt};
-            if (prof && first) HIPCHECK(ctx, hipEventRecord(ctx->pp0, ctx->stream));
-            HIPCHECK(ctx, hipLaunchKernel(point_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
-            if (prof && last) HIPCHECK(ctx, hipEventRecord(ctx->pp1, ctx->stream));
-            nl++;
+    const bool graph = ctx->opt.use_graph != 0 && per_step;
+    const bool prof = ctx->opt.profile != 0 && !graph;   // per-kernel events stay out of graphs
+    // the launches of steps [a0, a1), one range per launch
+    auto issue = [&](int64_t a0, int64_t a1, bool pf) -> int {
+        for (int64_t a = a0; a < a1; a += stride) {
+            const int64_t b = std::min(a1, a + stride);
+            const bool first = a == s0, last = b == s1;
+            if (ctx->n_point > 0) {
+                const int tpb = 64;
+                const int grid = (ctx->n_point + tpb - 1) / tpb;
+                float dt = ctx->dt;
+                int32_t n = ctx->n_point;
+                int64_t aa = a, bb = b;
+                void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &n,
+                                (void *)&aa, (void *)&bb, &row0, &dt};
+                if (pf && first) HIPCHECK(ctx, hipEventRecord(ctx->pp0, ctx->stream));
+                HIPCHECK(ctx, hipLaunchKernel(point_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
+                if (pf && last) HIPCHECK(ctx, hipEventRecord(ctx->pp1, ctx->stream));
+                nl++;
+            }
+            // tree buckets run concurrently, each on its own stream, joined back to the main stream
+            if (!ctx->buckets.empty()) HIPCHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+            for (auto &bk : ctx->buckets) {
+                float dt = ctx->dt;
+                int32_t ntask = (int32_t)bk.order.size();
+                int64_t aa = a, bb = b;
+                void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
+                                &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &ctx->d_tcm,
+                                &ctx->d_ctask, &ctx->d_gq, &ctx->d_gt, &ctx->d_inl, &ctx->d_cflag,
+                                &ctx->d_cx, &bk.cxmax_lds, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
+                                &bk.words_max, (void *)&aa, (void *)&bb, &row0, &dt};
+                HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
+                if (pf && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));
+                HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split), dim3(bk.grid),
+                                              dim3(bk.tpb + (bk.split ? 64 : 0)), args, bk.lds, bk.stream));
+                if (pf && last) HIPCHECK(ctx, hipEventRecord(bk.p1, bk.stream));
+                HIPCHECK(ctx, hipEventRecord(bk.done, bk.stream));
+                HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, bk.done, 0));
+                nl++;
+                if (pf) bk.launches++;
+            }
         }
-        // tree buckets run concurrently, each on its own stream, joined back to the main stream
-        if (!ctx->buckets.empty()) HIPCHECK(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
-        for (auto &bk : ctx->buckets) {
-            float dt = ctx->dt;
-            int32_t ntask = (int32_t)bk.order.size();
-            void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
-                            &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &ctx->d_tcm,
-                            &ctx->d_ctask, &ctx->d_gq, &ctx->d_gt, &ctx->d_inl, &ctx->d_cflag,
-                            &ctx->d_cx, &bk.cxmax_lds, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
-                            &bk.words_max, (void *)&a, (void *)&b, &row0, &dt};
-            HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
-            if (prof && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));
-            HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split), dim3(bk.grid), dim3(bk.tpb + (bk.split ? 64 : 0)),
-                                          args, bk.lds, bk.stream));
-            if (prof && last) HIPCHECK(ctx, hipEventRecord(bk.p1, bk.stream));
-            HIPCHECK(ctx, hipEventRecord(bk.done, bk.stream));
-            HIPCHECK(ctx, hipStreamWaitEvent(ctx->stream, bk.done, 0));
-            nl++;
-            if (prof) bk.launches++;
+        return 0;
+    };
+    if (!graph) {
+        TRY(issue(s0, s1, prof));
+    } else {
+        // per-step launches captured in graphs of up to 256 steps (kernels only: the kernels
+        // reset their own work queues, nothing is cleared with a memset node), kept for the
+        // next run over the same steps (after edenhip_reset)
+        constexpr int64_t G = 256;
+        for (int64_t a = s0; a < s1; a += G) {
+            const int64_t b = std::min(s1, a + G);
+            hipGraphExec_t exec = nullptr;
+            for (auto &g : ctx->graphs)
+                if (g.a == a && g.b == b && g.row0 == row0) exec = g.exec;
+            if (!exec) {
+                hipGraph_t gr = nullptr;
+                HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                const int rc = issue(a, b, false);
+                const hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
+                if (rc != 0) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return rc;
+                }
+                if (ec != hipSuccess) return fail(ctx, EDENHIP_E_DEVICE, hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (ei != hipSuccess) return fail(ctx, EDENHIP_E_DEVICE, hipGetErrorString(ei));
+                ctx->graphs.push_back({a, b, row0, exec});
+            } else {
+                nl += (ctx->n_point > 0 ? 1 : 0) * (b - a) + (int64_t)ctx->buckets.size() * (b - a);
+            }
+            HIPCHECK(ctx, hipGraphLaunch(exec, ctx->stream));
         }
+    }
+    if (graph && ctx->opt.profile) {
+        // whole-chunk time only: the graphs hold no per-kernel events
+        HIPCHECK(ctx, hipEventRecord(e1, ctx->stream));
+        HIPCHECK(ctx, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+        ctx->kernel_ms += ms;
+        ctx->launches += nl;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
     if (prof) {
         HIPCHECK(ctx, hipEventRecord(e1, ctx->stream));
@@ -1188,6 +1244,7 @@ void edenhip_destroy(edenhip_ctx *ctx) {
             if (b.p1) (void)hipEventDestroy(b.p1);
         }
         if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
         if (ctx->h_conflict) (void)hipHostFree(ctx->h_conflict);
         if (ctx->pp0) (void)hipEventDestroy(ctx->pp0);
         if (ctx->pp1) (void)hipEventDestroy(ctx->pp1);

@@ -162,7 +162,10 @@ typedef struct edenhip_options {
     int32_t persistent;           /* 1 (default when 0) = time loop inside one launch per chunk;
                                      -1 = one launch per step (the plain per-step launch) */
     int32_t chunk_steps;          /* steps per trace chunk; 0 = all steps */
-    int32_t use_graph;            /* reserved, 0 */
+    int32_t use_graph;            /* 1 = with one launch per step (persistent = -1, or networks
+                                     of multi-compartment cells): capture the launches in
+                                     hipGraphs of 256 steps, kept for later runs over the same
+                                     steps; the profile then has whole-chunk times only */
     int32_t profile;              /* 1 = time each launch with HIP events */
     int32_t device;               /* HIP device ordinal */
     int32_t log_synapse_events;   /* > 0: log up to this many synaptic events (see below) */

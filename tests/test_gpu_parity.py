@@ -108,6 +108,25 @@ def test_launch_shapes_agree(tpb):
     _check(m, threads_per_cell=tpb)
 
 
+def test_graph_replays_match_eager():
+    """One launch per step captured in hipGraphs (two graphs for 405 steps), replayed three
+    times with eager runs between the replays: every replay equals the persistent run."""
+    sizes = [40, 90, 300]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=10.0, seed=41)
+    with Simulation(m) as sim:
+        ref, ev_ref = sim.run_all(), sim.read_events()
+    with Simulation(m, persistent=-1, use_graph=True) as g, Simulation(m, persistent=-1) as e:
+        for rep in range(3):
+            _say(f"[graph] replay {rep}")
+            g.reset()
+            tr = g.run_all()
+            np.testing.assert_array_equal(tr, ref)
+            for a, b in zip(g.read_events(), ev_ref):
+                np.testing.assert_array_equal(a, b)
+            e.reset()
+            np.testing.assert_array_equal(e.run_all(), ref)
+
+
 def test_chunked_run_matches_single_run():
     m = configs.random_trees(4, sizes=[40, 80, 120, 160], t_final_ms=20.0, seed=19)
     with Simulation(m) as s:
