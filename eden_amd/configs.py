@@ -74,10 +74,8 @@ def hh_population(n_cells=1000, t_final_ms=100.0, seed=1001, amp_max_nA=0.2, dia
         name=f"hh_population_{n_cells}")
 
 
-def random_tree(rng, n):
-    """One random binary tree morphology: node k attaches to a uniform node with fewer than
-    two children; segment length U(10, 50) um; diameter linear in path distance from 2 um at
-    the root to 0.5 um at the farthest distal point. Returns parent, proximal, distal."""
+def random_binary_parents(rng, n):
+    """Node k attaches to a uniform node with fewer than two children."""
     parent = np.full(n, -1, np.int32)
     nchild = np.zeros(n, np.int32)
     open_nodes = [0]
@@ -90,6 +88,14 @@ def random_tree(rng, n):
             open_nodes[i] = open_nodes[-1]
             open_nodes.pop()
         open_nodes.append(k)
+    return parent
+
+
+def tree_geometry(rng, parent):
+    """Segments for a parent array: length U(10, 50) um, random direction, diameter linear in
+    path distance from 2 um at the root to 0.5 um at the farthest distal point. Returns
+    proximal, distal (x, y, z, diameter)."""
+    n = len(parent)
     length = rng.uniform(10.0, 50.0, n)
     direc = rng.normal(size=(n, 3))
     direc /= np.linalg.norm(direc, axis=1, keepdims=True)
@@ -111,16 +117,27 @@ def random_tree(rng, n):
     d1[0] = 2.0
     prox = np.concatenate([pstart, d0[:, None]], axis=1).astype(np.float32)
     dist = np.concatenate([pend, d1[:, None]], axis=1).astype(np.float32)
+    return prox, dist
+
+
+def random_tree(rng, n):
+    """One random binary tree morphology (random_binary_parents, tree_geometry). Returns
+    parent, proximal, distal."""
+    parent = random_binary_parents(rng, n)
+    prox, dist = tree_geometry(rng, parent)
     return parent, prox, dist
 
 
 def random_trees(n_cells=512, sizes=None, t_final_ms=250.0, seed=2001, hh_everywhere=False,
-                 density_scale=False, amp_range=(0.1, 0.5), sim_seed=1, record_all=False):
+                 density_scale=False, amp_range=(0.1, 0.5), sim_seed=1, record_all=False, parents=None):
     """Configs 2/4/5: random-tree cells with HH on the soma (or everywhere) and leak on every
     compartment; somatic current step U(amp_range) nA from 10 ms to the end.
     sizes: compartment count per cell (defaults to config 2's lognormal(ln 100, 0.5) clipped
-    to [20, 400])."""
+    to [20, 400]); parents: explicit parent arrays (cell-local, parents before children)
+    instead of random binary trees."""
     rng = np.random.default_rng(seed)
+    if parents is not None:
+        sizes = [len(p) for p in parents]
     if sizes is None:
         sizes = np.clip(np.round(rng.lognormal(np.log(100.0), 0.5, n_cells)), 20, 400).astype(np.int64)
     sizes = np.asarray(sizes, np.int64)
@@ -134,7 +151,11 @@ def random_trees(n_cells=512, sizes=None, t_final_ms=250.0, seed=2001, hh_everyw
     inv = np.empty(nc, np.float32)
     parent = np.empty(nc, np.int32)
     for c in range(n_cells):
-        par, prox, dist = random_tree(rng, int(sizes[c]))
+        if parents is not None:
+            par = np.asarray(parents[c], np.int32)
+            prox, dist = tree_geometry(rng, par)
+        else:
+            par, prox, dist = random_tree(rng, int(sizes[c]))
         a, cp, r, iv = N.cable_setup(par, prox, dist, CM, RA)
         sl = slice(begin[c], begin[c + 1])
         area[sl], cap[sl], rax[sl], inv[sl], parent[sl] = a, cp, r, iv, par

@@ -694,6 +694,7 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         cell.comp_begin = c0;
         cell.n = n;
         const TreePlan P = plan_tree(n, d->parent + c0, a->tpb);
+        if (!P.ok) return afail(a, EDENHIP_E_LIMIT, "the adjoint supports at most 1,023 children per compartment");
         cell.rounds_fwd = n > 1 ? P.rounds_fwd : 0;
         cell.rounds_bwd = n > 1 ? P.rounds_bwd : 0;
         cell.words_off = (int32_t)words.size();

@@ -100,6 +100,7 @@ struct edenhip_ctx {
     int64_t launches = 0;
     // per-kernel profile of the point kernel, and descriptor cell index of every point / tree cell
     hipEvent_t pp0 = nullptr, pp1 = nullptr;
+    hipEvent_t nw0 = nullptr, nw1 = nullptr;   // around each network window launch (profile)
     double point_ms = 0;
     int64_t point_launches = 0;
     std::vector<int32_t> point_desc, tree_desc;
@@ -663,6 +664,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     if (ctx->opt.profile) {
         HIPCHECK(ctx, hipEventCreate(&ctx->pp0));
         HIPCHECK(ctx, hipEventCreate(&ctx->pp1));
+        HIPCHECK(ctx, hipEventCreate(&ctx->nw0));
+        HIPCHECK(ctx, hipEventCreate(&ctx->nw1));
     }
     HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->ev_start, hipEventDisableTiming));
     TRY(build_netwin(ctx, d, point));
@@ -802,8 +805,19 @@ static int net_window(edenhip_ctx *ctx, int64_t a, int64_t b, int64_t row0, bool
     int32_t np = ctx->n_point;
     void *wargs[] = {&T, &N, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &np,
                      &a, &b, &row0, &dt, &cur};
+    const bool prof = ctx->opt.profile != 0;
+    if (prof) HIPCHECK(ctx, hipEventRecord(ctx->nw0, st));
     HIPCHECK(ctx, hipLaunchKernel(net_window_kernel_ptr(), dim3(N.n_blocks), dim3(64), wargs,
                                   (size_t)std::max(1, ctx->n_gates) * sizeof(DGate), st));
+    if (prof) {
+        // the window kernel's own time (the delivery kernels are not included)
+        HIPCHECK(ctx, hipEventRecord(ctx->nw1, st));
+        HIPCHECK(ctx, hipEventSynchronize(ctx->nw1));
+        float ms = 0;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->nw0, ctx->nw1));
+        ctx->point_ms += ms;
+        ctx->point_launches++;
+    }
     (*nl)++;
     const int grid = 1024;
     int commit = 0;
@@ -814,7 +828,6 @@ static int net_window(edenhip_ctx *ctx, int64_t a, int64_t b, int64_t row0, bool
         HIPCHECK(ctx, hipLaunchKernel(net_deliver_kernel_ptr(), dim3(grid), dim3(64), dargs, 0, st));
         HIPCHECK(ctx, hipMemcpyAsync(ctx->h_conflict, N.conflict, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIPCHECK(ctx, hipStreamSynchronize(st));
-        (*nl)++;
         if (*ctx->h_conflict || force) {
             *conflict = 1;
             return 0;
@@ -823,7 +836,6 @@ static int net_window(edenhip_ctx *ctx, int64_t a, int64_t b, int64_t row0, bool
     commit = 1;
     void *dargs[] = {&T, &N, &a, &b, &last, &dt, &commit};
     HIPCHECK(ctx, hipLaunchKernel(net_deliver_kernel_ptr(), dim3(grid), dim3(64), dargs, 0, st));
-    (*nl)++;
     ctx->net_cur ^= 1;
     return 0;
 }
@@ -883,9 +895,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             HIPCHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
             ctx->kernel_ms += ms;
             ctx->launches += nl;
-            HIPCHECK(ctx, hipEventElapsedTime(&ms, ctx->pp0, ctx->pp1));
-            ctx->point_ms += ms;
-            ctx->point_launches += nl;
+            (void)nl;   // window launches and their time: counted in net_window
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
         }
@@ -1248,6 +1258,8 @@ void edenhip_destroy(edenhip_ctx *ctx) {
         if (ctx->h_conflict) (void)hipHostFree(ctx->h_conflict);
         if (ctx->pp0) (void)hipEventDestroy(ctx->pp0);
         if (ctx->pp1) (void)hipEventDestroy(ctx->pp1);
+        if (ctx->nw0) (void)hipEventDestroy(ctx->nw0);
+        if (ctx->nw1) (void)hipEventDestroy(ctx->nw1);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -53,6 +53,7 @@ struct TreePlan {
     std::vector<int32_t> pos;          // [n] BFS position of each compartment
     std::vector<int32_t> parent_pos;   // [n] by position
     int32_t rounds_fwd = 0, rounds_bwd = 0;
+    bool ok = true;                    // false: a node has more than 1,023 children (10-bit count)
     std::vector<uint32_t> block;       // headers (rounds_fwd + rounds_bwd + 2), then task words
 };
 
@@ -88,6 +89,8 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
             nr++;
         }
     };
+    for (int32_t i = 0; i < n; i++)
+        if (ch[i].size() > 1023) P.ok = false;
     for (int32_t h = 1; h <= H; h++) {
         std::vector<uint32_t> tasks;
         for (int32_t i = 0; i < n; i++)

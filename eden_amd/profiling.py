@@ -46,17 +46,27 @@ def traffic(fetch, write, cfetch, cwrite, copy_mib, kernel_sub, workload, comman
     f_copy = max(f_copy)
     w_copy = max(w_copy)
     kf, kw = nbytes / f_copy, nbytes / w_copy
-    fk = {n: v for (n, _), v in _per_dispatch(fetch, "FETCH_SIZE").items() if "edenhip" in n}
-    wk = {n: v for (n, _), v in _per_dispatch(write, "WRITE_SIZE").items() if "edenhip" in n}
+    def mean_per_kernel(d):
+        acc = {}
+        for (n, _), v in d.items():
+            if "edenhip" in n:
+                t, c = acc.get(n, (0.0, 0))
+                acc[n] = (t + v, c + 1)
+        return {n: t / c for n, (t, c) in acc.items()}, {n: c for n, (t, c) in acc.items()}
+
+    fk, nf = mean_per_kernel(_per_dispatch(fetch, "FETCH_SIZE"))
+    wk, _ = mean_per_kernel(_per_dispatch(write, "WRITE_SIZE"))
     lines = ["# HBM traffic per dispatch, " + workload, "", "Commands (FETCH_SIZE run, then the same with WRITE_SIZE; the "
              "calibration copy with each counter):", "", "```", command, "```", "",
              f"Calibration: a plain {copy_mib} MiB device copy reads FETCH_SIZE {f_copy:.0f} KiB and WRITE_SIZE "
              f"{w_copy:.0f} KiB for {nbytes:.0f} KiB read and written: factors {kf:.3f} (fetch), {kw:.3f} (write).", "",
-             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes (calibrated) |", "|---|---|---|---|"]
+             "Per kernel: the mean over its dispatches.", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes (calibrated) |",
+             "|---|---|---|---|---|"]
     chosen = None
     for n in sorted(fk):
         t = (fk[n] * kf + wk.get(n, 0.0) * kw) * 1024.0
-        lines.append(f"| {_short(n)} | {fk[n]:.0f} | {wk.get(n, 0.0):.0f} | {t:.4g} |")
+        lines.append(f"| {_short(n)} | {nf[n]} | {fk[n]:.0f} | {wk.get(n, 0.0):.0f} | {t:.4g} |")
         if kernel_sub in n:
             chosen = (_short(n), t)
     lines.append("")

@@ -39,7 +39,9 @@ static std::vector<int32_t> make_parents(std::mt19937 &rng, int n, int shape) {
         }
         case 2: par[i] = i - 1; break;                               // unbranched cable
         case 3: par[i] = (int32_t)(rng() % std::min(i, 3)); break;  // star-like, wide fan-out
-        default: par[i] = i > 8 ? (int32_t)(i - 1 - rng() % 8) : i - 1; break;   // long bushy chains
+        case 4: par[i] = i > 8 ? (int32_t)(i - 1 - rng() % 8) : i - 1; break;   // long bushy chains
+        case 5: par[i] = 0; break;                                   // star: every node on the root
+        default: par[i] = i <= n / 2 ? i - 1 : n / 2; break;         // broom: chain, then a star
         }
     }
     return par;
@@ -212,8 +214,8 @@ int main(int argc, char **argv) {
     std::uniform_real_distribution<float> U(0.5f, 2.0f);
     int bad = 0, runs = 0;
     for (int trial = 0; trial < trials && bad == 0; trial++) {
-        const int n = 2 + (int)(rng() % (trial % 4 == 0 ? 2046 : 300));
-        const int shape = trial % 5;
+        const int n = trial % 9 == 0 ? 2047 : 2 + (int)(rng() % (trial % 4 == 0 ? 2046 : 300));
+        const int shape = trial % 7;
         const std::vector<int32_t> par = make_parents(rng, n, shape);
         std::vector<float> D0(n), X0(n), ui(n), uj(n);
         for (int i = 0; i < n; i++) {
@@ -256,6 +258,14 @@ int main(int argc, char **argv) {
         for (int tpb : {64, 128, 256, 512}) {
             runs++;
             const TreePlan P = plan_tree(n, par.data(), tpb);
+            // the round schedule's 10-bit child count: such trees are flagged, not planned
+            size_t widest = 0;
+            for (auto &c : ch) widest = std::max(widest, c.size());
+            if (!P.ok) {
+                if (widest <= 1023) bad++;
+                continue;
+            }
+            if (widest > 1023) bad++;
             const uint32_t *B = P.block.data();
             const int RF = P.rounds_fwd, RB = P.rounds_bwd;
             // LDS image by position, with the dummy slot n

@@ -71,6 +71,26 @@ def test_large_trees_workgroup_per_cell():
     assert res["raster_mismatch_cols"] == 0, res
 
 
+def _shapes():
+    rng = np.random.default_rng(99)
+    n = 2047
+    chain = np.arange(-1, n - 1, dtype=np.int32)                  # unbranched cable, depth 2046
+    star = np.zeros(n, np.int32)
+    star[0] = -1                                                   # root with 2046 children
+    broom = np.concatenate([np.arange(-1, 999), np.full(n - 1000, 999)]).astype(np.int32)
+    recursive = np.array([-1] + [int(rng.integers(i)) for i in range(1, n)], np.int32)   # any fan-out
+    return [chain, star, broom, recursive, np.array([-1, 0], np.int32), np.array([-1, 0, 0, 1, 1, 2], np.int32)]
+
+
+def test_extreme_tree_shapes_at_the_size_limit():
+    """2,047-compartment chain, star, broom and random recursive tree, plus 2- and 6-node
+    cells: deepest schedules, widest folds (the root fold over 2,046 children), every
+    compartment recorded."""
+    m = configs.random_trees(parents=_shapes(), t_final_ms=3.0, seed=77, record_all=True)
+    res, g, r = _check(m)
+    assert res["raster_mismatch_cols"] == 0, res
+
+
 def test_hh_everywhere_trees():
     sizes = [100] * 6
     res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=40.0, seed=5001,
