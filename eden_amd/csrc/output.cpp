@@ -6,9 +6,13 @@
 // trailing '\n' (LineWriter_File::PutFrameLines, eden/Eden.cpp:10633-10640).
 #include <stdint.h>
 
+#include <algorithm>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/edenhip_output.h"
@@ -50,20 +54,88 @@ int edenhip_write_trajectory(const char *path, int append, const double *time_us
     if (!path || n_rows < 0 || n_cols < 0 || (n_rows > 0 && (!time_us || (n_cols > 0 && (!trace || !scale))))) return -1;
     File out(path, append);
     if (!out.f) return -2;
-    std::string line;
-    char num[32];
-    for (int64_t r = 0; r < n_rows; r++) {
+    // Rows are formatted in blocks by worker threads and written in row order: the text is
+    // the same as one thread's, the formatting (most of the cost) runs in parallel.
+    constexpr int64_t BLOCK = 64;
+    const int64_t n_blocks = (n_rows + BLOCK - 1) / BLOCK;
+    const int nthr = (int)std::max<int64_t>(1, std::min<int64_t>(n_blocks, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    auto format_block = [&](int64_t blk, std::string &line) {
+        char num[32];
         line.clear();
-        fixed16(time_us[r] * US_TO_S, num);
-        line += num;
-        for (int64_t c = 0; c < n_cols; c++) {
-            line += ' ';
-            fixed16((double)trace[r * n_cols + c] * scale[c], num);
+        for (int64_t r = blk * BLOCK; r < std::min(n_rows, (blk + 1) * BLOCK); r++) {
+            fixed16(time_us[r] * US_TO_S, num);
             line += num;
+            for (int64_t c = 0; c < n_cols; c++) {
+                line += ' ';
+                fixed16((double)trace[r * n_cols + c] * scale[c], num);
+                line += num;
+            }
+            line += '\n';
         }
-        line += '\n';
-        if (std::fwrite(line.data(), 1, line.size(), out.f) != line.size()) return -2;
+    };
+    if (nthr <= 1) {
+        std::string line;
+        for (int64_t b = 0; b < n_blocks; b++) {
+            format_block(b, line);
+            if (std::fwrite(line.data(), 1, line.size(), out.f) != line.size()) return -2;
+        }
+        return std::ferror(out.f) ? -2 : 0;
     }
+    // a ring of formatted blocks: workers take block numbers in order; the writer (this
+    // thread) writes block b once it is ready, then frees its slot for block b + ring
+    const int64_t ring = 4 * nthr;
+    std::vector<std::string> bufs((size_t)ring);
+    std::vector<int> ready((size_t)ring, 0);
+    std::mutex mu;
+    std::condition_variable cv;
+    int64_t next = 0, written = 0;
+    bool failed = false;
+    auto worker = [&]() {
+        std::string tmp;
+        for (;;) {
+            int64_t b;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return failed || next >= n_blocks || next < written + ring; });
+                if (failed || next >= n_blocks) return;
+                b = next++;
+            }
+            format_block(b, tmp);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                bufs[(size_t)(b % ring)].swap(tmp);
+                ready[(size_t)(b % ring)] = 1;
+            }
+            cv.notify_all();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthr; t++) pool.emplace_back(worker);
+    int rc = 0;
+    for (int64_t b = 0; b < n_blocks; b++) {
+        std::string blk;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return ready[(size_t)(b % ring)] != 0; });
+            blk.swap(bufs[(size_t)(b % ring)]);
+            ready[(size_t)(b % ring)] = 0;
+        }
+        if (rc == 0 && std::fwrite(blk.data(), 1, blk.size(), out.f) != blk.size()) rc = -2;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            written = b + 1;
+            if (rc != 0) failed = true;
+        }
+        cv.notify_all();
+        if (rc != 0) break;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        failed = failed || rc != 0;
+    }
+    cv.notify_all();
+    for (auto &t : pool) t.join();
+    if (rc != 0) return rc;
     return std::ferror(out.f) ? -2 : 0;
 }
 

@@ -41,6 +41,20 @@ def test_trajectory_file_bytes(tmp_path):
     np.testing.assert_allclose(data[:, 1:], tr.astype(float) * 1e-3, rtol=1e-7)
 
 
+def test_large_trajectory_formatted_in_parallel_keeps_row_order(tmp_path):
+    """Many row blocks (the writer formats them on several threads): same bytes as the
+    format specification, rows in order."""
+    rows, cols = 3001, 7
+    rng = np.random.default_rng(5)
+    t_us = output.row_times_us(25.0, rows)
+    tr = (rng.standard_normal((rows, cols)) * 30.0 - 60.0).astype(np.float32)
+    p = tmp_path / "big.dat"
+    output.write_trajectory(p, t_us, tr)
+    want = "".join(fx(t_us[r] * 1e-6) + "".join(" " + fx(float(tr[r, c]) * 1e-3) for c in range(cols)) + "\n"
+                   for r in range(rows))
+    assert p.read_text() == want
+
+
 @pytest.mark.parametrize("fmt", ["TIME_ID", "ID_TIME", "ascii_v0"])
 def test_event_file_bytes(tmp_path, fmt):
     t_us = output.row_times_us(25.0, 100)
