@@ -258,9 +258,9 @@ def main():
     if not args.no_autotune and args.threads_per_cell == 0 and model.n_comps > model.n_cells:
         from eden_amd import autotune
         t0 = time.perf_counter()
-        limits, report = autotune.autotune(model, steps=400, repeats=2, verbose=True)
+        limits, report = autotune.autotune(model, steps=1500, repeats=3, verbose=True)
         tune = dict(size_limits=list(limits), seconds=time.perf_counter() - t0,
-                    timings=[dict(size_limits=list(l), ms_per_400_steps=t * 1e3) for l, t in report])
+                    timings=[dict(size_limits=list(l), ms_per_1500_steps=t * 1e3) for l, t in report])
         print(f"[bench] autotuned size limits {limits}", file=sys.stderr, flush=True)
     sim = Simulation(model, threads_per_cell=args.threads_per_cell, profile=True, size_limits=limits)
     tpb, slots = sim.launch_shape()

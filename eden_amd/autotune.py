@@ -27,7 +27,7 @@ def _shortened(model, steps):
     return m
 
 
-def time_limits(model, limits, steps=400, repeats=2, device=0):
+def time_limits(model, limits, steps=1500, repeats=3, device=0):
     """Seconds per `steps` steps of `model` with the given size limits (best of `repeats`)."""
     m = _shortened(model, steps)
     with Simulation(m, size_limits=limits, device=device) as sim:
@@ -41,7 +41,7 @@ def time_limits(model, limits, steps=400, repeats=2, device=0):
     return best
 
 
-def autotune(model, steps=400, repeats=2, device=0, verbose=False):
+def autotune(model, steps=1500, repeats=3, device=0, verbose=False):
     """Returns (limits, report): the fastest size limits found and every timing taken."""
     if model.n_comps == model.n_cells:
         return DEFAULT, []          # only one-compartment cells: nothing to tune

@@ -178,7 +178,7 @@ __device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t
 // the same time as the cable group's step; voltages and open fractions are then
 // double-buffered and the two meet at one workgroup barrier per step.
 template <int TPB, bool SPLIT>
-__global__ void __launch_bounds__(TPB + (SPLIT ? 64 : 0)) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(TPB + (SPLIT ? 64 : 0)) __attribute__((amdgpu_waves_per_eu(4)))
 tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const DSynType *__restrict__ stypes_g,
             int n_stypes, const TreeCell *__restrict__ cells, const int32_t *__restrict__ task_order, int32_t n_tasks,
             int32_t *__restrict__ queue, const uint32_t *__restrict__ words_g, const int32_t *__restrict__ pos_g,
