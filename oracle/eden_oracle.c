@@ -32,6 +32,16 @@
  *   Hines forward elimination / back-substitution         eden/Eden.cpp:6114-6225
  *   spike detection and atomic-OR send                    eden/Eden.cpp:3946-3995, 6248
  *   event logger (flag read + clear at output time)       eden/Eden.cpp:11598-11635
+ *
+ * Pinned by tests/test_oracle.py (published hash steps, closed forms, scipy LSODA, dense
+ * solves, hand-enumerated delay tables) except, PARITY UNPINNED:
+ *   - the float results of glibc expf/logf that EDEN's generated code calls: this file
+ *     rounds a double evaluation once (ORACLE_LIBM selects glibc's own); no reference output
+ *     fixes which one EDEN's build sees;
+ *   - the random-call counters of the Poisson input (rng_offset + call count,
+ *     eden/Eden.cpp:2455-2456): followed from the call sites, not fixed by any reference output;
+ *   - the instance order of synapses from several projections onto one compartment
+ *     (eden/Eden.cpp:8791-9281): followed from the source, not fixed by a fixture.
  */
 #include <math.h>
 #include <stdint.h>
