@@ -235,7 +235,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="skip eden_amd.autotune (tree-cell size limits by timed short runs) before the timed run")
+                    help="skip eden_amd.autotune (tree-cell size limits by timed runs) before the timed run")
+    ap.add_argument("--size-limits", default="",
+                    help="tree-cell size limits a,b,c,d for 64/128/256/512 threads (skips the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -255,43 +257,52 @@ def main():
     model, desc = build_workload(args.workload)
     rows = model.total_rows()
     limits, tune = None, None
-    if not args.no_autotune and args.threads_per_cell == 0 and model.n_comps > model.n_cells:
+    if args.size_limits:
+        limits = tuple(int(x) for x in args.size_limits.split(","))
+        tune = dict(size_limits=list(limits), given=True)
+    elif not args.no_autotune and args.threads_per_cell == 0 and model.n_comps > model.n_cells:
         from eden_amd import autotune
         t0 = time.perf_counter()
-        limits, report = autotune.autotune(model, steps=1500, repeats=3, verbose=True)
+        limits, report = autotune.autotune(model, steps=None, repeats=2, verbose=True)
         tune = dict(size_limits=list(limits), seconds=time.perf_counter() - t0,
-                    timings=[dict(size_limits=list(l), ms_per_1500_steps=t * 1e3) for l, t in report])
+                    timings=[dict(size_limits=list(l), ms_per_run=t * 1e3) for l, t in report])
         print(f"[bench] autotuned size limits {limits}", file=sys.stderr, flush=True)
-    sim = Simulation(model, threads_per_cell=args.threads_per_cell, profile=True, size_limits=limits)
+    # timed runs without per-launch HIP events; one more run with them gives the per-kernel
+    # durations for the roofline
+    sim = Simulation(model, threads_per_cell=args.threads_per_cell, size_limits=limits)
     tpb, slots = sim.launch_shape()
 
-    def one_step():
-        sim.reset()
+    def one_step(s):
+        s.reset()
         done = 0
         while done < rows:
-            n = sim.run(-1)
+            n = s.run(-1)
             if n == 0:
                 break
             done += n
         return done
 
     for w in range(args.warmup):
-        one_step()
+        one_step(sim)
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
     times = []
-    kern = {}
     for s in range(args.steps):
         t0 = time.perf_counter()
-        one_step()
+        one_step(sim)
         el = time.perf_counter() - t0
         times.append(el)
-        infos = sim.kernel_infos()
+        print(f"[bench] step {s}: {el * 1e3:.2f} ms wall", file=sys.stderr, flush=True)
+    kern = {}
+    with Simulation(model, threads_per_cell=args.threads_per_cell, profile=True, size_limits=limits) as psim:
+        one_step(psim)    # first launches
+        one_step(psim)
+        infos = psim.kernel_infos()
         for ki in infos:
             acc = kern.setdefault(ki["name"], dict(ki, total_ms=0.0, launches=0))
             acc["total_ms"] += ki["total_ms"]
             acc["launches"] += ki["launches"]
         msg = ", ".join(f"{ki['name']} {ki['total_ms']:.2f} ms/{ki['launches']}" for ki in infos)
-        print(f"[bench] step {s}: {el * 1e3:.2f} ms wall; {msg}", file=sys.stderr, flush=True)
+        print(f"[bench] profiled run: {msg}", file=sys.stderr, flush=True)
     net = None
     if model.n_synapses > 0:
         w, nwin, rerun, deliv = sim.network_stats()
@@ -304,7 +315,7 @@ def main():
     # dominant kernel: the one with the largest summed duration
     dom = max(kern.values(), key=lambda k: k["total_ms"])
     per_launch_ms = dom["total_ms"] / max(1, dom["launches"])
-    launches_per_step = dom["launches"] / max(1, args.steps)
+    launches_per_step = max(1, dom["launches"])   # launches of one (the profiled) run
     bytes_per_launch = algorithmic_bytes(model, rows, dom["cell_index"]) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms * 1e-3)
     # HBM bytes per launch of that kernel on this workload, from the committed PMC summary

@@ -1,8 +1,10 @@
 """Launch-shape autotuner for tree cells: picks `size_limits` (the largest cell given 64, 128,
 256 and 512 threads) by timing short runs of the model itself.
 
-Each candidate is a full context (its own Hines schedules) advanced `steps` steps, timed from
-the host around `edenhip_run` after one warm-up run; the state is rewound between runs. The
+Each candidate is a full context (its own Hines schedules) advanced `steps` steps (default:
+the whole run), timed from the host around `edenhip_run` after one warm-up run; the state is
+rewound between runs. Short runs are cheaper but their timings need not carry over to the
+whole run, where the buckets' tails overlap differently. The
 search is coordinate-wise over the three movable limits, from the size rule (256, 512, 1024):
 a limit moves down when the larger workgroup is faster for the cells between the two values.
 
@@ -22,26 +24,38 @@ CANDIDATES = ((128, 192, 256), (384, 448, 512), (768, 896, 1024))
 
 
 def _shortened(model, steps):
+    if steps is None:
+        return model
     m = copy.copy(model)
     m.t_final = float(steps) * float(model.dt)
     return m
 
 
-def time_limits(model, limits, steps=1500, repeats=3, device=0):
+def time_limits(model, limits, steps=None, repeats=2, device=0):
     """Seconds per `steps` steps of `model` with the given size limits (best of `repeats`)."""
     m = _shortened(model, steps)
     with Simulation(m, size_limits=limits, device=device) as sim:
-        sim.run(-1)          # warm-up (also the first-launch costs)
+        rows = m.total_rows()
+
+        def run_all():
+            done = 0
+            while done < rows:
+                n = sim.run(-1)
+                if n == 0:
+                    break
+                done += n
+
+        run_all()            # warm-up (also the first-launch costs)
         best = float("inf")
         for _ in range(repeats):
             sim.reset()
             t0 = time.perf_counter()
-            sim.run(-1)
+            run_all()
             best = min(best, time.perf_counter() - t0)
     return best
 
 
-def autotune(model, steps=1500, repeats=3, device=0, verbose=False):
+def autotune(model, steps=None, repeats=2, device=0, verbose=False):
     """Returns (limits, report): the fastest size limits found and every timing taken."""
     if model.n_comps == model.n_cells:
         return DEFAULT, []          # only one-compartment cells: nothing to tune
@@ -51,7 +65,8 @@ def autotune(model, steps=1500, repeats=3, device=0, verbose=False):
         t = time_limits(model, limits, steps, repeats, device)
         report.append((tuple(limits), t))
         if verbose:
-            print(f"[autotune] limits {tuple(limits)}: {t * 1e3:.2f} ms / {steps} steps", file=sys.stderr, flush=True)
+            what = "the whole run" if steps is None else f"{steps} steps"
+            print(f"[autotune] limits {tuple(limits)}: {t * 1e3:.2f} ms / {what}", file=sys.stderr, flush=True)
         return t
 
     best = list(DEFAULT)

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <atomic>
+#include <mutex>
 #include <cmath>
 #include <thread>
 #include <cstdio>
@@ -174,6 +175,27 @@ static std::vector<double> row_times(float dt, double t_final) {
 }
 
 static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point);
+
+// Streams come from a per-device pool created once per process and kept until it ends:
+// slot 0 is a context's main stream, slots 1-4 its tree buckets' streams. The runtime binds
+// each stream to one of a few hardware queues when the stream is created, so streams made
+// afresh for every context land on different queues from one context to the next (measured:
+// the same C4 run took 656 ms in a process's first context, 725 ms in later ones). Pool
+// streams keep one binding for every context. Contexts of one device share the pool's
+// streams, so their work is ordered on them (one host thread per context, as documented).
+static int pool_stream(edenhip_ctx *ctx, int slot, hipStream_t *out) {
+    static std::mutex mu;
+    static std::vector<std::vector<hipStream_t>> pools;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)pools.size() <= ctx->device) pools.resize((size_t)ctx->device + 1);
+    auto &p = pools[(size_t)ctx->device];
+    if (p.empty()) {
+        p.assign(5, nullptr);
+        for (auto &st : p) HIPCHECK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    }
+    *out = p[(size_t)slot];
+    return 0;
+}
 
 static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     if (d->abi_version != EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
@@ -654,7 +676,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(upload(ctx, &b.d_cells, bc));
         TRY(upload(ctx, &b.d_order, seq));
         TRY(alloc_zero(ctx, &b.d_queue, 2));
-        HIPCHECK(ctx, hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        TRY(pool_stream(ctx, 1 + (int)(&b - ctx->buckets.data()), &b.stream));
         HIPCHECK(ctx, hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
         if (ctx->opt.profile) {
             HIPCHECK(ctx, hipEventCreate(&b.p0));
@@ -1030,7 +1052,7 @@ int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, e
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ctx, EDENHIP_E_DEVICE, "no HIP device visible");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    HIPCHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    TRY(pool_stream(ctx, 0, &ctx->stream));
     return build(ctx, desc);
 }
 
@@ -1248,7 +1270,7 @@ void edenhip_destroy(edenhip_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (void *p : ctx->allocs) (void)hipFree(p);
         for (auto &b : ctx->buckets) {
-            if (b.stream) (void)hipStreamDestroy(b.stream);
+            (void)b;   // pool streams stay for the process
             if (b.done) (void)hipEventDestroy(b.done);
             if (b.p0) (void)hipEventDestroy(b.p0);
             if (b.p1) (void)hipEventDestroy(b.p1);
@@ -1260,7 +1282,7 @@ void edenhip_destroy(edenhip_ctx *ctx) {
         if (ctx->pp1) (void)hipEventDestroy(ctx->pp1);
         if (ctx->nw0) (void)hipEventDestroy(ctx->nw0);
         if (ctx->nw1) (void)hipEventDestroy(ctx->nw1);
-        (void)hipStreamDestroy(ctx->stream);
+        // ctx->stream is a pool stream: kept for the process
     }
     delete ctx;
 }
