@@ -126,20 +126,26 @@ def cpu_baseline(model, threads, budget_s=10.0):
                 sample=f"all {model.n_cells} cells, first {rows} of {total} rows ({el:.1f} s, OpenMP static)")
 
 
-def parity_sample(model_name, n_cells=16):
-    """Trace tolerance on a sub-population built the same way (first cells of the workload):
-    GPU vs oracle over all rows."""
+def parity_sample(model_name, model=None, n_cells=16):
+    """GPU vs oracle over all rows on a sample of the workload: for models of independent
+    cells, cells of the workload itself (spread over the sizes, `Model.subset`); otherwise a
+    sub-population built the same way."""
     from eden_amd import configs
     from eden_amd.model import Simulation
     from oracle import oracle
-    if model_name == "config2":
+    if model is not None and model.n_synapses == 0 and len(model.poisson_rate) == 0:
+        sizes = model.comps_per_cell()
+        order = np.argsort(sizes, kind="stable")
+        pick = order[np.linspace(0, len(order) - 1, min(n_cells, len(order))).astype(np.int64)]
+        m, _, _ = model.subset(pick)
+        what = "cells of the workload"
+    elif model_name == "config2":
         sizes = configs.config2_sizes()[:n_cells]
         m = configs.random_trees(n_cells, sizes=sizes, t_final_ms=250.0, seed=2001)
-    elif model_name == "config4":
-        sizes = configs.config4_sizes()[:n_cells]
-        m = configs.random_trees(n_cells, sizes=sizes, t_final_ms=500.0, seed=4001, density_scale=True)
+        what = "sub-population"
     else:
         m = configs.hh_population(n_cells=n_cells)
+        what = "sub-population"
     with Simulation(m) as s:
         tg = s.run_all()
         eg = s.read_events()
@@ -149,9 +155,9 @@ def parity_sample(model_name, n_cells=16):
     ev_of = {int(c): i for i, c in enumerate(m.event_comp)}
     rec2ev = [ev_of.get(int(c), 0) for c in m.record_comp]
     res = compare(tg, eg, tr, (ec, er), len(m.event_comp), rec2ev)
-    return dict(cells=n_cells, rows=int(res["rows"]), max_abs_mV=res["max_abs_mV"],
-                raster_mismatch_cells=res["raster_mismatch_cols"], spikes=res["spikes_ref"],
-                within_tol=bool(res["max_abs_mV"] <= TOL_MV))
+    return dict(sample=what, cells=int(m.n_cells), compartments=int(m.n_comps), rows=int(res["rows"]),
+                max_abs_mV=res["max_abs_mV"], raster_mismatch_cells=res["raster_mismatch_cols"],
+                spikes=res["spikes_ref"], within_tol=bool(res["max_abs_mV"] <= TOL_MV))
 
 
 def adjoint_bench(reps=2):
@@ -348,7 +354,7 @@ def main():
         out["network"] = net
     if not args.no_parity:
         print("[bench] parity sample", file=sys.stderr, flush=True)
-        out["parity"] = parity_sample(args.workload)
+        out["parity"] = parity_sample(args.workload, model)
     if not args.no_plain and args.workload == "config4":
         print("[bench] plain launch (one launch per step)", file=sys.stderr, flush=True)
         pl = plain_launch_bench(model, rows)

@@ -161,6 +161,43 @@ class Model:
     def comps_per_cell(self):
         return np.diff(self.cell_comp_begin)
 
+    def subset(self, cells):
+        """The model restricted to the given cells (in the given order): their compartments,
+        channels, pulses, recorded and event columns, RNG seeds. Models without synapses and
+        Poisson inputs only (cells are then independent, so each cell's trace does not depend
+        on the others). Returns (model, record_cols, event_cols): the columns of this model's
+        trace and event log that the subset's columns come from."""
+        if self.n_synapses or len(self.poisson_rate) or len(self.spike_target):
+            raise ValueError("subset: cells with synapses or inputs from other cells are not independent")
+        cells = np.asarray(cells, np.int64)
+        cb = self.cell_comp_begin
+        comps = np.concatenate([np.arange(cb[c], cb[c + 1]) for c in cells]).astype(np.int64)
+        new_of = np.full(self.n_comps, -1, np.int64)
+        new_of[comps] = np.arange(len(comps))
+
+        def ranges(begin, items):
+            idx = np.concatenate([np.arange(begin[g], begin[g + 1]) for g in items] + [np.zeros(0, np.int64)])
+            counts = np.array([begin[g + 1] - begin[g] for g in items], np.int64)
+            return idx.astype(np.int64), _csr(counts)
+
+        ch, ch_b = ranges(self.comp_chan_begin, comps)
+        pu, pu_b = ranges(self.comp_pulse_begin, comps)
+        rec_cols = np.flatnonzero(new_of[self.record_comp] >= 0)
+        ev_cols = np.flatnonzero(new_of[self.event_comp] >= 0)
+        # keep the full model's column order among the chosen cells' columns
+        m = Model(
+            dt=self.dt, t_final=self.t_final, cell_comp_begin=_csr(cb[cells + 1] - cb[cells]),
+            v0=self.v0[comps], capacitance=self.capacitance[comps], area=self.area[comps],
+            inv_rc_axial=self.inv_rc_axial[comps], r_axial=self.r_axial[comps], v_threshold=self.v_threshold[comps],
+            parent=self.parent[comps], channel_types=self.channel_types, comp_chan_begin=ch_b,
+            chan_type=self.chan_type[ch], chan_gbase=self.chan_gbase[ch], chan_erev=self.chan_erev[ch],
+            cell_rng_seed=self.cell_rng_seed[cells], comp_pulse_begin=pu_b, pulse_start=self.pulse_start[pu],
+            pulse_duration=self.pulse_duration[pu], pulse_amplitude=self.pulse_amplitude[pu],
+            pulse_weight=self.pulse_weight[pu], syn_types=self.syn_types,
+            record_comp=new_of[self.record_comp[rec_cols]], event_comp=new_of[self.event_comp[ev_cols]],
+            name=f"{self.name}_subset{len(cells)}")
+        return m, rec_cols, ev_cols
+
     def total_rows(self):
         """floor(t_final/dt) + 1 with the reference's double accumulation of float dt."""
         t, n, dt = 0.0, 0, float(np.float32(self.dt))
