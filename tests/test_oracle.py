@@ -403,3 +403,19 @@ def test_delay_rule_hand_enumerated(delay_ms):
     if delay_ms == 15.0:
         # inter-spike intervals are shorter than the delay: some spikes must be dropped
         assert len(want) < len(spike_steps)
+
+
+def test_model_subset_runs_as_in_the_full_batch():
+    """Model.subset keeps each cell's data: the oracle's trace and events of a subset equal the
+    full model's columns for those cells (cells without synapses are independent)."""
+    from eden_amd import configs
+    m = configs.random_trees(6, sizes=[20, 30, 40, 50, 60, 70], t_final_ms=12.0, seed=7, record_all=True)
+    sub, rc, ec = m.subset([4, 1, 5])
+    assert sub.n_cells == 3 and sub.n_comps == 60 + 30 + 70
+    tr, c_full, r_full, _ = oracle.run(m)
+    ts, c_sub, r_sub, _ = oracle.run(sub)
+    np.testing.assert_array_equal(tr[:, rc], ts)
+    keep = np.isin(c_full, ec)
+    new = {int(c): i for i, c in enumerate(ec)}
+    assert sorted(zip(r_full[keep].tolist(), [new[int(c)] for c in c_full[keep]])) == \
+        sorted(zip(r_sub.tolist(), c_sub.tolist()))
