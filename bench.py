@@ -307,7 +307,8 @@ def main():
             acc = kern.setdefault(ki["name"], dict(ki, total_ms=0.0, launches=0))
             acc["total_ms"] += ki["total_ms"]
             acc["launches"] += ki["launches"]
-        msg = ", ".join(f"{ki['name']} {ki['total_ms']:.2f} ms/{ki['launches']}" for ki in infos)
+        msg = ", ".join(f"{ki['name']} {ki['total_ms']:.2f} ms/{ki['launches']} (grid {ki['grid']}, "
+                        f"LDS {ki['lds_bytes']} B)" for ki in infos)
         print(f"[bench] profiled run: {msg}", file=sys.stderr, flush=True)
     net = None
     if model.n_synapses > 0:

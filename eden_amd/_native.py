@@ -67,7 +67,7 @@ class Options(C.Structure):
 
 class KernelInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("threads_per_block", C.c_int32), ("grid", C.c_int32),
-                ("n_cells", C.c_int32), ("reserved", C.c_int32), ("n_comps", C.c_int64),
+                ("n_cells", C.c_int32), ("lds_bytes", C.c_int32), ("n_comps", C.c_int64),
                 ("launches", C.c_int64), ("total_ms", C.c_double)]
 
 

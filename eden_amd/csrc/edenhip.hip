@@ -28,11 +28,24 @@ using namespace edenhip;
 
 namespace {
 
+// A tree whose non-root compartments have no gated channel and no Poisson input: their Hines
+// diagonals are the same every step (the tree kernel's constant-tree path).
+bool const_below_root(const edenhip_model_desc *d, int64_t c0, int32_t n) {
+    for (int32_t i = 1; i < n; i++) {
+        const int64_t g = c0 + i;
+        if (d->comp_poisson_begin[g + 1] != d->comp_poisson_begin[g]) return false;
+        for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++)
+            if (d->channel_types[d->chan_type[ch]].n_gates != 0) return false;
+    }
+    return true;
+}
+
 struct Bucket {
     int tpb = 64;
     bool split = false;   // one extra wave runs the channel phase concurrently
     std::vector<int32_t> order;    // tree-cell indices, largest first
     int32_t nmax = 0;
+    int32_t nvmax = 0;        // voltage slots in LDS (1 for a constant tree, else its size)
     int32_t ngmax = 0;
     int32_t ninmax = 0;
     int32_t nctmax = 0;
@@ -370,7 +383,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             // wave 0 updates the root compartment during the first Hines phase; 2- and 4-wave
             // cells also update their gates on the last cable wave (they have no channel wave)
             const bool gates_on_last = tpb == 128 || tpb == 256;
-            plans[c] = plan_tree_waves(n, d->parent + c0, tpb / 64, 0.5, gates_on_last ? 2.0 : 0.0);
+            // constant trees take merged fold tasks (their kernel path forms right-hand sides only)
+            plans[c] = plan_tree_waves(n, d->parent + c0, tpb / 64, 0.5, gates_on_last ? 2.0 : 0.0,
+                                       const_below_root(d, c0, n));
         };
         const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::atomic<int> next{0};
@@ -486,6 +501,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             bool ok = true;
             for (int32_t i = 1; i < n && ok; i++) ok = (tcm[c0 + i] & 0xF8u) == 0u;
             tc.const_tree = ok ? 1 : 0;
+            if (ok != P.merged) return fail(ctx, EDENHIP_E_INVALID, "internal: schedule format of a constant tree");
         }
         cells.push_back(tc);
         cell_tpb.push_back(tpb);
@@ -639,6 +655,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         b->order.push_back((int32_t)i);
         b->comps += cells[i].n;
         b->nmax = std::max(b->nmax, cells[i].n);
+        b->nvmax = std::max(b->nvmax, cells[i].const_tree ? 1 : cells[i].n);
         b->ngmax = std::max(b->ngmax, cells[i].n_gate);
         b->ninmax = std::max(b->ninmax, cells[i].n_input);
         b->nctmax = std::max(b->nctmax, cells[i].n_ctask);
@@ -665,7 +682,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         b.split = (b.tpb == 64 || b.tpb == 512) && ctx->opt.persistent >= 0;
         // complex-compartment records go to LDS when they take at most 20 KiB
         b.cxmax_lds = b.cxmax <= 256 ? b.cxmax : 0;
-        b.lds = (size_t)tree_lds(b.nmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, b.split).total *
+        b.lds = (size_t)tree_lds(b.nmax, b.nvmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, ctx->n_gates,
+                                 ctx->n_stypes, b.split).total *
                 sizeof(float);
         const void *kp = tree_kernel_ptr(b.tpb, b.split);
         if (!kp) return fail(ctx, EDENHIP_E_LIMIT, "no tree kernel for the chosen workgroup size");
@@ -952,7 +970,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &bk.d_cells,
                                 &bk.d_order, &ntask, &bk.d_queue, &ctx->d_words, &ctx->d_pos, &ctx->d_u, &ctx->d_tcm,
                                 &ctx->d_ctask, &ctx->d_gq, &ctx->d_gt, &ctx->d_inl, &ctx->d_cflag,
-                                &ctx->d_cx, &bk.cxmax_lds, &bk.nmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
+                                &ctx->d_cx, &bk.cxmax_lds, &bk.nmax, &bk.nvmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
                                 &bk.words_max, (void *)&aa, (void *)&bb, &row0, &dt};
                 HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
                 if (pf && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));
@@ -1213,6 +1231,7 @@ int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_in
     out->threads_per_block = b.tpb + (b.split ? 64 : 0);
     out->grid = b.grid;
     out->n_cells = (int32_t)b.order.size();
+    out->lds_bytes = (int32_t)b.lds;
     out->n_comps = b.comps;
     out->launches = b.launches;
     out->total_ms = b.ms;

@@ -184,17 +184,18 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             int32_t *__restrict__ queue, const uint32_t *__restrict__ words_g, const int32_t *__restrict__ pos_g,
             const float2 *__restrict__ u_g, const uint32_t *__restrict__ tcm_g, const uint32_t *__restrict__ ctask_g,
             float *__restrict__ gq_g, const uint32_t *__restrict__ gt_g, const uint32_t *__restrict__ inl_g,
-            const uint32_t *__restrict__ cflag_g, const TreeComplex *__restrict__ cx_g, int32_t cxmax_lds, int32_t nmax, int32_t ngmax, int32_t nctmax,
-            int32_t ninmax, int32_t words_max, int64_t s0, int64_t s1, int64_t row0, float dt) {
+            const uint32_t *__restrict__ cflag_g, const TreeComplex *__restrict__ cx_g, int32_t cxmax_lds, int32_t nmax,
+            int32_t nvmax, int32_t ngmax, int32_t nctmax, int32_t ninmax, int32_t words_max, int64_t s0, int64_t s1,
+            int64_t row0, float dt) {
     constexpr int K = TREE_SLOTS;
     constexpr int NT = TPB + (SPLIT ? 64 : 0);
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ DGate gates[MAX_CHAN_TYPES * GATES_PER_TYPE];
-    __shared__ DSynType stypes[64];
     __shared__ int32_t s_task;
     // dynamic LDS, laid out by tree_lds()
-    const TreeLds L = tree_lds(nmax, ngmax, nctmax, ninmax, words_max, cxmax_lds, SPLIT);
+    const TreeLds L = tree_lds(nmax, nvmax, ngmax, nctmax, ninmax, words_max, cxmax_lds, n_gates, n_stypes, SPLIT);
     float *sm = reinterpret_cast<float *>(smem);
+    DGate *gates = reinterpret_cast<DGate *>(sm + L.gtab);
+    DSynType *stypes = reinterpret_cast<DSynType *>(sm + L.stab);
     TreeComplex *sCX = reinterpret_cast<TreeComplex *>(sm + L.cx);
     float2 *sDX = reinterpret_cast<float2 *>(sm + L.dx);
     float2 *sU = reinterpret_cast<float2 *>(sm + L.u);
@@ -250,6 +251,10 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         const uint32_t dummy_off = 8u * (uint32_t)N;
         const uint2 idle_fold = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
         const uint2 idle_bwd = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
+        // constant trees take four-word fold tasks with merged children (plan_tree_waves
+        // with merge): c0, c1 and the grandchildren refolded through each, dummy where absent
+        const uint32_t dd = dummy_off | (dummy_off << 16);
+        const uint4 idle_fold4 = make_uint4(dd, dd, dd, dummy_off);
         char *const dxb = reinterpret_cast<char *>(sDX);
         char *const ub = reinterpret_cast<char *>(sU);
         auto run_list = [&](int dir, auto idle, auto body) {
@@ -289,7 +294,10 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             for (int i = tid; i < cell.n_cx * 5; i += NT) dst[i] = src[i];
             CX = sCX;
         }
-        for (int i = tid; i < N; i += NT) sVb[i] = T.V[c0 + i];
+        // voltages read from LDS: the root's in a constant tree (no gated channel or input
+        // below it), else every compartment's
+        const int32_t NV = const_tree ? 1 : N;
+        for (int i = tid; i < NV; i += NT) sVb[i] = T.V[c0 + i];
         if (tid == 0) {
             // dummy slot for threads without a task: finite values, never read back
             sU[N] = make_float2(0.0f, 0.0f);
@@ -358,18 +366,29 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 }
             }
             __syncthreads();
-            auto diag_task = [&](uint2 w) {
-                const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y;
+            // a fold task's diagonal part; a merged child's eliminated diagonal is formed again
+            // from its children, as that child's own task forms it in the same round (the
+            // dummy slot, D 1 and U 0, gives exactly zero terms for an unmerged child)
+            auto diag_task = [&](uint4 w) {
+                const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y & 0xFFFFu;
+                auto dprime = [&](uint32_t c, uint32_t g0, uint32_t g1) {
+                    const float dc = reinterpret_cast<const float2 *>(dxb + c)->x;
+                    const float d0 = reinterpret_cast<const float2 *>(dxb + g0)->x;
+                    const float d1 = reinterpret_cast<const float2 *>(dxb + g1)->x;
+                    const float2 u0 = *reinterpret_cast<const float2 *>(ub + g0);
+                    const float2 u1 = *reinterpret_cast<const float2 *>(ub + g1);
+                    return (dc - (u0.y / d0) * u0.x) - (u1.y / d1) * u1.x;
+                };
                 const float d = reinterpret_cast<const float2 *>(dxb + po)->x;
-                const float da = reinterpret_cast<const float2 *>(dxb + ao)->x;
-                const float db = reinterpret_cast<const float2 *>(dxb + bo)->x;
+                const float da = dprime(ao, w.y >> 16, w.z & 0xFFFFu);
+                const float db = dprime(bo, w.z >> 16, w.w);
                 const float2 ua = *reinterpret_cast<const float2 *>(ub + ao);
                 const float2 ubb = *reinterpret_cast<const float2 *>(ub + bo);
                 reinterpret_cast<float2 *>(dxb + po)->x = (d - (ua.y / da) * ua.x) - (ubb.y / db) * ubb.x;
             };
-            if (cable) run_list(2 * wv, idle_fold, diag_task);
+            if (cable) run_list(2 * wv, idle_fold4, diag_task);
             __syncthreads();
-            if (cable && wv == 0) run_list(2 * W, idle_fold, diag_task);
+            if (cable && wv == 0) run_list(2 * W, idle_fold4, diag_task);
             __syncthreads();
             // from here on, below the root: sDX[c] = (ratio Uj / D', right-hand side) and
             // sU[c] = (Ui, D'); the per-step diagonal register Ds holds the ratio
@@ -412,175 +431,16 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             Dn = (const_tree && pos[j] != 0) ? Ds[j] : 1 + Dd * dt;
             Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int)))) / k.C;
         };
-        // time_f64 accumulates `time += dt` in double, as the reference's engine loop does
-        double time = (s0 >= 1) ? T.t_row[s0 - 1] : 0.0;
         int cur = 0, fcur = 0;
-        for (int64_t s = s0; s < s1; s++) {
-            const bool init = s <= 0;
-            const float t = (float)time;
-            if (!init) time += (double)dt;
-            const int nxt = SPLIT ? cur ^ 1 : cur;
-            const float *sV = sVb + cur * (nmax + 1);
-            float *sVn = sVb + nxt * (nmax + 1);
-            // open fractions double-buffered in every mode: the root compartment's update, which
-            // reads them, runs after the gate updates may have written the next ones
-            const float *sF = sFb + fcur * nctmax;
-            float *sFn = sFb + (fcur ^ 1) * nctmax;
-            if (cable) {
-                // Poisson inputs of the compartments that have any
-                if (NIN > 0) {
-                    for (int i = tid; i < NIN; i += TPB) {
-                        const uint32_t e = sIL[i];
-                        const int c = e & 0x7FF;
-                        float Ii = 0.0f, Gi = 0.0f;
-                        const PoiState ps{T.poi_tsince, T.poi_isi, T.poi_A, T.poi_B};
-                        poisson_step(T, stypes, c0 + c, sV[c], init, s, dt, Ii, Gi, ps, ps);
-                        sIn[c] = make_float2(Ii, Gi);
-                    }
-                    lds_barrier<TPB>();
-                }
-                // compartments: currents from the present open fractions, semi-implicit RHS.
-                // Slots without a valid lane in this wave are skipped; a wave whose valid
-                // lanes in a slot are all simple compartments takes the short form of the
-                // same operations (the constant conductance and diagonal computed above).
-#pragma unroll
-                for (int j = 0; j < K; j++) {
-                    if (!slot_any[j]) continue;
-                    const int c = tid + TPB * j;
-                    // simple form: channel slot 0, Gbase * 1 (ungated) * (E - V) * 1e-5f * A,
-                    // then the empty input and synapse terms, as in the general form below
-                    const float I_ch_s = 0.0f + ((((g0[j] * 1.0f) * (e0[j] - V[j])) * 1e-5f) * Ar[j]);
-                    const float I_int_s = (I_ch_s + 0.0f) + 0.0f;
-                    float Dn = Ds[j];
-                    float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
-                    if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, Dn, Vn);
-                    sDX[pos[j]] = make_float2(Dn, Vn);
-                }
-                lds_barrier<TPB>();
-                if constexpr (!SPLIT) {
-                    // gates on the last cable wave (lanes of that wave, wave-local sync), before
-                    // its Hines rounds; the schedule gives that wave a lighter share of the tree.
-                    // The open fractions are read again only after the step's last barrier.
-                    if (wv == W - 1)
-                        channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, lane, 64, init, dt, [] { lds_barrier<64>(); });
-                }
-                if (!init) {
-                    // forward elimination: fold tasks of at most two children, applied in
-                    // ascending child order (the dummy slot N, with right-hand side and Ui 0,
-                    // stands in for a missing second child: its terms are exactly zero)
-                    auto fold_task = [&](uint2 w) {
-                        const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y;
-                        if (const_tree) {
-                            // below the root sDX[c] = (constant ratio Uj / D', rhs): only
-                            // right-hand sides change
-                            const float2 a = *reinterpret_cast<const float2 *>(dxb + ao);
-                            const float2 b = *reinterpret_cast<const float2 *>(dxb + bo);
-                            float *xp = reinterpret_cast<float *>(dxb + po) + 1;
-                            *xp = (*xp - a.x * a.y) - b.x * b.y;
-                        } else {
-                            float2 *dp = reinterpret_cast<float2 *>(dxb + po);
-                            const float2 dx = *dp;
-                            const float2 da = *reinterpret_cast<const float2 *>(dxb + ao);
-                            const float2 ua = *reinterpret_cast<const float2 *>(ub + ao);
-                            const float2 db = *reinterpret_cast<const float2 *>(dxb + bo);
-                            const float2 ubb = *reinterpret_cast<const float2 *>(ub + bo);
-                            const float r0 = ua.y / da.x;
-                            const float r1 = ubb.y / db.x;
-                            float2 o;
-                            o.x = (dx.x - r0 * ua.x) - r1 * ubb.x;
-                            o.y = (dx.y - r0 * da.y) - r1 * db.y;
-                            *dp = o;
-                        }
-                    };
-                    // back-substitution of one non-root node from its solved parent, or from its
-                    // solved grandparent through the parent (solved again, the same operations,
-                    // when the parent's own task shares the round); mid = dummy slot N for the
-                    // first form, whose extra solve is discarded
-                    auto bwd_task = [&](uint2 w) {
-                        const uint32_t po = w.x & 0xFFFFu, so = w.x >> 16, mo = w.y;
-                        const float xs = reinterpret_cast<const float2 *>(dxb + so)->y;
-                        const float2 dm = *reinterpret_cast<const float2 *>(dxb + mo);
-                        const float2 um = *reinterpret_cast<const float2 *>(ub + mo);
-                        const float2 dp = *reinterpret_cast<const float2 *>(dxb + po);
-                        const float2 up = *reinterpret_cast<const float2 *>(ub + po);
-                        float xm, x;
-                        if (const_tree) {
-                            // sU[c] = (Ui, D')
-                            xm = (dm.y - um.x * xs) / um.y;
-                            xm = mo == dummy_off ? xs : xm;
-                            x = (dp.y - up.x * xm) / up.y;
-                        } else {
-                            xm = (dm.y - um.x * xs) / dm.x;
-                            xm = mo == dummy_off ? xs : xm;
-                            x = (dp.y - up.x * xm) / dp.x;
-                        }
-                        reinterpret_cast<float2 *>(dxb + po)->y = x;
-                    };
-                    run_list(2 * wv, idle_fold, fold_task);
-                    if (tid == 0 && (fl[0] & TCX_COMPLEX)) {
-                        // the root compartment's update, left out of the compartment phase: only
-                        // the root fold reads the root's diagonal and right-hand side
-                        float Dn = Ds[0], Vn = V[0];
-                        complex_update(0, 0, sF, t, false, Dn, Vn);
-                        sDX[0] = make_float2(Dn, Vn);
-                    }
-                    lds_barrier<TPB>();
-                    if (wv == 0) {
-                        run_list(2 * W, idle_fold, fold_task);
-                        if (lane == 0) {
-                            // the root: all its children (positions 1 .. nr) in order, loads
-                            // issued four at a time, then its solve
-                            const int nr = cell.root_children;
-                            float2 r0 = sDX[0];
-                            int c = 1;
-                            for (; c + 3 <= nr; c += 4) {
-                                float2 a[4], u[4];
-#pragma unroll
-                                for (int k = 0; k < 4; k++) {
-                                    a[k] = sDX[c + k];
-                                    u[k] = sU[c + k];
-                                }
-#pragma unroll
-                                for (int k = 0; k < 4; k++) {
-                                    const float r = const_tree ? a[k].x : u[k].y / a[k].x;
-                                    r0.x = r0.x - r * u[k].x;
-                                    r0.y = r0.y - r * a[k].y;
-                                }
-                            }
-                            for (; c <= nr; c++) {
-                                const float2 a = sDX[c], u = sU[c];
-                                const float r = const_tree ? a.x : u.y / a.x;
-                                r0.x = r0.x - r * u.x;
-                                r0.y = r0.y - r * a.y;
-                            }
-                            r0.y = r0.y / r0.x;
-                            sDX[0] = r0;
-                        }
-                        run_list(2 * W + 1, idle_bwd, bwd_task);
-                    }
-                    lds_barrier<TPB>();
-                    run_list(2 * wv + 1, idle_bwd, bwd_task);
-                    lds_barrier<TPB>();
-                }
-#pragma unroll
-                for (int j = 0; j < K; j++) {
-                    const int c = tid + TPB * j;
-                    if (slot_any[j] && c < N) {
-                        // the trace row of step s holds the voltage the step started from
-                        if (!init && rec[j] >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + rec[j]] = V[j];
-                        const float Vn = init ? V[j] : sDX[pos[j]].y;
-                        if (fl[j] & TCX_COMPLEX) {
-                            const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
-                            const float vthr = cx.k.vthr;
-                            if (V[j] < vthr && vthr < Vn) log_event(T, cx.k.ev, s);
-                        }
-                        V[j] = Vn;
-                        sVn[c] = Vn;
-                    }
-                }
-            } else {
-                // channel wave: next open fractions from this step's voltages, overlapping the
-                // cable group's step. With several cable waves it takes part in their barriers
+        if (SPLIT && !cable) {
+            // channel wave, in a step loop of its own (so that none of the cable group's
+            // per-slot registers are live across its gate updates): next open fractions from
+            // this step's voltages, overlapping the cable group's step
+            for (int64_t s = s0; s < s1; s++) {
+                const bool init = s <= 0;
+                const float *sV = sVb + cur * (nvmax + 1);
+                float *sFn = sFb + (fcur ^ 1) * nctmax;
+                // With several cable waves the channel wave takes part in their barriers
                 // (the input and compartment barriers first, then, after its own work, the
                 // three of the Hines solve), so that every wave passes the same barriers.
                 if constexpr (TPB > 64) {
@@ -595,15 +455,203 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         lds_barrier<TPB>();
                     }
                 }
-            }
-            // the next step reads voltages and open fractions written above by other lanes
-            if constexpr (SPLIT) {
+                // the end-of-step barrier of the cable group's loop
                 asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-            } else {
-                lds_barrier<TPB>();
+                cur ^= 1;
+                fcur ^= 1;
             }
-            cur = nxt;
-            fcur ^= 1;
+        } else {
+            // time_f64 accumulates `time += dt` in double, as the reference's engine loop does
+            double time = (s0 >= 1) ? T.t_row[s0 - 1] : 0.0;
+            for (int64_t s = s0; s < s1; s++) {
+                const bool init = s <= 0;
+                const float t = (float)time;
+                if (!init) time += (double)dt;
+                const int nxt = SPLIT ? cur ^ 1 : cur;
+                const float *sV = sVb + cur * (nvmax + 1);
+                float *sVn = sVb + nxt * (nvmax + 1);
+                // open fractions double-buffered in every mode: the root compartment's update, which
+                // reads them, runs after the gate updates may have written the next ones
+                const float *sF = sFb + fcur * nctmax;
+                float *sFn = sFb + (fcur ^ 1) * nctmax;
+                if (cable) {
+                    // Poisson inputs of the compartments that have any
+                    if (NIN > 0) {
+                        for (int i = tid; i < NIN; i += TPB) {
+                            const uint32_t e = sIL[i];
+                            const int c = e & 0x7FF;
+                            float Ii = 0.0f, Gi = 0.0f;
+                            const PoiState ps{T.poi_tsince, T.poi_isi, T.poi_A, T.poi_B};
+                            poisson_step(T, stypes, c0 + c, sV[c], init, s, dt, Ii, Gi, ps, ps);
+                            sIn[c] = make_float2(Ii, Gi);
+                        }
+                        lds_barrier<TPB>();
+                    }
+                    // compartments: currents from the present open fractions, semi-implicit RHS.
+                    // Slots without a valid lane in this wave are skipped; a wave whose valid
+                    // lanes in a slot are all simple compartments takes the short form of the
+                    // same operations (the constant conductance and diagonal computed above).
+    #pragma unroll
+                    for (int j = 0; j < K; j++) {
+                        if (!slot_any[j]) continue;
+                        const int c = tid + TPB * j;
+                        // simple form: channel slot 0, Gbase * 1 (ungated) * (E - V) * 1e-5f * A,
+                        // then the empty input and synapse terms, as in the general form below
+                        const float I_ch_s = 0.0f + ((((g0[j] * 1.0f) * (e0[j] - V[j])) * 1e-5f) * Ar[j]);
+                        const float I_int_s = (I_ch_s + 0.0f) + 0.0f;
+                        float Dn = Ds[j];
+                        float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
+                        if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, Dn, Vn);
+                        sDX[pos[j]] = make_float2(Dn, Vn);
+                    }
+                    lds_barrier<TPB>();
+                    if constexpr (!SPLIT) {
+                        // gates on the last cable wave (lanes of that wave, wave-local sync), before
+                        // its Hines rounds; the schedule gives that wave a lighter share of the tree.
+                        // The open fractions are read again only after the step's last barrier.
+                        if (wv == W - 1)
+                            channel_phase(gates, sCT, sGT, sQ, sFn, sV, NG, NCT, lane, 64, init, dt, [] { lds_barrier<64>(); });
+                    }
+                    if (!init) {
+                        // forward elimination: fold tasks of at most two children, applied in
+                        // ascending child order (the dummy slot N, with right-hand side and Ui 0,
+                        // stands in for a missing second child: its terms are exactly zero)
+                        // constant tree: below the root sDX[c] = (constant ratio Uj / D', rhs), so
+                        // only right-hand sides change; a merged child's right-hand side is folded
+                        // again from its children, with the operations of its own task, which
+                        // reads the same values and writes that child in the same round
+                        auto fold_const = [&](uint4 w) {
+                            const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y & 0xFFFFu;
+                            auto rhs = [&](float x, uint32_t g0, uint32_t g1) {
+                                const float2 q0 = *reinterpret_cast<const float2 *>(dxb + g0);
+                                const float2 q1 = *reinterpret_cast<const float2 *>(dxb + g1);
+                                return (x - q0.x * q0.y) - q1.x * q1.y;
+                            };
+                            const float2 a = *reinterpret_cast<const float2 *>(dxb + ao);
+                            const float2 b = *reinterpret_cast<const float2 *>(dxb + bo);
+                            const float xa = rhs(a.y, w.y >> 16, w.z & 0xFFFFu);
+                            const float xb = rhs(b.y, w.z >> 16, w.w);
+                            float *xp = reinterpret_cast<float *>(dxb + po) + 1;
+                            *xp = (*xp - a.x * xa) - b.x * xb;
+                        };
+                        // general tree: diagonal and right-hand side
+                        auto fold_task = [&](uint2 w) {
+                            const uint32_t po = w.x & 0xFFFFu, ao = w.x >> 16, bo = w.y;
+                            float2 *dp = reinterpret_cast<float2 *>(dxb + po);
+                            const float2 dx = *dp;
+                            const float2 da = *reinterpret_cast<const float2 *>(dxb + ao);
+                            const float2 ua = *reinterpret_cast<const float2 *>(ub + ao);
+                            const float2 db = *reinterpret_cast<const float2 *>(dxb + bo);
+                            const float2 ubb = *reinterpret_cast<const float2 *>(ub + bo);
+                            const float r0 = ua.y / da.x;
+                            const float r1 = ubb.y / db.x;
+                            float2 o;
+                            o.x = (dx.x - r0 * ua.x) - r1 * ubb.x;
+                            o.y = (dx.y - r0 * da.y) - r1 * db.y;
+                            *dp = o;
+                        };
+                        // back-substitution of one non-root node from its solved parent, or from its
+                        // solved grandparent through the parent (solved again, the same operations,
+                        // when the parent's own task shares the round); mid = dummy slot N for the
+                        // first form, whose extra solve is discarded
+                        auto bwd_task = [&](uint2 w) {
+                            const uint32_t po = w.x & 0xFFFFu, so = w.x >> 16, mo = w.y;
+                            const float xs = reinterpret_cast<const float2 *>(dxb + so)->y;
+                            const float2 dm = *reinterpret_cast<const float2 *>(dxb + mo);
+                            const float2 um = *reinterpret_cast<const float2 *>(ub + mo);
+                            const float2 dp = *reinterpret_cast<const float2 *>(dxb + po);
+                            const float2 up = *reinterpret_cast<const float2 *>(ub + po);
+                            float xm, x;
+                            if (const_tree) {
+                                // sU[c] = (Ui, D')
+                                xm = (dm.y - um.x * xs) / um.y;
+                                xm = mo == dummy_off ? xs : xm;
+                                x = (dp.y - up.x * xm) / up.y;
+                            } else {
+                                xm = (dm.y - um.x * xs) / dm.x;
+                                xm = mo == dummy_off ? xs : xm;
+                                x = (dp.y - up.x * xm) / dp.x;
+                            }
+                            reinterpret_cast<float2 *>(dxb + po)->y = x;
+                        };
+                        if (const_tree)
+                            run_list(2 * wv, idle_fold4, fold_const);
+                        else
+                            run_list(2 * wv, idle_fold, fold_task);
+                        if (tid == 0 && (fl[0] & TCX_COMPLEX)) {
+                            // the root compartment's update, left out of the compartment phase: only
+                            // the root fold reads the root's diagonal and right-hand side
+                            float Dn = Ds[0], Vn = V[0];
+                            complex_update(0, 0, sF, t, false, Dn, Vn);
+                            sDX[0] = make_float2(Dn, Vn);
+                        }
+                        lds_barrier<TPB>();
+                        if (wv == 0) {
+                            if (const_tree)
+                                run_list(2 * W, idle_fold4, fold_const);
+                            else
+                                run_list(2 * W, idle_fold, fold_task);
+                            if (lane == 0) {
+                                // the root: all its children (positions 1 .. nr) in order, loads
+                                // issued four at a time, then its solve
+                                const int nr = cell.root_children;
+                                float2 r0 = sDX[0];
+                                int c = 1;
+                                for (; c + 3 <= nr; c += 4) {
+                                    float2 a[4], u[4];
+    #pragma unroll
+                                    for (int k = 0; k < 4; k++) {
+                                        a[k] = sDX[c + k];
+                                        u[k] = sU[c + k];
+                                    }
+    #pragma unroll
+                                    for (int k = 0; k < 4; k++) {
+                                        const float r = const_tree ? a[k].x : u[k].y / a[k].x;
+                                        r0.x = r0.x - r * u[k].x;
+                                        r0.y = r0.y - r * a[k].y;
+                                    }
+                                }
+                                for (; c <= nr; c++) {
+                                    const float2 a = sDX[c], u = sU[c];
+                                    const float r = const_tree ? a.x : u.y / a.x;
+                                    r0.x = r0.x - r * u.x;
+                                    r0.y = r0.y - r * a.y;
+                                }
+                                r0.y = r0.y / r0.x;
+                                sDX[0] = r0;
+                            }
+                            run_list(2 * W + 1, idle_bwd, bwd_task);
+                        }
+                        lds_barrier<TPB>();
+                        run_list(2 * wv + 1, idle_bwd, bwd_task);
+                        lds_barrier<TPB>();
+                    }
+    #pragma unroll
+                    for (int j = 0; j < K; j++) {
+                        const int c = tid + TPB * j;
+                        if (slot_any[j] && c < N) {
+                            // the trace row of step s holds the voltage the step started from
+                            if (!init && rec[j] >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + rec[j]] = V[j];
+                            const float Vn = init ? V[j] : sDX[pos[j]].y;
+                            if (fl[j] & TCX_COMPLEX) {
+                                const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
+                                const float vthr = cx.k.vthr;
+                                if (V[j] < vthr && vthr < Vn) log_event(T, cx.k.ev, s);
+                            }
+                            V[j] = Vn;
+                            if (c < NV) sVn[c] = Vn;
+                        }
+                    }
+                }
+                // the next step reads voltages and open fractions written above by other lanes
+                if constexpr (SPLIT) {
+                    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+                } else {
+                    lds_barrier<TPB>();
+                }
+                cur = nxt;
+                fcur ^= 1;
+            }
         }
         if (cable) {
 #pragma unroll
@@ -628,7 +676,8 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     template __global__ void tree_kernel<TPB, SPLIT>(                                                              \
         DevTables, const DGate *, int, const DSynType *, int, const TreeCell *, const int32_t *, int32_t, int32_t *,  \
         const uint32_t *, const int32_t *, const float2 *, const uint32_t *, const uint32_t *, float *,             \
-        const uint32_t *, const uint32_t *, const uint32_t *, const TreeComplex *, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
+        const uint32_t *, const uint32_t *, const uint32_t *, const TreeComplex *, int32_t, int32_t, int32_t, int32_t, \
+        int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
 EDENHIP_TREE_INST(64, true)
 EDENHIP_TREE_INST(64, false)
 EDENHIP_TREE_INST(128, false)

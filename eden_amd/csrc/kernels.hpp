@@ -38,17 +38,20 @@ constexpr int TREE_SLOTS = 4;        // compartments per thread
 
 
 
-// Dynamic LDS of the tree kernel, in 4-byte units: sCX [cxmax] TreeComplex (only when the
-// records are staged in LDS) | sDX, sU [nmax+1] float2 | sIn [nmax] float2 (only with
-// Poisson inputs) | sV [nbuf][nmax+1] | sQ [ngmax] | sGT [ngmax] | sF [nbuf][nctmax] |
-// sCT [nctmax] | sIL [ninmax] | schedule block [words_max]; nbuf = 2 when a separate wave
-// runs the channel phase, else 1.
+// Dynamic LDS of the tree kernel, in 4-byte units: sDX, sU [nmax+1] float2 | sIn [nmax] float2
+// (only with Poisson inputs) | sV [nbuf][nvmax+1] | sQ [ngmax] | sGT [ngmax] | sF [nbuf][nctmax]
+// | sCT [nctmax] | sIL [ninmax] | schedule block [words_max] (16-byte aligned) | sCX [cxmax]
+// TreeComplex (only when the records are staged in LDS) | gate table [n_gates] DGate |
+// synapse types [n_stypes] DSynType; nbuf = 2 when a separate wave runs the channel phase,
+// else 1. nvmax: voltage slots of the bucket's cells (1 for a constant tree, whose only
+// voltage read from LDS is the root's, else its compartment count).
 struct TreeLds {
-    int32_t cx, dx, u, in, v, q, gt, f, ct, il, w, total;
+    int32_t dx, u, in, v, q, gt, f, ct, il, w, cx, gtab, stab, total;
 };
 
-__host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t nctmax, int32_t ninmax,
-                                            int32_t words_max, int32_t cxmax, bool split) {
+__host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t nvmax, int32_t ngmax, int32_t nctmax,
+                                            int32_t ninmax, int32_t words_max, int32_t cxmax, int32_t n_gates,
+                                            int32_t n_stypes, bool split) {
     // sDX first, at the start of dynamic LDS: its address is then a constant, which the
     // compiler folds into the offsets of the Hines solve's LDS instructions
     TreeLds L;
@@ -57,14 +60,16 @@ __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t ngmax, int32_t
     L.u = L.dx + 2 * (nmax + 1);
     L.in = L.u + 2 * (nmax + 1);
     L.v = L.in + (ninmax > 0 ? 2 * nmax : 0);
-    L.q = L.v + nbuf * (nmax + 1);
+    L.q = L.v + nbuf * (nvmax + 1);
     L.gt = L.q + ngmax;
     L.f = L.gt + ngmax;
     L.ct = L.f + 2 * nctmax;                // open fractions, double-buffered
     L.il = L.ct + nctmax;
-    L.w = (L.il + ninmax + 1) & ~1;         // 8-byte task words
+    L.w = (L.il + ninmax + 3) & ~3;         // task words, read up to 16 bytes at a time
     L.cx = (L.w + words_max + 3) & ~3;      // 16-byte records
-    L.total = L.cx + 20 * cxmax;
+    L.gtab = L.cx + 20 * cxmax;             // 32-byte DGate records (multiple of 4 units)
+    L.stab = L.gtab + 8 * n_gates;          // 32-byte DSynType records
+    L.total = L.stab + 8 * n_stypes;
     return L;
 }
 

@@ -2,6 +2,7 @@
 // and round schedules of the Hines solve, and the launch shape per cell size.
 #pragma once
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <queue>
 #include <stdint.h>
@@ -145,14 +146,18 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
 //   directory  [2 w] = wave w's fold list, [2 w + 1] = its backward list (w < waves),
 //              [2 waves] = J's fold list, [2 waves + 1] = J's backward list;
 //              a list is (index of its first round header | round count << 16)
-//   headers    one per round: (index of its first task word, even | task count << 16),
-//              task count 1..64, then five empty sentinel headers
-//   words      fold task, two words (even index): (8 pos | 8 first_child << 16), 8 second
-//              child (8 n, the dummy slot, for a task with one child): byte offsets of
-//              8-byte slots; backward task, two words: (8 pos | 8 solved_pos << 16), 8 mid
-//              with solved_pos the parent and mid = 8 n, or solved_pos the grandparent and
-//              mid the parent
-//   padding    128 zero words, so that a wave may read 64 task words from any round
+//   headers    one per round: (index of its first task word | task count << 16), task
+//              count 1..64, then five empty sentinel headers; every round starts at an
+//              index that is a multiple of the fold task's word count
+//   words      fold task, two words: (8 pos | 8 first_child << 16), 8 second child (8 n,
+//              the dummy slot, for a task with one child): byte offsets of 8-byte slots;
+//              with merge, four words: (8 pos | 8 c0 << 16), (8 c1 | 8 g00 << 16),
+//              (8 g01 | 8 g10 << 16), 8 g11, where g00, g01 are the children of c0 when the
+//              task folds them into c0 again (merged child), else 8 n, and g10, g11 those of
+//              c1; backward task, two words: (8 pos | 8 solved_pos << 16), 8 mid with
+//              solved_pos the parent and mid = 8 n, or solved_pos the grandparent and mid the
+//              parent
+//   padding    zero words (wave_plan_pad), so that a wave may read 64 tasks from any round
 // The five rounds after a list's last round are always headers (of the next lists or the
 // sentinels), so a wave may read up to five headers past its list.
 struct WavePlan {
@@ -160,16 +165,18 @@ struct WavePlan {
     int32_t n_junction = 0;            // nodes in J (the root included)
     int32_t hines_waves = 1;           // waves with non-empty lists
     int32_t longest = 0;               // rounds on the longest path (waves + J, both sweeps)
+    bool merged = false;               // four-word fold tasks (merge)
     std::vector<uint32_t> block;
 };
 
-constexpr int32_t WAVE_PLAN_PAD = 128;
+// zero words at the end of a block: 64 lanes reading one task each from the last round
+inline int32_t wave_plan_pad(bool merge) { return merge ? 256 : 128; }
 
 // first_bias, last_bias: initial loads of wave 0 and of the last wave, in shares of n / waves,
 // for waves with other work in the same phase (the tree kernel's root compartment update on
 // wave 0, gate updates on the last wave).
 inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves, double first_bias = 0.0,
-                                double last_bias = 0.0) {
+                                double last_bias = 0.0, bool merge = false) {
     WavePlan P;
     std::vector<std::vector<int32_t>> ch(n);
     for (int32_t i = 1; i < n; i++) ch[parent[i]].push_back(i);   // ascending
@@ -185,26 +192,47 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
     for (int32_t i = 1; i < n; i++) depth[i] = depth[parent[i]] + 1;
     constexpr int32_t LANES = 64;
 
-    // Fold levels of the nodes of one list (owner == o): level of fold task k of node i is
-    // 1 + max(level of task k - 1, last level of children 2k and 2k + 1); nodes outside the
-    // list (done in an earlier phase) and leaves count as level 0. Returns the last level of
-    // every node (0 for nodes without folds in the list) and the per-level task counts.
-    auto fold_levels = [&](const std::vector<int32_t> &owner, int32_t o, std::vector<int32_t> &last,
-                           std::vector<int32_t> &count) {
+    // Fold tasks of the nodes of one list (owner == o), children before parents: task k of
+    // node i folds children 2k and 2k + 1 and is placed at level 1 + max(level of task k - 1,
+    // last level of those children); nodes outside the list (done in an earlier phase) and
+    // leaves count as level 0. With merge, a child whose only fold task merged no child of its
+    // own may instead share the task's level: the task then folds that child's children into
+    // the child's value again in its own lane (the same operations on the same values as the
+    // child's own task, which reads and writes in the same round) and folds the result, so
+    // two tree levels take one round. emit(level, node, k, merged child 0, merged child 1).
+    auto fold_tasks = [&](const std::vector<int32_t> &owner, int32_t o, auto &&emit) {
+        std::vector<int32_t> last(n, 0);
+        std::vector<char> single(n, 0);
         for (int32_t k = n - 1; k >= 0; k--) {   // reverse BFS: children before parents
             const int32_t i = order[k];
             if (owner[i] != o || i == 0 || ch[i].empty()) continue;
             int32_t lv = 0;
+            bool merged_any = false;
             for (size_t c = 0; c < ch[i].size(); c += 2) {
-                int32_t in = lv;
-                for (size_t d = c; d < std::min(c + 2, ch[i].size()); d++)
-                    if (owner[ch[i][d]] == o) in = std::max(in, last[ch[i][d]]);
-                lv = in + 1;
-                if ((int32_t)count.size() <= lv) count.resize(lv + 1, 0);
-                count[lv]++;
+                const size_t e = std::min(c + 2, ch[i].size());
+                int32_t need = lv + 1;
+                for (size_t d = c; d < e; d++) {
+                    const int32_t x = ch[i][d];
+                    if (owner[x] == o) need = std::max(need, merge && single[x] ? last[x] : last[x] + 1);
+                }
+                bool m[2] = {false, false};
+                for (size_t d = c; d < e; d++) {
+                    const int32_t x = ch[i][d];
+                    m[d - c] = owner[x] == o && merge && single[x] && last[x] == need;
+                    merged_any = merged_any || m[d - c];
+                }
+                emit(need, i, (int32_t)(c / 2), m[0], m[1]);
+                lv = need;
             }
             last[i] = lv;
+            single[i] = ch[i].size() <= 2 && !merged_any;
         }
+    };
+    auto fold_levels = [&](const std::vector<int32_t> &owner, int32_t o, std::vector<int32_t> &count) {
+        fold_tasks(owner, o, [&](int32_t lv, int32_t, int32_t, bool, bool) {
+            if ((int32_t)count.size() <= lv) count.resize(lv + 1, 0);
+            count[lv]++;
+        });
     };
     auto count_rounds = [&](const std::vector<int32_t> &per_level) {
         int32_t r = 0;
@@ -231,8 +259,8 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
     auto evaluate = [&](const std::vector<int32_t> &owner) {
         int32_t wf = 0, wb = 0, jf = 0, jb = 0;
         for (int32_t o = -1; o < waves; o++) {
-            std::vector<int32_t> last(n, 0), cnt, bc(n, 0);
-            fold_levels(owner, o, last, cnt);
+            std::vector<int32_t> cnt, bc(n, 0);
+            fold_levels(owner, o, cnt);
             for (int32_t i = 1; i < n; i++)
                 if (owner[i] == o) bc[depth[i]]++;
             if (o < 0) {
@@ -310,52 +338,50 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
     for (int32_t i = 0; i < n; i++) P.n_junction += owner[i] < 0 ? 1 : 0;
 
     // task lists: per wave, folds by level and backward tasks by depth; then J
+    using Task = std::array<uint32_t, 4>;
+    const size_t fold_nw = merge ? 4 : 2, align = fold_nw;
     std::vector<uint32_t> words;
     std::vector<std::pair<uint32_t, uint32_t>> rounds;   // (first word, count)
     std::vector<std::pair<uint32_t, uint32_t>> lists;    // (first round, round count)
-    auto emit_list = [&](std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &levels, bool two) {
+    auto emit_list = [&](std::vector<std::vector<Task>> &levels, size_t nw) {
         const uint32_t r0 = (uint32_t)rounds.size();
         for (auto &tasks : levels)
             for (size_t k = 0; k < tasks.size(); k += LANES) {
                 const size_t cnt = std::min(tasks.size() - k, (size_t)LANES);
-                if (words.size() & 1) words.push_back(0);   // every round starts at an even index
+                while (words.size() % align) words.push_back(0);   // rounds start at aligned indices
                 rounds.emplace_back((uint32_t)words.size(), (uint32_t)cnt);
-                for (size_t t = k; t < k + cnt; t++) {
-                    words.push_back(tasks[t].first);
-                    if (two) words.push_back(tasks[t].second);
-                }
+                for (size_t t = k; t < k + cnt; t++) words.insert(words.end(), tasks[t].begin(), tasks[t].begin() + (long)nw);
             }
         lists.emplace_back(r0, (uint32_t)rounds.size() - r0);
     };
     const uint32_t dummy = 8u * (uint32_t)n;
-    auto add_folds = [&](int32_t o, std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &f) {
-        std::vector<int32_t> last(n, 0), cnt;
-        fold_levels(owner, o, last, cnt);
-        f.assign(cnt.size() + 1, {});
-        std::vector<int32_t> lv(n, 0);
-        for (int32_t k = n - 1; k >= 0; k--) {
-            const int32_t i = order[k];
-            if (owner[i] != o || i == 0 || ch[i].empty()) continue;
-            int32_t l = 0;
-            for (size_t c = 0; c < ch[i].size(); c += 2) {
-                int32_t in = l;
-                for (size_t d = c; d < std::min(c + 2, ch[i].size()); d++)
-                    if (owner[ch[i][d]] == o) in = std::max(in, lv[ch[i][d]]);
-                l = in + 1;
-                const uint32_t a = 8u * (uint32_t)P.pos[ch[i][c]];
-                const uint32_t b = c + 1 < ch[i].size() ? 8u * (uint32_t)P.pos[ch[i][c + 1]] : dummy;
-                f[l].push_back({8u * (uint32_t)P.pos[i] | (a << 16), b});
+    auto off = [&](int32_t i) { return 8u * (uint32_t)P.pos[i]; };
+    auto add_folds = [&](int32_t o, std::vector<std::vector<Task>> &f) {
+        f.clear();
+        fold_tasks(owner, o, [&](int32_t lv, int32_t i, int32_t k, bool m0, bool m1) {
+            if ((int32_t)f.size() <= lv) f.resize(lv + 1);
+            const size_t c = 2 * (size_t)k;
+            const int32_t a = ch[i][c], b = c + 1 < ch[i].size() ? ch[i][c + 1] : -1;
+            auto grand = [&](bool m, int32_t x, size_t g) {
+                return m && g < ch[x].size() ? off(ch[x][g]) : dummy;
+            };
+            const uint32_t bo = b >= 0 ? off(b) : dummy;
+            if (!merge) {
+                f[lv].push_back({off(i) | (off(a) << 16), bo, 0u, 0u});
+            } else {
+                f[lv].push_back({off(i) | (off(a) << 16), bo | (grand(m0, a, 0) << 16),
+                                 grand(m0, a, 1) | ((b >= 0 ? grand(m1, b, 0) : dummy) << 16),
+                                 b >= 0 ? grand(m1, b, 1) : dummy});
             }
-            lv[i] = l;
-        }
+        });
         // position order within a level
-        for (auto &t : f) std::stable_sort(t.begin(), t.end(), [](auto &x, auto &y) { return (x.first & 0xFFFFu) < (y.first & 0xFFFFu); });
+        for (auto &t : f) std::stable_sort(t.begin(), t.end(), [](const Task &x, const Task &y) { return (x[0] & 0xFFFFu) < (y[0] & 0xFFFFu); });
     };
     // backward tasks by depth; two consecutive depths share a round when they have at most 64
     // tasks together: a node of the second depth whose parent is in the first solves that
     // parent again from the grandparent (the same operations on the same values) before
     // itself, so the round reads only values solved in earlier rounds
-    auto add_bwd = [&](int32_t o, std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &b) {
+    auto add_bwd = [&](int32_t o, std::vector<std::vector<Task>> &b) {
         int32_t Dm = 0;
         for (int32_t i = 0; i < n; i++) Dm = std::max(Dm, depth[i]);
         std::vector<std::vector<int32_t>> lvl(Dm + 2);
@@ -364,19 +390,16 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
             if (owner[i] == o && i > 0) lvl[depth[i]].push_back(i);
         }
         b.clear();
-        auto one = [&](int32_t i) {
-            return std::make_pair(8u * (uint32_t)P.pos[i] | ((8u * (uint32_t)P.pos[parent[i]]) << 16), dummy);
-        };
+        auto one = [&](int32_t i) { return Task{off(i) | (off(parent[i]) << 16), dummy, 0u, 0u}; };
         for (int32_t d = 1; d <= Dm; d++) {
             if (lvl[d].empty()) continue;
-            std::vector<std::pair<uint32_t, uint32_t>> t;
+            std::vector<Task> t;
             for (int32_t i : lvl[d]) t.push_back(one(i));
             if (!lvl[d + 1].empty() && lvl[d].size() + lvl[d + 1].size() <= (size_t)LANES) {
                 for (int32_t i : lvl[d + 1]) {
                     const int32_t q = parent[i];
                     if (owner[q] == o)
-                        t.push_back({8u * (uint32_t)P.pos[i] | ((8u * (uint32_t)P.pos[parent[q]]) << 16),
-                                     8u * (uint32_t)P.pos[q]});
+                        t.push_back({off(i) | (off(parent[q]) << 16), off(q), 0u, 0u});
                     else
                         t.push_back(one(i));
                 }
@@ -387,22 +410,23 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
     };
     for (int32_t w = 0; w <= waves; w++) {
         const int32_t o = w < waves ? w : -1;
-        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> f, b;
+        std::vector<std::vector<Task>> f, b;
         add_folds(o, f);
-        emit_list(f, true);
+        emit_list(f, fold_nw);
         add_bwd(o, b);
-        emit_list(b, true);
+        emit_list(b, 2);
     }
-    if (words.size() & 1) words.push_back(0);
+    while (words.size() % align) words.push_back(0);
     for (int k = 0; k < 5; k++) rounds.emplace_back((uint32_t)words.size(), 0u);   // sentinels
     const uint32_t nd = 2 * (uint32_t)waves + 2;
     uint32_t nh = (uint32_t)rounds.size();
-    if ((nd + nh) & 1) nh++;   // task words start at an even index
+    while ((nd + nh) % align) nh++;   // task words start at an aligned index
     for (auto &l : lists) P.block.push_back((nd + l.first) | (l.second << 16));
     for (auto &r : rounds) P.block.push_back((nd + nh + r.first) | (r.second << 16));
-    if (P.block.size() < nd + nh) P.block.push_back(0u);
+    while (P.block.size() < nd + nh) P.block.push_back(0u);
     P.block.insert(P.block.end(), words.begin(), words.end());
-    P.block.insert(P.block.end(), WAVE_PLAN_PAD, 0u);
+    P.block.insert(P.block.end(), (size_t)wave_plan_pad(merge), 0u);
+    P.merged = merge;
     return P;
 }
 

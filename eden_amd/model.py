@@ -323,7 +323,7 @@ class Simulation:
 
     def kernel_infos(self):
         """Per-kernel launch statistics: list of dicts (name, threads_per_block, grid, cells,
-        comps, launches, total_ms, cell_index array)."""
+        comps, lds_bytes, launches, total_ms, cell_index array)."""
         out = []
         for i in range(self.lib.edenhip_kernel_count(self.h)):
             ki = N.KernelInfo()
@@ -331,7 +331,8 @@ class Simulation:
             cells = np.empty(max(1, ki.n_cells), np.int32)
             self._check(self.lib.edenhip_get_kernel_cells(self.h, i, N.ptr(cells), len(cells)), "get_kernel_cells")
             out.append(dict(name=ki.name.decode(), threads_per_block=ki.threads_per_block, grid=ki.grid,
-                            cells=ki.n_cells, comps=ki.n_comps, launches=ki.launches, total_ms=ki.total_ms,
+                            cells=ki.n_cells, comps=ki.n_comps, lds_bytes=ki.lds_bytes, launches=ki.launches,
+                            total_ms=ki.total_ms,
                             cell_index=cells[: ki.n_cells]))
         return out
 

@@ -235,7 +235,7 @@ typedef struct edenhip_kernel_info {
     int32_t threads_per_block;
     int32_t grid;                /* workgroups per launch */
     int32_t n_cells;             /* cells this kernel advances */
-    int32_t reserved;
+    int32_t lds_bytes;           /* dynamic LDS per workgroup */
     int64_t n_comps;             /* compartments of those cells */
     int64_t launches;
     double total_ms;

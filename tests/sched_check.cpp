@@ -56,14 +56,21 @@ static std::vector<int32_t> make_parents(std::mt19937 &rng, int n, int shape) {
 // other in reverse order, and a read of a node that another wave writes in the same phase is
 // an error (so no order of the waves could change the result). Every node must receive its
 // children in ascending order, each exactly once.
+//
+// With merge (the constant-tree form), the fold lists are replayed twice, as the kernel runs
+// them: once for the eliminated diagonals (the set-up pre-sweep), then, with the ratios
+// Uj / D' fixed, for the right-hand sides; a merged child's value is formed again from the
+// grandchildren in the parent's task, from the values at the round's start.
 static int check_waves(int n, const std::vector<int32_t> &par, const std::vector<std::vector<int>> &ch,
                        const std::vector<float> &D0, const std::vector<float> &X0, const std::vector<float> &ui,
                        const std::vector<float> &uj, const std::vector<float> &Dr, const std::vector<float> &Xr,
-                       int waves, int trial, int shape) {
-    const WavePlan P = plan_tree_waves(n, par.data(), waves);
+                       int waves, int trial, int shape, bool merge) {
+    const WavePlan P = plan_tree_waves(n, par.data(), waves, 0.0, 0.0, merge);
     const std::vector<uint32_t> &B = P.block;
-    int bad = 0;
-    std::vector<float> dx(n + 1), dv(n + 1), su(n + 1), sj(n + 1);
+    int bad = P.merged == merge ? 0 : 1;
+    const int fnw = merge ? 4 : 2;        // words per fold task; rounds start at multiples of it
+    const int reach = 64 * fnw;           // words a wave may read from any round's start
+    std::vector<float> dx(n + 1), dv(n + 1), su(n + 1), sj(n + 1), rat(n + 1, 1.0f);
     for (int i = 0; i < n; i++) {
         dx[P.pos[i]] = D0[i];
         dv[P.pos[i]] = X0[i];
@@ -84,15 +91,17 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
         dv[p] = dv[p] - r * dv[c];
         if (c != n) got[p].push_back(c);
     };
-    auto run = [&](int dir, int wave, bool fwd, bool owner_check) -> bool {
+    enum Kind { FOLD, DIAG, RHS, BWD };
+    auto run = [&](int dir, int wave, Kind kind, bool owner_check) -> bool {
         if (dir >= (int)B.size()) return false;
         const int first = B[dir] & 0xFFFF, cnt = B[dir] >> 16;
         if (cnt > 0 && (first < nd || first + cnt + 5 > (int)B.size())) return false;
+        const int nw = kind == BWD || !merge ? 2 : 4;
         for (int r = 0; r < cnt; r++) {
             const uint32_t h = B[first + r];
             const int off = h & 0xFFFF, tc = h >> 16;
-            // every lane reads a (two-word) task word from the round: within the block
-            if (tc < 1 || tc > 64 || off + 128 > (int)B.size() || (off & 1)) return false;
+            // every lane reads a task from the round: within the block, aligned
+            if (tc < 1 || tc > 64 || off + reach > (int)B.size() || (off % fnw)) return false;
             // reads of the whole round first
             struct T { int p, a, b; float ox, oy; };
             std::vector<T> tasks;
@@ -102,17 +111,33 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
                 auto touch = [&](int x) {
                     if (owner_check && x != n && writer[x] >= 0 && writer[x] != wave) bad++;
                 };
-                if (fwd) {
-                    const uint32_t w0 = B[off + 2 * t], w1 = B[off + 2 * t + 1];
-                    const int p = slot(w0 & 0xFFFF, ok), a = slot(w0 >> 16, ok), b = slot(w1, ok);
+                const uint32_t *w = &B[off + nw * t];
+                if (kind == FOLD) {
+                    const int p = slot(w[0] & 0xFFFF, ok), a = slot(w[0] >> 16, ok), b = slot(w[1], ok);
                     if (!ok || p == n || a == n || p == 0) return false;
                     touch(p); touch(a); touch(b);
                     // the kernel's two-child form: ((D - r0 Ui0) - r1 Ui1), dummy b gives 0
                     const float r0 = sj[a] / dx[a], r1 = sj[b] / dx[b];
                     tasks.push_back({p, a, b, (dx[p] - r0 * su[a]) - r1 * su[b], (dv[p] - r0 * dv[a]) - r1 * dv[b]});
+                } else if (kind == DIAG || kind == RHS) {
+                    const int p = slot(w[0] & 0xFFFF, ok), a = slot(w[0] >> 16, ok), b = slot(w[1] & 0xFFFF, ok);
+                    const int a0 = slot(w[1] >> 16, ok), a1 = slot(w[2] & 0xFFFF, ok);
+                    const int b0 = slot(w[2] >> 16, ok), b1 = slot(w[3], ok);
+                    if (!ok || p == n || a == n || p == 0) return false;
+                    for (int x : {p, a, b, a0, a1, b0, b1}) touch(x);
+                    if (kind == DIAG) {
+                        auto dprime = [&](int c, int g0, int g1) {
+                            return (dx[c] - (sj[g0] / dx[g0]) * su[g0]) - (sj[g1] / dx[g1]) * su[g1];
+                        };
+                        const float da = dprime(a, a0, a1), db = dprime(b, b0, b1);
+                        tasks.push_back({p, a, b, (dx[p] - (sj[a] / da) * su[a]) - (sj[b] / db) * su[b], dv[p]});
+                    } else {
+                        auto rhs = [&](int c, int g0, int g1) { return (dv[c] - rat[g0] * dv[g0]) - rat[g1] * dv[g1]; };
+                        const float xa = rhs(a, a0, a1), xb = rhs(b, b0, b1);
+                        tasks.push_back({p, a, b, dx[p], (dv[p] - rat[a] * xa) - rat[b] * xb});
+                    }
                 } else {
-                    const uint32_t w0 = B[off + 2 * t], w1 = B[off + 2 * t + 1];
-                    const int p = slot(w0 & 0xFFFF, ok), sp = slot(w0 >> 16, ok), m = slot(w1, ok);
+                    const int p = slot(w[0] & 0xFFFF, ok), sp = slot(w[0] >> 16, ok), m = slot(w[1], ok);
                     if (!ok || p == n || sp == n || p == 0) return false;
                     touch(p); touch(sp); touch(m);
                     if (solved[sp] == 0) bad++;   // the solved node it starts from is not solved yet
@@ -130,10 +155,10 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
             for (auto &t : tasks) {
                 dx[t.p] = t.ox;
                 dv[t.p] = t.oy;
-                if (fwd) {
+                if (kind == FOLD || kind == RHS) {
                     got[t.p].push_back(t.a);
                     if (t.b != n) got[t.p].push_back(t.b);
-                } else {
+                } else if (kind == BWD) {
                     solved[t.p]++;
                 }
             }
@@ -143,25 +168,26 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
         return true;
     };
     bool ok = (int)B.size() >= nd + 1;
-    // every header a wave may read (its lists' and five past them) points at even offsets
-    // with 128 words of block after them
+    // every header a wave may read (its lists' and five past them) points at an aligned
+    // offset with a full round's reach of block after it
     for (int d = 0; d < nd && ok; d++) {
         const int first = B[d] & 0xFFFF, cnt = B[d] >> 16;
         for (int r = 0; r < cnt + 5 && ok; r++) {
             if (first + r >= (int)B.size()) { ok = false; break; }
             const int off = B[first + r] & 0xFFFF;
-            if ((off & 1) || off + 128 > (int)B.size()) ok = false;
+            if ((off % fnw) || off + reach > (int)B.size()) ok = false;
         }
     }
     // the nodes each wave writes in a phase, from its list (before any replay)
     auto owners = [&](int odd) {
+        const int nw = odd ? 2 : fnw;
         std::fill(writer.begin(), writer.end(), -1);
         for (int w = 0; w < waves && ok; w++) {
             const int first = B[2 * w + odd] & 0xFFFF, cnt = B[2 * w + odd] >> 16;
             for (int r = 0; r < cnt && ok; r++) {
                 const uint32_t h = B[first + r];
                 for (int t = 0; t < (int)(h >> 16); t++) {
-                    const int p = (int)((B[(h & 0xFFFF) + 2 * t] & 0xFFFF) / 8);
+                    const int p = (int)((B[(h & 0xFFFF) + nw * t] & 0xFFFF) / 8);
                     if (p > n || (writer[p] >= 0 && writer[p] != w)) bad++;
                     writer[p] = w;
                 }
@@ -169,18 +195,36 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
         }
     };
     owners(0);
-    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w, w, true, true);
+    if (merge) {
+        // set-up pre-sweep: eliminated diagonals below the root, then the fixed ratios
+        for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w, w, DIAG, true);
+        std::vector<int> keep = writer;
+        std::fill(writer.begin(), writer.end(), -1);
+        if (ok) ok = run(2 * waves, 0, DIAG, false);
+        writer = keep;
+        for (int i = 1; i < n; i++) rat[P.pos[i]] = sj[P.pos[i]] / dx[P.pos[i]];
+    }
+    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w, w, merge ? RHS : FOLD, true);
     std::fill(writer.begin(), writer.end(), -1);
-    if (ok) ok = run(2 * waves, 0, true, false);
+    if (ok) ok = run(2 * waves, 0, merge ? RHS : FOLD, false);
     // the root fold: every child of the root (positions 1 .. nc) in order, then the solve
-    for (size_t k = 0; k < ch[0].size() && ok; k++) fold(0, P.pos[ch[0][k]]);
+    for (size_t k = 0; k < ch[0].size() && ok; k++) {
+        const int c = P.pos[ch[0][k]];
+        if (merge) {
+            dx[0] = dx[0] - rat[c] * su[c];
+            dv[0] = dv[0] - rat[c] * dv[c];
+            got[0].push_back(c);
+        } else {
+            fold(0, c);
+        }
+    }
     dv[0] = dv[0] / dx[0];
     solved[0] = 1;
-    if (ok) ok = run(2 * waves + 1, 0, false, false);
+    if (ok) ok = run(2 * waves + 1, 0, BWD, false);
     if (ok) owners(1);
-    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w + 1, w, false, true);
+    for (int w = waves - 1; w >= 0 && ok; w--) ok = run(2 * w + 1, w, BWD, true);
     if (!ok) {
-        std::printf("trial %d shape %d n %d waves %d: malformed block\n", trial, shape, n, waves);
+        std::printf("trial %d shape %d n %d waves %d merge %d: malformed block\n", trial, shape, n, waves, merge);
         return bad + 1;
     }
     for (int i = 0; i < n; i++) {
@@ -188,14 +232,14 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
         std::vector<int> want;
         for (int c : ch[i]) want.push_back(P.pos[c]);
         if (got[p] != want) {
-            std::printf("trial %d shape %d n %d waves %d: node %d children folded out of order\n", trial, shape, n,
-                        waves, i);
+            std::printf("trial %d shape %d n %d waves %d merge %d: node %d children folded out of order\n", trial,
+                        shape, n, waves, merge, i);
             bad++;
         }
         if (solved[p] != 1) bad++;
         if (std::memcmp(&dv[p], &Xr[i], 4) != 0 || std::memcmp(&dx[p], &Dr[i], 4) != 0) {
-            std::printf("trial %d shape %d n %d waves %d: node %d gives %.9g, serial %.9g\n", trial, shape, n, waves,
-                        i, dv[p], Xr[i]);
+            std::printf("trial %d shape %d n %d waves %d merge %d: node %d gives %.9g, serial %.9g\n", trial, shape,
+                        n, waves, merge, i, dv[p], Xr[i]);
             bad++;
             break;
         }
@@ -251,10 +295,11 @@ int main(int argc, char **argv) {
             const int i = order[k], j = par[i];
             Xr[i] = (Xr[i] - ui[i] * Xr[j]) / Dr[i];
         }
-        for (int waves : {1, 2, 4, 8}) {
-            runs++;
-            bad += check_waves(n, par, ch, D0, X0, ui, uj, Dr, Xr, waves, trial, shape);
-        }
+        for (int waves : {1, 2, 4, 8})
+            for (bool merge : {false, true}) {
+                runs++;
+                bad += check_waves(n, par, ch, D0, X0, ui, uj, Dr, Xr, waves, trial, shape, merge);
+            }
         for (int tpb : {64, 128, 256, 512}) {
             runs++;
             const TreePlan P = plan_tree(n, par.data(), tpb);
