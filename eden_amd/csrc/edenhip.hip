@@ -450,11 +450,13 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         }
         tc.n_input = (int32_t)(inl.size() - tc.input_off);
         // complex compartments: anything but one ungated channel with no pulse, no Poisson
-        // input and no spike event column; their constants go to a per-cell record table
+        // input, no spike event column and no trace column; their constants go to a per-cell
+        // record table
         tc.cx_off = (int32_t)cxv.size();
         for (int32_t i = 0; i < n; i++) {
             const int64_t g = c0 + i;
-            const bool simple = (tcm[g] & 0xFFu) == 1u && pulse_begin[g + 1] == pulse_begin[g] && ev_col[g] < 0;
+            const bool simple =
+                (tcm[g] & 0xFFu) == 1u && pulse_begin[g + 1] == pulse_begin[g] && ev_col[g] < 0 && rec_col[g] < 0;
             if (simple) continue;
             TreeComplex x{};
             x.k.C = C[g]; x.k.A = A[g]; x.k.invrc = invrc[g]; x.k.vthr = vthr[g];

@@ -311,8 +311,9 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         // Slots beyond the cell behave as simple compartments with no channel conductance,
         // C = 1 and the dummy position N, so they compute exactly the dummy slot's (1, 0).
         float V[K], g0[K], e0[K], Ar[K], Cr[K], Gs[K], Ds[K];
-        int pos[K], rec[K];
+        // fl: the compartment's flag word (TCX_COMPLEX | record index, or 0) | BFS position << 16
         uint32_t fl[K];
+        auto posof = [&](int j) { return (int)((fl[j] >> 16) & 0x7FFu); };
         bool slot_any[K], slot_simple[K];   // per wave and slot: any valid lane, no complex lane
         const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
 #pragma unroll
@@ -320,13 +321,11 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             const int c = tid + TPB * j;
             const bool valid = cable && c < N;
             V[j] = 0.0f; g0[j] = 0.0f; e0[j] = 0.0f; Ar[j] = 0.0f; Cr[j] = 1.0f; Gs[j] = 0.0f; Ds[j] = 1.0f;
-            pos[j] = N; rec[j] = -1; fl[j] = 0;
+            fl[j] = (uint32_t)N << 16;
             if (valid) {
                 const int64_t g = c0 + c;
-                fl[j] = cflag_g[g];
+                fl[j] = cflag_g[g] | ((uint32_t)pos_g[g] << 16);
                 V[j] = T.V[g];
-                pos[j] = pos_g[g];
-                rec[j] = T.rec_col[g];
                 if (!(fl[j] & TCX_COMPLEX)) {
                     g0[j] = T.chg[g * NCH];
                     e0[j] = T.che[g * NCH];
@@ -362,7 +361,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const float Dd = 1 * (G_int / k.C) + k.invrc;
                         Dr = 1 + Dd * dt;
                     }
-                    sDX[pos[j]] = make_float2(Dr, 0.0f);
+                    sDX[posof(j)] = make_float2(Dr, 0.0f);
                 }
             }
             __syncthreads();
@@ -395,10 +394,10 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 const int c = tid + TPB * j;
-                if (cable && c < N && pos[j] != 0) {
-                    const float Dp = sDX[pos[j]].x;
-                    Ds[j] = sU[pos[j]].y / Dp;
-                    sU[pos[j]].y = Dp;
+                if (cable && c < N && posof(j) != 0) {
+                    const float Dp = sDX[posof(j)].x;
+                    Ds[j] = sU[posof(j)].y / Dp;
+                    sU[posof(j)].y = Dp;
                 }
             }
             // dummy slot: ratio 1 (times right-hand side 0), D' = 1 for the idle backward task
@@ -428,7 +427,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             const float I_int = I_ch + I_in + I_syn;
             const float G_int = G_ch + G_in + G_syn;
             const float Dd = 1 * (G_int / k.C) + k.invrc;
-            Dn = (const_tree && pos[j] != 0) ? Ds[j] : 1 + Dd * dt;
+            Dn = (const_tree && posof(j) != 0) ? Ds[j] : 1 + Dd * dt;
             Vn = init ? V[j] : V[j] + (dt * (I_int + ((V[j] * G_int)))) / k.C;
         };
         int cur = 0, fcur = 0;
@@ -502,7 +501,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         float Dn = Ds[j];
                         float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
                         if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, Dn, Vn);
-                        sDX[pos[j]] = make_float2(Dn, Vn);
+                        sDX[posof(j)] = make_float2(Dn, Vn);
                     }
                     lds_barrier<TPB>();
                     if constexpr (!SPLIT) {
@@ -630,11 +629,13 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     for (int j = 0; j < K; j++) {
                         const int c = tid + TPB * j;
                         if (slot_any[j] && c < N) {
-                            // the trace row of step s holds the voltage the step started from
-                            if (!init && rec[j] >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + rec[j]] = V[j];
-                            const float Vn = init ? V[j] : sDX[pos[j]].y;
+                            const float Vn = init ? V[j] : sDX[posof(j)].y;
                             if (fl[j] & TCX_COMPLEX) {
                                 const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
+                                // a recorded compartment is complex: its trace column is in its
+                                // record; the trace row of step s holds the voltage the step
+                                // started from
+                                if (!init && cx.k.rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + cx.k.rec] = V[j];
                                 const float vthr = cx.k.vthr;
                                 if (V[j] < vthr && vthr < Vn) log_event(T, cx.k.ev, s);
                             }

@@ -26,8 +26,9 @@ struct TreeCell {
 };
 
 // Per-compartment flag word of tree cells: 0 for a simple compartment (one ungated channel,
-// no pulse, no Poisson input, no spike event column), else TCX_COMPLEX | index of its
-// TreeComplex record within the cell.
+// no pulse, no Poisson input, no spike event column, not recorded), else TCX_COMPLEX | index
+// of its TreeComplex record within the cell (bits 0-15; the kernel keeps the compartment's
+// position in bits 16-26).
 constexpr uint32_t TCX_COMPLEX = 1u << 31;
 
 // Compartment word of the tree kernel: channel count (bits 0-2), gated-channel-slot mask
