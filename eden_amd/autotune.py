@@ -6,7 +6,9 @@ the whole run), timed from the host around `edenhip_run` after one warm-up run; 
 rewound between runs. Short runs are cheaper but their timings need not carry over to the
 whole run, where the buckets' tails overlap differently. The
 search is coordinate-wise over the three movable limits, from the size rule (256, 512, 1024):
-a limit moves down when the larger workgroup is faster for the cells between the two values.
+a limit moves down when the larger workgroup is faster for the cells between the two values
+by more than `margin` (default 2 %: whole runs of one context vary by about 1 % from run to
+run, so a smaller gain is not told apart from noise and the present limits are kept).
 
     from eden_amd.autotune import autotune
     limits, report = autotune(model)
@@ -55,7 +57,7 @@ def time_limits(model, limits, steps=None, repeats=2, device=0):
     return best
 
 
-def autotune(model, steps=None, repeats=2, device=0, verbose=False):
+def autotune(model, steps=None, repeats=2, device=0, verbose=False, margin=0.02):
     """Returns (limits, report): the fastest size limits found and every timing taken."""
     if model.n_comps == model.n_cells:
         return DEFAULT, []          # only one-compartment cells: nothing to tune
@@ -80,6 +82,6 @@ def autotune(model, steps=None, repeats=2, device=0, verbose=False):
             if any(trial[i] > trial[i + 1] for i in range(3)):
                 continue
             t = measure(trial)
-            if t < best_t:
+            if t < best_t * (1.0 - margin):
                 best, best_t = trial, t
     return tuple(best), report

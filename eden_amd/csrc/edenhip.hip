@@ -190,12 +190,15 @@ static std::vector<double> row_times(float dt, double t_final) {
 static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point);
 
 // Streams come from a per-device pool created once per process and kept until it ends:
-// slot 0 is a context's main stream, slots 1-4 its tree buckets' streams. The runtime binds
-// each stream to one of a few hardware queues when the stream is created, so streams made
-// afresh for every context land on different queues from one context to the next (measured:
-// the same C4 run took 656 ms in a process's first context, 725 ms in later ones). Pool
-// streams keep one binding for every context. Contexts of one device share the pool's
-// streams, so their work is ordered on them (one host thread per context, as documented).
+// slot 0 is a context's main stream, which also carries its first (largest) tree bucket, and
+// slots 1-3 the other buckets' streams. The runtime binds each stream to one of a few
+// hardware queues (4 by default) when the stream is created, so streams made afresh for
+// every context land on different queues from one context to the next (measured: the same
+// C4 run took 656 ms in a process's first context, 725 ms in later ones), and two streams
+// that share a queue run their kernels one after the other (measured with five streams: the
+// 64-thread bucket waited for the 128-thread one, 496 ms per C4 run). Four pool streams keep
+// one binding each for every context. Contexts of one device share the pool's streams, so
+// their work is ordered on them (one host thread per context, as documented).
 static int pool_stream(edenhip_ctx *ctx, int slot, hipStream_t *out) {
     static std::mutex mu;
     static std::vector<std::vector<hipStream_t>> pools;
@@ -203,7 +206,7 @@ static int pool_stream(edenhip_ctx *ctx, int slot, hipStream_t *out) {
     if ((int)pools.size() <= ctx->device) pools.resize((size_t)ctx->device + 1);
     auto &p = pools[(size_t)ctx->device];
     if (p.empty()) {
-        p.assign(5, nullptr);
+        p.assign(4, nullptr);
         for (auto &st : p) HIPCHECK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     }
     *out = p[(size_t)slot];
@@ -696,7 +699,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(upload(ctx, &b.d_cells, bc));
         TRY(upload(ctx, &b.d_order, seq));
         TRY(alloc_zero(ctx, &b.d_queue, 2));
-        TRY(pool_stream(ctx, 1 + (int)(&b - ctx->buckets.data()), &b.stream));
+        // the first bucket on the main stream (the fork and join events below are then
+        // recorded and waited on the same stream, which orders nothing more)
+        TRY(pool_stream(ctx, (int)(&b - ctx->buckets.data()), &b.stream));
         HIPCHECK(ctx, hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
         if (ctx->opt.profile) {
             HIPCHECK(ctx, hipEventCreate(&b.p0));

@@ -236,16 +236,16 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         }
         // Hines task lists (tree_schedule.hpp, plan_tree_waves): cable wave wv runs the lists
         // of its subtrees, wave 0 also those of the junction. run_list calls body(word) round
-        // after round with no barrier (a wave's LDS accesses are performed in order); lanes at
-        // or beyond a round's task count take the idle word, which points every access at the
-        // dummy slot N, so a round is straight-line code. Task words hold byte offsets into
-        // sDX, which starts at a constant LDS address. Every lane reads a word from the round
-        // (the block ends with padding for the lanes beyond a round's count). The loop is
-        // unrolled twice, so that no register moves between rounds: after a round's body is
-        // issued, the words of the round after next and the header of the round two after
-        // that are read into the registers the round had used (five sentinel headers end the
-        // block). Each value is then complete before its use, with no wait on the loads just
-        // issued.
+        // after round with no barrier (a wave's LDS accesses are performed in order), on the
+        // lanes below the round's task count; lanes beyond it are masked off (their word is
+        // the idle word, which points every access at the dummy slot N). Task words hold byte
+        // offsets into sDX, which starts at a constant LDS address. Every lane reads a word
+        // from the round (the block ends with padding for the lanes beyond a round's count).
+        // The loop is unrolled twice, so that no register moves between rounds: after a
+        // round's body is issued, the words of the round after next and the header of the
+        // round two after that are read into the registers the round had used (five sentinel
+        // headers end the block). Each value is then complete before its use, with no wait on
+        // the loads just issued.
         constexpr int W = TPB / 64;
         const int wv = tid >> 6, lane = tid & 63;
         const uint32_t dummy_off = 8u * (uint32_t)N;
@@ -261,19 +261,21 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             using Word = decltype(idle);
             const uint32_t list = __builtin_amdgcn_readfirstlane(sW[dir]);
             const int first = list & 0xFFFFu, cnt = list >> 16;
-            auto word = [&](uint32_t h) {
+            auto word = [&](uint32_t h, bool &act) {
                 const Word w = reinterpret_cast<const Word *>(sW + (h & 0xFFFFu))[lane];
-                return lane < (int)(h >> 16) ? w : idle;
+                act = lane < (int)(h >> 16);
+                return act ? w : idle;
             };
+            bool aa, ab;
             uint32_t ha = sW[first + 2], hb = sW[first + 3];
-            Word wa = word(sW[first]), wb = word(sW[first + 1]);
+            Word wa = word(sW[first], aa), wb = word(sW[first + 1], ab);
             for (int r = 0; r < cnt; r += 2) {
-                body(wa);
-                wa = word(ha);
+                if (aa) body(wa);
+                wa = word(ha, aa);
                 ha = sW[first + r + 4];
                 if (r + 1 < cnt) {
-                    body(wb);
-                    wb = word(hb);
+                    if (ab) body(wb);
+                    wb = word(hb, ab);
                     hb = sW[first + r + 5];
                 }
             }

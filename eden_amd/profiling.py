@@ -5,9 +5,13 @@ what the same counters read for a plain device copy of known size.
     python -m eden_amd.profiling stats <run_kernel_stats.csv> <title> <command...>
     python -m eden_amd.profiling traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
                                          <copy MiB> <kernel-substring> <workload> <command...>
+    python -m eden_amd.profiling counters <counters.csv> <title> <command...>
 
 The traffic summary ends with one line `traffic_bytes_per_launch: <kernel> | <workload> |
-<bytes>` that bench.py reads for its roofline's `traffic`.
+<bytes>` per kernel whose name contains the substring; bench.py reads the one of its dominant
+kernel and workload for its roofline's `traffic`. The counters summary gives the mean per
+dispatch of every counter in a --pmc run, per kernel, with the shares that the SQ counters
+imply (LDS bank-conflict cycles over LDS-active cycles, waiting over wave cycles).
 """
 import csv
 import re
@@ -26,6 +30,18 @@ def kernel_stats(path, title, command):
     for r in rows:
         lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                      f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} |")
+    # per-dispatch durations of this library's kernels, from the kernel trace beside the stats
+    trace = path.replace("kernel_stats.csv", "kernel_trace.csv")
+    try:
+        disp = [r for r in csv.DictReader(open(trace)) if "edenhip" in r["Kernel_Name"]]
+    except OSError:
+        disp = []
+    if disp:
+        lines += ["", "Dispatches in order of start (ms):", "", "| kernel | start | duration |", "|---|---|---|"]
+        t0 = min(int(r["Start_Timestamp"]) for r in disp)
+        for r in sorted(disp, key=lambda r: int(r["Start_Timestamp"])):
+            st, en = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            lines.append(f"| {_short(r['Kernel_Name'])} | {(st - t0) / 1e6:.1f} | {(en - st) / 1e6:.3f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -63,15 +79,44 @@ def traffic(fetch, write, cfetch, cwrite, copy_mib, kernel_sub, workload, comman
              "Per kernel: the mean over its dispatches.", "",
              "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes (calibrated) |",
              "|---|---|---|---|---|"]
-    chosen = None
+    chosen = []
     for n in sorted(fk):
         t = (fk[n] * kf + wk.get(n, 0.0) * kw) * 1024.0
         lines.append(f"| {_short(n)} | {nf[n]} | {fk[n]:.0f} | {wk.get(n, 0.0):.0f} | {t:.4g} |")
         if kernel_sub in n:
-            chosen = (_short(n), t)
+            chosen.append((_short(n), t))
     lines.append("")
-    if chosen:
-        lines.append(f"traffic_bytes_per_launch: {chosen[0]} | {workload} | {chosen[1]:.6g}")
+    for k, t in chosen:
+        lines.append(f"traffic_bytes_per_launch: {k} | {workload} | {t:.6g}")
+    return "\n".join(lines) + "\n"
+
+
+def counters(path, title, command):
+    acc, disp = {}, {}
+    for r in csv.DictReader(open(path)):
+        n = r["Kernel_Name"]
+        if "edenhip" not in n:
+            continue
+        d = acc.setdefault(n, {})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        disp.setdefault(n, set()).add(r["Dispatch_Id"])
+    names = sorted({c for d in acc.values() for c in d})
+    lines = [f"# {title}", "", "Command:", "", "```", command, "```", "",
+             "Mean per dispatch (summed over the device's shader engines).", "",
+             "| kernel | dispatches | " + " | ".join(names) + " |", "|---|---|" + "---|" * len(names)]
+    for n in sorted(acc):
+        k = len(disp[n])
+        lines.append(f"| {_short(n)} | {k} | " + " | ".join(f"{acc[n].get(c, 0.0) / k:.4g}" for c in names) + " |")
+    share = [("SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE", "LDS bank-conflict cycles / LDS-active cycles"),
+             ("SQ_WAIT_ANY", "SQ_WAVE_CYCLES", "wave cycles waiting / wave cycles"),
+             ("SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES", "wave cycles waiting on LDS instructions / wave cycles"),
+             ("SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "VALU-active cycles / wave cycles")]
+    rows = [(a, b, t) for a, b, t in share if a in names and b in names]
+    if rows:
+        lines += ["", "| kernel | " + " | ".join(t for _, _, t in rows) + " |", "|---|" + "---|" * len(rows)]
+        for n in sorted(acc):
+            d = acc[n]
+            lines.append(f"| {_short(n)} | " + " | ".join(f"{d[a] / d[b]:.3f}" if d[b] else "-" for a, b, _ in rows) + " |")
     return "\n".join(lines) + "\n"
 
 
@@ -94,3 +139,5 @@ if __name__ == "__main__":
     elif sys.argv[1] == "traffic":
         a = sys.argv[2:]
         sys.stdout.write(traffic(a[0], a[1], a[2], a[3], int(a[4]), a[5], a[6], " ".join(a[7:])))
+    elif sys.argv[1] == "counters":
+        sys.stdout.write(counters(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:])))
