@@ -681,10 +681,14 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // fractions and tasks,
         // input list, and the task words for workgroups of up to 128 threads
         b.nctmax = std::max(b.nctmax, 1);   // ungated slots read sF[first task], kept in bounds
-        // a separate channel wave (gates overlapping the cable step) pays off for one-wave
-        // and 8-wave workgroups; for 2 and 4 cable waves the extra wave costs a workgroup
-        // per CU, which those latency-bound sizes need more
-        b.split = (b.tpb == 64 || b.tpb == 512) && ctx->opt.persistent >= 0;
+        // a separate channel wave (gates overlapping the cable step) pays off for 8-wave
+        // workgroups, and for one-wave workgroups when the bucket is latency-bound (at most
+        // 4 cells per CU: every cell runs at once). With more one-wave cells the device runs
+        // out of wave slots first and the extra wave per cell costs more than it hides. For
+        // 2 and 4 cable waves the extra wave costs a workgroup per CU, which those sizes
+        // need more.
+        b.split = (b.tpb == 512 || (b.tpb == 64 && (int64_t)bc.size() <= 4 * (int64_t)ctx->n_cu)) &&
+                  ctx->opt.persistent >= 0;
         // complex-compartment records go to LDS when they take at most 20 KiB
         b.cxmax_lds = b.cxmax <= 256 ? b.cxmax : 0;
         b.lds = (size_t)tree_lds(b.nmax, b.nvmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, ctx->n_gates,

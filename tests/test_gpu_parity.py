@@ -107,6 +107,22 @@ def test_hh_everywhere_large_trees():
     assert res["raster_mismatch_cols"] == 0, res
 
 
+def test_many_small_trees_without_channel_wave():
+    """More one-wave cells than 4 per CU: the 64-thread bucket then updates gates on its cable
+    wave (no separate channel wave), the form that runs the small cells of config 4."""
+    rng = np.random.default_rng(29)
+    sizes = [int(x) for x in rng.integers(10, 60, 1100)]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=5.0, seed=29)
+    with Simulation(m, profile=True) as sim:
+        sim.run_all()
+        infos = sim.kernel_infos()
+    _say(f"[parity] kernels {[(k['name'], k['threads_per_block'], k['grid'], k['lds_bytes']) for k in infos]}")
+    assert [k["name"] for k in infos] == ["tree_kernel<64, false>"]
+    assert infos[0]["threads_per_block"] == 64 and infos[0]["lds_bytes"] > 0
+    res, g, r = _check(m)
+    assert res["raster_mismatch_cols"] == 0, res
+
+
 def test_size_limits_agree():
     """Cells moved to larger workgroups by size limits (other schedules, same operations)."""
     sizes = [60, 150, 300, 500, 900, 1300]
