@@ -257,18 +257,33 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         const uint4 idle_fold4 = make_uint4(dd, dd, dd, dummy_off);
         char *const dxb = reinterpret_cast<char *>(sDX);
         char *const ub = reinterpret_cast<char *>(sU);
-        auto run_list = [&](int dir, auto idle, auto body) {
+        // a list's directory entry and first four round headers: the same every step, so they
+        // are read once per cell into scalar registers, and a list starts with the reads of its
+        // first task words
+        struct ListHead {
+            uint32_t list, h0, h1, h2, h3;
+        };
+        auto list_head = [&](int dir) {
+            ListHead L;
+            L.list = __builtin_amdgcn_readfirstlane(sW[dir]);
+            const int first = L.list & 0xFFFFu;
+            L.h0 = __builtin_amdgcn_readfirstlane(sW[first]);
+            L.h1 = __builtin_amdgcn_readfirstlane(sW[first + 1]);
+            L.h2 = __builtin_amdgcn_readfirstlane(sW[first + 2]);
+            L.h3 = __builtin_amdgcn_readfirstlane(sW[first + 3]);
+            return L;
+        };
+        auto run_list = [&](const ListHead &L, auto idle, auto body) {
             using Word = decltype(idle);
-            const uint32_t list = __builtin_amdgcn_readfirstlane(sW[dir]);
-            const int first = list & 0xFFFFu, cnt = list >> 16;
+            const int first = L.list & 0xFFFFu, cnt = L.list >> 16;
             auto word = [&](uint32_t h, bool &act) {
                 const Word w = reinterpret_cast<const Word *>(sW + (h & 0xFFFFu))[lane];
                 act = lane < (int)(h >> 16);
                 return act ? w : idle;
             };
             bool aa, ab;
-            uint32_t ha = sW[first + 2], hb = sW[first + 3];
-            Word wa = word(sW[first], aa), wb = word(sW[first + 1], ab);
+            uint32_t ha = L.h2, hb = L.h3;
+            Word wa = word(L.h0, aa), wb = word(L.h1, ab);
             for (int r = 0; r < cnt; r += 2) {
                 if (aa) body(wa);
                 wa = word(ha, aa);
@@ -345,6 +360,9 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             slot_simple[j] = __ballot(valid && c != 0 && (fl[j] & TCX_COMPLEX)) == 0ull;
         }
         __syncthreads();
+        // the Hines lists of this wave (for the channel wave, unused) and of the junction
+        const ListHead Lf = list_head(2 * wv), Lb = list_head(2 * wv + 1);
+        const ListHead Ljf = list_head(2 * W), Ljb = list_head(2 * W + 1);
         if (const_tree) {
             // eliminated diagonals of the constant subtrees: the forward sweep's diagonal part
             // on this step-independent matrix, with the per-step operations
@@ -387,9 +405,9 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 const float2 ubb = *reinterpret_cast<const float2 *>(ub + bo);
                 reinterpret_cast<float2 *>(dxb + po)->x = (d - (ua.y / da) * ua.x) - (ubb.y / db) * ubb.x;
             };
-            if (cable) run_list(2 * wv, idle_fold4, diag_task);
+            if (cable) run_list(Lf, idle_fold4, diag_task);
             __syncthreads();
-            if (cable && wv == 0) run_list(2 * W, idle_fold4, diag_task);
+            if (cable && wv == 0) run_list(Ljf, idle_fold4, diag_task);
             __syncthreads();
             // from here on, below the root: sDX[c] = (ratio Uj / D', right-hand side) and
             // sU[c] = (Ui, D'); the per-step diagonal register Ds holds the ratio
@@ -576,9 +594,9 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                             reinterpret_cast<float2 *>(dxb + po)->y = x;
                         };
                         if (const_tree)
-                            run_list(2 * wv, idle_fold4, fold_const);
+                            run_list(Lf, idle_fold4, fold_const);
                         else
-                            run_list(2 * wv, idle_fold, fold_task);
+                            run_list(Lf, idle_fold, fold_task);
                         if (tid == 0 && (fl[0] & TCX_COMPLEX)) {
                             // the root compartment's update, left out of the compartment phase: only
                             // the root fold reads the root's diagonal and right-hand side
@@ -589,9 +607,9 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         lds_barrier<TPB>();
                         if (wv == 0) {
                             if (const_tree)
-                                run_list(2 * W, idle_fold4, fold_const);
+                                run_list(Ljf, idle_fold4, fold_const);
                             else
-                                run_list(2 * W, idle_fold, fold_task);
+                                run_list(Ljf, idle_fold, fold_task);
                             if (lane == 0) {
                                 // the root: all its children (positions 1 .. nr) in order, loads
                                 // issued four at a time, then its solve
@@ -621,10 +639,10 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                                 r0.y = r0.y / r0.x;
                                 sDX[0] = r0;
                             }
-                            run_list(2 * W + 1, idle_bwd, bwd_task);
+                            run_list(Ljb, idle_bwd, bwd_task);
                         }
                         lds_barrier<TPB>();
-                        run_list(2 * wv + 1, idle_bwd, bwd_task);
+                        run_list(Lb, idle_bwd, bwd_task);
                         lds_barrier<TPB>();
                     }
     #pragma unroll
