@@ -77,6 +77,7 @@ struct edenhip_ctx {
     int64_t NC = 0, n_poi = 0, n_slots = 0, total_rows = 0, rows_done = 0, next_step = -3;
     int n_cells = 0;
     bool network = false;
+    bool point_quad = false;   // point cells run four lanes per cell (no synapses, inputs, spikes)
     // host mirrors for output mapping
     std::vector<int32_t> desc_to_dev_col;   // per desc trace column
     int32_t n_dev_cols = 0;
@@ -548,6 +549,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     bool any_targets = n_spk > 0;
     ctx->n_slots = n_slots;
     ctx->network = any_targets;
+    ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0;
 
     // device uploads
     DevTables &T = ctx->T;
@@ -961,14 +963,20 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             const bool first = a == s0, last = b == s1;
             if (ctx->n_point > 0) {
                 const int tpb = 64;
-                const int grid = (ctx->n_point + tpb - 1) / tpb;
                 float dt = ctx->dt;
                 int32_t n = ctx->n_point;
                 int64_t aa = a, bb = b;
-                void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &n,
-                                (void *)&aa, (void *)&bb, &row0, &dt};
                 if (pf && first) HIPCHECK(ctx, hipEventRecord(ctx->pp0, ctx->stream));
-                HIPCHECK(ctx, hipLaunchKernel(point_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
+                if (ctx->point_quad) {
+                    const int grid = (int)(((int64_t)ctx->n_point * 4 + tpb - 1) / tpb);
+                    void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_point, &n, (void *)&aa, (void *)&bb, &row0, &dt};
+                    HIPCHECK(ctx, hipLaunchKernel(point_quad_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
+                } else {
+                    const int grid = (ctx->n_point + tpb - 1) / tpb;
+                    void *args[] = {&T, &ctx->d_gates, &ctx->n_gates, &ctx->d_stypes, &ctx->n_stypes, &ctx->d_point, &n,
+                                    (void *)&aa, (void *)&bb, &row0, &dt};
+                    HIPCHECK(ctx, hipLaunchKernel(point_kernel_ptr(), dim3(grid), dim3(tpb), args, 0, ctx->stream));
+                }
                 if (pf && last) HIPCHECK(ctx, hipEventRecord(ctx->pp1, ctx->stream));
                 nl++;
             }
@@ -1212,7 +1220,7 @@ int edenhip_kernel_stats(const edenhip_ctx *ctx, double *total_ms, int64_t *laun
     if (total_ms) *total_ms = ctx->kernel_ms;
     if (launches) *launches = ctx->launches;
     if (name && name_len > 0) {
-        std::string nm = ctx->buckets.empty() ? "point_kernel" : "tree_kernel";
+        std::string nm = !ctx->buckets.empty() ? "tree_kernel" : ctx->point_quad ? "point_quad_kernel" : "point_kernel";
         std::strncpy(name, nm.c_str(), (size_t)name_len - 1);
         name[name_len - 1] = 0;
     }
@@ -1228,9 +1236,9 @@ int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_in
     if (!ctx || !out || i < 0 || i >= edenhip_kernel_count(ctx)) return EDENHIP_E_INVALID;
     std::memset(out, 0, sizeof(*out));
     if (ctx->n_point > 0 && i == 0) {
-        std::snprintf(out->name, sizeof(out->name), ctx->netwin ? "net_window_kernel" : "point_kernel");
+        std::snprintf(out->name, sizeof(out->name), ctx->netwin ? "net_window_kernel" : ctx->point_quad ? "point_quad_kernel" : "point_kernel");
         out->threads_per_block = 64;
-        out->grid = (ctx->n_point + 63) / 64;
+        out->grid = ctx->point_quad ? (int32_t)(((int64_t)ctx->n_point * 4 + 63) / 64) : (ctx->n_point + 63) / 64;
         out->n_cells = ctx->n_point;
         out->n_comps = ctx->n_point;
         out->launches = ctx->point_launches;

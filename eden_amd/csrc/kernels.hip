@@ -2,6 +2,8 @@
 //
 //  point_kernel   one lane per single-compartment cell; time loop inside the launch, state in
 //                 registers (a launch may cover one step, as in the network path).
+//  point_quad_kernel  four lanes per uncoupled single-compartment cell, each advancing one
+//                 of its gate variables beside the voltage update.
 //  tree_kernel    one workgroup (TPB threads, one or more wave64s) per multi-compartment
 //                 cell, popped from a work queue sorted by compartment count (largest first);
 //                 compartment state in registers (4 compartments per thread), the Hines solve
@@ -90,6 +92,100 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
     T.V[g] = V;
 #pragma unroll
     for (int s = 0; s < NQ; s++) T.q[g * NQ + s] = q[s];
+}
+
+// Uncoupled one-compartment cells (no synapses, Poisson inputs or spike recipients), four
+// lanes per cell. The lanes of a quad compute the cell's currents and voltage update
+// identically (the same operations on the same values), and lane s of the quad also advances
+// gate variable s. A step's currents use the present gate values (gathered from the quad's
+// lanes), while the gate updates feed only the next step (eden/Eden.cpp:5330: each gate's
+// present value enters the open fraction before it is updated), so the gates' rate and exp
+// chains run side by side and beside the voltage update instead of one after another.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+point_quad_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const int32_t *__restrict__ comps,
+                  int32_t n, int64_t s0, int64_t s1, int64_t row0, float dt) {
+    __shared__ DGate gates[MAX_CHAN_TYPES * GATES_PER_TYPE];
+    for (int i = threadIdx.x; i < n_gates; i += blockDim.x) gates[i] = gates_g[i];
+    __syncthreads();
+    const int32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t idx = gid >> 2, qs = gid & 3;
+    if (idx >= n) return;   // the whole quad: its lanes share idx
+    const int64_t g = comps[idx];
+    CompConst k;
+    load_comp_const(T, g, k);
+    float V = T.V[g];
+    float qv = T.q[g * NQ + qs];   // gate variable qs of the cell
+    const int nq = (k.gmeta >> 24) & 7;
+    const int nch = (k.gmeta >> 28) & 7;
+    const bool mine = qs < nq;
+    DGate G{};
+    if (mine) {
+        const int cs = (k.gmeta >> (4 * qs)) & 3;
+        const int gi = (k.gmeta >> (4 * qs + 2)) & 3;
+        G = gates[((k.chmeta >> (8 * cs)) & 0xFF) * GATES_PER_TYPE + gi];
+    }
+    // time_f64 accumulates `time += dt` in double, as the reference's engine loop does
+    double time = (s0 >= 1) ? T.t_row[s0 - 1] : 0.0;
+    for (int64_t s = s0; s < s1; s++) {
+        const bool init = s <= 0;
+        const float t = (float)time;
+        if (!init && k.rec >= 0 && qs == 0) T.trace[(s - 1 - row0) * T.n_trace_cols + k.rec] = V;
+        float q[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; j++) q[j] = __shfl(qv, j, 4);
+        // open fractions from the present gate values, in gate order (channels_step)
+        float fop[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) fop[c] = 1.0f;
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            if (j < nq) {
+                const int cs = (k.gmeta >> (4 * j)) & 3;
+                const int gi = (k.gmeta >> (4 * j + 2)) & 3;
+                const int inst = (gates[((k.chmeta >> (8 * cs)) & 0xFF) * GATES_PER_TYPE + gi].forms >> 16) & 0xFF;
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    if (c == cs) {
+                        float pw = fop[c];
+                        for (int i = 0; i < inst; i++) pw = pw * q[j];
+                        fop[c] = pw;
+                    }
+                }
+            }
+        }
+        // this lane's gate variable for the next step
+        const float nv = mine ? gate_next(G, V, qv, init, dt) : qv;
+        float I_ch = 0.0f, G_ch = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            if (c < nch) {
+                const float Gscaled = k.g[c] * fop[c];
+                const float iDensity = Gscaled * (k.e[c] - V) * 1e-5f;
+                const float gDensity = (Gscaled) * 1e-5f;
+                I_ch += (iDensity * k.A);
+                G_ch += (gDensity * k.A);
+            }
+        }
+        const float I_in = pulses_current(T, k, g, t), G_in = 0.0f;
+        const float I_syn = 0.0f, G_syn = 0.0f;
+        const float I_int = I_ch + I_in + I_syn;
+        const float G_int = G_ch + G_in + G_syn;
+        float D = 1 * (G_int / k.C) + k.invrc;
+        float Vn;
+        if (init) {
+            Vn = V;
+        } else {
+            Vn = V + (dt * (I_int + ((V * G_int))) / k.C);
+            D = 1 + D * dt;
+            Vn = Vn / D;
+        }
+        if (V < k.vthr && k.vthr < Vn && qs == 0) log_event(T, k.ev, s);
+        V = Vn;
+        qv = nv;
+        if (!init) time += (double)dt;
+    }
+    if (qs == 0) T.V[g] = V;
+    T.q[g * NQ + qs] = qv;
 }
 
 // LDS-only workgroup barrier: waits for this wave's LDS operations, then (for more than one
@@ -724,5 +820,6 @@ const void *tree_kernel_ptr(int tpb, bool split) {
 }
 
 const void *point_kernel_ptr() { return reinterpret_cast<const void *>(&point_kernel); }
+const void *point_quad_kernel_ptr() { return reinterpret_cast<const void *>(&point_quad_kernel); }
 
 }  // namespace edenhip

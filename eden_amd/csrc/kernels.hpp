@@ -102,5 +102,6 @@ const void *tree_kernel_ptr(int tpb, bool split);
 const void *net_window_kernel_ptr();
 const void *net_deliver_kernel_ptr();
 const void *point_kernel_ptr();
+const void *point_quad_kernel_ptr();
 
 }  // namespace edenhip
