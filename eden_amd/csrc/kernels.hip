@@ -452,7 +452,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 }
             }
             slot_any[j] = __builtin_amdgcn_readfirstlane(wbase + TPB * j) < N && cable;
-            // the root (thread 0, slot 0) is updated separately, after wave 0's Hines rounds
+            // the root (thread 0, slot 0) is updated separately, after wave 0's fold rounds
             slot_simple[j] = __ballot(valid && c != 0 && (fl[j] & TCX_COMPLEX)) == 0ull;
         }
         __syncthreads();

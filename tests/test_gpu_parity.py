@@ -52,6 +52,27 @@ def test_hh_population_small():
     assert res["raster_mismatch_cols"] == 0, res
 
 
+def test_point_cells_with_four_gate_variables():
+    """Uncoupled one-compartment cells with four gate variables (leak, Na, K and a second K
+    distribution at half the density): every lane of a cell's quad advances a gate."""
+    import dataclasses
+    m = configs.hh_population(n_cells=300, t_final_ms=40.0, seed=5)
+    nc = m.n_cells
+    g3, e3 = m.chan_gbase[:3], m.chan_erev[:3]
+    m4 = dataclasses.replace(
+        m, comp_chan_begin=np.arange(nc + 1, dtype=np.int64) * 4,
+        chan_type=np.tile(np.array([0, 1, 2, 2], np.int32), nc),
+        chan_gbase=np.tile(np.array([g3[0], g3[1], g3[2], g3[2] * np.float32(0.5)], np.float32), nc),
+        chan_erev=np.tile(np.array([e3[0], e3[1], e3[2], e3[2]], np.float32), nc), name="hh_population_4gates")
+    with Simulation(m4, profile=True) as sim:
+        sim.run_all()
+        names = [k["name"] for k in sim.kernel_infos()]
+    assert names == ["point_quad_kernel"], names
+    res, g, r = _check(m4)
+    assert res["spikes_ref"] > 0
+    assert res["raster_mismatch_cols"] == 0, res
+
+
 def test_random_trees_small():
     sizes = [20, 35, 64, 65, 100, 128, 129, 200, 256, 257, 400, 31]
     res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=60.0, seed=7))
