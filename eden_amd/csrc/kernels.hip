@@ -265,8 +265,8 @@ __device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t
 // per step:
 //   inputs     per-cell list of compartments with Poisson inputs -> sIn (LDS)
 //   membrane   currents from the channels' open fractions (sF) and inputs, semi-implicit RHS
-//   Hines      in LDS on breadth-first positions (tree_schedule.hpp): sDX[p] = (diagonal,
-//              rhs), sU[p] = (Ui, Uj) of the edge from p to its parent, one task word per
+//   Hines      in LDS on breadth-first positions (tree_schedule.hpp): a 16-byte record per
+//              node, (diagonal, rhs) then (Ui, Uj) of the edge to its parent, one task word per
 //              thread per round
 //   readback   new voltages, spike detection
 // The channel phase (next gate values and open fractions from the present voltages) runs
@@ -293,8 +293,11 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     DGate *gates = reinterpret_cast<DGate *>(sm + L.gtab);
     DSynType *stypes = reinterpret_cast<DSynType *>(sm + L.stab);
     TreeComplex *sCX = reinterpret_cast<TreeComplex *>(sm + L.cx);
-    float2 *sDX = reinterpret_cast<float2 *>(sm + L.dx);
-    float2 *sU = reinterpret_cast<float2 *>(sm + L.u);
+    // Hines node records, 16 bytes at position p: sDX(p) = (diagonal, rhs), then sU(p) =
+    // (Ui, Uj) of the edge from p to its parent; one record serves a task's reads of both
+    char *const nodes = reinterpret_cast<char *>(sm + L.node);
+    auto sDX = [nodes](int p) -> float2 & { return *reinterpret_cast<float2 *>(nodes + 16 * p); };
+    auto sU = [nodes](int p) -> float2 & { return *reinterpret_cast<float2 *>(nodes + 16 * p + 8); };
     float2 *sIn = reinterpret_cast<float2 *>(sm + L.in);
     float *sVb = sm + L.v;
     float *sQ = sm + L.q;
@@ -335,7 +338,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         // after round with no barrier (a wave's LDS accesses are performed in order), on the
         // lanes below the round's task count; lanes beyond it are masked off (their word is
         // the idle word, which points every access at the dummy slot N). Task words hold byte
-        // offsets into sDX, which starts at a constant LDS address. Every lane reads a word
+        // offsets of node records, which start at a constant LDS address. Every lane reads a word
         // from the round (the block ends with padding for the lanes beyond a round's count).
         // The loop is unrolled twice, so that no register moves between rounds: after a
         // round's body is issued, the words of the round after next and the header of the
@@ -344,15 +347,15 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         // the loads just issued.
         constexpr int W = TPB / 64;
         const int wv = tid >> 6, lane = tid & 63;
-        const uint32_t dummy_off = 8u * (uint32_t)N;
+        const uint32_t dummy_off = 16u * (uint32_t)N;
         const uint2 idle_fold = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
         const uint2 idle_bwd = make_uint2(dummy_off | (dummy_off << 16), dummy_off);
         // constant trees take four-word fold tasks with merged children (plan_tree_waves
         // with merge): c0, c1 and the grandchildren refolded through each, dummy where absent
         const uint32_t dd = dummy_off | (dummy_off << 16);
         const uint4 idle_fold4 = make_uint4(dd, dd, dd, dummy_off);
-        char *const dxb = reinterpret_cast<char *>(sDX);
-        char *const ub = reinterpret_cast<char *>(sU);
+        char *const dxb = nodes;       // task words: byte offsets 16 p of node records
+        char *const ub = nodes + 8;
         // a list's directory entry and first four round headers: the same every step, so they
         // are read once per cell into scalar registers, and a list starts with the reads of its
         // first task words
@@ -391,7 +394,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 }
             }
         };
-        for (int i = tid; i < N; i += NT) sU[i] = u_g[c0 + i];
+        for (int i = tid; i < N; i += NT) sU(i) = u_g[c0 + i];
         for (int i = tid; i < NG; i += NT) {
             sQ[i] = gq_g[gq0 + i];
             sGT[i] = gt_g[gq0 + i];
@@ -413,8 +416,8 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         for (int i = tid; i < NV; i += NT) sVb[i] = T.V[c0 + i];
         if (tid == 0) {
             // dummy slot for threads without a task: finite values, never read back
-            sU[N] = make_float2(0.0f, 0.0f);
-            sDX[N] = make_float2(1.0f, 0.0f);
+            sU(N) = make_float2(0.0f, 0.0f);
+            sDX(N) = make_float2(1.0f, 0.0f);
         }
 
         // Per slot registers: voltage, BFS position, flag word, trace column, and for a simple
@@ -477,7 +480,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const float Dd = 1 * (G_int / k.C) + k.invrc;
                         Dr = 1 + Dd * dt;
                     }
-                    sDX[posof(j)] = make_float2(Dr, 0.0f);
+                    sDX(posof(j)) = make_float2(Dr, 0.0f);
                 }
             }
             __syncthreads();
@@ -511,13 +514,13 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             for (int j = 0; j < K; j++) {
                 const int c = tid + TPB * j;
                 if (cable && c < N && posof(j) != 0) {
-                    const float Dp = sDX[posof(j)].x;
-                    Ds[j] = sU[posof(j)].y / Dp;
-                    sU[posof(j)].y = Dp;
+                    const float Dp = sDX(posof(j)).x;
+                    Ds[j] = sU(posof(j)).y / Dp;
+                    sU(posof(j)).y = Dp;
                 }
             }
             // dummy slot: ratio 1 (times right-hand side 0), D' = 1 for the idle backward task
-            if (tid == 0) sU[N] = make_float2(0.0f, 1.0f);
+            if (tid == 0) sU(N) = make_float2(0.0f, 1.0f);
         }
         __syncthreads();
         // open fractions of the gated channels from the present gate values
@@ -617,7 +620,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         float Dn = Ds[j];
                         float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
                         if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, Dn, Vn);
-                        sDX[posof(j)] = make_float2(Dn, Vn);
+                        sDX(posof(j)) = make_float2(Dn, Vn);
                     }
                     lds_barrier<TPB>();
                     if constexpr (!SPLIT) {
@@ -698,7 +701,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                             // the root fold reads the root's diagonal and right-hand side
                             float Dn = Ds[0], Vn = V[0];
                             complex_update(0, 0, sF, t, false, Dn, Vn);
-                            sDX[0] = make_float2(Dn, Vn);
+                            sDX(0) = make_float2(Dn, Vn);
                         }
                         lds_barrier<TPB>();
                         if (wv == 0) {
@@ -715,14 +718,14 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                                 // the root: all its children (positions 1 .. nr) in order, loads
                                 // issued four at a time, then its solve
                                 const int nr = cell.root_children;
-                                float2 r0 = sDX[0];
+                                float2 r0 = sDX(0);
                                 int c = 1;
                                 for (; c + 3 <= nr; c += 4) {
                                     float2 a[4], u[4];
     #pragma unroll
                                     for (int k = 0; k < 4; k++) {
-                                        a[k] = sDX[c + k];
-                                        u[k] = sU[c + k];
+                                        a[k] = sDX(c + k);
+                                        u[k] = sU(c + k);
                                     }
     #pragma unroll
                                     for (int k = 0; k < 4; k++) {
@@ -732,13 +735,13 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                                     }
                                 }
                                 for (; c <= nr; c++) {
-                                    const float2 a = sDX[c], u = sU[c];
+                                    const float2 a = sDX(c), u = sU(c);
                                     const float r = const_tree ? a.x : u.y / a.x;
                                     r0.x = r0.x - r * u.x;
                                     r0.y = r0.y - r * a.y;
                                 }
                                 r0.y = r0.y / r0.x;
-                                sDX[0] = r0;
+                                sDX(0) = r0;
                             }
                             run_list(Ljb, idle_bwd, bwd_task);
                             if constexpr (!SPLIT) __builtin_amdgcn_s_setprio(0);
@@ -751,7 +754,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                     for (int j = 0; j < K; j++) {
                         const int c = tid + TPB * j;
                         if (slot_any[j] && c < N) {
-                            const float Vn = init ? V[j] : sDX[posof(j)].y;
+                            const float Vn = init ? V[j] : sDX(posof(j)).y;
                             if (fl[j] & TCX_COMPLEX) {
                                 const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
                                 // a recorded compartment is complex: its trace column is in its

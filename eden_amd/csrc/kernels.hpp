@@ -39,7 +39,8 @@ constexpr int TREE_SLOTS = 4;        // compartments per thread
 
 
 
-// Dynamic LDS of the tree kernel, in 4-byte units: sDX, sU [nmax+1] float2 | sIn [nmax] float2
+// Dynamic LDS of the tree kernel, in 4-byte units: Hines node records [nmax+1] of 16 bytes
+// ((diagonal, rhs), (Ui, Uj)) | sIn [nmax] float2
 // (only with Poisson inputs) | sV [nbuf][nvmax+1] | sQ [ngmax] | sGT [ngmax] | sF [nbuf][nctmax]
 // | sCT [nctmax] | sIL [ninmax] | schedule block [words_max] (16-byte aligned) | sCX [cxmax]
 // TreeComplex (only when the records are staged in LDS) | gate table [n_gates] DGate |
@@ -47,19 +48,18 @@ constexpr int TREE_SLOTS = 4;        // compartments per thread
 // else 1. nvmax: voltage slots of the bucket's cells (1 for a constant tree, whose only
 // voltage read from LDS is the root's, else its compartment count).
 struct TreeLds {
-    int32_t dx, u, in, v, q, gt, f, ct, il, w, cx, gtab, stab, total;
+    int32_t node, in, v, q, gt, f, ct, il, w, cx, gtab, stab, total;
 };
 
 __host__ __device__ inline TreeLds tree_lds(int32_t nmax, int32_t nvmax, int32_t ngmax, int32_t nctmax,
                                             int32_t ninmax, int32_t words_max, int32_t cxmax, int32_t n_gates,
                                             int32_t n_stypes, bool split) {
-    // sDX first, at the start of dynamic LDS: its address is then a constant, which the
-    // compiler folds into the offsets of the Hines solve's LDS instructions
+    // the node records first, at the start of dynamic LDS: their address is then a constant,
+    // which the compiler folds into the offsets of the Hines solve's LDS instructions
     TreeLds L;
     const int32_t nbuf = split ? 2 : 1;
-    L.dx = 0;
-    L.u = L.dx + 2 * (nmax + 1);
-    L.in = L.u + 2 * (nmax + 1);
+    L.node = 0;
+    L.in = L.node + 4 * (nmax + 1);
     L.v = L.in + (ninmax > 0 ? 2 * nmax : 0);
     L.q = L.v + nbuf * (nvmax + 1);
     L.gt = L.q + ngmax;

@@ -149,14 +149,14 @@ inline TreePlan plan_tree(int32_t n, const int32_t *parent, int32_t tpb) {
 //   headers    one per round: (index of its first task word | task count << 16), task
 //              count 1..64, then five empty sentinel headers; every round starts at an
 //              index that is a multiple of the fold task's word count
-//   words      fold task, two words: (8 pos | 8 first_child << 16), 8 second child (8 n,
-//              the dummy slot, for a task with one child): byte offsets of 8-byte slots;
-//              with merge, four words: (8 pos | 8 c0 << 16), (8 c1 | 8 g00 << 16),
-//              (8 g01 | 8 g10 << 16), 8 g11, where g00, g01 are the children of c0 when the
-//              task folds them into c0 again (merged child), else 8 n, and g10, g11 those of
-//              c1; backward task, two words: (8 pos | 8 solved_pos << 16), 8 mid with
-//              solved_pos the parent and mid = 8 n, or solved_pos the grandparent and mid the
-//              parent
+//   words      byte offsets of node records, o(x) = WAVE_NODE_BYTES * position of x (the
+//              dummy slot: o(n)). Fold task, two words: (o(pos) | o(first_child) << 16),
+//              o(second child) (o(n) for a task with one child); with merge, four words:
+//              (o(pos) | o(c0) << 16), (o(c1) | o(g00) << 16), (o(g01) | o(g10) << 16),
+//              o(g11), where g00, g01 are the children of c0 when the task folds them into c0
+//              again (merged child), else o(n), and g10, g11 those of c1. Backward task, two
+//              words: (o(pos) | o(solved) << 16), o(mid), with solved the parent and mid =
+//              o(n), or solved the grandparent and mid the parent
 //   padding    zero words (wave_plan_pad), so that a wave may read 64 tasks from any round
 // The five rounds after a list's last round are always headers (of the next lists or the
 // sentinels), so a wave may read up to five headers past its list.
@@ -168,6 +168,10 @@ struct WavePlan {
     bool merged = false;               // four-word fold tasks (merge)
     std::vector<uint32_t> block;
 };
+
+// bytes per node record of the tree kernel's Hines arrays: task words hold byte offsets
+// WAVE_NODE_BYTES * position (16-bit fields: at most 16 * 2047)
+constexpr uint32_t WAVE_NODE_BYTES = 16;
 
 // zero words at the end of a block: 64 lanes reading one task each from the last round
 inline int32_t wave_plan_pad(bool merge) { return merge ? 256 : 128; }
@@ -354,8 +358,8 @@ inline WavePlan plan_tree_waves(int32_t n, const int32_t *parent, int32_t waves,
             }
         lists.emplace_back(r0, (uint32_t)rounds.size() - r0);
     };
-    const uint32_t dummy = 8u * (uint32_t)n;
-    auto off = [&](int32_t i) { return 8u * (uint32_t)P.pos[i]; };
+    const uint32_t dummy = WAVE_NODE_BYTES * (uint32_t)n;
+    auto off = [&](int32_t i) { return WAVE_NODE_BYTES * (uint32_t)P.pos[i]; };
     auto add_folds = [&](int32_t o, std::vector<std::vector<Task>> &f) {
         f.clear();
         fold_tasks(owner, o, [&](int32_t lv, int32_t i, int32_t k, bool m0, bool m1) {

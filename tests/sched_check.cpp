@@ -82,8 +82,8 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
     std::vector<int> solved(n + 1, 0), writer(n + 1, -1);
     const int nd = 2 * waves + 2;
     auto slot = [&](uint32_t off, bool &ok) {
-        if (off % 8 != 0 || off / 8 > (uint32_t)n) ok = false;
-        return ok ? (int)(off / 8) : n;
+        if (off % WAVE_NODE_BYTES != 0 || off / WAVE_NODE_BYTES > (uint32_t)n) ok = false;
+        return ok ? (int)(off / WAVE_NODE_BYTES) : n;
     };
     auto fold = [&](int p, int c) {   // one child into p, the reference's operations
         const float r = sj[c] / dx[c];
@@ -187,7 +187,7 @@ static int check_waves(int n, const std::vector<int32_t> &par, const std::vector
             for (int r = 0; r < cnt && ok; r++) {
                 const uint32_t h = B[first + r];
                 for (int t = 0; t < (int)(h >> 16); t++) {
-                    const int p = (int)((B[(h & 0xFFFF) + nw * t] & 0xFFFF) / 8);
+                    const int p = (int)((B[(h & 0xFFFF) + nw * t] & 0xFFFF) / WAVE_NODE_BYTES);
                     if (p > n || (writer[p] >= 0 && writer[p] != w)) bad++;
                     writer[p] = w;
                 }
