@@ -246,7 +246,7 @@ def main():
                     help="tree-cell size limits a,b,c,d for 64/128/256/512 threads (skips the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py: the simulation runs on one GPU; multi-GPU runs are not supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:

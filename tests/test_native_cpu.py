@@ -55,7 +55,7 @@ def test_create_reports_errors_without_crashing():
     assert rc == N.E_INVALID
     assert lib.edenhip_last_error(h)
     lib.edenhip_destroy(h)
-    # a valid model on a machine without a GPU: device error, or (on a GPU box) success
+    # a valid model on a machine without a GPU: device error, or (where a GPU is present) success
     m = configs.random_trees(2, sizes=[10, 20], t_final_ms=1.0)
     try:
         with Simulation(m) as s:
