@@ -86,6 +86,30 @@ def test_random_trees_all_compartments_recorded():
     assert res["raster_mismatch_cols"] == 0, res
 
 
+def test_ragged_cells_one_to_max_compartments():
+    """One-, two- and three-compartment cells beside a tree at the size limit in one model: the
+    one-compartment cells take the point path, the rest the tree buckets, in the same run."""
+    sizes = [1, 2, 3, 1, 2047, 5, 64, 2]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=60.0, seed=17)
+    with Simulation(m, profile=True) as sim:
+        sim.run_all()
+        names = {k["name"] for k in sim.kernel_infos()}
+    assert any(n.startswith("point") for n in names), names
+    assert any(n.startswith("tree_kernel<512") for n in names), names
+    res, g, r = _check(m)
+    # the one- to three-compartment cells spike (25 spikes in 60 ms)
+    assert res["spikes_ref"] > 0
+    assert res["raster_mismatch_cols"] == 0, res
+
+
+def test_run_shorter_than_one_step():
+    """t_final below dt: one row, the initial state, and no step taken."""
+    m = configs.random_trees(2, sizes=[1, 40], t_final_ms=0.01, seed=3)
+    assert m.total_rows() == 1
+    res, g, r = _check(m)
+    assert g[0].shape == (1, len(m.record_comp))
+
+
 def test_large_trees_workgroup_per_cell():
     sizes = [513, 1000, 1500, 2000, 700]
     res, g, r = _check(configs.random_trees(len(sizes), sizes=sizes, t_final_ms=25.0, seed=13))
