@@ -1,9 +1,12 @@
 // output.cpp -- EDEN's trajectory and event text formats (include/edenhip_output.h).
 //
-// Number format: FixedWidthNumberPrinter with a 16-character column (eden/Eden.cpp:144-189):
-// "%+15.8g" into 15 characters, then the one-character delimiter ' '. Timestamps: engine
-// time (us) times 1e-6 (GetTimestampAscii, eden/Eden.cpp:184-189). Lines are written with a
-// trailing '\n' (LineWriter_File::PutFrameLines, eden/Eden.cpp:10633-10640).
+// Number format: the engine loop's printer is FixedWidthNumberPrinter(16, '\t', 0)
+// (eden/Eden.cpp:10570-10571): a 16-character column with a zero-length delimiter, so the
+// number takes all 16 characters, "%+16.8g" (8 = 16 - 3 - 5 significant digits,
+// eden/Eden.cpp:144-181), and nothing follows it. Columns of a line are joined by one ' '
+// (eden/Eden.cpp:10856-10870). Timestamps: engine time (us) times 1e-6 (GetTimestampAscii,
+// eden/Eden.cpp:184-189). Lines end with '\n' (LineWriter_File::PutFrameLines,
+// eden/Eden.cpp:10633-10640).
 #include <stdint.h>
 
 #include <algorithm>
@@ -21,15 +24,10 @@ namespace {
 
 constexpr double US_TO_S = 1e-6;
 
-// 16 characters: the number right-aligned in 15, then a space
-inline void fixed16(double v, char *out) {
-    char tmp[64];
-    std::snprintf(tmp, 16, "%+15.8g", v);   // at most 15 characters, as the reference's snprintf bound
-    const size_t n = std::strlen(tmp);
-    std::memcpy(out, tmp, n);
-    out[n] = ' ';
-    out[n + 1] = '\0';
-}
+// 16 characters: the number right-aligned in a 16-character field, no delimiter. The bound
+// of 17 bytes (16 characters and the terminator) is the reference's snprintf bound; "%+.8g"
+// never needs more than 15 characters, so nothing is cut.
+inline void fixed16(double v, char *out) { std::snprintf(out, 17, "%+16.8g", v); }
 
 struct File {
     FILE *f = nullptr;

@@ -2,7 +2,8 @@
 
 OutputFile: one line per recorded row, the time in seconds then each column's value times
 its scale (1e-3 turns the engine's mV into the volts a classic OutputFile holds), every
-number printed "%+15.8g" plus a space (eden/Eden.cpp:144-189, 10843-10884).
+number printed "%+16.8g" in a 16-character field and the columns joined by one space
+(FixedWidthNumberPrinter(16, '\t', 0), eden/Eden.cpp:144-181, 10570-10571, 10843-10884).
 EventOutputFile: TIME_ID / ID_TIME lines per event, or one ascii_v0 line per row with
 events (eden/Eden.cpp:10886-10940).
 """

@@ -2,8 +2,9 @@
  * edenhip_read_trace / edenhip_read_events return.
  *
  * Replaces the reference's TimeseriesWriter / EventWriter (eden/Eden.cpp:10843-10940) with
- * FixedWidthNumberPrinter(16) (eden/Eden.cpp:144-189): every number is printed with
- * "%+15.8g" followed by one space. Timestamps are the engine time (us) times 1e-6 (seconds,
+ * the engine loop's FixedWidthNumberPrinter(16, '\t', 0) (eden/Eden.cpp:144-181,
+ * 10570-10571): every number fills a 16-character field, "%+16.8g", with no delimiter of its
+ * own. Timestamps are the engine time (us) times 1e-6 (seconds,
  * GetTimestampAscii); trajectory values are the recorded float times each column's scale
  * factor (native unit to the output unit: 1e-3 for mV to V in a classic OutputFile).
  *
@@ -34,8 +35,8 @@ int edenhip_write_trajectory(const char *path, int append, const double *time_us
 int edenhip_write_events(const char *path, int append, int32_t format, const double *time_us, const int64_t *ev_row,
                          const int64_t *ev_col, int64_t n_events, const int64_t *ids, int64_t n_ids);
 
-/* The fixed-width text of one number as the writers print it ("%+15.8g" and a space,
- * 16 characters plus the terminator): for callers building their own lines. */
+/* The fixed-width text of one number as the writers print it ("%+16.8g": 16 characters plus
+ * the terminator, so buf_len >= 17): for callers building their own lines. */
 int edenhip_format_number(double value, char *buf, int32_t buf_len);
 
 #ifdef __cplusplus

@@ -282,16 +282,16 @@ def test_deviation_from_fp64_dt10_is_the_oracles():
 
 def test_output_files_from_a_gpu_run(tmp_path):
     """OutputFile / EventOutputFile written from a chunked GPU run, byte-compared with the
-    files built from the oracle's trace and events."""
+    text the reference's printer gives for the oracle's trace and events (tests/eden_text.py,
+    built from FixedWidthNumberPrinter's arithmetic, not from the library's writer)."""
     from eden_amd import output
+    from eden_text import assert_same_text, events_text, trajectory_text
     m = configs.hh_population(n_cells=40, t_final_ms=30.0)
     tp, ep = tmp_path / "v.dat", tmp_path / "spikes.dat"
     rows, nev = output.run_to_files(m, tp, ep, "TIME_ID", chunk_rows=333)
     _say(f"[output] rows {rows} events {nev}")
     tr, ec, er, _ = oracle.run(m)
     t_us = output.row_times_us(m.dt, rows + 1)
-    output.write_trajectory(tmp_path / "v_ref.dat", t_us, tr)
-    output.write_events(tmp_path / "spikes_ref.dat", t_us, er, ec, np.arange(len(m.event_comp)), "TIME_ID")
     assert nev > 0
-    assert tp.read_bytes() == (tmp_path / "v_ref.dat").read_bytes()
-    assert ep.read_bytes() == (tmp_path / "spikes_ref.dat").read_bytes()
+    assert_same_text(tp.read_text(), trajectory_text(t_us, tr))
+    assert_same_text(ep.read_text(), events_text(t_us, er, ec, np.arange(len(m.event_comp)), "TIME_ID"))

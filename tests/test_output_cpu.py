@@ -1,17 +1,21 @@
-"""EDEN's output text formats (include/edenhip_output.h), byte for byte against the format
-the reference's writers produce: FixedWidthNumberPrinter(16) = "%+15.8g" then a space
-(eden/Eden.cpp:144-189), trajectory lines time ' ' values (eden/Eden.cpp:10856-10870),
-event lines TIME_ID / ID_TIME / ascii_v0 (eden/Eden.cpp:10897-10927), '\\n' after every line
-(eden/Eden.cpp:10633-10640). Python's %-formatting and C's printf both print the correctly
-rounded decimal, so the expected text is built here from the format specification."""
+"""EDEN's output text formats (include/edenhip_output.h), byte for byte against the text the
+reference's writers produce, which tests/eden_text.py builds from the reference printer's
+arithmetic (FixedWidthNumberPrinter(16, '\\t', 0), eden/Eden.cpp:144-181, 10570-10571)."""
 import numpy as np
 import pytest
 
 from eden_amd import output
+from eden_text import PRINTER_FORMAT, assert_same_text, events_text, fx, trajectory_text
 
 
-def fx(v):
-    return "%+15.8g " % v
+def test_printer_arithmetic():
+    assert PRINTER_FORMAT == "%+16.8g"
+    assert fx(-0.065) == "          -0.065"
+    assert output.format_number(-0.065) == "          -0.065"
+    assert output.format_number(0.0) == "              +0"
+    assert output.format_number(1e-300) == "         +1e-300"
+    # the widest value "%+.8g" prints is 15 characters, so the 16-character field never cuts
+    assert len("%+.8g" % -1.2345678e-308) == 15
 
 
 def test_number_format_matches_the_printf_spec():
@@ -21,6 +25,7 @@ def test_number_format_matches_the_printf_spec():
                             -0.0655, 123456789.0, 1.23456785]])
     for v in vals:
         assert output.format_number(v) == fx(v)
+        assert len(fx(v)) == 16
 
 
 def test_trajectory_file_bytes(tmp_path):
@@ -28,8 +33,9 @@ def test_trajectory_file_bytes(tmp_path):
     tr = np.array([[-65.0, -64.5], [-64.9, 12.25], [0.0, 40.123456], [-70.0, -1e-3], [1.5, 2.5]], np.float32)
     p = tmp_path / "v.dat"
     output.write_trajectory(p, t_us, tr)
-    want = "".join(fx(t_us[r] * 1e-6) + "".join(" " + fx(float(tr[r, c]) * 1e-3) for c in range(2)) + "\n"
-                   for r in range(5))
+    want = trajectory_text(t_us, tr)
+    # one line, by hand: two blanks never appear, and no line ends in a blank
+    assert want.splitlines()[0] == "              +0" + " " + "          -0.065" + " " + "         -0.0645"
     assert p.read_text() == want
     # appended chunks give the same file as one write
     q = tmp_path / "w.dat"
@@ -50,9 +56,7 @@ def test_large_trajectory_formatted_in_parallel_keeps_row_order(tmp_path):
     tr = (rng.standard_normal((rows, cols)) * 30.0 - 60.0).astype(np.float32)
     p = tmp_path / "big.dat"
     output.write_trajectory(p, t_us, tr)
-    want = "".join(fx(t_us[r] * 1e-6) + "".join(" " + fx(float(tr[r, c]) * 1e-3) for c in range(cols)) + "\n"
-                   for r in range(rows))
-    assert p.read_text() == want
+    assert_same_text(p.read_text(), trajectory_text(t_us, tr))
 
 
 @pytest.mark.parametrize("fmt", ["TIME_ID", "ID_TIME", "ascii_v0"])
@@ -63,15 +67,7 @@ def test_event_file_bytes(tmp_path, fmt):
     ids = np.array([10, 11, 12])
     p = tmp_path / "e.dat"
     output.write_events(p, t_us, rows, cols, ids, fmt)
-    lines = []
-    if fmt == "ascii_v0":
-        for r in sorted(set(rows)):
-            lines.append(fx(t_us[r] * 1e-6) + "".join(" %d" % ids[c] for rr, c in zip(rows, cols) if rr == r))
-    else:
-        for r, c in zip(rows, cols):
-            t = fx(t_us[r] * 1e-6)
-            lines.append(t + " %d" % ids[c] if fmt == "TIME_ID" else "%d " % ids[c] + t.strip())
-    assert p.read_text() == "".join(line + "\n" for line in lines)
+    assert p.read_text() == events_text(t_us, rows, cols, ids, fmt)
 
 
 def test_event_writer_rejects_unsorted_events(tmp_path):
