@@ -269,3 +269,64 @@ def config5(n_cells=256, n_comps=100, t_final_ms=500.0, seed=2001, init_seed=500
     g0 = (np.asarray(m.chan_gbase, np.float64) * rng.uniform(0.5, 1.5, len(m.chan_gbase)))
     m.name = f"config5_{n_cells}x{n_comps}"
     return m, g0
+
+
+def mixed_network(tree_sizes=(40, 90, 150), n_point=8, n_conn=60, t_final_ms=60.0, seed=11, sim_seed=5,
+                  delay_ms=(1.0, 4.0), weight=(2.0, 6.0), hh_everywhere=False, amp_range=(0.3, 0.6)):
+    """Tree cells (as random_trees, somatic current steps) and one-compartment HH cells in one
+    network: n_conn connections from a random cell's soma to a random compartment (dendrites
+    included) of another cell, expTwo synapses (excitatory and inhibitory types, each target
+    compartment's instances sorted by type, then in connection order), delays U(delay_ms)
+    quantised to dt, weights U(weight). Spikes of tree somata and of point cells both reach
+    synapses on both kinds of cell."""
+    rng = np.random.default_rng(seed)
+    trees = random_trees(len(tree_sizes), sizes=list(tree_sizes), t_final_ms=t_final_ms, seed=seed,
+                         hh_everywhere=hh_everywhere, amp_range=amp_range)
+    pts = hh_population(n_cells=n_point, t_final_ms=t_final_ms, seed=seed + 1, amp_max_nA=0.15)
+    # concatenate the two models' compartment tables
+    nt, npt = trees.n_comps, pts.n_comps
+    nc = nt + npt
+    cat = lambda a, b: np.concatenate([np.asarray(a), np.asarray(b)])
+    cell_begin = cat(trees.cell_comp_begin, pts.cell_comp_begin[1:] + nt)
+    somata = cell_begin[:-1].copy()
+    ch_b = cat(trees.comp_chan_begin, pts.comp_chan_begin[1:] + trees.comp_chan_begin[-1])
+    pu_b = cat(trees.comp_pulse_begin, pts.comp_pulse_begin[1:] + trees.comp_pulse_begin[-1])
+    # connections: (source soma, target compartment, type, weight, delay)
+    src_cell = rng.integers(0, len(somata), n_conn)
+    tgt_comp = rng.integers(0, nc, n_conn)
+    tgt_cell = np.searchsorted(cell_begin, tgt_comp, side="right") - 1
+    keep = src_cell != tgt_cell
+    src_cell, tgt_comp = src_cell[keep], tgt_comp[keep]
+    n = len(src_cell)
+    stype = (rng.random(n) < 0.25).astype(np.int32)
+    w = rng.uniform(weight[0], weight[1], n).astype(np.float32)
+    delay = (np.round(rng.uniform(delay_ms[0], delay_ms[1], n) * MS / DT) * DT).astype(np.float32)
+    # synapse instances per target compartment: by type, then connection order
+    order = np.lexsort((np.arange(n), stype, tgt_comp))
+    syn_begin = np.zeros(nc + 1, np.int64)
+    np.cumsum(np.bincount(tgt_comp, minlength=nc), out=syn_begin[1:])
+    inst_of_conn = np.empty(n, np.int64)
+    inst_of_conn[order] = np.arange(n)
+    # spike recipients per source compartment, in connection order
+    src_comp = somata[src_cell]
+    sorder = np.argsort(src_comp, kind="stable")
+    spk_begin = np.zeros(nc + 1, np.int64)
+    np.cumsum(np.bincount(src_comp, minlength=nc), out=spk_begin[1:])
+    syn_types = [N.ExpTwoType(2e-3, 0.0, 0.5 * MS, 5.0 * MS, 1.0), N.ExpTwoType(4e-3, -80.0, 0.5 * MS, 5.0 * MS, 1.0)]
+    vthr = cat(trees.v_threshold, pts.v_threshold)
+    return Model(
+        dt=DT, t_final=t_final_ms * MS, cell_comp_begin=cell_begin,
+        v0=cat(trees.v0, pts.v0), capacitance=cat(trees.capacitance, pts.capacitance),
+        area=cat(trees.area, pts.area), inv_rc_axial=cat(trees.inv_rc_axial, pts.inv_rc_axial),
+        r_axial=cat(trees.r_axial, pts.r_axial), v_threshold=vthr, parent=cat(trees.parent, pts.parent),
+        channel_types=hh_squid_types(), comp_chan_begin=ch_b, chan_type=cat(trees.chan_type, pts.chan_type),
+        chan_gbase=cat(trees.chan_gbase, pts.chan_gbase), chan_erev=cat(trees.chan_erev, pts.chan_erev),
+        cell_rng_seed=cell_rng_seeds(sim_seed, len(somata)),
+        comp_pulse_begin=pu_b, pulse_start=cat(trees.pulse_start, pts.pulse_start),
+        pulse_duration=cat(trees.pulse_duration, pts.pulse_duration),
+        pulse_amplitude=cat(trees.pulse_amplitude, pts.pulse_amplitude),
+        pulse_weight=cat(trees.pulse_weight, pts.pulse_weight),
+        syn_types=syn_types, comp_syn_begin=syn_begin, syn_type=stype[order], syn_weight=w[order],
+        syn_delay=delay[order], comp_spike_begin=spk_begin, spike_target=inst_of_conn[sorder],
+        record_comp=np.concatenate([somata, tgt_comp[:8]]), event_comp=somata,
+        name=f"mixed_network_{len(tree_sizes)}t{n_point}p")

@@ -28,12 +28,14 @@ using namespace edenhip;
 
 namespace {
 
-// A tree whose non-root compartments have no gated channel and no Poisson input: their Hines
-// diagonals are the same every step (the tree kernel's constant-tree path).
+// A tree whose non-root compartments have no gated channel, no Poisson input and no chemical
+// synapse: their Hines diagonals are the same every step (the tree kernel's constant-tree
+// path).
 bool const_below_root(const edenhip_model_desc *d, int64_t c0, int32_t n) {
     for (int32_t i = 1; i < n; i++) {
         const int64_t g = c0 + i;
         if (d->comp_poisson_begin[g + 1] != d->comp_poisson_begin[g]) return false;
+        if (d->comp_syn_begin[g + 1] != d->comp_syn_begin[g]) return false;
         for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++)
             if (d->channel_types[d->chan_type[ch]].n_gates != 0) return false;
     }
@@ -76,7 +78,8 @@ struct edenhip_ctx {
     float dt = 0;
     int64_t NC = 0, n_poi = 0, n_slots = 0, total_rows = 0, rows_done = 0, next_step = -3;
     int n_cells = 0;
-    bool network = false;
+    bool network = false;      // spike recipients: one launch per step
+    bool tree_net = false;     // tree kernels of the NET form (network, or synapses on tree cells)
     bool point_quad = false;   // point cells run four lanes per cell (no synapses, inputs, spikes)
     // host mirrors for output mapping
     std::vector<int32_t> desc_to_dev_col;   // per desc trace column
@@ -452,15 +455,20 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             inl.push_back((uint32_t)i);
             tcm[g] |= TCM_INPUT;
         }
+        // compartments with chemical synapses run them in their complex update
+        for (int32_t i = 0; i < n; i++) {
+            const int64_t g = c0 + i;
+            if (d->comp_syn_begin[g + 1] != d->comp_syn_begin[g]) tcm[g] |= TCM_SYN;
+        }
         tc.n_input = (int32_t)(inl.size() - tc.input_off);
         // complex compartments: anything but one ungated channel with no pulse, no Poisson
-        // input, no spike event column and no trace column; their constants go to a per-cell
-        // record table
+        // input, no synapse, no spike event column, no spike recipients and no trace column;
+        // their constants go to a per-cell record table
         tc.cx_off = (int32_t)cxv.size();
         for (int32_t i = 0; i < n; i++) {
             const int64_t g = c0 + i;
-            const bool simple =
-                (tcm[g] & 0xFFu) == 1u && pulse_begin[g + 1] == pulse_begin[g] && ev_col[g] < 0 && rec_col[g] < 0;
+            const bool simple = (tcm[g] & 0xFFFFu) == 1u && pulse_begin[g + 1] == pulse_begin[g] && ev_col[g] < 0 &&
+                                rec_col[g] < 0 && spk_begin[g + 1] == spk_begin[g];
             if (simple) continue;
             TreeComplex x{};
             x.k.C = C[g]; x.k.A = A[g]; x.k.invrc = invrc[g]; x.k.vthr = vthr[g];
@@ -501,11 +509,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             const float uj = -d->dt / (R * d->capacitance[c0 + j]);
             uv[c0 + P.pos[i]] = make_float2(ui, uj);
         }
-        // constant tree below the root: no gated channel and no Poisson input on any other
+        // constant tree below the root: no gated channel, Poisson input or synapse on any other
         // compartment, so their Hines diagonals are the same every step
         {
             bool ok = true;
-            for (int32_t i = 1; i < n && ok; i++) ok = (tcm[c0 + i] & 0xF8u) == 0u;
+            for (int32_t i = 1; i < n && ok; i++) ok = (tcm[c0 + i] & (0xF8u | TCM_SYN)) == 0u;
             tc.const_tree = ok ? 1 : 0;
             if (ok != P.merged) return fail(ctx, EDENHIP_E_INVALID, "internal: schedule format of a constant tree");
         }
@@ -534,8 +542,30 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         n_slots += 64 * kmax;
         if (n_slots >= ((int64_t)1 << 31)) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapse slots");
     }
+    // the same layout for the tree compartments that have synapses, 64 compartments a block
+    {
+        std::vector<int64_t> tsyn;
+        for (int32_t c : ctx->tree_desc)
+            for (int64_t g = d->cell_comp_begin[c]; g < d->cell_comp_begin[c + 1]; g++)
+                if (d->comp_syn_begin[g + 1] > d->comp_syn_begin[g]) tsyn.push_back(g);
+        for (size_t b0 = 0; b0 < tsyn.size(); b0 += 64) {
+            int64_t kmax = 0;
+            for (size_t l = 0; l < 64 && b0 + l < tsyn.size(); l++)
+                kmax = std::max<int64_t>(kmax, d->comp_syn_begin[tsyn[b0 + l] + 1] - d->comp_syn_begin[tsyn[b0 + l]]);
+            for (size_t l = 0; l < 64 && b0 + l < tsyn.size(); l++) {
+                const int64_t g = tsyn[b0 + l];
+                const int64_t cnt = d->comp_syn_begin[g + 1] - d->comp_syn_begin[g];
+                syn_base[g] = (int32_t)(n_slots + (int64_t)l);
+                syn_cnt[g] = (int32_t)cnt;
+                for (int64_t k = 0; k < cnt; k++) blocked_of[d->comp_syn_begin[g] + k] = n_slots + 64 * k + (int64_t)l;
+            }
+            n_slots += 64 * kmax;
+            if (n_slots >= ((int64_t)1 << 31)) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapse slots");
+        }
+        ctx->tree_net = !tsyn.empty();
+    }
     for (int64_t s = 0; s < n_syn; s++)
-        if (blocked_of[s] < 0) return fail(ctx, EDENHIP_E_LIMIT, "chemical synapses on multi-compartment cells are not supported yet");
+        if (blocked_of[s] < 0) return fail(ctx, EDENHIP_E_INVALID, "internal: synapse without a slot");
     std::vector<int32_t> syn_type(n_slots, 0), syn_orig(n_slots, -1), spk_target(n_spk);
     std::vector<float> syn_weight(n_slots, 0.f), syn_delay(n_slots, 0.f), syn_next(n_slots, -INFINITY);
     for (int64_t s = 0; s < n_syn; s++) {
@@ -549,6 +579,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     bool any_targets = n_spk > 0;
     ctx->n_slots = n_slots;
     ctx->network = any_targets;
+    ctx->tree_net = ctx->tree_net || ctx->network;
     ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0;
 
     // device uploads
@@ -689,14 +720,15 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // out of wave slots first and the extra wave per cell costs more than it hides. For
         // 2 and 4 cable waves the extra wave costs a workgroup per CU, which those sizes
         // need more.
+        // Networks run one step per launch with the NET kernels, which have no channel wave.
         b.split = (b.tpb == 512 || (b.tpb == 64 && (int64_t)bc.size() <= 4 * (int64_t)ctx->n_cu)) &&
-                  ctx->opt.persistent >= 0;
+                  ctx->opt.persistent >= 0 && !ctx->tree_net;
         // complex-compartment records go to LDS when they take at most 20 KiB
         b.cxmax_lds = b.cxmax <= 256 ? b.cxmax : 0;
         b.lds = (size_t)tree_lds(b.nmax, b.nvmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, ctx->n_gates,
                                  ctx->n_stypes, b.split).total *
                 sizeof(float);
-        const void *kp = tree_kernel_ptr(b.tpb, b.split);
+        const void *kp = tree_kernel_ptr(b.tpb, b.split, ctx->tree_net);
         if (!kp) return fail(ctx, EDENHIP_E_LIMIT, "no tree kernel for the chosen workgroup size");
         int per_cu = 0;
         HIPCHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, b.tpb + (b.split ? 64 : 0), b.lds));
@@ -993,7 +1025,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                                 &bk.words_max, (void *)&aa, (void *)&bb, &row0, &dt};
                 HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
                 if (pf && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));
-                HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split), dim3(bk.grid),
+                HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split, ctx->tree_net), dim3(bk.grid),
                                               dim3(bk.tpb + (bk.split ? 64 : 0)), args, bk.lds, bk.stream));
                 if (pf && last) HIPCHECK(ctx, hipEventRecord(bk.p1, bk.stream));
                 HIPCHECK(ctx, hipEventRecord(bk.done, bk.stream));
@@ -1246,7 +1278,8 @@ int edenhip_get_kernel_info(const edenhip_ctx *ctx, int32_t i, edenhip_kernel_in
         return 0;
     }
     const Bucket &b = ctx->buckets[(size_t)(i - (ctx->n_point > 0 ? 1 : 0))];
-    std::snprintf(out->name, sizeof(out->name), "tree_kernel<%d, %s>", b.tpb, b.split ? "true" : "false");
+    std::snprintf(out->name, sizeof(out->name), "tree_kernel<%d, %s, %s>", b.tpb, b.split ? "true" : "false",
+                  ctx->tree_net ? "true" : "false");
     out->threads_per_block = b.tpb + (b.split ? 64 : 0);
     out->grid = b.grid;
     out->n_cells = (int32_t)b.order.size();

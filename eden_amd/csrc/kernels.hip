@@ -273,7 +273,12 @@ __device__ __forceinline__ void channel_phase(const DGate *gates, const uint32_t
 // either in the cable group after the membrane phase, or, with SPLIT, in one extra wave at
 // the same time as the cable group's step; voltages and open fractions are then
 // double-buffered and the two meet at one workgroup barrier per step.
-template <int TPB, bool SPLIT>
+// NET: networks, launched one step at a time. A compartment with chemical synapses
+// (TCM_SYN) runs them in its complex update, with the trigger flags of the step's parity as
+// the point kernel does, and a compartment with spike recipients sends its spikes to the
+// next step's flags; both are complex compartments, so the per-slot code of the other
+// compartments is unchanged and the NET = false kernels carry none of it.
+template <int TPB, bool SPLIT, bool NET>
 __global__ void __launch_bounds__(TPB + (SPLIT ? 64 : 0)) __attribute__((amdgpu_waves_per_eu(4)))
 tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const DSynType *__restrict__ stypes_g,
             int n_stypes, const TreeCell *__restrict__ cells, const int32_t *__restrict__ task_order, int32_t n_tasks,
@@ -531,7 +536,8 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         __syncthreads();
 
         // diagonal and right-hand side of a complex compartment (slot j, cell-local index c)
-        auto complex_update = [&](int j, int c, const float *sF, float t, bool init, float &Dn, float &Vn) {
+        auto complex_update = [&](int j, int c, const float *sF, float t, bool init, int64_t s, float &Dn,
+                                  float &Vn) {
             const TreeComplex &cx = CX[fl[j] & 0xFFFFu];
             const CompConst k = cx.k;
             float I_ch, G_ch;
@@ -542,7 +548,16 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 I_in += in.x;
                 G_in += in.y;
             }
-            const float I_syn = 0.0f, G_syn = 0.0f;
+            float I_syn = 0.0f, G_syn = 0.0f;
+            if constexpr (NET) {
+                // synapses of this compartment, with the flags of this step's parity (the
+                // trigger buffers alternate as in point_kernel)
+                if (cx.tcm & TCM_SYN) {
+                    const int64_t g = c0 + c;
+                    synapses_step(T, stypes, T.syn_base[g], T.syn_cnt[g], V[j], init, t, dt, (int)((s + 3) & 1), s,
+                                  I_syn, G_syn);
+                }
+            }
             const float I_int = I_ch + I_in + I_syn;
             const float G_int = G_ch + G_in + G_syn;
             const float Dd = 1 * (G_int / k.C) + k.invrc;
@@ -619,7 +634,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const float I_int_s = (I_ch_s + 0.0f) + 0.0f;
                         float Dn = Ds[j];
                         float Vn = init ? V[j] : V[j] + (dt * (I_int_s + ((V[j] * Gs[j])))) / Cr[j];
-                        if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, Dn, Vn);
+                        if (!slot_simple[j] && c != 0 && (fl[j] & TCX_COMPLEX)) complex_update(j, c, sF, t, init, s, Dn, Vn);
                         sDX(posof(j)) = make_float2(Dn, Vn);
                     }
                     lds_barrier<TPB>();
@@ -700,7 +715,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                             // the root compartment's update, left out of the compartment phase: only
                             // the root fold reads the root's diagonal and right-hand side
                             float Dn = Ds[0], Vn = V[0];
-                            complex_update(0, 0, sF, t, false, Dn, Vn);
+                            complex_update(0, 0, sF, t, false, s, Dn, Vn);
                             sDX(0) = make_float2(Dn, Vn);
                         }
                         lds_barrier<TPB>();
@@ -762,7 +777,10 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                                 // started from
                                 if (!init && cx.k.rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + cx.k.rec] = V[j];
                                 const float vthr = cx.k.vthr;
-                                if (V[j] < vthr && vthr < Vn) log_event(T, cx.k.ev, s);
+                                if (V[j] < vthr && vthr < Vn) {
+                                    log_event(T, cx.k.ev, s);
+                                    if constexpr (NET) send_spike(T, c0 + c, (int)((s + 3) & 1) ^ 1);
+                                }
                             }
                             V[j] = Vn;
                             if (c < NV) sVn[c] = Vn;
@@ -798,32 +816,46 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
     }
 }
 
-#define EDENHIP_TREE_INST(TPB, SPLIT)                                                                              \
-    template __global__ void tree_kernel<TPB, SPLIT>(                                                              \
+#define EDENHIP_TREE_INST(TPB, SPLIT, NET)                                                                         \
+    template __global__ void tree_kernel<TPB, SPLIT, NET>(                                                         \
         DevTables, const DGate *, int, const DSynType *, int, const TreeCell *, const int32_t *, int32_t, int32_t *,  \
         const uint32_t *, const int32_t *, const float2 *, const uint32_t *, const uint32_t *, float *,             \
         const uint32_t *, const uint32_t *, const uint32_t *, const TreeComplex *, int32_t, int32_t, int32_t, int32_t, \
         int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
-EDENHIP_TREE_INST(64, true)
-EDENHIP_TREE_INST(64, false)
-EDENHIP_TREE_INST(128, false)
-EDENHIP_TREE_INST(256, false)
-EDENHIP_TREE_INST(512, true)
-EDENHIP_TREE_INST(512, false)
+EDENHIP_TREE_INST(64, true, false)
+EDENHIP_TREE_INST(64, false, false)
+EDENHIP_TREE_INST(128, false, false)
+EDENHIP_TREE_INST(256, false, false)
+EDENHIP_TREE_INST(512, true, false)
+EDENHIP_TREE_INST(512, false, false)
+EDENHIP_TREE_INST(64, false, true)
+EDENHIP_TREE_INST(128, false, true)
+EDENHIP_TREE_INST(256, false, true)
+EDENHIP_TREE_INST(512, false, true)
 
-const void *tree_kernel_ptr(int tpb, bool split) {
+const void *tree_kernel_ptr(int tpb, bool split, bool net) {
+    if (net) {
+        if (split) return nullptr;
+        switch (tpb) {
+        case 64: return reinterpret_cast<const void *>(&tree_kernel<64, false, true>);
+        case 128: return reinterpret_cast<const void *>(&tree_kernel<128, false, true>);
+        case 256: return reinterpret_cast<const void *>(&tree_kernel<256, false, true>);
+        case 512: return reinterpret_cast<const void *>(&tree_kernel<512, false, true>);
+        default: return nullptr;
+        }
+    }
     if (split) {
         switch (tpb) {
-        case 64: return reinterpret_cast<const void *>(&tree_kernel<64, true>);
-        case 512: return reinterpret_cast<const void *>(&tree_kernel<512, true>);
+        case 64: return reinterpret_cast<const void *>(&tree_kernel<64, true, false>);
+        case 512: return reinterpret_cast<const void *>(&tree_kernel<512, true, false>);
         default: return nullptr;
         }
     }
     switch (tpb) {
-    case 64: return reinterpret_cast<const void *>(&tree_kernel<64, false>);
-    case 128: return reinterpret_cast<const void *>(&tree_kernel<128, false>);
-    case 256: return reinterpret_cast<const void *>(&tree_kernel<256, false>);
-    case 512: return reinterpret_cast<const void *>(&tree_kernel<512, false>);
+    case 64: return reinterpret_cast<const void *>(&tree_kernel<64, false, false>);
+    case 128: return reinterpret_cast<const void *>(&tree_kernel<128, false, false>);
+    case 256: return reinterpret_cast<const void *>(&tree_kernel<256, false, false>);
+    case 512: return reinterpret_cast<const void *>(&tree_kernel<512, false, false>);
     default: return nullptr;
     }
 }

@@ -32,8 +32,10 @@ struct TreeCell {
 constexpr uint32_t TCX_COMPLEX = 1u << 31;
 
 // Compartment word of the tree kernel: channel count (bits 0-2), gated-channel-slot mask
-// (bits 3-6), Poisson-input flag (bit 7), first channel task (bits 16-31).
+// (bits 3-6), Poisson-input flag (bit 7), chemical-synapse flag (bit 8), first channel task
+// (bits 16-31).
 constexpr uint32_t TCM_INPUT = 1u << 7;
+constexpr uint32_t TCM_SYN = 1u << 8;
 
 constexpr int TREE_SLOTS = 4;        // compartments per thread
 
@@ -98,7 +100,9 @@ struct NetTables {
     unsigned long long *n_deliv;   // synaptic events received (spike-triggered), counted
 };
 
-const void *tree_kernel_ptr(int tpb, bool split);
+// net: the variant for networks (synapses on tree compartments, spikes sent to recipients),
+// launched one step at a time; only without a channel wave.
+const void *tree_kernel_ptr(int tpb, bool split, bool net = false);
 const void *net_window_kernel_ptr();
 const void *net_deliver_kernel_ptr();
 const void *point_kernel_ptr();

@@ -40,6 +40,10 @@
  *     fixes which one EDEN's build sees;
  *   - the random-call counters of the Poisson input (rng_offset + call count,
  *     eden/Eden.cpp:2455-2456): followed from the call sites, not fixed by any reference output;
+ *   - the layout of randof's stamp (which field goes to the high word, the 2^24 / 2^40 field
+ *     widths, eden/Eden.cpp:3223-3226): the tests pin the hash, the fields' periods, the
+ *     2^-23 resolution and the uniformity, but swapped fields would pass them; only a
+ *     reference output could fix the layout, and none exists;
  *   - the instance order of synapses from several projections onto one compartment
  *     (eden/Eden.cpp:8791-9281): followed from the source, not fixed by a fixture.
  */
@@ -135,6 +139,27 @@ static real rate_value(const edenhip_rate *r, real V) {
 static double *g_trace64 = NULL;
 
 void FN(set_trace64)(double *buf) { g_trace64 = buf; }
+
+/* Optional Poisson input firing log: (input index, step) of every firing of a
+ * poissonFiringSynapse (its OnCondition, LEMS_CoreComponents.inc.xml:655-659), for the tests of
+ * the input process. */
+static int64_t *g_poi_ev = NULL;
+static int64_t g_poi_ev_max = 0;
+static int64_t g_poi_ev_n = 0;
+
+void FN(set_poisson_log)(int64_t *buf, int64_t max) {
+    g_poi_ev = buf;
+    g_poi_ev_max = max;
+    g_poi_ev_n = 0;
+}
+
+int64_t FN(poisson_log_count)(void) { return g_poi_ev_n; }
+
+static void log_poi_event(int64_t p, int64_t step) {
+    if (!g_poi_ev) return;
+    int64_t k = __atomic_fetch_add(&g_poi_ev_n, 1, __ATOMIC_RELAXED);
+    if (k < g_poi_ev_max) g_poi_ev[k] = (step << 32) | p;
+}
 
 static int64_t *g_syn_ev = NULL;
 static int64_t g_syn_ev_max = 0;
@@ -334,7 +359,12 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
             for (int64_t p = d->comp_poisson_begin[g]; p < d->comp_poisson_begin[g + 1]; p++) {
                 real weight = (real)d->poisson_weight[p];
                 real tsince = s->pT[now][p], isi = s->pI[now][p];
+#if defined(ORACLE_PLANT_POISSON_RATE_PER_MS)
+                /* planted mistake for the tests: the rate read as per millisecond */
+                real averageIsi = (real)1 / ((real)d->poisson_rate[p] * (real)1000);
+#else
                 real averageIsi = (real)1 / (real)d->poisson_rate[p];
+#endif
                 int32_t inst = d->poisson_instance[p], roff = d->poisson_rng_offset[p];
                 int fired = 0;
                 if (initial_state) {
@@ -346,9 +376,13 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                     s->pT[nxt][p] = tsince + dt * deriv;
                     s->pI[nxt][p] = isi;
                     if (tsince > isi) {
+#if !defined(ORACLE_PLANT_POISSON_NO_RESET)
+                        /* (the tests' planted mistake leaves tsince running) */
                         s->pT[nxt][p] = 0;
+#endif
                         s->pI[nxt][p] = (-averageIsi) * LOG((real)1 - (real)randof_f(1.0f, rng_object_id, inst, step, roff + 1));
                         fired = 1;
+                        log_poi_event(p, step);
                     }
                 }
                 const edenhip_exptwo_type *st = &d->syn_types[d->poisson_syn_type[p]];

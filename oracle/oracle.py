@@ -43,6 +43,12 @@ def lib():
             st = getattr(L, f"oracle_{sfx}_set_trace64")
             st.restype = None
             st.argtypes = [C.c_void_p]
+            pl = getattr(L, f"oracle_{sfx}_set_poisson_log")
+            pl.restype = None
+            pl.argtypes = [C.c_void_p, C.c_int64]
+            pc = getattr(L, f"oracle_{sfx}_poisson_log_count")
+            pc.restype = C.c_int64
+            pc.argtypes = []
             o = getattr(L, f"oracle_{sfx}_dfs_order")
             o.restype = None
             o.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
@@ -66,11 +72,44 @@ def run_with_synapse_log(model, n_rows=-1, precision="f32", max_syn_events=10_00
     return out, (ev >> 32, ev & 0xFFFFFFFF)
 
 
-def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_000_000, want_trace=True):
+def run_with_poisson_log(model, n_rows=-1, precision="f32", max_events=10_000_000, library=None, **kw):
+    """run() plus the Poisson inputs' firings as sorted (step, input index) pairs. `library`:
+    another build of the oracle's source (the tests' planted-mistake variants)."""
+    L = library or lib()
+    buf = np.zeros(max_events, np.int64)
+    getattr(L, f"oracle_{precision}_set_poisson_log")(buf.ctypes.data, max_events)
+    try:
+        out = run(model, n_rows=n_rows, precision=precision, library=L, **kw)
+        n = int(getattr(L, f"oracle_{precision}_poisson_log_count")())
+    finally:
+        getattr(L, f"oracle_{precision}_set_poisson_log")(None, 0)
+    if n > max_events:
+        raise RuntimeError("poisson log overflow")
+    ev = np.sort(buf[:n])
+    return out, (ev >> 32, ev & 0xFFFFFFFF)
+
+
+def load_variant(path):
+    """A separately built copy of the oracle (same source, other macros), with the argument
+    types of the f32 entry points set."""
+    L = C.CDLL(path)
+    run_ = L.oracle_f32_run
+    run_.restype = C.c_int64
+    run_.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                     C.c_int64, C.POINTER(C.c_int64), C.c_void_p]
+    L.oracle_f32_set_poisson_log.restype = None
+    L.oracle_f32_set_poisson_log.argtypes = [C.c_void_p, C.c_int64]
+    L.oracle_f32_poisson_log_count.restype = C.c_int64
+    L.oracle_f32_poisson_log_count.argtypes = []
+    return L
+
+
+def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_000_000, want_trace=True,
+        library=None):
     """Run the restatement on a Model. Returns (trace [rows x n_record] float32,
     event columns, event rows, final voltages). With substeps > 1, dt is divided and the
     trace keeps every substeps-th row; event rows are then in fine steps."""
-    L = lib()
+    L = library or lib()
     d = model.desc()
     total = model.total_rows() if n_rows < 0 else n_rows
     nrec = len(model.record_comp)

@@ -9,7 +9,10 @@
   the order in which per-segment gradient sums are added differs, so equal to rounding).
 
 Models: config 5's construction (random trees, HH on every compartment, somatic current
-steps) at 3 cells x 24 compartments and 40 ms, so the oracle runs in well under a second.
+steps) at 3 cells x 24 compartments and 40 ms, so the oracle runs in well under a second, in
+each workgroup size the adjoint kernels have (64, 128 and 256 threads per cell); and config 5
+itself (256 cells x 100 compartments, 20,001 rows, checkpoints every 500 steps), whose cells
+take the 256-thread kernels by the size rule.
 """
 import sys
 
@@ -63,9 +66,11 @@ def _gradients(adj, g0, target):
     return g.grad.detach().cpu().numpy(), a.grad.detach().cpu().numpy(), a0
 
 
-def test_adjoint_gradient_matches_central_fd():
+@pytest.mark.parametrize("tpb", [0, 128, 256])
+def test_adjoint_gradient_matches_central_fd(tpb):
     m, g0 = _model()
-    with Adjoint(m, checkpoint_steps=37) as adj:
+    _say(f"[adjoint] central fd, threads_per_cell {tpb}")
+    with Adjoint(m, checkpoint_steps=37, threads_per_cell=tpb) as adj:
         gt, a0 = adj.default_parameters()
         target = adj.forward(gt, a0)
         gg, ga, a0 = _gradients(adj, g0, target)
@@ -134,3 +139,47 @@ def test_checkpoint_interval_does_not_change_the_gradient():
             res.append((gg, ga))
     np.testing.assert_allclose(res[0][0], res[1][0], rtol=1e-11, atol=1e-14 * np.abs(res[1][0]).max())
     np.testing.assert_allclose(res[0][1], res[1][1], rtol=1e-11, atol=1e-14 * np.abs(res[1][1]).max())
+
+
+def test_config5_full_size_forward_and_gradient():
+    """Config 5 at full size: the FP64 forward equals the oracle's FP64 flavour on 16 of its
+    cells (the cells are independent), and the gradient agrees with central differences
+    (h = 1e-6 relative) of the GPU's FP64 forward on 8 channel densities and 2 amplitudes."""
+    m, g0 = configs.config5()
+    g0 = np.asarray(g0, np.float32).astype(np.float64)
+    _say(f"[adjoint] config 5: {m.n_cells} cells x {m.n_comps // m.n_cells} comps, {m.total_rows()} rows")
+    with Adjoint(m, checkpoint_steps=500) as adj:
+        gt, a0 = adj.default_parameters()
+        target = adj.forward(gt, a0)
+        tr = target.cpu().numpy()
+        cells = np.arange(0, m.n_cells, m.n_cells // 16)
+        sub, rc, _ = m.subset(cells)
+        ref = oracle.run_f64_trace(sub)
+        err = float(np.abs(tr[:, rc] - ref).max())
+        _say(f"[adjoint] config 5 forward, 16 cells: max |V_gpu64 - V_oracle64| = {err:.3g} mV")
+        assert err < 1e-8
+        gg, ga, a0 = _gradients(adj, g0, target)
+        fwd_ms, bwd_ms = adj.timing()
+        _say(f"[adjoint] config 5 forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms")
+
+        def L(g, a):
+            return float(_loss(adj.forward(torch.tensor(g, dtype=torch.float64, device=DEV), a), target))
+
+        rng = np.random.default_rng(1)
+        big = np.argsort(-np.abs(gg))[:4]
+        pick = np.unique(np.concatenate([big, rng.choice(len(g0), 4, replace=False)]))
+        scale = float(np.abs(gg).max())
+        for i in pick:
+            gp, gm = g0.copy(), g0.copy()
+            gp[i] += 1e-6 * abs(g0[i])
+            gm[i] -= 1e-6 * abs(g0[i])
+            fd = (L(gp, a0) - L(gm, a0)) / (gp[i] - gm[i])
+            _say(f"[adjoint] config 5 gbase[{i}]: adjoint {gg[i]:.9g}  fd {fd:.9g}")
+            assert abs(gg[i] - fd) <= 1e-4 * abs(fd) + 1e-9 * scale
+        for p in np.argsort(-np.abs(ga))[:2]:
+            ap, am = a0.clone(), a0.clone()
+            ap[p] += 1e-6 * abs(float(a0[p]))
+            am[p] -= 1e-6 * abs(float(a0[p]))
+            fd = (L(g0, ap) - L(g0, am)) / float(ap[p] - am[p])
+            _say(f"[adjoint] config 5 amplitude[{p}]: adjoint {ga[p]:.9g}  fd {fd:.9g}")
+            assert abs(ga[p] - fd) <= 1e-4 * abs(fd)

@@ -5,6 +5,9 @@ trace must not depend on the batch it runs in. A dozen cells of every workgroup 
 taken from the full run and compared, bit for bit, with the oracle run on those cells alone
 and with a GPU run of those cells alone.
 
+config 2 in full (512 cells, 10,001 rows): every cell's trace and spike raster against the
+oracle on the same model, exactly.
+
 config 3 at full scale (100,000 cells, 10.1M synapses), the first 15 ms: recorded voltages,
 spike raster and every synaptic delivery (step, synapse instance) against the oracle on the
 same model, exactly.
@@ -55,6 +58,19 @@ def test_config4_full_run_cells_equal_oracle_and_standalone():
         alone = s2.run_all()
     assert np.array_equal(alone, ref)
     _say(f"[fullsize] {len(pick)} cells x {tr.shape[0]} rows identical; {len(o_ev)} spikes")
+
+
+def test_config2_full_equals_oracle():
+    m = configs.config2()
+    _say(f"[fullsize] {m.name}: {m.n_cells} cells, {m.n_comps} compartments, {m.total_rows()} rows")
+    with Simulation(m) as sim:
+        tr = sim.run_all()
+        ev = sim.read_events()
+    ref, ec, er, _ = oracle.run(m)
+    _say(f"[fullsize] config 2: spikes gpu {len(ev[1])} oracle {len(er)}")
+    assert len(er) > 0
+    assert np.array_equal(tr, ref), float(np.abs(tr - ref).max())
+    assert np.array_equal(ev[0], ec) and np.array_equal(ev[1], er)
 
 
 def test_config3_full_scale_first_15ms_bit_exact():

@@ -162,7 +162,7 @@ def test_many_small_trees_without_channel_wave():
         sim.run_all()
         infos = sim.kernel_infos()
     _say(f"[parity] kernels {[(k['name'], k['threads_per_block'], k['grid'], k['lds_bytes']) for k in infos]}")
-    assert [k["name"] for k in infos] == ["tree_kernel<64, false>"]
+    assert [k["name"] for k in infos] == ["tree_kernel<64, false, false>"]
     assert infos[0]["threads_per_block"] == 64 and infos[0]["lds_bytes"] > 0
     res, g, r = _check(m)
     assert res["raster_mismatch_cols"] == 0, res
@@ -250,6 +250,45 @@ def test_synaptic_events_bit_exact(window):
     assert np.array_equal(ev_g[0], ec) and np.array_equal(ev_g[1], er)
     assert np.array_equal(g_step, o_step)
     assert np.array_equal(g_syn, o_syn)
+
+
+@pytest.mark.parametrize("hh_everywhere", [False, True])
+def test_network_with_tree_cells(hh_everywhere):
+    """Tree cells and one-compartment cells in one network: synapses on dendrites and somata
+    of tree cells, spikes sent by tree somata and by point cells. Traces, spike raster and
+    every synaptic delivery equal the oracle's, bit for bit."""
+    m = configs.mixed_network(tree_sizes=(40, 90, 150, 300, 700), n_point=24, n_conn=400, t_final_ms=80.0,
+                              seed=13 + int(hh_everywhere), hh_everywhere=hh_everywhere)
+    _say(f"[parity] {m.name} cells={m.n_cells} comps={m.n_comps} synapses={m.n_synapses}")
+    with Simulation(m, log_synapse_events=1_000_000, profile=True) as sim:
+        tr_g = sim.run_all()
+        ev_g = sim.read_events()
+        g_step, g_syn = sim.read_synapse_events()
+        names = [k["name"] for k in sim.kernel_infos()]
+    _say(f"[parity] kernels {names}")
+    assert all(n == "point_kernel" or n.endswith(", false, true>") for n in names), names
+    (tr_r, ec, er, _), (o_step, o_syn) = oracle.run_with_synapse_log(m)
+    _say(f"[parity] spikes gpu {len(ev_g[1])} oracle {len(er)}, synaptic events gpu {len(g_step)} oracle {len(o_step)}")
+    assert np.array_equal(ev_g[0], ec) and np.array_equal(ev_g[1], er)
+    assert np.array_equal(tr_g, tr_r)
+    assert np.array_equal(g_step, o_step) and np.array_equal(g_syn, o_syn)
+    # the case is exercised: tree somata spiked, and synapses on tree dendrites received events
+    n_tree = 5
+    assert np.any(ec < n_tree), "no tree soma spiked"
+    comp_of_syn = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
+    tgt = comp_of_syn[o_syn]
+    dendrite = np.isin(tgt, np.setdiff1d(np.arange(m.cell_comp_begin[n_tree]), m.cell_comp_begin[:n_tree]))
+    assert dendrite.sum() > 10, dendrite.sum()
+
+
+def test_tree_synapses_without_recipients():
+    """Synapses on tree compartments and no spike recipients anywhere: the NET tree kernels
+    in persistent launches; identical to the oracle."""
+    import dataclasses
+    m = configs.mixed_network(tree_sizes=(60, 120), n_point=0, n_conn=30, t_final_ms=20.0, seed=3)
+    m = dataclasses.replace(m, comp_spike_begin=np.zeros(m.n_comps + 1, np.int64), spike_target=np.zeros(0, np.int64),
+                            name="tree_synapses_no_recipients")
+    _check(m)
 
 
 @pytest.mark.parametrize("delay_ms", [1.0, 15.0])

@@ -419,3 +419,103 @@ def test_model_subset_runs_as_in_the_full_batch():
     new = {int(c): i for i, c in enumerate(ec)}
     assert sorted(zip(r_full[keep].tolist(), [new[int(c)] for c in c_full[keep]])) == \
         sorted(zip(r_sub.tolist(), c_sub.tolist()))
+
+
+# ---------------------------------------------------------------- Poisson input process
+# poissonFiringSynapse (LEMS_CoreComponents.inc.xml:623-672; emitted at eden/Eden.cpp:3859-3912):
+# OnStart tsince = 0, isi = -averageIsi log(1 - random(1)); every step tsince += dt; when
+# tsince > isi: tsince = 0, a new isi, a spike. After a firing at step s0 the step s0 + 1 sees
+# tsince = 0 and step s0 + 1 + j sees j dt, so the next firing is floor(isi / dt) + 2 steps
+# later: the firing interval in steps is k = floor(isi / dt) + 2 with isi ~ Exp(rate). Then
+# (k - 2 + U) dt, U ~ U(0, 1), is Exp(rate) up to the shape of the density inside one dt
+# (a relative error of order rate * dt), and the mean interval is
+# dt (e^{-r dt} / (1 - e^{-r dt}) + 2). OnStart draws the first isi at step 0, so step 0
+# starts an interval too. Intervals still open at the end of the run are never seen, which
+# would bias the sample to short ones: only intervals that start at least 15 mean intervals
+# before the end are used (the choice depends on the start alone, so it selects intervals
+# without regard to their length; one would still be open with probability e^-15).
+
+def _poisson_model(rate_hz, n_cells=200, t_final_ms=None):
+    if t_final_ms is None:
+        t_final_ms = 5000.0 if rate_hz < 50 else 2500.0
+    nc = n_cells
+    passive = [channel_type([])]
+    return Model(
+        dt=25.0, t_final=t_final_ms * 1000.0, cell_comp_begin=np.arange(nc + 1, dtype=np.int64),
+        v0=np.full(nc, -65.0), capacitance=np.full(nc, 10.0), area=np.full(nc, 1000.0),
+        inv_rc_axial=np.zeros(nc), r_axial=np.full(nc, np.nan), v_threshold=np.full(nc, np.nan),
+        parent=np.full(nc, -1), channel_types=passive, comp_chan_begin=np.arange(nc + 1, dtype=np.int64),
+        chan_type=np.zeros(nc), chan_gbase=np.full(nc, 0.3), chan_erev=np.full(nc, -65.0),
+        cell_rng_seed=configs.cell_rng_seeds(7, nc),
+        syn_types=[N.ExpTwoType(1e-6, 0.0, 500.0, 5000.0, 1.0)],
+        comp_poisson_begin=np.arange(nc + 1, dtype=np.int64), poisson_rate=np.full(nc, rate_hz * 1e-6),
+        poisson_syn_type=np.zeros(nc), poisson_weight=np.ones(nc), poisson_instance=np.zeros(nc),
+        poisson_rng_offset=np.zeros(nc), name=f"poisson_{rate_hz}Hz")
+
+
+def _poisson_stats(model, rate_hz, library=None):
+    """(count, expected count, KS p-value of the dt-corrected intervals against Exp(rate))."""
+    from scipy import stats
+    _, (step, inp) = oracle.run_with_poisson_log(model, library=library)
+    dt = 25.0
+    r = rate_hz * 1e-6
+    T = (model.total_rows() - 1) * dt
+    q = np.exp(-r * dt)
+    mean_isi = dt * (q / (1.0 - q) + 2.0)
+    expected = model.n_cells * T / mean_isi
+    t_cut = T - 15.0 / r
+    ks = []
+    for c in range(model.n_cells):
+        f = np.concatenate([[0], step[inp == c]])
+        ks.append(np.diff(f)[(f[:-1] * dt < t_cut)[: len(f) - 1]])
+    k = np.concatenate(ks)
+    u = np.random.default_rng(3).random(len(k))
+    x = (k - 2 + u) * dt
+    p = stats.kstest(x, "expon", args=(0.0, 1.0 / r)).pvalue if len(x) > 10 else 0.0
+    return len(step), expected, p
+
+
+@pytest.fixture(scope="module")
+def planted_oracles(tmp_path_factory):
+    """The oracle's source built with each planted mistake (test-only macros)."""
+    import os
+    import subprocess
+    src = os.path.join(os.path.dirname(oracle.__file__), "eden_oracle.c")
+    out = {}
+    d = tmp_path_factory.mktemp("planted")
+    for name, macro in (("rate_per_ms", "ORACLE_PLANT_POISSON_RATE_PER_MS"), ("no_reset", "ORACLE_PLANT_POISSON_NO_RESET")):
+        so = str(d / f"oracle_{name}.so")
+        subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-fopenmp", "-shared", "-DREAL=float", "-DSFX=f32",
+                        f"-D{macro}", src, "-o", so, "-lm"], check=True)
+        out[name] = oracle.load_variant(so)
+    return out
+
+
+@pytest.mark.parametrize("rate_hz", [5.0, 200.0])
+def test_poisson_input_count_and_interval_distribution(rate_hz):
+    m = _poisson_model(rate_hz)
+    n, expected, p = _poisson_stats(m, rate_hz)
+    assert m.total_rows() >= 100_000
+    assert abs(n - expected) <= 4.0 * np.sqrt(expected), (n, expected)
+    assert p > 1e-3, p
+
+
+@pytest.mark.parametrize("mistake", ["rate_per_ms", "no_reset"])
+def test_poisson_checks_fail_on_planted_mistakes(planted_oracles, mistake):
+    for rate_hz in (5.0, 200.0):
+        m = _poisson_model(rate_hz, t_final_ms=500.0)
+        n, expected, p = _poisson_stats(m, rate_hz, library=planted_oracles[mistake])
+        count_ok = abs(n - expected) <= 4.0 * np.sqrt(expected)
+        assert not (count_ok and p > 1e-3), (mistake, rate_hz, n, expected, p)
+
+
+def test_randof_field_periods():
+    """Each stamp field wraps at its width (eden/Eden.cpp:3223-3226): the invocation field at
+    2^24 (it is shifted by 40 in a 64-bit word), the instance at 2^24, the step at 2^40, and a
+    negative work item (sign-extended) differs from its 32-bit unsigned reading."""
+    for inv in (0, 1, 5):
+        assert oracle.randof(1.0, 9, 4, 77, inv) == oracle.randof(1.0, 9, 4, 77, inv + (1 << 24))
+        assert oracle.randof(1.0, 9, 4, 77, inv) != oracle.randof(1.0, 9, 4, 77, inv + (1 << 23))
+    assert oracle.randof(1.0, 9, 5, 77, 0) != oracle.randof(1.0, 9, 5 + (1 << 23), 77, 0)
+    assert oracle.randof(1.0, 9, 5, 77, 0) != oracle.randof(1.0, 9, 5, 77 + (1 << 39), 0)
+    assert oracle.randof(1.0, -1, 5, 77, 0) != oracle.randof(1.0, 0x7FFFFFFF, 5, 77, 0)
