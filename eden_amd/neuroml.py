@@ -1,0 +1,808 @@
+"""NeuroML 2 / LEMS reader for the GPU path: a LEMS or NeuroML file becomes the instantiated
+model (`eden_amd.model.Model`) and the description of its output files.
+
+This is the subset of EDEN's front end (ReadNeuroML, eden/NeuroML.cpp:9572-10068, and the
+instantiation in GenerateModel, eden/Eden.cpp:677-10377) that the per-step loop's mechanism
+set covers:
+
+  includes          <Include file> (LEMS) and <include href> (NeuroML), resolved relative to
+                    the including file; NeuroML's core type files are skipped
+                    (eden/NeuroML.cpp:9813-9891)
+  ion channels      ionChannelHH / ionChannel with gateHHrates (HHExpRate, HHExpLinearRate,
+                    HHSigmoidRate), q10Settings (q10Fixed, q10ExpTemp); channels without gates
+                    are passive (eden/NeuroML.cpp:4380-4598)
+  cells             <cell>: morphology (segments, parent / fractionAlong, implicit proximal
+                    points, segmentGroups with member / include), biophysicalProperties
+                    (channelDensity, specificCapacitance, initMembPotential, spikeThresh,
+                    resistivity), each over a segment group, later specs overriding earlier
+                    ones (eden/NeuroML.cpp:2530-3566); one compartment per segment (EDEN's
+                    free-standing segments, eden/Eden.cpp:1437-1491)
+  synapses          expTwoSynapse (eden/NeuroML.cpp:6506-6829)
+  inputs            pulseGenerator, poissonFiringSynapse (eden/NeuroML.cpp:6968-7107)
+  networks          population (size or instances), projection with connection /
+                    connectionWD, synapticConnection, explicitInput, inputList with
+                    input / inputW (eden/NeuroML.cpp:6136-6420)
+  simulation        <Simulation length step seed target> with OutputFile / OutputColumn and
+                    EventOutputFile / EventSelection, <Target> (eden/NeuroML.cpp:7979-8311)
+
+Instantiation follows the reference's order: cells numbered over populations in order (the
+work-item / RNG id, eden/Eden.cpp:7857-7911); per compartment, channel distributions in the
+order of their channelDensity elements; pulse and Poisson inputs in network input order
+(eden/Eden.cpp:8791-8919); synapse instances per (compartment, synapse type) in projection
+then connection order, the types of a compartment in the order their synapse elements were
+read (IdListRle is a sorted set of component sequence numbers, eden/Eden.cpp:1861-1930,
+8931-9281); spike recipients per source compartment in connection order
+(AddSpikeTarget, eden/Eden.cpp:8029-8046). Quantities convert to engine units as
+ParseQuantity does: the number read as a double, times 10^(unit power - native power), then
+stored as float (eden/NeuroML.cpp:3046-3077, eden/NeuroML.h:530-532).
+
+Constructs outside this set (cables, other channel and synapse types, artificial cells,
+continuous / electrical projections) raise NeuroMLError naming the element.
+"""
+import os
+import re
+import xml.etree.ElementTree as ET
+from collections import OrderedDict
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .model import Model, cell_rng_seeds, channel_type
+
+
+class NeuroMLError(ValueError):
+    pass
+
+
+# ---------------------------------------------------------------- units (eden/NeuroML.cpp:305-425)
+# dimension -> (native power of ten, {unit name: power of ten}); every scale factor is 1
+_DIMS = {
+    "time": (-6, {"s": 0, "ms": -3, "msec": -3, "us": -6, "usec": -6, "sec": 0, "second": 0, "seconds": 0,
+                  "hour": None}),
+    "frequency": (6, {"per_s": 0, "Hz": 0, "per_ms": 3, "kHz": 3, "per_us": 6, "MHz": 6}),
+    "length": (-6, {"m": 0, "cm": -2, "mm": -3, "um": -6}),
+    "voltage": (-3, {"V": 0, "mV": -3}),
+    "current": (-9, {"A": 0, "mA": -3, "uA": -6, "nA": -9, "pA": -12}),
+    "resistance": (6, {"ohm": 0, "kohm": 3, "Mohm": 6}),
+    "conductance": (-6, {"S": 0, "mS": -3, "uS": -6, "nS": -9, "pS": -12}),
+    "resistivity": (1, {"kohm_cm": 1, "ohm_m": 0, "ohm_cm": -2}),
+    "conductivity": (1, {"S_per_m2": 0, "mS_per_cm2": 1, "S_per_cm2": 4, "S_per_mm2": 6}),
+    "capacitance": (-12, {"F": 0, "mF": -3, "uF": -6, "nF": -9, "pF": -12}),
+    "specific_capacitance": (-2, {"uF_per_cm2": -2, "F_per_m2": 0, "uF_per_mm2": 0}),
+    "temperature": (0, {"K": 0, "degK": 0, "degC": "degC"}),
+}
+
+_NUM = re.compile(r"^\s*([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)\s*(\S*)\s*$")
+
+
+def convert(text, dim, what="quantity"):
+    """Engine-unit value (a Python float, the double EDEN computes before storing it) of a
+    NeuroML quantity such as '-65mV' or '3.0 S_per_m2'."""
+    m = _NUM.match(text or "")
+    if not m:
+        raise NeuroMLError(f"{what}: '{text}' is not a number followed by a unit")
+    value, unit = float(m.group(1)), m.group(2)
+    if dim is None:
+        if unit:
+            raise NeuroMLError(f"{what}: '{text}' should have no unit")
+        return value
+    native, units = _DIMS[dim]
+    if unit not in units:
+        raise NeuroMLError(f"{what}: unknown {dim} unit '{unit}' (supported: {' '.join(units)})")
+    p = units[unit]
+    if p == "degC":
+        return value + 273.15
+    if p is None:
+        return value * 3600.0 * 10.0 ** (0 - native)
+    # ScaleEntry::ConvertTo: value * (scale / to.scale) * pow10(pow - to.pow) + offset term 0
+    return value * (1.0 / 1.0) * 10.0 ** (p - native) + 0.0
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _tag(el):
+    t = el.tag
+    return t.rsplit("}", 1)[-1] if "}" in t else t
+
+
+# NeuroML's own type-definition files, included by LEMS files for jLEMS; EDEN embeds its own
+# core components instead and skips them (eden/NeuroML.cpp:9837-9862)
+_CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Synapses.xml", "Inputs.xml",
+                  "Channels.xml", "NeuroMLCoreDimensions.xml", "NeuroMLCoreCompTypes.xml"}
+
+# element names of the synapse and input types this reader instantiates, by role; a LEMS
+# <Component type="..."> of these types is read as the element itself
+_SYNAPSES = {"expTwoSynapse"}
+_INPUTS = {"pulseGenerator", "poissonFiringSynapse"}
+_UNSUPPORTED_SYNAPSES = {"alphaCurrentSynapse", "alphaSynapse", "expOneSynapse", "expThreeSynapse",
+                         "blockingPlasticSynapse", "silentSynapse", "gapJunction", "linearGradedSynapse",
+                         "gradedSynapse", "expCondSynapse", "alphaCondSynapse", "expCurrSynapse",
+                         "alphaCurrSynapse", "doubleSynapse"}
+_UNSUPPORTED_CELLS = {"iafCell", "iafTauCell", "iafRefCell", "iafTauRefCell", "izhikevichCell",
+                      "izhikevich2007Cell", "adExIaFCell", "fitzHughNagumoCell", "pinskyRinzelCA3Cell",
+                      "IF_curr_alpha", "IF_curr_exp", "IF_cond_alpha", "IF_cond_exp", "EIF_cond_exp_isfa_ista",
+                      "EIF_cond_alpha_isfa_ista", "HH_cond_exp", "SpikeSourcePoisson", "spikeGenerator",
+                      "spikeGeneratorRandom", "spikeGeneratorPoisson", "spikeArray", "timedSynapticInput"}
+
+
+@dataclass
+class OutputFileSpec:
+    id: str
+    file_name: str
+    columns: list = field(default_factory=list)      # (column id, quantity path, compartment)
+
+
+@dataclass
+class EventFileSpec:
+    id: str
+    file_name: str
+    format: str
+    selections: list = field(default_factory=list)   # (selection id int, select path, compartment)
+
+
+@dataclass
+class Outputs:
+    trajectories: list
+    events: list
+    base_dir: str
+
+
+class _Doc:
+    """Elements of a LEMS / NeuroML file and of the files it includes, in reading order."""
+
+    def __init__(self, path):
+        self.items = OrderedDict()       # id -> (role, element), role in cell/channel/synapse/input/...
+        self.order = []                  # (role, id) in document order
+        self.networks = OrderedDict()
+        self.simulations = OrderedDict()
+        self.morphologies = {}
+        self.biophysics = {}
+        self.target = None
+        self.seen = set()
+        self._read(os.path.abspath(path))
+
+    def _read(self, path):
+        if path in self.seen:
+            return
+        self.seen.add(path)
+        try:
+            root = ET.parse(path).getroot()
+        except (OSError, ET.ParseError) as e:
+            raise NeuroMLError(f"cannot read {path}: {e}") from None
+        base = os.path.dirname(path)
+        self._walk(root, base)
+
+    def _include(self, name, base):
+        p = os.path.join(base, name)
+        if not os.path.exists(p):
+            if os.path.basename(name) in _CORE_INCLUDES:
+                return
+            raise NeuroMLError(f"included file {name} not found (relative to {base})")
+        if os.path.basename(name) in _CORE_INCLUDES and not p.endswith(".nml"):
+            return
+        self._read(p)
+
+    def _add(self, role, el):
+        i = el.get("id")
+        if not i:
+            raise NeuroMLError(f"<{_tag(el)}> without an id")
+        if i in self.items:
+            raise NeuroMLError(f"id '{i}' defined twice")
+        self.items[i] = (role, el)
+        self.order.append((role, i))
+
+    def _walk(self, root, base):
+        for el in root:
+            t = _tag(el)
+            if t == "Component" and el.get("type"):
+                # a LEMS component of a NeuroML type: read as that element
+                el = _as_element(el)
+                t = _tag(el)
+            if t == "Include":
+                self._include(el.get("file", ""), base)
+            elif t == "include":
+                self._include(el.get("href", ""), base)
+            elif t == "Target":
+                self.target = el.get("component")
+            elif t in ("ionChannelHH", "ionChannel", "ionChannelPassive"):
+                self._add("channel", el)
+            elif t == "cell":
+                self._add("cell", el)
+            elif t == "morphology":
+                self.morphologies[el.get("id")] = el
+            elif t == "biophysicalProperties":
+                self.biophysics[el.get("id")] = el
+            elif t in _SYNAPSES:
+                self._add("synapse", el)
+            elif t in _INPUTS:
+                self._add("input", el)
+            elif t == "network":
+                self.networks[el.get("id")] = el
+            elif t == "Simulation":
+                self.simulations[el.get("id")] = el
+            elif t in _UNSUPPORTED_SYNAPSES or t in _UNSUPPORTED_CELLS:
+                # read, and refused only when the model uses it
+                self._add("unsupported", el)
+            elif t == "neuroml":
+                self._walk(el, base)
+
+
+def _as_element(comp):
+    """<Component type="X" id=... a=...> -> an element <X id=... a=...> with the same children."""
+    el = ET.Element(comp.get("type"), {k: v for k, v in comp.attrib.items() if k != "type"})
+    el.extend(list(comp))
+    return el
+
+
+# ---------------------------------------------------------------- ion channels
+
+_RATE_FORMS = {"HHExpRate": N.RATE_EXP, "HHExpLinearRate": N.RATE_EXPLINEAR, "HHSigmoidRate": N.RATE_SIGMOID}
+
+
+def _rate(el, what):
+    t = el.get("type")
+    if t not in _RATE_FORMS:
+        raise NeuroMLError(f"{what}: rate type '{t}' is not supported (HHExpRate, HHExpLinearRate, HHSigmoidRate)")
+    r = convert(el.get("rate"), "frequency", what + " rate")
+    mid = convert(el.get("midpoint"), "voltage", what + " midpoint")
+    sc = convert(el.get("scale"), "voltage", what + " scale")
+    return N.Rate(_RATE_FORMS[t], _f32(r), _f32(mid), _f32(sc))
+
+
+def _channel(el, temperature):
+    cid = el.get("id")
+    gates = []
+    for g in el:
+        t = _tag(g)
+        if t in ("notes", "annotation"):
+            continue
+        if t != "gateHHrates" and not (t == "gate" and g.get("type") == "gateHHrates"):
+            raise NeuroMLError(f"channel {cid}: <{t}> gates are not supported (gateHHrates only)")
+        inst = int(g.get("instances", "1"))
+        fw = rv = None
+        q10 = 1.0
+        for c in g:
+            ct = _tag(c)
+            if ct == "forwardRate":
+                fw = _rate(c, f"channel {cid} gate {g.get('id')} forward")
+            elif ct == "reverseRate":
+                rv = _rate(c, f"channel {cid} gate {g.get('id')} reverse")
+            elif ct == "q10Settings":
+                qt = c.get("type")
+                if qt == "q10Fixed":
+                    q10 = convert(c.get("fixedQ10"), None, "fixedQ10")
+                elif qt == "q10ExpTemp":
+                    if temperature is None:
+                        raise NeuroMLError(f"channel {cid}: q10ExpTemp needs the network's temperature")
+                    f = convert(c.get("q10Factor"), None, "q10Factor")
+                    t0 = convert(c.get("experimentalTemp"), "temperature", "experimentalTemp")
+                    q10 = f ** ((temperature - t0) / 10.0)
+                else:
+                    raise NeuroMLError(f"channel {cid}: q10Settings type '{qt}' is not supported")
+        if fw is None or rv is None:
+            raise NeuroMLError(f"channel {cid} gate {g.get('id')}: forwardRate and reverseRate are required")
+        gates.append((inst, fw, rv, _f32(q10)))
+    if len(gates) > N.MAX_GATES:
+        raise NeuroMLError(f"channel {cid}: more than {N.MAX_GATES} gates")
+    return channel_type(gates)
+
+
+# ---------------------------------------------------------------- cells
+
+@dataclass
+class _Cell:
+    id: str
+    seg_ids: list            # segment id per compartment (segments in file order, parents first)
+    parent: np.ndarray       # compartment parent (cell-local), -1 for the root
+    proximal: np.ndarray     # [n, 4]
+    distal: np.ndarray
+    groups: dict             # group id -> set of compartment indices
+    channels: list           # per compartment: [(channel id, gbase mS/cm^2, erev mV)]
+    cm: np.ndarray
+    ra: np.ndarray
+    v0: np.ndarray
+    vthr: np.ndarray
+    area: np.ndarray = None
+    cap: np.ndarray = None
+    r_axial: np.ndarray = None
+    inv_rc: np.ndarray = None
+
+    def comp_of_segment(self, seg_id, what):
+        try:
+            return self.seg_ids.index(int(seg_id))
+        except ValueError:
+            raise NeuroMLError(f"{what}: segment {seg_id} not in cell {self.id}") from None
+
+
+def _point(el):
+    return [convert(el.get(k), None, f"point {k}") for k in ("x", "y", "z", "diameter")]
+
+
+def _morphology(cid, m):
+    segs = [s for s in m if _tag(s) == "segment"]
+    if not segs:
+        raise NeuroMLError(f"cell {cid}: morphology without segments")
+    ids = [int(s.get("id")) for s in segs]
+    index = {sid: k for k, sid in enumerate(ids)}
+    n = len(segs)
+    parent = np.full(n, -1, np.int32)
+    prox = np.zeros((n, 4))
+    dist = np.zeros((n, 4))
+    for k, s in enumerate(segs):
+        pe = next((c for c in s if _tag(c) == "parent"), None)
+        pr = next((c for c in s if _tag(c) == "proximal"), None)
+        di = next((c for c in s if _tag(c) == "distal"), None)
+        if di is None:
+            raise NeuroMLError(f"cell {cid} segment {ids[k]}: no distal point")
+        dist[k] = _point(di)
+        if pe is not None:
+            p = index.get(int(pe.get("segment")))
+            if p is None or p >= k:
+                raise NeuroMLError(f"cell {cid} segment {ids[k]}: parent must be a segment listed before it")
+            parent[k] = p
+            f = convert(pe.get("fractionAlong", "1"), None, "fractionAlong")
+            if pr is None:
+                # the implicit proximal point lies fractionAlong along the parent
+                prox[k] = prox[p] + (dist[p] - prox[p]) * f
+        elif k != 0:
+            raise NeuroMLError(f"cell {cid}: segment {ids[k]} has no parent (only the first segment may be the root)")
+        if pr is not None:
+            prox[k] = _point(pr)
+        elif pe is None:
+            raise NeuroMLError(f"cell {cid} segment {ids[k]}: a root segment needs a proximal point")
+    groups = {"all": set(range(n))}
+    raw = {}
+    for g in m:
+        if _tag(g) != "segmentGroup":
+            continue
+        gid = g.get("id")
+        if g.get("neuroLexId") == "sao864921383":
+            raise NeuroMLError(f"cell {cid}: segment group {gid} is a cable (sao864921383); cable discretisation "
+                               "is not supported by this reader (one compartment per segment)")
+        raw[gid] = g
+
+    def resolve(gid, stack=()):
+        if gid in groups:
+            return groups[gid]
+        if gid not in raw or gid in stack:
+            raise NeuroMLError(f"cell {cid}: segment group {gid} not found")
+        out = set()
+        for c in raw[gid]:
+            t = _tag(c)
+            if t == "member":
+                s = index.get(int(c.get("segment")))
+                if s is None:
+                    raise NeuroMLError(f"cell {cid} group {gid}: segment {c.get('segment')} not found")
+                out.add(s)
+            elif t == "include":
+                out |= resolve(c.get("segmentGroup"), stack + (gid,))
+            elif t in ("path", "subTree"):
+                raise NeuroMLError(f"cell {cid} group {gid}: <{t}> is not supported")
+        groups[gid] = out
+        return out
+
+    for gid in raw:
+        resolve(gid)
+    return ids, parent, prox.astype(np.float32), dist.astype(np.float32), groups
+
+
+def _cell(doc, el, temperature, channel_index):
+    cid = el.get("id")
+    m = next((c for c in el if _tag(c) == "morphology"), None)
+    if m is None:
+        m = doc.morphologies.get(el.get("morphology"))
+    if m is None:
+        raise NeuroMLError(f"cell {cid}: no morphology")
+    b = next((c for c in el if _tag(c) == "biophysicalProperties"), None)
+    if b is None:
+        b = doc.biophysics.get(el.get("biophysicalProperties"))
+    if b is None:
+        raise NeuroMLError(f"cell {cid}: no biophysicalProperties")
+    ids, parent, prox, dist, groups = _morphology(cid, m)
+    n = len(ids)
+    chans = [[] for _ in range(n)]
+    cm = np.full(n, np.nan)
+    ra = np.full(n, np.nan)
+    v0 = np.full(n, np.nan)
+    vthr = np.full(n, 0.0)    # NeuroML's default spike threshold, 0 mV
+
+    def where(spec):
+        if spec.get("segment") is not None:
+            k = ids.index(int(spec.get("segment"))) if int(spec.get("segment")) in ids else None
+            if k is None:
+                raise NeuroMLError(f"cell {cid}: segment {spec.get('segment')} not found")
+            return [k]
+        g = spec.get("segmentGroup", "all")
+        if g not in groups:
+            raise NeuroMLError(f"cell {cid}: segment group {g} not found")
+        return sorted(groups[g])
+
+    for part in b:
+        pt = _tag(part)
+        if pt == "membraneProperties":
+            for s in part:
+                st = _tag(s)
+                if st == "channelDensity":
+                    ch = s.get("ionChannel")
+                    if ch not in channel_index:
+                        raise NeuroMLError(f"cell {cid}: ion channel {ch} not found or not supported")
+                    g = convert(s.get("condDensity"), "conductivity", "condDensity")
+                    e = convert(s.get("erev"), "voltage", "erev")
+                    for k in where(s):
+                        chans[k].append((ch, _f32(g), _f32(e)))
+                elif st == "specificCapacitance":
+                    for k in where(s):
+                        cm[k] = convert(s.get("value"), "specific_capacitance", "specificCapacitance")
+                elif st == "initMembPotential":
+                    for k in where(s):
+                        v0[k] = convert(s.get("value"), "voltage", "initMembPotential")
+                elif st == "spikeThresh":
+                    for k in where(s):
+                        vthr[k] = convert(s.get("value"), "voltage", "spikeThresh")
+                elif st in ("notes", "annotation"):
+                    pass
+                else:
+                    raise NeuroMLError(f"cell {cid}: <{st}> in membraneProperties is not supported")
+        elif pt == "intracellularProperties":
+            for s in part:
+                st = _tag(s)
+                if st == "resistivity":
+                    for k in where(s):
+                        ra[k] = convert(s.get("value"), "resistivity", "resistivity")
+                elif st in ("notes", "annotation"):
+                    pass
+                else:
+                    raise NeuroMLError(f"cell {cid}: <{st}> in intracellularProperties is not supported")
+    if np.isnan(cm).any():
+        raise NeuroMLError(f"cell {cid}: specificCapacitance missing on some segments")
+    if np.isnan(v0).any():
+        raise NeuroMLError(f"cell {cid}: initMembPotential missing on some segments")
+    if n > 1 and np.isnan(ra).any():
+        raise NeuroMLError(f"cell {cid}: resistivity missing on some segments")
+    ra = np.where(np.isnan(ra), 0.0, ra)
+    c = _Cell(cid, ids, parent, prox, dist, groups, chans, cm.astype(np.float32), ra.astype(np.float32),
+              v0.astype(np.float32), vthr.astype(np.float32))
+    try:
+        c.area, c.cap, c.r_axial, c.inv_rc = N.cable_setup(parent, prox, dist, c.cm, c.ra)
+    except ValueError as e:
+        raise NeuroMLError(f"cell {cid}: {e}") from None
+    return c
+
+
+# ---------------------------------------------------------------- paths
+
+_POP_IDX = re.compile(r"^([^\[/]+)\[(\d+)\]$")
+
+
+def _cell_ref(text):
+    """Population name and cell instance id of a cell reference ('../pop/3/cell', 'pop/3/cell',
+    'pop[3]', '/3' with the population known), as ParseSynapsePopulationRef /
+    ParseSynapseCellRef read them (eden/NeuroML.cpp:4603-4649)."""
+    s = text[3:] if text.startswith("../") else text
+    b, sl = s.find("["), s.find("/")
+    ends = [x for x in (b, sl) if x >= 0]
+    pop = s[: min(ends)] if ends else s
+    if b >= 0:
+        rest = s[b + 1:]
+    elif sl >= 0:
+        rest = s[sl + 1:]
+    else:
+        rest = s
+    m = re.match(r"\d+", rest)
+    if not m:
+        raise NeuroMLError(f"cell reference '{text}' has no instance id")
+    return pop, int(m.group(0))
+
+
+def _quantity_path(text):
+    """(population, instance id, segment id, variable) of an OutputColumn quantity:
+    'pop/3/cell/5/v', 'pop/3/cell/v', 'pop[3]/v'."""
+    parts = [p for p in text.split("/") if p]
+    m = _POP_IDX.match(parts[0])
+    if m:
+        pop, inst, rest = m.group(1), int(m.group(2)), parts[1:]
+    else:
+        if len(parts) < 3 or not parts[1].isdigit():
+            raise NeuroMLError(f"quantity path '{text}' not understood")
+        pop, inst, rest = parts[0], int(parts[1]), parts[3:] if not parts[2].isdigit() else parts[2:]
+    seg = 0
+    if len(rest) == 2 and rest[0].isdigit():
+        seg = int(rest[0])
+        rest = rest[1:]
+    if rest != ["v"]:
+        raise NeuroMLError(f"quantity path '{text}': only the membrane voltage 'v' of a segment can be recorded")
+    return pop, inst, seg
+
+
+def _select_path(text):
+    """(population, instance id, segment id) of an EventSelection: 'pop/3/cell', 'pop[3]'."""
+    parts = [p for p in text.split("/") if p]
+    m = _POP_IDX.match(parts[0])
+    if m:
+        return m.group(1), int(m.group(2)), 0
+    if len(parts) >= 2 and parts[1].isdigit():
+        seg = int(parts[3]) if len(parts) >= 4 and parts[3].isdigit() else 0
+        return parts[0], int(parts[1]), seg
+    raise NeuroMLError(f"event selection '{text}' not understood")
+
+
+# ---------------------------------------------------------------- model
+
+@dataclass
+class _Pop:
+    id: str
+    cell: _Cell
+    inst_ids: list           # instance id per sequential instance
+    first_cell: int          # global cell index of its first instance
+
+    def seq(self, inst_id, what):
+        try:
+            return self.inst_ids.index(inst_id)
+        except ValueError:
+            raise NeuroMLError(f"{what}: instance {inst_id} not in population {self.id}") from None
+
+
+def read(path, rng_seed=None):
+    """Read a LEMS or NeuroML file; return (Model, Outputs). rng_seed overrides the
+    simulation's seed (EDEN's rng_seed token)."""
+    doc = _Doc(path)
+    if not doc.simulations:
+        raise NeuroMLError("no <Simulation> found")
+    sim = doc.simulations.get(doc.target) if doc.target else None
+    if sim is None:
+        if doc.target:
+            raise NeuroMLError(f"<Target component='{doc.target}'> is not a Simulation")
+        sim = next(iter(doc.simulations.values()))
+    net_id = sim.get("target")
+    net = doc.networks.get(net_id)
+    if net is None:
+        raise NeuroMLError(f"simulation target network '{net_id}' not found")
+    dt = convert(sim.get("step"), "time", "Simulation step")
+    length = convert(sim.get("length"), "time", "Simulation length")
+    seed = rng_seed
+    if seed is None:
+        s = sim.get("seed")
+        if s is None:
+            raise NeuroMLError("no seed: give <Simulation seed=...> or the rng_seed option (EDEN would take the "
+                               "wall clock, which makes runs irreproducible)")
+        seed = int(s)
+    temperature = convert(net.get("temperature"), "temperature", "temperature") if net.get("temperature") else None
+
+    # channel types in document order, then the cells the network uses
+    channel_index, channel_types = {}, []
+    for role, i in doc.order:
+        if role == "channel":
+            channel_index[i] = len(channel_types)
+            channel_types.append(_channel(doc.items[i][1], temperature))
+    cells = {}
+
+    def get_cell(cid, what):
+        if cid in cells:
+            return cells[cid]
+        role, el = doc.items.get(cid, (None, None))
+        if role == "unsupported":
+            raise NeuroMLError(f"{what}: cell type <{_tag(el)}> is not supported")
+        if role != "cell":
+            raise NeuroMLError(f"{what}: cell '{cid}' not found")
+        cells[cid] = _cell(doc, el, temperature, channel_index)
+        return cells[cid]
+
+    pops = OrderedDict()
+    n_cells = 0
+    for el in net:
+        if _tag(el) != "population":
+            continue
+        pid = el.get("id")
+        c = get_cell(el.get("component"), f"population {pid}")
+        inst = [int(i.get("id")) for i in el if _tag(i) == "instance"]
+        if not inst:
+            inst = list(range(int(el.get("size", "0"))))
+        pops[pid] = _Pop(pid, c, inst, n_cells)
+        n_cells += len(inst)
+    if n_cells == 0:
+        raise NeuroMLError(f"network {net_id} has no cells")
+
+    # compartments: cells in population order, compartments in segment order
+    comp0 = []
+    pop_of_cell = []
+    nc = 0
+    for p in pops.values():
+        for _ in p.inst_ids:
+            comp0.append(nc)
+            pop_of_cell.append(p)
+            nc += len(p.cell.seg_ids)
+    comp0.append(nc)
+    cell_begin = np.array(comp0, np.int64)
+
+    def comp(pop_id, inst_id, seg_id, what):
+        if pop_id not in pops:
+            raise NeuroMLError(f"{what}: population {pop_id} not found")
+        p = pops[pop_id]
+        c = p.first_cell + p.seq(inst_id, what)
+        return int(cell_begin[c]) + p.cell.comp_of_segment(seg_id, what)
+
+    # synapse types in reading order (their component sequence numbers), inputs
+    syn_index, syn_types = {}, []
+    for role, i in doc.order:
+        if role == "synapse":
+            el = doc.items[i][1]
+            syn_index[i] = len(syn_types)
+            syn_types.append(N.ExpTwoType(
+                _f32(convert(el.get("gbase"), "conductance", f"{i} gbase")),
+                _f32(convert(el.get("erev"), "voltage", f"{i} erev")),
+                _f32(convert(el.get("tauRise"), "time", f"{i} tauRise")),
+                _f32(convert(el.get("tauDecay"), "time", f"{i} tauDecay")), 1.0))
+
+    def synapse(sid, what):
+        role, el = doc.items.get(sid, (None, None))
+        if role == "unsupported":
+            raise NeuroMLError(f"{what}: synapse type <{_tag(el)}> is not supported (expTwoSynapse only)")
+        if role != "synapse":
+            raise NeuroMLError(f"{what}: synapse '{sid}' not found")
+        return syn_index[sid]
+
+    # network inputs in order (explicitInput and inputList entries as they appear)
+    inputs = []    # (component id, compartment, weight)
+    conns = []     # (pre compartment, post compartment, synapse type, weight, delay us)
+    for el in net:
+        t = _tag(el)
+        if t == "explicitInput":
+            pop, inst = _cell_ref(el.get("target"))
+            inputs.append((el.get("input"), comp(pop, inst, int(el.get("segmentId", "0")), "explicitInput"),
+                           1.0))
+        elif t == "inputList":
+            pop = el.get("population")
+            for i in el:
+                it = _tag(i)
+                if it not in ("input", "inputW"):
+                    continue
+                _, inst = _cell_ref(i.get("target"))
+                w = convert(i.get("weight"), None, "inputW weight") if it == "inputW" else 1.0
+                inputs.append((el.get("component"), comp(pop, inst, int(i.get("segmentId", "0")), "input"), w))
+        elif t == "projection":
+            pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
+            st = synapse(el.get("synapse"), f"projection {el.get('id')}")
+            for c in el:
+                ct = _tag(c)
+                if ct not in ("connection", "connectionWD"):
+                    continue
+                _, pi = _cell_ref(c.get("preCellId"))
+                _, qi = _cell_ref(c.get("postCellId"))
+                a = comp(pre, pi, int(c.get("preSegmentId", "0")), "connection")
+                b = comp(post, qi, int(c.get("postSegmentId", "0")), "connection")
+                w = convert(c.get("weight"), None, "weight") if ct == "connectionWD" else 1.0
+                d = convert(c.get("delay"), "time", "delay") if ct == "connectionWD" else 0.0
+                conns.append((a, b, st, w, d))
+        elif t == "synapticConnection":
+            st = synapse(el.get("synapse"), "synapticConnection")
+            fp, fi = _cell_ref(el.get("from"))
+            tp, ti = _cell_ref(el.get("to"))
+            conns.append((comp(fp, fi, 0, "synapticConnection"), comp(tp, ti, 0, "synapticConnection"), st, 1.0, 0.0))
+        elif t in ("continuousProjection", "electricalProjection"):
+            if any(True for _ in el):
+                raise NeuroMLError(f"<{t}> {el.get('id')}: continuous and electrical projections are not supported")
+
+    # per-compartment tables
+    v0 = np.concatenate([pop_of_cell[c].cell.v0 for c in range(n_cells)])
+    cap = np.concatenate([pop_of_cell[c].cell.cap for c in range(n_cells)])
+    area = np.concatenate([pop_of_cell[c].cell.area for c in range(n_cells)])
+    inv = np.concatenate([pop_of_cell[c].cell.inv_rc for c in range(n_cells)])
+    rax = np.concatenate([pop_of_cell[c].cell.r_axial for c in range(n_cells)])
+    parent = np.concatenate([pop_of_cell[c].cell.parent for c in range(n_cells)])
+    vthr_all = np.concatenate([pop_of_cell[c].cell.vthr for c in range(n_cells)])
+    ch_t, ch_g, ch_e, ch_n = [], [], [], []
+    for c in range(n_cells):
+        for lst in pop_of_cell[c].cell.channels:
+            ch_n.append(len(lst))
+            for (ch, g, e) in lst:
+                ch_t.append(channel_index[ch])
+                ch_g.append(g)
+                ch_e.append(e)
+    ch_b = np.zeros(nc + 1, np.int64)
+    np.cumsum(ch_n, out=ch_b[1:])
+
+    # inputs
+    pulses = [[] for _ in range(nc)]
+    poisson = [[] for _ in range(nc)]
+    poi_count = {}    # (cell, input component) -> instances so far (the RNG stamp's instance)
+    for (iid, g, w) in inputs:
+        role, el = doc.items.get(iid, (None, None))
+        if role != "input":
+            raise NeuroMLError(f"input '{iid}' not found or not supported (pulseGenerator, poissonFiringSynapse)")
+        t = _tag(el)
+        if t == "pulseGenerator":
+            pulses[g].append((_f32(convert(el.get("delay"), "time", "delay")),
+                              _f32(convert(el.get("duration"), "time", "duration")),
+                              _f32(convert(el.get("amplitude"), "current", "amplitude")), _f32(w)))
+        else:
+            cell = int(np.searchsorted(cell_begin, g, side="right") - 1)
+            k = poi_count.get((cell, iid), 0)
+            poi_count[(cell, iid)] = k + 1
+            rate = _f32(convert(el.get("averageRate"), "frequency", "averageRate"))
+            poisson[g].append((rate, synapse(el.get("synapse"), f"poissonFiringSynapse {iid}"), _f32(w), k))
+    pu_b = np.zeros(nc + 1, np.int64)
+    np.cumsum([len(x) for x in pulses], out=pu_b[1:])
+    pu = [x for lst in pulses for x in lst]
+    po_b = np.zeros(nc + 1, np.int64)
+    np.cumsum([len(x) for x in poisson], out=po_b[1:])
+    po = [x for lst in poisson for x in lst]
+
+    # synapse instances: per compartment, by synapse type (reading order), then connection order
+    per_comp = [[] for _ in range(nc)]
+    for k, (a, b, st, w, d) in enumerate(conns):
+        per_comp[b].append((st, k))
+    syn_b = np.zeros(nc + 1, np.int64)
+    inst_of_conn = np.zeros(len(conns), np.int64)
+    syn_list = []
+    for g in range(nc):
+        for (st, k) in sorted(per_comp[g], key=lambda x: x[0]):   # stable: connection order within a type
+            inst_of_conn[k] = len(syn_list)
+            syn_list.append(k)
+        syn_b[g + 1] = len(syn_list)
+    spk = [[] for _ in range(nc)]
+    for k, (a, b, st, w, d) in enumerate(conns):
+        spk[a].append(inst_of_conn[k])
+    spk_b = np.zeros(nc + 1, np.int64)
+    np.cumsum([len(x) for x in spk], out=spk_b[1:])
+
+    # outputs
+    base_dir = os.path.dirname(os.path.abspath(path))
+    trajectories, events = [], []
+    rec, evc = [], []
+    for el in sim:
+        t = _tag(el)
+        if t == "OutputFile":
+            f = OutputFileSpec(el.get("id"), el.get("fileName"))
+            for c in el:
+                if _tag(c) == "OutputColumn":
+                    pop, inst, seg = _quantity_path(c.get("quantity"))
+                    g = comp(pop, inst, seg, f"OutputColumn {c.get('id')}")
+                    f.columns.append((c.get("id"), c.get("quantity"), len(rec)))
+                    rec.append(g)
+            trajectories.append(f)
+        elif t == "EventOutputFile":
+            fmt = el.get("format", "TIME_ID")
+            if fmt not in ("TIME_ID", "ID_TIME"):
+                raise NeuroMLError(f"EventOutputFile {el.get('id')}: format '{fmt}' is not supported")
+            f = EventFileSpec(el.get("id"), el.get("fileName"), fmt)
+            for c in el:
+                if _tag(c) == "EventSelection":
+                    port = c.get("eventPort", "spike")
+                    if port != "spike":
+                        raise NeuroMLError(f"EventSelection {c.get('id')}: only the 'spike' port is supported")
+                    pop, inst, seg = _select_path(c.get("select"))
+                    g = comp(pop, inst, seg, f"EventSelection {c.get('id')}")
+                    f.selections.append((int(c.get("id")), c.get("select"), len(evc)))
+                    evc.append(g)
+            events.append(f)
+
+    # spike thresholds only where spikes are read: event columns and spike sources
+    vthr = np.full(nc, np.nan, np.float32)
+    used = np.zeros(nc, bool)
+    used[np.asarray(evc, np.int64)] = True
+    used[np.diff(spk_b) > 0] = True
+    vthr[used] = vthr_all[used]
+
+    f32 = lambda xs, i: np.array([x[i] for x in xs], np.float32)
+    model = Model(
+        dt=_f32(dt), t_final=float(length), cell_comp_begin=cell_begin,
+        v0=v0, capacitance=cap, area=area, inv_rc_axial=inv, r_axial=rax, v_threshold=vthr, parent=parent,
+        channel_types=channel_types, comp_chan_begin=ch_b, chan_type=np.array(ch_t, np.int32),
+        chan_gbase=np.array(ch_g, np.float32), chan_erev=np.array(ch_e, np.float32),
+        cell_rng_seed=cell_rng_seeds(seed, n_cells),
+        comp_pulse_begin=pu_b, pulse_start=f32(pu, 0), pulse_duration=f32(pu, 1), pulse_amplitude=f32(pu, 2),
+        pulse_weight=f32(pu, 3),
+        syn_types=syn_types, comp_syn_begin=syn_b,
+        syn_type=np.array([conns[k][2] for k in syn_list], np.int32),
+        syn_weight=np.array([_f32(conns[k][3]) for k in syn_list], np.float32),
+        syn_delay=np.array([_f32(conns[k][4]) for k in syn_list], np.float32),
+        comp_spike_begin=spk_b, spike_target=np.array([x for lst in spk for x in lst], np.int64),
+        comp_poisson_begin=po_b, poisson_rate=f32(po, 0), poisson_syn_type=np.array([x[1] for x in po], np.int32),
+        poisson_weight=f32(po, 2), poisson_instance=np.array([x[3] for x in po], np.int32),
+        poisson_rng_offset=np.zeros(len(po), np.int32),
+        record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
+        name=os.path.splitext(os.path.basename(path))[0])
+    return model, Outputs(trajectories, events, base_dir)
