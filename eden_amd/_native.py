@@ -81,6 +81,7 @@ EXPORTS = {
     "edenhip_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(Options), C.POINTER(C.c_void_p)]),
     "edenhip_total_rows": (C.c_int64, [C.c_void_p]),
     "edenhip_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "edenhip_run_static": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "edenhip_reset": (C.c_int, [C.c_void_p]),
     "edenhip_read_trace": (C.c_int, [C.c_void_p, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "edenhip_trace_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -133,6 +133,9 @@ struct edenhip_ctx {
     int64_t net_windows = 0, net_redone = 0;
     std::string kernel_name;
     std::vector<void *> allocs;
+    // edenhip_run_static: the tree buckets' work-queue counters come from the device's static
+    // scratch array for the duration of that call (nullptr: each bucket's own d_queue)
+    int32_t *queue_base = nullptr;
 };
 
 #define HIPCHECK(ctx, call)                                                                       \
@@ -192,6 +195,35 @@ static std::vector<double> row_times(float dt, double t_final) {
 }
 
 static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point);
+
+// Scratch of edenhip_run_static: the tree kernels' work-queue counters (next task, finished
+// workgroups) of up to STATIC_QUEUE_BUCKETS buckets, in a file-scope device array that each
+// GPU holds for the life of the process. Host code never passes the array's name to a kernel
+// (in host code it is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns for the current device, looked up once per device when the
+// library first creates a context there.
+constexpr int STATIC_QUEUE_BUCKETS = 32;
+static __device__ __align__(256) int32_t scratch[2 * STATIC_QUEUE_BUCKETS];
+
+static int static_scratch_ptr(edenhip_ctx *ctx, int32_t **out) {
+    static std::mutex mu;
+    static std::vector<int32_t *> ptrs;   // per device ordinal
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)ptrs.size() <= ctx->device) ptrs.resize((size_t)ctx->device + 1, nullptr);
+    if (!ptrs[(size_t)ctx->device]) {
+        int32_t *p = nullptr;
+        HIPCHECK(ctx, hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(ctx, EDENHIP_E_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+        ptrs[(size_t)ctx->device] = p;
+    }
+    *out = ptrs[(size_t)ctx->device];
+    return 0;
+}
+
+// Zero n ints with a kernel (not a memset node, so that a caller may capture the call).
+__global__ void zero_i32_kernel(int32_t *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
 
 // Streams come from a per-device pool created once per process and kept until it ends:
 // slot 0 is a context's main stream, which also carries its first (largest) tree bucket, and
@@ -988,6 +1020,10 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
     }
     const bool graph = ctx->opt.use_graph != 0 && per_step;
     const bool prof = ctx->opt.profile != 0 && !graph;   // per-kernel events stay out of graphs
+    // work-queue counters of edenhip_run_static: bucket b uses queue_base + 2 b
+    std::vector<int32_t *> qs;
+    if (ctx->queue_base)
+        for (size_t b = 0; b < ctx->buckets.size(); b++) qs.push_back(ctx->queue_base + 2 * b);
     // the launches of steps [a0, a1), one range per launch
     auto issue = [&](int64_t a0, int64_t a1, bool pf) -> int {
         for (int64_t a = a0; a < a1; a += stride) {
@@ -1023,6 +1059,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                                 &ctx->d_ctask, &ctx->d_gq, &ctx->d_gt, &ctx->d_inl, &ctx->d_cflag,
                                 &ctx->d_cx, &bk.cxmax_lds, &bk.nmax, &bk.nvmax, &bk.ngmax, &bk.nctmax, &bk.ninmax,
                                 &bk.words_max, (void *)&aa, (void *)&bb, &row0, &dt};
+                if (ctx->queue_base) args[8] = &qs[&bk - ctx->buckets.data()];
                 HIPCHECK(ctx, hipStreamWaitEvent(bk.stream, ctx->ev_start, 0));
                 if (pf && first) HIPCHECK(ctx, hipEventRecord(bk.p0, bk.stream));
                 HIPCHECK(ctx, hipLaunchKernel(tree_kernel_ptr(bk.tpb, bk.split, ctx->tree_net), dim3(bk.grid),
@@ -1122,6 +1159,9 @@ int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, e
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ctx, EDENHIP_E_DEVICE, "no HIP device visible");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     TRY(pool_stream(ctx, 0, &ctx->stream));
+    // the device's static scratch address, looked up once per device (edenhip_run_static)
+    int32_t *sp = nullptr;
+    TRY(static_scratch_ptr(ctx, &sp));
     return build(ctx, desc);
 }
 
@@ -1148,6 +1188,35 @@ int edenhip_run(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done) {
     ctx->last_rows = want;
     if (rows_done) *rows_done = want;
     return 0;
+}
+
+int edenhip_run_static(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done) {
+    if (!ctx) return EDENHIP_E_INVALID;
+    if (!ctx->stream) return fail(ctx, EDENHIP_E_STATE, "context was not created successfully");
+    if (ctx->netwin)
+        return fail(ctx, EDENHIP_E_LIMIT,
+                    "edenhip_run_static: the windowed network path keeps scratch that grows with the network; "
+                    "use edenhip_run, or create the context with network_window = -1");
+    if (ctx->opt.use_graph || ctx->opt.profile)
+        return fail(ctx, EDENHIP_E_STATE,
+                    "edenhip_run_static: graph capture and profiling events allocate; create the context without "
+                    "use_graph and profile");
+    if ((int64_t)ctx->buckets.size() > STATIC_QUEUE_BUCKETS)
+        return fail(ctx, EDENHIP_E_LIMIT, "edenhip_run_static: more tree-cell buckets than the static scratch holds (32)");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int32_t *sp = nullptr;
+    TRY(static_scratch_ptr(ctx, &sp));
+    // the array keeps the last call's contents (a call that failed midway may leave counters
+    // behind): the counters start from zero
+    if (!ctx->buckets.empty()) {
+        const int n = 2 * (int)ctx->buckets.size();
+        hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, ctx->stream, sp, n);
+        HIPCHECK(ctx, hipGetLastError());
+    }
+    ctx->queue_base = sp;
+    const int rc = edenhip_run(ctx, n_rows, rows_done);
+    ctx->queue_base = nullptr;
+    return rc;
 }
 
 int edenhip_reset(edenhip_ctx *ctx) {

@@ -266,6 +266,13 @@ class Simulation:
         self._check(self.lib.edenhip_run(self.h, n_rows, C.byref(done)), "edenhip_run")
         return done.value
 
+    def run_static(self, n_rows=-1):
+        """run() through edenhip_run_static, the entry point that allocates nothing (its
+        scratch is a static device array shared by every call on the GPU)."""
+        done = C.c_int64()
+        self._check(self.lib.edenhip_run_static(self.h, n_rows, C.byref(done)), "edenhip_run_static")
+        return done.value
+
     def reset(self):
         self._check(self.lib.edenhip_reset(self.h), "edenhip_reset")
 

@@ -195,6 +195,19 @@ int64_t edenhip_total_rows(const edenhip_ctx *ctx);
  * go to its device event list. Returns the number of rows done in *rows_done. */
 int edenhip_run(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory while they
+ * run (code under stream capture, tight serving loops, harnesses that forbid allocation).
+ * Same as edenhip_run(), except that the tree-cell kernels' work-queue counters (2 ints per
+ * workgroup-size bucket, scratch that edenhip_run() takes from the context) come from a
+ * static device array of 32 buckets' counters that each GPU holds until the process ends;
+ * the call allocates nothing. The counters are cleared by a kernel at the start of each call.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time on one GPU (from
+ * several host threads) race. Rejected with EDENHIP_E_LIMIT: models on the windowed network
+ * path (their scratch grows with the network) and more than 32 buckets; with
+ * EDENHIP_E_STATE: contexts created with use_graph or profile (graph instantiation and
+ * profiling events allocate). */
+int edenhip_run_static(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done);
+
 /* Rewind to the state edenhip_create() built (before the initialisation steps), clearing the
  * event log; device tables stay resident. No counterpart in EDEN, which runs a model once. */
 int edenhip_reset(edenhip_ctx *ctx);
