@@ -60,7 +60,7 @@ def hh_population(n_cells=1000, t_final_ms=100.0, seed=1001, amp_max_nA=0.2, dia
     area, cap, rax, inv = N.cable_setup(np.array([-1]), prox, prox, CM, RA)
     nc = n_cells
     ch_b, ch_t, ch_g, ch_e = _hh_channels(nc, np.ones(nc, bool))
-    return Model(
+    m = Model(
         dt=DT, t_final=t_final_ms * MS,
         cell_comp_begin=np.arange(nc + 1, dtype=np.int64),
         v0=np.full(nc, V0), capacitance=np.full(nc, cap[0]), area=np.full(nc, area[0]),
@@ -72,6 +72,8 @@ def hh_population(n_cells=1000, t_final_ms=100.0, seed=1001, amp_max_nA=0.2, dia
         pulse_amplitude=amp, pulse_weight=np.ones(nc),
         record_comp=np.arange(nc), event_comp=np.arange(nc) if record_events else None,
         name=f"hh_population_{n_cells}")
+    m.morphologies = [(np.array([-1], np.int32), prox, prox)] * nc
+    return m
 
 
 def random_binary_parents(rng, n):
@@ -150,12 +152,14 @@ def random_trees(n_cells=512, sizes=None, t_final_ms=250.0, seed=2001, hh_everyw
     rax = np.empty(nc, np.float32)
     inv = np.empty(nc, np.float32)
     parent = np.empty(nc, np.int32)
+    morphologies = []
     for c in range(n_cells):
         if parents is not None:
             par = np.asarray(parents[c], np.int32)
             prox, dist = tree_geometry(rng, par)
         else:
             par, prox, dist = random_tree(rng, int(sizes[c]))
+        morphologies.append((par, prox, dist))
         a, cp, r, iv = N.cable_setup(par, prox, dist, CM, RA)
         sl = slice(begin[c], begin[c + 1])
         area[sl], cap[sl], rax[sl], inv[sl], parent[sl] = a, cp, r, iv, par
@@ -175,7 +179,7 @@ def random_trees(n_cells=512, sizes=None, t_final_ms=250.0, seed=2001, hh_everyw
     pulse_begin = np.cumsum(pulse_begin)
     vthr = np.full(nc, np.nan, np.float32)
     vthr[soma] = VT
-    return Model(
+    m = Model(
         dt=DT, t_final=t_final, cell_comp_begin=begin,
         v0=np.full(nc, V0), capacitance=cap, area=area, inv_rc_axial=inv, r_axial=rax, v_threshold=vthr,
         parent=parent, channel_types=hh_squid_types(), comp_chan_begin=ch_b, chan_type=ch_t, chan_gbase=ch_g,
@@ -184,6 +188,8 @@ def random_trees(n_cells=512, sizes=None, t_final_ms=250.0, seed=2001, hh_everyw
         pulse_duration=np.full(n_cells, t_final), pulse_amplitude=amp, pulse_weight=np.ones(n_cells),
         record_comp=np.arange(nc) if record_all else begin[:-1].copy(), event_comp=begin[:-1].copy(),
         name=f"random_trees_{n_cells}")
+    m.morphologies = morphologies
+    return m
 
 
 def config2_sizes(n_cells=512, seed=2001):
@@ -243,7 +249,7 @@ def spiking_network(n_cells=100_000, in_degree=100, t_final_ms=1000.0, seed=3001
     spk_begin = np.zeros(nc + 1, np.int64)
     np.cumsum(np.bincount(flat_pre, minlength=nc), out=spk_begin[1:])
     syn_types = [N.ExpTwoType(1e-3, 0.0, 0.5 * MS, 5.0 * MS, 1.0), N.ExpTwoType(1e-3, -80.0, 0.5 * MS, 5.0 * MS, 1.0)]
-    return Model(
+    m = Model(
         dt=DT, t_final=t_final_ms * MS, cell_comp_begin=np.arange(nc + 1, dtype=np.int64),
         v0=np.full(nc, V0), capacitance=np.full(nc, cap[0]), area=np.full(nc, area[0]),
         inv_rc_axial=np.zeros(nc), r_axial=np.full(nc, np.nan), v_threshold=np.full(nc, VT),
@@ -256,6 +262,8 @@ def spiking_network(n_cells=100_000, in_degree=100, t_final_ms=1000.0, seed=3001
         poisson_weight=np.ones(nc), poisson_instance=np.zeros(nc), poisson_rng_offset=np.zeros(nc),
         record_comp=np.arange(min(n_record, nc)), event_comp=np.arange(nc),
         name=f"spiking_network_{n_cells}")
+    m.morphologies = [(np.array([-1], np.int32), prox, prox)] * nc
+    return m
 
 
 def config5(n_cells=256, n_comps=100, t_final_ms=500.0, seed=2001, init_seed=5001):
@@ -314,7 +322,7 @@ def mixed_network(tree_sizes=(40, 90, 150), n_point=8, n_conn=60, t_final_ms=60.
     np.cumsum(np.bincount(src_comp, minlength=nc), out=spk_begin[1:])
     syn_types = [N.ExpTwoType(2e-3, 0.0, 0.5 * MS, 5.0 * MS, 1.0), N.ExpTwoType(4e-3, -80.0, 0.5 * MS, 5.0 * MS, 1.0)]
     vthr = cat(trees.v_threshold, pts.v_threshold)
-    return Model(
+    m = Model(
         dt=DT, t_final=t_final_ms * MS, cell_comp_begin=cell_begin,
         v0=cat(trees.v0, pts.v0), capacitance=cat(trees.capacitance, pts.capacitance),
         area=cat(trees.area, pts.area), inv_rc_axial=cat(trees.inv_rc_axial, pts.inv_rc_axial),
@@ -330,3 +338,5 @@ def mixed_network(tree_sizes=(40, 90, 150), n_point=8, n_conn=60, t_final_ms=60.
         syn_delay=delay[order], comp_spike_begin=spk_begin, spike_target=inst_of_conn[sorder],
         record_comp=np.concatenate([somata, tgt_comp[:8]]), event_comp=somata,
         name=f"mixed_network_{len(tree_sizes)}t{n_point}p")
+    m.morphologies = trees.morphologies + pts.morphologies
+    return m

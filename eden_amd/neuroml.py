@@ -41,6 +41,7 @@ continuous / electrical projections) raise NeuroMLError naming the element.
 """
 import os
 import re
+import time
 import xml.etree.ElementTree as ET
 from collections import OrderedDict
 from dataclasses import dataclass, field
@@ -562,13 +563,12 @@ def read(path, rng_seed=None):
         raise NeuroMLError(f"simulation target network '{net_id}' not found")
     dt = convert(sim.get("step"), "time", "Simulation step")
     length = convert(sim.get("length"), "time", "Simulation length")
+    # the rng_seed option, else <Simulation seed>, else the wall clock's seconds
+    # (eden/Eden.cpp:776-792; only random inputs depend on it)
     seed = rng_seed
     if seed is None:
         s = sim.get("seed")
-        if s is None:
-            raise NeuroMLError("no seed: give <Simulation seed=...> or the rng_seed option (EDEN would take the "
-                               "wall clock, which makes runs irreproducible)")
-        seed = int(s)
+        seed = int(s) if s is not None else int(time.time())
     temperature = convert(net.get("temperature"), "temperature", "temperature") if net.get("temperature") else None
 
     # channel types in document order, then the cells the network uses
@@ -805,4 +805,5 @@ def read(path, rng_seed=None):
         poisson_rng_offset=np.zeros(len(po), np.int32),
         record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
         name=os.path.splitext(os.path.basename(path))[0])
+    model.sim_seed = seed
     return model, Outputs(trajectories, events, base_dir)
