@@ -110,20 +110,31 @@ def build_workload(name):
 
 def cpu_baseline(model, threads, budget_s=10.0):
     """The oracle (CPU restatement, OpenMP over cells) on a bounded sample of the same
-    workload: all cells, the first R rows, R chosen for about budget_s seconds."""
+    workload: all cells, the first R rows, R chosen for about budget_s seconds. Timed in the
+    flavour built as EDEN builds its generated code (gcc -Ofast, glibc expf/logf; the
+    baseline `value`), and in the exact-arithmetic flavour the GPU reproduces bit for bit."""
     from oracle import oracle
     total = model.total_rows()
     probe = min(total, 100)
-    t0 = time.perf_counter()
-    oracle.run(model, n_rows=probe, threads=threads, want_trace=False)
-    dtp = time.perf_counter() - t0
-    rows = int(min(total, max(probe, probe * budget_s / max(dtp, 1e-6))))
-    t0 = time.perf_counter()
-    oracle.run(model, n_rows=rows, threads=threads, want_trace=False)
-    el = time.perf_counter() - t0
+
+    def timed(precision):
+        t0 = time.perf_counter()
+        oracle.run(model, n_rows=probe, threads=threads, want_trace=False, precision=precision)
+        dtp = time.perf_counter() - t0
+        rows = int(min(total, max(probe, probe * budget_s / max(dtp, 1e-6))))
+        t0 = time.perf_counter()
+        oracle.run(model, n_rows=rows, threads=threads, want_trace=False, precision=precision)
+        return rows, time.perf_counter() - t0
+
+    rows, el = timed("f32ofast")
+    rows_x, el_x = timed("f32")
     # the 4 initialisation steps are included in the sample time and not counted as rows
     return dict(value=model.n_comps * rows / el, unit="compartment-updates/s", cores=threads, kind="port",
-                sample=f"all {model.n_cells} cells, first {rows} of {total} rows ({el:.1f} s, OpenMP static)")
+                build="gcc -Ofast -march=x86-64-v3, glibc expf/logf (EDEN's generated-code flags)",
+                sample=f"all {model.n_cells} cells, first {rows} of {total} rows ({el:.1f} s, OpenMP static)",
+                exact_flavour=dict(value=model.n_comps * rows_x / el_x,
+                                   build="gcc -O2, no contraction, exp/log rounded once from double",
+                                   sample=f"first {rows_x} rows ({el_x:.1f} s)"))
 
 
 def parity_sample(model_name, model=None, n_cells=16):
@@ -151,13 +162,30 @@ def parity_sample(model_name, model=None, n_cells=16):
         eg = s.read_events()
     tr, ec, er, _ = oracle.run(m)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from parity import compare
+    from parity import compare, compare_fine
     ev_of = {int(c): i for i, c in enumerate(m.event_comp)}
     rec2ev = [ev_of.get(int(c), 0) for c in m.record_comp]
     res = compare(tg, eg, tr, (ec, er), len(m.event_comp), rec2ev)
+    # EDEN's own build of its generated code: -Ofast (reassociation, FMA contraction), glibc
+    # expf/logf; the spread among CPU builds: glibc expf/logf alone at -O2
+    to, eco, ero, _ = oracle.run(m, precision="f32ofast")
+    r_of = compare(tg, eg, to, (eco, ero), len(m.event_comp), rec2ev)
+    tl, ecl, erl, _ = oracle.run(m, precision="f32libm")
+    r_lm = compare(to, (eco, ero), tl, (ecl, erl), len(m.event_comp), rec2ev)
+    # the FP64 run at dt/10, compared wherever the rasters agree within one dt
+    t64, ec64, er64, _ = oracle.run(m, precision="f64", substeps=10)
+    d_g = compare_fine(tg, eg, t64, (ec64, er64), 10, len(m.event_comp), rec2ev)
+    d_o = compare_fine(to, (eco, ero), t64, (ec64, er64), 10, len(m.event_comp), rec2ev)
     return dict(sample=what, cells=int(m.n_cells), compartments=int(m.n_comps), rows=int(res["rows"]),
                 max_abs_mV=res["max_abs_mV"], raster_mismatch_cells=res["raster_mismatch_cols"],
-                spikes=res["spikes_ref"], within_tol=bool(res["max_abs_mV"] <= TOL_MV))
+                spikes=res["spikes_ref"], within_tol=bool(res["max_abs_mV"] <= TOL_MV),
+                vs_eden_build=dict(build="oracle at gcc -Ofast -march=x86-64-v3, glibc expf/logf",
+                                   max_abs_mV=r_of["max_abs_mV"], raster_mismatch_cells=r_of["raster_mismatch_cols"],
+                                   identical_raster_cells=r_of["identical_raster_cols"],
+                                   cpu_builds_spread_mV=r_lm["max_abs_mV"]),
+                fp64_dt10_dev_mV=dict(gpu=d_g["max_abs_mV"], eden_build=d_o["max_abs_mV"],
+                                      compared_samples=d_g["compared_samples"],
+                                      raster_mismatch_cells=d_g["raster_mismatch_cols"]))
 
 
 def adjoint_bench(reps=2):

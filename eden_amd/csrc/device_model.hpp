@@ -79,8 +79,10 @@ struct DevTables {
     // time
     const double *t_row;          // [rows] time (double, as accumulated) at each recorded row
     // outputs
-    float *trace;                 // [chunk_rows * n_trace_cols]
+    float *trace;                 // [n_trace_cols][trace_rows]: column-major, a column's rows
+                                  // contiguous (see trace_at)
     int32_t n_trace_cols;
+    int64_t trace_rows;           // rows of a chunk (the column stride)
     int64_t *events;              // [max_events] packed (row << 24 | col)
     uint32_t *n_events;
     uint32_t max_events;
@@ -280,6 +282,14 @@ __device__ __forceinline__ float randof(float x, int32_t work_item, int64_t inst
     uint64_t s = hash64shift(hash64shift(lo) ^ hi);
     float r = ((float)(s % (1ULL << 23))) / ((float)(1 << 23));
     return x * r;
+}
+
+// The trace element of device column `col` at chunk row `row`. Column-major: a recorded
+// compartment's consecutive steps go to consecutive words, so L2 merges a cell's stores of
+// successive steps into whole lines before they reach HBM (in a row-major layout every store
+// of a step lands in a line of its own).
+__device__ __forceinline__ float &trace_at(const DevTables &T, int32_t col, int64_t row) {
+    return T.trace[(int64_t)col * T.trace_rows + row];
 }
 
 // Per-compartment constants cached in registers for the whole time loop.

@@ -666,6 +666,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     const int64_t row_bytes = std::max<int64_t>(1, ctx->n_dev_cols) * 4;
     chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, ((int64_t)1 << 30) / row_bytes));
     ctx->chunk_rows = chunk;
+    T.trace_rows = chunk;
     TRY(alloc_zero(ctx, &T.trace, (size_t)(chunk * std::max<int64_t>(1, ctx->n_dev_cols))));
     T.max_events = 1u << 24;
     TRY(alloc_zero(ctx, &T.events, (size_t)T.max_events));
@@ -1253,10 +1254,14 @@ int edenhip_read_trace(edenhip_ctx *ctx, float *dst, int64_t max_rows, int64_t *
     if (rows) *rows = nr;
     if (row0) *row0 = ctx->last_row0;
     if (nr <= 0 || ncol == 0 || !dst) return 0;
-    std::vector<float> buf((size_t)(nr * ctx->n_dev_cols));
+    // device layout [column][chunk row]; the first nr rows of each column
+    const int64_t stride = ctx->chunk_rows;
+    std::vector<float> buf((size_t)(stride * ctx->n_dev_cols));
     HIPCHECK(ctx, hipMemcpy(buf.data(), ctx->T.trace, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (int64_t r = 0; r < nr; r++)
-        for (int64_t c = 0; c < ncol; c++) dst[r * ncol + c] = buf[(size_t)(r * ctx->n_dev_cols + ctx->desc_to_dev_col[c])];
+    for (int64_t c = 0; c < ncol; c++) {
+        const float *src = buf.data() + (size_t)(ctx->desc_to_dev_col[c] * stride);
+        for (int64_t r = 0; r < nr; r++) dst[r * ncol + c] = src[r];
+    }
     return 0;
 }
 

@@ -61,7 +61,7 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
         const bool init = s <= 0;
         const float t = (float)time;
         const int now = (int)((s + 3) & 1), nxt = now ^ 1;
-        if (!init && k.rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + k.rec] = V;
+        if (!init && k.rec >= 0) trace_at(T, k.rec, s - 1 - row0) = V;
         float I_ch, G_ch;
         channels_step(gates, k, V, q, init, dt, I_ch, G_ch);
         float I_syn = 0.0f, G_syn = 0.0f;
@@ -129,7 +129,7 @@ point_quad_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, c
     for (int64_t s = s0; s < s1; s++) {
         const bool init = s <= 0;
         const float t = (float)time;
-        if (!init && k.rec >= 0 && qs == 0) T.trace[(s - 1 - row0) * T.n_trace_cols + k.rec] = V;
+        if (!init && k.rec >= 0 && qs == 0) trace_at(T, k.rec, s - 1 - row0) = V;
         float q[NQ];
 #pragma unroll
         for (int j = 0; j < NQ; j++) q[j] = __shfl(qv, j, 4);
@@ -775,7 +775,7 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                                 // a recorded compartment is complex: its trace column is in its
                                 // record; the trace row of step s holds the voltage the step
                                 // started from
-                                if (!init && cx.k.rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + cx.k.rec] = V[j];
+                                if (!init && cx.k.rec >= 0) trace_at(T, cx.k.rec, s - 1 - row0) = V[j];
                                 const float vthr = cx.k.vthr;
                                 if (V[j] < vthr && vthr < Vn) {
                                     log_event(T, cx.k.ev, s);

@@ -134,7 +134,7 @@ net_window_kernel(DevTables T, NetTables Nt, const DGate *__restrict__ gates_g, 
             const bool init = s <= 0;
             const float t = (float)time;
             const int rb = s == s0 ? cur : nxt;
-            if (!init && k.rec >= 0) T.trace[(s - 1 - row0) * T.n_trace_cols + k.rec] = V;
+            if (!init && k.rec >= 0) trace_at(T, k.rec, s - 1 - row0) = V;
             float I_ch, G_ch;
             channels_step(gates, k, V, q, init, dt, I_ch, G_ch);
             float I_syn = 0.0f, G_syn = 0.0f;

@@ -216,7 +216,10 @@ int edenhip_reset(edenhip_ctx *ctx);
  * row r holding the state at time (row0 + r) * dt. dst is host memory. */
 int edenhip_read_trace(edenhip_ctx *ctx, float *dst, int64_t max_rows, int64_t *rows, int64_t *row0);
 
-/* Device pointer of the trace buffer of the last run (rows x n_record floats). */
+/* Device pointer of the trace buffer of the last run. Layout: column-major, one column per
+ * distinct recorded compartment (in order of first appearance in record_comp), each column
+ * a chunk's rows long (chunk_steps, or all rows): the value of that column at row r of the
+ * chunk is at column * chunk_rows + r. */
 int edenhip_trace_device_ptr(edenhip_ctx *ctx, const float **dptr);
 
 /* Spike events logged so far, as (event column, row) pairs in ascending (row, column)

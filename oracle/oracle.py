@@ -20,7 +20,7 @@ def lib():
         if not os.path.exists(LIB):
             raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
         L = C.CDLL(LIB)
-        for sfx in ("f32", "f32libm", "f64"):
+        for sfx in ("f32", "f32libm", "f32ofast", "f64"):
             run = getattr(L, f"oracle_{sfx}_run")
             run.restype = C.c_int64
             run.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,

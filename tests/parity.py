@@ -64,3 +64,13 @@ def compare(trace_gpu, ev_gpu, trace_ref, ev_ref, n_event_cols, record_to_event_
     return dict(rows=rows, max_abs_mV=max_err, compared_samples=compared, raster_mismatch_cols=n_mismatch,
                 identical_raster_cols=same, spikes_gpu=exact_spikes, spikes_ref=sum(len(b) for b in rb),
                 first_mismatch_row=min([m for m in mism if m is not None], default=None))
+
+
+def compare_fine(trace, ev, trace_fine, ev_fine, substeps, n_event_cols, record_to_event_col):
+    """A trace at dt against a run at dt / substeps that recorded every substeps-th row
+    (event rows of the fine run in fine steps): the fine spike rows are taken to the coarse
+    row at the same time (rounded up), rasters compared within one coarse dt, voltages
+    compared per cell up to its first spike that differs."""
+    cols, rows = ev_fine
+    coarse = (np.asarray(rows) + substeps - 1) // substeps
+    return compare(trace, ev, trace_fine, (np.asarray(cols), coarse), n_event_cols, record_to_event_col)
