@@ -73,7 +73,7 @@ class KernelInfo(C.Structure):
 
 class AdjOptions(C.Structure):
     _fields_ = [("checkpoint_steps", C.c_int32), ("threads_per_cell", C.c_int32), ("device", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("store_states", C.c_int32)]
 
 
 EXPORTS = {
@@ -83,6 +83,9 @@ EXPORTS = {
     "edenhip_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "edenhip_run_static": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "edenhip_reset": (C.c_int, [C.c_void_p]),
+    "edenhip_state_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "edenhip_get_state": (C.c_int, [C.c_void_p, _P, C.c_int64]),
+    "edenhip_set_state": (C.c_int, [C.c_void_p, _P, C.c_int64]),
     "edenhip_read_trace": (C.c_int, [C.c_void_p, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "edenhip_trace_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "edenhip_read_events": (C.c_int, [C.c_void_p, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),

@@ -22,11 +22,13 @@ class Adjoint:
     """Differentiable model: `model` is an eden_amd.Model without synapses, spike recipients
     or Poisson inputs. Parameters default to the model's chan_gbase and pulse_amplitude."""
 
-    def __init__(self, model, checkpoint_steps=500, threads_per_cell=0, device=0):
+    def __init__(self, model, checkpoint_steps=500, threads_per_cell=0, device=0, store_states=0):
+        """store_states: 0 = keep every step's state in device memory when it fits (no
+        recomputation in the backward pass), 1 = always, -1 = checkpoints and recompute."""
         self.lib = N.load_library()
         self.model = model
         self._desc = model.desc()
-        opt = N.AdjOptions(checkpoint_steps, threads_per_cell, device, 0)
+        opt = N.AdjOptions(checkpoint_steps, threads_per_cell, device, store_states)
         self.h = C.c_void_p()
         rc = self.lib.edenhip_adj_create(C.byref(self._desc), C.byref(opt), C.byref(self.h))
         if rc != 0:

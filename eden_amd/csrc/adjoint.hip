@@ -11,9 +11,11 @@
 // node's operations in the reference's order.
 //
 // One workgroup per cell. adj_forward runs the whole time loop, writes the trace and a
-// checkpoint of (V, q) at the start of every K-th recorded step. adj_backward handles one
-// segment of K steps: it recomputes the segment's states from its checkpoint into a
-// segment buffer, then walks the steps backwards:
+// checkpoint of (V, q) at the start of every K-th recorded step; when the state history fits
+// in device memory it also writes (V, q) of every step, in the segment layout the backward
+// pass reads. adj_backward handles one segment of K steps: it takes the segment's states from
+// that history, or recomputes them from the segment's checkpoint into a segment buffer, then
+// walks the steps backwards:
 //   lambda_b = M^-T lambda_V'   (Hines on the transposed matrix: Ui and Uj swap roles)
 //   lambda_D = -lambda_b V'
 //   per channel c of compartment i:  dL/dGscaled = dt/C * 1e-5f * A * (lambda_b E_c + lambda_D)
@@ -333,12 +335,18 @@ __device__ CellView<TPB> make_view(const AdjTables &T, const AdjCell &cell, unsi
 
 // Whole forward pass: initial state from v0/q0, trace rows, checkpoints of the state at the
 // start of steps 1, 1 + K, 1 + 2K, ... (ck[k] = V of all compartments then q of all gates).
+// hist (may be null): the state at the start of every step s >= 1, in the backward pass's
+// segment layout: segment k = (s - 1) / K holds steps 1 + k K .. 1 + (k + 1) K (K + 1
+// states; the last one is also the first of segment k + 1), hist_stride doubles apart, cell
+// c at seg_off[c], the state of step s at (s - 1 - k K) * (N + NG) after it.
 template <int TPB>
 __global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells, const double *__restrict__ v0,
                                                    const double *__restrict__ q0, const double *__restrict__ gbase,
                                                    const double *__restrict__ amp, double *__restrict__ trace,
                                                    double *__restrict__ ck, int64_t ck_stride, int64_t nc_total,
-                                                   int32_t K, AdjLds L, int32_t nqmax) {
+                                                   int32_t K, AdjLds L, int32_t nqmax, double *__restrict__ hist,
+                                                   int64_t hist_stride, int64_t n_seg,
+                                                   const int64_t *__restrict__ seg_off) {
     extern __shared__ __align__(16) unsigned char smem[];
     const AdjCell cell = T.cells[blockIdx.x];
     auto v = make_view<TPB>(T, cell, smem, L, nqmax);
@@ -359,6 +367,19 @@ __global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells,
                 const int rc = T.rec_col[cell.comp_begin + i];
                 if (rc >= 0) trace[(s - 1) * T.n_record + rc] = v.sV[i];
             }
+            if (hist) {
+                const int64_t per = (int64_t)cell.n + cell.n_gate;
+                const int64_t k = (s - 1) / K, j = (s - 1) - k * K;
+                auto put = [&](int64_t seg, int64_t idx) {
+                    double *h = hist + seg * hist_stride + seg_off[blockIdx.x] + idx * per;
+                    for (int i = tid; i < cell.n; i += TPB) h[i] = v.sV[i];
+                    for (int q = tid; q < cell.n_gate; q += TPB) h[cell.n + q] = v.sQ[cur * nqmax + q];
+                };
+                // the state of the last step opens no segment of its own when (rows - 1) is a
+                // multiple of K: it is only the closing state of the segment before
+                if (k < n_seg) put(k, j);
+                if (j == 0 && k > 0) put(k - 1, K);
+            }
         }
         if (s == T.rows) break;
         v.step(s, t, cur, nqmax, gbase, amp);
@@ -376,7 +397,7 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
                                                     const double *__restrict__ ck, int64_t nc_total, int64_t a,
                                                     int64_t b, double *__restrict__ seg, const int64_t *__restrict__ seg_off,
                                                     double *__restrict__ lam, double *__restrict__ grad_g,
-                                                    double *__restrict__ grad_a, AdjLds L, int32_t nqmax) {
+                                                    double *__restrict__ grad_a, AdjLds L, int32_t nqmax, int32_t stored) {
     extern __shared__ __align__(16) unsigned char smem[];
     const AdjCell cell = T.cells[blockIdx.x];
     auto v = make_view<TPB>(T, cell, smem, L, nqmax);
@@ -385,20 +406,22 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
     const int64_t per = (int64_t)N + NG;
     double *S = seg + seg_off[blockIdx.x];
     const double dt = T.dt;
-    // recompute the segment from its checkpoint
-    for (int i = tid; i < N; i += TPB) v.sV[i] = ck[cell.comp_begin + i];
-    for (int k = tid; k < NG; k += TPB) v.sQ[k] = ck[nc_total + cell.gate_begin + k];
     for (int c = tid; c < cell.n_chan; c += TPB) v.sGg[c] = 0.0;
     for (int p = tid; p < cell.n_pulse; p += TPB) v.sGa[p] = 0.0;
-    bar<TPB>();
-    int cur = 0;
-    for (int64_t s = a; s <= b; s++) {
-        double *Ss = S + (s - a) * per;
-        for (int i = tid; i < N; i += TPB) Ss[i] = v.sV[i];
-        for (int k = tid; k < NG; k += TPB) Ss[N + k] = v.sQ[cur * nqmax + k];
-        if (s == b) break;
-        v.step(s, T.t_row[s - 1], cur, nqmax, gbase, amp);
-        cur ^= 1;
+    if (!stored) {
+        // recompute the segment from its checkpoint (else seg is the forward pass's history)
+        for (int i = tid; i < N; i += TPB) v.sV[i] = ck[cell.comp_begin + i];
+        for (int k = tid; k < NG; k += TPB) v.sQ[k] = ck[nc_total + cell.gate_begin + k];
+        bar<TPB>();
+        int cur = 0;
+        for (int64_t s = a; s <= b; s++) {
+            double *Ss = S + (s - a) * per;
+            for (int i = tid; i < N; i += TPB) Ss[i] = v.sV[i];
+            for (int k = tid; k < NG; k += TPB) Ss[N + k] = v.sQ[cur * nqmax + k];
+            if (s == b) break;
+            v.step(s, T.t_row[s - 1], cur, nqmax, gbase, amp);
+            cur ^= 1;
+        }
     }
     // adjoint state at the start of step b
     for (int i = tid; i < N; i += TPB) v.sLV[i] = lam[cell.comp_begin + i];
@@ -529,6 +552,9 @@ struct edenhip_adj {
     std::vector<void *> allocs;
     double *d_v0 = nullptr, *d_q0 = nullptr, *d_gbase0 = nullptr, *d_amp0 = nullptr;
     double *d_ck = nullptr, *d_seg = nullptr, *d_lam = nullptr;
+    double *d_hist = nullptr;          // state history (store mode), n_seg segments
+    int64_t hist_stride = 0, n_seg = 0;
+    int32_t store = 0;                 // options.store_states
     int64_t *d_seg_off = nullptr;
     int64_t n_ck = 0, ck_stride = 0;
     const double *last_gbase = nullptr, *last_amp = nullptr;
@@ -779,6 +805,17 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
     ATRY(aupload(a, &a->d_seg_off, seg_off));
     ATRY(aalloc(a, (void **)&a->d_ck, (size_t)(a->n_ck * a->ck_stride) * sizeof(double)));
     ATRY(aalloc(a, (void **)&a->d_seg, (size_t)(seg_total * (a->K + 1)) * sizeof(double)));
+    // state history of every step (store mode): n_seg segments of seg_total (K + 1) doubles
+    a->hist_stride = seg_total * (a->K + 1);
+    a->n_seg = a->rows >= 2 ? (a->rows - 2) / a->K + 1 : 0;
+    {
+        const size_t bytes = (size_t)(a->hist_stride * std::max<int64_t>(a->n_seg, 1)) * sizeof(double);
+        size_t free_b = 0, total_b = 0;
+        AHIP(a, hipMemGetInfo(&free_b, &total_b));
+        const bool want = a->store > 0 || (a->store == 0 && bytes <= free_b / 2);
+        if (a->store > 0 && bytes > free_b) return afail(a, EDENHIP_E_LIMIT, "state history does not fit in device memory");
+        if (want && a->n_seg > 0) ATRY(aalloc(a, (void **)&a->d_hist, bytes));
+    }
     ATRY(aalloc(a, (void **)&a->d_lam, (size_t)a->ck_stride * sizeof(double)));
     if (!fwd_kernel(a->tpb) || !bwd_kernel(a->tpb)) return afail(a, EDENHIP_E_LIMIT, "unsupported threads_per_cell");
     return 0;
@@ -798,6 +835,7 @@ int edenhip_adj_create(const edenhip_model_desc *desc, const edenhip_adj_options
     if (opt) {
         a->K = opt->checkpoint_steps > 0 ? opt->checkpoint_steps : 500;
         a->device = opt->device;
+        a->store = opt->store_states;
         if (opt->threads_per_cell) {
             if (opt->threads_per_cell != 64 && opt->threads_per_cell != 128 && opt->threads_per_cell != 256)
                 return afail(a, EDENHIP_E_INVALID, "threads_per_cell must be 0, 64, 128 or 256");
@@ -838,7 +876,9 @@ int edenhip_adj_forward(edenhip_adj *a, const double *gbase, const double *amp, 
     int32_t ncell = a->n_cells, K = a->K, nq = a->nqmax;
     int64_t cks = a->ck_stride, nct = a->NC;
     AdjLds L = a->L;
-    void *args[] = {&T, &ncell, &a->d_v0, &a->d_q0, (void *)&gb, (void *)&am, &trace, &a->d_ck, &cks, &nct, &K, &L, &nq};
+    int64_t hs = a->hist_stride, ns = a->n_seg;
+    void *args[] = {&T, &ncell, &a->d_v0, &a->d_q0, (void *)&gb, (void *)&am, &trace, &a->d_ck, &cks, &nct, &K, &L, &nq,
+                    &a->d_hist, &hs, &ns, &a->d_seg_off};
     AHIP(a, hipLaunchKernel(fwd_kernel(a->tpb), dim3(a->n_cells), dim3(a->tpb), args, a->lds, a->stream));
     AHIP(a, hipEventRecord(e1, a->stream));
     AHIP(a, hipEventSynchronize(e1));
@@ -885,8 +925,10 @@ int edenhip_adj_backward(edenhip_adj *a, const double *grad_trace, double *grad_
         int64_t sa = 1 + k * a->K, sb = std::min<int64_t>(1 + (k + 1) * a->K, last);
         if (sa >= sb) continue;
         const double *ckp = a->d_ck + k * a->ck_stride;
+        int32_t stored = a->d_hist ? 1 : 0;
+        double *segp = a->d_hist ? a->d_hist + k * a->hist_stride : a->d_seg;
         void *args[] = {&T, (void *)&a->last_gbase, (void *)&a->last_amp, (void *)&grad_trace, (void *)&ckp, &nct, &sa, &sb,
-                        &a->d_seg, &a->d_seg_off, &a->d_lam, &gg, &ga, &L, &nq};
+                        &segp, &a->d_seg_off, &a->d_lam, &gg, &ga, &L, &nq, &stored};
         AHIP(a, hipLaunchKernel(bwd_kernel(a->tpb), dim3(a->n_cells), dim3(a->tpb), args, a->lds, a->stream));
     }
     AHIP(a, hipEventRecord(e1, a->stream));

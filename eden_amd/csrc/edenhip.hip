@@ -1247,6 +1247,77 @@ int edenhip_reset(edenhip_ctx *ctx) {
     return 0;
 }
 
+// State blob: a header, then every live state table in the order edenhip_reset restores them.
+struct StateHeader {
+    uint64_t magic;          // "EDENHIPS"
+    int64_t n_tables, total_bytes, next_step, rows_done;
+};
+constexpr uint64_t STATE_MAGIC = 0x5350494845444545ull;
+
+static int64_t state_bytes(const edenhip_ctx *ctx) {
+    int64_t b = (int64_t)sizeof(StateHeader) + (int64_t)ctx->snaps.size() * 8;
+    for (const auto &sn : ctx->snaps) b += (int64_t)sn.bytes;
+    return b;
+}
+
+int edenhip_state_size(const edenhip_ctx *ctx, int64_t *bytes) {
+    if (!ctx || !bytes) return EDENHIP_E_INVALID;
+    *bytes = state_bytes(ctx);
+    return 0;
+}
+
+int edenhip_get_state(edenhip_ctx *ctx, void *dst, int64_t bytes) {
+    if (!ctx || !dst) return EDENHIP_E_INVALID;
+    if (!ctx->stream) return fail(ctx, EDENHIP_E_STATE, "context was not created successfully");
+    if (bytes < state_bytes(ctx)) return fail(ctx, EDENHIP_E_INVALID, "state buffer smaller than edenhip_state_size");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    char *p = static_cast<char *>(dst);
+    StateHeader h{STATE_MAGIC, (int64_t)ctx->snaps.size(), state_bytes(ctx), ctx->next_step, ctx->rows_done};
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    for (const auto &sn : ctx->snaps) {
+        const int64_t nb = (int64_t)sn.bytes;
+        std::memcpy(p, &nb, 8);
+        p += 8;
+        HIPCHECK(ctx, hipMemcpy(p, sn.dst, sn.bytes, hipMemcpyDeviceToHost));
+        p += sn.bytes;
+    }
+    return 0;
+}
+
+int edenhip_set_state(edenhip_ctx *ctx, const void *src, int64_t bytes) {
+    if (!ctx || !src) return EDENHIP_E_INVALID;
+    if (!ctx->stream) return fail(ctx, EDENHIP_E_STATE, "context was not created successfully");
+    if (bytes < (int64_t)sizeof(StateHeader)) return fail(ctx, EDENHIP_E_INVALID, "state buffer too small");
+    StateHeader h;
+    std::memcpy(&h, src, sizeof(h));
+    if (h.magic != STATE_MAGIC || h.n_tables != (int64_t)ctx->snaps.size() || h.total_bytes != state_bytes(ctx) ||
+        bytes < h.total_bytes)
+        return fail(ctx, EDENHIP_E_INVALID, "state blob does not belong to a context of this model");
+    if (h.next_step < -3 || h.rows_done < 0 || h.rows_done > ctx->total_rows)
+        return fail(ctx, EDENHIP_E_INVALID, "state blob with invalid step counters");
+    const char *p = static_cast<const char *>(src) + sizeof(h);
+    for (const auto &sn : ctx->snaps) {
+        int64_t nb = 0;
+        std::memcpy(&nb, p, 8);
+        if (nb != (int64_t)sn.bytes) return fail(ctx, EDENHIP_E_INVALID, "state blob table size mismatch");
+        p += 8 + sn.bytes;
+    }
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    p = static_cast<const char *>(src) + sizeof(h);
+    for (const auto &sn : ctx->snaps) {
+        p += 8;
+        HIPCHECK(ctx, hipMemcpy(sn.dst, p, sn.bytes, hipMemcpyHostToDevice));
+        p += sn.bytes;
+    }
+    ctx->next_step = h.next_step;
+    ctx->rows_done = h.rows_done;
+    ctx->last_rows = 0;
+    ctx->last_row0 = h.rows_done;
+    ctx->net_cur = 0;
+    return 0;
+}
+
 int edenhip_read_trace(edenhip_ctx *ctx, float *dst, int64_t max_rows, int64_t *rows, int64_t *row0) {
     if (!ctx) return EDENHIP_E_INVALID;
     const int64_t nr = std::min(max_rows, ctx->last_rows);

@@ -273,6 +273,19 @@ class Simulation:
         self._check(self.lib.edenhip_run_static(self.h, n_rows, C.byref(done)), "edenhip_run_static")
         return done.value
 
+    def get_state(self):
+        """The simulation state as bytes (edenhip_get_state): restore it with set_state() in
+        a Simulation of the same model and options."""
+        n = C.c_int64()
+        self._check(self.lib.edenhip_state_size(self.h, C.byref(n)), "edenhip_state_size")
+        buf = np.empty(n.value, np.uint8)
+        self._check(self.lib.edenhip_get_state(self.h, N.ptr(buf), n.value), "edenhip_get_state")
+        return buf.tobytes()
+
+    def set_state(self, blob):
+        buf = np.frombuffer(blob, np.uint8).copy()
+        self._check(self.lib.edenhip_set_state(self.h, N.ptr(buf), len(buf)), "edenhip_set_state")
+
     def reset(self):
         self._check(self.lib.edenhip_reset(self.h), "edenhip_reset")
 

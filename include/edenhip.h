@@ -212,6 +212,18 @@ int edenhip_run_static(edenhip_ctx *ctx, int64_t n_rows, int64_t *rows_done);
  * event log; device tables stay resident. No counterpart in EDEN, which runs a model once. */
 int edenhip_reset(edenhip_ctx *ctx);
 
+/* Checkpoint and restore of the simulation state between edenhip_run() calls. EDEN keeps its
+ * step state in the double-buffered tables main() owns (eden/Eden.cpp:11093-11150) and has
+ * no save/restore; here the state is an opaque blob: every live state table of the context
+ * (compartment voltages, gate variables, synapse trigger flags, pending arrival times and
+ * LEMS state, Poisson input state, the network path's delivery ring) and the step counters.
+ * Outputs already logged (events) are not part of it. edenhip_set_state() accepts a blob of a
+ * context built from the same descriptor and options; running on then continues exactly as
+ * the context the blob came from would. */
+int edenhip_state_size(const edenhip_ctx *ctx, int64_t *bytes);
+int edenhip_get_state(edenhip_ctx *ctx, void *dst, int64_t bytes);
+int edenhip_set_state(edenhip_ctx *ctx, const void *src, int64_t bytes);
+
 /* Copy the trace rows of the last edenhip_run() call: [rows x n_record] floats (mV),
  * row r holding the state at time (row0 + r) * dt. dst is host memory. */
 int edenhip_read_trace(edenhip_ctx *ctx, float *dst, int64_t max_rows, int64_t *rows, int64_t *row0);

@@ -7,10 +7,11 @@
  * with the reference's operation order (as the FP64 restatement in oracle/ does). Spike
  * threshold crossings only feed event logs here, so they carry no gradient.
  *
- * The forward pass records the voltage trace of the recorded compartments (double) and a
- * checkpoint of the full state every `checkpoint_steps` steps; the backward pass takes
- * dLoss/dtrace and, segment by segment from the last checkpoint, recomputes the segment's
- * states and runs the adjoint recursion backwards, accumulating dLoss/dGbase per channel
+ * The forward pass records the voltage trace of the recorded compartments (double), a
+ * checkpoint of the full state every `checkpoint_steps` steps and, when it fits (see
+ * store_states), the full state of every step; the backward pass takes dLoss/dtrace and,
+ * segment by segment from the last, takes the segment's states from that history or
+ * recomputes them from the segment's checkpoint, and runs the adjoint recursion backwards, accumulating dLoss/dGbase per channel
  * distribution and dLoss/dAmplitude per pulse input. Gates clipped to [1e-6, 1 - 1e-6] in a
  * step pass no gradient through that step's update.
  *
@@ -32,7 +33,10 @@ typedef struct edenhip_adj_options {
     int32_t checkpoint_steps;    /* steps between checkpoints (0 = 500) */
     int32_t threads_per_cell;    /* 0 = by cell size, or 64/128/256 */
     int32_t device;
-    int32_t reserved;
+    int32_t store_states;        /* 0 = keep every step's state in device memory when it takes at
+                                    most half of the free memory, else checkpoints and recompute;
+                                    1 = always keep it (EDENHIP_E_LIMIT when it does not fit);
+                                    -1 = never (checkpoints every checkpoint_steps, recompute) */
 } edenhip_adj_options;
 
 /* Build the differentiable model from the same descriptor edenhip_create() takes. The model

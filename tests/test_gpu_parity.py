@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from eden_amd import configs
-from eden_amd.model import Simulation
+from eden_amd.model import EdenHipError, Simulation
 from oracle import oracle
 from parity import TOL_MV, compare  # noqa: E402
 
@@ -346,6 +346,49 @@ def test_deviation_from_the_eden_build(which):
     assert res["raster_mismatch_cols"] == 0 and res["spikes_ref"] > 0
     assert res["max_abs_mV"] == ref["max_abs_mV"]
     assert res["max_abs_mV"] <= bound
+
+
+@pytest.mark.parametrize("which", ["trees", "network_window", "mixed_network"])
+def test_state_checkpoint_and_restore(which):
+    """edenhip_get_state after some rows, edenhip_set_state into a fresh context of the same
+    model: the rows run after the restore equal, bit for bit, those the first context runs
+    on (tree cells; the windowed network path with its delivery ring; a per-step network of
+    tree and point cells)."""
+    if which == "trees":
+        m = configs.random_trees(5, sizes=[30, 80, 200, 600, 1500], t_final_ms=40.0, seed=8)
+    elif which == "network_window":
+        m = configs.spiking_network(n_cells=2000, in_degree=20, t_final_ms=60.0, poisson_hz=200.0)
+    else:
+        m = configs.mixed_network(tree_sizes=(40, 300), n_point=12, n_conn=120, t_final_ms=40.0, seed=6)
+    first = 701
+    with Simulation(m, chunk_steps=first) as a:
+        a.run(-1)
+        blob = a.get_state()
+        rest = []
+        while True:
+            n = a.run(-1)
+            if n == 0:
+                break
+            rest.append(a.read_trace()[0].copy())
+        ev_a = a.read_events()
+    with Simulation(m, chunk_steps=first) as b:
+        b.set_state(blob)
+        rest_b = []
+        while True:
+            n = b.run(-1)
+            if n == 0:
+                break
+            rest_b.append(b.read_trace()[0].copy())
+        ev_b = b.read_events()
+    ra, rb = np.concatenate(rest), np.concatenate(rest_b)
+    _say(f"[state] {which}: {len(blob)} bytes, {len(ra)} rows after the restore, spikes {len(ev_b[1])}")
+    assert np.array_equal(ra, rb)
+    # the restored context logs the spikes from the restore on
+    keep = ev_a[1] >= first + 1
+    assert np.array_equal(ev_a[1][keep], ev_b[1]) and np.array_equal(ev_a[0][keep], ev_b[0])
+    with Simulation(configs.random_trees(2, sizes=[30, 40], t_final_ms=5.0)) as other:
+        with pytest.raises(EdenHipError, match="does not belong"):
+            other.set_state(blob)
 
 
 def test_output_files_from_a_gpu_run(tmp_path):
