@@ -11,7 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ROOT, "libedenhip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_GATES = 4
 
 RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED = 0, 1, 2, 3
@@ -39,6 +39,11 @@ class ExpTwoType(C.Structure):
 _P = C.c_void_p
 
 
+class LemsSynapsesDesc(C.Structure):
+    _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
+                ("n_state", _P)]
+
+
 class ModelDesc(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("dt", C.c_float), ("t_final", C.c_double),
@@ -55,6 +60,7 @@ class ModelDesc(C.Structure):
         ("comp_poisson_begin", _P), ("poisson_rate", _P), ("poisson_syn_type", _P), ("poisson_weight", _P),
         ("poisson_instance", _P), ("poisson_rng_offset", _P),
         ("n_record", C.c_int64), ("record_comp", _P), ("n_event_cols", C.c_int64), ("event_comp", _P),
+        ("lems", C.POINTER(LemsSynapsesDesc)),
     ]
 
 

@@ -92,6 +92,21 @@ struct DevTables {
     uint32_t *n_syn_events;
     uint32_t max_syn_events;
     int32_t log_syn;
+    // generic LEMS synapses: per compartment (I_syn, G_syn) of the present step, written by
+    // lems_syn_kernel before the compartment kernels of the step run (null: synapses_step)
+    const float2 *syn_ext;
+};
+
+// Arguments of lems_syn_kernel (the generic LEMS synapse kernel, compiled at run time from
+// lems_kernel.hip): shared by that compilation and the host code that launches it.
+struct LemsArgs {
+    const int32_t *comps;         // compartments with synapses
+    int32_t n_comps;
+    const float *params;          // all types' parameter vectors
+    const int32_t *param_begin;   // [n_types + 1]
+    float *state;                 // [max state][stride]: variable k of slot s at k * stride + s
+    int64_t stride;
+    float2 *syn_ext;              // [NC] per-compartment (I_syn, G_syn) of the step
 };
 
 // Correctly rounded fp32 division whose expansion does not go through VCC.

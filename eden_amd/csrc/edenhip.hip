@@ -22,6 +22,7 @@
 #include "../../include/edenhip.h"
 #include "device_model.hpp"
 #include "kernels.hpp"
+#include "rtc.hpp"
 #include "tree_schedule.hpp"
 
 using namespace edenhip;
@@ -133,6 +134,13 @@ struct edenhip_ctx {
     int64_t net_windows = 0, net_redone = 0;
     std::string kernel_name;
     std::vector<void *> allocs;
+    // generic LEMS synapses: the run-time compiled synapse kernel and its tables
+    hipFunction_t lems_fn = nullptr;
+    int32_t *d_lems_comps = nullptr;
+    int32_t n_lems_comps = 0;
+    float *d_lems_params = nullptr, *d_lems_state = nullptr;
+    int32_t *d_lems_pbegin = nullptr;
+    float2 *d_syn_ext = nullptr;
     // edenhip_run_static: the tree buckets' work-queue counters come from the device's static
     // scratch array for the duration of that call (nullptr: each bucket's own d_queue)
     int32_t *queue_base = nullptr;
@@ -250,7 +258,17 @@ static int pool_stream(edenhip_ctx *ctx, int slot, hipStream_t *out) {
 }
 
 static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
-    if (d->abi_version != EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
+    if (d->abi_version != 1 && d->abi_version != EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
+    const edenhip_lems_synapses *LS = d->abi_version >= 2 ? d->lems : nullptr;
+    const bool lems = LS && LS->n_types > 0;
+    if (lems) {
+        if (!LS->source || !LS->param_begin || !LS->n_state) return fail(ctx, EDENHIP_E_INVALID, "lems: null table");
+        if (LS->n_types > 4096) return fail(ctx, EDENHIP_E_LIMIT, "lems: more than 4096 synapse types");
+        for (int t = 0; t < LS->n_types; t++) {
+            if (LS->n_state[t] < 0 || LS->n_state[t] > 8) return fail(ctx, EDENHIP_E_LIMIT, "lems: more than 8 state variables");
+            if (LS->param_begin[t + 1] < LS->param_begin[t]) return fail(ctx, EDENHIP_E_INVALID, "lems: param_begin decreasing");
+        }
+    }
     if (!(d->dt > 0) || !(d->t_final >= 0)) return fail(ctx, EDENHIP_E_INVALID, "dt must be > 0 and t_final >= 0");
     if (d->n_cells < 0 || d->n_comps < 0) return fail(ctx, EDENHIP_E_INVALID, "negative sizes");
     const int64_t NC = d->n_comps;
@@ -353,8 +371,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     for (int64_t g = 0; g < NC; g++)
         for (int64_t s = d->comp_syn_begin[g] + 1; s < d->comp_syn_begin[g + 1]; s++)
             if (d->syn_type[s] < d->syn_type[s - 1]) return fail(ctx, EDENHIP_E_INVALID, "synapses of a compartment must be sorted by type");
+    const int32_t n_inst_types = lems ? LS->n_types : d->n_syn_types;
     for (int64_t s = 0; s < n_syn; s++)
-        if (d->syn_type[s] < 0 || d->syn_type[s] >= d->n_syn_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
+        if (d->syn_type[s] < 0 || d->syn_type[s] >= n_inst_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
     for (int64_t k = 0; k < n_spk; k++)
         if (d->spike_target[k] < 0 || d->spike_target[k] >= n_syn) return fail(ctx, EDENHIP_E_INVALID, "spike target out of range");
     std::vector<float> poi_mean(n_poi), poi_w(n_poi);
@@ -610,7 +629,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     for (int64_t k = 0; k < n_spk; k++) spk_target[k] = (int32_t)blocked_of[d->spike_target[k]];
     bool any_targets = n_spk > 0;
     ctx->n_slots = n_slots;
-    ctx->network = any_targets;
+    // generic synapses run in their own kernel before the compartments of each step: one
+    // launch per step, like a network
+    ctx->network = any_targets || (lems && n_syn > 0);
     ctx->tree_net = ctx->tree_net || ctx->network;
     ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0;
 
@@ -655,6 +676,25 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     TRY(alloc_zero(ctx, &T.poi_isi, (size_t)n_poi));
     TRY(alloc_zero(ctx, &T.poi_A, (size_t)n_poi));
     TRY(alloc_zero(ctx, &T.poi_B, (size_t)n_poi));
+    if (lems && n_syn > 0) {
+        // generic synapse tables: compartments with synapses, parameters, state [k][slot]
+        std::vector<int32_t> lc;
+        for (int64_t g = 0; g < NC; g++)
+            if (syn_cnt[g] > 0) lc.push_back((int32_t)g);
+        ctx->n_lems_comps = (int32_t)lc.size();
+        int32_t smax = 1;
+        for (int t = 0; t < LS->n_types; t++) smax = std::max(smax, LS->n_state[t]);
+        std::vector<float> prm(LS->params, LS->params + LS->param_begin[LS->n_types]);
+        std::vector<int32_t> pb(LS->param_begin, LS->param_begin + LS->n_types + 1);
+        TRY(upload(ctx, &ctx->d_lems_comps, lc));
+        TRY(upload(ctx, &ctx->d_lems_params, prm));
+        TRY(upload(ctx, &ctx->d_lems_pbegin, pb));
+        TRY(alloc_zero(ctx, &ctx->d_lems_state, (size_t)(smax * n_slots)));
+        TRY(alloc_zero(ctx, &ctx->d_syn_ext, (size_t)NC));
+        T.syn_ext = ctx->d_syn_ext;
+        std::string err;
+        if (rtc_lems_kernel(LS->source, &ctx->lems_fn, err) != 0) return fail(ctx, EDENHIP_E_INVALID, err);
+    }
     TRY(upload(ctx, (uint32_t **)&T.cell_seed, seeds));
     TRY(upload(ctx, (int32_t **)&T.comp_cell, comp_cell));
     TRY(upload(ctx, (double **)&T.t_row, ts));
@@ -696,6 +736,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     TRY(snap(T.poi_isi, n_poi * sizeof(float)));
     TRY(snap(T.poi_A, n_poi * sizeof(float)));
     TRY(snap(T.poi_B, n_poi * sizeof(float)));
+    if (ctx->d_lems_state) {
+        int32_t smax = 1;
+        for (int t = 0; t < LS->n_types; t++) smax = std::max(smax, LS->n_state[t]);
+        TRY(snap(ctx->d_lems_state, (size_t)(smax * n_slots) * sizeof(float)));
+    }
 
     TRY(upload(ctx, &ctx->d_point, point));
     TRY(upload(ctx, &ctx->d_words, words));
@@ -794,7 +839,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
 // Windowed network path: eligibility, per-compartment type runs, spike-target ring
 // addresses, second state copies. Models it does not cover keep the per-step path.
 static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point) {
-    if (!ctx->network || !ctx->buckets.empty() || ctx->opt.network_window == -1) return 0;
+    if (!ctx->network || !ctx->buckets.empty() || ctx->opt.network_window == -1 || ctx->lems_fn) return 0;
     const int64_t NC = d->n_comps;
     if (d->n_syn_types > 64 || (int64_t)point.size() != NC) return 0;
     const double dt = (double)d->dt;
@@ -1030,6 +1075,17 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
         for (int64_t a = a0; a < a1; a += stride) {
             const int64_t b = std::min(a1, a + stride);
             const bool first = a == s0, last = b == s1;
+            if (ctx->lems_fn && ctx->n_lems_comps > 0) {
+                // generic synapses of step a: per-compartment currents for the kernels below
+                LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,
+                     ctx->n_slots, ctx->d_syn_ext};
+                int64_t st = a;
+                float dt = ctx->dt;
+                void *args[] = {&T, &la, &st, &dt};
+                HIPCHECK(ctx, hipModuleLaunchKernel(ctx->lems_fn, (unsigned)((ctx->n_lems_comps + 63) / 64), 1, 1, 64, 1, 1,
+                                                    0, ctx->stream, args, nullptr));
+                nl++;
+            }
             if (ctx->n_point > 0) {
                 const int tpb = 64;
                 float dt = ctx->dt;

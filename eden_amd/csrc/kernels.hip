@@ -65,7 +65,15 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
         float I_ch, G_ch;
         channels_step(gates, k, V, q, init, dt, I_ch, G_ch);
         float I_syn = 0.0f, G_syn = 0.0f;
-        if (has_syn) synapses_step(T, stypes, syn_base, syn_cnt, V, init, t, dt, now, s, I_syn, G_syn);
+        if (has_syn) {
+            if (T.syn_ext) {
+                const float2 e = T.syn_ext[g];   // generic synapses of this step (lems_syn_kernel)
+                I_syn = e.x;
+                G_syn = e.y;
+            } else {
+                synapses_step(T, stypes, syn_base, syn_cnt, V, init, t, dt, now, s, I_syn, G_syn);
+            }
+        }
         float I_in = pulses_current(T, k, g, t), G_in = 0.0f;
         if (has_poi) {
             const PoiState ps{T.poi_tsince, T.poi_isi, T.poi_A, T.poi_B};
@@ -554,8 +562,14 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                 // trigger buffers alternate as in point_kernel)
                 if (cx.tcm & TCM_SYN) {
                     const int64_t g = c0 + c;
-                    synapses_step(T, stypes, T.syn_base[g], T.syn_cnt[g], V[j], init, t, dt, (int)((s + 3) & 1), s,
-                                  I_syn, G_syn);
+                    if (T.syn_ext) {
+                        const float2 e = T.syn_ext[g];   // generic synapses of this step (lems_syn_kernel)
+                        I_syn = e.x;
+                        G_syn = e.y;
+                    } else {
+                        synapses_step(T, stypes, T.syn_base[g], T.syn_cnt[g], V[j], init, t, dt, (int)((s + 3) & 1), s,
+                                      I_syn, G_syn);
+                    }
                 }
             }
             const float I_int = I_ch + I_in + I_syn;

@@ -1,0 +1,77 @@
+// lems_kernel.hip -- synapse kernel template for generic LEMS synapse types, compiled at
+// context creation with hiprtc together with the generated functions (edenhip_lems_synapses:
+// `lems_syn`, emitted by eden_amd.lems), the counterpart of the reference's per-cell-type C
+// emission and gcc step (eden/Eden.cpp:2339-2995, 3561-3740, 6992-7216).
+//
+// One lane per compartment that has chemical synapses; per step, for each of its synapse
+// instances in list order: the trigger read and clear and the single-slot delay rule
+// (eden/Eden.cpp:3636-3707, the same code as synapses_step), then the instance's LEMS step
+// (lems_syn), its exposures times the connection weight aggregated per synapse type, the
+// types' aggregates summed in order (eden/Eden.cpp:3721-3735). The totals go to syn_ext,
+// where the compartment kernels of the same step read them in place of their own synapse
+// code. State is updated in place (the generated code reads an instance's whole present
+// state before writing).
+//
+// The generated source is inserted where LEMS_SYNAPSE_FUNCTIONS is included.
+#include "device_model.hpp"
+
+#define EXP edenhip::exp_f
+#define LOG edenhip::log_f
+
+#include "lems_synapse_functions.h"
+
+extern "C" __global__ void __launch_bounds__(64)
+lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float dt) {
+    using namespace edenhip;
+    const int32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= L.n_comps) return;
+    const int32_t g = L.comps[idx];
+    const bool init = step <= 0;
+    // time_f32 of the step, from the reference's double accumulation
+    const float t = (float)(step >= 1 ? T.t_row[step - 1] : 0.0);
+    const int now = (int)((step + 3) & 1);
+    const float V = T.V[g];
+    const int32_t base = T.syn_base[g], cnt = T.syn_cnt[g];
+    int32_t *trig_now = T.trig[now];
+    float I_syn = 0.0f, G_syn = 0.0f, I_agg = 0.0f, G_agg = 0.0f;
+    int32_t cur = -1;
+    for (int32_t k = 0; k < cnt; k++) {
+        const int32_t s = base + 64 * k;
+        const int32_t ty = T.syn_type[s];
+        if (ty != cur) {
+            if (cur >= 0) { I_syn += I_agg; G_syn += G_agg; }
+            I_agg = 0.0f; G_agg = 0.0f; cur = ty;
+        }
+        const float weight = T.syn_weight[s];
+        bool flag = false;
+        if (!init) {
+            flag = trig_now[s] != 0;
+            if (flag) trig_now[s] = 0;
+        }
+        const float delay = T.syn_delay[s];
+        const float next_spike = T.syn_next[s];
+        float nn = next_spike;
+        bool spike_now = false;
+        if (!init) {
+            if (t <= next_spike && next_spike < t + dt) spike_now = true;
+            if (flag) {
+                const float fresh = t + delay;
+                if (t <= fresh && fresh < t + dt) spike_now = true;
+                if (nn < t + dt && nn < fresh) nn = fresh;
+            }
+        }
+        if (nn != next_spike) T.syn_next[s] = nn;
+        if (spike_now && T.log_syn) {
+            const uint32_t slot = atomicAdd(T.n_syn_events, 1u);
+            if (slot < T.max_syn_events) T.syn_events[slot] = (step << 32) | (int64_t)T.syn_orig[s];
+        }
+        float i, gx;
+        float *S = L.state + s;
+        lems_syn(ty, L.params + L.param_begin[ty], S, S, (long)L.stride, V, init ? 1 : 0, spike_now ? 1 : 0, dt, &i, &gx);
+        I_agg += i * weight;
+        G_agg += gx * weight;
+    }
+    I_syn += I_agg;
+    G_syn += G_agg;
+    L.syn_ext[g] = make_float2(I_syn, G_syn);
+}

@@ -1,0 +1,14 @@
+// rtc.hpp -- run-time compilation of the generic LEMS synapse kernel (hiprtc).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace edenhip {
+
+// Compile lems_kernel.hip with the generated `lems_syn` source for the current device and load
+// it; returns hipSuccess-like 0 or -1 with `err` set. Compiled code objects are kept per
+// (device architecture, source) for the process, so contexts of one model compile once.
+int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, std::string &err);
+
+}  // namespace edenhip

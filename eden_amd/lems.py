@@ -1,0 +1,331 @@
+"""Generic LEMS synapse mechanisms: a restatement of the NeuroML core synapse ComponentTypes,
+a LEMS expression parser, and an emitter of their per-step update as C / HIP functions.
+
+The reference turns every LEMS ComponentType it instantiates into C text (DescribeLems,
+eden/Eden.cpp:2339-2995), compiles it with gcc and loads it with dlopen
+(eden/Eden.cpp:6992-7216). Here the same per-step semantics are emitted as device functions
+that the library compiles with hiprtc into a kernel template at context creation
+(eden_amd/csrc/lems_kernel.hip, include/edenhip.h `edenhip_lems_synapses`); the oracle
+compiles the C dialect of the same text with gcc.
+
+Semantics of one step of one instance (eden/Eden.cpp:2800-2995):
+  - parameters, derived parameters (from parameters), the `weight` property;
+  - derived variables from the present state, in dependency order;
+  - exposures i (current) and g (conductance; 0 for current-based synapses);
+  - on an initialisation step: each OnStart assignment, evaluated on the present state;
+  - otherwise: next = state + dt * derivative for every state variable with a time
+    derivative (each derivative from the present state), next = state for the others; then
+    OnEvent assignments (evaluated on the present state) replace the next value when the
+    instance receives its event this step.
+Expressions are printed fully parenthesised, left to right as parsed, with every literal
+rounded to float, `expf`/`logf` as EXP/LOG (the dialect's own: on the device and in the
+oracle, exp/log rounded once from double), so both compilers perform the same float
+operations in the same order. Engine units (us, mV, nA, uS) make every unit conversion
+factor of these types 1 (the reference's Convert::Suffix then emits nothing).
+
+The ComponentType table restates the dynamics of the NeuroML 2 core synapse types (the
+NeuroML standard's definitions, as embedded by the reference in
+eden/neuroml/LEMS_CoreComponents.inc.xml:85-229, 1834-1942).
+"""
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+class LemsError(ValueError):
+    pass
+
+
+# ---------------------------------------------------------------- component types
+
+@dataclass
+class SynapseType:
+    """A synapse ComponentType: `params` (name, NeuroML dimension or None for plain numbers),
+    `derived_params` (name, expression), `constants` (name, engine value), `state` (names),
+    `derived` (name, expression) in dependency order, `derivatives` (state name ->
+    expression), `on_start` and `on_event` (state name -> expression), exposures `i` and
+    (optionally) `g` (names of derived or state variables)."""
+    params: list
+    state: list
+    derived: list
+    derivatives: dict
+    on_start: dict
+    on_event: dict
+    expose_i: str
+    expose_g: str = None
+    derived_params: list = field(default_factory=list)
+    constants: list = field(default_factory=list)
+
+
+_COND = [("gbase", "conductance"), ("erev", "voltage")]
+_PYNN = [("tau_syn", None)]
+_PYNN_CONST = [("MVOLT", 1.0), ("MSEC", 1000.0), ("NAMP", 1.0)]   # 1 mV, 1 ms, 1 nA in engine units
+
+SYNAPSE_TYPES = {
+    "expOneSynapse": SynapseType(
+        params=_COND + [("tauDecay", "time")], state=["g"], derived=[("i", "g * (erev - v)")],
+        derivatives={"g": "-g / tauDecay"}, on_start={"g": "0"}, on_event={"g": "g + (gbase * weight)"},
+        expose_i="i", expose_g="g"),
+    "alphaCurrentSynapse": SynapseType(
+        params=[("tau", "time"), ("ibase", "current")], state=["I", "J"], derived=[("i", "I")],
+        derivatives={"I": "(2.7182818284590451*J - I)/tau", "J": "-J/tau"},
+        on_start={"I": "0", "J": "0"}, on_event={"J": "J + weight * ibase"}, expose_i="i"),
+    "alphaSynapse": SynapseType(
+        params=_COND + [("tau", "time")], state=["g", "A"], derived=[("i", "g * (erev - v)")],
+        derivatives={"g": "(2.7182818284590451 * A - g)/tau", "A": "-A / tau"},
+        on_start={"g": "0", "A": "0"}, on_event={"A": "A + (gbase*weight)"}, expose_i="i", expose_g="g"),
+    "expTwoSynapse": SynapseType(
+        params=_COND + [("tauRise", "time"), ("tauDecay", "time")],
+        derived_params=[("peakTime", "log(tauDecay / tauRise) * (tauRise * tauDecay)/(tauDecay - tauRise)"),
+                        ("waveformFactor", "1 / (-exp(-peakTime / tauRise) + exp(-peakTime / tauDecay))")],
+        state=["A", "B"], derived=[("g", "gbase * (B - A)"), ("i", "g * (erev - v)")],
+        derivatives={"A": "-A / tauRise", "B": "-B / tauDecay"}, on_start={"A": "0", "B": "0"},
+        on_event={"A": "A + (weight * waveformFactor)", "B": "B + (weight * waveformFactor)"},
+        expose_i="i", expose_g="g"),
+    "expThreeSynapse": SynapseType(
+        params=[("gbase1", "conductance"), ("gbase2", "conductance"), ("erev", "voltage"), ("tauRise", "time"),
+                ("tauDecay1", "time"), ("tauDecay2", "time")],
+        derived_params=[
+            ("peakTime1", "log(tauDecay1 / tauRise) * (tauRise * tauDecay1)/(tauDecay1 - tauRise)"),
+            ("waveformFactor1", "1 / (-exp(-peakTime1 / tauRise) + exp(-peakTime1 / tauDecay1))"),
+            ("peakTime2", "log(tauDecay2 / tauRise) * (tauRise * tauDecay2)/(tauDecay2 - tauRise)"),
+            ("waveformFactor2", "1 / (-exp(-peakTime2 / tauRise) + exp(-peakTime2 / tauDecay2))")],
+        state=["A", "B", "C"], derived=[("g", "gbase1*(B - A) + gbase2*(C-A)"), ("i", "g * (erev - v)")],
+        derivatives={"A": "-A / tauRise", "B": "-B / tauDecay1", "C": "-C / tauDecay2"},
+        on_start={"A": "0", "B": "0", "C": "0"},
+        on_event={"A": "A + (gbase1*weight * waveformFactor1 + gbase2*weight*waveformFactor2 )/(gbase1+gbase2)",
+                  "B": "B + (weight * waveformFactor1)", "C": "C + (weight * waveformFactor2)"},
+        expose_i="i", expose_g="g"),
+    "expCondSynapse": SynapseType(
+        params=_PYNN + [("e_rev", None)], constants=_PYNN_CONST, state=["g"],
+        derived=[("i", "g * (e_rev - (v/MVOLT)) * NAMP")], derivatives={"g": "-g / (tau_syn*MSEC)"},
+        on_start={}, on_event={"g": "g+weight"}, expose_i="i"),
+    "expCurrSynapse": SynapseType(
+        params=_PYNN, constants=_PYNN_CONST, state=["I"], derived=[("i", "I * NAMP")],
+        derivatives={"I": "-I / (tau_syn*MSEC)"}, on_start={}, on_event={"I": "I + weight"}, expose_i="i"),
+    "alphaCondSynapse": SynapseType(
+        params=_PYNN + [("e_rev", None)], constants=_PYNN_CONST, state=["g", "A"],
+        derived=[("i", "g * (e_rev - (v/MVOLT)) * NAMP")],
+        derivatives={"g": "(2.7182818*A - g)/(tau_syn*MSEC)", "A": "-A /(tau_syn*MSEC)"},
+        on_start={}, on_event={"A": "A + weight"}, expose_i="i"),
+    "alphaCurrSynapse": SynapseType(
+        params=_PYNN, constants=_PYNN_CONST, state=["I", "A"], derived=[("i", "I * NAMP")],
+        derivatives={"I": "(2.7182818*A - I)/(tau_syn*MSEC)", "A": "-A /(tau_syn*MSEC)"},
+        on_start={}, on_event={"A": "A + weight"}, expose_i="i"),
+}
+
+# PyNN synapses' numbers are plain (ms, mV); their base type gives tau_syn no default here
+
+
+# ---------------------------------------------------------------- expressions
+
+_TOKEN = re.compile(r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|([A-Za-z_]\w*)|(\*\*|[-+*/^(),]))")
+_FUNCS = {"exp": "EXP", "log": "LOG", "ln": "LOG", "sqrt": "sqrtf", "abs": "fabsf", "sin": "sinf", "cos": "cosf",
+          "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "ceil": "ceilf", "floor": "floorf"}
+
+
+def _tokens(text):
+    pos, out = 0, []
+    text = text.strip()
+    while pos < len(text):
+        m = _TOKEN.match(text, pos)
+        if not m or m.end() == pos:
+            raise LemsError(f"cannot read expression '{text}' at '{text[pos:]}'")
+        num, name, op = m.groups()
+        out.append(("num", num) if num else ("name", name) if name else ("op", op))
+        pos = m.end()
+    return out
+
+
+def parse(text):
+    """Expression tree: ('num', value) | ('sym', name) | ('neg', x) | (op, l, r) | ('fn', f, x),
+    with the usual precedence: unary minus, ^ (right-associative), * /, + - (left)."""
+    toks = _tokens(text)
+    pos = [0]
+
+    def peek():
+        return toks[pos[0]] if pos[0] < len(toks) else ("end", None)
+
+    def take():
+        t = peek()
+        pos[0] += 1
+        return t
+
+    def primary():
+        t = take()
+        if t[0] == "num":
+            return ("num", float(t[1]))
+        if t[0] == "name":
+            if peek() == ("op", "("):
+                take()
+                arg = additive()
+                if take() != ("op", ")"):
+                    raise LemsError(f"missing ')' in '{text}'")
+                if t[1] not in _FUNCS:
+                    raise LemsError(f"function {t[1]} is not supported")
+                return ("fn", t[1], arg)
+            return ("sym", t[1])
+        if t == ("op", "("):
+            e = additive()
+            if take() != ("op", ")"):
+                raise LemsError(f"missing ')' in '{text}'")
+            return e
+        raise LemsError(f"unexpected token {t[1]!r} in '{text}'")
+
+    def unary():
+        if peek() == ("op", "-"):
+            take()
+            return ("neg", unary())
+        if peek() == ("op", "+"):
+            take()
+            return unary()
+        return power()
+
+    def power():
+        b = primary()
+        if peek() in (("op", "^"), ("op", "**")):
+            take()
+            return ("pow", b, unary())
+        return b
+
+    def term():
+        e = unary()
+        while peek() in (("op", "*"), ("op", "/")):
+            op = take()[1]
+            e = ("*" if op == "*" else "/", e, unary())
+        return e
+
+    def additive():
+        e = term()
+        while peek() in (("op", "+"), ("op", "-")):
+            op = take()[1]
+            e = (op, e, term())
+        return e
+
+    tree = additive()
+    if pos[0] != len(toks):
+        raise LemsError(f"trailing tokens in '{text}'")
+    return tree
+
+
+def _lit(v):
+    return "( (float)%s)" % repr(float(v))
+
+
+def infix(tree, names):
+    """Fully parenthesised float C text of an expression tree; `names` maps LEMS symbols to
+    C variable names."""
+    k = tree[0]
+    if k == "num":
+        return _lit(tree[1])
+    if k == "sym":
+        if tree[1] not in names:
+            raise LemsError(f"unknown symbol {tree[1]}")
+        return names[tree[1]]
+    if k == "neg":
+        return "( -" + infix(tree[1], names) + " )"
+    if k == "fn":
+        return _FUNCS[tree[1]] + "( " + infix(tree[2], names) + " )"
+    if k == "pow":
+        return "powf( " + infix(tree[1], names) + " , " + infix(tree[2], names) + " )"
+    return "( ( " + infix(tree[1], names) + " " + k + " " + infix(tree[2], names) + " ) )"
+
+
+# ---------------------------------------------------------------- emitter
+
+def emit(name, st, fn_name, dialect):
+    """C / HIP text of one type's step function:
+
+        void fn(const float *P, const float *S, float *Sn, long stride, float v, int init,
+                int event, float dt, float *i_out, float *g_out)
+
+    P: the type's parameter vector (params in table order, then the `weight` property);
+    S / Sn: the instance's present / next state, variable k at k * stride (they may alias:
+    every present value is read before anything is written)."""
+    q = "__device__ __forceinline__ " if dialect == "hip" else "static "
+    L = [f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float v, int init, int event,"
+         f" float dt, float *i_out, float *g_out) {{",
+         f"    /* {name} */"]
+    names = {"v": "v", "dt": "dt"}
+    for k, (p, _) in enumerate(st.params):
+        L.append(f"    const float p_{p} = P[{k}];")
+        names[p] = f"p_{p}"
+    L.append(f"    const float p_weight = P[{len(st.params)}];")
+    names["weight"] = "p_weight"
+    for c, val in st.constants:
+        L.append(f"    const float c_{c} = {_lit(val)};")
+        names[c] = f"c_{c}"
+    for d, expr in st.derived_params:
+        L.append(f"    const float dp_{d} = {infix(parse(expr), names)};")
+        names[d] = f"dp_{d}"
+    for k, s in enumerate(st.state):
+        L.append(f"    const float s_{s} = S[{k} * stride];")
+        names[s] = f"s_{s}"
+    for d, expr in st.derived:
+        L.append(f"    const float d_{d} = {infix(parse(expr), names)};")
+        names[d] = f"d_{d}"
+    for s in st.state:
+        L.append(f"    float n_{s} = s_{s};")
+    L.append("    if (init) {")
+    for s, expr in st.on_start.items():
+        L.append(f"        n_{s} = {infix(parse(expr), names)};")
+    L.append("    } else {")
+    for k, s in enumerate(st.state):
+        if s in st.derivatives:
+            L.append(f"        const float der_{k} = {infix(parse(st.derivatives[s]), names)};")
+            L.append(f"        n_{s} = s_{s} + dt * der_{k};")
+    L.append("        if (event) {")
+    for s, expr in st.on_event.items():
+        L.append(f"            n_{s} = {infix(parse(expr), names)};")
+    L.append("        }")
+    L.append("    }")
+    for k, s in enumerate(st.state):
+        L.append(f"    Sn[{k} * stride] = n_{s};")
+    L.append(f"    *i_out = {names[st.expose_i]};")
+    L.append(f"    *g_out = {names[st.expose_g] if st.expose_g else '0.0f'};")
+    L.append("}")
+    return "\n".join(L)
+
+
+@dataclass
+class LemsSynapses:
+    """The generic synapse types of a model: per type its component type name, parameter
+    vector and state size, and the emitted source in both dialects."""
+    type_names: list
+    params: list          # per type: np.float32 vector
+    n_state: list
+    source_hip: str
+    source_c: str
+
+
+def build(types):
+    """types: list of (component type name, {param: engine value}, weight property). Returns
+    LemsSynapses whose functions are lems_syn_<k> and a dispatcher lems_syn(type, ...)."""
+    hip, c, params, nst, names = [], [], [], [], []
+    for k, (tname, vals, weight) in enumerate(types):
+        if tname not in SYNAPSE_TYPES:
+            raise LemsError(f"synapse type {tname} is not supported")
+        st = SYNAPSE_TYPES[tname]
+        missing = [p for p, _ in st.params if p not in vals]
+        if missing:
+            raise LemsError(f"{tname}: parameters {missing} missing")
+        params.append(np.array([vals[p] for p, _ in st.params] + [weight], np.float32))
+        nst.append(len(st.state))
+        names.append(tname)
+        hip.append(emit(tname, st, f"lems_syn_{k}", "hip"))
+        c.append(emit(tname, st, f"lems_syn_{k}", "c"))
+
+    def dispatcher(q):
+        L = [f"{q}void lems_syn(int type, const float *P, const float *S, float *Sn, long stride, float v, int init,"
+             " int event, float dt, float *i_out, float *g_out) {", "    switch (type) {"]
+        for k in range(len(types)):
+            L.append(f"    case {k}: lems_syn_{k}(P, S, Sn, stride, v, init, event, dt, i_out, g_out); break;")
+        L.append("    default: *i_out = 0.0f; *g_out = 0.0f; break;")
+        L.append("    }")
+        L.append("}")
+        return "\n".join(L)
+
+    src_hip = "\n\n".join(hip + [dispatcher("__device__ __forceinline__ ")]) + "\n"
+    src_c = "\n\n".join(c + [dispatcher("")]) + "\n"
+    return LemsSynapses(names, params, nst, src_hip, src_c)

@@ -104,6 +104,7 @@ class Model:
     poisson_rng_offset: np.ndarray = None
     record_comp: np.ndarray = None
     event_comp: np.ndarray = None
+    lems: object = None                # eden_amd.lems.LemsSynapses: generic synapse types
     name: str = "model"
 
     def __post_init__(self):
@@ -224,9 +225,18 @@ class Model:
         d.syn_types = C.cast(self._st, C.c_void_p)
         for name, _ in N.ModelDesc._fields_:
             if name in ("abi_version", "dt", "t_final", "n_cells", "n_comps", "n_channel_types", "channel_types",
-                        "n_syn_types", "syn_types", "n_record", "n_event_cols"):
+                        "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems"):
                 continue
             setattr(d, name, N.ptr(getattr(self, name)))
+        if self.lems is not None:
+            L = self.lems
+            pb = np.zeros(len(L.params) + 1, np.int32)
+            np.cumsum([len(p) for p in L.params], out=pb[1:])
+            self._lems_arrays = (pb, np.concatenate(L.params + [np.zeros(1, np.float32)]).astype(np.float32),
+                                 np.array(L.n_state, np.int32), L.source_hip.encode())
+            self._lems_desc = N.LemsSynapsesDesc(len(L.params), self._lems_arrays[3], N.ptr(self._lems_arrays[0]),
+                                                 N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]))
+            d.lems = C.pointer(self._lems_desc)
         d.n_record = len(self.record_comp)
         d.n_event_cols = len(self.event_comp)
         return d

@@ -17,7 +17,10 @@ set covers:
                     resistivity), each over a segment group, later specs overriding earlier
                     ones (eden/NeuroML.cpp:2530-3566); one compartment per segment (EDEN's
                     free-standing segments, eden/Eden.cpp:1437-1491)
-  synapses          expTwoSynapse (eden/NeuroML.cpp:6506-6829)
+  synapses          expTwoSynapse natively; expOneSynapse, alphaSynapse, alphaCurrentSynapse,
+                    expThreeSynapse and the PyNN synapses (expCond, expCurr, alphaCond,
+                    alphaCurr) as generic LEMS types (eden_amd.lems), compiled by the library
+                    at run time (eden/NeuroML.cpp:6506-6829)
   inputs            pulseGenerator, poissonFiringSynapse (eden/NeuroML.cpp:6968-7107)
   networks          population (size or instances), projection with connection /
                     connectionWD, synapticConnection, explicitInput, inputList with
@@ -49,6 +52,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as N
+from . import lems as lems_mod
 from .model import Model, cell_rng_seeds, channel_type
 
 
@@ -116,12 +120,10 @@ _CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Sy
 
 # element names of the synapse and input types this reader instantiates, by role; a LEMS
 # <Component type="..."> of these types is read as the element itself
-_SYNAPSES = {"expTwoSynapse"}
+_SYNAPSES = set(lems_mod.SYNAPSE_TYPES)
 _INPUTS = {"pulseGenerator", "poissonFiringSynapse"}
-_UNSUPPORTED_SYNAPSES = {"alphaCurrentSynapse", "alphaSynapse", "expOneSynapse", "expThreeSynapse",
-                         "blockingPlasticSynapse", "silentSynapse", "gapJunction", "linearGradedSynapse",
-                         "gradedSynapse", "expCondSynapse", "alphaCondSynapse", "expCurrSynapse",
-                         "alphaCurrSynapse", "doubleSynapse"}
+_UNSUPPORTED_SYNAPSES = {"blockingPlasticSynapse", "silentSynapse", "gapJunction", "linearGradedSynapse",
+                         "gradedSynapse", "doubleSynapse"}
 _UNSUPPORTED_CELLS = {"iafCell", "iafTauCell", "iafRefCell", "iafTauRefCell", "izhikevichCell",
                       "izhikevich2007Cell", "adExIaFCell", "fitzHughNagumoCell", "pinskyRinzelCA3Cell",
                       "IF_curr_alpha", "IF_curr_exp", "IF_cond_alpha", "IF_cond_exp", "EIF_cond_exp_isfa_ista",
@@ -624,25 +626,46 @@ def read(path, rng_seed=None):
         c = p.first_cell + p.seq(inst_id, what)
         return int(cell_begin[c]) + p.cell.comp_of_segment(seg_id, what)
 
-    # synapse types in reading order (their component sequence numbers), inputs
-    syn_index, syn_types = {}, []
-    for role, i in doc.order:
-        if role == "synapse":
-            el = doc.items[i][1]
-            syn_index[i] = len(syn_types)
+    # synapse types in reading order (their component sequence numbers). Models whose
+    # projections use only expTwoSynapse run it natively; otherwise every synapse element
+    # becomes a generic LEMS type (in the same order, so per-compartment type order is the
+    # reference's), and expTwo types stay native for the Poisson inputs that drive them.
+    syn_elems = [(i, doc.items[i][1]) for role, i in doc.order if role == "synapse"]
+    native_index, syn_types = {}, []
+    for i, el in syn_elems:
+        if _tag(el) == "expTwoSynapse":
+            native_index[i] = len(syn_types)
             syn_types.append(N.ExpTwoType(
                 _f32(convert(el.get("gbase"), "conductance", f"{i} gbase")),
                 _f32(convert(el.get("erev"), "voltage", f"{i} erev")),
                 _f32(convert(el.get("tauRise"), "time", f"{i} tauRise")),
                 _f32(convert(el.get("tauDecay"), "time", f"{i} tauDecay")), 1.0))
+    used = set()
+    for el in net:
+        if _tag(el) in ("projection", "synapticConnection"):
+            used.add(el.get("synapse"))
+    generic = any(doc.items.get(u, (None, None))[0] == "synapse" and _tag(doc.items[u][1]) != "expTwoSynapse"
+                  for u in used)
+    generic_index, generic_types = {}, []
+    if generic:
+        for i, el in syn_elems:
+            st = lems_mod.SYNAPSE_TYPES[_tag(el)]
+            vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
+            generic_index[i] = len(generic_types)
+            generic_types.append((_tag(el), vals, 1.0))
 
-    def synapse(sid, what):
+    def synapse(sid, what, native=False):
         role, el = doc.items.get(sid, (None, None))
         if role == "unsupported":
-            raise NeuroMLError(f"{what}: synapse type <{_tag(el)}> is not supported (expTwoSynapse only)")
+            raise NeuroMLError(f"{what}: synapse type <{_tag(el)}> is not supported")
         if role != "synapse":
             raise NeuroMLError(f"{what}: synapse '{sid}' not found")
-        return syn_index[sid]
+        if native or not generic:
+            if sid not in native_index:
+                raise NeuroMLError(f"{what}: synapse '{sid}' ({_tag(el)}) needs the generic synapse path; "
+                                   "Poisson inputs drive expTwoSynapse only")
+            return native_index[sid]
+        return generic_index[sid]
 
     # network inputs in order (explicitInput and inputList entries as they appear)
     inputs = []    # (component id, compartment, weight)
@@ -722,7 +745,7 @@ def read(path, rng_seed=None):
             k = poi_count.get((cell, iid), 0)
             poi_count[(cell, iid)] = k + 1
             rate = _f32(convert(el.get("averageRate"), "frequency", "averageRate"))
-            poisson[g].append((rate, synapse(el.get("synapse"), f"poissonFiringSynapse {iid}"), _f32(w), k))
+            poisson[g].append((rate, synapse(el.get("synapse"), f"poissonFiringSynapse {iid}", native=True), _f32(w), k))
     pu_b = np.zeros(nc + 1, np.int64)
     np.cumsum([len(x) for x in pulses], out=pu_b[1:])
     pu = [x for lst in pulses for x in lst]
@@ -804,6 +827,7 @@ def read(path, rng_seed=None):
         poisson_weight=f32(po, 2), poisson_instance=np.array([x[3] for x in po], np.int32),
         poisson_rng_offset=np.zeros(len(po), np.int32),
         record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
+        lems=lems_mod.build(generic_types) if generic else None,
         name=os.path.splitext(os.path.basename(path))[0])
     model.sim_seed = seed
     return model, Outputs(trajectories, events, base_dir)

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define EDENHIP_ABI_VERSION 1
+#define EDENHIP_ABI_VERSION 2   /* 2 adds edenhip_model_desc.lems; version 1 descriptors are accepted */
 
 enum {
     EDENHIP_OK = 0,
@@ -81,6 +81,25 @@ typedef struct edenhip_exptwo_type {
     float tau_decay;     /* us */
     float weight_prop;   /* the LEMS `weight` property, default 1 */
 } edenhip_exptwo_type;
+
+/* Generic LEMS synapse types (alphaSynapse, expThreeSynapse, the PyNN synapses, ...). The
+ * reference emits C for each lemsified ComponentType and compiles it with gcc
+ * (eden/Eden.cpp:2339-2995, 6992-7216); here the caller supplies HIP source defining
+ *   __device__ void lems_syn(int type, const float *P, const float *S, float *Sn, long stride,
+ *                            float v, int init, int event, float dt, float *i, float *g)
+ * (one instance's step: P its type's parameters, S / Sn its present / next state with
+ * variable k at k * stride, v the compartment voltage, event whether it receives its event
+ * this step; i, g its current and conductance exposures), which edenhip_create() compiles
+ * with hiprtc into the library's synapse kernel template (eden_amd.lems emits it). When
+ * present, every chemical synapse instance's syn_type indexes these types, and syn_types
+ * serves only the Poisson inputs. */
+typedef struct edenhip_lems_synapses {
+    int32_t n_types;
+    const char *source;           /* HIP C++ source, NUL-terminated */
+    const int32_t *param_begin;   /* [n_types + 1] into params */
+    const float *params;          /* per type: its parameter vector */
+    const int32_t *n_state;       /* [n_types] state variables per instance (at most 8) */
+} edenhip_lems_synapses;
 
 /*
  * The instantiated model. Global compartment index g runs over all cells in cell order;
@@ -153,6 +172,9 @@ typedef struct edenhip_model_desc {
     const int64_t *record_comp;                  /* [n_record] compartment whose V is logged */
     int64_t n_event_cols;                        /* spike-event logger columns */
     const int64_t *event_comp;                   /* [n_event_cols] compartment spiking */
+
+    /* (abi_version >= 2) generic LEMS synapse types, or NULL */
+    const edenhip_lems_synapses *lems;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

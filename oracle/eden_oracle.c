@@ -161,6 +161,15 @@ static void log_poi_event(int64_t p, int64_t step) {
     if (k < g_poi_ev_max) g_poi_ev[k] = (step << 32) | p;
 }
 
+/* Generic LEMS synapse types (edenhip_model_desc.lems): the step function `lems_syn` of the
+ * C dialect of the source the model carries (eden_amd.lems emits both dialects from one
+ * expression tree), compiled by the test harness and handed over here. fp32 flavours only. */
+typedef void (*lems_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float v, int init,
+                          int event, float dt, float *i_out, float *g_out);
+static lems_fn_t g_lems = NULL;
+
+void FN(set_lems)(void *fn) { g_lems = (lems_fn_t)fn; }
+
 static int64_t *g_syn_ev = NULL;
 static int64_t g_syn_ev_max = 0;
 static int64_t g_syn_ev_n = 0;
@@ -220,6 +229,7 @@ typedef struct {
     real *sA[2], *sB[2];    /* expTwo A, B per synapse */
     real *pT[2], *pI[2];    /* Poisson tsince, isi */
     real *pA[2], *pB[2];    /* Poisson synapse A, B */
+    float *lS[2];           /* generic LEMS synapse state [k][synapse] */
     int64_t *evflag[2];     /* event logger flags */
     /* derived tables */
     int64_t *gate_begin;    /* per channel instance */
@@ -294,7 +304,6 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                     if (cur_type >= 0) { I_synapses_total += I_agg; G_synapses_total += G_agg; }
                     I_agg = 0; G_agg = 0; cur_type = ty;
                 }
-                const edenhip_exptwo_type *st = &d->syn_types[ty];
                 real weight = (real)d->syn_weight[sy];
                 int spike_in_flag = 0;
                 if (!initial_state) {
@@ -316,7 +325,19 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                 spike_in_flag = spike_now;
                 if (spike_now) log_syn_event(sy, step);
                 s->nspk[nxt][sy] = next_next_spike;
+                if (s->lS[0]) {
+                    /* a generic LEMS synapse: its type's step function */
+                    const edenhip_lems_synapses *L = d->lems;
+                    const int64_t NSYN = d->comp_syn_begin[d->n_comps];
+                    float iv, gv;
+                    g_lems(ty, L->params + L->param_begin[ty], s->lS[now] + sy, s->lS[nxt] + sy, (long)NSYN,
+                           (float)Vcomp, initial_state, spike_in_flag, (float)dt, &iv, &gv);
+                    I_agg += (real)iv * weight;
+                    G_agg += (real)gv * weight;
+                    continue;
+                }
                 /* expTwoSynapse */
+                const edenhip_exptwo_type *st = &d->syn_types[ty];
                 real A = s->sA[now][sy], B = s->sB[now][sy];
                 real tr = (real)st->tau_rise, td = (real)st->tau_decay;
                 real peakTime = (LOG(td / tr) * (tr * td)) / (td - tr);
@@ -479,7 +500,12 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
 int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, int32_t n_threads,
                 float *trace, int64_t *ev_col, int64_t *ev_row, int64_t max_ev, int64_t *n_ev,
                 float *final_v) {
-    if (!d || d->abi_version != EDENHIP_ABI_VERSION || substeps < 1) return -1;
+    if (!d || (d->abi_version != 1 && d->abi_version != EDENHIP_ABI_VERSION) || substeps < 1) return -1;
+    const int lems = d->abi_version >= 2 && d->lems && d->lems->n_types > 0;
+#if defined(ORACLE_F64)
+    if (lems) return -1; /* generic synapses: fp32 flavours only */
+#endif
+    if (lems && !g_lems) return -1;
     const int64_t NC = d->n_comps;
     const int64_t NCH = d->comp_chan_begin[NC];
     const int64_t NSY = d->comp_syn_begin[NC];
@@ -499,6 +525,11 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     }
     ALLOC(s.D, NC, real);
     ALLOC(s.order, NC, int32_t);
+    if (lems) {
+        int32_t smax = 1;
+        for (int32_t t = 0; t < d->lems->n_types; t++) smax = d->lems->n_state[t] > smax ? d->lems->n_state[t] : smax;
+        for (int b = 0; b < 2; b++) ALLOC(s.lS[b], (int64_t)smax * (NSY > 0 ? NSY : 1), float);
+    }
     for (int32_t c = 0; c < d->n_cells; c++) {
         int64_t c0 = d->cell_comp_begin[c];
         dfs_order((int32_t)(d->cell_comp_begin[c + 1] - c0), d->parent + c0, s.order + c0);
@@ -576,5 +607,6 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
         free(s.pT[b]); free(s.pI[b]); free(s.pA[b]); free(s.pB[b]); free(s.evflag[b]);
     }
     free(s.D); free(s.order); free(s.gate_begin); free(s.comp_ev_begin); free(s.comp_ev);
+    free(s.lS[0]); free(s.lS[1]);
     return rows;
 }

@@ -49,6 +49,9 @@ def lib():
             pc = getattr(L, f"oracle_{sfx}_poisson_log_count")
             pc.restype = C.c_int64
             pc.argtypes = []
+            sl = getattr(L, f"oracle_{sfx}_set_lems")
+            sl.restype = None
+            sl.argtypes = [C.c_void_p]
             o = getattr(L, f"oracle_{sfx}_dfs_order")
             o.restype = None
             o.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
@@ -104,6 +107,44 @@ def load_variant(path):
     return L
 
 
+_LEMS_PRELUDE = {
+    # the oracle's own exp/log: rounded once from a double evaluation (as the f32 flavour)
+    "exact": "#include <math.h>\nstatic inline float EXP(float x) { return (float)exp((double)x); }\n"
+             "static inline float LOG(float x) { return (float)log((double)x); }\n",
+    # glibc expf/logf (the f32libm and f32ofast flavours)
+    "libm": "#include <math.h>\n#define EXP expf\n#define LOG logf\n",
+}
+_LEMS_FLAGS = {
+    "exact": ["-O2", "-ffp-contract=off"],
+    "libm": ["-O2", "-ffp-contract=off"],
+    "ofast": ["-Ofast", "-march=x86-64-v3", "-mtune=generic"],
+}
+_lems_libs = {}
+
+
+def lems_library(source_c, flavour="exact"):
+    """The C dialect of a model's generic synapse source compiled with gcc into a shared
+    library (cached per source and flavour); returns the ctypes library."""
+    import hashlib
+    import subprocess
+    import tempfile
+    key = hashlib.sha1((flavour + "\n" + source_c).encode()).hexdigest()
+    if key in _lems_libs:
+        return _lems_libs[key]
+    d = os.path.join(tempfile.gettempdir(), "eden_oracle_lems")
+    os.makedirs(d, exist_ok=True)
+    src, so = os.path.join(d, key + ".c"), os.path.join(d, key + ".so")
+    if not os.path.exists(so):
+        with open(src, "w") as f:
+            f.write(_LEMS_PRELUDE["exact" if flavour == "exact" else "libm"] + source_c)
+        tmp = so + f".{os.getpid()}.tmp"
+        subprocess.run(["gcc", "-std=c11", "-fPIC", "-shared", *_LEMS_FLAGS[flavour], src, "-o", tmp, "-lm"], check=True)
+        os.replace(tmp, so)
+    lib_ = C.CDLL(so)
+    _lems_libs[key] = lib_
+    return lib_
+
+
 def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_000_000, want_trace=True,
         library=None):
     """Run the restatement on a Model. Returns (trace [rows x n_record] float32,
@@ -111,6 +152,12 @@ def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_
     trace keeps every substeps-th row; event rows are then in fine steps."""
     L = library or lib()
     d = model.desc()
+    if getattr(model, "lems", None) is not None:
+        flav = {"f32": "exact", "f32libm": "libm", "f32ofast": "ofast"}.get(precision)
+        if flav is None:
+            raise RuntimeError("generic LEMS synapses: the oracle runs them in its fp32 flavours only")
+        fn = C.cast(lems_library(model.lems.source_c, flav).lems_syn, C.c_void_p)
+        getattr(L, f"oracle_{precision}_set_lems")(fn)
     total = model.total_rows() if n_rows < 0 else n_rows
     nrec = len(model.record_comp)
     trace = np.zeros((max(total, 1), max(nrec, 1)), np.float32) if want_trace else None

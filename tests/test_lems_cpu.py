@@ -1,0 +1,146 @@
+"""Generic LEMS synapse types (eden_amd.lems) on the CPU.
+
+* Expression parsing: precedence and associativity of the printed, fully parenthesised text.
+* Each type's emitted step function (the C dialect, compiled with gcc) against closed forms
+  of its discrete dynamics: after one event from rest, the sum over steps of the conductance
+  (or current) times dt equals a value fixed by the type's parameters. For a variable that
+  decays by Euler steps, x_{n+1} = x_n (1 - dt / tau), the sum of x_n dt is x_0 tau exactly
+  (a geometric series); for the alpha forms, g_{n+1} = g_n + dt (e A_n - g_n) / tau, summing
+  both sides over all steps gives sum g_n dt = e sum A_n dt = e A_0 tau; for expTwo and
+  expThree the charge is the weighted difference of two such series. A wrong coefficient,
+  a missing factor e or a swapped time constant fails one of them.
+* expTwoSynapse emitted from its LEMS definition equals the library's native expTwo in the
+  oracle, bit for bit, over a network run.
+* The reference's CoreSynapses model (its chemical part) reads into one generic type per
+  core synapse, in the reference's instance order, and runs in the oracle.
+"""
+import ctypes as C
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+
+from eden_amd import configs, lems, neuroml
+from oracle import oracle
+
+E = math.e
+
+
+def test_parser_precedence_and_associativity():
+    names = {k: k for k in "abcdx"}
+    assert lems.infix(lems.parse("a - b - c"), names) == "( ( ( ( a - b ) ) - c ) )"
+    assert lems.infix(lems.parse("a / b / c"), names) == "( ( ( ( a / b ) ) / c ) )"
+    assert lems.infix(lems.parse("a + b * c"), names) == "( ( a + ( ( b * c ) ) ) )"
+    assert lems.infix(lems.parse("-a / b"), names) == "( ( ( -a ) / b ) )"
+    assert lems.infix(lems.parse("exp(-x)"), names) == "EXP( ( -x ) )"
+    assert lems.infix(lems.parse("2^3"), names) == "powf( ( (float)2.0) , ( (float)3.0) )"
+    with pytest.raises(lems.LemsError):
+        lems.parse("a + (b")
+    with pytest.raises(lems.LemsError):
+        lems.infix(lems.parse("q * 2"), names)
+
+
+def _stepper(tname, vals, weight=1.0):
+    ls = lems.build([(tname, vals, weight)])
+    lib = oracle.lems_library(ls.source_c)
+    fn = lib.lems_syn
+    fn.restype = None
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_int, C.c_int, C.c_float,
+                   C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    P = ls.params[0]
+    S = np.zeros(ls.n_state[0], np.float32)
+
+    def step(v, init, event, dt):
+        i, g = C.c_float(), C.c_float()
+        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, init, event, dt, C.byref(i), C.byref(g))
+        return i.value, g.value
+    return step
+
+
+def _charge(step, v, dt, n=400_000, use="g"):
+    step(v, 1, 0, dt)                    # an initialisation step
+    tot, first = 0.0, None
+    for k in range(n):
+        i, g = step(v, 0, 1 if k == 0 else 0, dt)
+        x = g if use == "g" else i
+        tot += x * dt
+    return tot
+
+
+DT = 25.0
+
+
+@pytest.mark.parametrize("tname,vals,use,expected", [
+    ("expOneSynapse", dict(gbase=2e-3, erev=0.0, tauDecay=3000.0), "g", lambda p: p["gbase"] * p["tauDecay"]),
+    ("alphaSynapse", dict(gbase=5e-3, erev=0.0, tau=2000.0), "g", lambda p: E * p["gbase"] * p["tau"]),
+    ("alphaCurrentSynapse", dict(tau=4000.0, ibase=0.1), "i", lambda p: E * p["ibase"] * p["tau"]),
+    ("expTwoSynapse", dict(gbase=2e-2, erev=0.0, tauRise=500.0, tauDecay=1000.0), "g", None),
+    ("expThreeSynapse", dict(gbase1=3e-2, gbase2=3e-2, erev=0.0, tauRise=500.0, tauDecay1=1500.0, tauDecay2=5000.0),
+     "g", None),
+    ("expCurrSynapse", dict(tau_syn=2.0), "i", lambda p: 1.0 * p["tau_syn"] * 1000.0),
+    ("alphaCurrSynapse", dict(tau_syn=2.0), "i", lambda p: 2.7182818 * p["tau_syn"] * 1000.0),
+])
+def test_charge_after_one_event(tname, vals, use, expected):
+    step = _stepper(tname, vals, weight=1.0)
+    got = _charge(step, -65.0, DT, use=use)
+    if tname == "expTwoSynapse":
+        tr, td = vals["tauRise"], vals["tauDecay"]
+        tp = math.log(td / tr) * tr * td / (td - tr)
+        wf = 1.0 / (-math.exp(-tp / tr) + math.exp(-tp / td))
+        want = vals["gbase"] * wf * (td - tr)
+    elif tname == "expThreeSynapse":
+        tr, d1, d2, g1, g2 = vals["tauRise"], vals["tauDecay1"], vals["tauDecay2"], vals["gbase1"], vals["gbase2"]
+        wf = [1.0 / (-math.exp(-(math.log(d / tr) * tr * d / (d - tr)) / tr) +
+                     math.exp(-(math.log(d / tr) * tr * d / (d - tr)) / d)) for d in (d1, d2)]
+        A0 = (g1 * wf[0] + g2 * wf[1]) / (g1 + g2)
+        want = g1 * (wf[0] * d1 - A0 * tr) + g2 * (wf[1] * d2 - A0 * tr)
+    else:
+        want = expected(vals)
+    assert got == pytest.approx(want, rel=2e-3), (tname, got, want)
+
+
+def test_conductance_synapse_current_sign_and_size():
+    """i = g (erev - v): at v = erev the current vanishes, below it is inward (positive)."""
+    for tname, vals in (("alphaSynapse", dict(gbase=5e-3, erev=0.0, tau=2000.0)),
+                        ("expCondSynapse", dict(tau_syn=2.0, e_rev=0.0))):
+        step = _stepper(tname, vals)
+        step(-65.0, 1, 0, DT)
+        step(-65.0, 0, 1, DT)
+        for _ in range(20):
+            i_low, g = step(-65.0, 0, 0, DT)
+        assert i_low > 0
+    step = _stepper("expOneSynapse", dict(gbase=2e-3, erev=-10.0, tauDecay=3000.0))
+    step(-10.0, 1, 0, DT)
+    step(-10.0, 0, 1, DT)
+    i, g = step(-10.0, 0, 0, DT)
+    assert i == 0.0 and g > 0
+
+
+def test_generic_exptwo_equals_native_exptwo():
+    m = configs.mixed_network(tree_sizes=(40, 90), n_point=10, n_conn=100, t_final_ms=40.0, seed=5)
+    types = [("expTwoSynapse", dict(gbase=st.gbase, erev=st.erev, tauRise=st.tau_rise, tauDecay=st.tau_decay),
+              st.weight_prop) for st in m.syn_types]
+    g = dataclasses.replace(m, lems=lems.build(types), name="generic")
+    (a, ea, ra, _), (sa, ya) = oracle.run_with_synapse_log(m)
+    (b, eb, rb, _), (sb, yb) = oracle.run_with_synapse_log(g)
+    assert len(sa) > 50
+    assert np.array_equal(a, b) and np.array_equal(ra, rb) and np.array_equal(sa, sb) and np.array_equal(ya, yb)
+
+
+def test_reference_core_synapses_chemical_part(tmp_path):
+    from coresyn import write_chemical_subset
+    m, outs = neuroml.read(write_chemical_subset(str(tmp_path)))
+    assert m.lems.type_names == ["alphaCurrentSynapse", "alphaSynapse", "expOneSynapse", "expTwoSynapse",
+                                 "expThreeSynapse", "expCondSynapse", "alphaCondSynapse", "expCurrSynapse",
+                                 "alphaCurrSynapse"]
+    # ChemPost cells 1..10 and 21 receive one synapse each, of the projections' types in order
+    post = [int(np.flatnonzero(np.diff(m.comp_syn_begin))[k]) for k in range(m.n_synapses)]
+    assert post == [2, 3, 4, 5, 6, 8, 9, 10, 11, 22]
+    assert list(m.syn_type) == [0, 1, 2, 3, 4, 5, 6, 7, 8, 0]
+    assert m.syn_weight[-1] == np.float32(1.7) and m.syn_delay[-1] == 5000.0
+    (tr, ec, er, _), (st, sy) = oracle.run_with_synapse_log(m)
+    assert sorted(set(sy.tolist())) == list(range(10))       # every synapse received events
+    # every postsynaptic cell left rest
+    cols = [c[2] for c in outs.trajectories[0].columns][1:12]
+    assert np.all(np.abs(tr[:, cols] - (-65.0)).max(axis=0) > 0.05)

@@ -10,8 +10,8 @@ command-line front end, on the CPU.
 * Every synthetic workload written as LEMS by the exporter reads back into the same
   instantiated model, array for array (spike recipients per source as a set: their order
   does not change results, flags being idempotent).
-* The reference's EdenTest_CoreSynapses model, whose synapse types lie outside the per-step
-  loop's mechanism set, is refused with an error naming the type.
+* The reference's full EdenTest_CoreSynapses model, which holds constructs outside the
+  per-step loop's mechanism set, is refused with an error naming the first of them.
 """
 import dataclasses
 import os
@@ -98,7 +98,9 @@ def test_exported_workloads_read_back_to_the_same_model(tmp_path, which):
 
 
 def test_reference_core_synapses_model_is_refused_by_name():
-    with pytest.raises(neuroml.NeuroMLError, match="alphaCurrentSynapse"):
+    """The full model holds a blocking/plastic synapse, graded and gap-junction projections,
+    which the per-step loop does not cover: refused, naming the first of them."""
+    with pytest.raises(neuroml.NeuroMLError, match="blockingPlasticSynapse"):
         neuroml.read(os.path.join(GOLD, "LEMS_EdenTest_CoreSynapses.xml"))
 
 
