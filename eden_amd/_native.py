@@ -105,6 +105,10 @@ EXPORTS = {
     "edenhip_adj_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int64)]),
     "edenhip_adj_forward": (C.c_int, [C.c_void_p, _P, _P, _P]),
+    "edenhip_adj_forward_w": (C.c_int, [C.c_void_p, _P, _P, _P, _P]),
+    "edenhip_adj_backward_w": (C.c_int, [C.c_void_p, _P, _P, _P, _P]),
+    "edenhip_adj_set_events": (C.c_int, [C.c_void_p, _P, _P, C.c_int64]),
+    "edenhip_adj_n_synapses": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "edenhip_adj_backward": (C.c_int, [C.c_void_p, _P, _P, _P]),
     "edenhip_adj_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "edenhip_adj_last_error": (C.c_char_p, [C.c_void_p]),

@@ -1,9 +1,11 @@
 """Gradients of voltage-trace losses through the discrete adjoint (include/edenhip_adjoint.h).
 
-`Adjoint` wraps one differentiable model on the GPU; `trace(adj, gbase, amplitude)` is the
-differentiable simulation for PyTorch: it returns the recorded voltage trace (rows x
+`Adjoint` wraps one differentiable model on the GPU; `trace(adj, gbase, amplitude, weight)`
+is the differentiable simulation for PyTorch: it returns the recorded voltage trace (rows x
 recorded compartments, float64, mV) and its backward pass runs the hand-written adjoint
-kernels, giving dLoss/dGbase per channel distribution and dLoss/dAmplitude per pulse input.
+kernels, giving dLoss/dGbase per channel distribution, dLoss/dAmplitude per pulse input and
+dLoss/dWeight per synapse instance (with the synaptic event schedule fixed:
+`Adjoint.set_events`).
 
 The differentiated map is the reference's fixed-step update evaluated in double precision
 (as the FP64 restatement in oracle/ computes it); the fp32 path of eden_amd.Simulation is
@@ -36,9 +38,11 @@ class Adjoint:
             self.lib.edenhip_adj_destroy(self.h)
             self.h = None
             raise EdenHipError(f"adjoint create failed ({rc}): {msg}")
-        rows, nrec, nch, npu = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        rows, nrec, nch, npu, nsy = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         self.lib.edenhip_adj_sizes(self.h, C.byref(rows), C.byref(nrec), C.byref(nch), C.byref(npu))
+        self.lib.edenhip_adj_n_synapses(self.h, C.byref(nsy))
         self.rows, self.n_record, self.n_chan, self.n_pulse = rows.value, nrec.value, nch.value, npu.value
+        self.n_syn = nsy.value
         self.device = torch.device("cuda", device)
 
     def _check(self, rc, what):
@@ -49,6 +53,17 @@ class Adjoint:
         g = torch.tensor(self.model.chan_gbase, dtype=torch.float64, device=self.device)
         a = torch.tensor(self.model.pulse_amplitude, dtype=torch.float64, device=self.device)
         return g, a
+
+    def default_weights(self):
+        return torch.tensor(self.model.syn_weight, dtype=torch.float64, device=self.device)
+
+    def set_events(self, steps, synapses):
+        """The synapses' fixed event schedule: (step, synapse instance) pairs, steps from 1,
+        as Simulation.read_synapse_events returns them."""
+        import numpy as np
+        st = np.ascontiguousarray(steps, np.int64)
+        sy = np.ascontiguousarray(synapses, np.int64)
+        self._check(self.lib.edenhip_adj_set_events(self.h, N.ptr(st), N.ptr(sy), len(st)), "adjoint set_events")
 
     @staticmethod
     def _ptr(t):
@@ -61,14 +76,15 @@ class Adjoint:
             raise ValueError(f"{name}: expected {n} float64 values on {self.device}")
         return t.contiguous()
 
-    def forward(self, gbase=None, amplitude=None):
+    def forward(self, gbase=None, amplitude=None, weight=None):
         gbase = self._param(gbase, self.n_chan, "gbase")
         amplitude = self._param(amplitude, self.n_pulse, "amplitude")
+        weight = self._param(weight, self.n_syn, "weight")
         out = torch.empty((self.rows, self.n_record), dtype=torch.float64, device=self.device)
         torch.cuda.synchronize(self.device)
-        self._check(self.lib.edenhip_adj_forward(self.h, self._ptr(gbase), self._ptr(amplitude), self._ptr(out)),
-                    "adjoint forward")
-        self._last = (gbase, amplitude)   # keep the parameters alive for the backward pass
+        self._check(self.lib.edenhip_adj_forward_w(self.h, self._ptr(gbase), self._ptr(amplitude), self._ptr(weight),
+                                                   self._ptr(out)), "adjoint forward")
+        self._last = (gbase, amplitude, weight)   # keep the parameters alive for the backward pass
         return out
 
     def backward(self, grad_trace):
@@ -77,10 +93,11 @@ class Adjoint:
             raise ValueError(f"grad_trace: expected shape {(self.rows, self.n_record)}")
         gg = torch.empty(max(1, self.n_chan), dtype=torch.float64, device=self.device)
         ga = torch.empty(max(1, self.n_pulse), dtype=torch.float64, device=self.device)
+        gw = torch.empty(max(1, self.n_syn), dtype=torch.float64, device=self.device)
         torch.cuda.synchronize(self.device)
-        self._check(self.lib.edenhip_adj_backward(self.h, self._ptr(g), self._ptr(gg), self._ptr(ga)),
+        self._check(self.lib.edenhip_adj_backward_w(self.h, self._ptr(g), self._ptr(gg), self._ptr(ga), self._ptr(gw)),
                     "adjoint backward")
-        return gg[: self.n_chan], ga[: self.n_pulse]
+        return gg[: self.n_chan], ga[: self.n_pulse], gw[: self.n_syn]
 
     def timing(self):
         f, b = C.c_double(), C.c_double()
@@ -103,20 +120,23 @@ class Adjoint:
 
 
 class TraceFunction(torch.autograd.Function):
-    """trace = simulate(gbase, amplitude); backward through the adjoint kernels."""
+    """trace = simulate(gbase, amplitude, weight); backward through the adjoint kernels."""
 
     @staticmethod
-    def forward(ctx, gbase, amplitude, adj):
+    def forward(ctx, gbase, amplitude, weight, adj):
         ctx.adj = adj
-        return adj.forward(gbase.detach(), amplitude.detach())
+        return adj.forward(gbase.detach(), amplitude.detach(), weight.detach())
 
     @staticmethod
     def backward(ctx, grad_trace):
-        gg, ga = ctx.adj.backward(grad_trace)
-        return gg, ga, None
+        gg, ga, gw = ctx.adj.backward(grad_trace)
+        return gg, ga, gw, None
 
 
-def trace(adj, gbase, amplitude):
-    """Differentiable voltage trace of `adj`'s model for parameters gbase[n_chan] and
-    amplitude[n_pulse] (float64 tensors on the device)."""
-    return TraceFunction.apply(gbase, amplitude, adj)
+def trace(adj, gbase, amplitude, weight=None):
+    """Differentiable voltage trace of `adj`'s model for parameters gbase[n_chan],
+    amplitude[n_pulse] and (when the model has synapses) weight[n_syn] (float64 tensors on
+    the device; weight None = the model's connection weights, not differentiated)."""
+    if weight is None:
+        weight = adj.default_weights()
+    return TraceFunction.apply(gbase, amplitude, weight, adj)

@@ -49,6 +49,13 @@ struct AdjCell {
     int64_t comp_begin;
     int32_t n, rounds_fwd, rounds_bwd, words_off, words_len;
     int32_t chan_begin, n_chan, gate_begin, n_gate, pulse_begin, n_pulse;
+    int32_t syn_begin, n_syn;
+};
+
+// expTwoSynapse type in double: parameters and the waveform factor (a derived parameter,
+// evaluated with the reference's expression tree)
+struct AdjSynType {
+    double gbase, erev, tau_rise, tau_decay, weight_prop, wf;
 };
 
 struct AdjTables {
@@ -73,6 +80,14 @@ struct AdjTables {
     const double2 *u;
     const uint32_t *words;
     const double *t_row;
+    // chemical synapses (expTwo) with a fixed event schedule: per compartment CSR of
+    // synapse instances, per instance its type, per instance the sorted steps at which it
+    // receives its event
+    const int32_t *s_begin;
+    const int32_t *s_type;
+    const AdjSynType *s_types;
+    const int32_t *ev_begin;
+    const int64_t *ev_step;
 };
 
 constexpr double K1E5 = (double)1e-5f;           // the reference's 1e-5f literal, widened
@@ -155,10 +170,11 @@ __device__ __forceinline__ double gate64(const AdjGate &G, double V, double q, b
 
 // LDS of one cell, in doubles
 struct AdjLds {
-    int32_t v, q, f, lf, d, x, u, lv, lq, gv, gg, ga, w, total;
+    int32_t v, q, f, lf, d, x, u, lv, lq, gv, gg, ga, sa, sb, sc, gw, w, total;
 };
 
-__host__ __device__ inline AdjLds adj_lds(int32_t nmax, int32_t nqmax, int32_t nchmax, int32_t npmax, int32_t wmax) {
+__host__ __device__ inline AdjLds adj_lds(int32_t nmax, int32_t nqmax, int32_t nchmax, int32_t npmax, int32_t wmax,
+                                          int32_t nsmax) {
     AdjLds L;
     L.v = 0;
     L.q = L.v + nmax;
@@ -172,7 +188,11 @@ __host__ __device__ inline AdjLds adj_lds(int32_t nmax, int32_t nqmax, int32_t n
     L.gv = L.lq + 2 * nqmax;
     L.gg = L.gv + nqmax;
     L.ga = L.gg + nchmax;
-    L.w = L.ga + npmax;
+    L.sa = L.ga + npmax;           // synapse state A, B, event cursor (as an index), weight gradient
+    L.sb = L.sa + nsmax;
+    L.sc = L.sb + nsmax;
+    L.gw = L.sc + nsmax;
+    L.w = L.gw + nsmax;
     L.total = L.w + (wmax + 1) / 2;
     return L;
 }
@@ -182,7 +202,8 @@ struct CellView {
     const AdjTables &T;
     const AdjCell &cell;
     int tid, wbase;
-    double *sV, *sQ, *sF, *sLF, *sD, *sX, *sLV, *sLQ, *sGV, *sGg, *sGa;
+    double *sV, *sQ, *sF, *sLF, *sD, *sX, *sLV, *sLQ, *sGV, *sGg, *sGa, *sA, *sB, *sGw;
+    int64_t *sCur;
     double2 *sU;
     const uint32_t *sW;
 
@@ -244,9 +265,59 @@ struct CellView {
         }
     }
 
+    // synaptic current and conductance of compartment i from the present synapse state:
+    // per instance g = gbase (B - A), i = g (erev - V), times the connection weight, summed per
+    // type in list order, the types' sums in order (as the oracle's FP64 flavour)
+    __device__ void synapse_sums(int i, double V, const double *weight, double &Is, double &Gs) const {
+        const int64_t g = cell.comp_begin + i;
+        Is = 0.0;
+        Gs = 0.0;
+        double Ia = 0.0, Ga = 0.0;
+        int32_t cur = -1;
+        for (int sy = T.s_begin[g]; sy < T.s_begin[g + 1]; sy++) {
+            const int32_t ty = T.s_type[sy];
+            if (ty != cur) {
+                if (cur >= 0) { Is += Ia; Gs += Ga; }
+                Ia = 0.0; Ga = 0.0; cur = ty;
+            }
+            const AdjSynType st = T.s_types[ty];
+            const int ls = sy - cell.syn_begin;
+            const double gs = st.gbase * (sB[ls] - sA[ls]);
+            const double is = gs * (st.erev - V);
+            Ia += is * weight[sy];
+            Ga += gs * weight[sy];
+        }
+        if (cur >= 0) { Is += Ia; Gs += Ga; }
+    }
+
+    // next synapse state (in place) at step s: Euler decay, or on the instance's event the
+    // OnEvent assignment; zero on initialisation steps
+    __device__ void synapse_update(int64_t s, double dt) {
+        for (int ls = tid; ls < cell.n_syn; ls += TPB) {
+            const int sy = cell.syn_begin + ls;
+            const AdjSynType st = T.s_types[T.s_type[sy]];
+            const double A = sA[ls], B = sB[ls];
+            double An, Bn;
+            if (s <= 0) {
+                An = 0.0; Bn = 0.0;
+            } else {
+                An = A + dt * ((-A) / st.tau_rise);
+                Bn = B + dt * ((-B) / st.tau_decay);
+                int64_t c = sCur[ls];
+                if (c < T.ev_begin[sy + 1] && T.ev_step[c] == s) {
+                    An = A + (st.weight_prop * st.wf);
+                    Bn = B + (st.weight_prop * st.wf);
+                    sCur[ls] = c + 1;
+                }
+            }
+            sA[ls] = An;
+            sB[ls] = Bn;
+        }
+    }
+
     // membrane currents of compartment i (local) at time t: conductance and current sums
     __device__ void currents(int i, double V, double t, const double *gbase, const double *amp, double &I,
-                             double &G) const {
+                             double &G, const double *weight) const {
         const int64_t g = cell.comp_begin + i;
         const double A = T.A[g];
         double Ich = 0.0, Gch = 0.0;
@@ -265,12 +336,15 @@ struct CellView {
             Iin += Ip;
             Iin += 0.0;
         }
-        I = Ich + Iin + 0.0;
-        G = Gch + 0.0 + 0.0;
+        double Isyn = 0.0, Gsyn = 0.0;
+        if (cell.n_syn > 0) synapse_sums(i, V, weight, Isyn, Gsyn);
+        I = Ich + Iin + Isyn;
+        G = Gch + 0.0 + Gsyn;
     }
 
     // one forward step s from the state in sV / sQ[cur] into sV / sQ[1 - cur]
-    __device__ void step(int64_t s, double t, int cur, int nqmax, const double *gbase, const double *amp) {
+    __device__ void step(int64_t s, double t, int cur, int nqmax, const double *gbase, const double *amp,
+                         const double *weight) {
         const bool init = s <= 0;
         const double dt = T.dt;
         const int N = cell.n;
@@ -280,7 +354,7 @@ struct CellView {
             const int64_t g = cell.comp_begin + i;
             const double V = sV[i];
             double I, G;
-            currents(i, V, t, gbase, amp, I, G);
+            currents(i, V, t, gbase, amp, I, G, weight);
             const double C = T.C[g];
             const double D = 1 * (G / C) + T.invrc[g];
             const int p = T.pos[g];
@@ -300,6 +374,8 @@ struct CellView {
             sX[N] = 0.0;
         }
         bar<TPB>();
+        // the synapses' next state (read above for this step's currents)
+        if (cell.n_syn > 0) synapse_update(s, dt);
         if (!init) hines(false);
         for (int i = tid; i < N; i += TPB) sV[i] = sX[T.pos[cell.comp_begin + i]];
         bar<TPB>();
@@ -324,6 +400,16 @@ __device__ CellView<TPB> make_view(const AdjTables &T, const AdjCell &cell, unsi
     v.sGV = sm + L.gv;
     v.sGg = sm + L.gg;
     v.sGa = sm + L.ga;
+    v.sA = sm + L.sa;
+    v.sB = sm + L.sb;
+    v.sCur = reinterpret_cast<int64_t *>(sm + L.sc);
+    v.sGw = sm + L.gw;
+    for (int ls = tid; ls < cell.n_syn; ls += TPB) {
+        v.sA[ls] = 0.0;
+        v.sB[ls] = 0.0;
+        v.sCur[ls] = T.ev_begin[cell.syn_begin + ls];
+        v.sGw[ls] = 0.0;
+    }
     uint32_t *w = reinterpret_cast<uint32_t *>(sm + L.w);
     for (int i = tid; i < cell.words_len; i += TPB) w[i] = T.words[cell.words_off + i];
     for (int i = tid; i < cell.n; i += TPB) v.sU[i] = T.u[cell.comp_begin + i];
@@ -346,7 +432,8 @@ __global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells,
                                                    double *__restrict__ ck, int64_t ck_stride, int64_t nc_total,
                                                    int32_t K, AdjLds L, int32_t nqmax, double *__restrict__ hist,
                                                    int64_t hist_stride, int64_t n_seg,
-                                                   const int64_t *__restrict__ seg_off) {
+                                                   const int64_t *__restrict__ seg_off,
+                                                   const double *__restrict__ weight) {
     extern __shared__ __align__(16) unsigned char smem[];
     const AdjCell cell = T.cells[blockIdx.x];
     auto v = make_view<TPB>(T, cell, smem, L, nqmax);
@@ -368,12 +455,17 @@ __global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells,
                 if (rc >= 0) trace[(s - 1) * T.n_record + rc] = v.sV[i];
             }
             if (hist) {
-                const int64_t per = (int64_t)cell.n + cell.n_gate;
+                const int64_t per = (int64_t)cell.n + cell.n_gate + 2 * cell.n_syn;
                 const int64_t k = (s - 1) / K, j = (s - 1) - k * K;
                 auto put = [&](int64_t seg, int64_t idx) {
                     double *h = hist + seg * hist_stride + seg_off[blockIdx.x] + idx * per;
                     for (int i = tid; i < cell.n; i += TPB) h[i] = v.sV[i];
                     for (int q = tid; q < cell.n_gate; q += TPB) h[cell.n + q] = v.sQ[cur * nqmax + q];
+                    double *hs = h + cell.n + cell.n_gate;
+                    for (int ls = tid; ls < cell.n_syn; ls += TPB) {
+                        hs[ls] = v.sA[ls];
+                        hs[cell.n_syn + ls] = v.sB[ls];
+                    }
                 };
                 // the state of the last step opens no segment of its own when (rows - 1) is a
                 // multiple of K: it is only the closing state of the segment before
@@ -382,7 +474,7 @@ __global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells,
             }
         }
         if (s == T.rows) break;
-        v.step(s, t, cur, nqmax, gbase, amp);
+        v.step(s, t, cur, nqmax, gbase, amp, weight);
         cur ^= 1;
     }
 }
@@ -397,13 +489,14 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
                                                     const double *__restrict__ ck, int64_t nc_total, int64_t a,
                                                     int64_t b, double *__restrict__ seg, const int64_t *__restrict__ seg_off,
                                                     double *__restrict__ lam, double *__restrict__ grad_g,
-                                                    double *__restrict__ grad_a, AdjLds L, int32_t nqmax, int32_t stored) {
+                                                    double *__restrict__ grad_a, AdjLds L, int32_t nqmax, int32_t stored,
+                                                    const double *__restrict__ weight, double *__restrict__ grad_w) {
     extern __shared__ __align__(16) unsigned char smem[];
     const AdjCell cell = T.cells[blockIdx.x];
     auto v = make_view<TPB>(T, cell, smem, L, nqmax);
     const int tid = threadIdx.x;
-    const int N = cell.n, NG = cell.n_gate;
-    const int64_t per = (int64_t)N + NG;
+    const int N = cell.n, NG = cell.n_gate, NS = cell.n_syn;
+    const int64_t per = (int64_t)N + NG + 2 * NS;
     double *S = seg + seg_off[blockIdx.x];
     const double dt = T.dt;
     for (int c = tid; c < cell.n_chan; c += TPB) v.sGg[c] = 0.0;
@@ -419,7 +512,7 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
             for (int i = tid; i < N; i += TPB) Ss[i] = v.sV[i];
             for (int k = tid; k < NG; k += TPB) Ss[N + k] = v.sQ[cur * nqmax + k];
             if (s == b) break;
-            v.step(s, T.t_row[s - 1], cur, nqmax, gbase, amp);
+            v.step(s, T.t_row[s - 1], cur, nqmax, gbase, amp, weight);
             cur ^= 1;
         }
     }
@@ -442,6 +535,10 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
         // state of step s into LDS; open fractions
         for (int i = tid; i < N; i += TPB) v.sV[i] = Ss[i];
         for (int k = tid; k < NG; k += TPB) v.sQ[k] = Ss[N + k];
+        for (int ls = tid; ls < NS; ls += TPB) {
+            v.sA[ls] = Ss[N + NG + ls];
+            v.sB[ls] = Ss[N + NG + NS + ls];
+        }
         bar<TPB>();
         v.fopen(0, nqmax);
         bar<TPB>();
@@ -449,7 +546,7 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
         for (int i = tid; i < N; i += TPB) {
             const int64_t g = cell.comp_begin + i;
             double I, G;
-            v.currents(i, v.sV[i], t, gbase, amp, I, G);
+            v.currents(i, v.sV[i], t, gbase, amp, I, G, weight);
             const double D = 1 * (G / T.C[g]) + T.invrc[g];
             const int p = T.pos[g];
             v.sD[p] = 1 + D * dt;
@@ -478,6 +575,14 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
             for (int p = T.p_begin[g]; p < T.p_begin[g + 1]; p++)
                 if (T.p_start[p] <= t && t <= T.p_start[p] + T.p_dur[p])
                     v.sGa[p - cell.pulse_begin] += lb * dt / C * T.p_weight[p];
+            // synapse instances: a conductance w g_s(t) into reversal erev, like a channel with
+            // no area or density factor; g_s does not depend on V or on the parameters
+            for (int sy = T.s_begin[g]; sy < T.s_begin[g + 1]; sy++) {
+                const AdjSynType st = T.s_types[T.s_type[sy]];
+                const int ls = sy - cell.syn_begin;
+                const double gs = st.gbase * (v.sB[ls] - v.sA[ls]);
+                v.sGw[ls] += dt / C * (lb * st.erev + lD) * gs;
+            }
             v.sLV[i] = lb;
         }
         bar<TPB>();
@@ -526,6 +631,7 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
     for (int k = tid; k < NG; k += TPB) lam[nc_total + cell.gate_begin + k] = v.sLQ[k];
     for (int c = tid; c < cell.n_chan; c += TPB) grad_g[cell.chan_begin + c] += v.sGg[c];
     for (int p = tid; p < cell.n_pulse; p += TPB) grad_a[cell.pulse_begin + p] += v.sGa[p];
+    for (int ls = tid; ls < NS; ls += TPB) grad_w[cell.syn_begin + ls] += v.sGw[ls];
 }
 
 template <int TPB>
@@ -544,13 +650,17 @@ struct edenhip_adj {
     hipStream_t stream = nullptr;
     int tpb = 64;
     int32_t K = 500;
-    int64_t rows = 0, n_record = 0, NC = 0, NQ = 0, NCH = 0, NP = 0;
+    int64_t rows = 0, n_record = 0, NC = 0, NQ = 0, NCH = 0, NP = 0, NSYN = 0;
     int32_t n_cells = 0, nqmax = 0;
     AdjTables T{};
     AdjLds L{};
     size_t lds = 0;
     std::vector<void *> allocs;
-    double *d_v0 = nullptr, *d_q0 = nullptr, *d_gbase0 = nullptr, *d_amp0 = nullptr;
+    double *d_v0 = nullptr, *d_q0 = nullptr, *d_gbase0 = nullptr, *d_amp0 = nullptr, *d_w0 = nullptr;
+    int32_t *d_ev_begin = nullptr;
+    int64_t *d_ev_step = nullptr;
+    std::vector<void *> ev_allocs;        // the event schedule's tables (replaced by set_events)
+    const double *last_w = nullptr;
     double *d_ck = nullptr, *d_seg = nullptr, *d_lam = nullptr;
     double *d_hist = nullptr;          // state history (store mode), n_seg segments
     int64_t hist_stride = 0, n_seg = 0;
@@ -620,11 +730,18 @@ const void *bwd_kernel(int tpb) {
 }
 
 int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
-    if (d->abi_version != EDENHIP_ABI_VERSION) return afail(a, EDENHIP_E_INVALID, "descriptor ABI version mismatch");
+    if (d->abi_version != 1 && d->abi_version != EDENHIP_ABI_VERSION)
+        return afail(a, EDENHIP_E_INVALID, "descriptor ABI version mismatch");
     if (d->n_cells <= 0 || d->n_comps <= 0 || !(d->dt > 0.0f)) return afail(a, EDENHIP_E_INVALID, "empty model or dt <= 0");
     const int64_t NC = d->n_comps;
-    if (d->comp_syn_begin[NC] != 0 || d->comp_spike_begin[NC] != 0 || d->comp_poisson_begin[NC] != 0)
-        return afail(a, EDENHIP_E_INVALID, "the adjoint supports channels and pulse inputs only (no synapses, spike recipients or Poisson inputs)");
+    if (d->comp_poisson_begin[NC] != 0)
+        return afail(a, EDENHIP_E_INVALID, "the adjoint supports channels, pulse inputs and expTwo synapses (no Poisson inputs)");
+    if (d->abi_version >= 2 && d->lems && d->lems->n_types > 0)
+        return afail(a, EDENHIP_E_INVALID, "the adjoint supports expTwo synapses only (no generic LEMS synapse types)");
+    a->NSYN = d->comp_syn_begin[NC];
+    if (a->NSYN >= ((int64_t)1 << 31)) return afail(a, EDENHIP_E_LIMIT, "more than 2^31 synapses");
+    for (int64_t sy = 0; sy < a->NSYN; sy++)
+        if (d->syn_type[sy] < 0 || d->syn_type[sy] >= d->n_syn_types) return afail(a, EDENHIP_E_INVALID, "synapse type index out of range");
     a->NC = NC;
     a->NCH = d->comp_chan_begin[NC];
     a->NP = d->comp_pulse_begin[NC];
@@ -685,6 +802,22 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         if (rec[g] >= 0) return afail(a, EDENHIP_E_INVALID, "a compartment may be recorded once in the adjoint model");
         rec[g] = (int32_t)r;
     }
+    // synapses: per compartment CSR, types in double with the waveform factor evaluated in
+    // double with the reference's expression tree (as the oracle's FP64 flavour per step)
+    std::vector<int32_t> sb(NC + 1), sty(std::max<int64_t>(a->NSYN, 1), 0);
+    for (int64_t g = 0; g <= NC; g++) sb[g] = (int32_t)d->comp_syn_begin[g];
+    for (int64_t sy = 0; sy < a->NSYN; sy++) sty[sy] = d->syn_type[sy];
+    std::vector<AdjSynType> stypes(std::max(1, d->n_syn_types));
+    for (int t = 0; t < d->n_syn_types; t++) {
+        const edenhip_exptwo_type &e = d->syn_types[t];
+        const double tr = (double)e.tau_rise, td = (double)e.tau_decay;
+        const double peak = (std::log(td / tr) * (tr * td)) / (td - tr);
+        const double wf = (double)1 / ((-std::exp((-peak) / tr)) + std::exp((-peak) / td));
+        stypes[t] = AdjSynType{(double)e.gbase, (double)e.erev, tr, td, (double)e.weight_prop, wf};
+    }
+    std::vector<double> w0(std::max<int64_t>(a->NSYN, 1), 0.0);
+    for (int64_t sy = 0; sy < a->NSYN; sy++) w0[sy] = (double)d->syn_weight[sy];
+    int32_t nsmax = 1;
     std::vector<AdjCell> cells(d->n_cells);
     std::vector<uint32_t> words;
     std::vector<int64_t> seg_off(d->n_cells);
@@ -757,16 +890,19 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         cell.n_gate = (int32_t)(q0.size() - (size_t)cell.gate_begin);
         cell.pulse_begin = (int32_t)d->comp_pulse_begin[c0];
         cell.n_pulse = (int32_t)(d->comp_pulse_begin[c1] - d->comp_pulse_begin[c0]);
+        cell.syn_begin = (int32_t)d->comp_syn_begin[c0];
+        cell.n_syn = (int32_t)(d->comp_syn_begin[c1] - d->comp_syn_begin[c0]);
+        nsmax = std::max(nsmax, cell.n_syn);
         nqmax = std::max(nqmax, cell.n_gate);
         nchmax = std::max(nchmax, cell.n_chan);
         npmax = std::max(npmax, cell.n_pulse);
         cells[c] = cell;
         seg_off[c] = seg_total;
-        seg_total += ((int64_t)n + cell.n_gate);
+        seg_total += ((int64_t)n + cell.n_gate + 2 * (int64_t)cell.n_syn);
     }
     a->NQ = (int64_t)q0.size();
     a->nqmax = nqmax;
-    a->L = adj_lds(nmax, nqmax, nchmax, npmax, wmax);
+    a->L = adj_lds(nmax, nqmax, nchmax, npmax, wmax, nsmax);
     a->lds = (size_t)a->L.total * sizeof(double);
     if (a->lds > 160 * 1024) return afail(a, EDENHIP_E_LIMIT, "cell too large for the adjoint kernels' LDS");
     a->K = std::max(1, a->K);
@@ -798,6 +934,15 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
     ATRY(aupload(a, (double2 **)&T.u, u));
     ATRY(aupload(a, (uint32_t **)&T.words, words));
     ATRY(aupload(a, (double **)&T.t_row, trow));
+    ATRY(aupload(a, (int32_t **)&T.s_begin, sb));
+    ATRY(aupload(a, (int32_t **)&T.s_type, sty));
+    ATRY(aupload(a, (AdjSynType **)&T.s_types, stypes));
+    ATRY(aupload(a, &a->d_w0, w0));
+    // no events until edenhip_adj_set_events
+    ATRY(aupload(a, &a->d_ev_begin, std::vector<int32_t>((size_t)a->NSYN + 1, 0)));
+    ATRY(aupload(a, &a->d_ev_step, std::vector<int64_t>(1, 0)));
+    T.ev_begin = a->d_ev_begin;
+    T.ev_step = a->d_ev_step;
     ATRY(aupload(a, &a->d_v0, v0));
     ATRY(aupload(a, &a->d_q0, q0));
     ATRY(aupload(a, &a->d_gbase0, gb0));
@@ -853,6 +998,48 @@ int edenhip_adj_create(const edenhip_model_desc *desc, const edenhip_adj_options
     return build_adj(a, desc);
 }
 
+int edenhip_adj_set_events(edenhip_adj *a, const int64_t *step, const int64_t *syn, int64_t n) {
+    if (!a || n < 0 || (n > 0 && (!step || !syn))) return EDENHIP_E_INVALID;
+    if (!a->stream) return afail(a, EDENHIP_E_STATE, "adjoint context was not created successfully");
+    if (n >= ((int64_t)1 << 31)) return afail(a, EDENHIP_E_LIMIT, "more than 2^31 - 1 events");
+    AHIP(a, hipSetDevice(a->device));
+    // per synapse, its event steps in ascending order
+    std::vector<int32_t> begin((size_t)a->NSYN + 1, 0);
+    for (int64_t k = 0; k < n; k++) {
+        if (syn[k] < 0 || syn[k] >= a->NSYN) return afail(a, EDENHIP_E_INVALID, "event synapse index out of range");
+        if (step[k] < 1) return afail(a, EDENHIP_E_INVALID, "event steps count from 1");
+        begin[(size_t)syn[k] + 1]++;
+    }
+    for (int64_t sy = 0; sy < a->NSYN; sy++) begin[(size_t)sy + 1] += begin[(size_t)sy];
+    std::vector<int64_t> steps((size_t)std::max<int64_t>(n, 1), 0);
+    std::vector<int32_t> fill(begin.begin(), begin.end() - 1);
+    for (int64_t k = 0; k < n; k++) steps[(size_t)fill[(size_t)syn[k]]++] = step[k];
+    for (int64_t sy = 0; sy < a->NSYN; sy++) {
+        std::sort(steps.begin() + begin[(size_t)sy], steps.begin() + begin[(size_t)sy + 1]);
+        for (int32_t k = begin[(size_t)sy] + 1; k < begin[(size_t)sy + 1]; k++)
+            if (steps[(size_t)k] == steps[(size_t)k - 1]) return afail(a, EDENHIP_E_INVALID, "two events of one synapse at one step");
+    }
+    for (void *p : a->ev_allocs) (void)hipFree(p);
+    a->ev_allocs.clear();
+    void *pb = nullptr, *ps = nullptr;
+    AHIP(a, hipMalloc(&pb, begin.size() * sizeof(int32_t)));
+    a->ev_allocs.push_back(pb);
+    AHIP(a, hipMalloc(&ps, steps.size() * sizeof(int64_t)));
+    a->ev_allocs.push_back(ps);
+    AHIP(a, hipMemcpy(pb, begin.data(), begin.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    AHIP(a, hipMemcpy(ps, steps.data(), steps.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    a->T.ev_begin = (const int32_t *)pb;
+    a->T.ev_step = (const int64_t *)ps;
+    a->have_forward = false;
+    return 0;
+}
+
+int edenhip_adj_n_synapses(const edenhip_adj *a, int64_t *n_syn) {
+    if (!a || !n_syn) return EDENHIP_E_INVALID;
+    *n_syn = a->NSYN;
+    return 0;
+}
+
 int edenhip_adj_sizes(const edenhip_adj *a, int64_t *rows, int64_t *n_record, int64_t *n_chan, int64_t *n_pulse) {
     if (!a) return EDENHIP_E_INVALID;
     if (rows) *rows = a->rows;
@@ -862,12 +1049,15 @@ int edenhip_adj_sizes(const edenhip_adj *a, int64_t *rows, int64_t *n_record, in
     return 0;
 }
 
-int edenhip_adj_forward(edenhip_adj *a, const double *gbase, const double *amp, double *trace) {
+int edenhip_adj_forward_w(edenhip_adj *a, const double *gbase, const double *amp, const double *weight, double *trace) {
     if (!a || !trace) return EDENHIP_E_INVALID;
     if (!a->stream) return afail(a, EDENHIP_E_STATE, "adjoint context was not created successfully");
     AHIP(a, hipSetDevice(a->device));
     const double *gb = gbase ? gbase : a->d_gbase0;
     const double *am = amp ? amp : a->d_amp0;
+    const double *wt = weight ? weight : a->d_w0;
+    if (a->NSYN > 0 && !a->d_hist)
+        return afail(a, EDENHIP_E_STATE, "synapses need the state history in device memory (store_states)");
     hipEvent_t e0, e1;
     AHIP(a, hipEventCreate(&e0));
     AHIP(a, hipEventCreate(&e1));
@@ -878,7 +1068,7 @@ int edenhip_adj_forward(edenhip_adj *a, const double *gbase, const double *amp, 
     AdjLds L = a->L;
     int64_t hs = a->hist_stride, ns = a->n_seg;
     void *args[] = {&T, &ncell, &a->d_v0, &a->d_q0, (void *)&gb, (void *)&am, &trace, &a->d_ck, &cks, &nct, &K, &L, &nq,
-                    &a->d_hist, &hs, &ns, &a->d_seg_off};
+                    &a->d_hist, &hs, &ns, &a->d_seg_off, (void *)&wt};
     AHIP(a, hipLaunchKernel(fwd_kernel(a->tpb), dim3(a->n_cells), dim3(a->tpb), args, a->lds, a->stream));
     AHIP(a, hipEventRecord(e1, a->stream));
     AHIP(a, hipEventSynchronize(e1));
@@ -889,17 +1079,27 @@ int edenhip_adj_forward(edenhip_adj *a, const double *gbase, const double *amp, 
     a->fwd_ms = ms;
     a->last_gbase = gb;
     a->last_amp = am;
+    a->last_w = wt;
     a->have_forward = true;
     return 0;
 }
 
-int edenhip_adj_backward(edenhip_adj *a, const double *grad_trace, double *grad_gbase, double *grad_amp) {
+int edenhip_adj_forward(edenhip_adj *a, const double *gbase, const double *amp, double *trace) {
+    return edenhip_adj_forward_w(a, gbase, amp, nullptr, trace);
+}
+
+int edenhip_adj_backward_w(edenhip_adj *a, const double *grad_trace, double *grad_gbase, double *grad_amp,
+                           double *grad_weight) {
     if (!a || !grad_trace) return EDENHIP_E_INVALID;
     if (!a->stream) return afail(a, EDENHIP_E_STATE, "adjoint context was not created successfully");
     if (!a->have_forward) return afail(a, EDENHIP_E_STATE, "backward before forward");
     AHIP(a, hipSetDevice(a->device));
-    double *gg = grad_gbase, *ga = grad_amp;
-    void *tmp_g = nullptr, *tmp_a = nullptr;
+    double *gg = grad_gbase, *ga = grad_amp, *gw = grad_weight;
+    void *tmp_g = nullptr, *tmp_a = nullptr, *tmp_w = nullptr;
+    if (!gw) {
+        AHIP(a, hipMalloc(&tmp_w, (size_t)std::max<int64_t>(a->NSYN, 1) * sizeof(double)));
+        gw = (double *)tmp_w;
+    }
     if (!gg) {
         AHIP(a, hipMalloc(&tmp_g, (size_t)std::max<int64_t>(a->NCH, 1) * sizeof(double)));
         gg = (double *)tmp_g;
@@ -914,6 +1114,7 @@ int edenhip_adj_backward(edenhip_adj *a, const double *grad_trace, double *grad_
     AHIP(a, hipEventRecord(e0, a->stream));
     AHIP(a, hipMemsetAsync(gg, 0, (size_t)std::max<int64_t>(a->NCH, 1) * sizeof(double), a->stream));
     AHIP(a, hipMemsetAsync(ga, 0, (size_t)std::max<int64_t>(a->NP, 1) * sizeof(double), a->stream));
+    AHIP(a, hipMemsetAsync(gw, 0, (size_t)std::max<int64_t>(a->NSYN, 1) * sizeof(double), a->stream));
     AHIP(a, hipMemsetAsync(a->d_lam, 0, (size_t)a->ck_stride * sizeof(double), a->stream));
     AdjTables T = a->T;
     int32_t nq = a->nqmax;
@@ -928,7 +1129,7 @@ int edenhip_adj_backward(edenhip_adj *a, const double *grad_trace, double *grad_
         int32_t stored = a->d_hist ? 1 : 0;
         double *segp = a->d_hist ? a->d_hist + k * a->hist_stride : a->d_seg;
         void *args[] = {&T, (void *)&a->last_gbase, (void *)&a->last_amp, (void *)&grad_trace, (void *)&ckp, &nct, &sa, &sb,
-                        &segp, &a->d_seg_off, &a->d_lam, &gg, &ga, &L, &nq, &stored};
+                        &segp, &a->d_seg_off, &a->d_lam, &gg, &ga, &L, &nq, &stored, (void *)&a->last_w, &gw};
         AHIP(a, hipLaunchKernel(bwd_kernel(a->tpb), dim3(a->n_cells), dim3(a->tpb), args, a->lds, a->stream));
     }
     AHIP(a, hipEventRecord(e1, a->stream));
@@ -940,7 +1141,12 @@ int edenhip_adj_backward(edenhip_adj *a, const double *grad_trace, double *grad_
     a->bwd_ms = ms;
     if (tmp_g) (void)hipFree(tmp_g);
     if (tmp_a) (void)hipFree(tmp_a);
+    if (tmp_w) (void)hipFree(tmp_w);
     return 0;
+}
+
+int edenhip_adj_backward(edenhip_adj *a, const double *grad_trace, double *grad_gbase, double *grad_amp) {
+    return edenhip_adj_backward_w(a, grad_trace, grad_gbase, grad_amp, nullptr);
 }
 
 int edenhip_adj_timing(const edenhip_adj *a, double *forward_ms, double *backward_ms) {
@@ -961,6 +1167,7 @@ void edenhip_adj_destroy(edenhip_adj *a) {
         (void)hipSetDevice(a->device);
         (void)hipStreamSynchronize(a->stream);
         for (void *p : a->allocs) (void)hipFree(p);
+        for (void *p : a->ev_allocs) (void)hipFree(p);
         (void)hipStreamDestroy(a->stream);
     }
     delete a;

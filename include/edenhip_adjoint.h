@@ -1,5 +1,6 @@
-/* edenhip_adjoint.h -- discrete adjoint of the fixed-step integrator for cells without
- * synapses (HH-style channels, passive leak, pulse inputs, tree cables).
+/* edenhip_adjoint.h -- discrete adjoint of the fixed-step integrator: HH-style channels,
+ * passive leak, pulse inputs, expTwo chemical synapses with a fixed event schedule, tree
+ * cables.
  *
  * No counterpart in the reference (EDEN has no gradients). The differentiated map is the
  * per-step update of eden/Eden.cpp:5236-5259 (gates), 5805-5898 (channel currents), 3798-3826
@@ -12,8 +13,18 @@
  * store_states), the full state of every step; the backward pass takes dLoss/dtrace and,
  * segment by segment from the last, takes the segment's states from that history or
  * recomputes them from the segment's checkpoint, and runs the adjoint recursion backwards, accumulating dLoss/dGbase per channel
- * distribution and dLoss/dAmplitude per pulse input. Gates clipped to [1e-6, 1 - 1e-6] in a
- * step pass no gradient through that step's update.
+ * distribution, dLoss/dAmplitude per pulse input and dLoss/dWeight per synapse instance. Gates
+ * clipped to [1e-6, 1 - 1e-6] in a step pass no gradient through that step's update.
+ *
+ * Synapses: threshold crossings are fixed events (no gradient through spike timing). The
+ * steps at which each synapse instance receives its event are given by the caller
+ * (edenhip_adj_set_events, e.g. the schedule edenhip_read_synapse_events logs for a run of
+ * the same model); each cell then evolves independently. An instance's conductance
+ * g(t) = gbase (B - A) follows its expTwo dynamics (eden/Eden.cpp:2614-2994) and does not
+ * depend on V or on the weights; the connection weight w multiplies its current and
+ * conductance (eden/Eden.cpp:3710), so dLoss/dw = sum over steps of g dt / C
+ * (lambda_b erev + lambda_D), as for a channel. Models with synapses need the state history
+ * kept in device memory (store_states).
  *
  * Pointers marked (device) are device memory of the context's GPU; calls synchronise their
  * work before returning. */
@@ -44,6 +55,14 @@ typedef struct edenhip_adj_options {
  * The descriptor's chan_gbase and pulse_amplitude are the default parameter values. */
 int edenhip_adj_create(const edenhip_model_desc *desc, const edenhip_adj_options *opt, edenhip_adj **out);
 
+/* The fixed event schedule of the synapses: n pairs (step, synapse instance), steps counting
+ * from 1 as edenhip_read_synapse_events reports them (the step whose time window holds the
+ * arrival). Replaces the previous schedule; none until called. */
+int edenhip_adj_set_events(edenhip_adj *adj, const int64_t *step, const int64_t *syn, int64_t n);
+
+/* Synapse instances of the model (the length of the weight vectors). */
+int edenhip_adj_n_synapses(const edenhip_adj *adj, int64_t *n_syn);
+
 /* Sizes: rows of the trace, recorded columns, channel distributions, pulse inputs. */
 int edenhip_adj_sizes(const edenhip_adj *adj, int64_t *rows, int64_t *n_record, int64_t *n_chan, int64_t *n_pulse);
 
@@ -52,10 +71,19 @@ int edenhip_adj_sizes(const edenhip_adj *adj, int64_t *rows, int64_t *n_record, 
  * at time r * dt in mV) and keeps the checkpoints for edenhip_adj_backward. */
 int edenhip_adj_forward(edenhip_adj *adj, const double *gbase, const double *amplitude, double *trace);
 
+/* Forward pass with connection weights weight[n_syn] too (device, double; NULL = the
+ * descriptor's syn_weight). */
+int edenhip_adj_forward_w(edenhip_adj *adj, const double *gbase, const double *amplitude, const double *weight,
+                          double *trace);
+
 /* Backward pass for the parameters of the last forward pass: grad_trace[rows x n_record]
  * (device, double) is dLoss/dtrace; writes grad_gbase[n_chan] and grad_amplitude[n_pulse]
  * (device, double; either may be NULL). */
 int edenhip_adj_backward(edenhip_adj *adj, const double *grad_trace, double *grad_gbase, double *grad_amplitude);
+
+/* Backward pass with dLoss/dWeight too: grad_weight[n_syn] (device, double; may be NULL). */
+int edenhip_adj_backward_w(edenhip_adj *adj, const double *grad_trace, double *grad_gbase, double *grad_amplitude,
+                           double *grad_weight);
 
 /* Milliseconds of device time of the last forward and backward passes (HIP events). */
 int edenhip_adj_timing(const edenhip_adj *adj, double *forward_ms, double *backward_ms);

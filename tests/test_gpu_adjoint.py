@@ -190,3 +190,126 @@ def test_config5_full_size_forward_and_gradient():
             fd = (L(g0, ap) - L(g0, am)) / float(ap[p] - am[p])
             _say(f"[adjoint] config 5 amplitude[{p}]: adjoint {ga[p]:.9g}  fd {fd:.9g}")
             assert abs(ga[p] - fd) <= 1e-4 * abs(fd)
+
+
+# --- synaptic weights (expTwo synapses, fixed event schedule) ---------------------------------
+
+def _net_model():
+    """A small network of tree and one-compartment HH cells joined by excitatory and
+    inhibitory expTwo synapses (40 ms; the cells spike and the synapses receive events)."""
+    m = configs.mixed_network(tree_sizes=(20, 30), n_point=4, n_conn=30, t_final_ms=40.0, seed=3)
+    # the adjoint records a compartment once: keep the first of repeated recording columns
+    _, first = np.unique(m.record_comp, return_index=True)
+    m.record_comp = np.asarray(m.record_comp)[np.sort(first)]
+    return m
+
+
+def _schedule(m, weight=None):
+    """The oracle's FP64 run: its synaptic event schedule (step, synapse) and its FP64 trace."""
+    base = m.syn_weight
+    if weight is not None:
+        m.syn_weight = weight
+    try:
+        _, (st, sy) = oracle.run_with_synapse_log(m, precision="f64")
+        tr = oracle.run_f64_trace(m)
+    finally:
+        m.syn_weight = base
+    return np.asarray(st), np.asarray(sy), tr
+
+
+def test_adjoint_synapses_forward_matches_oracle_f64():
+    """With the oracle's own event schedule, the adjoint's FP64 forward of a network
+    reproduces the oracle's FP64 network run, every cell's trace."""
+    m = _net_model()
+    st, sy, ref = _schedule(m)
+    _say(f"[adjoint] network forward: cells={m.n_cells} comps={m.n_comps} synapses={len(m.syn_weight)} "
+         f"events={len(st)} rows={m.total_rows()}")
+    assert len(st) > 10 and len(np.unique(sy)) > 5
+    with Adjoint(m, checkpoint_steps=100) as adj:
+        assert adj.n_syn == len(m.syn_weight)
+        adj.set_events(st, sy)
+        tr = adj.forward().cpu().numpy()
+        # without the schedule the synapses stay silent and the trace differs
+        adj.set_events([], [])
+        quiet = adj.forward().cpu().numpy()
+    err = float(np.max(np.abs(tr - ref)))
+    _say(f"[adjoint] network: max |V_gpu64 - V_oracle64| = {err:.3g} mV; without events {np.abs(quiet - ref).max():.3g} mV")
+    assert err < 1e-8
+    assert np.abs(quiet - ref).max() > 1.0
+
+
+@pytest.mark.parametrize("tpb", [0, 128])
+def test_adjoint_weight_gradient_matches_central_fd(tpb):
+    """dLoss/dWeight (and dLoss/dGbase with synapses present) against central differences of
+    the GPU's FP64 forward with the same schedule, h = 1e-6 relative: 1e-4 relative."""
+    m = _net_model()
+    st, sy, _ = _schedule(m)
+    w0 = np.asarray(m.syn_weight, np.float64)
+    with Adjoint(m, checkpoint_steps=37, threads_per_cell=tpb) as adj:
+        adj.set_events(st, sy)
+        g0, a0 = adj.default_parameters()
+        wt = torch.tensor(w0 * np.linspace(0.6, 1.4, len(w0)), dtype=torch.float64, device=DEV)
+        target = adj.forward(g0, a0, wt)
+        w = torch.tensor(w0, dtype=torch.float64, device=DEV, requires_grad=True)
+        g = g0.clone().requires_grad_(True)
+        L = _loss(trace(adj, g, a0, w), target)
+        L.backward()
+        gw = w.grad.detach().cpu().numpy()
+        gg = g.grad.detach().cpu().numpy()
+
+        def Lw(wv, gv=g0):
+            return float(_loss(adj.forward(gv, a0, torch.tensor(wv, dtype=torch.float64, device=DEV)), target))
+
+        active = np.unique(sy)
+        assert np.all(gw[np.setdiff1d(np.arange(len(w0)), active)] == 0.0)   # no event, no gradient
+        rng = np.random.default_rng(2)
+        big = np.argsort(-np.abs(gw))[:5]
+        pick = np.unique(np.concatenate([big, rng.choice(active, 5, replace=False)]))
+        scale = float(np.abs(gw).max())
+        assert scale > 0.0
+        for i in pick:
+            wp, wm = w0.copy(), w0.copy()
+            wp[i] += 1e-6 * abs(w0[i])
+            wm[i] -= 1e-6 * abs(w0[i])
+            fd = (Lw(wp) - Lw(wm)) / (wp[i] - wm[i])
+            _say(f"[adjoint] weight[{i}] = {w0[i]:.4g}: adjoint {gw[i]:.9g}  fd {fd:.9g}")
+            assert abs(gw[i] - fd) <= 1e-4 * abs(fd) + 1e-9 * scale
+        gn = g0.cpu().numpy()
+        for i in np.argsort(-np.abs(gg))[:3]:
+            gp, gm = gn.copy(), gn.copy()
+            gp[i] += 1e-6 * abs(gn[i])
+            gm[i] -= 1e-6 * abs(gn[i])
+            fd = (Lw(w0, torch.tensor(gp, device=DEV)) - Lw(w0, torch.tensor(gm, device=DEV))) / (gp[i] - gm[i])
+            _say(f"[adjoint] gbase[{i}] with synapses: adjoint {gg[i]:.9g}  fd {fd:.9g}")
+            assert abs(gg[i] - fd) <= 1e-4 * abs(fd)
+
+
+def test_adjoint_weight_gradient_matches_fd_of_the_oracle_network():
+    """Central differences of the oracle's FP64 network run (fp32 weights, steps of 1e-4
+    relative) where the perturbed runs keep the unperturbed event schedule: there the fixed
+    schedule is the network's own, and the adjoint gradient is the network's gradient."""
+    m = _net_model()
+    st, sy, ref = _schedule(m)
+    w0 = np.asarray(m.syn_weight, np.float32)
+    target = ref * 0.9
+    with Adjoint(m) as adj:
+        adj.set_events(st, sy)
+        g0, a0 = adj.default_parameters()
+        w = torch.tensor(w0.astype(np.float64), dtype=torch.float64, device=DEV, requires_grad=True)
+        _loss(trace(adj, g0, a0, w), torch.tensor(target, device=DEV)).backward()
+        gw = w.grad.detach().cpu().numpy()
+    checked = 0
+    for i in np.argsort(-np.abs(gw))[:4]:
+        wp, wm = w0.copy(), w0.copy()
+        wp[i] = np.float32(w0[i] * (1 + 1e-4))
+        wm[i] = np.float32(w0[i] * (1 - 1e-4))
+        sp, yp, tp = _schedule(m, wp)
+        sm, ym, tm = _schedule(m, wm)
+        same = all(np.array_equal(a, b) for a, b in ((sp, st), (yp, sy), (sm, st), (ym, sy)))
+        fd = (float(((tp - target) ** 2).sum() / tp.size) - float(((tm - target) ** 2).sum() / tm.size)) / (
+            float(wp[i]) - float(wm[i]))
+        _say(f"[adjoint] oracle fd weight[{i}]: adjoint {gw[i]:.9g}  fd {fd:.9g}  schedule kept: {same}")
+        if same:
+            assert abs(gw[i] - fd) <= 1e-3 * abs(fd)
+            checked += 1
+    assert checked >= 2
