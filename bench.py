@@ -221,7 +221,8 @@ def adjoint_bench(reps=2):
     grad = float(np.min(tot))
     return dict(workload="256 random trees x 100 compartments, HH on every compartment, 20,001 rows, "
                          "Gbase per channel distribution trainable (76,800 parameters), MSE on somatic V, "
-                         "every step's state kept in device memory (checkpoints every 500 steps as fallback)",
+                         "every step's state and Jacobian factors kept in device memory (checkpoints every 500 steps "
+                         "as fallback)",
                 # headline: one gradient (FP64 forward + backward) against one fp32 forward pass of
                 # the product path; target <= 3
                 gradient_over_forward_fp32=grad / f32, target=3.0,

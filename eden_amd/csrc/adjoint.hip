@@ -5,22 +5,28 @@
 //   gates          eden/Eden.cpp:5236-5259, 5336-5375 (rates, tau, inf, cnexp, clipping)
 //   channels       eden/Eden.cpp:5679-5721, 5805-5898 (fopen, Gbase*fopen*(E-V)*1e-5f*A)
 //   pulses         eden/Eden.cpp:3798-3826 (inclusive window, Imax * weight)
+//   synapses       eden/Eden.cpp:2614-2994, 3640-3643, 3710 (expTwo, weight times i and g)
 //   membrane       eden/Eden.cpp:6057-6107 (D = G/C + invRC, V' = V + dt(I + V G)/C)
 //   Hines          eden/Eden.cpp:6114-6225 (D = 1 + D dt, elimination in DFS order)
 // The Hines solve runs in the breadth-first rounds of tree_schedule.hpp, which apply each
-// node's operations in the reference's order.
+// node's operations in the reference's order; one wave runs all rounds, and a node's
+// eliminated diagonal is kept as its reciprocal (ratio = Uj * (1/D) rather than Uj / D).
 //
-// One workgroup per cell. adj_forward runs the whole time loop, writes the trace and a
-// checkpoint of (V, q) at the start of every K-th recorded step; when the state history fits
-// in device memory it also writes (V, q) of every step, in the segment layout the backward
-// pass reads. adj_backward handles one segment of K steps: it takes the segment's states from
-// that history, or recomputes them from the segment's checkpoint into a segment buffer, then
-// walks the steps backwards:
-//   lambda_b = M^-T lambda_V'   (Hines on the transposed matrix: Ui and Uj swap roles)
+// One workgroup per cell. A kernel first copies its cell's static records (compartments,
+// channels, gates, pulses, synapse types, schedule) and parameters into LDS. adj_forward runs
+// the whole time loop, writes the trace and a checkpoint of (V, q) at the start of every K-th
+// recorded step; when the history fits in device memory it also writes a record of every
+// step: its starting state and its Jacobian factors (the reciprocal eliminated diagonals and
+// each gate's dq'/dq and dq'/dV), in the segment layout the backward pass reads.
+// adj_backward handles one segment of K steps: it takes the segment's records from that
+// history, or recomputes them from the segment's checkpoint, then walks the steps backwards:
+//   lambda_b = M^-T lambda_V'   (Hines on the transposed matrix: Ui and Uj swap roles; the
+//                                stored reciprocal diagonals serve both matrices)
 //   lambda_D = -lambda_b V'
 //   per channel c of compartment i:  dL/dGscaled = dt/C * 1e-5f * A * (lambda_b E_c + lambda_D)
 //       dL/dGbase += dL/dGscaled * fopen,   dL/dfopen = dL/dGscaled * Gbase
 //   per pulse p active at t:          dL/dAmplitude += lambda_b dt / C * weight
+//   per synapse instance:             dL/dw += dt / C * (lambda_b erev + lambda_D) * g_s
 //   per gate g (unclipped update):    dq'/dq = blend,  dq'/dV = (q - inf) dblend/dV + (1 - blend) dinf/dV
 //   lambda_V = lambda_b + sum_g lambda_q'_g dq'_g/dV + dLoss/dV,  lambda_q = blend lambda_q' + dfopen/dq lambda_fopen
 // (b = V + dt(I + V G)/C has db/dV = 1: the semi-implicit form's V terms cancel.)
@@ -58,37 +64,78 @@ struct AdjSynType {
     double gbase, erev, tau_rise, tau_decay, weight_prop, wf;
 };
 
+// Static records of a cell, indexed locally within the cell. A kernel copies its cell's
+// records into LDS when it starts, so the step loops read no table from global memory.
+struct CompRec {
+    float A, C, invrc;      // the descriptor's fp32 values, widened (exactly) where used
+    int32_t rec_col;        // trace column, or -1
+    uint16_t pos;           // breadth-first position in the Hines schedule
+    uint16_t leaf;          // 1: no children (its reciprocal diagonal is formed in the compartment phase)
+    uint16_t ch_b, ch_e;    // channel distributions [ch_b, ch_e)
+    uint16_t p_b, p_e;      // pulse inputs
+    uint16_t s_b, s_e;      // synapse instances
+};
+static_assert(sizeof(CompRec) == 32, "CompRec is four doubles");
+
+struct ChanRec {
+    float erev;
+    uint16_t comp, q0;      // compartment, first gate variable
+    uint32_t shape;         // gate count in bits 0-2, instances of gate k in bits 3 + 6k .. 8 + 6k
+    uint32_t pad;
+};
+static_assert(sizeof(ChanRec) == 16, "ChanRec is two doubles");
+
+struct GateRec {
+    uint16_t comp, chan;    // compartment whose V drives the gate, the gate's channel
+    uint16_t param, k;      // gate table entry (channel type * 4 + k), gate index within the channel
+};
+static_assert(sizeof(GateRec) == 8, "GateRec is one double");
+
+struct PulseRec {
+    float start, dur, weight, pad;
+};
+
 struct AdjTables {
     double dt;
     int64_t rows;
-    int32_t n_record;
+    int32_t n_record, n_gate_params, n_syn_types;
     const AdjCell *cells;
-    // per compartment
-    const double *C, *A, *invrc;
-    const int32_t *pos, *ch_begin, *p_begin, *rec_col;
-    // per channel distribution
-    const int32_t *ch_type, *ch_gate, *ch_comp;
-    const double *ch_erev;
-    // per gate variable
-    const int32_t *g_chan, *g_k;
-    // channel types: gate count, gate table (4 per type)
-    const int32_t *type_ngates;
-    const AdjGate *gates;
-    // pulses
-    const double *p_start, *p_dur, *p_weight;
+    const CompRec *comp;     // by compartment
+    const ChanRec *chan;     // by channel distribution
+    const GateRec *gate;     // by gate variable
+    const PulseRec *pulse;   // by pulse input
+    const AdjGate *gates;    // gate table, 4 entries per channel type
+    const int32_t *s_type;   // by synapse instance
+    const AdjSynType *s_types;
     // edge constants (Ui, Uj) by global position, schedule blocks, row times
     const double2 *u;
     const uint32_t *words;
     const double *t_row;
-    // chemical synapses (expTwo) with a fixed event schedule: per compartment CSR of
-    // synapse instances, per instance its type, per instance the sorted steps at which it
-    // receives its event
-    const int32_t *s_begin;
-    const int32_t *s_type;
-    const AdjSynType *s_types;
+    // the synapses' fixed event schedule: per instance its event steps, ascending
     const int32_t *ev_begin;
     const int64_t *ev_step;
 };
+
+// One step's record of a cell (history or segment buffer): the state at the start of the
+// step, V[n] q[ng] A[ns] B[ns], then the step's Jacobian factors: the reciprocal eliminated
+// Hines diagonal by position inv[n + 1] (the last entry, the dummy node's, is 1), and per
+// gate variable dq'/dq and dq'/dV. The backward pass copies a record into LDS as it is and
+// reads its fields in place.
+struct RecLayout {
+    int32_t q, a, b, inv, dq, dv, per;
+};
+
+__host__ __device__ inline RecLayout rec_layout(int32_t n, int32_t ng, int32_t ns) {
+    RecLayout R;
+    R.q = n;
+    R.a = R.q + ng;
+    R.b = R.a + ns;
+    R.inv = R.b + ns;
+    R.dq = R.inv + n + 1;
+    R.dv = R.dq + ng;
+    R.per = R.dv + ng;
+    return R;
+}
 
 constexpr double K1E5 = (double)1e-5f;           // the reference's 1e-5f literal, widened
 constexpr double QLO = (double)(1e-6f);          // clipping bounds as the FP64 restatement writes them
@@ -104,38 +151,38 @@ __device__ __forceinline__ void bar() {
     }
 }
 
-// rate value and its derivative in V
-__device__ __forceinline__ double rate64(int form, double r, double m, double s, double V, double &dv) {
-    switch (form) {
-    case EDENHIP_RATE_EXP: {
-        const double v = r * exp((V - m) / s);
-        dv = v / s;
-        return v;
-    }
-    case EDENHIP_RATE_EXPLINEAR: {
-        const double x = (V - m) / s;
-        if (V == m) {
-            dv = r / (2.0 * s);
-            return r;
-        }
-        const double em = exp(-(V - m) / s);
-        const double v = r * (x / (1.0 - em));
+// rounds of one wave: its LDS accesses execute in program order, so only the compiler has
+// to be kept from moving them across a round boundary
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// rate value and, when asked, its derivative in V. One exponential per rate whatever the
+// form: exp(x) for EXP, exp(-x) for EXPLINEAR and SIGMOID, x = (V - m) / s ((m - V) / s and
+// -(V - m) / s equal -x exactly), and the forms' remaining operations as selects, so the
+// lanes of a wave holding gates of different forms do not run every form's branch. Each
+// form's value is the same operations as its textbook expression.
+__device__ __forceinline__ double rate64(int form, double r, double m, double s, double V, bool deriv, double &dv) {
+    const bool is_exp = form == EDENHIP_RATE_EXP, is_lin = form == EDENHIP_RATE_EXPLINEAR;
+    const bool is_sig = form == EDENHIP_RATE_SIGMOID, is_fix = !(is_exp || is_lin || is_sig);
+    // EXPLINEAR at V == m is its limit r (derivative r / 2s); selected, not branched
+    const bool at_mid = is_lin && V == m;
+    const double x = is_fix ? 0.0 : (V - m) / s;
+    const double e = is_fix ? 0.0 : exp(is_exp ? x : -x);
+    const double den = is_lin ? 1.0 - e : is_sig ? 1.0 + e : 1.0;
+    const double t = (is_exp ? e : is_lin ? x : r) / den;
+    const double v = at_mid ? r : (is_exp || is_lin) ? r * t : t;   // r e, r (x / (1 - e^-x)), r / (1 + e), r
+    if (deriv) {
         // d/dx x/(1 - e^-x) = ((1 - e^-x) - x e^-x) / (1 - e^-x)^2, with its series near 0
-        const double den = 1.0 - em;
-        const double gx = fabs(x) < 1e-5 ? 0.5 + x / 6.0 : ((den - x * em) / (den * den));
-        dv = r * gx / s;
-        return v;
-    }
-    case EDENHIP_RATE_SIGMOID: {
-        const double e = exp((m - V) / s);
-        const double v = r / (1.0 + e);
-        dv = r * e / (s * (1.0 + e) * (1.0 + e));
-        return v;
-    }
-    default:
+        const double gl = fabs(x) < 1e-5 ? 0.5 + x * (1.0 / 6.0) : ((den - x * e) / (den * den));
+        const double dnum = is_exp ? v : is_lin ? r * gl : is_sig ? r * e : 0.0;
+        const double dden = is_sig ? s * (1.0 + e) * (1.0 + e) : is_fix ? 1.0 : s;
+        dv = at_mid ? r / (2.0 * s) : dnum / dden;
+    } else {
         dv = 0.0;
-        return r;
     }
+    return v;
 }
 
 // next gate value (the reference's order, widened), and, when asked, the partial
@@ -143,8 +190,9 @@ __device__ __forceinline__ double rate64(int form, double r, double m, double s,
 __device__ __forceinline__ double gate64(const AdjGate &G, double V, double q, bool init, double dt, double *dq,
                                          double *dV) {
     double da, db;
-    const double alpha = rate64(G.alpha_form, G.ar, G.am, G.as, V, da);
-    const double beta = rate64(G.beta_form, G.br, G.bm, G.bs, V, db);
+    const bool deriv = dq != nullptr;
+    const double alpha = rate64(G.alpha_form, G.ar, G.am, G.as, V, deriv, da);
+    const double beta = rate64(G.beta_form, G.br, G.bm, G.bs, V, deriv, db);
     const double tau = 1.0 / (alpha + beta);
     const double inf = alpha / (alpha + beta);
     if (init) {
@@ -170,30 +218,45 @@ __device__ __forceinline__ double gate64(const AdjGate &G, double V, double q, b
 
 // LDS of one cell, in doubles
 struct AdjLds {
-    int32_t v, q, f, lf, d, x, u, lv, lq, gv, gg, ga, sa, sb, sc, gw, w, total;
+    int32_t v, vn, q, f, lf, x, inv, u, lv, lq, gv, gg, ga, gw, sa, sb, sc, sn, gb, amp, wt, stage, comp, chan, gate,
+        pulse, stype, gtab, sttab, w, total;
 };
 
 __host__ __device__ inline AdjLds adj_lds(int32_t nmax, int32_t nqmax, int32_t nchmax, int32_t npmax, int32_t wmax,
-                                          int32_t nsmax) {
+                                          int32_t nsmax, int32_t n_gate_params, int32_t n_syn_types) {
     AdjLds L;
-    L.v = 0;
-    L.q = L.v + nmax;
-    L.f = L.q + 2 * nqmax;
-    L.lf = L.f + nchmax;
-    L.d = L.lf + nchmax;
-    L.x = L.d + (nmax + 1);
-    L.u = L.x + (nmax + 1);
-    L.lv = L.u + 2 * (nmax + 1);
-    L.lq = L.lv + nmax;
-    L.gv = L.lq + 2 * nqmax;
-    L.gg = L.gv + nqmax;
-    L.ga = L.gg + nchmax;
-    L.sa = L.ga + npmax;           // synapse state A, B, event cursor (as an index), weight gradient
-    L.sb = L.sa + nsmax;
-    L.sc = L.sb + nsmax;
-    L.gw = L.sc + nsmax;
-    L.w = L.gw + nsmax;
-    L.total = L.w + (wmax + 1) / 2;
+    int32_t o = 0;
+    L.u = o;     o += 2 * (nmax + 1);      // double2, first so it stays 16-byte aligned
+    L.v = o;     o += nmax;
+    L.vn = o;    o += nmax;                // V at the start of the next step (backward)
+    L.q = o;     o += 2 * nqmax;           // gate values, two buffers
+    L.f = o;     o += nchmax;              // open fractions
+    L.lf = o;    o += nchmax;              // dLoss/dfopen
+    L.x = o;     o += nmax + 1;            // Hines right-hand side / solution, by position
+    L.inv = o;   o += nmax + 1;            // diagonal, then its reciprocal after elimination, by position
+    L.lv = o;    o += nmax;
+    L.lq = o;    o += 2 * nqmax;
+    L.gv = o;    o += nqmax;
+    L.gg = o;    o += nchmax;              // gradient accumulators: Gbase, amplitude, weight
+    L.ga = o;    o += npmax;
+    L.gw = o;    o += nsmax;
+    L.sa = o;    o += nsmax;               // synapse state A, B
+    L.sb = o;    o += nsmax;
+    L.sc = o;    o += nsmax;               // event cursor and the step of the next event (int64)
+    L.sn = o;    o += nsmax;
+    L.gb = o;    o += nchmax;              // parameters: Gbase, amplitude, weight
+    L.amp = o;   o += npmax;
+    L.wt = o;    o += nsmax;
+    L.stage = o; o += rec_layout(nmax, nqmax, nsmax).per;   // the step's record (backward)
+    L.comp = o;  o += 4 * nmax;            // static records
+    L.chan = o;  o += 2 * nchmax;
+    L.gate = o;  o += nqmax;
+    L.pulse = o; o += 2 * npmax;
+    L.stype = o; o += (nsmax + 1) / 2;
+    L.gtab = o;  o += 8 * n_gate_params;
+    L.sttab = o; o += 6 * n_syn_types;
+    L.w = o;     o += (wmax + 1) / 2;
+    L.total = o;
     return L;
 }
 
@@ -201,91 +264,147 @@ template <int TPB>
 struct CellView {
     const AdjTables &T;
     const AdjCell &cell;
-    int tid, wbase;
-    double *sV, *sQ, *sF, *sLF, *sD, *sX, *sLV, *sLQ, *sGV, *sGg, *sGa, *sA, *sB, *sGw;
-    int64_t *sCur;
+    int tid, nqmax;
+    double *sV, *sVn, *sQ, *sF, *sLF, *sX, *sInv, *sLV, *sLQ, *sGV, *sGg, *sGa, *sGw, *sA, *sB, *sGb, *sAmp, *sWt,
+        *sDQ, *sDV, *sStage;
+    int64_t *sCur, *sNext;
     double2 *sU;
-    const uint32_t *sW;
+    CompRec *cr;
+    ChanRec *chr;
+    GateRec *gr;
+    PulseRec *pr;
+    int32_t *sty;
+    AdjGate *gtab;
+    AdjSynType *sttab;
+    uint32_t *sW;
 
-    // Hines solve of (D, X) in place; transposed: the matrix with Ui and Uj exchanged
-    __device__ void hines(bool transposed) {
-        const int N = cell.n;
-        const int RF = cell.rounds_fwd, RB = cell.rounds_bwd;
-        const uint32_t idle_fwd = (uint32_t)N | ((uint32_t)N << 11) | (1u << 22);
-        const uint32_t idle_bwd = (uint32_t)N | ((uint32_t)N << 11);
-        for (int r = 0; r < RF; r++) {
-            const uint32_t h = sW[r];
-            const int cnt = (int)(__builtin_amdgcn_readfirstlane(h) >> 16);
-            if (wbase < cnt) {
-                const uint32_t w = tid < cnt ? sW[(h & 0xFFFFu) + tid] : idle_fwd;
+    // The schedule block (tree_schedule.hpp, rounds of at most 64 tasks) is read two rounds
+    // ahead: while round r runs, a lane reads its task word of round r + 1 (from that round's
+    // header, read during round r - 1) and the header of round r + 2. Lanes beyond a round's
+    // tasks take the idle word, which points every access at the dummy node n.
+    __device__ uint32_t task_word(uint32_t h, uint32_t idle) const {
+        const uint32_t hu = __builtin_amdgcn_readfirstlane(h);
+        return (uint32_t)tid < (hu >> 16) ? sW[(hu & 0xFFFFu) + tid] : idle;
+    }
+
+    // forward Hines solve (wave 0 runs every round; the other waves wait at the closing
+    // barrier): each fold task eliminates its node's children in ascending order with the
+    // children's reciprocal diagonals, then forms its own; back-substitution multiplies by it
+    __device__ void hines_fwd() {
+        if (tid < 64) {
+            const int N = cell.n, RF = cell.rounds_fwd, RB = cell.rounds_bwd;
+            const uint32_t idle_fwd = (uint32_t)N | ((uint32_t)N << 11) | (1u << 22);
+            const uint32_t idle_bwd = (uint32_t)N | ((uint32_t)N << 11);
+            uint32_t w = task_word(sW[0], idle_fwd), hn = sW[1];
+            for (int r = 0; r < RF; r++) {
+                const uint32_t wn = task_word(hn, idle_fwd);
+                const uint32_t hnn = sW[r + 2];
                 const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
-                double dp = sD[p], xp = sX[p];
+                double dp = sInv[p], xp = sX[p];
                 for (int i = 0; i < nc; i++) {
                     const int c = fc + i;
                     const double2 u = sU[c];
-                    const double ui = transposed ? u.y : u.x, uj = transposed ? u.x : u.y;
-                    const double ratio = uj / sD[c];
-                    dp -= ratio * ui;
+                    const double ratio = u.y * sInv[c];
+                    dp -= ratio * u.x;
                     xp -= ratio * sX[c];
                 }
-                sD[p] = dp;
                 sX[p] = xp;
+                sInv[p] = 1.0 / dp;
+                wave_sync();
+                w = wn;
+                hn = hnn;
             }
-            bar<TPB>();
-        }
-        if (tid == 0) sX[0] = sX[0] / sD[0];
-        bar<TPB>();
-        for (int r = 0; r < RB; r++) {
-            const uint32_t h = sW[RF + 1 + r];
-            const int cnt = (int)(__builtin_amdgcn_readfirstlane(h) >> 16);
-            if (wbase < cnt) {
-                const uint32_t w = tid < cnt ? sW[(h & 0xFFFFu) + tid] : idle_bwd;
+            w = task_word(sW[RF + 1], idle_bwd);
+            hn = RB > 0 ? sW[RF + 2] : 0u;
+            if (tid == 0) sX[0] = sX[0] * sInv[0];
+            wave_sync();
+            for (int r = 0; r < RB; r++) {
+                const uint32_t wn = task_word(hn, idle_bwd);
+                const uint32_t hnn = r + 2 <= RB ? sW[RF + 3 + r] : 0u;
                 const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
-                const double2 u = sU[p];
-                const double ui = transposed ? u.y : u.x;
-                sX[p] = (sX[p] - ui * sX[pp]) / sD[p];
+                sX[p] = (sX[p] - sU[p].x * sX[pp]) * sInv[p];
+                wave_sync();
+                w = wn;
+                hn = hnn;
             }
-            bar<TPB>();
         }
+        bar<TPB>();
+    }
+
+    // transposed solve with a step's stored reciprocal diagonals (Ui and Uj exchange roles;
+    // the eliminated diagonals of the transposed matrix are the same)
+    __device__ void hines_T() {
+        if (tid < 64) {
+            const int N = cell.n, RF = cell.rounds_fwd, RB = cell.rounds_bwd;
+            const uint32_t idle_fwd = (uint32_t)N | ((uint32_t)N << 11) | (1u << 22);
+            const uint32_t idle_bwd = (uint32_t)N | ((uint32_t)N << 11);
+            uint32_t w = task_word(sW[0], idle_fwd), hn = sW[1];
+            for (int r = 0; r < RF; r++) {
+                const uint32_t wn = task_word(hn, idle_fwd);
+                const uint32_t hnn = sW[r + 2];
+                const int p = w & 0x7FF, fc = (w >> 11) & 0x7FF, nc = w >> 22;
+                double xp = sX[p];
+                for (int i = 0; i < nc; i++) {
+                    const int c = fc + i;
+                    xp -= (sU[c].x * sInv[c]) * sX[c];
+                }
+                sX[p] = xp;
+                wave_sync();
+                w = wn;
+                hn = hnn;
+            }
+            w = task_word(sW[RF + 1], idle_bwd);
+            hn = RB > 0 ? sW[RF + 2] : 0u;
+            if (tid == 0) sX[0] = sX[0] * sInv[0];
+            wave_sync();
+            for (int r = 0; r < RB; r++) {
+                const uint32_t wn = task_word(hn, idle_bwd);
+                const uint32_t hnn = r + 2 <= RB ? sW[RF + 3 + r] : 0u;
+                const int p = w & 0x7FF, pp = (w >> 11) & 0x7FF;
+                sX[p] = (sX[p] - sU[p].y * sX[pp]) * sInv[p];
+                wave_sync();
+                w = wn;
+                hn = hnn;
+            }
+        }
+        bar<TPB>();
     }
 
     // open fractions of the cell's channels from the gate values in sQ[buf]
-    __device__ void fopen(int buf, int nqmax) {
+    __device__ void fopen(int buf) {
         const double *q = sQ + buf * nqmax;
         for (int c = tid; c < cell.n_chan; c += TPB) {
-            const int ch = cell.chan_begin + c;
-            const int ty = T.ch_type[ch];
-            const int q0 = T.ch_gate[ch] - cell.gate_begin;
+            const ChanRec ch = chr[c];
+            const int ng = (int)(ch.shape & 7u);
             double f = 1.0;
-            for (int k = 0; k < T.type_ngates[ty]; k++) {
-                const double qv = q[q0 + k];
-                for (int i = 0; i < T.gates[ty * 4 + k].instances; i++) f = f * qv;
+            for (int k = 0; k < ng; k++) {
+                const double qv = q[ch.q0 + k];
+                const int inst = (int)((ch.shape >> (3 + 6 * k)) & 63u);
+                for (int i = 0; i < inst; i++) f = f * qv;
             }
             sF[c] = f;
         }
     }
 
-    // synaptic current and conductance of compartment i from the present synapse state:
+    // synaptic current and conductance of a compartment from the present synapse state:
     // per instance g = gbase (B - A), i = g (erev - V), times the connection weight, summed per
     // type in list order, the types' sums in order (as the oracle's FP64 flavour)
-    __device__ void synapse_sums(int i, double V, const double *weight, double &Is, double &Gs) const {
-        const int64_t g = cell.comp_begin + i;
+    __device__ void synapse_sums(const CompRec &c, double V, double &Is, double &Gs) const {
         Is = 0.0;
         Gs = 0.0;
         double Ia = 0.0, Ga = 0.0;
         int32_t cur = -1;
-        for (int sy = T.s_begin[g]; sy < T.s_begin[g + 1]; sy++) {
-            const int32_t ty = T.s_type[sy];
+        for (int ls = c.s_b; ls < c.s_e; ls++) {
+            const int32_t ty = sty[ls];
             if (ty != cur) {
                 if (cur >= 0) { Is += Ia; Gs += Ga; }
                 Ia = 0.0; Ga = 0.0; cur = ty;
             }
-            const AdjSynType st = T.s_types[ty];
-            const int ls = sy - cell.syn_begin;
+            const AdjSynType &st = sttab[ty];
             const double gs = st.gbase * (sB[ls] - sA[ls]);
             const double is = gs * (st.erev - V);
-            Ia += is * weight[sy];
-            Ga += gs * weight[sy];
+            Ia += is * sWt[ls];
+            Ga += gs * sWt[ls];
         }
         if (cur >= 0) { Is += Ia; Gs += Ga; }
     }
@@ -294,8 +413,7 @@ struct CellView {
     // OnEvent assignment; zero on initialisation steps
     __device__ void synapse_update(int64_t s, double dt) {
         for (int ls = tid; ls < cell.n_syn; ls += TPB) {
-            const int sy = cell.syn_begin + ls;
-            const AdjSynType st = T.s_types[T.s_type[sy]];
+            const AdjSynType &st = sttab[sty[ls]];
             const double A = sA[ls], B = sB[ls];
             double An, Bn;
             if (s <= 0) {
@@ -303,11 +421,12 @@ struct CellView {
             } else {
                 An = A + dt * ((-A) / st.tau_rise);
                 Bn = B + dt * ((-B) / st.tau_decay);
-                int64_t c = sCur[ls];
-                if (c < T.ev_begin[sy + 1] && T.ev_step[c] == s) {
+                if (sNext[ls] == s) {
                     An = A + (st.weight_prop * st.wf);
                     Bn = B + (st.weight_prop * st.wf);
-                    sCur[ls] = c + 1;
+                    const int64_t c = sCur[ls] + 1;
+                    sCur[ls] = c;
+                    sNext[ls] = c < T.ev_begin[cell.syn_begin + ls + 1] ? T.ev_step[c] : -1;
                 }
             }
             sA[ls] = An;
@@ -315,135 +434,202 @@ struct CellView {
         }
     }
 
-    // membrane currents of compartment i (local) at time t: conductance and current sums
-    __device__ void currents(int i, double V, double t, const double *gbase, const double *amp, double &I,
-                             double &G, const double *weight) const {
-        const int64_t g = cell.comp_begin + i;
-        const double A = T.A[g];
+    // membrane currents of a compartment at time t: conductance and current sums
+    __device__ void currents(const CompRec &c, double V, double t, double &I, double &G) const {
+        const double A = (double)c.A;
         double Ich = 0.0, Gch = 0.0;
-        for (int ch = T.ch_begin[g]; ch < T.ch_begin[g + 1]; ch++) {
-            const double Gs = gbase[ch] * sF[ch - cell.chan_begin];
-            const double iD = Gs * (T.ch_erev[ch] - V) * K1E5;
+        for (int ch = c.ch_b; ch < c.ch_e; ch++) {
+            const double Gs = sGb[ch] * sF[ch];
+            const double iD = Gs * ((double)chr[ch].erev - V) * K1E5;
             const double gD = (Gs) * K1E5;
             Ich += (iD * A);
             Gch += (gD * A);
         }
         double Iin = 0.0;
-        if (T.p_begin[g + 1] > T.p_begin[g]) {
+        if (c.p_e > c.p_b) {
             double Ip = 0.0;
-            for (int p = T.p_begin[g]; p < T.p_begin[g + 1]; p++)
-                if (T.p_start[p] <= t && t <= T.p_start[p] + T.p_dur[p]) Ip += amp[p] * T.p_weight[p];
+            for (int p = c.p_b; p < c.p_e; p++) {
+                const double ps = (double)pr[p].start, pd = (double)pr[p].dur;
+                if (ps <= t && t <= ps + pd) Ip += sAmp[p] * (double)pr[p].weight;
+            }
             Iin += Ip;
             Iin += 0.0;
         }
         double Isyn = 0.0, Gsyn = 0.0;
-        if (cell.n_syn > 0) synapse_sums(i, V, weight, Isyn, Gsyn);
+        if (c.s_e > c.s_b) synapse_sums(c, V, Isyn, Gsyn);
         I = Ich + Iin + Isyn;
         G = Gch + 0.0 + Gsyn;
     }
 
-    // one forward step s from the state in sV / sQ[cur] into sV / sQ[1 - cur]
-    __device__ void step(int64_t s, double t, int cur, int nqmax, const double *gbase, const double *amp,
-                         const double *weight) {
+    // the state at the start of a step into a record
+    __device__ void put_state(double *rec, int cur) const {
+        const int N = cell.n, NG = cell.n_gate, NS = cell.n_syn;
+        for (int i = tid; i < N; i += TPB) rec[i] = sV[i];
+        for (int k = tid; k < NG; k += TPB) rec[N + k] = sQ[cur * nqmax + k];
+        for (int ls = tid; ls < NS; ls += TPB) {
+            rec[N + NG + ls] = sA[ls];
+            rec[N + NG + NS + ls] = sB[ls];
+        }
+    }
+
+    // one forward step s from the state in sV / sQ[cur] into sV / sQ[1 - cur]; with rec, the
+    // step's Jacobian factors go into the step's record
+    __device__ void step(int64_t s, double t, int cur, double *rec, const RecLayout &R) {
         const bool init = s <= 0;
+        const bool keep = rec != nullptr && !init;
         const double dt = T.dt;
         const int N = cell.n;
-        fopen(cur, nqmax);
+        fopen(cur);
         bar<TPB>();
         for (int i = tid; i < N; i += TPB) {
-            const int64_t g = cell.comp_begin + i;
+            const CompRec c = cr[i];
             const double V = sV[i];
             double I, G;
-            currents(i, V, t, gbase, amp, I, G, weight);
-            const double C = T.C[g];
-            const double D = 1 * (G / C) + T.invrc[g];
-            const int p = T.pos[g];
-            sD[p] = 1 + D * dt;
-            sX[p] = init ? V : V + (dt * (I + ((V * G))) / C);
+            currents(c, V, t, I, G);
+            const double C = (double)c.C;
+            const double D = 1 * (G / C) + (double)c.invrc;
+            const double d = 1 + D * dt;
+            sX[c.pos] = init ? V : V + (dt * (I + ((V * G))) / C);
+            sInv[c.pos] = c.leaf ? 1.0 / d : d;
         }
         const double *q = sQ + cur * nqmax;
         double *qn = sQ + (1 - cur) * nqmax;
-        for (int k = tid; k < cell.n_gate; k += TPB) {
-            const int gq = cell.gate_begin + k;
-            const int ch = T.g_chan[gq];
-            const int ty = T.ch_type[ch];
-            qn[k] = gate64(T.gates[ty * 4 + T.g_k[gq]], sV[T.ch_comp[ch]], q[k], init, dt, nullptr, nullptr);
+        // gates k and k + TPB side by side (independent chains of divisions and exponentials;
+        // a lane without a second gate repeats its first and discards the result)
+        for (int k = tid; k < cell.n_gate; k += 2 * TPB) {
+            const int k2 = k + TPB < cell.n_gate ? k + TPB : k;
+            const GateRec g = gr[k], g2 = gr[k2];
+            if (keep) {
+                double dq, dv, dq2, dv2;
+                const double a = gate64(gtab[g.param], sV[g.comp], q[k], false, dt, &dq, &dv);
+                const double b = gate64(gtab[g2.param], sV[g2.comp], q[k2], false, dt, &dq2, &dv2);
+                qn[k] = a;
+                rec[R.dq + k] = dq;
+                rec[R.dv + k] = dv;
+                if (k2 != k) {
+                    qn[k2] = b;
+                    rec[R.dq + k2] = dq2;
+                    rec[R.dv + k2] = dv2;
+                }
+            } else {
+                const double a = gate64(gtab[g.param], sV[g.comp], q[k], init, dt, nullptr, nullptr);
+                const double b = gate64(gtab[g2.param], sV[g2.comp], q[k2], init, dt, nullptr, nullptr);
+                qn[k] = a;
+                if (k2 != k) qn[k2] = b;
+            }
         }
         if (tid == 0) {
-            sD[N] = 1.0;
             sX[N] = 0.0;
+            sInv[N] = 1.0;
         }
         bar<TPB>();
         // the synapses' next state (read above for this step's currents)
         if (cell.n_syn > 0) synapse_update(s, dt);
-        if (!init) hines(false);
-        for (int i = tid; i < N; i += TPB) sV[i] = sX[T.pos[cell.comp_begin + i]];
+        if (!init) hines_fwd();
+        for (int i = tid; i < N; i += TPB) sV[i] = sX[cr[i].pos];
+        if (keep)
+            for (int p = tid; p <= N; p += TPB) rec[R.inv + p] = p < N ? sInv[p] : 1.0;
         bar<TPB>();
     }
 };
 
+template <typename R>
+__device__ __forceinline__ void copy_recs(R *dst, const R *src, int n, int tid, int tpb) {
+    for (int i = tid; i < n; i += tpb) dst[i] = src[i];
+}
+
+// A cell's view of its LDS: static records, parameters and the synapses' initial state
 template <int TPB>
 __device__ CellView<TPB> make_view(const AdjTables &T, const AdjCell &cell, unsigned char *smem, const AdjLds &L,
-                                   int nqmax) {
+                                   int nqmax, const double *gbase, const double *amp, const double *weight) {
     double *sm = reinterpret_cast<double *>(smem);
     const int tid = threadIdx.x;
-    CellView<TPB> v{T, cell, tid, __builtin_amdgcn_readfirstlane(tid & ~63)};
+    CellView<TPB> v{T, cell, tid, nqmax};
+    v.sU = reinterpret_cast<double2 *>(sm + L.u);
     v.sV = sm + L.v;
+    v.sVn = sm + L.vn;
     v.sQ = sm + L.q;
     v.sF = sm + L.f;
     v.sLF = sm + L.lf;
-    v.sD = sm + L.d;
     v.sX = sm + L.x;
-    v.sU = reinterpret_cast<double2 *>(sm + L.u);
+    v.sInv = sm + L.inv;
     v.sLV = sm + L.lv;
     v.sLQ = sm + L.lq;
     v.sGV = sm + L.gv;
     v.sGg = sm + L.gg;
     v.sGa = sm + L.ga;
+    v.sGw = sm + L.gw;
     v.sA = sm + L.sa;
     v.sB = sm + L.sb;
     v.sCur = reinterpret_cast<int64_t *>(sm + L.sc);
-    v.sGw = sm + L.gw;
+    v.sNext = reinterpret_cast<int64_t *>(sm + L.sn);
+    v.sGb = sm + L.gb;
+    v.sAmp = sm + L.amp;
+    v.sWt = sm + L.wt;
+    v.sStage = sm + L.stage;
+    v.sDQ = nullptr;   // the backward pass points these into its record copy
+    v.sDV = nullptr;
+    v.cr = reinterpret_cast<CompRec *>(sm + L.comp);
+    v.chr = reinterpret_cast<ChanRec *>(sm + L.chan);
+    v.gr = reinterpret_cast<GateRec *>(sm + L.gate);
+    v.pr = reinterpret_cast<PulseRec *>(sm + L.pulse);
+    v.sty = reinterpret_cast<int32_t *>(sm + L.stype);
+    v.gtab = reinterpret_cast<AdjGate *>(sm + L.gtab);
+    v.sttab = reinterpret_cast<AdjSynType *>(sm + L.sttab);
+    v.sW = reinterpret_cast<uint32_t *>(sm + L.w);
+    copy_recs(v.cr, T.comp + cell.comp_begin, cell.n, tid, TPB);
+    copy_recs(v.chr, T.chan + cell.chan_begin, cell.n_chan, tid, TPB);
+    copy_recs(v.gr, T.gate + cell.gate_begin, cell.n_gate, tid, TPB);
+    copy_recs(v.pr, T.pulse + cell.pulse_begin, cell.n_pulse, tid, TPB);
+    copy_recs(v.sty, T.s_type + cell.syn_begin, cell.n_syn, tid, TPB);
+    copy_recs(v.gtab, T.gates, T.n_gate_params, tid, TPB);
+    copy_recs(v.sttab, T.s_types, T.n_syn_types, tid, TPB);
+    copy_recs(v.sW, T.words + cell.words_off, cell.words_len, tid, TPB);
+    copy_recs(v.sU, T.u + cell.comp_begin, cell.n, tid, TPB);
+    if (tid == 0) v.sU[cell.n] = make_double2(0.0, 0.0);
+    copy_recs(v.sGb, gbase + cell.chan_begin, cell.n_chan, tid, TPB);
+    copy_recs(v.sAmp, amp + cell.pulse_begin, cell.n_pulse, tid, TPB);
+    copy_recs(v.sWt, weight + cell.syn_begin, cell.n_syn, tid, TPB);
     for (int ls = tid; ls < cell.n_syn; ls += TPB) {
         v.sA[ls] = 0.0;
         v.sB[ls] = 0.0;
-        v.sCur[ls] = T.ev_begin[cell.syn_begin + ls];
-        v.sGw[ls] = 0.0;
+        const int64_t c = T.ev_begin[cell.syn_begin + ls];
+        v.sCur[ls] = c;
+        v.sNext[ls] = c < T.ev_begin[cell.syn_begin + ls + 1] ? T.ev_step[c] : -1;
     }
-    uint32_t *w = reinterpret_cast<uint32_t *>(sm + L.w);
-    for (int i = tid; i < cell.words_len; i += TPB) w[i] = T.words[cell.words_off + i];
-    for (int i = tid; i < cell.n; i += TPB) v.sU[i] = T.u[cell.comp_begin + i];
-    if (tid == 0) v.sU[cell.n] = make_double2(0.0, 0.0);
-    v.sW = w;
-    (void)nqmax;
     return v;
 }
 
 // Whole forward pass: initial state from v0/q0, trace rows, checkpoints of the state at the
 // start of steps 1, 1 + K, 1 + 2K, ... (ck[k] = V of all compartments then q of all gates).
-// hist (may be null): the state at the start of every step s >= 1, in the backward pass's
-// segment layout: segment k = (s - 1) / K holds steps 1 + k K .. 1 + (k + 1) K (K + 1
-// states; the last one is also the first of segment k + 1), hist_stride doubles apart, cell
-// c at seg_off[c], the state of step s at (s - 1 - k K) * (N + NG) after it.
+// hist (may be null): a record of every step s >= 1 (its starting state and its Jacobian
+// factors), in the backward pass's segment layout: segment k = (s - 1) / K holds steps
+// 1 + k K .. 1 + (k + 1) K (K + 1 records; the last one is also the first of segment k + 1,
+// of which the backward pass reads only the state), hist_stride doubles apart, cell c at
+// seg_off[c], step s's record at (s - 1 - k K) * per after it.
 template <int TPB>
-__global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells, const double *__restrict__ v0,
+__global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, const double *__restrict__ v0,
                                                    const double *__restrict__ q0, const double *__restrict__ gbase,
-                                                   const double *__restrict__ amp, double *__restrict__ trace,
-                                                   double *__restrict__ ck, int64_t ck_stride, int64_t nc_total,
-                                                   int32_t K, AdjLds L, int32_t nqmax, double *__restrict__ hist,
-                                                   int64_t hist_stride, int64_t n_seg,
-                                                   const int64_t *__restrict__ seg_off,
-                                                   const double *__restrict__ weight) {
+                                                   const double *__restrict__ amp, const double *__restrict__ weight,
+                                                   double *__restrict__ trace, double *__restrict__ ck, int64_t ck_stride,
+                                                   int64_t nc_total, int32_t K, AdjLds L, int32_t nqmax,
+                                                   double *__restrict__ hist, int64_t hist_stride, int64_t n_seg,
+                                                   const int64_t *__restrict__ seg_off) {
     extern __shared__ __align__(16) unsigned char smem[];
     const AdjCell cell = T.cells[blockIdx.x];
-    auto v = make_view<TPB>(T, cell, smem, L, nqmax);
+    auto v = make_view<TPB>(T, cell, smem, L, nqmax, gbase, amp, weight);
     const int tid = threadIdx.x;
+    const RecLayout R = rec_layout(cell.n, cell.n_gate, cell.n_syn);
     for (int i = tid; i < cell.n; i += TPB) v.sV[i] = v0[cell.comp_begin + i];
     for (int k = tid; k < cell.n_gate; k += TPB) v.sQ[k] = q0[cell.gate_begin + k];
+    const int64_t soff = hist ? seg_off[blockIdx.x] : 0;
+    double t_next = T.t_row[0];
     bar<TPB>();
     int cur = 0;
     for (int64_t s = -3; s <= T.rows; s++) {
-        const double t = s >= 1 ? T.t_row[s - 1] : 0.0;
+        const double t = s >= 1 ? t_next : 0.0;
+        if (s >= 1) t_next = T.t_row[s];    // read a step ahead (t_row has rows + 1 entries)
+        double *rec = nullptr;
         if (s >= 1) {
             if ((s - 1) % K == 0) {
                 double *c = ck + ((s - 1) / K) * ck_stride;
@@ -451,135 +637,126 @@ __global__ void __launch_bounds__(TPB) adj_forward(AdjTables T, int32_t n_cells,
                 for (int k = tid; k < cell.n_gate; k += TPB) c[nc_total + cell.gate_begin + k] = v.sQ[cur * nqmax + k];
             }
             for (int i = tid; i < cell.n; i += TPB) {
-                const int rc = T.rec_col[cell.comp_begin + i];
+                const int rc = v.cr[i].rec_col;
                 if (rc >= 0) trace[(s - 1) * T.n_record + rc] = v.sV[i];
             }
             if (hist) {
-                const int64_t per = (int64_t)cell.n + cell.n_gate + 2 * cell.n_syn;
                 const int64_t k = (s - 1) / K, j = (s - 1) - k * K;
-                auto put = [&](int64_t seg, int64_t idx) {
-                    double *h = hist + seg * hist_stride + seg_off[blockIdx.x] + idx * per;
-                    for (int i = tid; i < cell.n; i += TPB) h[i] = v.sV[i];
-                    for (int q = tid; q < cell.n_gate; q += TPB) h[cell.n + q] = v.sQ[cur * nqmax + q];
-                    double *hs = h + cell.n + cell.n_gate;
-                    for (int ls = tid; ls < cell.n_syn; ls += TPB) {
-                        hs[ls] = v.sA[ls];
-                        hs[cell.n_syn + ls] = v.sB[ls];
-                    }
-                };
+                double *base = hist + soff;
                 // the state of the last step opens no segment of its own when (rows - 1) is a
                 // multiple of K: it is only the closing state of the segment before
-                if (k < n_seg) put(k, j);
-                if (j == 0 && k > 0) put(k - 1, K);
+                if (k < n_seg) {
+                    rec = base + k * hist_stride + j * R.per;
+                    v.put_state(rec, cur);
+                }
+                if (j == 0 && k > 0) v.put_state(base + (k - 1) * hist_stride + (int64_t)K * R.per, cur);
             }
         }
         if (s == T.rows) break;
-        v.step(s, t, cur, nqmax, gbase, amp, weight);
+        v.step(s, t, cur, rec, R);
         cur ^= 1;
     }
 }
 
-// One backward segment: steps [a, b) with a = 1 + k K. seg holds the states at the start of
-// steps a..b (per step: V of the cell's compartments then its q). lam (global) carries
-// (lambda_V, lambda_q) at the start of step b in and at the start of step a out; grad_g and
-// grad_a accumulate.
+// One backward segment: steps [a, b) with a = 1 + k K. seg holds the records of steps a..b
+// (the forward pass's history, or recomputed here from the segment's checkpoint). lam
+// (global) carries (lambda_V, lambda_q) at the start of step b in and at the start of step a
+// out; grad_g, grad_a and grad_w accumulate.
 template <int TPB>
 __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *__restrict__ gbase,
-                                                    const double *__restrict__ amp, const double *__restrict__ gtrace,
-                                                    const double *__restrict__ ck, int64_t nc_total, int64_t a,
-                                                    int64_t b, double *__restrict__ seg, const int64_t *__restrict__ seg_off,
-                                                    double *__restrict__ lam, double *__restrict__ grad_g,
-                                                    double *__restrict__ grad_a, AdjLds L, int32_t nqmax, int32_t stored,
-                                                    const double *__restrict__ weight, double *__restrict__ grad_w) {
+                                                    const double *__restrict__ amp, const double *__restrict__ weight,
+                                                    const double *__restrict__ gtrace, const double *__restrict__ ck,
+                                                    int64_t nc_total, int64_t a, int64_t b, double *__restrict__ seg,
+                                                    const int64_t *__restrict__ seg_off, double *__restrict__ lam,
+                                                    double *__restrict__ grad_g, double *__restrict__ grad_a,
+                                                    double *__restrict__ grad_w, AdjLds L, int32_t nqmax, int32_t stored) {
     extern __shared__ __align__(16) unsigned char smem[];
     const AdjCell cell = T.cells[blockIdx.x];
-    auto v = make_view<TPB>(T, cell, smem, L, nqmax);
+    auto v = make_view<TPB>(T, cell, smem, L, nqmax, gbase, amp, weight);
     const int tid = threadIdx.x;
     const int N = cell.n, NG = cell.n_gate, NS = cell.n_syn;
-    const int64_t per = (int64_t)N + NG + 2 * NS;
+    const RecLayout R = rec_layout(N, NG, NS);
     double *S = seg + seg_off[blockIdx.x];
     const double dt = T.dt;
     for (int c = tid; c < cell.n_chan; c += TPB) v.sGg[c] = 0.0;
     for (int p = tid; p < cell.n_pulse; p += TPB) v.sGa[p] = 0.0;
+    for (int ls = tid; ls < NS; ls += TPB) v.sGw[ls] = 0.0;
     if (!stored) {
-        // recompute the segment from its checkpoint (else seg is the forward pass's history)
+        // recompute the segment's records from its checkpoint (each thread later reads back
+        // the entries it wrote here)
         for (int i = tid; i < N; i += TPB) v.sV[i] = ck[cell.comp_begin + i];
         for (int k = tid; k < NG; k += TPB) v.sQ[k] = ck[nc_total + cell.gate_begin + k];
         bar<TPB>();
         int cur = 0;
         for (int64_t s = a; s <= b; s++) {
-            double *Ss = S + (s - a) * per;
-            for (int i = tid; i < N; i += TPB) Ss[i] = v.sV[i];
-            for (int k = tid; k < NG; k += TPB) Ss[N + k] = v.sQ[cur * nqmax + k];
+            double *Ss = S + (s - a) * R.per;
+            v.put_state(Ss, cur);
             if (s == b) break;
-            v.step(s, T.t_row[s - 1], cur, nqmax, gbase, amp, weight);
+            v.step(s, T.t_row[s - 1], cur, Ss, R);
             cur ^= 1;
         }
+        // the records are read below by other threads of the workgroup
+        __threadfence_block();
+        bar<TPB>();
+        __threadfence_block();
     }
-    // adjoint state at the start of step b
+    // the record of the step being differentiated, in LDS: state, reciprocal diagonals, gate
+    // derivatives read in place
+    v.sV = v.sStage;
+    v.sQ = v.sStage + R.q;
+    v.sA = v.sStage + R.a;
+    v.sB = v.sStage + R.b;
+    v.sInv = v.sStage + R.inv;
+    v.sDQ = v.sStage + R.dq;
+    v.sDV = v.sStage + R.dv;
+    // adjoint state at the start of step b, and V there
     for (int i = tid; i < N; i += TPB) v.sLV[i] = lam[cell.comp_begin + i];
     for (int k = tid; k < NG; k += TPB) v.sLQ[k] = lam[nc_total + cell.gate_begin + k];
+    for (int i = tid; i < N; i += TPB) v.sVn[i] = S[(b - a) * R.per + i];
+    double t_next = T.t_row[b - 2];
     bar<TPB>();
     for (int64_t s = b - 1; s >= a; s--) {
-        const double *Ss = S + (s - a) * per;        // state at the start of step s
-        const double *Sn = Ss + per;                   // state at the start of step s + 1
-        const double t = T.t_row[s - 1];
+        const double *Ss = S + (s - a) * R.per;        // record of step s
+        const double t = t_next;
+        if (s >= 2) t_next = T.t_row[s - 2];          // read a step ahead
+        // step s's record (reads issued in batches)
+#pragma unroll 4
+        for (int j = tid; j < R.per; j += TPB) v.sStage[j] = Ss[j];
         // dLoss/dV at the start of step s + 1 (trace row s)
         if (s < T.rows) {
             const double *gr = gtrace + s * T.n_record;
             for (int i = tid; i < N; i += TPB) {
-                const int rc = T.rec_col[cell.comp_begin + i];
+                const int rc = v.cr[i].rec_col;
                 if (rc >= 0) v.sLV[i] += gr[rc];
             }
         }
-        // state of step s into LDS; open fractions
-        for (int i = tid; i < N; i += TPB) v.sV[i] = Ss[i];
-        for (int k = tid; k < NG; k += TPB) v.sQ[k] = Ss[N + k];
-        for (int ls = tid; ls < NS; ls += TPB) {
-            v.sA[ls] = Ss[N + NG + ls];
-            v.sB[ls] = Ss[N + NG + NS + ls];
-        }
         bar<TPB>();
-        v.fopen(0, nqmax);
+        v.fopen(0);
+        // right-hand side lambda_V' of the transposed solve
+        for (int i = tid; i < N; i += TPB) v.sX[v.cr[i].pos] = v.sLV[i];
+        if (tid == 0) v.sX[N] = 0.0;
         bar<TPB>();
-        // diagonal of step s and right-hand side lambda_V' for the transposed solve
+        v.hines_T();
+        // membrane, channel, pulse and synapse adjoints
         for (int i = tid; i < N; i += TPB) {
-            const int64_t g = cell.comp_begin + i;
-            double I, G;
-            v.currents(i, v.sV[i], t, gbase, amp, I, G, weight);
-            const double D = 1 * (G / T.C[g]) + T.invrc[g];
-            const int p = T.pos[g];
-            v.sD[p] = 1 + D * dt;
-            v.sX[p] = v.sLV[i];
-        }
-        if (tid == 0) {
-            v.sD[N] = 1.0;
-            v.sX[N] = 0.0;
-        }
-        bar<TPB>();
-        v.hines(true);
-        // membrane and channel adjoints
-        for (int i = tid; i < N; i += TPB) {
-            const int64_t g = cell.comp_begin + i;
-            const double lb = v.sX[T.pos[g]];
-            const double Vn = Sn[i];
-            const double lD = -lb * Vn;
-            const double C = T.C[g], A = T.A[g];
+            const CompRec c = v.cr[i];
+            const double lb = v.sX[c.pos];
+            const double lD = -lb * v.sVn[i];
+            const double C = (double)c.C, A = (double)c.A;
             const double k = dt / C * K1E5 * A;
-            for (int ch = T.ch_begin[g]; ch < T.ch_begin[g + 1]; ch++) {
-                const int c = ch - cell.chan_begin;
-                const double lGs = k * (lb * T.ch_erev[ch] + lD);
-                v.sGg[c] += lGs * v.sF[c];
-                v.sLF[c] = lGs * gbase[ch];
+            for (int ch = c.ch_b; ch < c.ch_e; ch++) {
+                const double lGs = k * (lb * (double)v.chr[ch].erev + lD);
+                v.sGg[ch] += lGs * v.sF[ch];
+                v.sLF[ch] = lGs * v.sGb[ch];
             }
-            for (int p = T.p_begin[g]; p < T.p_begin[g + 1]; p++)
-                if (T.p_start[p] <= t && t <= T.p_start[p] + T.p_dur[p])
-                    v.sGa[p - cell.pulse_begin] += lb * dt / C * T.p_weight[p];
+            for (int p = c.p_b; p < c.p_e; p++) {
+                const double ps = (double)v.pr[p].start, pd = (double)v.pr[p].dur;
+                if (ps <= t && t <= ps + pd) v.sGa[p] += lb * dt / C * (double)v.pr[p].weight;
+            }
             // synapse instances: a conductance w g_s(t) into reversal erev, like a channel with
             // no area or density factor; g_s does not depend on V or on the parameters
-            for (int sy = T.s_begin[g]; sy < T.s_begin[g + 1]; sy++) {
-                const AdjSynType st = T.s_types[T.s_type[sy]];
-                const int ls = sy - cell.syn_begin;
+            for (int ls = c.s_b; ls < c.s_e; ls++) {
+                const AdjSynType &st = v.sttab[v.sty[ls]];
                 const double gs = st.gbase * (v.sB[ls] - v.sA[ls]);
                 v.sGw[ls] += dt / C * (lb * st.erev + lD) * gs;
             }
@@ -589,20 +766,15 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
         // gate adjoints: lambda_q of step s, and each gate's term of lambda_V
         double *lqn = v.sLQ + nqmax;
         for (int kq = tid; kq < NG; kq += TPB) {
-            const int gq = cell.gate_begin + kq;
-            const int ch = T.g_chan[gq];
-            const int ty = T.ch_type[ch];
-            const int kk = T.g_k[gq];
-            const int q0 = T.ch_gate[ch] - cell.gate_begin;
-            const double V = v.sV[T.ch_comp[ch]];
-            double dq, dV;
-            (void)gate64(T.gates[ty * 4 + kk], V, v.sQ[kq], false, dt, &dq, &dV);
-            // dfopen/dq_kk: product over the channel's gates with one factor q_kk removed
+            const GateRec g = v.gr[kq];
+            const ChanRec ch = v.chr[g.chan];
+            const int ng = (int)(ch.shape & 7u);
+            // dfopen/dq_k: product over the channel's gates with one factor q_k removed
             double df = 1.0;
-            for (int m = 0; m < T.type_ngates[ty]; m++) {
-                const double qv = v.sQ[q0 + m];
-                const int inst = T.gates[ty * 4 + m].instances;
-                if (m == kk) {
+            for (int m = 0; m < ng; m++) {
+                const double qv = v.sQ[ch.q0 + m];
+                const int inst = (int)((ch.shape >> (3 + 6 * m)) & 63u);
+                if (m == g.k) {
                     double pw = (double)inst;
                     for (int r = 0; r < inst - 1; r++) pw = pw * qv;
                     df = df * (inst > 0 ? pw : 0.0);
@@ -611,18 +783,20 @@ __global__ void __launch_bounds__(TPB) adj_backward(AdjTables T, const double *_
                 }
             }
             const double lq1 = v.sLQ[kq];
-            lqn[kq] = lq1 * dq + v.sLF[ch - cell.chan_begin] * df;
-            v.sGV[kq] = lq1 * dV;
+            lqn[kq] = lq1 * v.sDQ[kq] + v.sLF[g.chan] * df;
+            v.sGV[kq] = lq1 * v.sDV[kq];
         }
         bar<TPB>();
         for (int i = tid; i < N; i += TPB) {
-            const int64_t g = cell.comp_begin + i;
+            const CompRec c = v.cr[i];
             double lv = v.sLV[i];
-            for (int ch = T.ch_begin[g]; ch < T.ch_begin[g + 1]; ch++) {
-                const int q0 = T.ch_gate[ch] - cell.gate_begin;
-                for (int m = 0; m < T.type_ngates[T.ch_type[ch]]; m++) lv += v.sGV[q0 + m];
+            for (int ch = c.ch_b; ch < c.ch_e; ch++) {
+                const ChanRec r = v.chr[ch];
+                const int ng = (int)(r.shape & 7u);
+                for (int m = 0; m < ng; m++) lv += v.sGV[r.q0 + m];
             }
             v.sLV[i] = lv;
+            v.sVn[i] = v.sV[i];
         }
         for (int kq = tid; kq < NG; kq += TPB) v.sLQ[kq] = lqn[kq];
         bar<TPB>();
@@ -760,54 +934,44 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
     a->rows = (int64_t)trow.size();
     trow.push_back(trow.empty() ? 0.0 : trow.back() + dt);
     a->T.rows = a->rows;
-    // channel types
-    std::vector<int32_t> tng(std::max(1, d->n_channel_types));
-    std::vector<AdjGate> gates((size_t)std::max(1, d->n_channel_types) * 4);
+    // channel types: the gate table (4 entries per type) and each type's gate shape word
+    const int n_types = std::max(1, d->n_channel_types);
+    if ((int64_t)n_types * 4 > 65535) return afail(a, EDENHIP_E_LIMIT, "more than 16,383 channel types");
+    std::vector<AdjGate> gates((size_t)n_types * 4);
+    std::vector<uint32_t> type_shape((size_t)n_types, 0u);
     for (int ty = 0; ty < d->n_channel_types; ty++) {
         const edenhip_channel_type &ct = d->channel_types[ty];
         if (ct.n_gates < 0 || ct.n_gates > 4) return afail(a, EDENHIP_E_LIMIT, "channel type with more than 4 gates");
-        tng[ty] = ct.n_gates;
+        uint32_t shape = (uint32_t)ct.n_gates;
         for (int k = 0; k < ct.n_gates; k++) {
             const edenhip_gate &g = ct.gates[k];
+            if (g.instances < 0 || g.instances > 63) return afail(a, EDENHIP_E_LIMIT, "gate with more than 63 instances");
             AdjGate x{};
             x.alpha_form = g.alpha.form; x.beta_form = g.beta.form; x.instances = g.instances;
             x.q10 = (double)g.q10;
             x.ar = (double)g.alpha.rate; x.am = (double)g.alpha.midpoint; x.as = (double)g.alpha.scale;
             x.br = (double)g.beta.rate; x.bm = (double)g.beta.midpoint; x.bs = (double)g.beta.scale;
             gates[(size_t)ty * 4 + k] = x;
+            shape |= (uint32_t)g.instances << (3 + 6 * k);
         }
+        type_shape[(size_t)ty] = shape;
     }
-    // per compartment / channel / gate / pulse tables
-    std::vector<double> C(NC), A(NC), invrc(NC), v0(NC), erev(a->NCH), gb0(a->NCH), amp0(a->NP), ps(a->NP), pd(a->NP),
-        pw(a->NP);
-    std::vector<int32_t> pos(NC), chb(NC + 1), pb(NC + 1), rec(NC, -1), cht(a->NCH), chg(a->NCH), chc(a->NCH);
-    std::vector<int32_t> gch, gk;
-    std::vector<double> q0;
-    std::vector<double2> u(NC, make_double2(0.0, 0.0));
-    for (int64_t g = 0; g < NC; g++) {
-        C[g] = (double)d->capacitance[g]; A[g] = (double)d->area[g]; invrc[g] = (double)d->inv_rc_axial[g];
-        v0[g] = (double)d->v0[g];
-        chb[g] = (int32_t)d->comp_chan_begin[g]; pb[g] = (int32_t)d->comp_pulse_begin[g];
-    }
-    chb[NC] = (int32_t)a->NCH; pb[NC] = (int32_t)a->NP;
-    for (int64_t p = 0; p < a->NP; p++) {
-        ps[p] = (double)d->pulse_start[p]; pd[p] = (double)d->pulse_duration[p];
-        pw[p] = (double)d->pulse_weight[p]; amp0[p] = (double)d->pulse_amplitude[p];
-    }
+    // recorded compartments
     a->n_record = d->n_record;
     a->T.n_record = (int32_t)d->n_record;
+    std::vector<int32_t> rec(NC, -1);
     for (int64_t r = 0; r < d->n_record; r++) {
         const int64_t g = d->record_comp[r];
         if (g < 0 || g >= NC) return afail(a, EDENHIP_E_INVALID, "record_comp out of range");
         if (rec[g] >= 0) return afail(a, EDENHIP_E_INVALID, "a compartment may be recorded once in the adjoint model");
         rec[g] = (int32_t)r;
     }
-    // synapses: per compartment CSR, types in double with the waveform factor evaluated in
-    // double with the reference's expression tree (as the oracle's FP64 flavour per step)
-    std::vector<int32_t> sb(NC + 1), sty(std::max<int64_t>(a->NSYN, 1), 0);
-    for (int64_t g = 0; g <= NC; g++) sb[g] = (int32_t)d->comp_syn_begin[g];
+    // synapses: types in double with the waveform factor evaluated in double with the
+    // reference's expression tree (as the oracle's FP64 flavour per step)
+    std::vector<int32_t> sty(std::max<int64_t>(a->NSYN, 1), 0);
     for (int64_t sy = 0; sy < a->NSYN; sy++) sty[sy] = d->syn_type[sy];
-    std::vector<AdjSynType> stypes(std::max(1, d->n_syn_types));
+    const int n_stypes = std::max(1, d->n_syn_types);
+    std::vector<AdjSynType> stypes((size_t)n_stypes);
     for (int t = 0; t < d->n_syn_types; t++) {
         const edenhip_exptwo_type &e = d->syn_types[t];
         const double tr = (double)e.tau_rise, td = (double)e.tau_decay;
@@ -817,6 +981,19 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
     }
     std::vector<double> w0(std::max<int64_t>(a->NSYN, 1), 0.0);
     for (int64_t sy = 0; sy < a->NSYN; sy++) w0[sy] = (double)d->syn_weight[sy];
+    // per compartment / channel / gate / pulse records and initial values
+    std::vector<CompRec> comp(NC);
+    std::vector<ChanRec> chan(std::max<int64_t>(a->NCH, 1));
+    std::vector<GateRec> gate;
+    std::vector<PulseRec> pulse(std::max<int64_t>(a->NP, 1));
+    std::vector<double> v0(NC), gb0(std::max<int64_t>(a->NCH, 1)), amp0(std::max<int64_t>(a->NP, 1)), q0;
+    std::vector<double2> u(NC, make_double2(0.0, 0.0));
+    for (int64_t g = 0; g < NC; g++) v0[g] = (double)d->v0[g];
+    for (int64_t p = 0; p < a->NP; p++) {
+        pulse[p] = PulseRec{d->pulse_start[p], d->pulse_duration[p], d->pulse_weight[p], 0.0f};
+        amp0[p] = (double)d->pulse_amplitude[p];
+    }
+    for (int64_t ch = 0; ch < a->NCH; ch++) gb0[ch] = (double)d->chan_gbase[ch];
     int32_t nsmax = 1;
     std::vector<AdjCell> cells(d->n_cells);
     std::vector<uint32_t> words;
@@ -852,7 +1029,8 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         AdjCell cell{};
         cell.comp_begin = c0;
         cell.n = n;
-        const TreePlan P = plan_tree(n, d->parent + c0, a->tpb);
+        // rounds of at most 64 tasks: one wave runs the whole solve
+        const TreePlan P = plan_tree(n, d->parent + c0, 64);
         if (!P.ok) return afail(a, EDENHIP_E_LIMIT, "the adjoint supports at most 1,023 children per compartment");
         cell.rounds_fwd = n > 1 ? P.rounds_fwd : 0;
         cell.rounds_bwd = n > 1 ? P.rounds_bwd : 0;
@@ -860,8 +1038,9 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         cell.words_len = (int32_t)P.block.size();
         words.insert(words.end(), P.block.begin(), P.block.end());
         wmax = std::max(wmax, cell.words_len);
-        for (int32_t i = 0; i < n; i++) pos[c0 + i] = P.pos[i];
+        std::vector<char> has_child((size_t)n, 0);
         for (int32_t i = 1; i < n; i++) {
+            has_child[(size_t)d->parent[c0 + i]] = 1;
             // Ui = -dt/(R C_i), Uj = -dt/(R C_parent) with R = R_axial of the child (eden/Eden.cpp:6136-6150)
             const double R = (double)d->r_axial[c0 + i];
             const int32_t j = d->parent[c0 + i];
@@ -869,72 +1048,82 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         }
         cell.chan_begin = (int32_t)d->comp_chan_begin[c0];
         cell.n_chan = (int32_t)(d->comp_chan_begin[c1] - d->comp_chan_begin[c0]);
-        cell.gate_begin = (int32_t)q0.size();
-        for (int64_t g = c0; g < c1; g++)
-            for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
-                const int ty = d->chan_type[ch];
-                if (ty < 0 || ty >= d->n_channel_types) return afail(a, EDENHIP_E_INVALID, "channel type index out of range");
-                cht[ch] = ty;
-                erev[ch] = (double)d->chan_erev[ch];
-                gb0[ch] = (double)d->chan_gbase[ch];
-                chg[ch] = (int32_t)q0.size();
-                chc[ch] = (int32_t)(g - c0);
-                for (int k = 0; k < d->channel_types[ty].n_gates; k++) {
-                    const edenhip_gate &gt = d->channel_types[ty].gates[k];
-                    const double al = edenhip_rate_host(gt.alpha, d->v0[g]), be = edenhip_rate_host(gt.beta, d->v0[g]);
-                    q0.push_back(al / (al + be));
-                    gch.push_back((int32_t)ch);
-                    gk.push_back(k);
-                }
-            }
-        cell.n_gate = (int32_t)(q0.size() - (size_t)cell.gate_begin);
         cell.pulse_begin = (int32_t)d->comp_pulse_begin[c0];
         cell.n_pulse = (int32_t)(d->comp_pulse_begin[c1] - d->comp_pulse_begin[c0]);
         cell.syn_begin = (int32_t)d->comp_syn_begin[c0];
         cell.n_syn = (int32_t)(d->comp_syn_begin[c1] - d->comp_syn_begin[c0]);
+        if (cell.n_chan > 65535 || cell.n_pulse > 65535 || cell.n_syn > 65535)
+            return afail(a, EDENHIP_E_LIMIT, "more than 65,535 channel distributions, pulse inputs or synapses on one cell");
+        cell.gate_begin = (int32_t)q0.size();
+        for (int64_t g = c0; g < c1; g++) {
+            CompRec r{};
+            r.A = d->area[g];
+            r.C = d->capacitance[g];
+            r.invrc = d->inv_rc_axial[g];
+            r.rec_col = rec[g];
+            r.pos = (uint16_t)P.pos[g - c0];
+            r.leaf = has_child[(size_t)(g - c0)] ? 0 : 1;
+            r.ch_b = (uint16_t)(d->comp_chan_begin[g] - cell.chan_begin);
+            r.ch_e = (uint16_t)(d->comp_chan_begin[g + 1] - cell.chan_begin);
+            r.p_b = (uint16_t)(d->comp_pulse_begin[g] - cell.pulse_begin);
+            r.p_e = (uint16_t)(d->comp_pulse_begin[g + 1] - cell.pulse_begin);
+            r.s_b = (uint16_t)(d->comp_syn_begin[g] - cell.syn_begin);
+            r.s_e = (uint16_t)(d->comp_syn_begin[g + 1] - cell.syn_begin);
+            comp[g] = r;
+            for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
+                const int ty = d->chan_type[ch];
+                if (ty < 0 || ty >= d->n_channel_types) return afail(a, EDENHIP_E_INVALID, "channel type index out of range");
+                ChanRec x{};
+                x.erev = d->chan_erev[ch];
+                x.comp = (uint16_t)(g - c0);
+                x.q0 = (uint16_t)(q0.size() - (size_t)cell.gate_begin);
+                x.shape = type_shape[(size_t)ty];
+                chan[ch] = x;
+                for (int k = 0; k < d->channel_types[ty].n_gates; k++) {
+                    const edenhip_gate &gt = d->channel_types[ty].gates[k];
+                    const double al = edenhip_rate_host(gt.alpha, d->v0[g]), be = edenhip_rate_host(gt.beta, d->v0[g]);
+                    q0.push_back(al / (al + be));
+                    gate.push_back(GateRec{(uint16_t)(g - c0), (uint16_t)(ch - cell.chan_begin), (uint16_t)(ty * 4 + k), (uint16_t)k});
+                }
+            }
+        }
+        cell.n_gate = (int32_t)(q0.size() - (size_t)cell.gate_begin);
+        if (cell.n_gate > 65535) return afail(a, EDENHIP_E_LIMIT, "more than 65,535 gate variables on one cell");
         nsmax = std::max(nsmax, cell.n_syn);
         nqmax = std::max(nqmax, cell.n_gate);
         nchmax = std::max(nchmax, cell.n_chan);
         npmax = std::max(npmax, cell.n_pulse);
         cells[c] = cell;
         seg_off[c] = seg_total;
-        seg_total += ((int64_t)n + cell.n_gate + 2 * (int64_t)cell.n_syn);
+        seg_total += rec_layout(n, cell.n_gate, cell.n_syn).per;
     }
     a->NQ = (int64_t)q0.size();
     a->nqmax = nqmax;
-    a->L = adj_lds(nmax, nqmax, nchmax, npmax, wmax, nsmax);
+    a->L = adj_lds(nmax, nqmax, nchmax, npmax, wmax, nsmax, n_types * 4, n_stypes);
     a->lds = (size_t)a->L.total * sizeof(double);
-    if (a->lds > 160 * 1024) return afail(a, EDENHIP_E_LIMIT, "cell too large for the adjoint kernels' LDS");
+    if (a->lds > 160 * 1024) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "cell too large for the adjoint kernels' LDS (%zu bytes needed, 163,840 available)", a->lds);
+        return afail(a, EDENHIP_E_LIMIT, msg);
+    }
     a->K = std::max(1, a->K);
     const int64_t steps = a->rows;   // steps 1..rows-1 are differentiated
     a->n_ck = (steps + a->K - 1) / a->K + 1;
     a->ck_stride = NC + a->NQ;
-    // segment buffer: (K + 1) states per cell
+    // segment buffer: (K + 1) records per cell
     for (auto &o : seg_off) o *= (int64_t)(a->K + 1);
     AdjTables &T = a->T;
+    T.n_gate_params = n_types * 4;
+    T.n_syn_types = n_stypes;
     ATRY(aupload(a, (AdjCell **)&T.cells, cells));
-    ATRY(aupload(a, (double **)&T.C, C));
-    ATRY(aupload(a, (double **)&T.A, A));
-    ATRY(aupload(a, (double **)&T.invrc, invrc));
-    ATRY(aupload(a, (int32_t **)&T.pos, pos));
-    ATRY(aupload(a, (int32_t **)&T.ch_begin, chb));
-    ATRY(aupload(a, (int32_t **)&T.p_begin, pb));
-    ATRY(aupload(a, (int32_t **)&T.rec_col, rec));
-    ATRY(aupload(a, (int32_t **)&T.ch_type, cht));
-    ATRY(aupload(a, (int32_t **)&T.ch_gate, chg));
-    ATRY(aupload(a, (int32_t **)&T.ch_comp, chc));
-    ATRY(aupload(a, (double **)&T.ch_erev, erev));
-    ATRY(aupload(a, (int32_t **)&T.g_chan, gch));
-    ATRY(aupload(a, (int32_t **)&T.g_k, gk));
-    ATRY(aupload(a, (int32_t **)&T.type_ngates, tng));
+    ATRY(aupload(a, (CompRec **)&T.comp, comp));
+    ATRY(aupload(a, (ChanRec **)&T.chan, chan));
+    ATRY(aupload(a, (GateRec **)&T.gate, gate));
+    ATRY(aupload(a, (PulseRec **)&T.pulse, pulse));
     ATRY(aupload(a, (AdjGate **)&T.gates, gates));
-    ATRY(aupload(a, (double **)&T.p_start, ps));
-    ATRY(aupload(a, (double **)&T.p_dur, pd));
-    ATRY(aupload(a, (double **)&T.p_weight, pw));
     ATRY(aupload(a, (double2 **)&T.u, u));
     ATRY(aupload(a, (uint32_t **)&T.words, words));
     ATRY(aupload(a, (double **)&T.t_row, trow));
-    ATRY(aupload(a, (int32_t **)&T.s_begin, sb));
     ATRY(aupload(a, (int32_t **)&T.s_type, sty));
     ATRY(aupload(a, (AdjSynType **)&T.s_types, stypes));
     ATRY(aupload(a, &a->d_w0, w0));
@@ -950,7 +1139,7 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
     ATRY(aupload(a, &a->d_seg_off, seg_off));
     ATRY(aalloc(a, (void **)&a->d_ck, (size_t)(a->n_ck * a->ck_stride) * sizeof(double)));
     ATRY(aalloc(a, (void **)&a->d_seg, (size_t)(seg_total * (a->K + 1)) * sizeof(double)));
-    // state history of every step (store mode): n_seg segments of seg_total (K + 1) doubles
+    // history of every step (store mode): n_seg segments of seg_total (K + 1) doubles
     a->hist_stride = seg_total * (a->K + 1);
     a->n_seg = a->rows >= 2 ? (a->rows - 2) / a->K + 1 : 0;
     {
@@ -1063,12 +1252,12 @@ int edenhip_adj_forward_w(edenhip_adj *a, const double *gbase, const double *amp
     AHIP(a, hipEventCreate(&e1));
     AHIP(a, hipEventRecord(e0, a->stream));
     AdjTables T = a->T;
-    int32_t ncell = a->n_cells, K = a->K, nq = a->nqmax;
+    int32_t K = a->K, nq = a->nqmax;
     int64_t cks = a->ck_stride, nct = a->NC;
     AdjLds L = a->L;
     int64_t hs = a->hist_stride, ns = a->n_seg;
-    void *args[] = {&T, &ncell, &a->d_v0, &a->d_q0, (void *)&gb, (void *)&am, &trace, &a->d_ck, &cks, &nct, &K, &L, &nq,
-                    &a->d_hist, &hs, &ns, &a->d_seg_off, (void *)&wt};
+    void *args[] = {&T, &a->d_v0, &a->d_q0, (void *)&gb, (void *)&am, (void *)&wt, &trace, &a->d_ck, &cks, &nct, &K, &L, &nq,
+                    &a->d_hist, &hs, &ns, &a->d_seg_off};
     AHIP(a, hipLaunchKernel(fwd_kernel(a->tpb), dim3(a->n_cells), dim3(a->tpb), args, a->lds, a->stream));
     AHIP(a, hipEventRecord(e1, a->stream));
     AHIP(a, hipEventSynchronize(e1));
@@ -1128,8 +1317,8 @@ int edenhip_adj_backward_w(edenhip_adj *a, const double *grad_trace, double *gra
         const double *ckp = a->d_ck + k * a->ck_stride;
         int32_t stored = a->d_hist ? 1 : 0;
         double *segp = a->d_hist ? a->d_hist + k * a->hist_stride : a->d_seg;
-        void *args[] = {&T, (void *)&a->last_gbase, (void *)&a->last_amp, (void *)&grad_trace, (void *)&ckp, &nct, &sa, &sb,
-                        &segp, &a->d_seg_off, &a->d_lam, &gg, &ga, &L, &nq, &stored, (void *)&a->last_w, &gw};
+        void *args[] = {&T, (void *)&a->last_gbase, (void *)&a->last_amp, (void *)&a->last_w, (void *)&grad_trace,
+                        (void *)&ckp, &nct, &sa, &sb, &segp, &a->d_seg_off, &a->d_lam, &gg, &ga, &gw, &L, &nq, &stored};
         AHIP(a, hipLaunchKernel(bwd_kernel(a->tpb), dim3(a->n_cells), dim3(a->tpb), args, a->lds, a->stream));
     }
     AHIP(a, hipEventRecord(e1, a->stream));

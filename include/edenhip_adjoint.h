@@ -5,16 +5,25 @@
  * No counterpart in the reference (EDEN has no gradients). The differentiated map is the
  * per-step update of eden/Eden.cpp:5236-5259 (gates), 5805-5898 (channel currents), 3798-3826
  * (pulses), 6057-6107 (membrane) and 6114-6225 (Hines solve), evaluated in double precision
- * with the reference's operation order (as the FP64 restatement in oracle/ does). Spike
- * threshold crossings only feed event logs here, so they carry no gradient.
+ * with the reference's operations (as the FP64 restatement in oracle/ does), except that the
+ * Hines solve multiplies by each node's reciprocal eliminated diagonal where the reference
+ * divides by the diagonal (results differ in the last bits). Spike threshold crossings only
+ * feed event logs here, so they carry no gradient.
  *
  * The forward pass records the voltage trace of the recorded compartments (double), a
  * checkpoint of the full state every `checkpoint_steps` steps and, when it fits (see
- * store_states), the full state of every step; the backward pass takes dLoss/dtrace and,
- * segment by segment from the last, takes the segment's states from that history or
- * recomputes them from the segment's checkpoint, and runs the adjoint recursion backwards, accumulating dLoss/dGbase per channel
- * distribution, dLoss/dAmplitude per pulse input and dLoss/dWeight per synapse instance. Gates
- * clipped to [1e-6, 1 - 1e-6] in a step pass no gradient through that step's update.
+ * store_states), a record of every step: its starting state and its Jacobian factors (the
+ * reciprocal eliminated Hines diagonals and each gate's dq'/dq and dq'/dV). The backward pass
+ * takes dLoss/dtrace and, segment by segment from the last, takes the segment's records from
+ * that history or recomputes them from the segment's checkpoint, and runs the adjoint
+ * recursion backwards, accumulating dLoss/dGbase per channel distribution, dLoss/dAmplitude
+ * per pulse input and dLoss/dWeight per synapse instance. Gates clipped to [1e-6, 1 - 1e-6]
+ * in a step pass no gradient through that step's update.
+ *
+ * Sizes: one cell's tables, state and step record live in LDS (at most 160 KiB per cell; an
+ * HH cell of 100 compartments takes about 50 KiB); larger cells are refused with
+ * EDENHIP_E_LIMIT. The history takes (2 n + 1 + 3 gates + 2 synapses) doubles per cell and
+ * step.
  *
  * Synapses: threshold crossings are fixed events (no gradient through spike timing). The
  * steps at which each synapse instance receives its event are given by the caller
@@ -51,8 +60,9 @@ typedef struct edenhip_adj_options {
 } edenhip_adj_options;
 
 /* Build the differentiable model from the same descriptor edenhip_create() takes. The model
- * must have no synapses, spike recipients or Poisson inputs (EDENHIP_E_INVALID otherwise).
- * The descriptor's chan_gbase and pulse_amplitude are the default parameter values. */
+ * must have no Poisson inputs or generic LEMS synapse types (EDENHIP_E_INVALID otherwise);
+ * spike recipients are ignored (the synapses' events come from edenhip_adj_set_events). The
+ * descriptor's chan_gbase, pulse_amplitude and syn_weight are the default parameter values. */
 int edenhip_adj_create(const edenhip_model_desc *desc, const edenhip_adj_options *opt, edenhip_adj **out);
 
 /* The fixed event schedule of the synapses: n pairs (step, synapse instance), steps counting

@@ -1,7 +1,8 @@
 """GPU discrete adjoint (include/edenhip_adjoint.h) against finite differences.
 
-* the adjoint's FP64 forward against the oracle's FP64 flavour (same model, same order of
-  operations; only the double exp/log implementations differ);
+* the adjoint's FP64 forward against the oracle's FP64 flavour (same model and operations,
+  except the double exp/log implementations and the Hines solve's multiplications by
+  reciprocal diagonals where the oracle divides): 1e-8 mV;
 * dLoss/dGbase and dLoss/dAmplitude against central finite differences in FP64 of the
   GPU's own FP64 forward (h = 1e-6 relative): 1e-4 relative;
 * the same gradient against central finite differences of the oracle's FP64 CPU path;
