@@ -233,6 +233,33 @@ def adjoint_bench(reps=2):
                 loss=float(loss.detach()))
 
 
+def lds_entry(kernel, workload):
+    """The dominant kernel's LDS figures from the committed SQ counter summary of the same
+    workload (profiles/*_sq_counters.md), or None: LDS utilisation = LDS-active cycles over
+    the CU cycles of the dispatch (the summary's LDS_UTIL), against a peak of 1 (the LDS busy
+    every cycle); the share of those cycles lost to bank conflicts; LDS instructions per
+    launch; and the shares of wave cycles spent waiting and issuing vector ALU work, which say
+    how far the kernel is latency-bound instead."""
+    import glob
+    from eden_amd.profiling import read_counters
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_sq_counters.md"))):
+        c = read_counters(f, kernel, workload)
+        if not c or "LDS_UTIL" not in c or not c.get("SQ_LDS_IDX_ACTIVE"):
+            continue
+        active = c["SQ_LDS_IDX_ACTIVE"]
+        util = c["LDS_UTIL"]
+        out = dict(bound="lds", metric="LDS-active cycles / CU cycles", achieved=util, peak=1.0, frac=util,
+                   bank_conflict_share=c.get("SQ_LDS_BANK_CONFLICT", 0.0) / active,
+                   source=os.path.join("profiles", os.path.basename(f)))
+        if "SQ_INSTS_LDS" in c:
+            out["lds_insts_per_launch"] = c["SQ_INSTS_LDS"]
+        if c.get("SQ_WAVE_CYCLES"):
+            out["wave_wait_share"] = c.get("SQ_WAIT_ANY", 0.0) / c["SQ_WAVE_CYCLES"]
+            out["valu_active_share"] = c.get("SQ_ACTIVE_INST_VALU", 0.0) / c["SQ_WAVE_CYCLES"]
+        return out
+    return None
+
+
 def plain_launch_bench(model, rows):
     """The same workload with one kernel launch per step (state back in HBM between steps)
     instead of the persistent in-kernel time loop: launched one by one, and captured in
@@ -372,6 +399,7 @@ def main():
     traffic = None
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
         traffic = traffic or read_traffic(f, dom["name"], wkey)
+    lds = lds_entry(dom["name"], wkey)
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK, traffic=traffic, kernel=dom["name"],
                     kernel_cells=int(dom["cells"]), kernel_comps=int(dom["comps"]),
@@ -383,6 +411,8 @@ def main():
                     # over the mean wall time of a run (kernels overlap)
                     aggregate=dict(achieved=algorithmic_bytes(model, rows) / t_step / 1e9,
                                    frac=algorithmic_bytes(model, rows) / t_step / HBM_PEAK))
+    if lds is not None:
+        roofline["lds"] = lds
     out = {
         "metric": METRIC, "value": value, "unit": "compartment-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak",

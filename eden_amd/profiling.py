@@ -5,13 +5,18 @@ what the same counters read for a plain device copy of known size.
     python -m eden_amd.profiling stats <run_kernel_stats.csv> <title> <command...>
     python -m eden_amd.profiling traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
                                          <copy MiB> <kernel-substring> <workload> <command...>
-    python -m eden_amd.profiling counters <counters.csv> <title> <command...>
+    python -m eden_amd.profiling counters <counters.csv> <title> <workload> <CUs per XCD> <command...>
 
 The traffic summary ends with one line `traffic_bytes_per_launch: <kernel> | <workload> |
 <bytes>` per kernel whose name contains the substring; bench.py reads the one of its dominant
 kernel and workload for its roofline's `traffic`. The counters summary gives the mean per
 dispatch of every counter in a --pmc run, per kernel, with the shares that the SQ counters
-imply (LDS bank-conflict cycles over LDS-active cycles, waiting over wave cycles).
+imply (LDS bank-conflict cycles over LDS-active cycles, waiting over wave cycles, and with
+GRBM_GUI_ACTIVE the LDS utilisation: LDS-active cycles over the CU cycles of the dispatch,
+the utilisation formula of AMD's rocprof-compute; GRBM_GUI_ACTIVE is summed over the XCDs,
+so the CU cycles are GRBM_GUI_ACTIVE x CUs per XCD), and ends with one line
+`counters_per_launch: <kernel> | <workload> | NAME=value;...` per kernel (with LDS_UTIL, the
+utilisation), which bench.py reads for its roofline's `lds` entry.
 """
 import csv
 import re
@@ -91,7 +96,7 @@ def traffic(fetch, write, cfetch, cwrite, copy_mib, kernel_sub, workload, comman
     return "\n".join(lines) + "\n"
 
 
-def counters(path, title, command):
+def counters(path, title, workload, cus_per_xcd, command):
     acc, disp = {}, {}
     for r in csv.DictReader(open(path)):
         n = r["Kernel_Name"]
@@ -112,12 +117,39 @@ def counters(path, title, command):
              ("SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES", "wave cycles waiting on LDS instructions / wave cycles"),
              ("SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "VALU-active cycles / wave cycles")]
     rows = [(a, b, t) for a, b, t in share if a in names and b in names]
-    if rows:
-        lines += ["", "| kernel | " + " | ".join(t for _, _, t in rows) + " |", "|---|" + "---|" * len(rows)]
+    util = "SQ_LDS_IDX_ACTIVE" in names and "GRBM_GUI_ACTIVE" in names
+    if rows or util:
+        head = [t for _, _, t in rows] + ([f"LDS utilisation (LDS-active cycles / (GRBM_GUI_ACTIVE x {cus_per_xcd} "
+                                          f"CUs per XCD))"] if util else [])
+        lines += ["", "| kernel | " + " | ".join(head) + " |", "|---|" + "---|" * len(head)]
         for n in sorted(acc):
             d = acc[n]
-            lines.append(f"| {_short(n)} | " + " | ".join(f"{d[a] / d[b]:.3f}" if d[b] else "-" for a, b, _ in rows) + " |")
+            cells = [f"{d[a] / d[b]:.3f}" if d[b] else "-" for a, b, _ in rows]
+            if util:
+                d["LDS_UTIL"] = d["SQ_LDS_IDX_ACTIVE"] / (d["GRBM_GUI_ACTIVE"] * cus_per_xcd) if d["GRBM_GUI_ACTIVE"] else 0.0
+                cells.append(f"{d['LDS_UTIL']:.3f}")
+            lines.append(f"| {_short(n)} | " + " | ".join(cells) + " |")
+    lines.append("")
+    for n in sorted(acc):
+        k = len(disp[n])
+        vals = [f"{c}={acc[n].get(c, 0.0) / k:.6g}" for c in names]
+        if "LDS_UTIL" in acc[n]:
+            vals.append(f"LDS_UTIL={acc[n]['LDS_UTIL']:.6g}")
+        lines.append(f"counters_per_launch: {_short(n)} | {workload} | " + ";".join(vals))
     return "\n".join(lines) + "\n"
+
+
+def read_counters(path, kernel, workload):
+    """Mean counter values per launch recorded in a counters summary, as a dict, or None."""
+    try:
+        for line in open(path):
+            if line.startswith("counters_per_launch:"):
+                k, w, v = [x.strip() for x in line.split(":", 1)[1].split("|")]
+                if k == kernel and w == workload:
+                    return {a: float(b) for a, b in (x.split("=") for x in v.split(";") if x)}
+    except OSError:
+        return None
+    return None
 
 
 def read_traffic(path, kernel, workload):
@@ -140,4 +172,4 @@ if __name__ == "__main__":
         a = sys.argv[2:]
         sys.stdout.write(traffic(a[0], a[1], a[2], a[3], int(a[4]), a[5], a[6], " ".join(a[7:])))
     elif sys.argv[1] == "counters":
-        sys.stdout.write(counters(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:])))
+        sys.stdout.write(counters(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]), " ".join(sys.argv[6:])))
