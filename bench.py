@@ -302,6 +302,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip eden_amd.autotune (tree-cell size limits by timed runs) before the timed run")
+    ap.add_argument("--channel-wave", default="",
+                    help="channel wave per workgroup size a,b,c,d for 64/128/256/512 threads: 0 rule, 1 on, "
+                         "-1 off (skips the channel-wave search)")
     ap.add_argument("--size-limits", default="",
                     help="tree-cell size limits a,b,c,d for 64/128/256/512 threads (skips the autotuner)")
     args = ap.parse_args()
@@ -322,20 +325,30 @@ def main():
     print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
     model, desc = build_workload(args.workload)
     rows = model.total_rows()
-    limits, tune = None, None
+    limits, tune, waves = None, None, None
+    if args.channel_wave:
+        waves = tuple(int(x) for x in args.channel_wave.split(","))
     if args.size_limits:
         limits = tuple(int(x) for x in args.size_limits.split(","))
         tune = dict(size_limits=list(limits), given=True)
+        if waves is not None:
+            tune["channel_wave"] = list(waves)
     elif not args.no_autotune and args.threads_per_cell == 0 and model.n_comps > model.n_cells:
         from eden_amd import autotune
         t0 = time.perf_counter()
         limits, report = autotune.autotune(model, steps=None, repeats=2, verbose=True)
-        tune = dict(size_limits=list(limits), seconds=time.perf_counter() - t0,
-                    timings=[dict(size_limits=list(l), ms_per_run=t * 1e3) for l, t in report])
-        print(f"[bench] autotuned size limits {limits}", file=sys.stderr, flush=True)
+        best_t = min(t for l, t in report if tuple(l) == tuple(limits))
+        if waves is None:
+            waves, wreport = autotune.autotune_channel_wave(model, limits, repeats=2, verbose=True, baseline=best_t)
+        else:
+            wreport = []
+        tune = dict(size_limits=list(limits), channel_wave=list(waves), seconds=time.perf_counter() - t0,
+                    timings=[dict(size_limits=list(l), ms_per_run=t * 1e3) for l, t in report]
+                    + [dict(size_limits=list(limits), channel_wave=list(w), ms_per_run=t * 1e3) for w, t in wreport])
+        print(f"[bench] autotuned size limits {limits}, channel waves {waves}", file=sys.stderr, flush=True)
     # timed runs without per-launch HIP events; one more run with them gives the per-kernel
     # durations for the roofline
-    sim = Simulation(model, threads_per_cell=args.threads_per_cell, size_limits=limits)
+    sim = Simulation(model, threads_per_cell=args.threads_per_cell, size_limits=limits, channel_wave=waves)
     tpb, slots = sim.launch_shape()
 
     def one_step(s):
@@ -359,7 +372,8 @@ def main():
         times.append(el)
         print(f"[bench] step {s}: {el * 1e3:.2f} ms wall", file=sys.stderr, flush=True)
     kern = {}
-    with Simulation(model, threads_per_cell=args.threads_per_cell, profile=True, size_limits=limits) as psim:
+    with Simulation(model, threads_per_cell=args.threads_per_cell, profile=True, size_limits=limits,
+                    channel_wave=waves) as psim:
         one_step(psim)    # first launches
         one_step(psim)
         infos = psim.kernel_infos()
@@ -395,7 +409,10 @@ def main():
     from eden_amd.profiling import read_traffic
     wkey = args.workload
     if model.n_comps > model.n_cells:
-        wkey += " (size limits " + ",".join(str(x) for x in (limits or DEFAULT_LIMITS)) + ")"
+        wkey += " (size limits " + ",".join(str(x) for x in (limits or DEFAULT_LIMITS))
+        if waves and any(waves):
+            wkey += ", channel waves " + ",".join(str(x) for x in waves)
+        wkey += ")"
     traffic = None
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md"))):
         traffic = traffic or read_traffic(f, dom["name"], wkey)

@@ -11,7 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ROOT, "libedenhip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_GATES = 4
 
 RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED = 0, 1, 2, 3
@@ -68,7 +68,7 @@ class Options(C.Structure):
     _fields_ = [("threads_per_cell", C.c_int32), ("persistent", C.c_int32), ("chunk_steps", C.c_int32),
                 ("use_graph", C.c_int32), ("profile", C.c_int32), ("device", C.c_int32),
                 ("log_synapse_events", C.c_int32), ("network_window", C.c_int32),
-                ("size_limits", C.c_int32 * 4)]
+                ("size_limits", C.c_int32 * 4), ("channel_wave", C.c_int32 * 4)]
 
 
 class KernelInfo(C.Structure):

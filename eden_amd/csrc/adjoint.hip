@@ -904,7 +904,7 @@ const void *bwd_kernel(int tpb) {
 }
 
 int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
-    if (d->abi_version != 1 && d->abi_version != EDENHIP_ABI_VERSION)
+    if (d->abi_version < 1 || d->abi_version > EDENHIP_ABI_VERSION)
         return afail(a, EDENHIP_E_INVALID, "descriptor ABI version mismatch");
     if (d->n_cells <= 0 || d->n_comps <= 0 || !(d->dt > 0.0f)) return afail(a, EDENHIP_E_INVALID, "empty model or dt <= 0");
     const int64_t NC = d->n_comps;

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <thread>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -258,7 +259,7 @@ static int pool_stream(edenhip_ctx *ctx, int slot, hipStream_t *out) {
 }
 
 static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
-    if (d->abi_version != 1 && d->abi_version != EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
+    if (d->abi_version < 1 || d->abi_version > EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
     const edenhip_lems_synapses *LS = d->abi_version >= 2 ? d->lems : nullptr;
     const bool lems = LS && LS->n_types > 0;
     if (lems) {
@@ -799,8 +800,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // 2 and 4 cable waves the extra wave costs a workgroup per CU, which those sizes
         // need more.
         // Networks run one step per launch with the NET kernels, which have no channel wave.
-        b.split = (b.tpb == 512 || (b.tpb == 64 && (int64_t)bc.size() <= 4 * (int64_t)ctx->n_cu)) &&
-                  ctx->opt.persistent >= 0 && !ctx->tree_net;
+        // channel_wave overrides the rule per workgroup size
+        const int cw = ctx->opt.channel_wave[b.tpb == 64 ? 0 : b.tpb == 128 ? 1 : b.tpb == 256 ? 2 : 3];
+        const bool rule = b.tpb == 512 || (b.tpb == 64 && (int64_t)bc.size() <= 4 * (int64_t)ctx->n_cu);
+        b.split = (cw > 0 || (cw == 0 && rule)) && ctx->opt.persistent >= 0 && !ctx->tree_net;
         // complex-compartment records go to LDS when they take at most 20 KiB
         b.cxmax_lds = b.cxmax <= 256 ? b.cxmax : 0;
         b.lds = (size_t)tree_lds(b.nmax, b.nvmax, b.ngmax, b.nctmax, b.ninmax, b.words_max, b.cxmax_lds, ctx->n_gates,
@@ -1202,7 +1205,9 @@ int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, e
     *out = nullptr;
     edenhip_ctx *ctx = new (std::nothrow) edenhip_ctx();
     if (!ctx) return EDENHIP_E_INVALID;
-    if (opt) ctx->opt = *opt;
+    // options of version 1 and 2 callers end before channel_wave
+    if (opt) std::memcpy(&ctx->opt, opt, desc && desc->abi_version >= 3 ? sizeof(edenhip_options)
+                                                                         : offsetof(edenhip_options, channel_wave));
     if (ctx->opt.persistent == 0) ctx->opt.persistent = 1;
     ctx->device = ctx->opt.device;
     *out = ctx;
@@ -1212,6 +1217,9 @@ int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, e
         if (L < 0 || L > 4 * (64 << k) || (k > 0 && L != 0 && L < ctx->opt.size_limits[k - 1]))
             return fail(ctx, EDENHIP_E_INVALID, "size_limits: each in [0, 4 x threads], nondecreasing");
     }
+    for (int k = 0; k < 4; k++)
+        if (ctx->opt.channel_wave[k] < -1 || ctx->opt.channel_wave[k] > 1)
+            return fail(ctx, EDENHIP_E_INVALID, "channel_wave: each -1, 0 or 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ctx, EDENHIP_E_DEVICE, "no HIP device visible");
     HIPCHECK(ctx, hipSetDevice(ctx->device));

@@ -838,7 +838,9 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
         int32_t, int32_t, int32_t, int64_t, int64_t, int64_t, float);
 EDENHIP_TREE_INST(64, true, false)
 EDENHIP_TREE_INST(64, false, false)
+EDENHIP_TREE_INST(128, true, false)
 EDENHIP_TREE_INST(128, false, false)
+EDENHIP_TREE_INST(256, true, false)
 EDENHIP_TREE_INST(256, false, false)
 EDENHIP_TREE_INST(512, true, false)
 EDENHIP_TREE_INST(512, false, false)
@@ -861,6 +863,8 @@ const void *tree_kernel_ptr(int tpb, bool split, bool net) {
     if (split) {
         switch (tpb) {
         case 64: return reinterpret_cast<const void *>(&tree_kernel<64, true, false>);
+        case 128: return reinterpret_cast<const void *>(&tree_kernel<128, true, false>);
+        case 256: return reinterpret_cast<const void *>(&tree_kernel<256, true, false>);
         case 512: return reinterpret_cast<const void *>(&tree_kernel<512, true, false>);
         default: return nullptr;
         }

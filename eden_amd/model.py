@@ -250,13 +250,14 @@ class Simulation:
     """One model on one GPU. Rows are recorded steps (state at t = row * dt)."""
 
     def __init__(self, model, threads_per_cell=0, persistent=1, chunk_steps=0, profile=False, device=0,
-                 log_synapse_events=0, network_window=0, size_limits=None, use_graph=False):
+                 log_synapse_events=0, network_window=0, size_limits=None, use_graph=False, channel_wave=None):
         self.lib = N.load_library()
         self.model = model
         self._desc = model.desc()
         limits = (C.c_int32 * 4)(*(size_limits or (0, 0, 0, 0)))
+        waves = (C.c_int32 * 4)(*(channel_wave or (0, 0, 0, 0)))
         opt = N.Options(threads_per_cell, persistent, chunk_steps, 1 if use_graph else 0, 1 if profile else 0, device,
-                        log_synapse_events, network_window, limits)
+                        log_synapse_events, network_window, limits, waves)
         h = C.c_void_p()
         rc = self.lib.edenhip_create(C.byref(self._desc), C.byref(opt), C.byref(h))
         self.h = h

@@ -29,7 +29,9 @@
 extern "C" {
 #endif
 
-#define EDENHIP_ABI_VERSION 2   /* 2 adds edenhip_model_desc.lems; version 1 descriptors are accepted */
+#define EDENHIP_ABI_VERSION 3   /* 2 adds edenhip_model_desc.lems, 3 adds edenhip_options.channel_wave;
+                                   descriptors of versions 1 and 2 are accepted (their options are
+                                   read without channel_wave) */
 
 enum {
     EDENHIP_OK = 0,
@@ -200,6 +202,14 @@ typedef struct edenhip_options {
                                      the first size whose limit holds it (each limit at most 4
                                      compartments per thread; the 512 limit must be 2047).
                                      eden_amd.autotune picks them by timing short runs. */
+    int32_t channel_wave[4];      /* (abi_version >= 3) tree cells with 64, 128, 256, 512 threads:
+                                     0 = by the rule (512: on; 64: on when the bucket has at most
+                                     4 cells per CU; 128 and 256: off), 1 = on, -1 = off. On:
+                                     one extra wave per workgroup updates the gates and open
+                                     fractions for the next step while the cable waves run the
+                                     step. Applies to persistent launches of models without
+                                     synapses or spike recipients on tree cells.
+                                     eden_amd.autotune times both settings per size. */
 } edenhip_options;
 
 typedef struct edenhip_ctx edenhip_ctx;

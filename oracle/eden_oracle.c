@@ -500,7 +500,7 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
 int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, int32_t n_threads,
                 float *trace, int64_t *ev_col, int64_t *ev_row, int64_t max_ev, int64_t *n_ev,
                 float *final_v) {
-    if (!d || (d->abi_version != 1 && d->abi_version != EDENHIP_ABI_VERSION) || substeps < 1) return -1;
+    if (!d || d->abi_version < 1 || d->abi_version > EDENHIP_ABI_VERSION || substeps < 1) return -1;
     const int lems = d->abi_version >= 2 && d->lems && d->lems->n_types > 0;
 #if defined(ORACLE_F64)
     if (lems) return -1; /* generic synapses: fp32 flavours only */

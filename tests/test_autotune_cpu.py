@@ -70,3 +70,51 @@ def test_point_cells_only_skip_the_search(monkeypatch):
     monkeypatch.setattr(autotune, "time_limits", fail)
     limits, report = autotune.autotune(configs.hh_population(n_cells=10, t_final_ms=1.0))
     assert limits == autotune.DEFAULT and report == []
+
+
+def _tune_waves(monkeypatch, timing, rule=(True, False, None, True), margin=0.02, baseline=None):
+    calls = []
+
+    def fake(model, limits, steps=None, repeats=2, device=0, channel_wave=None):
+        calls.append(tuple(channel_wave))
+        return timing(tuple(channel_wave))
+
+    monkeypatch.setattr(autotune, "time_limits", fake)
+    monkeypatch.setattr(autotune, "rule_channel_wave", lambda model, limits, device=0: list(rule))
+    waves, report = autotune.autotune_channel_wave(_model(), autotune.DEFAULT, margin=margin, baseline=baseline)
+    assert [w for w, _ in report] == calls
+    return waves, calls
+
+
+def test_channel_wave_search_flips_each_used_size_once(monkeypatch):
+    waves, calls = _tune_waves(monkeypatch, lambda w: 1.0)
+    assert waves == (0, 0, 0, 0)
+    # the rule, then each size with cells flipped against the rule's choice; the 256 size has
+    # no cells and is not timed
+    assert calls == [(0, 0, 0, 0), (-1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 0, -1)]
+
+
+def test_channel_wave_flip_kept_when_faster_beyond_the_margin(monkeypatch):
+    def timing(w):
+        return 0.9 if w[1] == 1 else (0.995 if w[0] == -1 else 1.0)
+
+    waves, calls = _tune_waves(monkeypatch, timing)
+    assert waves == (0, 1, 0, 0)          # 10 % faster kept, 0.5 % faster is noise
+    # later flips are timed on top of the kept one
+    assert calls[-1] == (0, 1, 0, -1)
+
+
+def test_channel_wave_baseline_not_timed_again(monkeypatch):
+    waves, calls = _tune_waves(monkeypatch, lambda w: 1.0, baseline=1.0)
+    assert (0, 0, 0, 0) not in calls
+
+
+def test_channel_wave_point_cells_only_skip_the_search(monkeypatch):
+    def fail(*a, **k):
+        raise AssertionError("no timing for one-compartment cells")
+
+    monkeypatch.setattr(autotune, "time_limits", fail)
+    monkeypatch.setattr(autotune, "rule_channel_wave", fail)
+    waves, report = autotune.autotune_channel_wave(configs.hh_population(n_cells=10, t_final_ms=1.0),
+                                                   autotune.DEFAULT)
+    assert waves == (0, 0, 0, 0) and report == []

@@ -27,3 +27,11 @@ def test_autotune_small_batch():
     with Simulation(m) as a, Simulation(m, size_limits=limits) as b:
         ta, tb = a.run_all(), b.run_all()
     np.testing.assert_array_equal(ta, tb)
+    # channel waves: the rule's choice per workgroup size, then each used size flipped
+    waves, wreport = autotune.autotune_channel_wave(m, limits, steps=60, repeats=1)
+    print(f"[autotune] channel waves {waves} from {len(wreport)} timings", flush=True)
+    rule = autotune.rule_channel_wave(m, limits)
+    assert len(wreport) == 1 + sum(r is not None for r in rule)
+    assert all(w == 0 or rule[k] is not None for k, w in enumerate(waves))
+    with Simulation(m, size_limits=limits, channel_wave=waves) as c:
+        np.testing.assert_array_equal(c.run_all(), ta)

@@ -176,10 +176,28 @@ def test_size_limits_agree():
     assert res["raster_mismatch_cols"] == 0, res
 
 
+@pytest.mark.parametrize("wave", [1, -1])
+@pytest.mark.parametrize("hh", [False, True])
+def test_channel_wave_settings_agree(wave, hh):
+    """Every workgroup size with the separate channel wave forced on, then off (the kernels'
+    other variants; same operations), HH on the soma only and on every compartment."""
+    sizes = [40, 200, 400, 800, 1500]
+    m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=15.0, seed=37, hh_everywhere=hh)
+    with Simulation(m, profile=True, channel_wave=(wave,) * 4) as sim:
+        names = sorted(k["name"] for k in sim.kernel_infos())
+    _say(f"[parity] channel_wave {wave}: {names}")
+    flag = "true" if wave > 0 else "false"
+    assert names and all(n.split(", ")[1] == flag for n in names)
+    res, g, r = _check(m, channel_wave=(wave,) * 4)
+    assert res["raster_mismatch_cols"] == 0, res
+
+
 def test_size_limits_rejected_when_invalid():
     m = configs.random_trees(2, sizes=[30, 40], t_final_ms=1.0, seed=3)
     with pytest.raises(Exception, match="size_limits"):
         Simulation(m, size_limits=(300, 0, 0, 0))
+    with pytest.raises(Exception, match="channel_wave"):
+        Simulation(m, channel_wave=(2, 0, 0, 0))
 
 
 @pytest.mark.parametrize("tpb", [64, 128, 256, 512])
