@@ -123,6 +123,7 @@ EXPORTS = {
     "edenhip_last_error": (C.c_char_p, [C.c_void_p]),
     "edenhip_destroy": (None, [C.c_void_p]),
     "edenhip_cable_setup": (C.c_int, [C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "edenhip_cable_setup_slices": (C.c_int, [C.c_int32, _P, _P, _P, _P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -172,3 +173,27 @@ def cable_setup(parent, proximal, distal, cm, ra):
     if rc != 0:
         raise ValueError(f"edenhip_cable_setup failed ({rc}): malformed tree or zero axial resistance")
     return area, cap, rax, inv
+
+
+def cable_setup_slices(proximal, distal, cm, ra, parent, slice_begin, slice_seg, slice_from, slice_to):
+    """Per-compartment (area, C, R to parent, 1/RC) of one cell whose compartments are made of
+    segment slices; see include/edenhip_setup.h."""
+    lib = load_library()
+    ns = len(cm)
+    n = len(parent)
+    proximal = np.ascontiguousarray(proximal, dtype=np.float32).reshape(ns, 4)
+    distal = np.ascontiguousarray(distal, dtype=np.float32).reshape(ns, 4)
+    cm = np.ascontiguousarray(cm, dtype=np.float32)
+    ra = np.ascontiguousarray(ra, dtype=np.float32)
+    parent = np.ascontiguousarray(parent, dtype=np.int32)
+    slice_begin = np.ascontiguousarray(slice_begin, dtype=np.int32)
+    slice_seg = np.ascontiguousarray(slice_seg, dtype=np.int32)
+    slice_from = np.ascontiguousarray(slice_from, dtype=np.float32)
+    slice_to = np.ascontiguousarray(slice_to, dtype=np.float32)
+    out = [np.empty(n, np.float32) for _ in range(4)]
+    rc = lib.edenhip_cable_setup_slices(ns, ptr(proximal), ptr(distal), ptr(cm), ptr(ra), n, ptr(parent),
+                                        ptr(slice_begin), ptr(slice_seg), ptr(slice_from), ptr(slice_to),
+                                        *[ptr(o) for o in out])
+    if rc != 0:
+        raise ValueError(f"edenhip_cable_setup_slices failed ({rc}): malformed compartments or zero axial resistance")
+    return tuple(out)

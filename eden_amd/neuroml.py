@@ -15,8 +15,13 @@ set covers:
                     points, segmentGroups with member / include), biophysicalProperties
                     (channelDensity, specificCapacitance, initMembPotential, spikeThresh,
                     resistivity), each over a segment group, later specs overriding earlier
-                    ones (eden/NeuroML.cpp:2530-3566); one compartment per segment (EDEN's
-                    free-standing segments, eden/Eden.cpp:1437-1491)
+                    ones (eden/NeuroML.cpp:2530-3566)
+  discretisation    a cable segment group (neuroLexId sao864921383) becomes
+                    numberInternalDivisions compartments of equal path length, any other
+                    segment one compartment; mechanisms and V0 apply to every compartment a
+                    spec's segments overlap; inputs, connections and recordings go to the
+                    compartment holding their segment's fractionAlong (0.5 when absent)
+                    (eden/Eden.cpp:1387-1772, 1015-1038, 4511-4560)
   synapses          expTwoSynapse natively; expOneSynapse, alphaSynapse, alphaCurrentSynapse,
                     expThreeSynapse and the PyNN synapses (expCond, expCurr, alphaCond,
                     alphaCurr) as generic LEMS types (eden_amd.lems), compiled by the library
@@ -39,8 +44,8 @@ read (IdListRle is a sorted set of component sequence numbers, eden/Eden.cpp:186
 ParseQuantity does: the number read as a double, times 10^(unit power - native power), then
 stored as float (eden/NeuroML.cpp:3046-3077, eden/NeuroML.h:530-532).
 
-Constructs outside this set (cables, other channel and synapse types, artificial cells,
-continuous / electrical projections) raise NeuroMLError naming the element.
+Constructs outside this set (other channel and synapse types, non-uniform channel densities,
+artificial cells, continuous / electrical projections) raise NeuroMLError naming the element.
 """
 import os
 import re
@@ -298,26 +303,39 @@ def _channel(el, temperature):
 @dataclass
 class _Cell:
     id: str
-    seg_ids: list            # segment id per compartment (segments in file order, parents first)
+    seg_ids: list            # segment ids in file order (parents first)
     parent: np.ndarray       # compartment parent (cell-local), -1 for the root
-    proximal: np.ndarray     # [n, 4]
-    distal: np.ndarray
-    groups: dict             # group id -> set of compartment indices
+    seg_map: list            # per segment: [(compartment, fractionAlong where it ends on the segment)]
     channels: list           # per compartment: [(channel id, gbase mS/cm^2, erev mV)]
-    cm: np.ndarray
-    ra: np.ndarray
-    v0: np.ndarray
+    v0: np.ndarray           # per compartment
     vthr: np.ndarray
     area: np.ndarray = None
     cap: np.ndarray = None
     r_axial: np.ndarray = None
     inv_rc: np.ndarray = None
 
-    def comp_of_segment(self, seg_id, what):
+    @property
+    def n_comp(self):
+        return len(self.parent)
+
+    def comp_of_segment(self, seg_id, what, fraction_along=0.5):
+        """The compartment that holds a location on a segment: the first whose span on the
+        segment ends at or beyond it (GetCompartmentForSegmentLocation, eden/Eden.cpp:1015-1038)."""
         try:
-            return self.seg_ids.index(int(seg_id))
+            k = self.seg_ids.index(int(seg_id))
         except ValueError:
             raise NeuroMLError(f"{what}: segment {seg_id} not in cell {self.id}") from None
+        return _comp_at(self.seg_map, k, fraction_along)
+
+
+def _comp_at(seg_map, k, fraction_along):
+    f = np.float32(fraction_along)
+    for comp, f_end in seg_map[k]:
+        if f <= f_end:
+            return comp
+    if not seg_map[k]:
+        raise NeuroMLError(f"segment index {k} is in no compartment")
+    return seg_map[k][-1][0]
 
 
 def _point(el):
@@ -332,6 +350,7 @@ def _morphology(cid, m):
     index = {sid: k for k, sid in enumerate(ids)}
     n = len(segs)
     parent = np.full(n, -1, np.int32)
+    frac = np.ones(n, np.float32)
     prox = np.zeros((n, 4))
     dist = np.zeros((n, 4))
     for k, s in enumerate(segs):
@@ -347,6 +366,9 @@ def _morphology(cid, m):
                 raise NeuroMLError(f"cell {cid} segment {ids[k]}: parent must be a segment listed before it")
             parent[k] = p
             f = convert(pe.get("fractionAlong", "1"), None, "fractionAlong")
+            if not 0 <= f <= 1:
+                raise NeuroMLError(f"cell {cid} segment {ids[k]}: fractionAlong {f} outside [0, 1]")
+            frac[k] = f
             if pr is None:
                 # the implicit proximal point lies fractionAlong along the parent
                 prox[k] = prox[p] + (dist[p] - prox[p]) * f
@@ -358,14 +380,14 @@ def _morphology(cid, m):
             raise NeuroMLError(f"cell {cid} segment {ids[k]}: a root segment needs a proximal point")
     groups = {"all": set(range(n))}
     raw = {}
+    cable_ids = []
     for g in m:
         if _tag(g) != "segmentGroup":
             continue
         gid = g.get("id")
-        if g.get("neuroLexId") == "sao864921383":
-            raise NeuroMLError(f"cell {cid}: segment group {gid} is a cable (sao864921383); cable discretisation "
-                               "is not supported by this reader (one compartment per segment)")
         raw[gid] = g
+        if g.get("neuroLexId") == "sao864921383":
+            cable_ids.append(gid)
 
     def resolve(gid, stack=()):
         if gid in groups:
@@ -389,7 +411,122 @@ def _morphology(cid, m):
 
     for gid in raw:
         resolve(gid)
-    return ids, parent, prox.astype(np.float32), dist.astype(np.float32), groups
+    # cables: the group's segments in file order, and its number of compartments (the
+    # numberInternalDivisions property, 1 when absent; eden/NeuroML.cpp:2709-2860)
+    cables = []
+    for gid in cable_ids:
+        nseg = 1
+        for c in raw[gid]:
+            if _tag(c) == "property" and c.get("tag") == "numberInternalDivisions":
+                nseg = int(c.get("value"))
+                if nseg < 1:
+                    raise NeuroMLError(f"cell {cid} group {gid}: numberInternalDivisions must be at least 1")
+        cables.append((gid, sorted(groups[gid]), nseg))
+    for g in raw.values():
+        if g.get("neuroLexId") != "sao864921383":
+            for c in g:
+                if _tag(c) == "property" and c.get("tag") == "numberInternalDivisions":
+                    raise NeuroMLError(f"cell {cid}: group {g.get('id')} has numberInternalDivisions but is not a "
+                                       "cable (neuroLexId sao864921383)")
+    return ids, parent, frac, prox.astype(np.float32), dist.astype(np.float32), groups, cables
+
+
+def _discretise(cid, ids, parent, frac, prox, dist, cables):
+    """Compartments of a cell (eden/Eden.cpp:1387-1772): every cable group becomes nseg
+    compartments of equal path length, allotted when its first segment comes up in segment
+    order; every other segment is one compartment. Float arithmetic as the reference's (Real
+    is float). Returns (compartment parents, per compartment its slices [(segment, from, to)],
+    per segment [(compartment, fractionAlong where it ends)])."""
+    F = np.float32
+    n = len(ids)
+    cable_of = [-1] * n
+    for ci, (gid, segs, nseg) in enumerate(cables):
+        for s_ in segs:
+            if cable_of[s_] >= 0:
+                raise NeuroMLError(f"cell {cid}: cables {cables[cable_of[s_]][0]} and {gid} overlap on segment {ids[s_]}")
+            cable_of[s_] = ci
+    free_comp = [-1] * n
+    cable_off = [-1] * len(cables)
+    ncomp = 0
+    for s_ in range(n):
+        ci = cable_of[s_]
+        if ci >= 0:
+            if cable_off[ci] < 0:
+                cable_off[ci] = ncomp
+                ncomp += cables[ci][2]
+        else:
+            free_comp[s_] = ncomp
+            ncomp += 1
+    slices = [[] for _ in range(ncomp)]
+    seg_map = [[] for _ in range(n)]
+
+    def add(comp, s_, f0, f1):
+        slices[comp].append((s_, F(f0), F(f1)))
+        seg_map[s_].append((comp, F(f1)))
+
+    def seg_len(s_):
+        d = dist[s_, :3] - prox[s_, :3]
+        return F(np.sqrt(F(F(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])))
+
+    for ci, (gid, segs, nseg) in enumerate(cables):
+        cum = []
+        total = F(0)
+        for i, s_ in enumerate(segs):
+            if i > 0:
+                if parent[s_] != segs[i - 1]:
+                    raise NeuroMLError(f"cell {cid} cable {gid}: not an unbranched cable (segment {ids[s_]})")
+                if parent[s_] >= 0 and frac[s_] != 1:
+                    raise NeuroMLError(f"cell {cid} cable {gid}: segment {ids[s_]} is not attached to the end of "
+                                       "its parent")
+            total = F(total + seg_len(s_))
+            cum.append(total)
+
+        def at(si, q):
+            a = cum[si - 1] if si > 0 else F(0)
+            b = cum[si]
+            f = F(F(q - a) / F(b - a)) if b != a else F(np.nan)
+            if abs(float(F(b - a))) < 0.001:
+                f = F(0)
+            return f
+
+        under = 0
+        for k in range(nseg):
+            start = F(F(F(k) / F(nseg)) * total)
+            end = F(F(F(k + 1) / F(nseg)) * total)
+            comp = cable_off[ci] + k
+            idx = start
+            while under < len(cum) and cum[under] <= end:
+                fa = at(under, idx)
+                if not abs(float(F(fa - F(1)))) < 0.0001:
+                    add(comp, segs[under], fa, 1.0)
+                idx = cum[under]
+                under += 1
+            if under < len(cum) and end < cum[under]:
+                f0, f1 = at(under, idx), at(under, end)
+                if not abs(float(F(f1 - f0))) < 0.0001:
+                    add(comp, segs[under], f0, f1)
+    for s_ in range(n):
+        if free_comp[s_] >= 0:
+            add(free_comp[s_], s_, 0.0, 1.0)
+    cparent = np.full(ncomp, -1, np.int32)
+    for s_ in range(n):
+        if free_comp[s_] >= 0 and parent[s_] >= 0:
+            cparent[free_comp[s_]] = _comp_at(seg_map, parent[s_], frac[s_])
+    for ci, (gid, segs, nseg) in enumerate(cables):
+        c0 = cable_off[ci]
+        for c in range(c0 + 1, c0 + nseg):
+            cparent[c] = c - 1
+        if not slices[c0]:
+            raise NeuroMLError(f"cell {cid} cable {gid}: its first compartment holds no segment")
+        first = slices[c0][0][0]
+        if parent[first] >= 0:
+            cparent[c0] = _comp_at(seg_map, parent[first], frac[first])
+    for c in range(ncomp):
+        if not slices[c]:
+            raise NeuroMLError(f"cell {cid}: compartment {c} holds no segment")
+        if (c == 0) != (cparent[c] < 0) or cparent[c] >= c:
+            raise NeuroMLError(f"cell {cid}: the compartments do not list parents before children")
+    return cparent, slices, seg_map
 
 
 def _cell(doc, el, temperature, channel_index):
@@ -404,13 +541,18 @@ def _cell(doc, el, temperature, channel_index):
         b = doc.biophysics.get(el.get("biophysicalProperties"))
     if b is None:
         raise NeuroMLError(f"cell {cid}: no biophysicalProperties")
-    ids, parent, prox, dist, groups = _morphology(cid, m)
+    ids, parent, frac, prox, dist, groups, cables = _morphology(cid, m)
     n = len(ids)
-    chans = [[] for _ in range(n)]
+    cparent, slices, seg_map = _discretise(cid, ids, parent, frac, prox, dist, cables)
+    ncomp = len(cparent)
+    # capacitance and resistivity per segment; channels, V0 and threshold per compartment,
+    # each spec over every compartment its segments overlap (SpecToCompList,
+    # eden/Eden.cpp:4511-4560)
+    chans = [[] for _ in range(ncomp)]
     cm = np.full(n, np.nan)
     ra = np.full(n, np.nan)
-    v0 = np.full(n, np.nan)
-    vthr = np.full(n, 0.0)    # NeuroML's default spike threshold, 0 mV
+    v0 = np.full(ncomp, np.nan)
+    vthr = np.full(ncomp, 0.0)    # NeuroML's default spike threshold, 0 mV
 
     def where(spec):
         if spec.get("segment") is not None:
@@ -423,6 +565,9 @@ def _cell(doc, el, temperature, channel_index):
             raise NeuroMLError(f"cell {cid}: segment group {g} not found")
         return sorted(groups[g])
 
+    def comps(spec):
+        return sorted({c for k in where(spec) for c, _ in seg_map[k]})
+
     for part in b:
         pt = _tag(part)
         if pt == "membraneProperties":
@@ -434,16 +579,16 @@ def _cell(doc, el, temperature, channel_index):
                         raise NeuroMLError(f"cell {cid}: ion channel {ch} not found or not supported")
                     g = convert(s.get("condDensity"), "conductivity", "condDensity")
                     e = convert(s.get("erev"), "voltage", "erev")
-                    for k in where(s):
+                    for k in comps(s):
                         chans[k].append((ch, _f32(g), _f32(e)))
                 elif st == "specificCapacitance":
                     for k in where(s):
                         cm[k] = convert(s.get("value"), "specific_capacitance", "specificCapacitance")
                 elif st == "initMembPotential":
-                    for k in where(s):
+                    for k in comps(s):
                         v0[k] = convert(s.get("value"), "voltage", "initMembPotential")
                 elif st == "spikeThresh":
-                    for k in where(s):
+                    for k in comps(s):
                         vthr[k] = convert(s.get("value"), "voltage", "spikeThresh")
                 elif st in ("notes", "annotation"):
                     pass
@@ -465,11 +610,18 @@ def _cell(doc, el, temperature, channel_index):
         raise NeuroMLError(f"cell {cid}: initMembPotential missing on some segments")
     if n > 1 and np.isnan(ra).any():
         raise NeuroMLError(f"cell {cid}: resistivity missing on some segments")
-    ra = np.where(np.isnan(ra), 0.0, ra)
-    c = _Cell(cid, ids, parent, prox, dist, groups, chans, cm.astype(np.float32), ra.astype(np.float32),
-              v0.astype(np.float32), vthr.astype(np.float32))
+    ra = np.where(np.isnan(ra), 0.0, ra).astype(np.float32)
+    cm = cm.astype(np.float32)
+    c = _Cell(cid, ids, cparent, seg_map, chans, v0.astype(np.float32), vthr.astype(np.float32))
     try:
-        c.area, c.cap, c.r_axial, c.inv_rc = N.cable_setup(parent, prox, dist, c.cm, c.ra)
+        if not cables:
+            c.area, c.cap, c.r_axial, c.inv_rc = N.cable_setup(parent, prox, dist, cm, ra)
+        else:
+            sb = np.zeros(ncomp + 1, np.int32)
+            np.cumsum([len(x) for x in slices], out=sb[1:])
+            flat = [x for lst in slices for x in lst]
+            c.area, c.cap, c.r_axial, c.inv_rc = N.cable_setup_slices(
+                prox, dist, cm, ra, cparent, sb, [x[0] for x in flat], [x[1] for x in flat], [x[2] for x in flat])
     except ValueError as e:
         raise NeuroMLError(f"cell {cid}: {e}") from None
     return c
@@ -615,16 +767,22 @@ def read(path, rng_seed=None):
         for _ in p.inst_ids:
             comp0.append(nc)
             pop_of_cell.append(p)
-            nc += len(p.cell.seg_ids)
+            nc += p.cell.n_comp
     comp0.append(nc)
     cell_begin = np.array(comp0, np.int64)
 
-    def comp(pop_id, inst_id, seg_id, what):
+    def comp(pop_id, inst_id, seg_id, what, fraction_along=0.5):
         if pop_id not in pops:
             raise NeuroMLError(f"{what}: population {pop_id} not found")
         p = pops[pop_id]
         c = p.first_cell + p.seq(inst_id, what)
-        return int(cell_begin[c]) + p.cell.comp_of_segment(seg_id, what)
+        return int(cell_begin[c]) + p.cell.comp_of_segment(seg_id, what, fraction_along)
+
+    def frac(el, attr):
+        f = convert(el.get(attr, "0.5"), None, attr)
+        if not 0 <= f <= 1:
+            raise NeuroMLError(f"{attr} {f} outside [0, 1]")
+        return f
 
     # synapse types in reading order (their component sequence numbers). Models whose
     # projections use only expTwoSynapse run it natively; otherwise every synapse element
@@ -674,8 +832,8 @@ def read(path, rng_seed=None):
         t = _tag(el)
         if t == "explicitInput":
             pop, inst = _cell_ref(el.get("target"))
-            inputs.append((el.get("input"), comp(pop, inst, int(el.get("segmentId", "0")), "explicitInput"),
-                           1.0))
+            inputs.append((el.get("input"), comp(pop, inst, int(el.get("segmentId", "0")), "explicitInput",
+                                                 frac(el, "fractionAlong")), 1.0))
         elif t == "inputList":
             pop = el.get("population")
             for i in el:
@@ -684,7 +842,8 @@ def read(path, rng_seed=None):
                     continue
                 _, inst = _cell_ref(i.get("target"))
                 w = convert(i.get("weight"), None, "inputW weight") if it == "inputW" else 1.0
-                inputs.append((el.get("component"), comp(pop, inst, int(i.get("segmentId", "0")), "input"), w))
+                inputs.append((el.get("component"), comp(pop, inst, int(i.get("segmentId", "0")), "input",
+                                                         frac(i, "fractionAlong")), w))
         elif t == "projection":
             pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
             st = synapse(el.get("synapse"), f"projection {el.get('id')}")
@@ -694,8 +853,8 @@ def read(path, rng_seed=None):
                     continue
                 _, pi = _cell_ref(c.get("preCellId"))
                 _, qi = _cell_ref(c.get("postCellId"))
-                a = comp(pre, pi, int(c.get("preSegmentId", "0")), "connection")
-                b = comp(post, qi, int(c.get("postSegmentId", "0")), "connection")
+                a = comp(pre, pi, int(c.get("preSegmentId", "0")), "connection", frac(c, "preFractionAlong"))
+                b = comp(post, qi, int(c.get("postSegmentId", "0")), "connection", frac(c, "postFractionAlong"))
                 w = convert(c.get("weight"), None, "weight") if ct == "connectionWD" else 1.0
                 d = convert(c.get("delay"), "time", "delay") if ct == "connectionWD" else 0.0
                 conns.append((a, b, st, w, d))

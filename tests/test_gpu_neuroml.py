@@ -4,6 +4,8 @@ printer gives for the oracle's run of the same instantiated model (tests/eden_te
 
 * the reference's single-compartment HH example (tests/golden/neuroml), by run_lems, by
   runEden (the reference's return shape) and by the command line;
+* the reference's cable discretisation model (tests/golden/neuroml), whose cable groups the
+  reader cuts into compartments;
 * synthetic workloads written as LEMS (config 1's construction, a tree-cell / point-cell
   network with synapses on dendrites): the GPU run of the file equals the GPU run of the
   configs model it was written from, and the oracle's run of the model read back.
@@ -55,6 +57,28 @@ def test_reference_hh_example_files(tmp_path):
     assert n_spikes > 3
     for name, text in want.items():
         assert_same_text((tmp_path / name).read_text(), text)
+
+
+def test_reference_cable_discretisation_files(tmp_path):
+    """The reference's EdenTest_CableDiscretisation model (cable groups cut into compartments
+    by the reader): output file byte-equal to the oracle's run of the same model; the single
+    segment cut in five, the five segments in a five-compartment cable and the five free
+    segments give identical voltages (their compartments are identical), which respond to the
+    current steps; the merged cable's three recording points are one compartment."""
+    path = os.path.join(HERE, "golden", "neuroml", "LEMS_EdenTest_CableDiscretisation.xml")
+    res = run_lems(path, out_dir=str(tmp_path))
+    m, outs = res["model"], res["outputs"]
+    _say(f"[neuroml] cable discretisation: {m.n_comps} compartments, {res['rows']} rows, run {res['run_s']:.3f} s")
+    want, _ = _expected_files(m, outs)
+    for name, text in want.items():
+        assert_same_text((tmp_path / name).read_text(), text)
+    with Simulation(m) as sim:
+        tr = sim.run_all()
+    assert np.array_equal(tr[:, 0], tr[:, 1]) and np.array_equal(tr[:, 0], tr[:, 2])
+    sub, expl, eff = tr[:, 3], tr[:, 4:9], tr[:, 9:14]
+    assert np.array_equal(expl, eff)
+    assert np.array_equal(sub, expl[:, 2])
+    assert float(np.ptp(expl[:, 0])) > 1.0
 
 
 def test_runeden_return_shape(tmp_path, monkeypatch):

@@ -12,6 +12,10 @@ command-line front end, on the CPU.
   does not change results, flags being idempotent).
 * The reference's full EdenTest_CoreSynapses model, which holds constructs outside the
   per-step loop's mechanism set, is refused with an error naming the first of them.
+* Cable discretisation: the reference's EdenTest_CableDiscretisation model gives identical
+  compartments for a segment cut in five and five segments; tapering cables of uneven
+  segments cut into 1, 3 and 7 compartments match the cone-section closed forms; locations
+  by fractionAlong.
 """
 import dataclasses
 import os
@@ -120,3 +124,107 @@ def test_cli_reports_missing_model_and_unknown_tokens(capsys):
     assert 'skipping unknown token "frobnicate"' in out
     assert "NeuroML model not selected" in out
     assert main(["nml", "/nonexistent/model.xml"]) == 1
+
+
+# --- cable discretisation (segment groups with neuroLexId sao864921383) -------------------
+
+def test_reference_cable_discretisation_model():
+    """The reference's EdenTest_CableDiscretisation model (input data under tests/golden/):
+    one 1000 um segment cut into 5 compartments by its cable group, five 200 um segments
+    in a 5-compartment cable, and the same five segments with no cable give identical
+    compartments, inputs on the same compartments (the inputs' fractionAlong 0.1 and 0.7 on
+    the single segment fall in its first and fourth fifths), and recordings at fractionAlong
+    0.5; the three segments of the default (nseg 1) cable merge into one compartment whose
+    area is their summed frustum area."""
+    m, outs = neuroml.read(os.path.join(GOLD, "LEMS_EdenTest_CableDiscretisation.xml"))
+    cb = m.cell_comp_begin
+    assert list(m.comps_per_cell()) == [1, 5, 5, 5]
+    span = [slice(cb[c], cb[c + 1]) for c in range(4)]
+    for arr in (m.area, m.capacitance, m.inv_rc_axial, m.parent, np.nan_to_num(m.r_axial)):
+        a = np.asarray(arr)
+        assert np.array_equal(a[span[1]], a[span[2]]) and np.array_equal(a[span[1]], a[span[3]])
+    assert np.allclose(np.asarray(m.area)[span[1]], np.pi * 1.0 * 200.0, rtol=1e-6)     # d = 1 um cylinders
+    pulses = np.repeat(np.arange(m.n_comps), np.diff(m.comp_pulse_begin))
+    assert list(pulses) == [0, 0, 1, 4, 6, 9, 11, 14]
+    assert list(m.record_comp) == [0, 0, 0, 3, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15]
+    # the merged cable's area: the three segments' frustum surfaces
+    import xml.etree.ElementTree as ET
+    root = ET.parse(os.path.join(GOLD, "EdenTest_CableDiscretisation.nml")).getroot()
+    cell = next(c for c in root if c.tag.endswith("cell") and c.get("id") == "Cable_Unified")
+    segs = [s for s in cell.iter() if s.tag.endswith("}segment")]
+    pts, total = {}, 0.0
+    for s in segs:
+        get = lambda tag: next((c for c in s if c.tag.endswith(tag)), None)
+        d = get("distal")
+        p = get("proximal")
+        par = get("parent")
+        q = [float(d.get(k)) for k in ("x", "y", "z", "diameter")]
+        a = [float(p.get(k)) for k in ("x", "y", "z", "diameter")] if p is not None else pts[par.get("segment")]
+        pts[s.get("id")] = q
+        L = np.linalg.norm(np.subtract(q[:3], a[:3]))
+        total += np.pi / 2 * (a[3] + q[3]) * np.sqrt(((a[3] - q[3]) / 2) ** 2 + L * L)
+    assert np.isclose(float(m.area[0]), total, rtol=1e-6)
+
+
+def _cable_file(tmp_path, lengths, d0, d1, nseg, ra_kohm_cm=0.5):
+    """A LEMS file of one cell: an unbranched cable along x of the given segment lengths,
+    diameter tapering linearly from d0 to d1, in one cable group of nseg compartments."""
+    x = np.concatenate([[0.0], np.cumsum(lengths)])
+    d = d0 + (d1 - d0) * x / x[-1]
+    v = lambda a: repr(float(a))
+    segs = [f'<segment id="0"><proximal x="0" y="0" z="0" diameter="{v(d[0])}"/>'
+            f'<distal x="{v(x[1])}" y="0" z="0" diameter="{v(d[1])}"/></segment>']
+    for k in range(1, len(lengths)):
+        segs.append(f'<segment id="{k}"><parent segment="{k - 1}"/><distal x="{v(x[k + 1])}" y="0" z="0" '
+                    f'diameter="{v(d[k + 1])}"/></segment>')
+    members = "".join(f'<member segment="{k}"/>' for k in range(len(lengths)))
+    p = tmp_path / f"cable_{nseg}.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml>'
+        '<ionChannelHH id="pas" conductance="10pS"/>'
+        f'<cell id="c"><morphology id="m">{"".join(segs)}'
+        f'<segmentGroup id="axon" neuroLexId="sao864921383"><property tag="numberInternalDivisions" '
+        f'value="{nseg}"/>{members}</segmentGroup></morphology>'
+        '<biophysicalProperties id="b"><membraneProperties>'
+        '<channelDensity id="leak" ionChannel="pas" condDensity="0.3 mS_per_cm2" erev="-65mV"/>'
+        '<specificCapacitance value="1.0 uF_per_cm2"/><initMembPotential value="-65mV"/>'
+        f'</membraneProperties><intracellularProperties><resistivity value="{ra_kohm_cm} kohm_cm"/>'
+        '</intracellularProperties></biophysicalProperties></cell>'
+        '<network id="n"><population id="p" component="c" size="1"/></network></neuroml>'
+        '<Simulation id="s" length="1ms" step="0.025ms" target="n" seed="1">'
+        '<OutputFile id="o" fileName="o.txt"><OutputColumn id="v" quantity="p/0/c/0/v"/></OutputFile>'
+        '</Simulation></Lems>')
+    return str(p), x, d
+
+
+@pytest.mark.parametrize("nseg", [1, 3, 7])
+def test_cable_compartments_match_frustum_closed_forms(tmp_path, nseg):
+    """A tapering cable of uneven segments cut into nseg compartments of equal path length:
+    a compartment's area, capacitance and the resistance between neighbouring midpoints are
+    those of the cone sections they span (lateral area pi/2 (d_a + d_b) sqrt(((d_a - d_b)/2)^2
+    + l^2), C = Cm A, R = Ra 4 l / (pi d_a d_b), all additive over sections), whatever the
+    segment boundaries inside a compartment; the compartments form a chain."""
+    lengths = [100.0, 37.0, 250.0, 13.0, 60.0]
+    path, x, d = _cable_file(tmp_path, lengths, 4.0, 1.0, nseg)
+    m, _ = neuroml.read(path)
+    assert m.n_comps == nseg
+    L = x[-1]
+    dia = lambda xx: 4.0 + (1.0 - 4.0) * xx / L
+    edges = np.linspace(0.0, L, nseg + 1)
+    area = [np.pi / 2 * (dia(a) + dia(b)) * np.sqrt(((dia(a) - dia(b)) / 2) ** 2 + (b - a) ** 2)
+            for a, b in zip(edges[:-1], edges[1:])]
+    np.testing.assert_allclose(m.area, area, rtol=2e-5)
+    np.testing.assert_allclose(m.capacitance, np.asarray(area) * 1e-2, rtol=2e-5)   # 1 uF/cm^2 x um^2 in pF
+    assert list(m.parent) == [-1] + list(range(nseg - 1))
+    mid = (edges[:-1] + edges[1:]) / 2
+    r = [0.5 * 10.0 * 4 * (b - a) / (np.pi * dia(a) * dia(b)) for a, b in zip(mid[:-1], mid[1:])]   # MOhm
+    np.testing.assert_allclose(np.asarray(m.r_axial)[1:], r, rtol=2e-5)
+
+
+def test_cable_locations_by_fraction_along(tmp_path):
+    """Locations on a segment map to the compartment whose span holds them; a compartment
+    boundary belongs to the compartment it ends."""
+    path, x, d = _cable_file(tmp_path, [1000.0], 2.0, 2.0, 5)
+    doc_cells = neuroml._Doc(path)
+    c = neuroml._cell(doc_cells, next(el for role, el in doc_cells.items.values() if role == "cell"), None, {"pas": 0})
+    assert [c.comp_of_segment(0, "t", f) for f in (0.0, 0.1, 0.2, 0.2001, 0.5, 0.7, 0.99, 1.0)] == [0, 0, 0, 1, 2, 3, 4, 4]
