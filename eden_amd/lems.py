@@ -232,6 +232,34 @@ def infix(tree, names):
     return "( ( " + infix(tree[1], names) + " " + k + " " + infix(tree[2], names) + " ) )"
 
 
+_NP_FUNCS = {"exp": np.exp, "log": np.log, "ln": np.log, "sqrt": np.sqrt, "abs": np.abs, "sin": np.sin,
+             "cos": np.cos, "tan": np.tan, "sinh": np.sinh, "cosh": np.cosh, "tanh": np.tanh, "ceil": np.ceil,
+             "floor": np.floor}
+
+
+def evaluate(tree, values):
+    """Value of an expression tree in float arithmetic (literals and symbols rounded to float,
+    every operation in float), the evaluation the reader needs at instantiation time (e.g.
+    inhomogeneous channel densities); `values` maps symbols to numbers."""
+    F = np.float32
+    k = tree[0]
+    if k == "num":
+        return F(tree[1])
+    if k == "sym":
+        if tree[1] not in values:
+            raise LemsError(f"unknown symbol {tree[1]}")
+        return F(values[tree[1]])
+    if k == "neg":
+        return F(-evaluate(tree[1], values))
+    if k == "fn":
+        return F(_NP_FUNCS[tree[1]](evaluate(tree[2], values)))
+    a, b = evaluate(tree[1], values), evaluate(tree[2], values)
+    with np.errstate(all="ignore"):
+        if k == "pow":
+            return F(np.power(a, b))
+        return F({"+": a + b, "-": a - b, "*": a * b, "/": a / b}[k])
+
+
 # ---------------------------------------------------------------- emitter
 
 def emit(name, st, fn_name, dialect):

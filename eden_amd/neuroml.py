@@ -12,9 +12,10 @@ set covers:
                     HHSigmoidRate), q10Settings (q10Fixed, q10ExpTemp); channels without gates
                     are passive (eden/NeuroML.cpp:4380-4598)
   cells             <cell>: morphology (segments, parent / fractionAlong, implicit proximal
-                    points, segmentGroups with member / include), biophysicalProperties
-                    (channelDensity, specificCapacitance, initMembPotential, spikeThresh,
-                    resistivity), each over a segment group, later specs overriding earlier
+                    points, segmentGroups with member / include, inhomogeneousParameter),
+                    biophysicalProperties (channelDensity, channelDensityNonUniform with a
+                    condDensity expression of the path length from the root,
+                    specificCapacitance, initMembPotential, spikeThresh, resistivity), each over a segment group, later specs overriding earlier
                     ones (eden/NeuroML.cpp:2530-3566)
   discretisation    a cable segment group (neuroLexId sao864921383) becomes
                     numberInternalDivisions compartments of equal path length, any other
@@ -44,8 +45,8 @@ read (IdListRle is a sorted set of component sequence numbers, eden/Eden.cpp:186
 ParseQuantity does: the number read as a double, times 10^(unit power - native power), then
 stored as float (eden/NeuroML.cpp:3046-3077, eden/NeuroML.h:530-532).
 
-Constructs outside this set (other channel and synapse types, non-uniform channel densities,
-artificial cells, continuous / electrical projections) raise NeuroMLError naming the element.
+Constructs outside this set (other channel and synapse types, artificial cells, continuous /
+electrical projections) raise NeuroMLError naming the element.
 """
 import os
 import re
@@ -381,6 +382,7 @@ def _morphology(cid, m):
     groups = {"all": set(range(n))}
     raw = {}
     cable_ids = []
+    inho = {}    # inhomogeneous parameter id -> (variable, group, subtract the minimum, divide by the maximum)
     for g in m:
         if _tag(g) != "segmentGroup":
             continue
@@ -388,6 +390,21 @@ def _morphology(cid, m):
         raw[gid] = g
         if g.get("neuroLexId") == "sao864921383":
             cable_ids.append(gid)
+        for ip in g:
+            if _tag(ip) != "inhomogeneousParameter":
+                continue
+            if (ip.get("metric") or "").lower() != "path length from root":
+                raise NeuroMLError(f"cell {cid} group {gid}: inhomogeneous parameter metric "
+                                   f"'{ip.get('metric')}' is not supported (Path Length from root)")
+            prox_el = next((c for c in ip if _tag(c) == "proximal"), None)
+            dist_el = next((c for c in ip if _tag(c) == "distal"), None)
+            ts = prox_el.get("translationStart") if prox_el is not None else None
+            ne = dist_el.get("normalizationEnd") if dist_el is not None else None
+            if ts is not None and float(ts) != 0:
+                raise NeuroMLError(f"cell {cid}: translationStart must be 0")
+            if ne is not None and float(ne) != 1:
+                raise NeuroMLError(f"cell {cid}: normalizationEnd must be 1")
+            inho[ip.get("id")] = (ip.get("variable"), gid, ts is not None, ne is not None)
 
     def resolve(gid, stack=()):
         if gid in groups:
@@ -428,7 +445,55 @@ def _morphology(cid, m):
                 if _tag(c) == "property" and c.get("tag") == "numberInternalDivisions":
                     raise NeuroMLError(f"cell {cid}: group {g.get('id')} has numberInternalDivisions but is not a "
                                        "cable (neuroLexId sao864921383)")
-    return ids, parent, frac, prox.astype(np.float32), dist.astype(np.float32), groups, cables
+    return ids, parent, frac, prox.astype(np.float32), dist.astype(np.float32), groups, cables, inho
+
+
+def _lerp_len(p, q, f0, f1):
+    """Float length of the slice [f0, f1] of the segment p -> q (LerpPt3d, DistancePt3d)."""
+    F = np.float32
+    a = [F(p[i] + F(F(q[i] - p[i]) * F(f0))) for i in range(3)]
+    b = [F(p[i] + F(F(q[i] - p[i]) * F(f1))) for i in range(3)]
+    d = [F(b[i] - a[i]) for i in range(3)]
+    return F(np.sqrt(F(F(F(d[0] * d[0]) + F(d[1] * d[1])) + F(d[2] * d[2]))))
+
+
+def _path_length_from_root(cparent, slices, prox, dist):
+    """Path length of each compartment's midpoint from the root's proximal end: the root's
+    half length, then the parent's plus half of each of the two lengths (eden/Eden.cpp:4119-4134)."""
+    F = np.float32
+    L = np.zeros(len(cparent), np.float32)
+    for c, sl in enumerate(slices):
+        acc = F(0)
+        for s_, f0, f1 in sl:
+            acc = F(acc + _lerp_len(prox[s_], dist[s_], f0, f1))
+        L[c] = acc
+    out = np.zeros(len(cparent), np.float32)
+    for c in range(len(cparent)):
+        p = cparent[c]
+        out[c] = F(L[c] / F(2)) if p < 0 else F(out[p] + F(F(L[p] + L[c]) / F(2)))
+    return out
+
+
+def _inhomogeneous_values(expr, var, vals, subtract_min, divide_max, cid):
+    """Channel densities (mS/cm^2) of a channelDensityNonUniform over its compartments: the
+    parameter (path length from root, um) optionally less its minimum and over its maximum,
+    with NEURON's guard values (minimum from 1e9 down, maximum from 1 up), put into the value
+    expression, whose result is S/m^2 (eden/Eden.cpp:4560-4657)."""
+    F = np.float32
+    vals = np.asarray(vals, np.float32).copy()
+    p0 = F(min([F(1e9)] + list(vals)))
+    p1 = F(max([F(1)] + list(vals)))
+    if subtract_min:
+        vals = (vals - p0).astype(np.float32)
+        p1 = F(p1 - p0)
+    if divide_max:
+        vals = (vals / p1).astype(np.float32)
+    try:
+        tree = lems_mod.parse(expr)
+        out = [lems_mod.evaluate(tree, {var: v}) for v in vals]
+    except lems_mod.LemsError as e:
+        raise NeuroMLError(f"cell {cid}: inhomogeneousValue '{expr}': {e}") from None
+    return [F(F(g) * F(0.1)) for g in out]      # S/m^2 -> mS/cm^2
 
 
 def _discretise(cid, ids, parent, frac, prox, dist, cables):
@@ -541,10 +606,11 @@ def _cell(doc, el, temperature, channel_index):
         b = doc.biophysics.get(el.get("biophysicalProperties"))
     if b is None:
         raise NeuroMLError(f"cell {cid}: no biophysicalProperties")
-    ids, parent, frac, prox, dist, groups, cables = _morphology(cid, m)
+    ids, parent, frac, prox, dist, groups, cables, inho = _morphology(cid, m)
     n = len(ids)
     cparent, slices, seg_map = _discretise(cid, ids, parent, frac, prox, dist, cables)
     ncomp = len(cparent)
+    path_len = _path_length_from_root(cparent, slices, prox, dist)
     # capacitance and resistivity per segment; channels, V0 and threshold per compartment,
     # each spec over every compartment its segments overlap (SpecToCompList,
     # eden/Eden.cpp:4511-4560)
@@ -581,6 +647,27 @@ def _cell(doc, el, temperature, channel_index):
                     e = convert(s.get("erev"), "voltage", "erev")
                     for k in comps(s):
                         chans[k].append((ch, _f32(g), _f32(e)))
+                elif st == "channelDensityNonUniform":
+                    ch = s.get("ionChannel")
+                    if ch not in channel_index:
+                        raise NeuroMLError(f"cell {cid}: ion channel {ch} not found or not supported")
+                    e = convert(s.get("erev"), "voltage", "erev")
+                    for vp in s:
+                        if _tag(vp) != "variableParameter":
+                            continue
+                        if vp.get("parameter") != "condDensity":
+                            raise NeuroMLError(f"cell {cid}: variableParameter '{vp.get('parameter')}' is not "
+                                               "supported (condDensity)")
+                        iv = next((c for c in vp if _tag(c) == "inhomogeneousValue"), None)
+                        pid = iv.get("inhomogeneousParameter") if iv is not None else None
+                        if pid not in inho or inho[pid][1] != vp.get("segmentGroup"):
+                            raise NeuroMLError(f"cell {cid}: inhomogeneous parameter {pid} not found in group "
+                                               f"{vp.get('segmentGroup')}")
+                        var, _, sub_min, div_max = inho[pid]
+                        cl = comps(vp)
+                        for k, g in zip(cl, _inhomogeneous_values(iv.get("value"), var, path_len[cl], sub_min,
+                                                                  div_max, cid)):
+                            chans[k].append((ch, g, _f32(e)))
                 elif st == "specificCapacitance":
                     for k in where(s):
                         cm[k] = convert(s.get("value"), "specific_capacitance", "specificCapacitance")

@@ -5,7 +5,8 @@ printer gives for the oracle's run of the same instantiated model (tests/eden_te
 * the reference's single-compartment HH example (tests/golden/neuroml), by run_lems, by
   runEden (the reference's return shape) and by the command line;
 * the reference's cable discretisation model (tests/golden/neuroml), whose cable groups the
-  reader cuts into compartments;
+  reader cuts into compartments, and its inhomogeneous-parameters model (channel densities
+  that vary with path length);
 * synthetic workloads written as LEMS (config 1's construction, a tree-cell / point-cell
   network with synapses on dendrites): the GPU run of the file equals the GPU run of the
   configs model it was written from, and the oracle's run of the model read back.
@@ -79,6 +80,24 @@ def test_reference_cable_discretisation_files(tmp_path):
     assert np.array_equal(expl, eff)
     assert np.array_equal(sub, expl[:, 2])
     assert float(np.ptp(expl[:, 0])) > 1.0
+
+
+def test_reference_inhomogeneous_parameters_files(tmp_path):
+    """The reference's EdenTest_InhomogeneousParameters model (channel densities from
+    path-length expressions): output file byte-equal to the oracle's run; the cell whose
+    non-uniform density is a constant gives the uniform cell's voltages exactly, and the
+    cells with decaying densities differ from it."""
+    path = os.path.join(HERE, "golden", "neuroml", "LEMS_EdenTest_InhomogeneousParameters.xml")
+    res = run_lems(path, out_dir=str(tmp_path))
+    m, outs = res["model"], res["outputs"]
+    _say(f"[neuroml] inhomogeneous parameters: {m.n_comps} compartments, {res['rows']} rows")
+    want, _ = _expected_files(m, outs)
+    for name, text in want.items():
+        assert_same_text((tmp_path / name).read_text(), text)
+    with Simulation(m) as sim:
+        tr = sim.run_all()
+    assert np.array_equal(tr[:, 0:3], tr[:, 3:6])
+    assert float(np.abs(tr[:, 6:9] - tr[:, 0:3]).max()) > 1.0
 
 
 def test_runeden_return_shape(tmp_path, monkeypatch):

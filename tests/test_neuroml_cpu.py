@@ -228,3 +228,23 @@ def test_cable_locations_by_fraction_along(tmp_path):
     doc_cells = neuroml._Doc(path)
     c = neuroml._cell(doc_cells, next(el for role, el in doc_cells.items.values() if role == "cell"), None, {"pas": 0})
     assert [c.comp_of_segment(0, "t", f) for f in (0.0, 0.1, 0.2, 0.2001, 0.5, 0.7, 0.99, 1.0)] == [0, 0, 0, 1, 2, 3, 4, 4]
+
+
+def test_reference_inhomogeneous_parameters_model():
+    """The reference's EdenTest_InhomogeneousParameters model: a channelDensityNonUniform
+    whose value is a constant 1200 S/m^2 equals the uniform 120 mS/cm^2 cell; '500 *
+    exp(-p/20)' over the dendrite's compartments, p their midpoints' path length from the
+    root (15 and 25 um for 10 um compartments after a 10 um soma), and the normalised form
+    (p - min) / (max - min) give the closed-form densities."""
+    m, _ = neuroml.read(os.path.join(GOLD, "LEMS_EdenTest_InhomogeneousParameters.xml"))
+    assert list(m.comps_per_cell()) == [3, 3, 3, 3]
+    cb, chb = m.cell_comp_begin, m.comp_chan_begin
+
+    def dens(cell):
+        return [list(map(float, m.chan_gbase[chb[g]:chb[g + 1]])) for g in range(cb[cell], cb[cell + 1])]
+
+    flat, inhf, inho, inhn = (dens(c) for c in range(4))
+    assert flat == inhf == [[], [120.0, 36.0], [120.0, 36.0]]
+    np.testing.assert_allclose([inho[1][0], inho[2][0]], [50 * np.exp(-15 / 20), 50 * np.exp(-25 / 20)], rtol=1e-6)
+    np.testing.assert_allclose([inhn[1][0], inhn[2][0]], [50.0, 50 * np.exp(-1 / 20)], rtol=1e-6)
+    assert inho[1][1] == inho[2][1] == inhn[1][1] == 36.0
