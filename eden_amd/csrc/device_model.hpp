@@ -107,6 +107,7 @@ struct LemsArgs {
     float *state;                 // [max state][stride]: variable k of slot s at k * stride + s
     int64_t stride;
     float2 *syn_ext;              // [NC] per-compartment (I_syn, G_syn) of the step
+    const int32_t *peer;          // per synapse slot: the gap junction's peer compartment, or -1
 };
 
 // Correctly rounded fp32 division whose expansion does not go through VCC.

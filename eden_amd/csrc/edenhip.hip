@@ -141,6 +141,7 @@ struct edenhip_ctx {
     int32_t n_lems_comps = 0;
     float *d_lems_params = nullptr, *d_lems_state = nullptr;
     int32_t *d_lems_pbegin = nullptr;
+    int32_t *d_lems_peer = nullptr;      // per synapse slot: gap junction peer compartment, or -1
     float2 *d_syn_ext = nullptr;
     // edenhip_run_static: the tree buckets' work-queue counters come from the device's static
     // scratch array for the duration of that call (nullptr: each bucket's own d_queue)
@@ -268,6 +269,12 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         for (int t = 0; t < LS->n_types; t++) {
             if (LS->n_state[t] < 0 || LS->n_state[t] > 8) return fail(ctx, EDENHIP_E_LIMIT, "lems: more than 8 state variables");
             if (LS->param_begin[t + 1] < LS->param_begin[t]) return fail(ctx, EDENHIP_E_INVALID, "lems: param_begin decreasing");
+        }
+        if (d->abi_version >= 3 && LS->peer) {
+            const int64_t nsyn = d->comp_syn_begin[d->n_comps];
+            for (int64_t k = 0; k < nsyn; k++)
+                if (LS->peer[k] < -1 || LS->peer[k] >= d->n_comps)
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: peer compartment out of range");
         }
     }
     if (!(d->dt > 0) || !(d->t_final >= 0)) return fail(ctx, EDENHIP_E_INVALID, "dt must be > 0 and t_final >= 0");
@@ -691,6 +698,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(upload(ctx, &ctx->d_lems_params, prm));
         TRY(upload(ctx, &ctx->d_lems_pbegin, pb));
         TRY(alloc_zero(ctx, &ctx->d_lems_state, (size_t)(smax * n_slots)));
+        std::vector<int32_t> peer((size_t)n_slots, -1);
+        if (d->abi_version >= 3 && LS->peer)
+            for (int64_t k = 0; k < n_slots; k++)
+                if (syn_orig[(size_t)k] >= 0) peer[(size_t)k] = (int32_t)LS->peer[syn_orig[(size_t)k]];
+        TRY(upload(ctx, &ctx->d_lems_peer, peer));
         TRY(alloc_zero(ctx, &ctx->d_syn_ext, (size_t)NC));
         T.syn_ext = ctx->d_syn_ext;
         std::string err;
@@ -1081,7 +1093,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             if (ctx->lems_fn && ctx->n_lems_comps > 0) {
                 // generic synapses of step a: per-compartment currents for the kernels below
                 LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,
-                     ctx->n_slots, ctx->d_syn_ext};
+                     ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &la, &st, &dt};

@@ -6,8 +6,10 @@
 // One lane per compartment that has chemical synapses; per step, for each of its synapse
 // instances in list order: the trigger read and clear and the single-slot delay rule
 // (eden/Eden.cpp:3636-3707, the same code as synapses_step), then the instance's LEMS step
-// (lems_syn), its exposures times the connection weight aggregated per synapse type, the
-// types' aggregates summed in order (eden/Eden.cpp:3721-3735). The totals go to syn_ext,
+// (lems_syn, given the voltage at the start of the step of the instance's peer compartment
+// for gap junctions), its exposures times the connection weight aggregated per synapse type,
+// the types' aggregates summed in order (eden/Eden.cpp:3721-3735). Every compartment's V is
+// the start-of-step value here: this kernel runs before the step's compartment kernels. The totals go to syn_ext,
 // where the compartment kernels of the same step read them in place of their own synapse
 // code. State is updated in place (the generated code reads an instance's whole present
 // state before writing).
@@ -67,7 +69,10 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
         }
         float i, gx;
         float *S = L.state + s;
-        lems_syn(ty, L.params + L.param_begin[ty], S, S, (long)L.stride, V, init ? 1 : 0, spike_now ? 1 : 0, dt, &i, &gx);
+        const int32_t pc = L.peer[s];
+        const float vpeer = pc >= 0 ? T.V[pc] : V;
+        lems_syn(ty, L.params + L.param_begin[ty], S, S, (long)L.stride, V, vpeer, init ? 1 : 0, spike_now ? 1 : 0, dt,
+                 &i, &gx);
         I_agg += i * weight;
         G_agg += gx * weight;
     }

@@ -113,6 +113,12 @@ SYNAPSE_TYPES = {
         params=_PYNN, constants=_PYNN_CONST, state=["I", "A"], derived=[("i", "I * NAMP")],
         derivatives={"I": "(2.7182818*A - I)/(tau_syn*MSEC)", "A": "-A /(tau_syn*MSEC)"},
         on_start={}, on_event={"A": "A + weight"}, expose_i="i"),
+    # a linear gap junction: current conductance (vpeer - v), no conductance exposure (the
+    # reference's Exposure_g = Gsyn * 0, eden/Eden.cpp:3407-3421), vpeer the voltage of the
+    # compartment at the other end at the start of the step
+    "gapJunction": SynapseType(
+        params=[("conductance", "conductance")], state=[], derived=[("i", "conductance * (vpeer - v)")],
+        derivatives={}, on_start={}, on_event={}, expose_i="i"),
 }
 
 # PyNN synapses' numbers are plain (ms, mV); their base type gives tau_syn no default here
@@ -265,17 +271,19 @@ def evaluate(tree, values):
 def emit(name, st, fn_name, dialect):
     """C / HIP text of one type's step function:
 
-        void fn(const float *P, const float *S, float *Sn, long stride, float v, int init,
-                int event, float dt, float *i_out, float *g_out)
+        void fn(const float *P, const float *S, float *Sn, long stride, float v, float vpeer,
+                int init, int event, float dt, float *i_out, float *g_out)
 
-    P: the type's parameter vector (params in table order, then the `weight` property);
+    P: the type's parameter vector (params in table order, then the `weight` property); vpeer:
+    the voltage at the other end of a gap junction (v for other types);
     S / Sn: the instance's present / next state, variable k at k * stride (they may alias:
     every present value is read before anything is written)."""
     q = "__device__ __forceinline__ " if dialect == "hip" else "static "
-    L = [f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float v, int init, int event,"
-         f" float dt, float *i_out, float *g_out) {{",
-         f"    /* {name} */"]
-    names = {"v": "v", "dt": "dt"}
+    L = [f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float v, float vpeer, int init,"
+         f" int event, float dt, float *i_out, float *g_out) {{",
+         f"    /* {name} */",
+         "    (void)S; (void)Sn; (void)stride; (void)vpeer; (void)init; (void)event; (void)dt;"]
+    names = {"v": "v", "vpeer": "vpeer", "dt": "dt"}
     for k, (p, _) in enumerate(st.params):
         L.append(f"    const float p_{p} = P[{k}];")
         names[p] = f"p_{p}"
@@ -345,10 +353,10 @@ def build(types):
         c.append(emit(tname, st, f"lems_syn_{k}", "c"))
 
     def dispatcher(q):
-        L = [f"{q}void lems_syn(int type, const float *P, const float *S, float *Sn, long stride, float v, int init,"
-             " int event, float dt, float *i_out, float *g_out) {", "    switch (type) {"]
+        L = [f"{q}void lems_syn(int type, const float *P, const float *S, float *Sn, long stride, float v, float vpeer,"
+             " int init, int event, float dt, float *i_out, float *g_out) {", "    switch (type) {"]
         for k in range(len(types)):
-            L.append(f"    case {k}: lems_syn_{k}(P, S, Sn, stride, v, init, event, dt, i_out, g_out); break;")
+            L.append(f"    case {k}: lems_syn_{k}(P, S, Sn, stride, v, vpeer, init, event, dt, i_out, g_out); break;")
         L.append("    default: *i_out = 0.0f; *g_out = 0.0f; break;")
         L.append("    }")
         L.append("}")

@@ -105,6 +105,7 @@ class Model:
     record_comp: np.ndarray = None
     event_comp: np.ndarray = None
     lems: object = None                # eden_amd.lems.LemsSynapses: generic synapse types
+    syn_peer: np.ndarray = None        # per synapse instance: the gap junction's peer compartment, or -1
     name: str = "model"
 
     def __post_init__(self):
@@ -234,8 +235,13 @@ class Model:
             np.cumsum([len(p) for p in L.params], out=pb[1:])
             self._lems_arrays = (pb, np.concatenate(L.params + [np.zeros(1, np.float32)]).astype(np.float32),
                                  np.array(L.n_state, np.int32), L.source_hip.encode())
+            peer = None
+            if self.syn_peer is not None:
+                peer = np.ascontiguousarray(self.syn_peer, np.int64)
+                self._lems_peer = peer
             self._lems_desc = N.LemsSynapsesDesc(len(L.params), self._lems_arrays[3], N.ptr(self._lems_arrays[0]),
-                                                 N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]))
+                                                 N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]),
+                                                 N.ptr(peer))
             d.lems = C.pointer(self._lems_desc)
         d.n_record = len(self.record_comp)
         d.n_event_cols = len(self.event_comp)

@@ -24,13 +24,15 @@ set covers:
                     compartment holding their segment's fractionAlong (0.5 when absent)
                     (eden/Eden.cpp:1387-1772, 1015-1038, 4511-4560)
   synapses          expTwoSynapse natively; expOneSynapse, alphaSynapse, alphaCurrentSynapse,
-                    expThreeSynapse and the PyNN synapses (expCond, expCurr, alphaCond,
-                    alphaCurr) as generic LEMS types (eden_amd.lems), compiled by the library
-                    at run time (eden/NeuroML.cpp:6506-6829)
+                    expThreeSynapse, gapJunction and the PyNN synapses (expCond, expCurr,
+                    alphaCond, alphaCurr) as generic LEMS types (eden_amd.lems), compiled by
+                    the library at run time (eden/NeuroML.cpp:6506-6829)
   inputs            pulseGenerator, poissonFiringSynapse (eden/NeuroML.cpp:6968-7107)
   networks          population (size or instances), projection with connection /
-                    connectionWD, synapticConnection, explicitInput, inputList with
-                    input / inputW (eden/NeuroML.cpp:6136-6420)
+                    connectionWD, synapticConnection, electricalProjection (gapJunction:
+                    an instance on each end, each with the other end's voltage as its peer),
+                    explicitInput, inputList with input / inputW
+                    (eden/NeuroML.cpp:6136-6420, eden/Eden.cpp:9225-9248)
   simulation        <Simulation length step seed target> with OutputFile / OutputColumn and
                     EventOutputFile / EventSelection, <Target> (eden/NeuroML.cpp:7979-8311)
 
@@ -128,7 +130,7 @@ _CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Sy
 # <Component type="..."> of these types is read as the element itself
 _SYNAPSES = set(lems_mod.SYNAPSE_TYPES)
 _INPUTS = {"pulseGenerator", "poissonFiringSynapse"}
-_UNSUPPORTED_SYNAPSES = {"blockingPlasticSynapse", "silentSynapse", "gapJunction", "linearGradedSynapse",
+_UNSUPPORTED_SYNAPSES = {"blockingPlasticSynapse", "silentSynapse", "linearGradedSynapse",
                          "gradedSynapse", "doubleSynapse"}
 _UNSUPPORTED_CELLS = {"iafCell", "iafTauCell", "iafRefCell", "iafTauRefCell", "izhikevichCell",
                       "izhikevich2007Cell", "adExIaFCell", "fitzHughNagumoCell", "pinskyRinzelCA3Cell",
@@ -889,6 +891,8 @@ def read(path, rng_seed=None):
     for el in net:
         if _tag(el) in ("projection", "synapticConnection"):
             used.add(el.get("synapse"))
+        elif _tag(el) == "electricalProjection":
+            used |= {c.get("synapse") for c in el if c.get("synapse")}
     generic = any(doc.items.get(u, (None, None))[0] == "synapse" and _tag(doc.items[u][1]) != "expTwoSynapse"
                   for u in used)
     generic_index, generic_types = {}, []
@@ -944,15 +948,39 @@ def read(path, rng_seed=None):
                 b = comp(post, qi, int(c.get("postSegmentId", "0")), "connection", frac(c, "postFractionAlong"))
                 w = convert(c.get("weight"), None, "weight") if ct == "connectionWD" else 1.0
                 d = convert(c.get("delay"), "time", "delay") if ct == "connectionWD" else 0.0
-                conns.append((a, b, st, w, d))
+                conns.append((a, b, st, w, d, -1))
         elif t == "synapticConnection":
             st = synapse(el.get("synapse"), "synapticConnection")
             fp, fi = _cell_ref(el.get("from"))
             tp, ti = _cell_ref(el.get("to"))
-            conns.append((comp(fp, fi, 0, "synapticConnection"), comp(tp, ti, 0, "synapticConnection"), st, 1.0, 0.0))
-        elif t in ("continuousProjection", "electricalProjection"):
+            conns.append((comp(fp, fi, 0, "synapticConnection"), comp(tp, ti, 0, "synapticConnection"), st, 1.0, 0.0,
+                          -1))
+        elif t == "electricalProjection":
+            # gap junctions: an instance on each end, the postsynaptic end's first, each with
+            # the other end as its peer (eden/Eden.cpp:9225-9248)
+            pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
+            for c in el:
+                ct = _tag(c)
+                if ct not in ("electricalConnection", "electricalConnectionInstance", "electricalConnectionInstanceW"):
+                    continue
+                what = f"electricalProjection {el.get('id')}"
+                sid = c.get("synapse")
+                if doc.items.get(sid, (None, None))[0] == "synapse" and _tag(doc.items[sid][1]) != "gapJunction":
+                    raise NeuroMLError(f"{what}: synapse '{sid}' is not a gapJunction")
+                st = synapse(sid, what)
+                if ct == "electricalConnection":
+                    pi, qi = int(c.get("preCell")), int(c.get("postCell"))
+                else:
+                    _, pi = _cell_ref(c.get("preCell"))
+                    _, qi = _cell_ref(c.get("postCell"))
+                a = comp(pre, pi, int(c.get("preSegment", "0")), what, frac(c, "preFractionAlong"))
+                b = comp(post, qi, int(c.get("postSegment", "0")), what, frac(c, "postFractionAlong"))
+                w = convert(c.get("weight"), None, "weight") if ct == "electricalConnectionInstanceW" else 1.0
+                conns.append((-1, b, st, w, 0.0, a))
+                conns.append((-1, a, st, w, 0.0, b))
+        elif t == "continuousProjection":
             if any(True for _ in el):
-                raise NeuroMLError(f"<{t}> {el.get('id')}: continuous and electrical projections are not supported")
+                raise NeuroMLError(f"<{t}> {el.get('id')}: continuous projections are not supported")
 
     # per-compartment tables
     v0 = np.concatenate([pop_of_cell[c].cell.v0 for c in range(n_cells)])
@@ -1001,7 +1029,7 @@ def read(path, rng_seed=None):
 
     # synapse instances: per compartment, by synapse type (reading order), then connection order
     per_comp = [[] for _ in range(nc)]
-    for k, (a, b, st, w, d) in enumerate(conns):
+    for k, (a, b, st, w, d, peer) in enumerate(conns):
         per_comp[b].append((st, k))
     syn_b = np.zeros(nc + 1, np.int64)
     inst_of_conn = np.zeros(len(conns), np.int64)
@@ -1012,8 +1040,9 @@ def read(path, rng_seed=None):
             syn_list.append(k)
         syn_b[g + 1] = len(syn_list)
     spk = [[] for _ in range(nc)]
-    for k, (a, b, st, w, d) in enumerate(conns):
-        spk[a].append(inst_of_conn[k])
+    for k, (a, b, st, w, d, peer) in enumerate(conns):
+        if a >= 0:
+            spk[a].append(inst_of_conn[k])
     spk_b = np.zeros(nc + 1, np.int64)
     np.cumsum([len(x) for x in spk], out=spk_b[1:])
 
@@ -1068,6 +1097,7 @@ def read(path, rng_seed=None):
         syn_type=np.array([conns[k][2] for k in syn_list], np.int32),
         syn_weight=np.array([_f32(conns[k][3]) for k in syn_list], np.float32),
         syn_delay=np.array([_f32(conns[k][4]) for k in syn_list], np.float32),
+        syn_peer=(np.array([conns[k][5] for k in syn_list], np.int64) if any(c[5] >= 0 for c in conns) else None),
         comp_spike_begin=spk_b, spike_target=np.array([x for lst in spk for x in lst], np.int64),
         comp_poisson_begin=po_b, poisson_rate=f32(po, 0), poisson_syn_type=np.array([x[1] for x in po], np.int32),
         poisson_weight=f32(po, 2), poisson_instance=np.array([x[3] for x in po], np.int32),

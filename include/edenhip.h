@@ -84,23 +84,28 @@ typedef struct edenhip_exptwo_type {
     float weight_prop;   /* the LEMS `weight` property, default 1 */
 } edenhip_exptwo_type;
 
-/* Generic LEMS synapse types (alphaSynapse, expThreeSynapse, the PyNN synapses, ...). The
- * reference emits C for each lemsified ComponentType and compiles it with gcc
- * (eden/Eden.cpp:2339-2995, 6992-7216); here the caller supplies HIP source defining
+/* Generic LEMS synapse types (alphaSynapse, expThreeSynapse, the PyNN synapses, gap
+ * junctions, ...). The reference emits C for each lemsified ComponentType and compiles it
+ * with gcc (eden/Eden.cpp:2339-2995, 6992-7216); here the caller supplies HIP source defining
  *   __device__ void lems_syn(int type, const float *P, const float *S, float *Sn, long stride,
- *                            float v, int init, int event, float dt, float *i, float *g)
+ *                            float v, float vpeer, int init, int event, float dt, float *i,
+ *                            float *g)
  * (one instance's step: P its type's parameters, S / Sn its present / next state with
- * variable k at k * stride, v the compartment voltage, event whether it receives its event
- * this step; i, g its current and conductance exposures), which edenhip_create() compiles
- * with hiprtc into the library's synapse kernel template (eden_amd.lems emits it). When
- * present, every chemical synapse instance's syn_type indexes these types, and syn_types
- * serves only the Poisson inputs. */
+ * variable k at k * stride, v the compartment voltage, vpeer the voltage at the start of the
+ * step of the instance's peer compartment (gap junctions; v when it has none), event whether
+ * it receives its event this step; i, g its current and conductance exposures), which
+ * edenhip_create() compiles with hiprtc into the library's synapse kernel template
+ * (eden_amd.lems emits it). When present, every synapse instance's syn_type indexes these
+ * types, and syn_types serves only the Poisson inputs. */
 typedef struct edenhip_lems_synapses {
     int32_t n_types;
     const char *source;           /* HIP C++ source, NUL-terminated */
     const int32_t *param_begin;   /* [n_types + 1] into params */
     const float *params;          /* per type: its parameter vector */
     const int32_t *n_state;       /* [n_types] state variables per instance (at most 8) */
+    const int64_t *peer;          /* (abi_version >= 3) [n_syn] the global compartment whose voltage
+                                     is the instance's vpeer (the other end of a gap junction,
+                                     eden/Eden.cpp:9225-9248), or -1; NULL = none */
 } edenhip_lems_synapses;
 
 /*

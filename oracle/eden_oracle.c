@@ -164,8 +164,8 @@ static void log_poi_event(int64_t p, int64_t step) {
 /* Generic LEMS synapse types (edenhip_model_desc.lems): the step function `lems_syn` of the
  * C dialect of the source the model carries (eden_amd.lems emits both dialects from one
  * expression tree), compiled by the test harness and handed over here. fp32 flavours only. */
-typedef void (*lems_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float v, int init,
-                          int event, float dt, float *i_out, float *g_out);
+typedef void (*lems_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float v, float vpeer,
+                          int init, int event, float dt, float *i_out, float *g_out);
 static lems_fn_t g_lems = NULL;
 
 void FN(set_lems)(void *fn) { g_lems = (lems_fn_t)fn; }
@@ -330,8 +330,12 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                     const edenhip_lems_synapses *L = d->lems;
                     const int64_t NSYN = d->comp_syn_begin[d->n_comps];
                     float iv, gv;
+                    /* a gap junction's peer voltage: the start-of-step state of the other end
+                       (eden/Eden.cpp:3689-3703) */
+                    const int64_t pc = (d->abi_version >= 3 && L->peer) ? L->peer[sy] : -1;
+                    const real Vpeer = pc >= 0 ? V[pc] : Vcomp;
                     g_lems(ty, L->params + L->param_begin[ty], s->lS[now] + sy, s->lS[nxt] + sy, (long)NSYN,
-                           (float)Vcomp, initial_state, spike_in_flag, (float)dt, &iv, &gv);
+                           (float)Vcomp, (float)Vpeer, initial_state, spike_in_flag, (float)dt, &iv, &gv);
                     I_agg += (real)iv * weight;
                     G_agg += (real)gv * weight;
                     continue;

@@ -5,7 +5,9 @@ of the same text: traces, spike raster and every synaptic delivery bit for bit.
 * the reference's CoreSynapses model (its chemical part, tests/coresyn.py): one synapse of
   each core type, through the NeuroML reader and the output files;
 * expTwoSynapse as a generic type equals the library's native expTwo path;
-* a network of tree and point cells with generic synapses on dendrites and somata.
+* a network of tree and point cells with generic synapses on dendrites and somata;
+* gap junctions (a pair of membranes; the reference's DomainDecomposition model, whose
+  electrical projections join compartments of different tree cells).
 """
 import dataclasses
 import sys
@@ -80,3 +82,29 @@ def test_tree_and_point_network_with_generic_synapses(hh):
     g = dataclasses.replace(m, lems=lems.build(types), name=f"mixed_generic_{hh}")
     tr, er, ost = _check(g)
     assert len(ost) > 50
+
+
+def test_gap_junction_pair(tmp_path):
+    """Two membranes joined by a gap junction (tests/test_lems_cpu.py pins the oracle's
+    closed form): the GPU equals the oracle bit for bit."""
+    from test_lems_cpu import _gap_pair_file
+    m, _ = neuroml.read(_gap_pair_file(tmp_path, -65.0, -55.0, 200.0))
+    tr, _, _ = _check(m)
+    assert abs(float(tr[-1, 0] - tr[-1, 1])) < 5.0
+
+
+def test_reference_domain_decomposition_model(tmp_path):
+    """The reference's EdenTest_DomainDecomposition model: tree cells joined by gap junctions
+    between compartments of different cells, beside chemical synapses; output file byte-equal
+    to the oracle's run of the same model."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "neuroml",
+                        "LEMS_EdenTest_DomainDecomposition.xml")
+    m, outs = neuroml.read(path)
+    assert m.syn_peer is not None and int((m.syn_peer >= 0).sum()) >= 2
+    tr, er, ost = _check(m)
+    res = run_lems(path, out_dir=str(tmp_path))
+    t_us = output.row_times_us(m.dt, m.total_rows() + 1)
+    for f in outs.trajectories:
+        assert_same_text((tmp_path / f.file_name).read_text(), trajectory_text(t_us, tr[:, [c[2] for c in f.columns]]))
+    _say(f"[lems] domain decomposition: {m.n_cells} cells, {m.n_comps} compartments, {res['rows']} rows")

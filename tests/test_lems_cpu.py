@@ -46,14 +46,14 @@ def _stepper(tname, vals, weight=1.0):
     lib = oracle.lems_library(ls.source_c)
     fn = lib.lems_syn
     fn.restype = None
-    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_int, C.c_int, C.c_float,
-                   C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_float, C.c_int, C.c_int,
+                   C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     P = ls.params[0]
     S = np.zeros(ls.n_state[0], np.float32)
 
     def step(v, init, event, dt):
         i, g = C.c_float(), C.c_float()
-        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, init, event, dt, C.byref(i), C.byref(g))
+        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, v, init, event, dt, C.byref(i), C.byref(g))
         return i.value, g.value
     return step
 
@@ -131,16 +131,61 @@ def test_generic_exptwo_equals_native_exptwo():
 def test_reference_core_synapses_chemical_part(tmp_path):
     from coresyn import write_chemical_subset
     m, outs = neuroml.read(write_chemical_subset(str(tmp_path)))
+    # every synapse element in reading order (the gap junction's is read although its
+    # projection was removed)
     assert m.lems.type_names == ["alphaCurrentSynapse", "alphaSynapse", "expOneSynapse", "expTwoSynapse",
-                                 "expThreeSynapse", "expCondSynapse", "alphaCondSynapse", "expCurrSynapse",
-                                 "alphaCurrSynapse"]
+                                 "expThreeSynapse", "gapJunction", "expCondSynapse", "alphaCondSynapse",
+                                 "expCurrSynapse", "alphaCurrSynapse"]
     # ChemPost cells 1..10 and 21 receive one synapse each, of the projections' types in order
     post = [int(np.flatnonzero(np.diff(m.comp_syn_begin))[k]) for k in range(m.n_synapses)]
     assert post == [2, 3, 4, 5, 6, 8, 9, 10, 11, 22]
-    assert list(m.syn_type) == [0, 1, 2, 3, 4, 5, 6, 7, 8, 0]
+    assert list(m.syn_type) == [0, 1, 2, 3, 4, 6, 7, 8, 9, 0]
     assert m.syn_weight[-1] == np.float32(1.7) and m.syn_delay[-1] == 5000.0
     (tr, ec, er, _), (st, sy) = oracle.run_with_synapse_log(m)
     assert sorted(set(sy.tolist())) == list(range(10))       # every synapse received events
     # every postsynaptic cell left rest
     cols = [c[2] for c in outs.trajectories[0].columns][1:12]
     assert np.all(np.abs(tr[:, cols] - (-65.0)).max(axis=0) > 0.05)
+
+
+def _gap_pair_file(tmp_path, v0a, v0b, g_ps, cm_uf=1.0, diam=10.0):
+    """Two one-compartment cells with no membrane conductance, coupled by one gap junction."""
+    p = tmp_path / "gap_pair.xml"
+    cell = (f'<cell id="{{}}"><morphology id="m{{}}"><segment id="0"><proximal x="0" y="0" z="0" diameter="{diam}"/>'
+            f'<distal x="0" y="0" z="0" diameter="{diam}"/></segment></morphology>'
+            '<biophysicalProperties id="b{}"><membraneProperties>'
+            '<channelDensity id="leak" ionChannel="pas" condDensity="0 mS_per_cm2" erev="-65mV"/>'
+            f'<specificCapacitance value="{cm_uf} uF_per_cm2"/><initMembPotential value="{{}}mV"/>'
+            '</membraneProperties><intracellularProperties><resistivity value="0.1 kohm_cm"/>'
+            '</intracellularProperties></biophysicalProperties></cell>')
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml><ionChannelHH id="pas" conductance="10pS"/>'
+        f'<gapJunction id="gj" conductance="{g_ps}pS"/>'
+        + cell.format("A", "A", "A", v0a) + cell.format("B", "B", "B", v0b) +
+        '<network id="n"><population id="pa" component="A" size="1"/><population id="pb" component="B" size="1"/>'
+        '<electricalProjection id="e" presynapticPopulation="pa" postsynapticPopulation="pb">'
+        '<electricalConnection id="0" preCell="0" preSegment="0" postCell="0" postSegment="0" synapse="gj"/>'
+        '</electricalProjection></network></neuroml>'
+        '<Simulation id="s" length="20ms" step="0.025ms" target="n" seed="1">'
+        '<OutputFile id="o" fileName="o.txt"><OutputColumn id="a" quantity="pa/0/A/0/v"/>'
+        '<OutputColumn id="b" quantity="pb/0/B/0/v"/></OutputFile></Simulation></Lems>')
+    return str(p)
+
+
+def test_gap_junction_closed_form(tmp_path):
+    """Two membranes with no conductance of their own joined by a gap junction g: with the
+    current g (Vpeer - V) taken from the start of each step, the difference x = Va - Vb obeys
+    x' = x (1 - 2 g dt / C) exactly per step and Va + Vb stays fixed; each end holds one
+    instance whose peer is the other end."""
+    m, _ = neuroml.read(_gap_pair_file(tmp_path, -65.0, -55.0, 200.0))
+    assert m.lems.type_names == ["gapJunction"]
+    assert list(m.syn_peer) == [1, 0] and list(np.diff(m.comp_syn_begin)) == [1, 1]
+    tr, _, _, _ = oracle.run(m, precision="f32")
+    C = float(m.capacitance[0])
+    g = 200e-6                                  # uS
+    x = tr[:, 0].astype(np.float64) - tr[:, 1].astype(np.float64)
+    k = 1 - 2 * g * m.dt / C
+    pred = -10.0 * k ** np.arange(len(x))
+    np.testing.assert_allclose(x, pred, rtol=1e-4, atol=1e-4)
+    assert x[-1] / x[0] < 0.5                   # the voltages approach each other
+    np.testing.assert_allclose(tr[:, 0].astype(np.float64) + tr[:, 1], -120.0, atol=1e-3)
