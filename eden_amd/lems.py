@@ -123,6 +123,130 @@ SYNAPSE_TYPES = {
 
 # PyNN synapses' numbers are plain (ms, mV); their base type gives tau_syn no default here
 
+# ---------------------------------------------------------------- user ComponentTypes
+
+# the NeuroML 2 base synapse types (its Synapses.xml) and the parameters they contribute
+BASE_SYNAPSES = {
+    "baseSynapse": [], "baseVoltageDepSynapse": [], "baseCurrentBasedSynapse": [], "baseGradedSynapse": [],
+    "baseConductanceBasedSynapse": [("gbase", "conductance"), ("erev", "voltage")],
+    "baseConductanceBasedSynapseTwo": [("gbase1", "conductance"), ("gbase2", "conductance"), ("erev", "voltage")],
+}
+# LEMS dimension names -> the reader's unit dimensions (None: dimensionless)
+LEMS_DIMS = {"time": "time", "per_time": "frequency", "voltage": "voltage", "current": "current",
+             "conductance": "conductance", "capacitance": "capacitance", "length": "length",
+             "resistance": "resistance", "conductanceDensity": "conductivity", "temperature": "temperature",
+             "none": None, "": None}
+
+
+def _ltag(el):
+    t = el.tag
+    return t.rsplit("}", 1)[-1] if isinstance(t, str) else ""
+
+
+def is_synapse_component_type(name, component_types):
+    """Whether a ComponentType (by name) extends, through any chain, a NeuroML base synapse."""
+    seen = set()
+    while name in component_types and name not in seen:
+        seen.add(name)
+        name = component_types[name].get("extends", "")
+    return name in BASE_SYNAPSES
+
+
+def synapse_type_from_lems(name, component_types, convert):
+    """SynapseType of a user-defined synapse <ComponentType> (the reference turns any lemsified
+    ComponentType into C, eden/Eden.cpp:2339-2995): Parameters and DerivedParameters, Constants
+    (converted to engine units by `convert(text, dimension)`), the `weight` Property, and the
+    nearest Dynamics along the extends chain: StateVariables, DerivedVariables (ordered by
+    their dependencies), TimeDerivatives, OnStart and OnEvent (port `in`) StateAssignments;
+    exposures `i` and `g`. Returns (SynapseType, default weight)."""
+    chain, n = [], name
+    while n in component_types:
+        chain.append(component_types[n])
+        n = component_types[n].get("extends", "")
+    if n not in BASE_SYNAPSES:
+        raise LemsError(f"ComponentType {name} does not extend a base synapse type")
+    params = list(BASE_SYNAPSES[n])
+    dparams, consts, weight, dyn = [], [], 1.0, None
+    for ct in reversed(chain):                     # base first, so derived types override
+        for el in ct:
+            t = _ltag(el)
+            if t == "Parameter":
+                dim = el.get("dimension", "none")
+                if dim not in LEMS_DIMS:
+                    raise LemsError(f"{name}: dimension '{dim}' of parameter {el.get('name')} is not supported")
+                params = [p for p in params if p[0] != el.get("name")] + [(el.get("name"), LEMS_DIMS[dim])]
+            elif t == "DerivedParameter":
+                dparams.append((el.get("name"), el.get("value")))
+            elif t == "Constant":
+                dim = LEMS_DIMS.get(el.get("dimension", "none"), "unsupported")
+                if dim == "unsupported":
+                    raise LemsError(f"{name}: dimension of constant {el.get('name')} is not supported")
+                consts.append((el.get("name"), convert(el.get("value"), dim)))
+            elif t == "Property" and el.get("name") == "weight":
+                weight = float(el.get("defaultValue", "1"))
+            elif t == "Dynamics":
+                dyn = el
+    if dyn is None:
+        raise LemsError(f"ComponentType {name} has no Dynamics")
+    state, derived, ders, on_start, on_event, expose = [], {}, {}, {}, {}, {}
+    for el in dyn:
+        t = _ltag(el)
+        if t == "StateVariable":
+            state.append(el.get("name"))
+            if el.get("exposure"):
+                expose[el.get("exposure")] = el.get("name")
+        elif t == "DerivedVariable":
+            if el.get("value") is None:
+                raise LemsError(f"{name}: derived variable {el.get('name')} without a value (select) is not supported")
+            derived[el.get("name")] = el.get("value")
+            if el.get("exposure"):
+                expose[el.get("exposure")] = el.get("name")
+        elif t == "TimeDerivative":
+            ders[el.get("variable")] = el.get("value")
+        elif t in ("OnStart", "OnEvent"):
+            if t == "OnEvent" and el.get("port", "in") != "in":
+                raise LemsError(f"{name}: OnEvent port '{el.get('port')}' is not supported")
+            tgt = on_start if t == "OnStart" else on_event
+            for a in el:
+                if _ltag(a) == "StateAssignment":
+                    tgt[a.get("variable")] = a.get("value")
+                elif _ltag(a) != "EventOut":
+                    raise LemsError(f"{name}: <{_ltag(a)}> in <{t}> is not supported")
+        elif t in ("OnCondition", "ConditionalDerivedVariable", "Regime", "KineticScheme"):
+            raise LemsError(f"{name}: <{t}> is not supported")
+    if "i" not in expose:
+        raise LemsError(f"ComponentType {name} exposes no current i")
+
+    def syms(expr):
+        out, stack = set(), [parse(expr)]
+        while stack:
+            x = stack.pop()
+            if x[0] == "sym":
+                out.add(x[1])
+            elif x[0] != "num":
+                stack.extend(y for y in x[1:] if isinstance(y, tuple))
+        return out
+
+    order, done = [], set()
+
+    def visit(v, path=()):
+        if v in done:
+            return
+        if v in path:
+            raise LemsError(f"{name}: derived variables depend on each other in a cycle ({v})")
+        for u in syms(derived[v]):
+            if u in derived:
+                visit(u, path + (v,))
+        done.add(v)
+        order.append((v, derived[v]))
+
+    for v in derived:
+        visit(v)
+    st = SynapseType(params=params, state=state, derived=order, derivatives=ders, on_start=on_start,
+                     on_event=on_event, expose_i=expose["i"], expose_g=expose.get("g"), derived_params=dparams,
+                     constants=consts)
+    return st, weight
+
 
 # ---------------------------------------------------------------- expressions
 
@@ -335,14 +459,16 @@ class LemsSynapses:
     source_c: str
 
 
-def build(types):
-    """types: list of (component type name, {param: engine value}, weight property). Returns
-    LemsSynapses whose functions are lems_syn_<k> and a dispatcher lems_syn(type, ...)."""
+def build(types, custom=None):
+    """types: list of (component type name, {param: engine value}, weight property); custom:
+    user ComponentTypes by name (SynapseType). Returns LemsSynapses whose functions are
+    lems_syn_<k> and a dispatcher lems_syn(type, ...)."""
     hip, c, params, nst, names = [], [], [], [], []
+    custom = custom or {}
     for k, (tname, vals, weight) in enumerate(types):
-        if tname not in SYNAPSE_TYPES:
+        if tname not in SYNAPSE_TYPES and tname not in custom:
             raise LemsError(f"synapse type {tname} is not supported")
-        st = SYNAPSE_TYPES[tname]
+        st = custom[tname] if tname in custom else SYNAPSE_TYPES[tname]
         missing = [p for p, _ in st.params if p not in vals]
         if missing:
             raise LemsError(f"{tname}: parameters {missing} missing")

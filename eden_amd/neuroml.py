@@ -26,7 +26,9 @@ set covers:
   synapses          expTwoSynapse natively; expOneSynapse, alphaSynapse, alphaCurrentSynapse,
                     expThreeSynapse, gapJunction and the PyNN synapses (expCond, expCurr,
                     alphaCond, alphaCurr) as generic LEMS types (eden_amd.lems), compiled by
-                    the library at run time (eden/NeuroML.cpp:6506-6829)
+                    the library at run time (eden/NeuroML.cpp:6506-6829); user-defined
+                    <ComponentType>s that extend a base synapse type, from their LEMS
+                    Dynamics (eden/Eden.cpp:2339-2995)
   inputs            pulseGenerator, poissonFiringSynapse (eden/NeuroML.cpp:6968-7107)
   networks          population (size or instances), projection with connection /
                     connectionWD, synapticConnection, electricalProjection (gapJunction:
@@ -171,6 +173,7 @@ class _Doc:
         self.simulations = OrderedDict()
         self.morphologies = {}
         self.biophysics = {}
+        self.component_types = {}        # user <ComponentType> elements by name
         self.target = None
         self.seen = set()
         self._read(os.path.abspath(path))
@@ -226,7 +229,9 @@ class _Doc:
                 self.morphologies[el.get("id")] = el
             elif t == "biophysicalProperties":
                 self.biophysics[el.get("id")] = el
-            elif t in _SYNAPSES:
+            elif t == "ComponentType":
+                self.component_types[el.get("name")] = el
+            elif t in _SYNAPSES or lems_mod.is_synapse_component_type(t, self.component_types):
                 self._add("synapse", el)
             elif t in _INPUTS:
                 self._add("input", el)
@@ -895,13 +900,23 @@ def read(path, rng_seed=None):
             used |= {c.get("synapse") for c in el if c.get("synapse")}
     generic = any(doc.items.get(u, (None, None))[0] == "synapse" and _tag(doc.items[u][1]) != "expTwoSynapse"
                   for u in used)
-    generic_index, generic_types = {}, []
+    generic_index, generic_types, custom = {}, [], {}
     if generic:
         for i, el in syn_elems:
-            st = lems_mod.SYNAPSE_TYPES[_tag(el)]
+            t = _tag(el)
+            weight = 1.0
+            if t in lems_mod.SYNAPSE_TYPES:
+                st = lems_mod.SYNAPSE_TYPES[t]
+            else:
+                if t not in custom:
+                    try:
+                        custom[t] = lems_mod.synapse_type_from_lems(t, doc.component_types, convert)
+                    except lems_mod.LemsError as e:
+                        raise NeuroMLError(str(e)) from None
+                st, weight = custom[t]
             vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
             generic_index[i] = len(generic_types)
-            generic_types.append((_tag(el), vals, 1.0))
+            generic_types.append((t, vals, weight))
 
     def synapse(sid, what, native=False):
         role, el = doc.items.get(sid, (None, None))
@@ -964,10 +979,7 @@ def read(path, rng_seed=None):
                 if ct not in ("electricalConnection", "electricalConnectionInstance", "electricalConnectionInstanceW"):
                     continue
                 what = f"electricalProjection {el.get('id')}"
-                sid = c.get("synapse")
-                if doc.items.get(sid, (None, None))[0] == "synapse" and _tag(doc.items[sid][1]) != "gapJunction":
-                    raise NeuroMLError(f"{what}: synapse '{sid}' is not a gapJunction")
-                st = synapse(sid, what)
+                st = synapse(c.get("synapse"), what)
                 if ct == "electricalConnection":
                     pi, qi = int(c.get("preCell")), int(c.get("postCell"))
                 else:
@@ -1103,7 +1115,7 @@ def read(path, rng_seed=None):
         poisson_weight=f32(po, 2), poisson_instance=np.array([x[3] for x in po], np.int32),
         poisson_rng_offset=np.zeros(len(po), np.int32),
         record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
-        lems=lems_mod.build(generic_types) if generic else None,
+        lems=lems_mod.build(generic_types, {k: v[0] for k, v in custom.items()}) if generic else None,
         name=os.path.splitext(os.path.basename(path))[0])
     model.sim_seed = seed
     return model, Outputs(trajectories, events, base_dir)

@@ -7,7 +7,8 @@ of the same text: traces, spike raster and every synaptic delivery bit for bit.
 * expTwoSynapse as a generic type equals the library's native expTwo path;
 * a network of tree and point cells with generic synapses on dendrites and somata;
 * gap junctions (a pair of membranes; the reference's DomainDecomposition model, whose
-  electrical projections join compartments of different tree cells).
+  electrical projections join compartments of different tree cells);
+* user-defined synapse ComponentTypes read from LEMS.
 """
 import dataclasses
 import sys
@@ -108,3 +109,14 @@ def test_reference_domain_decomposition_model(tmp_path):
     for f in outs.trajectories:
         assert_same_text((tmp_path / f.file_name).read_text(), trajectory_text(t_us, tr[:, [c[2] for c in f.columns]]))
     _say(f"[lems] domain decomposition: {m.n_cells} cells, {m.n_comps} compartments, {res['rows']} rows")
+
+
+def test_user_defined_component_types(tmp_path):
+    """User ComponentTypes (an exponential synapse and a gap junction written as LEMS
+    dynamics) compiled by hiprtc: the GPU equals the oracle, and the user types' cells equal
+    the core types' cells."""
+    from test_lems_cpu import _user_types_file
+    m, _ = neuroml.read(_user_types_file(tmp_path))
+    tr, er, ost = _check(m)
+    assert len(ost) > 2
+    assert np.array_equal(tr[:, 1], tr[:, 2]) and np.array_equal(tr[:, 3:5], tr[:, 5:7])

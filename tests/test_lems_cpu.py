@@ -13,10 +13,13 @@
   oracle, bit for bit, over a network run.
 * The reference's CoreSynapses model (its chemical part) reads into one generic type per
   core synapse, in the reference's instance order, and runs in the oracle.
+* Gap junctions: a coupled pair of membranes follows the discrete closed form.
+* User-defined ComponentTypes with the dynamics of core types run as the core types do.
 """
 import ctypes as C
 import dataclasses
 import math
+import os
 
 import numpy as np
 import pytest
@@ -189,3 +192,78 @@ def test_gap_junction_closed_form(tmp_path):
     np.testing.assert_allclose(x, pred, rtol=1e-4, atol=1e-4)
     assert x[-1] / x[0] < 0.5                   # the voltages approach each other
     np.testing.assert_allclose(tr[:, 0].astype(np.float64) + tr[:, 1], -120.0, atol=1e-3)
+
+
+_USER_TYPES = """
+<ComponentType name="myGapJunction" extends="baseGradedSynapse">
+    <Parameter name="conductance" dimension="conductance"/>
+    <Dynamics>
+        <DerivedVariable name="i" exposure="i" dimension="current" value="(vpeer - v) * conductance"/>
+    </Dynamics>
+</ComponentType>
+<ComponentType name="myExpOneSynapse" extends="baseConductanceBasedSynapse">
+    <Property name="weight" dimension="none" defaultValue="1"/>
+    <Parameter name="tauDecay" dimension="time"/>
+    <Dynamics>
+        <StateVariable name="g" dimension="conductance" exposure="g"/>
+        <DerivedVariable name="i" exposure="i" dimension="current" value="g * (erev - v)"/>
+        <TimeDerivative variable="g" value="-g / tauDecay"/>
+        <OnStart><StateAssignment variable="g" value="0"/></OnStart>
+        <OnEvent port="in"><StateAssignment variable="g" value="g + (weight * gbase)"/></OnEvent>
+    </Dynamics>
+</ComponentType>
+"""
+
+
+def _user_types_file(tmp_path):
+    """An HH cell (the reference's example cell, tests/golden/neuroml) spiking under a current
+    step, projecting through expOneSynapse onto one passive cell and through a user-defined
+    ComponentType of the same dynamics onto another; two passive pairs joined by gapJunction
+    and by a user-defined gap junction type."""
+    import shutil
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "neuroml")
+    shutil.copy(os.path.join(gold, "NML2_SingleCompHHCell.nml"), tmp_path / "NML2_SingleCompHHCell.nml")
+    pas = ('<cell id="{n}"><morphology id="m{n}"><segment id="0"><proximal x="0" y="0" z="0" diameter="10"/>'
+           '<distal x="0" y="0" z="0" diameter="10"/></segment></morphology><biophysicalProperties id="b{n}">'
+           '<membraneProperties><channelDensity id="leak" ionChannel="passiveChan" condDensity="0.3 mS_per_cm2" '
+           'erev="-65mV"/><specificCapacitance value="1.0 uF_per_cm2"/><initMembPotential value="{v}mV"/>'
+           '</membraneProperties><intracellularProperties><resistivity value="0.1 kohm_cm"/>'
+           '</intracellularProperties></biophysicalProperties></cell>')
+    cells = "".join(pas.format(n=n, v=v) for n, v in (("P1", -65), ("P2", -65), ("G1", -65), ("G2", -50),
+                                                        ("G3", -65), ("G4", -50)))
+    pops = "".join(f'<population id="p{n}" component="{n}" size="1"/>' for n in ("P1", "P2", "G1", "G2", "G3", "G4"))
+    conn = ('<projection id="{id}" presynapticPopulation="hhpop" postsynapticPopulation="{post}" synapse="{syn}">'
+            '<connectionWD id="0" preCellId="../hhpop/0/hhcell" postCellId="../{post}/0/{cell}" weight="2" '
+            'delay="1ms"/></projection>')
+    gap = ('<electricalProjection id="{id}" presynapticPopulation="{a}" postsynapticPopulation="{b}">'
+           '<electricalConnection id="0" preCell="0" preSegment="0" postCell="0" postSegment="0" synapse="{syn}"/>'
+           '</electricalProjection>')
+    net = (conn.format(id="c1", post="pP1", cell="P1", syn="e1") + conn.format(id="c2", post="pP2", cell="P2", syn="e2")
+           + gap.format(id="g1", a="pG1", b="pG2", syn="gj1") + gap.format(id="g2", a="pG3", b="pG4", syn="gj2"))
+    cols = "".join(f'<OutputColumn id="{n}" quantity="p{n}/0/{n}/0/v"/>' for n in ("P1", "P2", "G1", "G2", "G3", "G4"))
+    p = tmp_path / "user_types.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><Include file="NML2_SingleCompHHCell.nml"/><neuroml>' + _USER_TYPES +
+        '<expOneSynapse id="e1" gbase="0.5nS" erev="0mV" tauDecay="3ms"/>'
+        '<myExpOneSynapse id="e2" gbase="0.5nS" erev="0mV" tauDecay="3ms"/>'
+        '<gapJunction id="gj1" conductance="300pS"/><myGapJunction id="gj2" conductance="300pS"/>'
+        + cells + '<network id="n2"><population id="hhpop" component="hhcell" size="1"/>' + pops +
+        '<explicitInput target="hhpop[0]" input="pulseGen1"/>' + net + '</network></neuroml>'
+        '<Simulation id="s" length="300ms" step="0.025ms" target="n2" seed="1">'
+        f'<OutputFile id="o" fileName="o.txt"><OutputColumn id="hh" quantity="hhpop/0/hhcell/0/v"/>{cols}</OutputFile>'
+        '</Simulation></Lems>')
+    return str(p)
+
+
+def test_user_defined_component_types_equal_the_core_types(tmp_path):
+    """User ComponentTypes with the dynamics of expOneSynapse and gapJunction (as the
+    reference's EdenTestNetwork4 defines them) read into generic types whose runs equal the
+    core types' bit for bit; their parameters come from the base types they extend."""
+    m, outs = neuroml.read(_user_types_file(tmp_path))
+    assert m.lems.type_names == ["expOneSynapse", "myExpOneSynapse", "gapJunction", "myGapJunction"]
+    assert [list(p) for p in m.lems.params[:2]] == [list(m.lems.params[0])] * 2
+    (tr, ec, er, _), (st, sy) = oracle.run_with_synapse_log(m)
+    assert len(st) > 2                                           # the HH cell's spikes arrived
+    assert np.array_equal(tr[:, 1], tr[:, 2])                    # expOne vs the user type
+    assert np.array_equal(tr[:, 3:5], tr[:, 5:7])                # gap junction pair vs the user type
+    assert float(np.abs(tr[:, 1] + 65.0).max()) > 0.1 and float(np.abs(tr[:, 3] + 65.0).max()) > 0.1
