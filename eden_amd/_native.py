@@ -61,6 +61,7 @@ class ModelDesc(C.Structure):
         ("poisson_instance", _P), ("poisson_rng_offset", _P),
         ("n_record", C.c_int64), ("record_comp", _P), ("n_event_cols", C.c_int64), ("event_comp", _P),
         ("lems", C.POINTER(LemsSynapsesDesc)),
+        ("record_gate", _P),
     ]
 
 

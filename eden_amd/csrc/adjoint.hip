@@ -964,6 +964,8 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         const int64_t g = d->record_comp[r];
         if (g < 0 || g >= NC) return afail(a, EDENHIP_E_INVALID, "record_comp out of range");
         if (rec[g] >= 0) return afail(a, EDENHIP_E_INVALID, "a compartment may be recorded once in the adjoint model");
+        if (d->abi_version >= 3 && d->record_gate && d->record_gate[r] != -1)
+            return afail(a, EDENHIP_E_INVALID, "the adjoint model records membrane voltages only (record_gate -1)");
         rec[g] = (int32_t)r;
     }
     // synapses: types in double with the waveform factor evaluated in double with the

@@ -13,6 +13,7 @@
 #include <mutex>
 #include <cmath>
 #include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
@@ -86,6 +87,11 @@ struct edenhip_ctx {
     // host mirrors for output mapping
     std::vector<int32_t> desc_to_dev_col;   // per desc trace column
     int32_t n_dev_cols = 0;
+    // recorded gate variables: device column and source (>= 0: index into T.q of a point
+    // cell; < 0: ~index into the tree cells' gate table), written by record_gates_kernel
+    int32_t *d_grec_col = nullptr;
+    int64_t *d_grec_src = nullptr;
+    int32_t n_grec = 0;
     std::vector<std::vector<int64_t>> ev_expand;   // dev event column -> desc columns
     int64_t chunk_rows = 0, last_row0 = 0, last_rows = 0;
     DevTables T{};
@@ -233,6 +239,16 @@ static int static_scratch_ptr(edenhip_ctx *ctx, int32_t **out) {
 // Zero n ints with a kernel (not a memset node, so that a caller may capture the call).
 __global__ void zero_i32_kernel(int32_t *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Recorded gate variables into trace row `row`: column col[i] takes the point-cell gate state
+// T.q[src[i]] (src >= 0) or the tree-cell gate state gq[~src[i]] (src < 0).
+__global__ void record_gates_kernel(DevTables T, const float *__restrict__ gq, const int32_t *__restrict__ col,
+                                    const int64_t *__restrict__ src, int32_t n, int64_t row) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int64_t k = src[i];
+    trace_at(T, col[i], row) = k >= 0 ? T.q[k] : gq[~k];
 }
 
 // Streams come from a per-device pool created once per process and kept until it ends:
@@ -401,11 +417,33 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     std::vector<int32_t> rec_col(NC, -1), ev_col(NC, -1);
     ctx->desc_to_dev_col.assign(d->n_record, -1);
     ctx->n_dev_cols = 0;
+    // V columns first, then one column per distinct recorded gate variable
+    std::vector<std::pair<int64_t, int32_t>> grec;   // (compartment, gate) of gate columns
+    std::unordered_map<int64_t, int32_t> grec_at;    // compartment * 8 + gate -> index in grec
+    const bool gate_rec = d->abi_version >= 3 && d->record_gate;
     for (int64_t r = 0; r < d->n_record; r++) {
         int64_t g = d->record_comp[r];
         if (g < 0 || g >= NC) return fail(ctx, EDENHIP_E_INVALID, "record_comp out of range");
+        const int32_t j = gate_rec ? d->record_gate[r] : -1;
+        if (j < -1 || j >= (int32_t)((gmeta[g] >> 24) & 7u))
+            return fail(ctx, EDENHIP_E_INVALID, "record_gate: -1 or a gate variable of the compartment");
+        if (j >= 0) continue;
         if (rec_col[g] < 0) rec_col[g] = ctx->n_dev_cols++;
         ctx->desc_to_dev_col[r] = rec_col[g];
+    }
+    for (int64_t r = 0; gate_rec && r < d->n_record; r++) {
+        const int32_t j = d->record_gate[r];
+        if (j < 0) continue;
+        const int64_t g = d->record_comp[r];
+        auto it = grec_at.emplace(g * 8 + j, (int32_t)grec.size()).first;
+        if (it->second == (int32_t)grec.size()) grec.push_back({g, j});
+        ctx->desc_to_dev_col[r] = ctx->n_dev_cols + it->second;
+    }
+    ctx->n_dev_cols += (int32_t)grec.size();
+    // state read back between launches: recorded gate variables need one launch per step
+    if (!grec.empty()) {
+        ctx->opt.persistent = -1;
+        ctx->opt.network_window = -1;
     }
     if (d->n_event_cols >= (int64_t)1 << 24) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^24 event columns");
     ctx->ev_expand.assign((size_t)d->n_event_cols, {});
@@ -435,6 +473,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     std::vector<uint32_t> cflag(NC, 0);
     std::vector<TreeComplex> cxv;
     std::vector<uint32_t> gtv;   // per gate variable: its channel task (local to the cell) | gate index << 16
+    std::vector<int64_t> gq_first(NC, -1);   // tree compartments: index of the first gate variable in gqv
     // Hines schedules of the tree cells, planned in parallel (independent per cell); cells
     // with malformed parents are left unplanned here and rejected in the loop below
     std::vector<WavePlan> plans(d->n_cells);
@@ -483,6 +522,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         tc.ctask_off = (int32_t)ctask.size();
         for (int32_t i = 0; i < n; i++) {
             const int64_t g = c0 + i;
+            gq_first[g] = (int64_t)gqv.size();
             const int32_t first_task = (int32_t)(ctask.size() - tc.ctask_off);
             uint32_t mask = 0;
             for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
@@ -642,6 +682,21 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     ctx->network = any_targets || (lems && n_syn > 0);
     ctx->tree_net = ctx->tree_net || ctx->network;
     ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0;
+
+    // recorded gate variables: point cells keep theirs in T.q (NQ slots per compartment in
+    // distribution / gate order), tree cells in the gate table gqv
+    std::vector<int32_t> grec_col(grec.size());
+    std::vector<int64_t> grec_src(grec.size());
+    for (size_t k = 0; k < grec.size(); k++) {
+        const int64_t g = grec[k].first;
+        grec_col[k] = ctx->n_dev_cols - (int32_t)grec.size() + (int32_t)k;
+        grec_src[k] = gq_first[g] >= 0 ? ~(gq_first[g] + grec[k].second) : g * NQ + grec[k].second;
+    }
+    ctx->n_grec = (int32_t)grec.size();
+    if (ctx->n_grec > 0) {
+        TRY(upload(ctx, &ctx->d_grec_col, grec_col));
+        TRY(upload(ctx, &ctx->d_grec_src, grec_src));
+    }
 
     // device uploads
     DevTables &T = ctx->T;
@@ -1090,6 +1145,16 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
         for (int64_t a = a0; a < a1; a += stride) {
             const int64_t b = std::min(a1, a + stride);
             const bool first = a == s0, last = b == s1;
+            if (ctx->n_grec > 0 && a >= 1) {
+                // gate variables after step a - 1 into row a - 1, before step a changes them
+                // (one step per launch whenever gates are recorded)
+                const int64_t row = a - 1 - row0;
+                hipLaunchKernelGGL(record_gates_kernel, dim3((ctx->n_grec + 255) / 256), dim3(256), 0, ctx->stream, T,
+                                   (const float *)ctx->d_gq, (const int32_t *)ctx->d_grec_col,
+                                   (const int64_t *)ctx->d_grec_src, ctx->n_grec, row);
+                HIPCHECK(ctx, hipGetLastError());
+                nl++;
+            }
             if (ctx->lems_fn && ctx->n_lems_comps > 0) {
                 // generic synapses of step a: per-compartment currents for the kernels below
                 LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,

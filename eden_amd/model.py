@@ -106,6 +106,7 @@ class Model:
     event_comp: np.ndarray = None
     lems: object = None                # eden_amd.lems.LemsSynapses: generic synapse types
     syn_peer: np.ndarray = None        # per synapse instance: the gap junction's peer compartment, or -1
+    record_gate: np.ndarray = None     # per recorded column: -1 V, j >= 0 the compartment's j-th gate variable
     name: str = "model"
 
     def __post_init__(self):
@@ -147,6 +148,10 @@ class Model:
         for dt, names in conv.items():
             for n in names:
                 setattr(self, n, np.ascontiguousarray(getattr(self, n), dtype=dt))
+        if self.record_gate is not None:
+            self.record_gate = np.ascontiguousarray(self.record_gate, dtype=np.int32)
+            if self.record_gate.shape != self.record_comp.shape:
+                raise ValueError("record_gate: one entry per recorded column")
 
     @property
     def n_cells(self):
@@ -197,6 +202,7 @@ class Model:
             pulse_duration=self.pulse_duration[pu], pulse_amplitude=self.pulse_amplitude[pu],
             pulse_weight=self.pulse_weight[pu], syn_types=self.syn_types,
             record_comp=new_of[self.record_comp[rec_cols]], event_comp=new_of[self.event_comp[ev_cols]],
+            record_gate=None if self.record_gate is None else self.record_gate[rec_cols],
             name=f"{self.name}_subset{len(cells)}")
         return m, rec_cols, ev_cols
 

@@ -314,9 +314,10 @@ class _Cell:
     seg_ids: list            # segment ids in file order (parents first)
     parent: np.ndarray       # compartment parent (cell-local), -1 for the root
     seg_map: list            # per segment: [(compartment, fractionAlong where it ends on the segment)]
-    channels: list           # per compartment: [(channel id, gbase mS/cm^2, erev mV)]
+    channels: list           # per compartment: [(channel id, gbase mS/cm^2, erev mV, distribution id)]
     v0: np.ndarray           # per compartment
     vthr: np.ndarray
+    bioph: str = ""          # id of the biophysicalProperties
     area: np.ndarray = None
     cap: np.ndarray = None
     r_axial: np.ndarray = None
@@ -653,7 +654,7 @@ def _cell(doc, el, temperature, channel_index):
                     g = convert(s.get("condDensity"), "conductivity", "condDensity")
                     e = convert(s.get("erev"), "voltage", "erev")
                     for k in comps(s):
-                        chans[k].append((ch, _f32(g), _f32(e)))
+                        chans[k].append((ch, _f32(g), _f32(e), s.get("id")))
                 elif st == "channelDensityNonUniform":
                     ch = s.get("ionChannel")
                     if ch not in channel_index:
@@ -674,7 +675,7 @@ def _cell(doc, el, temperature, channel_index):
                         cl = comps(vp)
                         for k, g in zip(cl, _inhomogeneous_values(iv.get("value"), var, path_len[cl], sub_min,
                                                                   div_max, cid)):
-                            chans[k].append((ch, g, _f32(e)))
+                            chans[k].append((ch, g, _f32(e), s.get("id")))
                 elif st == "specificCapacitance":
                     for k in where(s):
                         cm[k] = convert(s.get("value"), "specific_capacitance", "specificCapacitance")
@@ -706,7 +707,7 @@ def _cell(doc, el, temperature, channel_index):
         raise NeuroMLError(f"cell {cid}: resistivity missing on some segments")
     ra = np.where(np.isnan(ra), 0.0, ra).astype(np.float32)
     cm = cm.astype(np.float32)
-    c = _Cell(cid, ids, cparent, seg_map, chans, v0.astype(np.float32), vthr.astype(np.float32))
+    c = _Cell(cid, ids, cparent, seg_map, chans, v0.astype(np.float32), vthr.astype(np.float32), b.get("id", ""))
     try:
         if not cables:
             c.area, c.cap, c.r_axial, c.inv_rc = N.cable_setup(parent, prox, dist, cm, ra)
@@ -747,8 +748,11 @@ def _cell_ref(text):
 
 
 def _quantity_path(text):
-    """(population, instance id, segment id, variable) of an OutputColumn quantity:
-    'pop/3/cell/5/v', 'pop/3/cell/v', 'pop[3]/v'."""
+    """(population, instance id, segment id, gate) of an OutputColumn quantity: the membrane
+    voltage ('pop/3/cell/5/v', 'pop/3/cell/v', 'pop[3]/v'; gate None) or a channel gate's
+    state, 'pop/3/cell/5/<biophysicalProperties id>/membraneProperties/<channel distribution
+    id>/<ion channel id>/<gate id>/q' (gate = (biophysics, distribution, channel, gate); the
+    path grammar of eden/NeuroML.cpp:1370-1371, 1820-1905)."""
     parts = [p for p in text.split("/") if p]
     m = _POP_IDX.match(parts[0])
     if m:
@@ -761,9 +765,12 @@ def _quantity_path(text):
     if len(rest) == 2 and rest[0].isdigit():
         seg = int(rest[0])
         rest = rest[1:]
+    if len(rest) == 7 and rest[0].isdigit() and rest[2] == "membraneProperties" and rest[6] == "q":
+        return pop, inst, int(rest[0]), (rest[1], rest[3], rest[4], rest[5])
     if rest != ["v"]:
-        raise NeuroMLError(f"quantity path '{text}': only the membrane voltage 'v' of a segment can be recorded")
-    return pop, inst, seg
+        raise NeuroMLError(f"quantity path '{text}': only the membrane voltage 'v' of a segment or a channel "
+                           "gate's 'q' can be recorded")
+    return pop, inst, seg, None
 
 
 def _select_path(text):
@@ -820,11 +827,12 @@ def read(path, rng_seed=None):
     temperature = convert(net.get("temperature"), "temperature", "temperature") if net.get("temperature") else None
 
     # channel types in document order, then the cells the network uses
-    channel_index, channel_types = {}, []
+    channel_index, channel_types, gate_ids = {}, [], {}
     for role, i in doc.order:
         if role == "channel":
             channel_index[i] = len(channel_types)
             channel_types.append(_channel(doc.items[i][1], temperature))
+            gate_ids[i] = [g.get("id") for g in doc.items[i][1] if _tag(g) not in ("notes", "annotation")]
     cells = {}
 
     def get_cell(cid, what):
@@ -1006,7 +1014,7 @@ def read(path, rng_seed=None):
     for c in range(n_cells):
         for lst in pop_of_cell[c].cell.channels:
             ch_n.append(len(lst))
-            for (ch, g, e) in lst:
+            for (ch, g, e, _) in lst:
                 ch_t.append(channel_index[ch])
                 ch_g.append(g)
                 ch_e.append(e)
@@ -1061,17 +1069,39 @@ def read(path, rng_seed=None):
     # outputs
     base_dir = os.path.dirname(os.path.abspath(path))
     trajectories, events = [], []
-    rec, evc = [], []
+    rec, evc, rec_gate = [], [], []
+
+    def gate_variable(cell, g, gate, what):
+        """The compartment's gate variable a gate path names: the first channel distribution
+        of that id on the compartment (eden/NeuroML.cpp:1843-1853), the gate's place in its
+        channel, counted after the gates of the compartment's earlier distributions."""
+        bioph, dist, chan, gid = gate
+        if bioph not in (cell.bioph, "biophysicalProperties"):
+            raise NeuroMLError(f"{what}: cell {cell.id} has no biophysicalProperties '{bioph}'")
+        lst = cell.channels[g - int(cell_begin[np.searchsorted(cell_begin, g, side="right") - 1])]
+        j = 0
+        for (ch, _, _, did) in lst:
+            if did == dist:
+                if ch != chan:
+                    raise NeuroMLError(f"{what}: distribution {dist} holds ion channel {ch}, not {chan}")
+                if gid not in gate_ids[ch]:
+                    raise NeuroMLError(f"{what}: ion channel {ch} has no gate {gid}")
+                return j + gate_ids[ch].index(gid)
+            j += len(gate_ids[ch])
+        raise NeuroMLError(f"{what}: no channel distribution {dist} on that segment of cell {cell.id}")
+
     for el in sim:
         t = _tag(el)
         if t == "OutputFile":
             f = OutputFileSpec(el.get("id"), el.get("fileName"))
             for c in el:
                 if _tag(c) == "OutputColumn":
-                    pop, inst, seg = _quantity_path(c.get("quantity"))
-                    g = comp(pop, inst, seg, f"OutputColumn {c.get('id')}")
+                    pop, inst, seg, gate = _quantity_path(c.get("quantity"))
+                    what = f"OutputColumn {c.get('id')}"
+                    g = comp(pop, inst, seg, what)
                     f.columns.append((c.get("id"), c.get("quantity"), len(rec)))
                     rec.append(g)
+                    rec_gate.append(-1 if gate is None else gate_variable(pops[pop].cell, g, gate, what))
             trajectories.append(f)
         elif t == "EventOutputFile":
             fmt = el.get("format", "TIME_ID")
@@ -1115,6 +1145,7 @@ def read(path, rng_seed=None):
         poisson_weight=f32(po, 2), poisson_instance=np.array([x[3] for x in po], np.int32),
         poisson_rng_offset=np.zeros(len(po), np.int32),
         record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
+        record_gate=np.array(rec_gate, np.int32) if any(j >= 0 for j in rec_gate) else None,
         lems=lems_mod.build(generic_types, {k: v[0] for k, v in custom.items()}) if generic else None,
         name=os.path.splitext(os.path.basename(path))[0])
     model.sim_seed = seed

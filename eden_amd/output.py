@@ -38,6 +38,16 @@ def write_trajectory(path, time_us, trace, scale=MV_TO_V, append=False):
         raise OSError(f"write_trajectory({path}) failed ({rc})")
 
 
+def column_scales(model):
+    """Per recorded column, the factor from engine units to the file's: 1e-3 for a membrane
+    voltage (mV to V), 1 for a gate variable (dimensionless, LEMS_Pure in
+    eden/NeuroML.cpp:1016)."""
+    sc = np.full(len(model.record_comp), MV_TO_V)
+    if model.record_gate is not None:
+        sc[np.asarray(model.record_gate) >= 0] = 1.0
+    return sc
+
+
 def write_events(path, time_us, ev_row, ev_col, ids, fmt="TIME_ID", append=False):
     """Write events (row, column), sorted by row then column; column c is printed as ids[c]."""
     ev_row = np.ascontiguousarray(ev_row, np.int64)

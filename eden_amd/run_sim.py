@@ -38,6 +38,7 @@ def run_lems(path, rng_seed=None, chunk_rows=None, verbose=False, out_dir=None, 
     t_us = output.row_times_us(model.dt, rows_total + 1)
     where = (lambda f: os.path.join(out_dir, f)) if out_dir else (lambda f: f)
     traj = [(where(f.file_name), [c[2] for c in f.columns]) for f in outs.trajectories]
+    scales = output.column_scales(model)
     with Simulation(model, chunk_steps=chunk_rows, **sim_options) as sim:
         t2 = time.time()
         # writer thread: formats and writes chunk k while the GPU runs chunk k + 1 (the
@@ -55,7 +56,8 @@ def run_lems(path, rng_seed=None, chunk_rows=None, verbose=False, out_dir=None, 
                 tr, row0, first = item
                 try:
                     for fname, cols in traj:
-                        output.write_trajectory(fname, t_us[row0:row0 + len(tr)], tr[:, cols], append=not first)
+                        output.write_trajectory(fname, t_us[row0:row0 + len(tr)], tr[:, cols], scale=scales[cols],
+                                                append=not first)
                 except Exception as e:   # reported after the run
                     err.append(e)
 

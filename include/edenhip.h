@@ -182,6 +182,12 @@ typedef struct edenhip_model_desc {
 
     /* (abi_version >= 2) generic LEMS synapse types, or NULL */
     const edenhip_lems_synapses *lems;
+
+    /* (abi_version >= 3) what each recorded column holds: -1 = the compartment's V; j >= 0 =
+     * its j-th gate variable (its channel distributions in order, each type's gates in order),
+     * as an OutputColumn of a channel gate's q records it. NULL = V everywhere. A model that
+     * records gate variables runs one launch per step. */
+    const int32_t *record_gate;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

@@ -240,6 +240,17 @@ typedef struct {
 
 #define ALLOC(p, n, T) do { (p) = (T *)calloc((size_t)((n) > 0 ? (n) : 1), sizeof(T)); } while (0)
 
+/* The recorded quantity of column r: the compartment's V, or (record_gate[r] = j >= 0) the
+ * j-th gate variable of its channel instances taken in order (an OutputColumn path ending in
+ * a gate's q, eden/NeuroML.cpp:1820-1905, names one gate of one distribution; its value is a
+ * state variable of the step, read at output time like V). */
+static real recorded(const edenhip_model_desc *d, const ostate *s, int now, int64_t r) {
+    const int32_t g = d->record_comp[r];
+    const int32_t j = (d->abi_version >= 3 && d->record_gate) ? d->record_gate[r] : -1;
+    if (j < 0) return s->V[now][g];
+    return s->q[now][s->gate_begin[d->comp_chan_begin[g]] + j];
+}
+
 static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int now, int nxt, int64_t step,
                     double time_f64, real dt) {
     const int initial_state = (step <= 0);
@@ -514,6 +525,14 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     const int64_t NCH = d->comp_chan_begin[NC];
     const int64_t NSY = d->comp_syn_begin[NC];
     const int64_t NP = d->comp_poisson_begin[NC];
+    if (d->abi_version >= 3 && d->record_gate)
+        for (int64_t r = 0; r < d->n_record; r++) {
+            const int32_t g = d->record_comp[r];
+            int64_t ng = 0;
+            for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++)
+                ng += d->channel_types[d->chan_type[ch]].n_gates;
+            if (d->record_gate[r] < -1 || d->record_gate[r] >= ng) return -1;
+        }
     ostate s;
     memset(&s, 0, sizeof(s));
     ALLOC(s.gate_begin, NCH + 1, int64_t);
@@ -582,9 +601,9 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
             if (fine % substeps == 0) {
                 if (n_rows >= 0 && rows >= n_rows) break;
                 if (trace)
-                    for (int64_t r = 0; r < d->n_record; r++) trace[rows * d->n_record + r] = (float)s.V[now][d->record_comp[r]];
+                    for (int64_t r = 0; r < d->n_record; r++) trace[rows * d->n_record + r] = (float)recorded(d, &s, now, r);
                 if (g_trace64)
-                    for (int64_t r = 0; r < d->n_record; r++) g_trace64[rows * d->n_record + r] = (double)s.V[now][d->record_comp[r]];
+                    for (int64_t r = 0; r < d->n_record; r++) g_trace64[rows * d->n_record + r] = (double)recorded(d, &s, now, r);
                 rows++;
             }
             for (int64_t e = 0; e < d->n_event_cols; e++) {

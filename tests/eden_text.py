@@ -12,6 +12,8 @@ the timestamp, then ' ' + number per column (eden/Eden.cpp:10856-10870); event l
 are engine microseconds times 1e-6 (GetTimestampAscii, eden/Eden.cpp:184-189). Python's
 %-formatting and C's printf both print the correctly rounded decimal.
 """
+import numpy as np
+
 COLUMN_SIZE, DELIM_CHAR, DELIM_SIZE = 16, "\t", 0
 NUMBER_SIZE = COLUMN_SIZE - DELIM_SIZE
 DIGITS = COLUMN_SIZE - 3 - 5
@@ -25,8 +27,10 @@ def fx(v):
 
 
 def trajectory_text(t_us, trace, scale=1e-3):
+    """`scale`: one factor for every column, or one per column."""
     rows, cols = trace.shape
-    return "".join(fx(t_us[r] * 1e-6) + "".join(" " + fx(float(trace[r, c]) * scale) for c in range(cols)) + "\n"
+    sc = [float(x) for x in np.broadcast_to(np.asarray(scale, np.float64), (cols,))]
+    return "".join(fx(t_us[r] * 1e-6) + "".join(" " + fx(float(trace[r, c]) * sc[c]) for c in range(cols)) + "\n"
                    for r in range(rows))
 
 

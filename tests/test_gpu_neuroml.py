@@ -39,8 +39,10 @@ def _expected_files(model, outs):
     tr, ec, er, _ = oracle.run(model)
     t_us = output.row_times_us(model.dt, model.total_rows() + 1)
     files = {}
+    scales = output.column_scales(model)
     for f in outs.trajectories:
-        files[f.file_name] = trajectory_text(t_us, tr[:, [c[2] for c in f.columns]])
+        cols = [c[2] for c in f.columns]
+        files[f.file_name] = trajectory_text(t_us, tr[:, cols], scales[cols])
     for f in outs.events:
         cols = [s[2] for s in f.selections]
         keep = np.isin(ec, cols)
@@ -98,6 +100,25 @@ def test_reference_inhomogeneous_parameters_files(tmp_path):
         tr = sim.run_all()
     assert np.array_equal(tr[:, 0:3], tr[:, 3:6])
     assert float(np.abs(tr[:, 6:9] - tr[:, 0:3]).max()) > 1.0
+
+
+@pytest.mark.parametrize("name", ["LEMS_EdenTest2.xml", "LEMS_EdenTest1.xml"])
+def test_reference_gate_recording_files(tmp_path, name):
+    """The reference's EdenTest1 (a network of multi-compartment cells recording the m gate
+    of a dendrite compartment) and EdenTest2 (one HH cell recording m, h and n next to V):
+    output files byte-equal to the oracle's run, the GPU trace equal to the oracle's."""
+    path = os.path.join(HERE, "golden", "neuroml", name)
+    res = run_lems(path, out_dir=str(tmp_path))
+    m, outs = res["model"], res["outputs"]
+    _say(f"[neuroml] {name}: {m.n_comps} compartments, gate columns {list(m.record_gate)}, {res['rows']} rows")
+    want, n_ev = _expected_files(m, outs)
+    for fname, text in want.items():
+        assert_same_text((tmp_path / fname).read_text(), text)
+    with Simulation(m) as sim:
+        tr = sim.run_all()
+    tr_r, _, _, _ = oracle.run(m)
+    assert np.array_equal(tr, tr_r)
+    assert (m.record_gate >= 0).any()
 
 
 def test_runeden_return_shape(tmp_path, monkeypatch):

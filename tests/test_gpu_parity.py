@@ -424,3 +424,60 @@ def test_output_files_from_a_gpu_run(tmp_path):
     assert nev > 0
     assert_same_text(tp.read_text(), trajectory_text(t_us, tr))
     assert_same_text(ep.read_text(), events_text(t_us, er, ec, np.arange(len(m.event_comp)), "TIME_ID"))
+
+
+def _with_gate_columns(m, comps, name):
+    """The model also recording every gate variable of the given compartments, after its own
+    V columns (and V of those compartments again, as a column of its own)."""
+    import dataclasses
+    rc, rg = list(map(int, m.record_comp)), [-1] * len(m.record_comp)
+    for g in comps:
+        ng = sum(m.channel_types[int(t)].n_gates for t in m.chan_type[m.comp_chan_begin[g]:m.comp_chan_begin[g + 1]])
+        rc += [int(g)] * (ng + 1)
+        rg += list(range(ng)) + [-1]
+    return dataclasses.replace(m, record_comp=np.array(rc, np.int64), record_gate=np.array(rg, np.int32), name=name)
+
+
+@pytest.mark.parametrize("kind", ["point", "trees", "trees_graph_chunks", "network"])
+def test_gate_columns_bit_exact(kind):
+    """Gate variables recorded next to V (point cells from T.q, tree cells from the gate
+    table, a network of both): every column equals the oracle's exactly, the V columns equal
+    those of the same model recording V only (run in its default persistent mode), and the
+    gate columns hold values in [0, 1] that move."""
+    if kind == "point":
+        m = configs.hh_population(n_cells=40, t_final_ms=30.0, seed=5)
+        comps, network, kw = [0, 7, 39], False, {}
+    elif kind == "network":
+        m = configs.mixed_network(t_final_ms=30.0, hh_everywhere=True)
+        comps, network, kw = [0, 5, 41, m.n_comps - 1], True, {}
+    else:
+        sizes = [40, 90, 300]
+        m = configs.random_trees(len(sizes), sizes=sizes, t_final_ms=15.0, seed=41, hh_everywhere=True)
+        comps, network = [0, 3, 40, 129, 200], False
+        kw = {} if kind == "trees" else dict(use_graph=True, chunk_steps=97)
+    n_v = len(m.record_comp)
+    with Simulation(m) as sim:
+        v_only = sim.run_all()
+    mg = _with_gate_columns(m, comps, f"{m.name}_gates")
+    _say(f"[gates] {kind}: {len(mg.record_comp) - n_v} extra columns, opts={kw}")
+    tr_g, ev_g = _gpu_run(mg, **kw)
+    tr_r, ec, er, _ = oracle.run(mg)
+    assert np.array_equal(tr_g, tr_r)
+    assert np.array_equal(tr_g[:, :n_v], v_only)
+    gate = mg.record_gate[n_v:] >= 0
+    q = tr_g[:, n_v:][:, gate]
+    assert q.min() >= 0.0 and q.max() <= 1.0 and float(np.ptp(q, axis=0).min()) > 1e-3
+    # the extra V columns repeat the compartments' V
+    again = np.flatnonzero(mg.record_gate[n_v:] < 0) + n_v
+    assert np.array_equal(tr_g[:, again], tr_r[:, again])
+    if network:
+        assert len(er) > 0
+
+
+def test_gate_columns_refused_when_out_of_range():
+    m = configs.hh_population(n_cells=4, t_final_ms=1.0, seed=5)
+    import dataclasses
+    for j in (3, -2):   # the HH compartment has gate variables 0, 1, 2
+        bad = dataclasses.replace(m, record_gate=np.array([j] + [-1] * (len(m.record_comp) - 1), np.int32))
+        with pytest.raises(EdenHipError, match="record_gate"):
+            Simulation(bad)

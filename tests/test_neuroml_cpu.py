@@ -248,3 +248,67 @@ def test_reference_inhomogeneous_parameters_model():
     np.testing.assert_allclose([inho[1][0], inho[2][0]], [50 * np.exp(-15 / 20), 50 * np.exp(-25 / 20)], rtol=1e-6)
     np.testing.assert_allclose([inhn[1][0], inhn[2][0]], [50.0, 50 * np.exp(-1 / 20)], rtol=1e-6)
     assert inho[1][1] == inho[2][1] == inhn[1][1] == 36.0
+
+
+def _hh_rates(v):
+    """Hodgkin-Huxley rates (1/ms) of the m, h and n gates at v (mV), as the reference's
+    EdenTestNetwork2 channels state them (HHExpLinearRate, HHExpRate, HHSigmoidRate)."""
+    am = 1.0 * ((v + 40) / 10) / (1 - np.exp(-(v + 40) / 10))
+    bm = 4.0 * np.exp((v + 65) / -18)
+    ah = 0.07 * np.exp((v + 65) / -20)
+    bh = 1.0 / (1 + np.exp(-(v + 35) / 10))
+    an = 0.1 * ((v + 55) / 10) / (1 - np.exp(-(v + 55) / 10))
+    bn = 0.125 * np.exp((v + 65) / -80)
+    return np.array([[am, bm], [ah, bh], [an, bn]])
+
+
+def test_reference_gate_recording_model_reads_its_gate_columns():
+    """The reference's EdenTest2 model records m, h and n of the naChans / kChans
+    distributions next to V: the paths resolve to the compartment's gate variables 0, 1 and
+    2 (the leak distribution before them has no gates), read at scale 1 (V at 1e-3)."""
+    from eden_amd import output
+    m, outs = neuroml.read(os.path.join(GOLD, "LEMS_EdenTest2.xml"))
+    assert list(m.record_comp) == [0, 0, 0, 0]
+    assert list(m.record_gate) == [-1, 0, 1, 2]
+    assert list(output.column_scales(m)) == [1e-3, 1.0, 1.0, 1.0]
+    m1, _ = neuroml.read(os.path.join(GOLD, "LEMS_EdenTest1.xml"))
+    # pop0/2/MultiCompCell/1 is compartment 1 of the third cell; m is its first gate variable
+    assert list(m1.record_comp) == [0, 2, 5, 4, 5] and list(m1.record_gate) == [-1, -1, -1, -1, 0]
+    assert neuroml._quantity_path("p/0/c/3/bio/membraneProperties/d/ch/g/q") == ("p", 0, 3, ("bio", "d", "ch", "g"))
+
+
+def test_gate_paths_that_name_nothing_are_refused(tmp_path):
+    src = open(os.path.join(GOLD, "LEMS_EdenTest2.xml")).read()
+    import shutil
+    shutil.copy(os.path.join(GOLD, "EdenTestNetwork2.nml"), tmp_path / "EdenTestNetwork2.nml")
+    for bad, why in (("naChans/naChan/x/q", "no gate x"), ("naChans/kChan/n/q", "holds ion channel naChan"),
+                     ("caChans/caChan/m/q", "no channel distribution caChans"),
+                     ("naChans/naChan/m/tau", "can be recorded")):
+        p = tmp_path / "LEMS_bad.xml"
+        p.write_text(src.replace("naChans/naChan/m/q", bad))
+        with pytest.raises(neuroml.NeuroMLError, match=why):
+            neuroml.read(str(p))
+
+
+def test_oracle_gate_columns_are_hh_steady_states_then_cnexp_steps():
+    """Oracle gate columns of EdenTest2: row 0 holds the textbook resting values
+    alpha / (alpha + beta) at -65 mV (m 0.0529, h 0.5961, n 0.3177); every later row follows
+    from the one before by the exponential-Euler gate step at that row's recorded V
+    (q' = q_inf + (q - q_inf) exp(-dt (alpha + beta))), so V and the gates of a row belong
+    to the same instant. A gate column taken from the wrong channel instance or gate, or a
+    row late or early, fails one of these."""
+    from oracle import oracle
+    m, _ = neuroml.read(os.path.join(GOLD, "LEMS_EdenTest2.xml"))
+    tr, _, _, _ = oracle.run(m)
+    tr = tr.astype(np.float64)
+    r = _hh_rates(-65.0)
+    np.testing.assert_allclose(tr[0, 1:], r[:, 0] / r.sum(1), rtol=2e-6)
+    np.testing.assert_allclose(tr[0, 1:], [0.0529325, 0.5961208, 0.3176769], rtol=1e-5)
+    dt = float(np.float32(m.dt)) * 1e-3   # ms
+    for row in range(len(tr) - 1):
+        r = _hh_rates(tr[row, 0])
+        tot = r.sum(1)
+        inf = r[:, 0] / tot
+        want = inf + (tr[row, 1:] - inf) * np.exp(-dt * tot)
+        np.testing.assert_allclose(tr[row + 1, 1:], want, atol=2e-6, err_msg=f"row {row + 1}")
+    assert tr[:, 0].max() > 0.0    # the cell fires: the gates move far from rest
