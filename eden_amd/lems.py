@@ -8,15 +8,21 @@ that the library compiles with hiprtc into a kernel template at context creation
 (eden_amd/csrc/lems_kernel.hip, include/edenhip.h `edenhip_lems_synapses`); the oracle
 compiles the C dialect of the same text with gcc.
 
-Semantics of one step of one instance (eden/Eden.cpp:2800-2995):
+Semantics of one step of one instance (eden/Eden.cpp:2700-2995):
   - parameters, derived parameters (from parameters), the `weight` property;
-  - derived variables from the present state, in dependency order;
-  - exposures i (current) and g (conductance; 0 for current-based synapses);
-  - on an initialisation step: each OnStart assignment, evaluated on the present state;
+  - derived variables from the present state, in dependency order; a conditional derived
+    variable takes the value of its first case whose condition holds, else its default case,
+    else 0 (eden/Eden.cpp:2749-2776);
+  - on an initialisation step: the OnStart assignments in order;
   - otherwise: next = state + dt * derivative for every state variable with a time
     derivative (each derivative from the present state), next = state for the others; then
-    OnEvent assignments (evaluated on the present state) replace the next value when the
-    instance receives its event this step.
+    the OnCondition blocks in order, each whose test holds running its assignments, then the
+    OnEvent assignments when the instance receives its event this step;
+  - an assignment sets the variable's present value and its next value (Emit_AssignState,
+    eden/Eden.cpp:2865-2887), so later tests and assignments of the step read it, while
+    derived variables keep the values computed at the top;
+  - exposures i (current) and g (conductance; 0 for current-based synapses) last, a state
+    variable's as assigned.
 Expressions are printed fully parenthesised, left to right as parsed, with every literal
 rounded to float, `expf`/`logf` as EXP/LOG (the dialect's own: on the device and in the
 oracle, exp/log rounded once from double), so both compilers perform the same float
@@ -25,7 +31,8 @@ factor of these types 1 (the reference's Convert::Suffix then emits nothing).
 
 The ComponentType table restates the dynamics of the NeuroML 2 core synapse types (the
 NeuroML standard's definitions, as embedded by the reference in
-eden/neuroml/LEMS_CoreComponents.inc.xml:85-229, 1834-1942).
+eden/neuroml/LEMS_CoreComponents.inc.xml:7-372, 1834-1942); a blockingPlasticSynapse is
+composed with its block and plasticity mechanisms into one type (blocking_plastic_type).
 """
 import re
 from dataclasses import dataclass, field
@@ -43,8 +50,10 @@ class LemsError(ValueError):
 class SynapseType:
     """A synapse ComponentType: `params` (name, NeuroML dimension or None for plain numbers),
     `derived_params` (name, expression), `constants` (name, engine value), `state` (names),
-    `derived` (name, expression) in dependency order, `derivatives` (state name ->
-    expression), `on_start` and `on_event` (state name -> expression), exposures `i` and
+    `derived` (name, expression, or name, [(condition or None for the default, expression),
+    ...] for a conditional one) in dependency order, `derivatives` (state name ->
+    expression), `on_start` and `on_event` (state name -> expression, in order),
+    `on_conditions` [(test, [(state name, expression), ...]), ...], exposures `i` and
     (optionally) `g` (names of derived or state variables)."""
     params: list
     state: list
@@ -56,6 +65,7 @@ class SynapseType:
     expose_g: str = None
     derived_params: list = field(default_factory=list)
     constants: list = field(default_factory=list)
+    on_conditions: list = field(default_factory=list)
 
 
 _COND = [("gbase", "conductance"), ("erev", "voltage")]
@@ -63,10 +73,13 @@ _PYNN = [("tau_syn", None)]
 _PYNN_CONST = [("MVOLT", 1.0), ("MSEC", 1000.0), ("NAMP", 1.0)]   # 1 mV, 1 ms, 1 nA in engine units
 
 SYNAPSE_TYPES = {
+    # the reference runs expOneSynapse natively, exposing the conductance of the step's start
+    # also on an event step (eden/Eden.cpp:3424-3460): hence the exposure through a derived
+    # variable, not the state variable an event assignment changes
     "expOneSynapse": SynapseType(
-        params=_COND + [("tauDecay", "time")], state=["g"], derived=[("i", "g * (erev - v)")],
+        params=_COND + [("tauDecay", "time")], state=["g"], derived=[("i", "g * (erev - v)"), ("g0", "g")],
         derivatives={"g": "-g / tauDecay"}, on_start={"g": "0"}, on_event={"g": "g + (gbase * weight)"},
-        expose_i="i", expose_g="g"),
+        expose_i="i", expose_g="g0"),
     "alphaCurrentSynapse": SynapseType(
         params=[("tau", "time"), ("ibase", "current")], state=["I", "J"], derived=[("i", "I")],
         derivatives={"I": "(2.7182818284590451*J - I)/tau", "J": "-J/tau"},
@@ -119,7 +132,65 @@ SYNAPSE_TYPES = {
     "gapJunction": SynapseType(
         params=[("conductance", "conductance")], state=[], derived=[("i", "conductance * (vpeer - v)")],
         derivatives={}, on_start={}, on_event={}, expose_i="i"),
+    # the two ends of a continuous (graded) connection; vpeer the voltage at the other end
+    "silentSynapse": SynapseType(
+        params=[], state=[], derived=[("i", "0")], derivatives={}, on_start={}, on_event={}, expose_i="i"),
+    "linearGradedSynapse": SynapseType(
+        params=[("conductance", "conductance")], state=[], derived=[("i", "conductance * (vpeer - v)")],
+        derivatives={}, on_start={}, on_event={}, expose_i="i"),
+    "gradedSynapse": SynapseType(
+        params=[("conductance", "conductance"), ("delta", "voltage"), ("k", "frequency"), ("Vth", "voltage"),
+                ("erev", "voltage")],
+        state=["s"],
+        derived=[("inf", "1/(1 + exp((Vth - vpeer)/delta))"), ("tau", "(1-inf)/k"),
+                 ("i", "conductance * s * (erev-v)"),
+                 ("s_rate", [("(1-inf) .gt. 1e-4", "(inf - s)/tau"), (None, "0")])],
+        derivatives={"s": "s_rate"}, on_start={"s": "0"}, on_event={},
+        on_conditions=[("(1-inf) .lt. 1e-4", [("s", "inf")])], expose_i="i"),
 }
+
+# blockingPlasticSynapse's child mechanisms (LEMS_CoreComponents.inc.xml:231-312): parameters,
+# state, derived variables (exposing blockFactor / plasticityFactor), derivatives, OnStart and
+# OnEvent (port in: the synapse's own event)
+BLOCK_MECHANISMS = {
+    "voltageConcDepBlockMechanism": dict(
+        params=[("blockConcentration", "concentration"), ("scalingConc", "concentration"), ("scalingVolt", "voltage")],
+        derived=[("blockFactor", "1/(1 + (blockConcentration / scalingConc)* exp(-1 * (v / scalingVolt)))")]),
+}
+PLASTICITY_MECHANISMS = {
+    "tsodyksMarkramDepMechanism": dict(
+        params=[("initReleaseProb", None), ("tauRec", "time")], state=["R"],
+        derived=[("U", "initReleaseProb"), ("plasticityFactor", "R * U")], derivatives={"R": "(1 - R) / tauRec"},
+        on_start={"R": "1"}, on_event={"R": "R * (1 - U)"}),
+    "tsodyksMarkramDepFacMechanism": dict(
+        params=[("initReleaseProb", None), ("tauRec", "time"), ("tauFac", "time")], state=["R", "U"],
+        derived=[("plasticityFactor", "R * U")],
+        derivatives={"R": "(1 - R) / tauRec", "U": "(initReleaseProb - U) / tauFac"},
+        on_start={"R": "1", "U": "initReleaseProb"},
+        on_event={"R": "R * (1 - U)", "U": "U + initReleaseProb * (1 - U)"}),
+}
+
+
+def blocking_plastic_type(block=None, plasticity=None):
+    """One SynapseType for a blockingPlasticSynapse with the given block and plasticity
+    mechanism types (None: absent, factor 1). The reference runs the three components one
+    after the other for each instance, block first, then plasticity, then the synapse, which
+    reads the mechanisms' exposures as computed from their present state
+    (eden/Eden.cpp:3491-3532, LEMS_CoreComponents.inc.xml:315-372); their variables do not
+    share names, so the composition is one namespace in that order."""
+    b = BLOCK_MECHANISMS[block] if block else dict(params=[], derived=[("blockFactor", "1")])
+    p = PLASTICITY_MECHANISMS[plasticity] if plasticity else dict(
+        params=[], state=[], derived=[("plasticityFactor", "1")], derivatives={}, on_start={}, on_event={})
+    return SynapseType(
+        params=_COND + [("tauRise", "time"), ("tauDecay", "time")] + b["params"] + p["params"],
+        derived_params=SYNAPSE_TYPES["expTwoSynapse"].derived_params,
+        state=p["state"] + ["A", "B"],
+        derived=b["derived"] + p["derived"] + [("g", "blockFactor * gbase * (B - A)"), ("i", "g * (erev - v)")],
+        derivatives={**p["derivatives"], "A": "-A / tauRise", "B": "-B / tauDecay"},
+        on_start={**p["on_start"], "A": "0", "B": "0"},
+        on_event={**p["on_event"], "A": "A + (weight * plasticityFactor * waveformFactor)",
+                  "B": "B + (weight * plasticityFactor * waveformFactor)"},
+        expose_i="i", expose_g="g")
 
 # PyNN synapses' numbers are plain (ms, mV); their base type gives tau_syn no default here
 
@@ -134,6 +205,7 @@ BASE_SYNAPSES = {
 # LEMS dimension names -> the reader's unit dimensions (None: dimensionless)
 LEMS_DIMS = {"time": "time", "per_time": "frequency", "voltage": "voltage", "current": "current",
              "conductance": "conductance", "capacitance": "capacitance", "length": "length",
+             "concentration": "concentration",
              "resistance": "resistance", "conductanceDensity": "conductivity", "temperature": "temperature",
              "none": None, "": None}
 
@@ -156,9 +228,10 @@ def synapse_type_from_lems(name, component_types, convert):
     """SynapseType of a user-defined synapse <ComponentType> (the reference turns any lemsified
     ComponentType into C, eden/Eden.cpp:2339-2995): Parameters and DerivedParameters, Constants
     (converted to engine units by `convert(text, dimension)`), the `weight` Property, and the
-    nearest Dynamics along the extends chain: StateVariables, DerivedVariables (ordered by
-    their dependencies), TimeDerivatives, OnStart and OnEvent (port `in`) StateAssignments;
-    exposures `i` and `g`. Returns (SynapseType, default weight)."""
+    nearest Dynamics along the extends chain: StateVariables, DerivedVariables and
+    ConditionalDerivedVariables (ordered by their dependencies), TimeDerivatives, OnStart,
+    OnCondition and OnEvent (port `in`) StateAssignments; exposures `i` and `g`. Returns
+    (SynapseType, default weight)."""
     chain, n = [], name
     while n in component_types:
         chain.append(component_types[n])
@@ -188,7 +261,7 @@ def synapse_type_from_lems(name, component_types, convert):
                 dyn = el
     if dyn is None:
         raise LemsError(f"ComponentType {name} has no Dynamics")
-    state, derived, ders, on_start, on_event, expose = [], {}, {}, {}, {}, {}
+    state, derived, ders, on_start, on_event, expose, on_cond = [], {}, {}, {}, {}, {}, []
     for el in dyn:
         t = _ltag(el)
         if t == "StateVariable":
@@ -201,8 +274,23 @@ def synapse_type_from_lems(name, component_types, convert):
             derived[el.get("name")] = el.get("value")
             if el.get("exposure"):
                 expose[el.get("exposure")] = el.get("name")
+        elif t == "ConditionalDerivedVariable":
+            cases = [(c.get("condition"), c.get("value")) for c in el if _ltag(c) == "Case"]
+            if not cases:
+                raise LemsError(f"{name}: conditional derived variable {el.get('name')} has no cases")
+            derived[el.get("name")] = cases
+            if el.get("exposure"):
+                expose[el.get("exposure")] = el.get("name")
         elif t == "TimeDerivative":
             ders[el.get("variable")] = el.get("value")
+        elif t == "OnCondition":
+            assigns = []
+            for a in el:
+                if _ltag(a) == "StateAssignment":
+                    assigns.append((a.get("variable"), a.get("value")))
+                else:
+                    raise LemsError(f"{name}: <{_ltag(a)}> in <OnCondition> is not supported in a synapse")
+            on_cond.append((el.get("test"), assigns))
         elif t in ("OnStart", "OnEvent"):
             if t == "OnEvent" and el.get("port", "in") != "in":
                 raise LemsError(f"{name}: OnEvent port '{el.get('port')}' is not supported")
@@ -212,13 +300,14 @@ def synapse_type_from_lems(name, component_types, convert):
                     tgt[a.get("variable")] = a.get("value")
                 elif _ltag(a) != "EventOut":
                     raise LemsError(f"{name}: <{_ltag(a)}> in <{t}> is not supported")
-        elif t in ("OnCondition", "ConditionalDerivedVariable", "Regime", "KineticScheme"):
+        elif t in ("Regime", "KineticScheme"):
             raise LemsError(f"{name}: <{t}> is not supported")
     if "i" not in expose:
         raise LemsError(f"ComponentType {name} exposes no current i")
 
     def syms(expr):
-        out, stack = set(), [parse(expr)]
+        texts = [expr] if isinstance(expr, str) else [x for case in expr for x in case if x is not None]
+        out, stack = set(), [parse(x) for x in texts]
         while stack:
             x = stack.pop()
             if x[0] == "sym":
@@ -244,13 +333,17 @@ def synapse_type_from_lems(name, component_types, convert):
         visit(v)
     st = SynapseType(params=params, state=state, derived=order, derivatives=ders, on_start=on_start,
                      on_event=on_event, expose_i=expose["i"], expose_g=expose.get("g"), derived_params=dparams,
-                     constants=consts)
+                     constants=consts, on_conditions=on_cond)
     return st, weight
 
 
 # ---------------------------------------------------------------- expressions
 
-_TOKEN = re.compile(r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|([A-Za-z_]\w*)|(\*\*|[-+*/^(),]))")
+_TOKEN = re.compile(r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|([A-Za-z_]\w*)|"
+                    r"(\*\*|[-+*/^(),]|\.(?:gt|lt|geq|leq|ge|le|eq|neq|ne|and|or|not)\.))")
+# LEMS relational operators and their C forms
+_REL = {".gt.": ">", ".lt.": "<", ".geq.": ">=", ".ge.": ">=", ".leq.": "<=", ".le.": "<=", ".eq.": "==",
+        ".neq.": "!=", ".ne.": "!="}
 _FUNCS = {"exp": "EXP", "log": "LOG", "ln": "LOG", "sqrt": "sqrtf", "abs": "fabsf", "sin": "sinf", "cos": "cosf",
           "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "ceil": "ceilf", "floor": "floorf"}
 
@@ -269,8 +362,10 @@ def _tokens(text):
 
 
 def parse(text):
-    """Expression tree: ('num', value) | ('sym', name) | ('neg', x) | (op, l, r) | ('fn', f, x),
-    with the usual precedence: unary minus, ^ (right-associative), * /, + - (left)."""
+    """Expression tree: ('num', value) | ('sym', name) | ('neg', x) | (op, l, r) | ('fn', f, x)
+    | ('cmp', c_op, l, r) | ('and', l, r) | ('or', l, r) | ('not', x), with the usual
+    precedence: unary minus, ^ (right-associative), * /, + - (left), relations (.gt. .lt.
+    .geq. .leq. .eq. .neq.), .not., .and., .or. (left)."""
     toks = _tokens(text)
     pos = [0]
 
@@ -289,7 +384,7 @@ def parse(text):
         if t[0] == "name":
             if peek() == ("op", "("):
                 take()
-                arg = additive()
+                arg = logical_or()
                 if take() != ("op", ")"):
                     raise LemsError(f"missing ')' in '{text}'")
                 if t[1] not in _FUNCS:
@@ -297,7 +392,7 @@ def parse(text):
                 return ("fn", t[1], arg)
             return ("sym", t[1])
         if t == ("op", "("):
-            e = additive()
+            e = logical_or()
             if take() != ("op", ")"):
                 raise LemsError(f"missing ')' in '{text}'")
             return e
@@ -333,7 +428,34 @@ def parse(text):
             e = (op, e, term())
         return e
 
-    tree = additive()
+    def relation():
+        e = additive()
+        if peek()[0] == "op" and peek()[1] in _REL:
+            op = _REL[take()[1]]
+            e = ("cmp", op, e, additive())
+        return e
+
+    def negation():
+        if peek() == ("op", ".not."):
+            take()
+            return ("not", negation())
+        return relation()
+
+    def logical_and():
+        e = negation()
+        while peek() == ("op", ".and."):
+            take()
+            e = ("and", e, negation())
+        return e
+
+    def logical_or():
+        e = logical_and()
+        while peek() == ("op", ".or."):
+            take()
+            e = ("or", e, logical_and())
+        return e
+
+    tree = logical_or()
     if pos[0] != len(toks):
         raise LemsError(f"trailing tokens in '{text}'")
     return tree
@@ -359,6 +481,13 @@ def infix(tree, names):
         return _FUNCS[tree[1]] + "( " + infix(tree[2], names) + " )"
     if k == "pow":
         return "powf( " + infix(tree[1], names) + " , " + infix(tree[2], names) + " )"
+    if k == "cmp":
+        return "( ( " + infix(tree[2], names) + " " + tree[1] + " " + infix(tree[3], names) + " ) )"
+    if k == "not":
+        return "( !" + infix(tree[1], names) + " )"
+    if k in ("and", "or"):
+        op = "&&" if k == "and" else "||"
+        return "( ( " + infix(tree[1], names) + " " + op + " " + infix(tree[2], names) + " ) )"
     return "( ( " + infix(tree[1], names) + " " + k + " " + infix(tree[2], names) + " ) )"
 
 
@@ -383,6 +512,14 @@ def evaluate(tree, values):
         return F(-evaluate(tree[1], values))
     if k == "fn":
         return F(_NP_FUNCS[tree[1]](evaluate(tree[2], values)))
+    if k == "cmp":
+        a, b = evaluate(tree[2], values), evaluate(tree[3], values)
+        return F({">": a > b, "<": a < b, ">=": a >= b, "<=": a <= b, "==": a == b, "!=": a != b}[tree[1]])
+    if k == "not":
+        return F(not evaluate(tree[1], values))
+    if k in ("and", "or"):
+        a, b = bool(evaluate(tree[1], values)), bool(evaluate(tree[2], values))
+        return F((a and b) if k == "and" else (a or b))
     a, b = evaluate(tree[1], values), evaluate(tree[2], values)
     with np.errstate(all="ignore"):
         if k == "pow":
@@ -392,6 +529,34 @@ def evaluate(tree, values):
 
 # ---------------------------------------------------------------- emitter
 
+def _emit_derived(L, st, names, indent="    "):
+    """Derived variables in order: plain ones as constants, conditional ones as an if / else
+    if chain over their cases with the default case last, 0 when none applies
+    (eden/Eden.cpp:2739-2776)."""
+    for d, expr in st.derived:
+        if isinstance(expr, str):
+            L.append(f"{indent}const float d_{d} = {infix(parse(expr), names)};")
+        else:
+            L.append(f"{indent}float d_{d} = 0.0f;")
+            L.append(f"{indent}if (0) {{ }}")
+            for cond, val in expr:
+                if cond is not None:
+                    L.append(f"{indent}else if ({infix(parse(cond), names)}) {{ d_{d} = {infix(parse(val), names)}; }}")
+            default = [val for cond, val in expr if cond is None]
+            if default:
+                L.append(f"{indent}else {{ d_{d} = {infix(parse(default[0]), names)}; }}")
+        names[d] = f"d_{d}"
+
+
+def _emit_assign(L, assigns, names, indent):
+    """State assignments in order, each setting the present and the next value (later
+    expressions of the step read the assigned value)."""
+    for s_, expr in assigns:
+        if f"n_{s_}" not in {f"n_{x}" for x in names.get("__state__", ())}:
+            raise LemsError(f"assignment to {s_}, which is not a state variable")
+        L.append(f"{indent}s_{s_} = {infix(parse(expr), names)}; n_{s_} = s_{s_};")
+
+
 def emit(name, st, fn_name, dialect):
     """C / HIP text of one type's step function:
 
@@ -399,7 +564,7 @@ def emit(name, st, fn_name, dialect):
                 int init, int event, float dt, float *i_out, float *g_out)
 
     P: the type's parameter vector (params in table order, then the `weight` property); vpeer:
-    the voltage at the other end of a gap junction (v for other types);
+    the voltage at the other end of a gap junction or graded connection (v for other types);
     S / Sn: the instance's present / next state, variable k at k * stride (they may alias:
     every present value is read before anything is written)."""
     q = "__device__ __forceinline__ " if dialect == "hip" else "static "
@@ -419,29 +584,30 @@ def emit(name, st, fn_name, dialect):
     for d, expr in st.derived_params:
         L.append(f"    const float dp_{d} = {infix(parse(expr), names)};")
         names[d] = f"dp_{d}"
-    for k, s in enumerate(st.state):
-        L.append(f"    const float s_{s} = S[{k} * stride];")
-        names[s] = f"s_{s}"
-    for d, expr in st.derived:
-        L.append(f"    const float d_{d} = {infix(parse(expr), names)};")
-        names[d] = f"d_{d}"
-    for s in st.state:
-        L.append(f"    float n_{s} = s_{s};")
+    for k, s_ in enumerate(st.state):
+        L.append(f"    float s_{s_} = S[{k} * stride];")
+        names[s_] = f"s_{s_}"
+    names["__state__"] = tuple(st.state)
+    _emit_derived(L, st, names)
+    for s_ in st.state:
+        L.append(f"    float n_{s_} = s_{s_};")
     L.append("    if (init) {")
-    for s, expr in st.on_start.items():
-        L.append(f"        n_{s} = {infix(parse(expr), names)};")
+    _emit_assign(L, st.on_start.items(), names, "        ")
     L.append("    } else {")
-    for k, s in enumerate(st.state):
-        if s in st.derivatives:
-            L.append(f"        const float der_{k} = {infix(parse(st.derivatives[s]), names)};")
-            L.append(f"        n_{s} = s_{s} + dt * der_{k};")
+    for k, s_ in enumerate(st.state):
+        if s_ in st.derivatives:
+            L.append(f"        const float der_{k} = {infix(parse(st.derivatives[s_]), names)};")
+            L.append(f"        n_{s_} = s_{s_} + dt * der_{k};")
+    for test, assigns in st.on_conditions:
+        L.append(f"        if ({infix(parse(test), names)}) {{")
+        _emit_assign(L, assigns, names, "            ")
+        L.append("        }")
     L.append("        if (event) {")
-    for s, expr in st.on_event.items():
-        L.append(f"            n_{s} = {infix(parse(expr), names)};")
+    _emit_assign(L, st.on_event.items(), names, "            ")
     L.append("        }")
     L.append("    }")
-    for k, s in enumerate(st.state):
-        L.append(f"    Sn[{k} * stride] = n_{s};")
+    for k, s_ in enumerate(st.state):
+        L.append(f"    Sn[{k} * stride] = n_{s_};")
     L.append(f"    *i_out = {names[st.expose_i]};")
     L.append(f"    *g_out = {names[st.expose_g] if st.expose_g else '0.0f'};")
     L.append("}")

@@ -24,17 +24,22 @@ set covers:
                     compartment holding their segment's fractionAlong (0.5 when absent)
                     (eden/Eden.cpp:1387-1772, 1015-1038, 4511-4560)
   synapses          expTwoSynapse natively; expOneSynapse, alphaSynapse, alphaCurrentSynapse,
-                    expThreeSynapse, gapJunction and the PyNN synapses (expCond, expCurr,
-                    alphaCond, alphaCurr) as generic LEMS types (eden_amd.lems), compiled by
-                    the library at run time (eden/NeuroML.cpp:6506-6829); user-defined
+                    expThreeSynapse, blockingPlasticSynapse (voltageConcDepBlockMechanism,
+                    tsodyksMarkramDepMechanism, tsodyksMarkramDepFacMechanism), gapJunction,
+                    silentSynapse, linearGradedSynapse, gradedSynapse and the PyNN synapses
+                    (expCond, expCurr, alphaCond, alphaCurr) as generic LEMS types
+                    (eden_amd.lems), compiled by the library at run time
+                    (eden/NeuroML.cpp:6506-6829); user-defined
                     <ComponentType>s that extend a base synapse type, from their LEMS
                     Dynamics (eden/Eden.cpp:2339-2995)
   inputs            pulseGenerator, poissonFiringSynapse (eden/NeuroML.cpp:6968-7107)
   networks          population (size or instances), projection with connection /
                     connectionWD, synapticConnection, electricalProjection (gapJunction:
                     an instance on each end, each with the other end's voltage as its peer),
-                    explicitInput, inputList with input / inputW
-                    (eden/NeuroML.cpp:6136-6420, eden/Eden.cpp:9225-9248)
+                    continuousProjection (continuousConnection[Instance[W]]: the post
+                    component on the post end, the pre component on the pre end, peers as
+                    for gap junctions), explicitInput, inputList with input / inputW
+                    (eden/NeuroML.cpp:6136-6420, eden/Eden.cpp:9225-9270)
   simulation        <Simulation length step seed target> with OutputFile / OutputColumn and
                     EventOutputFile / EventSelection, <Target> (eden/NeuroML.cpp:7979-8311)
 
@@ -49,8 +54,8 @@ read (IdListRle is a sorted set of component sequence numbers, eden/Eden.cpp:186
 ParseQuantity does: the number read as a double, times 10^(unit power - native power), then
 stored as float (eden/NeuroML.cpp:3046-3077, eden/NeuroML.h:530-532).
 
-Constructs outside this set (other channel and synapse types, artificial cells, continuous /
-electrical projections) raise NeuroMLError naming the element.
+Constructs outside this set (other channel and synapse types, artificial cells, other
+inputs) raise NeuroMLError naming the element.
 """
 import os
 import re
@@ -86,6 +91,7 @@ _DIMS = {
     "capacitance": (-12, {"F": 0, "mF": -3, "uF": -6, "nF": -9, "pF": -12}),
     "specific_capacitance": (-2, {"uF_per_cm2": -2, "F_per_m2": 0, "uF_per_mm2": 0}),
     "temperature": (0, {"K": 0, "degK": 0, "degC": "degC"}),
+    "concentration": (-3, {"mol_per_m3": 0, "mol_per_cm3": 6, "M": 3, "mM": 0, "uM": -3, "nM": -6}),
 }
 
 _NUM = re.compile(r"^\s*([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)\s*(\S*)\s*$")
@@ -130,10 +136,11 @@ _CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Sy
 
 # element names of the synapse and input types this reader instantiates, by role; a LEMS
 # <Component type="..."> of these types is read as the element itself
-_SYNAPSES = set(lems_mod.SYNAPSE_TYPES)
+_SYNAPSES = set(lems_mod.SYNAPSE_TYPES) | {"blockingPlasticSynapse"}
 _INPUTS = {"pulseGenerator", "poissonFiringSynapse"}
-_UNSUPPORTED_SYNAPSES = {"blockingPlasticSynapse", "silentSynapse", "linearGradedSynapse",
-                         "gradedSynapse", "doubleSynapse"}
+# synapse types that may sit on either end of a continuous connection (no spike input)
+_GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction"}
+_UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
 _UNSUPPORTED_CELLS = {"iafCell", "iafTauCell", "iafRefCell", "iafTauRefCell", "izhikevichCell",
                       "izhikevich2007Cell", "adExIaFCell", "fitzHughNagumoCell", "pinskyRinzelCA3Cell",
                       "IF_curr_alpha", "IF_curr_exp", "IF_cond_alpha", "IF_cond_exp", "EIF_cond_exp_isfa_ista",
@@ -906,6 +913,8 @@ def read(path, rng_seed=None):
             used.add(el.get("synapse"))
         elif _tag(el) == "electricalProjection":
             used |= {c.get("synapse") for c in el if c.get("synapse")}
+        elif _tag(el) == "continuousProjection":
+            used |= {c.get(k) for c in el for k in ("preComponent", "postComponent") if c.get(k)}
     generic = any(doc.items.get(u, (None, None))[0] == "synapse" and _tag(doc.items[u][1]) != "expTwoSynapse"
                   for u in used)
     generic_index, generic_types, custom = {}, [], {}
@@ -913,7 +922,29 @@ def read(path, rng_seed=None):
         for i, el in syn_elems:
             t = _tag(el)
             weight = 1.0
-            if t in lems_mod.SYNAPSE_TYPES:
+            extra = {}
+            if t == "blockingPlasticSynapse":
+                # composed with its block and plasticity mechanisms (their parameters are the
+                # child elements' attributes), one type per combination
+                mech = {}
+                for ch in el:
+                    ct = _tag(ch)
+                    if ct not in ("blockMechanism", "plasticityMechanism"):
+                        continue
+                    table = lems_mod.BLOCK_MECHANISMS if ct == "blockMechanism" else lems_mod.PLASTICITY_MECHANISMS
+                    if ch.get("type") not in table:
+                        raise NeuroMLError(f"synapse {i}: {ct} type '{ch.get('type')}' is not supported")
+                    mech[ct] = ch.get("type")
+                    for p_, d_ in table[ch.get("type")]["params"]:
+                        if ch.get(p_) is None:
+                            raise NeuroMLError(f"synapse {i}: {ct} parameter {p_} missing")
+                        extra[p_] = convert(ch.get(p_), d_, f"{i} {p_}")
+                t = f"blockingPlasticSynapse/{mech.get('blockMechanism', '')}/{mech.get('plasticityMechanism', '')}"
+                if t not in custom:
+                    custom[t] = (lems_mod.blocking_plastic_type(mech.get("blockMechanism"),
+                                                                mech.get("plasticityMechanism")), 1.0)
+                st, weight = custom[t]
+            elif t in lems_mod.SYNAPSE_TYPES:
                 st = lems_mod.SYNAPSE_TYPES[t]
             else:
                 if t not in custom:
@@ -923,6 +954,7 @@ def read(path, rng_seed=None):
                         raise NeuroMLError(str(e)) from None
                 st, weight = custom[t]
             vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
+            vals.update(extra)
             generic_index[i] = len(generic_types)
             generic_types.append((t, vals, weight))
 
@@ -999,8 +1031,31 @@ def read(path, rng_seed=None):
                 conns.append((-1, b, st, w, 0.0, a))
                 conns.append((-1, a, st, w, 0.0, b))
         elif t == "continuousProjection":
-            if any(True for _ in el):
-                raise NeuroMLError(f"<{t}> {el.get('id')}: continuous projections are not supported")
+            # graded connections: the postsynaptic component on the post end, then the
+            # presynaptic one on the pre end, each with the other end as its peer
+            # (eden/Eden.cpp:9254-9270)
+            pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
+            for c in el:
+                ct = _tag(c)
+                if ct not in ("continuousConnection", "continuousConnectionInstance", "continuousConnectionInstanceW"):
+                    continue
+                what = f"continuousProjection {el.get('id')}"
+                s_pre = synapse(c.get("preComponent"), what)
+                s_post = synapse(c.get("postComponent"), what)
+                for sid in (c.get("preComponent"), c.get("postComponent")):
+                    if _tag(doc.items[sid][1]) not in _GRADED:
+                        raise NeuroMLError(f"{what}: component {sid} ({_tag(doc.items[sid][1])}) is not a graded "
+                                           f"synapse ({', '.join(sorted(_GRADED))})")
+                if ct == "continuousConnection":
+                    pi, qi = int(c.get("preCell")), int(c.get("postCell"))
+                else:
+                    _, pi = _cell_ref(c.get("preCell"))
+                    _, qi = _cell_ref(c.get("postCell"))
+                a = comp(pre, pi, int(c.get("preSegment", "0")), what, frac(c, "preFractionAlong"))
+                b = comp(post, qi, int(c.get("postSegment", "0")), what, frac(c, "postFractionAlong"))
+                w = convert(c.get("weight"), None, "weight") if ct == "continuousConnectionInstanceW" else 1.0
+                conns.append((-1, b, s_post, w, 0.0, a))
+                conns.append((-1, a, s_pre, w, 0.0, b))
 
     # per-compartment tables
     v0 = np.concatenate([pop_of_cell[c].cell.v0 for c in range(n_cells)])

@@ -2,8 +2,9 @@
 hiprtc into its synapse kernel at context creation. Against the oracle running the C dialect
 of the same text: traces, spike raster and every synaptic delivery bit for bit.
 
-* the reference's CoreSynapses model (its chemical part, tests/coresyn.py): one synapse of
-  each core type, through the NeuroML reader and the output files;
+* the reference's CoreSynapses model (tests/coresyn.py): one synapse of each core type
+  (the blocking/plastic one included), graded and electrical connections, through the
+  NeuroML reader and the output files;
 * expTwoSynapse as a generic type equals the library's native expTwo path;
 * a network of tree and point cells with generic synapses on dendrites and somata;
 * gap junctions (a pair of membranes; the reference's DomainDecomposition model, whose
@@ -50,11 +51,11 @@ def _check(m):
 
 
 def test_reference_core_synapses(tmp_path):
-    from coresyn import write_chemical_subset
-    path = write_chemical_subset(str(tmp_path))
+    from coresyn import write_model
+    path = write_model(str(tmp_path))
     m, outs = neuroml.read(path)
     tr, er, ost = _check(m)
-    assert sorted(set(oracle.run_with_synapse_log(m)[1][1].tolist())) == list(range(10))
+    assert sorted(set(oracle.run_with_synapse_log(m)[1][1].tolist())) == list(range(11))
     res = run_lems(path, out_dir=str(tmp_path))
     t_us = output.row_times_us(m.dt, m.total_rows() + 1)
     f = outs.trajectories[0]

@@ -11,8 +11,10 @@
   a missing factor e or a swapped time constant fails one of them.
 * expTwoSynapse emitted from its LEMS definition equals the library's native expTwo in the
   oracle, bit for bit, over a network run.
-* The reference's CoreSynapses model (its chemical part) reads into one generic type per
-  core synapse, in the reference's instance order, and runs in the oracle.
+* The reference's CoreSynapses model reads into one generic type per core synapse (the
+  blocking/plastic one composed with its mechanisms), in the reference's instance order,
+  and runs in the oracle.
+* Graded synapses and the blocking / plastic synapse's mechanisms against closed forms.
 * Gap junctions: a coupled pair of membranes follows the discrete closed form.
 * User-defined ComponentTypes with the dynamics of core types run as the core types do.
 """
@@ -131,21 +133,24 @@ def test_generic_exptwo_equals_native_exptwo():
     assert np.array_equal(a, b) and np.array_equal(ra, rb) and np.array_equal(sa, sb) and np.array_equal(ya, yb)
 
 
-def test_reference_core_synapses_chemical_part(tmp_path):
-    from coresyn import write_chemical_subset
-    m, outs = neuroml.read(write_chemical_subset(str(tmp_path)))
-    # every synapse element in reading order (the gap junction's is read although its
-    # projection was removed)
+def test_reference_core_synapses_model(tmp_path):
+    from coresyn import write_model
+    m, outs = neuroml.read(write_model(str(tmp_path)))
+    # every synapse element in reading order, the blocking/plastic one composed with its
+    # mechanisms
     assert m.lems.type_names == ["alphaCurrentSynapse", "alphaSynapse", "expOneSynapse", "expTwoSynapse",
-                                 "expThreeSynapse", "gapJunction", "expCondSynapse", "alphaCondSynapse",
-                                 "expCurrSynapse", "alphaCurrSynapse"]
-    # ChemPost cells 1..10 and 21 receive one synapse each, of the projections' types in order
-    post = [int(np.flatnonzero(np.diff(m.comp_syn_begin))[k]) for k in range(m.n_synapses)]
-    assert post == [2, 3, 4, 5, 6, 8, 9, 10, 11, 22]
-    assert list(m.syn_type) == [0, 1, 2, 3, 4, 6, 7, 8, 9, 0]
-    assert m.syn_weight[-1] == np.float32(1.7) and m.syn_delay[-1] == 5000.0
+                                 "expThreeSynapse",
+                                 "blockingPlasticSynapse/voltageConcDepBlockMechanism/tsodyksMarkramDepFacMechanism",
+                                 "silentSynapse", "gapJunction", "linearGradedSynapse", "gradedSynapse",
+                                 "expCondSynapse", "alphaCondSynapse", "expCurrSynapse", "alphaCurrSynapse"]
+    # ChemPost cells 1..10 and 21 receive one synapse each, of the projections' types in
+    # order; then the graded and electrical ends on ContPre 1-4 and ContPost 1-4
+    comp_of = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
+    assert list(comp_of) == [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 22, 32, 33, 34, 35, 52, 53, 54, 55]
+    assert list(m.syn_type) == [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 0, 8, 7, 6, 6, 6, 7, 8, 9]
+    assert m.syn_weight[10] == np.float32(1.7) and m.syn_delay[10] == 5000.0
     (tr, ec, er, _), (st, sy) = oracle.run_with_synapse_log(m)
-    assert sorted(set(sy.tolist())) == list(range(10))       # every synapse received events
+    assert sorted(set(sy.tolist())) == list(range(11))       # every chemical synapse received events
     # every postsynaptic cell left rest
     cols = [c[2] for c in outs.trajectories[0].columns][1:12]
     assert np.all(np.abs(tr[:, cols] - (-65.0)).max(axis=0) > 0.05)
@@ -205,8 +210,9 @@ _USER_TYPES = """
     <Property name="weight" dimension="none" defaultValue="1"/>
     <Parameter name="tauDecay" dimension="time"/>
     <Dynamics>
-        <StateVariable name="g" dimension="conductance" exposure="g"/>
+        <StateVariable name="g" dimension="conductance"/>
         <DerivedVariable name="i" exposure="i" dimension="current" value="g * (erev - v)"/>
+        <DerivedVariable name="g0" exposure="g" dimension="conductance" value="g"/>
         <TimeDerivative variable="g" value="-g / tauDecay"/>
         <OnStart><StateAssignment variable="g" value="0"/></OnStart>
         <OnEvent port="in"><StateAssignment variable="g" value="g + (weight * gbase)"/></OnEvent>
@@ -267,3 +273,136 @@ def test_user_defined_component_types_equal_the_core_types(tmp_path):
     assert np.array_equal(tr[:, 1], tr[:, 2])                    # expOne vs the user type
     assert np.array_equal(tr[:, 3:5], tr[:, 5:7])                # gap junction pair vs the user type
     assert float(np.abs(tr[:, 1] + 65.0).max()) > 0.1 and float(np.abs(tr[:, 3] + 65.0).max()) > 0.1
+
+
+def test_state_exposure_reads_the_value_an_event_assigned():
+    """A state variable exposed directly is read after the step's assignments (the
+    reference's exposures follow the update and read the assigned present value), a derived
+    one before them: on an event step the first shows the jump, the second does not."""
+    src = ('<Lems><ComponentType name="s1" extends="baseConductanceBasedSynapse">'
+           '<Property name="weight" dimension="none" defaultValue="1"/><Parameter name="tauDecay" dimension="time"/>'
+           '<Dynamics><StateVariable name="g" dimension="conductance" exposure="g"/>'
+           '<DerivedVariable name="i" exposure="i" dimension="current" value="g * (erev - v)"/>'
+           '<TimeDerivative variable="g" value="-g / tauDecay"/>'
+           '<OnEvent port="in"><StateAssignment variable="g" value="g + (weight * gbase)"/></OnEvent>'
+           '</Dynamics></ComponentType></Lems>')
+    import xml.etree.ElementTree as ET
+    cts = {el.get("name"): el for el in ET.fromstring(src)}
+    st, w = lems.synapse_type_from_lems("s1", cts, neuroml.convert)
+    vals = dict(gbase=2e-3, erev=0.0, tauDecay=3000.0)
+    for tname, custom, jump_seen in (("s1", {"s1": st}, True), ("expOneSynapse", None, False)):
+        step = _stepper_peer(tname, vals, custom)
+        step(-65.0, -65.0, 1, 0)
+        i, g, S = step(-65.0, -65.0, 0, 1)
+        assert S[0] == np.float32(2e-3) and i == 0.0
+        assert g == (np.float32(2e-3) if jump_seen else 0.0), tname
+
+
+def _stepper_peer(tname, vals, custom=None, weight=1.0):
+    """Step function of one instance taking (v, vpeer, init, event, dt); returns (i, g, state)."""
+    ls = lems.build([(tname, vals, weight)], custom)
+    lib = oracle.lems_library(ls.source_c)
+    fn = lib.lems_syn
+    fn.restype = None
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_float, C.c_int, C.c_int,
+                   C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    P = ls.params[0]
+    S = np.zeros(max(1, ls.n_state[0]), np.float32)
+
+    def step(v, vpeer, init, event, dt=DT):
+        i, g = C.c_float(), C.c_float()
+        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, vpeer, init, event, dt, C.byref(i), C.byref(g))
+        return i.value, g.value, S.copy()
+    return step
+
+
+def test_graded_synapses_follow_their_closed_forms():
+    """silentSynapse carries no current; linearGradedSynapse is conductance (vpeer - v) in
+    float arithmetic; gradedSynapse's s relaxes by Euler steps towards
+    inf = 1 / (1 + exp((Vth - vpeer) / delta)) with tau = (1 - inf) / k, so from s = 0,
+    s_n = inf (1 - (1 - dt / tau)^n) and i = conductance s (erev - v); where 1 - inf < 1e-4 the
+    OnCondition sets s = inf in one step (the reference's guard against tau ~ 0). Swapping
+    v and vpeer, or the sign of the exponent, fails these."""
+    step = _stepper_peer("silentSynapse", {})
+    assert step(-40.0, 10.0, 0, 0)[:2] == (0.0, 0.0)
+    step = _stepper_peer("linearGradedSynapse", dict(conductance=1e-3))
+    i, g, _ = step(-65.0, -20.0, 0, 0)
+    assert i == np.float32(1e-3) * np.float32(-20.0 - -65.0) and g == 0.0
+    vals = dict(conductance=1e-3, delta=5.0, k=2.5e-5, Vth=-55.0, erev=0.0)
+    step = _stepper_peer("gradedSynapse", vals)
+    v, vpeer = -65.0, -50.0
+    inf = 1.0 / (1.0 + math.exp((vals["Vth"] - vpeer) / vals["delta"]))
+    tau = (1.0 - inf) / vals["k"]
+    step(v, vpeer, 1, 0)
+    for n in range(1, 2001):
+        i, _, S = step(v, vpeer, 0, 0)
+        want_s = inf * (1.0 - (1.0 - DT / tau) ** n)
+        assert abs(S[0] - want_s) <= 2e-5 * inf, (n, S[0], want_s)
+    # the current of a step uses the state at its start
+    s_prev = S[0]
+    i, _, S = step(v, vpeer, 0, 0)
+    assert i == pytest.approx(vals["conductance"] * s_prev * (vals["erev"] - v), rel=1e-6)
+    # saturated activation: 1 - inf < 1e-4 -> s = inf after one step, no Euler blow-up
+    step = _stepper_peer("gradedSynapse", vals)
+    step(v, 0.0, 1, 0)
+    _, _, S = step(v, 0.0, 0, 0)
+    assert S[0] == np.float32(1.0) / (np.float32(1.0) + np.float32(math.exp((vals["Vth"] - 0.0) / vals["delta"])))
+
+
+def _bps(block, plast, extra):
+    st = lems.blocking_plastic_type(block, plast)
+    key = f"bps/{block}/{plast}"
+    vals = dict(gbase=2e-2, erev=0.0, tauRise=500.0, tauDecay=1000.0, **extra)
+    return _stepper_peer(key, vals, {key: st}), vals
+
+
+def _event_charges(step, v, events, gap):
+    """Charge sum(g dt) after each of `events` events `gap` steps apart."""
+    step(v, v, 1, 0)
+    out = []
+    for e in range(events):
+        q = 0.0
+        for k in range(gap):
+            _, g, _ = step(v, v, 0, 1 if k == 0 else 0)
+            q += g * DT
+        out.append(q)
+    return out
+
+
+def test_blocking_factor_scales_the_exptwo_charge():
+    """With the Mg block alone, one event's charge is the expTwo charge gbase wf (tauDecay -
+    tauRise) times 1 / (1 + (c / c_s) exp(-v / v_s)) at the (clamped) membrane voltage."""
+    extra = dict(blockConcentration=1.0, scalingConc=2000.0, scalingVolt=150.0)
+    for v in (-65.0, -20.0, 10.0):
+        step, vals = _bps("voltageConcDepBlockMechanism", None, extra)
+        q = _event_charges(step, v, 1, 20000)[0]
+        tr, td = vals["tauRise"], vals["tauDecay"]
+        tp = math.log(td / tr) * tr * td / (td - tr)
+        wf = 1.0 / (-math.exp(-tp / tr) + math.exp(-tp / td))
+        block = 1.0 / (1.0 + (1.0 / 2000.0) * math.exp(-v / 150.0))
+        assert q == pytest.approx(vals["gbase"] * wf * (td - tr) * block, rel=2e-3), v
+
+
+def test_tsodyks_markram_release_follows_the_event_maps():
+    """Each event releases in proportion to R U of the present state. Without recovery
+    (time constants far beyond the run), depression-facilitation has U_k = 1 - (1 - p)^(k+1)
+    and R_k = prod_{j<k} (1 - U_j) at event k, so event k's charge over event 0's is
+    R_k U_k / p.
+    Depression alone with recovery between periodic events reaches the steady state
+    R* = (1 - a^m) / (1 - (1 - p) a^m), a = 1 - dt / tauRec, m Euler steps between events
+    (the event step's own update is the reset), which the late events' charges show."""
+    p = 0.25
+    step, _ = _bps(None, "tsodyksMarkramDepFacMechanism", dict(initReleaseProb=p, tauRec=1e30, tauFac=1e30))
+    q = _event_charges(step, -65.0, 5, 4000)
+    U = [1.0 - (1.0 - p) ** (k + 1) for k in range(5)]             # present U at event k
+    R = [float(np.prod([1.0 - U[j] for j in range(k)])) for k in range(5)]
+    for k in range(5):
+        assert q[k] / q[0] == pytest.approx(R[k] * U[k] / p, rel=1e-5), k
+    step, _ = _bps(None, "tsodyksMarkramDepMechanism", dict(initReleaseProb=p, tauRec=20000.0))
+    gap = 4000
+    q = _event_charges(step, -65.0, 12, gap)
+    a = 1.0 - DT / 20000.0
+    m = gap - 1
+    r_ss = (1.0 - a ** m) / (1.0 - (1.0 - p) * a ** m)
+    assert q[-1] / q[0] == pytest.approx(r_ss, rel=1e-4)
+    assert q[1] / q[0] == pytest.approx(1.0 - p * a ** m, rel=1e-4)     # 1 - R recovers as a^m

@@ -10,8 +10,9 @@ command-line front end, on the CPU.
 * Every synthetic workload written as LEMS by the exporter reads back into the same
   instantiated model, array for array (spike recipients per source as a set: their order
   does not change results, flags being idempotent).
-* The reference's full EdenTest_CoreSynapses model, which holds constructs outside the
-  per-step loop's mechanism set, is refused with an error naming the first of them.
+* The reference's full EdenTest_CoreSynapses model (blocking/plastic synapse, continuous and
+  electrical projections) reads in the reference's instance order, each continuous
+  connection's postsynaptic component first, each end with the other as its peer.
 * Cable discretisation: the reference's EdenTest_CableDiscretisation model gives identical
   compartments for a segment cut in five and five segments; tapering cables of uneven
   segments cut into 1, 3 and 7 compartments match the cone-section closed forms; locations
@@ -101,11 +102,33 @@ def test_exported_workloads_read_back_to_the_same_model(tmp_path, which):
         assert sorted(m.spike_target[s0:s1]) == sorted(r.spike_target[s0:s1])
 
 
-def test_reference_core_synapses_model_is_refused_by_name():
-    """The full model holds a blocking/plastic synapse, graded and gap-junction projections,
-    which the per-step loop does not cover: refused, naming the first of them."""
-    with pytest.raises(neuroml.NeuroMLError, match="blockingPlasticSynapse"):
-        neuroml.read(os.path.join(GOLD, "LEMS_EdenTest_CoreSynapses.xml"))
+def test_reference_core_synapses_full_model_reads_in_instance_order():
+    """The full model: the blocking/plastic synapse (composed with its Mg block and
+    depression-facilitation mechanisms, their parameters from the child elements), the
+    continuous projections (linearGraded / silent / graded ends) and the gap junction. Each
+    continuous or electrical connection puts its postsynaptic component on the post cell
+    first, then its presynaptic one on the pre cell, each with the other end as its peer
+    (eden/Eden.cpp:9233-9270)."""
+    m, _ = neuroml.read(os.path.join(GOLD, "LEMS_EdenTest_CoreSynapses.xml"))
+    names = m.lems.type_names
+    assert names[5] == "blockingPlasticSynapse/voltageConcDepBlockMechanism/tsodyksMarkramDepFacMechanism"
+    assert names[6:10] == ["silentSynapse", "gapJunction", "linearGradedSynapse", "gradedSynapse"]
+    p = m.lems.params[5]
+    # gbase, erev, tauRise, tauDecay, blockConcentration, scalingConc, scalingVolt (uM, mV),
+    # initReleaseProb, tauRec, tauFac (us), weight
+    np.testing.assert_allclose(p, [0.01, 0.0, 500.0, 1000.0, 1.0, 2000.0, 150.0, 0.25, 15000.0, 5000.0, 1.0],
+                               rtol=1e-6)
+    comp_of = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
+    graded = [(int(comp_of[k]), names[m.syn_type[k]], int(m.syn_peer[k])) for k in range(m.n_synapses)
+              if m.syn_peer[k] >= 0]
+    # ContPre / ContPost cells: the projections connect cell k of each (k = 1, 2, 3, 4)
+    pre = {k: int(m.cell_comp_begin[31 + k]) for k in range(1, 5)}
+    post = {k: int(m.cell_comp_begin[51 + k]) for k in range(1, 5)}
+    want = [(post[1], "silentSynapse", pre[1]), (post[2], "gapJunction", pre[2]),
+            (post[3], "linearGradedSynapse", pre[3]), (post[4], "gradedSynapse", pre[4]),
+            (pre[1], "linearGradedSynapse", post[1]), (pre[2], "gapJunction", post[2]),
+            (pre[3], "silentSynapse", post[3]), (pre[4], "silentSynapse", post[4])]
+    assert sorted(graded) == sorted(want)
 
 
 def test_reader_errors_name_the_problem(tmp_path):
