@@ -11,7 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ROOT, "libedenhip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_GATES = 4
 
 RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED = 0, 1, 2, 3
@@ -44,6 +44,11 @@ class LemsSynapsesDesc(C.Structure):
                 ("n_state", _P), ("peer", _P)]
 
 
+class LemsCellsDesc(C.Structure):
+    _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
+                ("n_state", _P), ("v_state", _P), ("cell_type", _P)]
+
+
 class ModelDesc(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("dt", C.c_float), ("t_final", C.c_double),
@@ -62,6 +67,7 @@ class ModelDesc(C.Structure):
         ("n_record", C.c_int64), ("record_comp", _P), ("n_event_cols", C.c_int64), ("event_comp", _P),
         ("lems", C.POINTER(LemsSynapsesDesc)),
         ("record_gate", _P),
+        ("lems_cells", C.POINTER(LemsCellsDesc)),
     ]
 
 

@@ -110,6 +110,18 @@ struct LemsArgs {
     const int32_t *peer;          // per synapse slot: the gap junction's peer compartment, or -1
 };
 
+// Arguments of lems_cell_kernel (LEMS point neurons, compiled at run time with the synapse
+// kernel): one slot per point-neuron cell.
+struct LemsCellArgs {
+    const int32_t *comps;         // [n] the cell's compartment
+    const int32_t *type;          // [n] its type
+    int32_t n;
+    const float *params;          // all types' parameter vectors
+    const int32_t *param_begin;   // [n_types + 1]
+    const int32_t *v_state;       // [n_types] membrane-voltage state variable, or -1
+    float *state;                 // [max state][n]: variable k of slot i at k * n + i
+};
+
 // Correctly rounded fp32 division whose expansion does not go through VCC.
 //
 // hipcc expands n / d into: v_div_scale of d, v_rcp, one Newton step on the reciprocal,
@@ -214,6 +226,25 @@ __device__ __forceinline__ float log_f(float x) {
 #else
     return (float)log((double)x);
 #endif
+}
+
+// x^y for generated LEMS code, rounded once from the double power (the oracle's C dialect
+// does the same with the host's pow).
+__device__ __forceinline__ float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+
+// A spike detected while computing step `step` is logged at row `step` (the next row).
+__device__ __forceinline__ void log_event(const DevTables &T, int32_t col, int64_t step) {
+    if (col >= 0 && step < T.total_rows) {
+        uint32_t slot = atomicAdd(T.n_events, 1u);
+        if (slot < T.max_events) T.events[slot] = ((int64_t)step << 24) | (int64_t)col;
+    }
+}
+
+// A spike of compartment gidx sets its recipients' trigger flags for the next step.
+__device__ __forceinline__ void send_spike(const DevTables &T, int64_t gidx, int nxt) {
+    const int32_t b = T.spk_begin[gidx], e = T.spk_begin[gidx + 1];
+    int32_t *trig = T.trig[nxt];
+    for (int32_t k = b; k < e; k++) trig[T.spk_target[k]] = 1;   // idempotent flag (EDEN: atomic OR 1)
 }
 
 // Rate forms without divergent branches: one exp and at most one extra division per rate,

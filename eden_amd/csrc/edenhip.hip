@@ -149,6 +149,11 @@ struct edenhip_ctx {
     int32_t *d_lems_pbegin = nullptr;
     int32_t *d_lems_peer = nullptr;      // per synapse slot: gap junction peer compartment, or -1
     float2 *d_syn_ext = nullptr;
+    // LEMS point neurons: the run-time compiled cell kernel and its tables (one slot per cell)
+    hipFunction_t lems_cell_fn = nullptr;
+    int32_t n_acells = 0;
+    int32_t *d_acell_comps = nullptr, *d_acell_type = nullptr, *d_acell_pbegin = nullptr, *d_acell_vstate = nullptr;
+    float *d_acell_params = nullptr, *d_acell_state = nullptr;
     // edenhip_run_static: the tree buckets' work-queue counters come from the device's static
     // scratch array for the duration of that call (nullptr: each bucket's own d_queue)
     int32_t *queue_base = nullptr;
@@ -241,14 +246,20 @@ __global__ void zero_i32_kernel(int32_t *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// Recorded gate variables into trace row `row`: column col[i] takes the point-cell gate state
-// T.q[src[i]] (src >= 0) or the tree-cell gate state gq[~src[i]] (src < 0).
-__global__ void record_gates_kernel(DevTables T, const float *__restrict__ gq, const int32_t *__restrict__ col,
-                                    const int64_t *__restrict__ src, int32_t n, int64_t row) {
+// Sources of recorded gate and state variables: the kind in the top bits, the index below.
+constexpr int64_t REC_TREE_GATE = (int64_t)1 << 60;   // the tree cells' gate table
+constexpr int64_t REC_CELL_STATE = (int64_t)2 << 60;  // the point neurons' state table
+constexpr int64_t REC_INDEX = ((int64_t)1 << 60) - 1; // (kind 0: the point cells' gates, T.q)
+
+// Recorded gate and state variables into trace row `row`: column col[i] takes the value src[i]
+// names.
+__global__ void record_gates_kernel(DevTables T, const float *__restrict__ gq, const float *__restrict__ cs,
+                                    const int32_t *__restrict__ col, const int64_t *__restrict__ src, int32_t n,
+                                    int64_t row) {
     const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    const int64_t k = src[i];
-    trace_at(T, col[i], row) = k >= 0 ? T.q[k] : gq[~k];
+    const int64_t k = src[i] & REC_INDEX, kind = src[i] >> 60;
+    trace_at(T, col[i], row) = kind == 0 ? T.q[k] : kind == 1 ? gq[k] : cs[k];
 }
 
 // Streams come from a per-device pool created once per process and kept until it ends:
@@ -293,6 +304,22 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                     return fail(ctx, EDENHIP_E_INVALID, "lems: peer compartment out of range");
         }
     }
+    const edenhip_lems_cells *LC = d->abi_version >= 4 ? d->lems_cells : nullptr;
+    if (LC && LC->n_types > 0) {
+        if (!LC->source || !LC->param_begin || !LC->params || !LC->n_state || !LC->v_state || !LC->cell_type)
+            return fail(ctx, EDENHIP_E_INVALID, "lems_cells: null table");
+        if (LC->n_types > 4096) return fail(ctx, EDENHIP_E_LIMIT, "lems_cells: more than 4096 cell types");
+        for (int t = 0; t < LC->n_types; t++) {
+            if (LC->n_state[t] < 0 || LC->n_state[t] > 16)
+                return fail(ctx, EDENHIP_E_LIMIT, "lems_cells: more than 16 state variables");
+            if (LC->v_state[t] < -1 || LC->v_state[t] >= LC->n_state[t])
+                return fail(ctx, EDENHIP_E_INVALID, "lems_cells: v_state out of range");
+            if (LC->param_begin[t + 1] < LC->param_begin[t])
+                return fail(ctx, EDENHIP_E_INVALID, "lems_cells: param_begin decreasing");
+        }
+    } else {
+        LC = nullptr;
+    }
     if (!(d->dt > 0) || !(d->t_final >= 0)) return fail(ctx, EDENHIP_E_INVALID, "dt must be > 0 and t_final >= 0");
     if (d->n_cells < 0 || d->n_comps < 0) return fail(ctx, EDENHIP_E_INVALID, "negative sizes");
     const int64_t NC = d->n_comps;
@@ -304,6 +331,25 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     ctx->NC = NC;
     ctx->n_cells = d->n_cells;
     ctx->dt = d->dt;
+
+    // LEMS point neurons: one compartment each, without channels or Poisson inputs; their
+    // synapses take the generic path
+    std::vector<int32_t> acell_slot(NC, -1), acell_comp, acell_type;
+    for (int c = 0; LC && c < d->n_cells; c++) {
+        const int32_t ty = LC->cell_type[c];
+        if (ty < 0) continue;
+        if (ty >= LC->n_types) return fail(ctx, EDENHIP_E_INVALID, "lems_cells: cell type out of range");
+        const int64_t g = d->cell_comp_begin[c];
+        if (d->cell_comp_begin[c + 1] != g + 1) return fail(ctx, EDENHIP_E_INVALID, "a point neuron has one compartment");
+        if (d->comp_chan_begin[g + 1] != d->comp_chan_begin[g] || d->comp_poisson_begin[g + 1] != d->comp_poisson_begin[g])
+            return fail(ctx, EDENHIP_E_INVALID, "a point neuron takes no channel distributions or Poisson inputs");
+        if (d->comp_syn_begin[g + 1] != d->comp_syn_begin[g] && !lems)
+            return fail(ctx, EDENHIP_E_INVALID, "synapses on a point neuron need the generic (lems) synapse path");
+        acell_slot[g] = (int32_t)acell_comp.size();
+        acell_comp.push_back((int32_t)g);
+        acell_type.push_back(ty);
+    }
+    ctx->n_acells = (int32_t)acell_comp.size();
 
     // channel type table
     std::vector<DGate> gates((size_t)std::max(d->n_channel_types, 1) * GATES_PER_TYPE);
@@ -418,24 +464,35 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     ctx->desc_to_dev_col.assign(d->n_record, -1);
     ctx->n_dev_cols = 0;
     // V columns first, then one column per distinct recorded gate variable
-    std::vector<std::pair<int64_t, int32_t>> grec;   // (compartment, gate) of gate columns
-    std::unordered_map<int64_t, int32_t> grec_at;    // compartment * 8 + gate -> index in grec
+    // (a point neuron's columns all read its state variables: V is its membrane-voltage one)
+    std::vector<std::pair<int64_t, int32_t>> grec;   // (compartment, gate or state variable) of these columns
+    std::unordered_map<int64_t, int32_t> grec_at;    // compartment * 32 + variable -> index in grec
     const bool gate_rec = d->abi_version >= 3 && d->record_gate;
+    std::vector<int32_t> rec_var(d->n_record, -1);
     for (int64_t r = 0; r < d->n_record; r++) {
         int64_t g = d->record_comp[r];
         if (g < 0 || g >= NC) return fail(ctx, EDENHIP_E_INVALID, "record_comp out of range");
         const int32_t j = gate_rec ? d->record_gate[r] : -1;
+        if (acell_slot[g] >= 0) {
+            const int32_t ty = acell_type[(size_t)acell_slot[g]];
+            const int32_t v = j < 0 ? LC->v_state[ty] : j;
+            if (j < -1 || v < 0 || v >= LC->n_state[ty])
+                return fail(ctx, EDENHIP_E_INVALID, "record_gate: a point neuron's state variable (-1: its membrane voltage)");
+            rec_var[r] = v;
+            continue;
+        }
         if (j < -1 || j >= (int32_t)((gmeta[g] >> 24) & 7u))
             return fail(ctx, EDENHIP_E_INVALID, "record_gate: -1 or a gate variable of the compartment");
+        rec_var[r] = j;
         if (j >= 0) continue;
         if (rec_col[g] < 0) rec_col[g] = ctx->n_dev_cols++;
         ctx->desc_to_dev_col[r] = rec_col[g];
     }
-    for (int64_t r = 0; gate_rec && r < d->n_record; r++) {
-        const int32_t j = d->record_gate[r];
-        if (j < 0) continue;
+    for (int64_t r = 0; r < d->n_record; r++) {
         const int64_t g = d->record_comp[r];
-        auto it = grec_at.emplace(g * 8 + j, (int32_t)grec.size()).first;
+        const int32_t j = rec_var[r];
+        if (j < 0) continue;
+        auto it = grec_at.emplace(g * 32 + j, (int32_t)grec.size()).first;
         if (it->second == (int32_t)grec.size()) grec.push_back({g, j});
         ctx->desc_to_dev_col[r] = ctx->n_dev_cols + it->second;
     }
@@ -509,6 +566,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         if (d->parent[c0] != -1) return fail(ctx, EDENHIP_E_INVALID, "compartment 0 of a cell must be its root");
         for (int32_t i = 1; i < n; i++)
             if (d->parent[c0 + i] < 0 || d->parent[c0 + i] >= i) return fail(ctx, EDENHIP_E_INVALID, "parents must precede children");
+        if (acell_slot[c0] >= 0) continue;   // a point neuron: lems_cell_kernel
         if (n == 1) {
             point.push_back((int32_t)c0);
             ctx->point_desc.push_back(c);
@@ -663,6 +721,23 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         }
         ctx->tree_net = !tsyn.empty();
     }
+    // and for the point neurons that have synapses
+    for (size_t b0 = 0; b0 < acell_comp.size(); b0 += 64) {
+        int64_t kmax = 0;
+        for (size_t l = 0; l < 64 && b0 + l < acell_comp.size(); l++) {
+            const int64_t g = acell_comp[b0 + l];
+            kmax = std::max<int64_t>(kmax, d->comp_syn_begin[g + 1] - d->comp_syn_begin[g]);
+        }
+        for (size_t l = 0; l < 64 && b0 + l < acell_comp.size(); l++) {
+            const int64_t g = acell_comp[b0 + l];
+            const int64_t cnt = d->comp_syn_begin[g + 1] - d->comp_syn_begin[g];
+            syn_base[g] = (int32_t)(n_slots + (int64_t)l);
+            syn_cnt[g] = (int32_t)cnt;
+            for (int64_t k = 0; k < cnt; k++) blocked_of[d->comp_syn_begin[g] + k] = n_slots + 64 * k + (int64_t)l;
+        }
+        n_slots += 64 * kmax;
+        if (n_slots >= ((int64_t)1 << 31)) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapse slots");
+    }
     for (int64_t s = 0; s < n_syn; s++)
         if (blocked_of[s] < 0) return fail(ctx, EDENHIP_E_INVALID, "internal: synapse without a slot");
     std::vector<int32_t> syn_type(n_slots, 0), syn_orig(n_slots, -1), spk_target(n_spk);
@@ -679,7 +754,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     ctx->n_slots = n_slots;
     // generic synapses run in their own kernel before the compartments of each step: one
     // launch per step, like a network
-    ctx->network = any_targets || (lems && n_syn > 0);
+    ctx->network = any_targets || (lems && n_syn > 0) || ctx->n_acells > 0;
     ctx->tree_net = ctx->tree_net || ctx->network;
     ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0;
 
@@ -689,8 +764,14 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     std::vector<int64_t> grec_src(grec.size());
     for (size_t k = 0; k < grec.size(); k++) {
         const int64_t g = grec[k].first;
+        const int64_t j = grec[k].second;
         grec_col[k] = ctx->n_dev_cols - (int32_t)grec.size() + (int32_t)k;
-        grec_src[k] = gq_first[g] >= 0 ? ~(gq_first[g] + grec[k].second) : g * NQ + grec[k].second;
+        if (acell_slot[g] >= 0)
+            grec_src[k] = REC_CELL_STATE | (j * ctx->n_acells + acell_slot[g]);
+        else if (gq_first[g] >= 0)
+            grec_src[k] = REC_TREE_GATE | (gq_first[g] + j);
+        else
+            grec_src[k] = g * NQ + j;
     }
     ctx->n_grec = (int32_t)grec.size();
     if (ctx->n_grec > 0) {
@@ -760,8 +841,36 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(upload(ctx, &ctx->d_lems_peer, peer));
         TRY(alloc_zero(ctx, &ctx->d_syn_ext, (size_t)NC));
         T.syn_ext = ctx->d_syn_ext;
+    }
+    if ((lems && n_syn > 0) || ctx->n_acells > 0) {
+        // the generated synapse and point-neuron functions, each side stubbed when absent
+        std::string gen = lems ? std::string(LS->source)
+                               : std::string("__device__ __forceinline__ void lems_syn(int, const float *, const float *, "
+                                             "float *, long, float, float, int, int, float, float *i, float *g) "
+                                             "{ *i = 0.0f; *g = 0.0f; }\n");
+        gen += "\n";
+        gen += LC ? std::string(LC->source)
+                  : std::string("__device__ __forceinline__ void lems_cell(int, const float *, const float *, float *, "
+                                "long, float, float, int, float, int *spike) { *spike = 0; }\n");
         std::string err;
-        if (rtc_lems_kernel(LS->source, &ctx->lems_fn, err) != 0) return fail(ctx, EDENHIP_E_INVALID, err);
+        hipFunction_t syn_fn = nullptr, cell_fn = nullptr;
+        if (rtc_lems_kernel(gen, &syn_fn, &cell_fn, err) != 0) return fail(ctx, EDENHIP_E_INVALID, err);
+        if (lems && n_syn > 0) ctx->lems_fn = syn_fn;
+        if (ctx->n_acells > 0) ctx->lems_cell_fn = cell_fn;
+    }
+    if (ctx->n_acells > 0) {
+        // point-neuron tables: compartments, types, parameters, state [variable][slot]
+        int32_t smax = 1;
+        for (int t = 0; t < LC->n_types; t++) smax = std::max(smax, LC->n_state[t]);
+        std::vector<float> prm(LC->params, LC->params + LC->param_begin[LC->n_types]);
+        std::vector<int32_t> pb(LC->param_begin, LC->param_begin + LC->n_types + 1);
+        std::vector<int32_t> vs(LC->v_state, LC->v_state + LC->n_types);
+        TRY(upload(ctx, &ctx->d_acell_comps, acell_comp));
+        TRY(upload(ctx, &ctx->d_acell_type, acell_type));
+        TRY(upload(ctx, &ctx->d_acell_params, prm));
+        TRY(upload(ctx, &ctx->d_acell_pbegin, pb));
+        TRY(upload(ctx, &ctx->d_acell_vstate, vs));
+        TRY(alloc_zero(ctx, &ctx->d_acell_state, (size_t)smax * (size_t)ctx->n_acells));
     }
     TRY(upload(ctx, (uint32_t **)&T.cell_seed, seeds));
     TRY(upload(ctx, (int32_t **)&T.comp_cell, comp_cell));
@@ -808,6 +917,11 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         int32_t smax = 1;
         for (int t = 0; t < LS->n_types; t++) smax = std::max(smax, LS->n_state[t]);
         TRY(snap(ctx->d_lems_state, (size_t)(smax * n_slots) * sizeof(float)));
+    }
+    if (ctx->d_acell_state) {
+        int32_t smax = 1;
+        for (int t = 0; t < LC->n_types; t++) smax = std::max(smax, LC->n_state[t]);
+        TRY(snap(ctx->d_acell_state, (size_t)smax * (size_t)ctx->n_acells * sizeof(float)));
     }
 
     TRY(upload(ctx, &ctx->d_point, point));
@@ -1150,8 +1264,9 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 // (one step per launch whenever gates are recorded)
                 const int64_t row = a - 1 - row0;
                 hipLaunchKernelGGL(record_gates_kernel, dim3((ctx->n_grec + 255) / 256), dim3(256), 0, ctx->stream, T,
-                                   (const float *)ctx->d_gq, (const int32_t *)ctx->d_grec_col,
-                                   (const int64_t *)ctx->d_grec_src, ctx->n_grec, row);
+                                   (const float *)ctx->d_gq, (const float *)ctx->d_acell_state,
+                                   (const int32_t *)ctx->d_grec_col, (const int64_t *)ctx->d_grec_src, ctx->n_grec,
+                                   row);
                 HIPCHECK(ctx, hipGetLastError());
                 nl++;
             }
@@ -1164,6 +1279,17 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 void *args[] = {&T, &la, &st, &dt};
                 HIPCHECK(ctx, hipModuleLaunchKernel(ctx->lems_fn, (unsigned)((ctx->n_lems_comps + 63) / 64), 1, 1, 64, 1, 1,
                                                     0, ctx->stream, args, nullptr));
+                nl++;
+            }
+            if (ctx->lems_cell_fn && ctx->n_acells > 0) {
+                // point neurons of step a, after the synapse currents of the step
+                LemsCellArgs ca{ctx->d_acell_comps, ctx->d_acell_type, ctx->n_acells, ctx->d_acell_params,
+                                ctx->d_acell_pbegin, ctx->d_acell_vstate, ctx->d_acell_state};
+                int64_t st = a;
+                float dt = ctx->dt;
+                void *args[] = {&T, &ca, &st, &dt};
+                HIPCHECK(ctx, hipModuleLaunchKernel(ctx->lems_cell_fn, (unsigned)((ctx->n_acells + 63) / 64), 1, 1, 64, 1,
+                                                    1, 0, ctx->stream, args, nullptr));
                 nl++;
             }
             if (ctx->n_point > 0) {
@@ -1282,7 +1408,7 @@ int edenhip_create(const edenhip_model_desc *desc, const edenhip_options *opt, e
     *out = nullptr;
     edenhip_ctx *ctx = new (std::nothrow) edenhip_ctx();
     if (!ctx) return EDENHIP_E_INVALID;
-    // options of version 1 and 2 callers end before channel_wave
+    // options of version 1 and 2 callers end before channel_wave (versions 3 and 4 share them)
     if (opt) std::memcpy(&ctx->opt, opt, desc && desc->abi_version >= 3 ? sizeof(edenhip_options)
                                                                          : offsetof(edenhip_options, channel_wave));
     if (ctx->opt.persistent == 0) ctx->opt.persistent = 1;

@@ -18,20 +18,6 @@
 
 namespace edenhip {
 
-__device__ __forceinline__ void log_event(const DevTables &T, int32_t col, int64_t step) {
-    // spike detected while computing step `step` is logged at row `step` (the next row)
-    if (col >= 0 && step < T.total_rows) {
-        uint32_t slot = atomicAdd(T.n_events, 1u);
-        if (slot < T.max_events) T.events[slot] = ((int64_t)step << 24) | (int64_t)col;
-    }
-}
-
-__device__ __forceinline__ void send_spike(const DevTables &T, int64_t gidx, int nxt) {
-    const int32_t b = T.spk_begin[gidx], e = T.spk_begin[gidx + 1];
-    int32_t *trig = T.trig[nxt];
-    for (int32_t k = b; k < e; k++) trig[T.spk_target[k]] = 1;   // idempotent flag (EDEN: atomic OR 1)
-}
-
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
 point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const DSynType *__restrict__ stypes_g,
              int n_stypes, const int32_t *__restrict__ comps, int32_t n, int64_t s0, int64_t s1, int64_t row0,

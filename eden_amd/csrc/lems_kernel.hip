@@ -14,11 +14,20 @@
 // code. State is updated in place (the generated code reads an instance's whole present
 // state before writing).
 //
-// The generated source is inserted where LEMS_SYNAPSE_FUNCTIONS is included.
+// lems_cell_kernel runs LEMS point neurons (the reference's artificial cells,
+// eden/Eden.cpp:6603-6910): one lane per cell; per step, the external current (the cell's
+// synapse currents from syn_ext, then its pulse inputs), the generated `lems_cell` step, the
+// compartment's V set to the membrane-voltage state variable, and a spike sent to the
+// recipients and logged as a physical cell's is.
+//
+// The generated source (`lems_syn` and `lems_cell`) is inserted where
+// lems_synapse_functions.h is included.
 #include "device_model.hpp"
 
 #define EXP edenhip::exp_f
 #define LOG edenhip::log_f
+#define POW edenhip::pow_f
+__device__ __forceinline__ float stepf(float x) { return x < 0.0f ? 0.0f : 1.0f; }
 
 #include "lems_synapse_functions.h"
 
@@ -79,4 +88,37 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
     I_syn += I_agg;
     G_syn += G_agg;
     L.syn_ext[g] = make_float2(I_syn, G_syn);
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+lems_cell_kernel(edenhip::DevTables T, edenhip::LemsCellArgs A, int64_t step, float dt) {
+    using namespace edenhip;
+    const int32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= A.n) return;
+    const int32_t g = A.comps[idx], ty = A.type[idx];
+    const bool init = step <= 0;
+    const float t = (float)(step >= 1 ? T.t_row[step - 1] : 0.0);
+    const int nxt = (int)((step + 3) & 1) ^ 1;
+    // external current: synapses, then inputs (eden/Eden.cpp:6856-6880)
+    float I_syn = 0.0f;
+    if (T.syn_ext && T.syn_cnt[g] > 0) I_syn = T.syn_ext[g].x;
+    float I_in = 0.0f;
+    const int32_t pb = T.pulse_begin[g], pe = T.pulse_begin[g + 1];
+    if (pe > pb) {
+        float Ip = 0.0f;
+        for (int32_t p = pb; p < pe; p++)
+            if (T.pulse_start[p] <= t && t <= T.pulse_end[p]) Ip += T.pulse_amp[p];
+        I_in += Ip;
+        I_in += 0.0f;
+    }
+    const float ext = I_syn + I_in;
+    float *S = A.state + idx;
+    int spike = 0;
+    lems_cell(ty, A.params + A.param_begin[ty], S, S, (long)A.n, t, dt, init ? 1 : 0, ext, &spike);
+    const int32_t vs = A.v_state[ty];
+    if (vs >= 0) T.V[g] = S[(int64_t)vs * A.n];
+    if (spike) {
+        send_spike(T, g, nxt);
+        log_event(T, T.ev_col[g], step);
+    }
 }

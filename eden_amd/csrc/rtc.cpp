@@ -1,7 +1,7 @@
-// rtc.cpp -- run-time compilation of the generic LEMS synapse kernel with hiprtc. The kernel
-// template (lems_kernel.hip) and the device header it includes (device_model.hpp) are
-// embedded in the library at build time (_embedded_sources.inc, written by eden_amd/build.py);
-// the generated synapse functions come from the caller. This replaces the reference's
+// rtc.cpp -- run-time compilation of the generic LEMS synapse and point-neuron kernels with
+// hiprtc. The kernel template (lems_kernel.hip) and the device header it includes
+// (device_model.hpp) are embedded in the library at build time (_embedded_sources.inc,
+// written by eden_amd/build.py); the generated synapse and cell functions come from the caller. This replaces the reference's
 // system("gcc ...") and dlopen of the emitted C (eden/Eden.cpp:6992-7216) with an in-process
 // compile for the GPU.
 #include "rtc.hpp"
@@ -20,12 +20,12 @@ namespace {
 
 struct Compiled {
     hipModule_t module = nullptr;
-    hipFunction_t fn = nullptr;
+    hipFunction_t fn = nullptr, cell_fn = nullptr;
 };
 
 }  // namespace
 
-int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, std::string &err) {
+int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, hipFunction_t *cell_fn, std::string &err) {
     static std::mutex mu;
     static std::map<std::string, Compiled> cache;   // key: device arch + '\n' + source
     int dev = 0;
@@ -46,6 +46,7 @@ int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, std::string
     auto it = cache.find(key);
     if (it != cache.end()) {
         *fn = it->second.fn;
+        *cell_fn = it->second.cell_fn;
         return 0;
     }
     hiprtcProgram prog;
@@ -75,7 +76,7 @@ int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, std::string
         hiprtcGetProgramLogSize(prog, &n);
         std::string log(n, '\0');
         if (n) hiprtcGetProgramLog(prog, &log[0]);
-        err = "hiprtc could not compile the synapse kernel: " + log;
+        err = "hiprtc could not compile the generated LEMS kernels: " + log;
         hiprtcDestroyProgram(&prog);
         return -1;
     }
@@ -89,12 +90,14 @@ int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, std::string
         err = "hipModuleLoadData of the synapse kernel failed";
         return -1;
     }
-    if (hipModuleGetFunction(&c.fn, c.module, "lems_syn_kernel") != hipSuccess) {
-        err = "lems_syn_kernel not found in the compiled module";
+    if (hipModuleGetFunction(&c.fn, c.module, "lems_syn_kernel") != hipSuccess ||
+        hipModuleGetFunction(&c.cell_fn, c.module, "lems_cell_kernel") != hipSuccess) {
+        err = "lems_syn_kernel / lems_cell_kernel not found in the compiled module";
         return -1;
     }
     cache[key] = c;
     *fn = c.fn;
+    *cell_fn = c.cell_fn;
     return 0;
 }
 

@@ -24,8 +24,9 @@ Semantics of one step of one instance (eden/Eden.cpp:2700-2995):
   - exposures i (current) and g (conductance; 0 for current-based synapses) last, a state
     variable's as assigned.
 Expressions are printed fully parenthesised, left to right as parsed, with every literal
-rounded to float, `expf`/`logf` as EXP/LOG (the dialect's own: on the device and in the
-oracle, exp/log rounded once from double), so both compilers perform the same float
+rounded to float, `expf`/`logf`/`powf` as EXP/LOG/POW (the dialect's own: on the device and
+in the oracle, exp/log/pow rounded once from double), H as stepf (0 below zero, else 1, as
+the reference defines it, eden/Eden.cpp:3197), so both compilers perform the same float
 operations in the same order. Engine units (us, mV, nA, uS) make every unit conversion
 factor of these types 1 (the reference's Convert::Suffix then emits nothing).
 
@@ -205,7 +206,8 @@ BASE_SYNAPSES = {
 # LEMS dimension names -> the reader's unit dimensions (None: dimensionless)
 LEMS_DIMS = {"time": "time", "per_time": "frequency", "voltage": "voltage", "current": "current",
              "conductance": "conductance", "capacitance": "capacitance", "length": "length",
-             "concentration": "concentration",
+             "concentration": "concentration", "conductance_per_voltage": "conductance_per_voltage",
+             "voltage_per_time": "voltage_per_time", "currentDensity": "currentDensity", "per_voltage": "per_voltage",
              "resistance": "resistance", "conductanceDensity": "conductivity", "temperature": "temperature",
              "none": None, "": None}
 
@@ -345,7 +347,8 @@ _TOKEN = re.compile(r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|
 _REL = {".gt.": ">", ".lt.": "<", ".geq.": ">=", ".ge.": ">=", ".leq.": "<=", ".le.": "<=", ".eq.": "==",
         ".neq.": "!=", ".ne.": "!="}
 _FUNCS = {"exp": "EXP", "log": "LOG", "ln": "LOG", "sqrt": "sqrtf", "abs": "fabsf", "sin": "sinf", "cos": "cosf",
-          "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "ceil": "ceilf", "floor": "floorf"}
+          "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "ceil": "ceilf", "floor": "floorf",
+          "H": "stepf"}
 
 
 def _tokens(text):
@@ -480,7 +483,7 @@ def infix(tree, names):
     if k == "fn":
         return _FUNCS[tree[1]] + "( " + infix(tree[2], names) + " )"
     if k == "pow":
-        return "powf( " + infix(tree[1], names) + " , " + infix(tree[2], names) + " )"
+        return "POW( " + infix(tree[1], names) + " , " + infix(tree[2], names) + " )"
     if k == "cmp":
         return "( ( " + infix(tree[2], names) + " " + tree[1] + " " + infix(tree[3], names) + " ) )"
     if k == "not":
@@ -493,7 +496,7 @@ def infix(tree, names):
 
 _NP_FUNCS = {"exp": np.exp, "log": np.log, "ln": np.log, "sqrt": np.sqrt, "abs": np.abs, "sin": np.sin,
              "cos": np.cos, "tan": np.tan, "sinh": np.sinh, "cosh": np.cosh, "tanh": np.tanh, "ceil": np.ceil,
-             "floor": np.floor}
+             "floor": np.floor, "H": lambda x: np.float32(0.0 if x < 0 else 1.0)}
 
 
 def evaluate(tree, values):

@@ -107,6 +107,9 @@ class Model:
     lems: object = None                # eden_amd.lems.LemsSynapses: generic synapse types
     syn_peer: np.ndarray = None        # per synapse instance: the gap junction's peer compartment, or -1
     record_gate: np.ndarray = None     # per recorded column: -1 V, j >= 0 the compartment's j-th gate variable
+                                       # (a point neuron's j-th state variable)
+    lems_cells: object = None          # eden_amd.lems_cells.LemsCells, or None
+    cell_lems_type: np.ndarray = None  # per cell: its point-neuron type, or -1
     name: str = "model"
 
     def __post_init__(self):
@@ -232,7 +235,7 @@ class Model:
         d.syn_types = C.cast(self._st, C.c_void_p)
         for name, _ in N.ModelDesc._fields_:
             if name in ("abi_version", "dt", "t_final", "n_cells", "n_comps", "n_channel_types", "channel_types",
-                        "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems"):
+                        "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems", "lems_cells"):
                 continue
             setattr(d, name, N.ptr(getattr(self, name)))
         if self.lems is not None:
@@ -249,6 +252,18 @@ class Model:
                                                  N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]),
                                                  N.ptr(peer))
             d.lems = C.pointer(self._lems_desc)
+        if self.lems_cells is not None:
+            LC = self.lems_cells
+            pb = np.zeros(len(LC.params) + 1, np.int32)
+            np.cumsum([len(p) for p in LC.params], out=pb[1:])
+            ctype = np.ascontiguousarray(self.cell_lems_type, np.int32)
+            self._lc_arrays = (pb, np.concatenate(LC.params + [np.zeros(1, np.float32)]).astype(np.float32),
+                               np.array(LC.n_state, np.int32), np.array(LC.v_state, np.int32), ctype,
+                               LC.source_hip.encode())
+            a = self._lc_arrays
+            self._lc_desc = N.LemsCellsDesc(len(LC.params), a[5], N.ptr(a[0]), N.ptr(a[1]), N.ptr(a[2]), N.ptr(a[3]),
+                                            N.ptr(a[4]))
+            d.lems_cells = C.pointer(self._lc_desc)
         d.n_record = len(self.record_comp)
         d.n_event_cols = len(self.event_comp)
         return d

@@ -39,12 +39,23 @@ def write_trajectory(path, time_us, trace, scale=MV_TO_V, append=False):
 
 
 def column_scales(model):
-    """Per recorded column, the factor from engine units to the file's: 1e-3 for a membrane
-    voltage (mV to V), 1 for a gate variable (dimensionless, LEMS_Pure in
-    eden/NeuroML.cpp:1016)."""
+    """Per recorded column, the factor from engine units to the file's SI units: 1e-3 for a
+    membrane voltage (mV to V), 1 for a gate variable (dimensionless, LEMS_Pure in
+    eden/NeuroML.cpp:1016), and for a point neuron's state variable the native unit of its
+    dimension (nA: 1e-9, us: 1e-6, ...)."""
     sc = np.full(len(model.record_comp), MV_TO_V)
     if model.record_gate is not None:
         sc[np.asarray(model.record_gate) >= 0] = 1.0
+    LC = getattr(model, "lems_cells", None)
+    if LC is not None:
+        cell_of = np.searchsorted(model.cell_comp_begin, model.record_comp, side="right") - 1
+        for r, c in enumerate(cell_of):
+            ty = int(model.cell_lems_type[c])
+            if ty < 0:
+                continue
+            j = -1 if model.record_gate is None else int(model.record_gate[r])
+            k = LC.v_state[ty] if j < 0 else j
+            sc[r] = 10.0 ** LC.state_pow10[ty][k]
     return sc
 
 

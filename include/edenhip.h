@@ -29,9 +29,11 @@
 extern "C" {
 #endif
 
-#define EDENHIP_ABI_VERSION 3   /* 2 adds edenhip_model_desc.lems, 3 adds edenhip_options.channel_wave;
-                                   descriptors of versions 1 and 2 are accepted (their options are
-                                   read without channel_wave) */
+#define EDENHIP_ABI_VERSION 4   /* 2 adds edenhip_model_desc.lems, 3 adds edenhip_options.channel_wave,
+                                   edenhip_model_desc.record_gate and edenhip_lems_synapses.peer,
+                                   4 adds edenhip_model_desc.lems_cells; descriptors of versions 1-3
+                                   are accepted (options of versions 1 and 2 are read without
+                                   channel_wave) */
 
 enum {
     EDENHIP_OK = 0,
@@ -107,6 +109,29 @@ typedef struct edenhip_lems_synapses {
                                      is the instance's vpeer (the other end of a gap junction,
                                      eden/Eden.cpp:9225-9248), or -1; NULL = none */
 } edenhip_lems_synapses;
+
+/* LEMS point neurons (the reference's artificial cells: iafCell, izhikevich2007Cell,
+ * adExIaFCell, the PyNN cells, user-defined cell ComponentTypes, ...), whose generated C the
+ * reference compiles per cell type (eden/Eden.cpp:6603-6910). The caller supplies HIP source
+ * defining
+ *   __device__ void lems_cell(int type, const float *P, const float *S, float *Sn, long stride,
+ *                             float t, float dt, int init, float isyn, int *spike)
+ * (one cell's step: P its type's parameters, S / Sn its present / next state with variable k
+ * at k * stride, t the step's time, isyn its external current (its synapses' currents, then
+ * its inputs'), *spike set when it emits a spike), which edenhip_create() compiles with hiprtc
+ * beside lems_syn (eden_amd.lems_cells emits it). A point-neuron cell has one compartment,
+ * without channels or Poisson inputs; its synapses take the generic path (lems), and the
+ * compartment's V mirrors the type's membrane-voltage state variable (its synapses' and gap
+ * junction peers' voltage). Models with point neurons run one launch per step. */
+typedef struct edenhip_lems_cells {
+    int32_t n_types;
+    const char *source;           /* HIP C++ source, NUL-terminated */
+    const int32_t *param_begin;   /* [n_types + 1] into params */
+    const float *params;          /* per type: its parameter vector */
+    const int32_t *n_state;       /* [n_types] state variables per cell (at most 16) */
+    const int32_t *v_state;       /* [n_types] the state variable exposed as the membrane voltage, or -1 */
+    const int32_t *cell_type;     /* [n_cells] each cell's type, -1 for a physical cell */
+} edenhip_lems_cells;
 
 /*
  * The instantiated model. Global compartment index g runs over all cells in cell order;
@@ -185,9 +210,13 @@ typedef struct edenhip_model_desc {
 
     /* (abi_version >= 3) what each recorded column holds: -1 = the compartment's V; j >= 0 =
      * its j-th gate variable (its channel distributions in order, each type's gates in order),
-     * as an OutputColumn of a channel gate's q records it. NULL = V everywhere. A model that
-     * records gate variables runs one launch per step. */
+     * as an OutputColumn of a channel gate's q records it; for a point neuron (abi_version >= 4)
+     * j >= 0 is its j-th state variable and -1 its membrane-voltage state variable. NULL = V
+     * everywhere. A model that records gate variables runs one launch per step. */
     const int32_t *record_gate;
+
+    /* (abi_version >= 4) LEMS point neurons, or NULL */
+    const edenhip_lems_cells *lems_cells;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

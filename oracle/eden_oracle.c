@@ -32,6 +32,8 @@
  *   Hines forward elimination / back-substitution         eden/Eden.cpp:6114-6225
  *   spike detection and atomic-OR send                    eden/Eden.cpp:3946-3995, 6248
  *   event logger (flag read + clear at output time)       eden/Eden.cpp:11598-11635
+ *   LEMS point neurons (artificial cells)                 eden/Eden.cpp:6603-6910 (the generated
+ *                                                         step: eden_amd/lems_cells.py, C dialect)
  *
  * Pinned by tests/test_oracle.py (published hash steps, closed forms, scipy LSODA, dense
  * solves, hand-enumerated delay tables) except, PARITY UNPINNED:
@@ -45,7 +47,10 @@
  *     2^-23 resolution and the uniformity, but swapped fields would pass them; only a
  *     reference output could fix the layout, and none exists;
  *   - the instance order of synapses from several projections onto one compartment
- *     (eden/Eden.cpp:8791-9281): followed from the source, not fixed by a fixture.
+ *     (eden/Eden.cpp:8791-9281): followed from the source, not fixed by a fixture;
+ *   - the point neurons' and generic synapses' generated steps beyond the closed forms the
+ *     tests check (integrate-and-fire relaxation, reset and refractory period, one Izhikevich
+ *     step by hand, the synapse charges): no reference output fixes their float results.
  */
 #include <math.h>
 #include <stdint.h>
@@ -167,8 +172,13 @@ static void log_poi_event(int64_t p, int64_t step) {
 typedef void (*lems_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float v, float vpeer,
                           int init, int event, float dt, float *i_out, float *g_out);
 static lems_fn_t g_lems = NULL;
+/* LEMS point neurons (the C dialect of the model's lems_cell source, compiled by the caller) */
+typedef void (*lems_cell_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float t, float dt,
+                               int init, float isyn, int *spike);
+static lems_cell_fn_t g_lems_cell = NULL;
 
 void FN(set_lems)(void *fn) { g_lems = (lems_fn_t)fn; }
+void FN(set_lems_cells)(void *fn) { g_lems_cell = (lems_cell_fn_t)fn; }
 
 static int64_t *g_syn_ev = NULL;
 static int64_t g_syn_ev_max = 0;
@@ -230,6 +240,9 @@ typedef struct {
     real *pT[2], *pI[2];    /* Poisson tsince, isi */
     real *pA[2], *pB[2];    /* Poisson synapse A, B */
     float *lS[2];           /* generic LEMS synapse state [k][synapse] */
+    float *cS[2];           /* LEMS point-neuron state [k][slot] */
+    int32_t *cslot;         /* per compartment: its point-neuron slot, or -1 */
+    int64_t n_cslots;
     int64_t *evflag[2];     /* event logger flags */
     /* derived tables */
     int64_t *gate_begin;    /* per channel instance */
@@ -247,6 +260,14 @@ typedef struct {
 static real recorded(const edenhip_model_desc *d, const ostate *s, int now, int64_t r) {
     const int32_t g = d->record_comp[r];
     const int32_t j = (d->abi_version >= 3 && d->record_gate) ? d->record_gate[r] : -1;
+    if (s->cslot && s->cslot[g] >= 0) {
+        /* a point neuron: its state variable j (-1: the membrane voltage one) */
+        const edenhip_lems_cells *LC = d->lems_cells;
+        int32_t cell = 0;
+        while (d->cell_comp_begin[cell + 1] <= g) cell++;
+        const int32_t k = j < 0 ? LC->v_state[LC->cell_type[cell]] : j;
+        return s->cS[now][(int64_t)k * s->n_cslots + s->cslot[g]];
+    }
     if (j < 0) return s->V[now][g];
     return s->q[now][s->gate_begin[d->comp_chan_begin[g]] + j];
 }
@@ -449,6 +470,27 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
             I_input_total += I_agg;
             G_input_total += G_agg;
         }
+        if (s->cslot && s->cslot[g] >= 0) {
+            /* a LEMS point neuron (eden/Eden.cpp:6855-6900): its external current is the synapse
+               currents plus the input currents; its membrane-voltage state is the compartment's V */
+            const edenhip_lems_cells *LC = d->lems_cells;
+            const int64_t slot = s->cslot[g], ns = s->n_cslots;
+            const int32_t ty = LC->cell_type[cell];
+            const float ext = (float)I_synapses_total + (float)I_input_total;
+            int spike = 0;
+            g_lems_cell(ty, LC->params + LC->param_begin[ty], s->cS[now] + slot, s->cS[nxt] + slot, (long)ns,
+                        (float)time_r, (float)dt, initial_state, ext, &spike);
+            const int32_t vs = LC->v_state[ty];
+            Vn[g] = vs >= 0 ? (real)s->cS[nxt][(int64_t)vs * ns + slot] : V[g];
+            if (spike) {
+                for (int64_t k = d->comp_spike_begin[g]; k < d->comp_spike_begin[g + 1]; k++)
+                    __atomic_fetch_or(&s->trig[nxt][d->spike_target[k]], (int64_t)1, __ATOMIC_RELAXED);
+                for (int64_t k = s->comp_ev_begin[g]; k < s->comp_ev_begin[g + 1]; k++)
+                    __atomic_fetch_or(&s->evflag[nxt][s->comp_ev[k]], (int64_t)1, __ATOMIC_RELAXED);
+            }
+            (void)G_synapses_total; (void)G_input_total; (void)G_channels_total; (void)I_channels_total;
+            return;   /* one compartment, no cable */
+        }
         real I_internal = I_channels_total + I_input_total + I_synapses_total;
         real G_internal = G_channels_total + G_input_total + G_synapses_total;
         const real Cg = (real)d->capacitance[g];
@@ -521,6 +563,11 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     if (lems) return -1; /* generic synapses: fp32 flavours only */
 #endif
     if (lems && !g_lems) return -1;
+    const edenhip_lems_cells *LC = (d->abi_version >= 4 && d->lems_cells && d->lems_cells->n_types > 0) ? d->lems_cells : NULL;
+#if defined(ORACLE_F64)
+    if (LC) return -1; /* point neurons: fp32 flavours only */
+#endif
+    if (LC && !g_lems_cell) return -1;
     const int64_t NC = d->n_comps;
     const int64_t NCH = d->comp_chan_begin[NC];
     const int64_t NSY = d->comp_syn_begin[NC];
@@ -528,6 +575,16 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     if (d->abi_version >= 3 && d->record_gate)
         for (int64_t r = 0; r < d->n_record; r++) {
             const int32_t g = d->record_comp[r];
+            if (LC) {
+                int32_t cell = 0;
+                while (d->cell_comp_begin[cell + 1] <= g) cell++;
+                const int32_t ty = LC->cell_type[cell];
+                if (ty >= 0) {
+                    const int32_t j = d->record_gate[r];
+                    if (j < -1 || j >= LC->n_state[ty] || (j == -1 && LC->v_state[ty] < 0)) return -1;
+                    continue;
+                }
+            }
             int64_t ng = 0;
             for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++)
                 ng += d->channel_types[d->chan_type[ch]].n_gates;
@@ -548,6 +605,15 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     }
     ALLOC(s.D, NC, real);
     ALLOC(s.order, NC, int32_t);
+    if (LC) {
+        int32_t smax = 1;
+        for (int32_t t = 0; t < LC->n_types; t++) smax = LC->n_state[t] > smax ? LC->n_state[t] : smax;
+        ALLOC(s.cslot, NC, int32_t);
+        for (int64_t g = 0; g < NC; g++) s.cslot[g] = -1;
+        for (int32_t c = 0; c < d->n_cells; c++)
+            if (LC->cell_type[c] >= 0) s.cslot[d->cell_comp_begin[c]] = (int32_t)s.n_cslots++;
+        for (int b = 0; b < 2; b++) ALLOC(s.cS[b], (int64_t)smax * (s.n_cslots > 0 ? s.n_cslots : 1), float);
+    }
     if (lems) {
         int32_t smax = 1;
         for (int32_t t = 0; t < d->lems->n_types; t++) smax = d->lems->n_state[t] > smax ? d->lems->n_state[t] : smax;
@@ -631,5 +697,6 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     }
     free(s.D); free(s.order); free(s.gate_begin); free(s.comp_ev_begin); free(s.comp_ev);
     free(s.lS[0]); free(s.lS[1]);
+    free(s.cS[0]); free(s.cS[1]); free(s.cslot);
     return rows;
 }

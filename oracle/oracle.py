@@ -49,9 +49,10 @@ def lib():
             pc = getattr(L, f"oracle_{sfx}_poisson_log_count")
             pc.restype = C.c_int64
             pc.argtypes = []
-            sl = getattr(L, f"oracle_{sfx}_set_lems")
-            sl.restype = None
-            sl.argtypes = [C.c_void_p]
+            for nm in ("set_lems", "set_lems_cells"):
+                sl = getattr(L, f"oracle_{sfx}_{nm}")
+                sl.restype = None
+                sl.argtypes = [C.c_void_p]
             o = getattr(L, f"oracle_{sfx}_dfs_order")
             o.restype = None
             o.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
@@ -110,9 +111,12 @@ def load_variant(path):
 _LEMS_PRELUDE = {
     # the oracle's own exp/log: rounded once from a double evaluation (as the f32 flavour)
     "exact": "#include <math.h>\nstatic inline float EXP(float x) { return (float)exp((double)x); }\n"
-             "static inline float LOG(float x) { return (float)log((double)x); }\n",
-    # glibc expf/logf (the f32libm and f32ofast flavours)
-    "libm": "#include <math.h>\n#define EXP expf\n#define LOG logf\n",
+             "static inline float LOG(float x) { return (float)log((double)x); }\n"
+             "static inline float POW(float x, float y) { return (float)pow((double)x, (double)y); }\n"
+             "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
+    # glibc expf/logf/powf (the f32libm and f32ofast flavours)
+    "libm": "#include <math.h>\n#define EXP expf\n#define LOG logf\n#define POW powf\n"
+            "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
 }
 _LEMS_FLAGS = {
     "exact": ["-O2", "-ffp-contract=off"],
@@ -158,6 +162,12 @@ def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_
             raise RuntimeError("generic LEMS synapses: the oracle runs them in its fp32 flavours only")
         fn = C.cast(lems_library(model.lems.source_c, flav).lems_syn, C.c_void_p)
         getattr(L, f"oracle_{precision}_set_lems")(fn)
+    if getattr(model, "lems_cells", None) is not None:
+        flav = {"f32": "exact", "f32libm": "libm", "f32ofast": "ofast"}.get(precision)
+        if flav is None:
+            raise RuntimeError("LEMS point neurons: the oracle runs them in its fp32 flavours only")
+        fn = C.cast(lems_library(model.lems_cells.source_c, flav).lems_cell, C.c_void_p)
+        getattr(L, f"oracle_{precision}_set_lems_cells")(fn)
     total = model.total_rows() if n_rows < 0 else n_rows
     nrec = len(model.record_comp)
     trace = np.zeros((max(total, 1), max(nrec, 1)), np.float32) if want_trace else None

@@ -1,0 +1,132 @@
+"""LEMS point neurons (eden_amd.lems_cells) on the CPU.
+
+* The dimension calculus of the emitted code: where the operands' native units do not
+  multiply to the result's native unit, the product carries the power of ten between them
+  (izhikevich2007Cell's k (nS/mV) times a voltage: 1e-3; the time derivative of its current u:
+  1e3), and nowhere else (iafCell).
+* Closed forms of the oracle's runs: iafTauCell relaxes as v_n = E + (v_0 - E)(1 - dt/tau)^n
+  between spikes, spikes on the step after its present v passes the threshold and restarts
+  from the reset value; iafRefCell holds v at the reset value for the refractory period;
+  iafCell under a constant current relaxes towards E + I / g; izhikevich2007Cell's first step
+  from rest is the hand-computed Euler step of its equations in engine units.
+* The reference's ArtificialCells model reads every core point-neuron type with its
+  parameters in engine units and runs in the oracle.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from eden_amd import lems_cells, neuroml
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_emitted_unit_factors():
+    src = lems_cells.emit_cell(lems_cells.CELL_TYPES["izhikevich2007Cell"], "f", "c")
+    assert "( ( p_k * ( ( s_v - p_vr ) ) ) * 1e-3f )" in src
+    assert "n_u = s_u + dt * der_1 * 1e3f;" in src
+    assert "n_v = s_v + dt * der_0;" in src
+    src = lems_cells.emit_cell(lems_cells.CELL_TYPES["iafCell"], "f", "c")
+    assert " * 1e" not in src
+
+
+def _single(tmp_path, cell_xml, length_ms=200.0, dt_ms=0.025, pulse=None, record="v"):
+    """A LEMS file of one point neuron (optionally with a current pulse) recording `record`."""
+    pul = ""
+    inp = ""
+    if pulse:
+        delay, dur, amp = pulse
+        pul = f'<pulseGenerator id="p" delay="{delay}ms" duration="{dur}ms" amplitude="{amp}nA"/>'
+        inp = '<explicitInput target="pop[0]" input="p"/>'
+    p = tmp_path / "single.xml"
+    p.write_text(
+        f'<Lems><Target component="s"/><neuroml>{cell_xml}{pul}<network id="n">'
+        f'<population id="pop" component="c" size="1"/>{inp}</network></neuroml>'
+        f'<Simulation id="s" length="{length_ms}ms" step="{dt_ms}ms" target="n" seed="1">'
+        f'<OutputFile id="o" fileName="o.txt"><OutputColumn id="v" quantity="pop[0]/{record}"/></OutputFile>'
+        '<EventOutputFile id="e" fileName="e.txt" format="TIME_ID">'
+        '<EventSelection id="0" select="pop[0]" eventPort="spike"/></EventOutputFile>'
+        '</Simulation></Lems>')
+    m, _ = neuroml.read(str(p))
+    tr, ec, er, _ = oracle.run(m)
+    return m, tr[:, 0].astype(np.float64), er
+
+
+def test_iaf_tau_cell_closed_form(tmp_path):
+    E, th, reset, tau, dt = -50.0, -55.0, -70.0, 30000.0, 25.0
+    m, v, er = _single(tmp_path, f'<iafTauCell id="c" leakReversal="{E}mV" thresh="{th}mV" reset="{reset}mV" '
+                                  f'tau="{tau / 1000}ms"/>')
+    # the reference recurrence in double: spike when the present v passes the threshold
+    a = 1.0 - dt / tau
+    want, spikes, x = [E], [], E
+    for r in range(1, len(v)):
+        if x > th:
+            spikes.append(r)
+            x = reset
+        else:
+            x = E + (x - E) * a
+        want.append(x)
+    np.testing.assert_allclose(v, want, atol=2e-4)
+    assert list(er) == spikes and len(spikes) == 5
+
+
+def test_iaf_ref_cell_holds_reset_while_refractory(tmp_path):
+    m, v, er = _single(tmp_path, '<iafRefCell id="c" leakReversal="-50mV" thresh="-55mV" reset="-70mV" C="0.2nF" '
+                                 'leakConductance="0.01uS" refract="5ms"/>', pulse=(10, 150, 0.2))
+    assert len(er) >= 3
+    for r in er[er + 202 < len(v)]:
+        # spike on the step giving row r (lastSpikeTime = that step's t); refractory while
+        # t <= lastSpikeTime + refract: rows r .. r + 200 hold v = reset, row r + 201 moves
+        assert np.all(v[r:r + 201] == np.float32(-70.0)), r
+        assert v[r + 201] != np.float32(-70.0)
+
+
+def test_iaf_cell_relaxes_to_the_current_clamp_value(tmp_path):
+    """C dv/dt = g (E - v) + I: Euler steps relax to E + I/g with factor 1 - dt g / C (g in uS,
+    C in pF: dt g / C = 25 * 0.01 / 200)."""
+    m, v, er = _single(tmp_path, '<iafCell id="c" leakReversal="-70mV" thresh="0mV" reset="-80mV" C="0.2nF" '
+                                 'leakConductance="0.01uS"/>', pulse=(0, 400, 0.1), length_ms=300.0)
+    assert len(er) == 0
+    target = -70.0 + 0.1 / 0.01
+    a = 1.0 - 25.0 * 0.01 / 200.0
+    n = np.arange(len(v))
+    # the pulse starts at t = 0 (present from the first step on)
+    want = target + (-70.0 - target) * a ** n
+    # in float32 the approach stalls where a step's increment falls below half an ulp of v:
+    # |v - target| <= ulp(60) / 2 / (dt g / C) = 2^-18 / 1.25e-3 ~ 1.5e-3 mV
+    np.testing.assert_allclose(v, want, atol=2e-3)
+    assert abs(v[-1] - target) <= 2.0 ** -18 / 1.25e-3 + 1e-9
+
+
+def test_izhikevich2007_first_step_by_hand(tmp_path):
+    """From rest (v = v0, u = 0): v' = (k (v - vr)(v - vt) - u) / C, u' = a (b (v - vr) - u), one
+    Euler step of 25 us in SI units, then back to mV and nA."""
+    m, v, er = _single(tmp_path, '<izhikevich2007Cell id="c" v0="-70mV" C="100 pF" k="0.7 nS_per_mV" vr="-60 mV" '
+                                 'vt="-40 mV" vpeak="35 mV" a="0.03 per_ms" b="-2 nS" c="-50 mV" d="100 pA"/>',
+                       length_ms=1.0)
+    k, vr, vt, C = 0.7e-9 / 1e-3, -60e-3, -40e-3, 100e-12
+    v0 = -70e-3
+    dv = k * (v0 - vr) * (v0 - vt) / C * 25e-6
+    assert v[0] == np.float32(-70.0)
+    assert v[1] == pytest.approx((v0 + dv) * 1e3, abs=2e-5)
+
+
+def test_reference_artificial_cells_model_reads(tmp_path):
+    from artcells import write_model
+    m, outs = neuroml.read(write_model(str(tmp_path)))
+    names = m.lems_cells.type_names
+    assert names[:6] == ["iafTau", "iafTauRef", "iaf", "iafRef", "izBurst", "iz2007RS"]
+    assert "iafTauQuad" in names and "Test_HH_cond_exp" in names
+    assert list(m.cell_lems_type[:3]) == [-1, -1, -1]            # the passive targets
+    iz = m.lems_cells.params[names.index("iz2007RS")]
+    ct = lems_cells.CELL_TYPES["izhikevich2007Cell"]
+    p = dict(zip([x for x, _ in ct.params], iz.tolist()))
+    assert p["C"] == 100.0 and p["k"] == pytest.approx(0.7) and p["a"] == pytest.approx(3e-5)
+    assert p["b"] == pytest.approx(-2e-3) and p["d"] == pytest.approx(0.1)
+    tr, ec, er, _ = oracle.run(m)
+    counts = np.bincount(ec, minlength=len(m.event_comp))
+    assert counts.min() >= 2          # every recorded cell fires
+    assert math.isfinite(float(np.abs(tr).max()))

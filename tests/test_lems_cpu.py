@@ -39,7 +39,7 @@ def test_parser_precedence_and_associativity():
     assert lems.infix(lems.parse("a + b * c"), names) == "( ( a + ( ( b * c ) ) ) )"
     assert lems.infix(lems.parse("-a / b"), names) == "( ( ( -a ) / b ) )"
     assert lems.infix(lems.parse("exp(-x)"), names) == "EXP( ( -x ) )"
-    assert lems.infix(lems.parse("2^3"), names) == "powf( ( (float)2.0) , ( (float)3.0) )"
+    assert lems.infix(lems.parse("2^3"), names) == "POW( ( (float)2.0) , ( (float)3.0) )"
     with pytest.raises(lems.LemsError):
         lems.parse("a + (b")
     with pytest.raises(lems.LemsError):
