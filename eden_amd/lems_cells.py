@@ -168,6 +168,7 @@ _PYNN_IAF = ([*_PYNN[0], ("tau_refrac", "none"), ("v_thresh", "none"), ("tau_m",
               ("v_reset", "none")], _PYNN[1], _PYNN[2])
 BASE_CELLS = {
     "baseCell": ([], [], {"t": "time"}),
+    "baseSpikeSource": ([], [], {"t": "time"}),
     "baseSpikingCell": ([], [], {"t": "time"}),
     "baseCellMembPot": ([], [], {"t": "time"}),
     "baseCellMembPotDL": ([], [], {"t": "time"}),
@@ -301,6 +302,11 @@ CELL_TYPES = {
         on_start=[("v", "x0 * v_scaling"), ("y", "y0"), ("z", "z0")],
         on_cond=[("v .gt. 0 .and. spiking .lt. 0.5", [("spiking", "1")], True),
                  ("v .lt. 0", [("spiking", "0")], False)]),
+    "spikeGenerator": _cell(
+        "spikeGenerator", "baseSpikeSource", params=[("period", "time")], consts=[("SMALL_TIME", "time", 1e-6)],
+        state=[("tsince", "time"), ("tnext", "time")], ders={"tsince": "1", "tnext": "0"},
+        on_start=[("tsince", "0"), ("tnext", "period")],
+        on_cond=[("tnext-t .lt. SMALL_TIME", [("tsince", "0"), ("tnext", "tnext+period")], True)]),
     "IF_curr_alpha": _pynn_iaf("IF_curr_alpha", "basePyNNIaFCell"),
     "IF_curr_exp": _pynn_iaf("IF_curr_exp", "basePyNNIaFCell"),
     "IF_cond_alpha": _pynn_iaf("IF_cond_alpha", "basePyNNIaFCondCell"),
@@ -448,6 +454,46 @@ def cell_type_from_lems(name, component_types, convert):
                     derived_params=dparams, v_state=v_state, spikes=True)
 
 
+@dataclass
+class SpikeList:
+    """A spikeArray: spikes at listed times (the reference runs it natively as a spike list,
+    eden/Eden.cpp:6737-6790). Parameters: the times (us) in listed order, then +inf; state:
+    the position of the next spike."""
+    name: str = "spikeArray"
+    params: list = field(default_factory=list)
+    state: list = field(default_factory=lambda: [("pos", "none")])
+    v_state: str = None
+
+    @property
+    def v_index(self):
+        return -1
+
+
+def emit_spike_list(fn_name, dialect):
+    """Step of a spike list: on initialisation the position is 0; otherwise every listed time
+    before the end of the step (t + dt) emits the spike and advances the position (the +inf
+    after the last time stops the scan)."""
+    q = "__device__ __forceinline__ " if dialect == "hip" else "static "
+    return "\n".join([
+        f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float t, float dt, int init,"
+        " float isyn, int *spike) {",
+        "    /* spikeArray */",
+        "    (void)stride; (void)isyn;",
+        "    int pos = (int)S[0];",
+        "    int ev_spike = 0;",
+        "    if (!init) {",
+        "        while (P[pos] < t + dt) {",
+        "            ev_spike = 1;",
+        "            pos++;",
+        "        }",
+        "    } else {",
+        "        pos = 0;",
+        "    }",
+        "    Sn[0] = (float)pos;",
+        "    *spike = ev_spike;",
+        "}"])
+
+
 # ---------------------------------------------------------------- emitter
 
 def emit_cell(ct, fn_name, dialect):
@@ -553,17 +599,24 @@ def build(types, custom=None):
         if tname not in CELL_TYPES and tname not in custom:
             raise LemsError(f"cell type {tname} is not supported")
         ct = custom[tname] if tname in custom else CELL_TYPES[tname]
-        missing = [p for p, _ in ct.params if p not in vals]
-        if missing:
-            raise LemsError(f"{tname}: parameters {missing} missing")
-        params.append(np.array([vals[p] for p, _ in ct.params], np.float32))
+        if isinstance(ct, SpikeList):
+            params.append(np.array(list(vals["times"]) + [np.inf], np.float32))
+        else:
+            missing = [p for p, _ in ct.params if p not in vals]
+            if missing:
+                raise LemsError(f"{tname}: parameters {missing} missing")
+            params.append(np.array([vals[p] for p, _ in ct.params], np.float32))
         nst.append(len(ct.state))
         vst.append(ct.v_index)
         names.append(tname)
         snames.append([s for s, _ in ct.state])
         spow.append([native(dim(d)) for _, d in ct.state])
-        hip.append(emit_cell(ct, f"lems_cell_{k}", "hip"))
-        c.append(emit_cell(ct, f"lems_cell_{k}", "c"))
+        if isinstance(ct, SpikeList):
+            hip.append(emit_spike_list(f"lems_cell_{k}", "hip"))
+            c.append(emit_spike_list(f"lems_cell_{k}", "c"))
+        else:
+            hip.append(emit_cell(ct, f"lems_cell_{k}", "hip"))
+            c.append(emit_cell(ct, f"lems_cell_{k}", "c"))
 
     def dispatcher(q):
         L = [f"{q}void lems_cell(int type, const float *P, const float *S, float *Sn, long stride, float t, float dt,"

@@ -154,8 +154,8 @@ _GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction
 _UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
 _UNSUPPORTED_INPUTS = {"pulseGeneratorDL", "sineGenerator", "sineGeneratorDL", "rampGenerator", "rampGeneratorDL",
                        "voltageClamp", "voltageClampTriple", "transientPoissonFiringSynapse"}
-_UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell", "SpikeSourcePoisson", "spikeGenerator", "spikeGeneratorRandom",
-                      "spikeGeneratorPoisson", "spikeArray", "timedSynapticInput"}
+_UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell", "SpikeSourcePoisson", "spikeGeneratorRandom", "spikeGeneratorPoisson",
+                      "spikeGeneratorRefPoisson", "timedSynapticInput"}
 
 
 @dataclass
@@ -252,7 +252,7 @@ class _Doc:
                 self._add("synapse", el)
             elif t in _INPUTS:
                 self._add("input", el)
-            elif t in cells_mod.CELL_TYPES or cells_mod.base_cell_of(t, self.component_types):
+            elif t in cells_mod.CELL_TYPES or t == "spikeArray" or cells_mod.base_cell_of(t, self.component_types):
                 self._add("acell", el)
             elif t == "network":
                 self.networks[el.get("id")] = el
@@ -634,6 +634,13 @@ def _point_cell(doc, el):
     """A LEMS point neuron (artificial cell) element as a one-compartment cell: its type, its
     parameters in engine units (eden/NeuroML.cpp:8688-8745 for the base types' parameters)."""
     cid, t = el.get("id"), _tag(el)
+    if t == "spikeArray":
+        times = [_f32(convert(sp.get("time"), "time", f"spikeArray {cid} spike")) for sp in el if _tag(sp) == "spike"]
+        c = _Cell(cid, [0], np.array([-1], np.int32), [[(0, 1.0)]], [[]], np.zeros(1, np.float32),
+                  np.full(1, np.nan, np.float32), "", point=(t, {"times": times}, cells_mod.SpikeList()))
+        c.area, c.cap = np.ones(1, np.float32), np.ones(1, np.float32)
+        c.r_axial, c.inv_rc = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        return c
     try:
         ct = cells_mod.CELL_TYPES.get(t) or cells_mod.cell_type_from_lems(
             t, doc.component_types, lambda text, d: _lems_value(text, d, f"ComponentType {t} constant"))

@@ -1,9 +1,9 @@
 """The reference's EdenTest_ArtificialCells model (tests/golden/neuroml, input data): one cell
 of each NeuroML core point-neuron type (integrate-and-fire, Izhikevich, adaptive exponential,
 FitzHugh-Nagumo, Hindmarsh-Rose, the PyNN cells, a user-defined quadratic integrate-and-fire
-ComponentType) beside passive physical cells, most driven by a current pulse. Removed: the
-spikeArray source population and the dimensionless pulse (pulseGeneratorDL) into the
-izhikevichCell, inputs this build does not run. A seed is added to the simulation."""
+ComponentType) beside passive physical cells, most driven by a current pulse, and a
+spikeArray source population. Removed: the dimensionless pulse (pulseGeneratorDL) into the
+izhikevichCell, an input this build does not run. A seed is added to the simulation."""
 import os
 import xml.etree.ElementTree as ET
 
@@ -18,8 +18,7 @@ def write_model(out_dir, length_ms=None):
     net = t.getroot().find(NS + "network")
     for el in list(net):
         tag = el.tag.replace(NS, "")
-        if (tag == "population" and el.get("component") == "spkArr") or \
-                (tag == "explicitInput" and el.get("input") == "pulseDL_1"):
+        if tag == "explicitInput" and el.get("input") == "pulseDL_1":
             net.remove(el)
     t.write(os.path.join(out_dir, "EdenTest_ArtificialCells.nml"))
     text = open(os.path.join(GOLD, "LEMS_EdenTest_ArtificialCells.xml")).read()

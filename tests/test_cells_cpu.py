@@ -11,6 +11,7 @@
   from rest is the hand-computed Euler step of its equations in engine units.
 * The reference's ArtificialCells model reads every core point-neuron type with its
   parameters in engine units and runs in the oracle.
+* Spike sources: a spikeArray and a spikeGenerator fire on the rows their times fix.
 """
 import math
 import os
@@ -118,9 +119,11 @@ def test_reference_artificial_cells_model_reads(tmp_path):
     from artcells import write_model
     m, outs = neuroml.read(write_model(str(tmp_path)))
     names = m.lems_cells.type_names
-    assert names[:6] == ["iafTau", "iafTauRef", "iaf", "iafRef", "izBurst", "iz2007RS"]
+    assert names[:7] == ["spkArr", "iafTau", "iafTauRef", "iaf", "iafRef", "izBurst", "iz2007RS"]
     assert "iafTauQuad" in names and "Test_HH_cond_exp" in names
-    assert list(m.cell_lems_type[:3]) == [-1, -1, -1]            # the passive targets
+    # three spike arrays, the three passive targets, then one cell of each point-neuron type
+    assert list(m.cell_lems_type[:7]) == [0, 0, 0, -1, -1, -1, 1]
+    assert list(m.lems_cells.params[0]) == [10000.0, 20000.0, 35000.0, 50000.0, 70000.0, np.inf]
     iz = m.lems_cells.params[names.index("iz2007RS")]
     ct = lems_cells.CELL_TYPES["izhikevich2007Cell"]
     p = dict(zip([x for x, _ in ct.params], iz.tolist()))
@@ -130,3 +133,23 @@ def test_reference_artificial_cells_model_reads(tmp_path):
     counts = np.bincount(ec, minlength=len(m.event_comp))
     assert counts.min() >= 2          # every recorded cell fires
     assert math.isfinite(float(np.abs(tr).max()))
+
+
+def test_spike_sources_fire_at_their_times(tmp_path):
+    """A spikeArray fires on the step whose end passes each listed time (t + dt > time); a
+    spikeGenerator on the step whose start reaches its next planned time (within 1e-9 ms),
+    planning the next one a period later: with dt = 25 us, time T is logged at row T/dt + 1."""
+    p = tmp_path / "sources.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml>'
+        '<spikeArray id="sa"><spike id="0" time="10 ms"/><spike id="1" time="20 ms"/><spike id="2" time="35 ms"/>'
+        '<spike id="3" time="50.01 ms"/></spikeArray><spikeGenerator id="sg" period="15 ms"/>'
+        '<network id="n"><population id="a" component="sa" size="1"/><population id="g" component="sg" size="2"/>'
+        '</network></neuroml><Simulation id="s" length="60ms" step="0.025ms" target="n" seed="1">'
+        '<EventOutputFile id="e" fileName="e.txt" format="TIME_ID">'
+        '<EventSelection id="0" select="a[0]" eventPort="spike"/><EventSelection id="1" select="g[1]" eventPort="spike"/>'
+        '</EventOutputFile></Simulation></Lems>')
+    m, _ = neuroml.read(str(p))
+    _, ec, er, _ = oracle.run(m)
+    assert list(er[ec == 0]) == [401, 801, 1401, 2001]          # 50.01 ms: the step from 50.000 to 50.025
+    assert list(er[ec == 1]) == [601, 1201, 1801]

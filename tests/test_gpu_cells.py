@@ -6,8 +6,9 @@ same text: traces, spike rasters and synaptic deliveries bit for bit.
   the output files;
 * a network of point neurons and physical cells: an HH cell driving integrate-and-fire,
   Izhikevich and adaptive exponential cells through synapses, point neurons driving passive
-  cells, a gap junction between a point neuron and a physical cell, state variables recorded,
-  in the plain, graph and chunked launch modes.
+  cells, spike sources (spikeArray, spikeGenerator) driving both kinds, a gap junction
+  between a point neuron and a physical cell, state variables recorded, in the plain, graph
+  and chunked launch modes.
 """
 import os
 import shutil
@@ -80,7 +81,9 @@ def _network_file(tmp_path):
              '<izhikevich2007Cell id="iz" v0="-60mV" C="100 pF" k="0.7 nS_per_mV" vr="-60 mV" vt="-40 mV" '
              'vpeak="35 mV" a="0.03 per_ms" b="-2 nS" c="-50 mV" d="100 pA"/>'
              '<adExIaFCell id="adex" C="281pF" gL="30nS" EL="-70.6mV" reset="-48.5mV" VT="-50.4mV" thresh="-40.4mV" '
-             'refract="0.2ms" delT="2mV" tauw="40ms" a="4nS" b="0.08nA"/>')
+             'refract="0.2ms" delT="2mV" tauw="40ms" a="4nS" b="0.08nA"/>'
+             '<spikeArray id="sa"><spike id="0" time="12 ms"/><spike id="1" time="30.01 ms"/><spike id="2" time="31 ms"/>'
+             '</spikeArray><spikeGenerator id="sg" period="7ms"/>')
     syn = ('<expOneSynapse id="e1" gbase="3nS" erev="0mV" tauDecay="3ms"/>'
            '<expTwoSynapse id="e2" gbase="2nS" erev="0mV" tauRise="0.5ms" tauDecay="5ms"/>'
            '<alphaSynapse id="a1" gbase="1nS" erev="0mV" tau="2ms"/>'
@@ -88,7 +91,7 @@ def _network_file(tmp_path):
            '<pulseGenerator id="hhstim" delay="5ms" duration="200ms" amplitude="0.08nA"/>'
            '<pulseGenerator id="pstim" delay="20ms" duration="100ms" amplitude="0.15nA"/>')
     pops = "".join(f'<population id="p{n}" component="{n}" size="{k}"/>'
-                   for n, k in (("P1", 2), ("P2", 1), ("iaf", 3), ("iz", 2), ("adex", 1)))
+                   for n, k in (("P1", 2), ("P2", 1), ("iaf", 3), ("iz", 2), ("adex", 1), ("sa", 1), ("sg", 1)))
     conn = ('<projection id="{id}" presynapticPopulation="{pre}" postsynapticPopulation="{post}" synapse="{syn}">'
             '{c}</projection>')
     cw = '<connectionWD id="{k}" preCellId="../{pre}/{i}/{pc}" postCellId="../{post}/{j}/{qc}" weight="{w}" delay="{d}ms"/>'
@@ -104,6 +107,10 @@ def _network_file(tmp_path):
                                 for j in range(3)))
         + conn.format(id="c4", pre="piz", post="padex", syn="e2",
                       c=cw.format(k=0, pre="piz", i=0, pc="iz", post="padex", j=0, qc="adex", w=30, d=0.5))
+        + conn.format(id="c5", pre="psa", post="pP2", syn="e2",
+                      c=cw.format(k=0, pre="psa", i=0, pc="sa", post="pP2", j=0, qc="P2", w=10, d=1))
+        + conn.format(id="c6", pre="psg", post="piaf", syn="e1",
+                      c=cw.format(k=0, pre="psg", i=0, pc="sg", post="piaf", j=2, qc="iaf", w=5, d=0))
         + '<electricalProjection id="g1" presynapticPopulation="piz" postsynapticPopulation="pP2">'
           '<electricalConnection id="0" preCell="1" preSegment="0" postCell="0" postSegment="0" synapse="gj"/>'
           '</electricalProjection>'
@@ -116,7 +123,7 @@ def _network_file(tmp_path):
               '<OutputColumn id="w" quantity="padex[0]/w"/><OutputColumn id="ls" quantity="padex[0]/lastSpikeTime"/>'
               '<OutputColumn id="p1" quantity="pP1/0/P1/0/v"/><OutputColumn id="p2" quantity="pP2/0/P2/0/v"/>')
     evs = "".join(f'<EventSelection id="{k}" select="{s}" eventPort="spike"/>' for k, s in enumerate(
-        ["hhpop[0]", "piaf[0]", "piaf[1]", "piaf[2]", "piz[0]", "piz[1]", "padex[0]"]))
+        ["hhpop[0]", "piaf[0]", "piaf[1]", "piaf[2]", "piz[0]", "piz[1]", "padex[0]", "psa[0]", "psg[0]"]))
     p = tmp_path / "cells_net.xml"
     p.write_text(
         '<Lems><Target component="s"/><Include file="NML2_SingleCompHHCell.nml"/><neuroml>' + syn + cells +
@@ -137,7 +144,8 @@ def test_point_neuron_network_bit_exact(tmp_path, opts):
     assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))
     assert np.array_equal(ev[0], ec) and np.array_equal(ev[1], er)
     assert np.array_equal(st, ost) and np.array_equal(sy, osy)
-    counts = np.bincount(ec, minlength=7)
-    assert counts[0] > 0 and counts[1:].sum() > 5            # the HH cell and the point neurons fire
+    counts = np.bincount(ec, minlength=9)
+    assert counts[0] > 0 and counts[1:7].sum() > 5           # the HH cell and the point neurons fire
+    assert counts[7] == 3 and counts[8] == 21                 # the spike sources' listed and periodic spikes
     assert float(np.ptp(tr[:, 5])) > 0.01                      # iz's u moves
     assert float(np.abs(tr[:, 11] + 65.0).max()) > 0.05        # the gap junction moves P2
