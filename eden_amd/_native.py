@@ -41,7 +41,7 @@ _P = C.c_void_p
 
 class LemsSynapsesDesc(C.Structure):
     _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
-                ("n_state", _P), ("peer", _P)]
+                ("n_state", _P), ("peer", _P), ("type_flags", _P)]
 
 
 class LemsCellsDesc(C.Structure):

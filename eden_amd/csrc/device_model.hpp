@@ -95,6 +95,9 @@ struct DevTables {
     // generic LEMS synapses: per compartment (I_syn, G_syn) of the present step, written by
     // lems_syn_kernel before the compartment kernels of the step run (null: synapses_step)
     const float2 *syn_ext;
+    // LEMS inputs: per compartment (I, G) of the present step, summed after the native inputs
+    // (null: none)
+    const float2 *inp_ext;
 };
 
 // Arguments of lems_syn_kernel (the generic LEMS synapse kernel, compiled at run time from
@@ -108,6 +111,8 @@ struct LemsArgs {
     int64_t stride;
     float2 *syn_ext;              // [NC] per-compartment (I_syn, G_syn) of the step
     const int32_t *peer;          // per synapse slot: the gap junction's peer compartment, or -1
+    const int32_t *flags;         // per type: bit 0 an input type (null: none)
+    float2 *inp_ext;              // [NC] per-compartment (I, G) of the LEMS inputs of the step, or null
 };
 
 // Arguments of lems_cell_kernel (LEMS point neurons, compiled at run time with the synapse

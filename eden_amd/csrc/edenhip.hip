@@ -149,6 +149,8 @@ struct edenhip_ctx {
     int32_t *d_lems_pbegin = nullptr;
     int32_t *d_lems_peer = nullptr;      // per synapse slot: gap junction peer compartment, or -1
     float2 *d_syn_ext = nullptr;
+    int32_t *d_lems_flags = nullptr;     // per generic type: bit 0 an input type
+    float2 *d_inp_ext = nullptr;         // per compartment: the LEMS inputs' (I, G) of the step
     // LEMS point neurons: the run-time compiled cell kernel and its tables (one slot per cell)
     hipFunction_t lems_cell_fn = nullptr;
     int32_t n_acells = 0;
@@ -841,12 +843,22 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(upload(ctx, &ctx->d_lems_peer, peer));
         TRY(alloc_zero(ctx, &ctx->d_syn_ext, (size_t)NC));
         T.syn_ext = ctx->d_syn_ext;
+        if (d->abi_version >= 4 && LS->type_flags) {
+            std::vector<int32_t> fl(LS->type_flags, LS->type_flags + LS->n_types);
+            bool any_input = false;
+            for (int64_t k = 0; k < n_syn; k++) any_input |= (fl[(size_t)d->syn_type[k]] & 1) != 0;
+            if (any_input) {
+                TRY(upload(ctx, &ctx->d_lems_flags, fl));
+                TRY(alloc_zero(ctx, &ctx->d_inp_ext, (size_t)NC));
+                T.inp_ext = ctx->d_inp_ext;
+            }
+        }
     }
     if ((lems && n_syn > 0) || ctx->n_acells > 0) {
         // the generated synapse and point-neuron functions, each side stubbed when absent
         std::string gen = lems ? std::string(LS->source)
                                : std::string("__device__ __forceinline__ void lems_syn(int, const float *, const float *, "
-                                             "float *, long, float, float, int, int, float, float *i, float *g) "
+                                             "float *, long, float, float, float, int, int, float, float *i, float *g) "
                                              "{ *i = 0.0f; *g = 0.0f; }\n");
         gen += "\n";
         gen += LC ? std::string(LC->source)
@@ -1273,7 +1285,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             if (ctx->lems_fn && ctx->n_lems_comps > 0) {
                 // generic synapses of step a: per-compartment currents for the kernels below
                 LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,
-                     ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer};
+                     ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer, ctx->d_lems_flags, ctx->d_inp_ext};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &la, &st, &dt};

@@ -65,6 +65,11 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
             const PoiState ps{T.poi_tsince, T.poi_isi, T.poi_A, T.poi_B};
             poisson_step(T, stypes, g, V, init, s, dt, I_in, G_in, ps, ps);
         }
+        if (T.inp_ext && has_syn) {
+            const float2 e = T.inp_ext[g];   // LEMS inputs of this step, after the native ones
+            I_in += e.x;
+            G_in += e.y;
+        }
         const float I_int = I_ch + I_in + I_syn;
         const float G_int = G_ch + G_in + G_syn;
         float D = 1 * (G_int / k.C) + k.invrc;
@@ -552,6 +557,11 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
                         const float2 e = T.syn_ext[g];   // generic synapses of this step (lems_syn_kernel)
                         I_syn = e.x;
                         G_syn = e.y;
+                        if (T.inp_ext) {
+                            const float2 e2 = T.inp_ext[g];   // LEMS inputs, after the native ones
+                            I_in += e2.x;
+                            G_in += e2.y;
+                        }
                     } else {
                         synapses_step(T, stypes, T.syn_base[g], T.syn_cnt[g], V[j], init, t, dt, (int)((s + 3) & 1), s,
                                       I_syn, G_syn);

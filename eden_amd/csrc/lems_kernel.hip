@@ -27,6 +27,12 @@
 #define EXP edenhip::exp_f
 #define LOG edenhip::log_f
 #define POW edenhip::pow_f
+#define SIN(x) ((float)sin((double)(x)))
+#define COS(x) ((float)cos((double)(x)))
+#define TAN(x) ((float)tan((double)(x)))
+#define SINH(x) ((float)sinh((double)(x)))
+#define COSH(x) ((float)cosh((double)(x)))
+#define TANH(x) ((float)tanh((double)(x)))
 __device__ __forceinline__ float stepf(float x) { return x < 0.0f ? 0.0f : 1.0f; }
 
 #include "lems_synapse_functions.h"
@@ -45,11 +51,13 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
     const int32_t base = T.syn_base[g], cnt = T.syn_cnt[g];
     int32_t *trig_now = T.trig[now];
     float I_syn = 0.0f, G_syn = 0.0f, I_agg = 0.0f, G_agg = 0.0f;
+    float I_inp = 0.0f, G_inp = 0.0f;   // LEMS inputs: i * weight per instance (eden/Eden.cpp:3929)
     int32_t cur = -1;
     for (int32_t k = 0; k < cnt; k++) {
         const int32_t s = base + 64 * k;
         const int32_t ty = T.syn_type[s];
-        if (ty != cur) {
+        const bool input = L.flags && (L.flags[ty] & 1);
+        if (ty != cur && !input) {
             if (cur >= 0) { I_syn += I_agg; G_syn += G_agg; }
             I_agg = 0.0f; G_agg = 0.0f; cur = ty;
         }
@@ -80,14 +88,20 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
         float *S = L.state + s;
         const int32_t pc = L.peer[s];
         const float vpeer = pc >= 0 ? T.V[pc] : V;
-        lems_syn(ty, L.params + L.param_begin[ty], S, S, (long)L.stride, V, vpeer, init ? 1 : 0, spike_now ? 1 : 0, dt,
-                 &i, &gx);
-        I_agg += i * weight;
-        G_agg += gx * weight;
+        lems_syn(ty, L.params + L.param_begin[ty], S, S, (long)L.stride, V, vpeer, t, init ? 1 : 0, spike_now ? 1 : 0,
+                 dt, &i, &gx);
+        if (input) {
+            I_inp += i * weight;
+            G_inp += gx * weight;
+        } else {
+            I_agg += i * weight;
+            G_agg += gx * weight;
+        }
     }
     I_syn += I_agg;
     G_syn += G_agg;
     L.syn_ext[g] = make_float2(I_syn, G_syn);
+    if (L.inp_ext) L.inp_ext[g] = make_float2(I_inp, G_inp);
 }
 
 extern "C" __global__ void __launch_bounds__(64)
@@ -111,6 +125,7 @@ lems_cell_kernel(edenhip::DevTables T, edenhip::LemsCellArgs A, int64_t step, fl
         I_in += Ip;
         I_in += 0.0f;
     }
+    if (T.inp_ext && T.syn_cnt[g] > 0) I_in += T.inp_ext[g].x;   // LEMS inputs, after the native ones
     const float ext = I_syn + I_in;
     float *S = A.state + idx;
     int spike = 0;

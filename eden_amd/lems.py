@@ -24,8 +24,9 @@ Semantics of one step of one instance (eden/Eden.cpp:2700-2995):
   - exposures i (current) and g (conductance; 0 for current-based synapses) last, a state
     variable's as assigned.
 Expressions are printed fully parenthesised, left to right as parsed, with every literal
-rounded to float, `expf`/`logf`/`powf` as EXP/LOG/POW (the dialect's own: on the device and
-in the oracle, exp/log/pow rounded once from double), H as stepf (0 below zero, else 1, as
+rounded to float, `expf`/`logf`/`powf` and the trigonometric and hyperbolic functions as
+EXP/LOG/POW/SIN/... (the dialect's own: on the device and in the oracle, each rounded once
+from its double evaluation), H as stepf (0 below zero, else 1, as
 the reference defines it, eden/Eden.cpp:3197), so both compilers perform the same float
 operations in the same order. Engine units (us, mV, nA, uS) make every unit conversion
 factor of these types 1 (the reference's Convert::Suffix then emits nothing).
@@ -67,6 +68,7 @@ class SynapseType:
     derived_params: list = field(default_factory=list)
     constants: list = field(default_factory=list)
     on_conditions: list = field(default_factory=list)
+    is_input: bool = False     # an input (its current joins the compartment's input total)
 
 
 _COND = [("gbase", "conductance"), ("erev", "voltage")]
@@ -150,6 +152,63 @@ SYNAPSE_TYPES = {
         on_conditions=[("(1-inf) .lt. 1e-4", [("s", "inf")])], expose_i="i"),
 }
 
+# LEMS current inputs (LEMS_CoreComponents.inc.xml:402-622; the pulseGenerator and
+# poissonFiringSynapse are native): their current, or dimensionless current I, joins the
+# compartment's input total as i * weight per instance (eden/Eden.cpp:3914-3931)
+_WINDOW = [("t .lt. delay", "0"), ("t .geq. delay .and. t .lt. duration+delay", None), ("t .geq. duration+delay", "0")]
+
+
+def _windowed(var, active, params):
+    conds = [(test, [(var, active if val is None else val)]) for test, val in _WINDOW]
+    return SynapseType(params=params, state=[var], derived=[], derivatives={}, on_start={}, on_event={},
+                       on_conditions=conds, expose_i=var, is_input=True)
+
+
+def _ramp(var, dim):
+    base, ramp = "weight * baselineAmplitude", \
+        "weight * (startAmplitude + (finishAmplitude - startAmplitude) * (t - delay) / (duration))"
+    return SynapseType(
+        params=[("delay", "time"), ("duration", "time"), ("startAmplitude", dim), ("finishAmplitude", dim),
+                ("baselineAmplitude", dim)],
+        state=[var], derived=[], derivatives={}, on_start={var: "baselineAmplitude"}, on_event={},
+        on_conditions=[("t .lt. delay", [(var, base)]), ("t .geq. delay .and. t .lt. duration+delay", [(var, ramp)]),
+                       ("t .geq. duration+delay", [(var, base)])],
+        expose_i=var, is_input=True)
+
+
+_SINE = "weight * amplitude * sin(phase + (2 * 3.14159265 * (t-delay)/period) )"
+_CLAMP = [("delay", "time"), ("duration", "time"), ("simpleSeriesResistance", "resistance")]
+INPUT_TYPES = {
+    "pulseGeneratorDL": _windowed("I", "weight * amplitude", [("delay", "time"), ("duration", "time"),
+                                                             ("amplitude", None)]),
+    "sineGenerator": _windowed("i", _SINE, [("phase", None), ("delay", "time"), ("duration", "time"),
+                                            ("amplitude", "current"), ("period", "time")]),
+    "sineGeneratorDL": _windowed("I", _SINE, [("phase", None), ("delay", "time"), ("duration", "time"),
+                                              ("amplitude", None), ("period", "time")]),
+    "rampGenerator": _ramp("i", "current"),
+    "rampGeneratorDL": _ramp("I", None),
+    "voltageClamp": SynapseType(
+        params=_CLAMP + [("targetVoltage", "voltage")], state=["i"], derived=[], derivatives={}, on_start={},
+        on_event={}, on_conditions=[("t .lt. delay", [("i", "0")]),
+                                    ("t .geq. delay", [("i", "weight * (targetVoltage - v) / simpleSeriesResistance")]),
+                                    ("t .gt. duration + delay", [("i", "0")])],
+        expose_i="i", is_input=True),
+    "voltageClampTriple": SynapseType(
+        params=_CLAMP + [("active", None), ("conditioningVoltage", "voltage"), ("testingVoltage", "voltage"),
+                         ("returnVoltage", "voltage")],
+        state=["i"], derived=[], derivatives={}, on_start={}, on_event={},
+        on_conditions=[("active .eq. 1 .and. t .lt. delay",
+                        [("i", "weight * (conditioningVoltage - v) / simpleSeriesResistance")]),
+                       ("active .eq. 1 .and. t .geq. delay",
+                        [("i", "weight * (testingVoltage - v) / simpleSeriesResistance")]),
+                       ("active .eq. 1 .and. t .gt. duration + delay",
+                        [("i", "weight * (returnVoltage - v) / simpleSeriesResistance")])],
+        expose_i="i", is_input=True),
+}
+# the NeuroML base input types a user input ComponentType may extend
+BASE_INPUTS = {"basePointCurrent": [], "basePointCurrentDL": [], "baseVoltageDepPointCurrent": [],
+               "baseVoltageDepPointCurrentDL": []}
+
 # blockingPlasticSynapse's child mechanisms (LEMS_CoreComponents.inc.xml:231-312): parameters,
 # state, derived variables (exposing blockFactor / plasticityFactor), derivatives, OnStart and
 # OnEvent (port in: the synapse's own event)
@@ -217,13 +276,22 @@ def _ltag(el):
     return t.rsplit("}", 1)[-1] if isinstance(t, str) else ""
 
 
-def is_synapse_component_type(name, component_types):
-    """Whether a ComponentType (by name) extends, through any chain, a NeuroML base synapse."""
+def _base_of(name, component_types):
     seen = set()
     while name in component_types and name not in seen:
         seen.add(name)
         name = component_types[name].get("extends", "")
-    return name in BASE_SYNAPSES
+    return name
+
+
+def is_synapse_component_type(name, component_types):
+    """Whether a ComponentType (by name) extends, through any chain, a NeuroML base synapse."""
+    return _base_of(name, component_types) in BASE_SYNAPSES
+
+
+def is_input_component_type(name, component_types):
+    """Whether a ComponentType (by name) extends, through any chain, a NeuroML base current input."""
+    return _base_of(name, component_types) in BASE_INPUTS
 
 
 def synapse_type_from_lems(name, component_types, convert):
@@ -238,9 +306,10 @@ def synapse_type_from_lems(name, component_types, convert):
     while n in component_types:
         chain.append(component_types[n])
         n = component_types[n].get("extends", "")
-    if n not in BASE_SYNAPSES:
-        raise LemsError(f"ComponentType {name} does not extend a base synapse type")
-    params = list(BASE_SYNAPSES[n])
+    if n not in BASE_SYNAPSES and n not in BASE_INPUTS:
+        raise LemsError(f"ComponentType {name} does not extend a base synapse or input type")
+    is_input = n in BASE_INPUTS
+    params = list(BASE_SYNAPSES.get(n, BASE_INPUTS.get(n, [])))
     dparams, consts, weight, dyn = [], [], 1.0, None
     for ct in reversed(chain):                     # base first, so derived types override
         for el in ct:
@@ -304,6 +373,8 @@ def synapse_type_from_lems(name, component_types, convert):
                     raise LemsError(f"{name}: <{_ltag(a)}> in <{t}> is not supported")
         elif t in ("Regime", "KineticScheme"):
             raise LemsError(f"{name}: <{t}> is not supported")
+    if is_input and "i" not in expose and "I" in expose:
+        expose["i"] = expose["I"]          # a dimensionless input's current
     if "i" not in expose:
         raise LemsError(f"ComponentType {name} exposes no current i")
 
@@ -335,7 +406,7 @@ def synapse_type_from_lems(name, component_types, convert):
         visit(v)
     st = SynapseType(params=params, state=state, derived=order, derivatives=ders, on_start=on_start,
                      on_event=on_event, expose_i=expose["i"], expose_g=expose.get("g"), derived_params=dparams,
-                     constants=consts, on_conditions=on_cond)
+                     constants=consts, on_conditions=on_cond, is_input=is_input)
     return st, weight
 
 
@@ -346,8 +417,8 @@ _TOKEN = re.compile(r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|
 # LEMS relational operators and their C forms
 _REL = {".gt.": ">", ".lt.": "<", ".geq.": ">=", ".ge.": ">=", ".leq.": "<=", ".le.": "<=", ".eq.": "==",
         ".neq.": "!=", ".ne.": "!="}
-_FUNCS = {"exp": "EXP", "log": "LOG", "ln": "LOG", "sqrt": "sqrtf", "abs": "fabsf", "sin": "sinf", "cos": "cosf",
-          "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "ceil": "ceilf", "floor": "floorf",
+_FUNCS = {"exp": "EXP", "log": "LOG", "ln": "LOG", "sqrt": "sqrtf", "abs": "fabsf", "sin": "SIN", "cos": "COS",
+          "tan": "TAN", "sinh": "SINH", "cosh": "COSH", "tanh": "TANH", "ceil": "ceilf", "floor": "floorf",
           "H": "stepf"}
 
 
@@ -564,18 +635,19 @@ def emit(name, st, fn_name, dialect):
     """C / HIP text of one type's step function:
 
         void fn(const float *P, const float *S, float *Sn, long stride, float v, float vpeer,
-                int init, int event, float dt, float *i_out, float *g_out)
+                float t, int init, int event, float dt, float *i_out, float *g_out)
 
     P: the type's parameter vector (params in table order, then the `weight` property); vpeer:
-    the voltage at the other end of a gap junction or graded connection (v for other types);
+    the voltage at the other end of a gap junction or graded connection (v for other types); t:
+    the step's time (time_f32);
     S / Sn: the instance's present / next state, variable k at k * stride (they may alias:
     every present value is read before anything is written)."""
     q = "__device__ __forceinline__ " if dialect == "hip" else "static "
-    L = [f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float v, float vpeer, int init,"
-         f" int event, float dt, float *i_out, float *g_out) {{",
+    L = [f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float v, float vpeer, float t,"
+         f" int init, int event, float dt, float *i_out, float *g_out) {{",
          f"    /* {name} */",
-         "    (void)S; (void)Sn; (void)stride; (void)vpeer; (void)init; (void)event; (void)dt;"]
-    names = {"v": "v", "vpeer": "vpeer", "dt": "dt"}
+         "    (void)S; (void)Sn; (void)stride; (void)vpeer; (void)t; (void)init; (void)event; (void)dt;"]
+    names = {"v": "v", "vpeer": "vpeer", "dt": "dt", "t": "t"}
     for k, (p, _) in enumerate(st.params):
         L.append(f"    const float p_{p} = P[{k}];")
         names[p] = f"p_{p}"
@@ -626,18 +698,20 @@ class LemsSynapses:
     n_state: list
     source_hip: str
     source_c: str
+    type_flags: list = field(default_factory=list)   # per type: bit 0 an input type
 
 
 def build(types, custom=None):
     """types: list of (component type name, {param: engine value}, weight property); custom:
     user ComponentTypes by name (SynapseType). Returns LemsSynapses whose functions are
     lems_syn_<k> and a dispatcher lems_syn(type, ...)."""
-    hip, c, params, nst, names = [], [], [], [], []
+    hip, c, params, nst, names, flags = [], [], [], [], [], []
     custom = custom or {}
     for k, (tname, vals, weight) in enumerate(types):
-        if tname not in SYNAPSE_TYPES and tname not in custom:
+        if tname not in SYNAPSE_TYPES and tname not in INPUT_TYPES and tname not in custom:
             raise LemsError(f"synapse type {tname} is not supported")
-        st = custom[tname] if tname in custom else SYNAPSE_TYPES[tname]
+        st = custom[tname] if tname in custom else SYNAPSE_TYPES.get(tname) or INPUT_TYPES[tname]
+        flags.append(1 if st.is_input else 0)
         missing = [p for p, _ in st.params if p not in vals]
         if missing:
             raise LemsError(f"{tname}: parameters {missing} missing")
@@ -649,9 +723,10 @@ def build(types, custom=None):
 
     def dispatcher(q):
         L = [f"{q}void lems_syn(int type, const float *P, const float *S, float *Sn, long stride, float v, float vpeer,"
+             " float t,"
              " int init, int event, float dt, float *i_out, float *g_out) {", "    switch (type) {"]
         for k in range(len(types)):
-            L.append(f"    case {k}: lems_syn_{k}(P, S, Sn, stride, v, vpeer, init, event, dt, i_out, g_out); break;")
+            L.append(f"    case {k}: lems_syn_{k}(P, S, Sn, stride, v, vpeer, t, init, event, dt, i_out, g_out); break;")
         L.append("    default: *i_out = 0.0f; *g_out = 0.0f; break;")
         L.append("    }")
         L.append("}")
@@ -659,4 +734,4 @@ def build(types, custom=None):
 
     src_hip = "\n\n".join(hip + [dispatcher("__device__ __forceinline__ ")]) + "\n"
     src_c = "\n\n".join(c + [dispatcher("")]) + "\n"
-    return LemsSynapses(names, params, nst, src_hip, src_c)
+    return LemsSynapses(names, params, nst, src_hip, src_c, flags)

@@ -248,9 +248,10 @@ class Model:
             if self.syn_peer is not None:
                 peer = np.ascontiguousarray(self.syn_peer, np.int64)
                 self._lems_peer = peer
+            self._lems_flags = np.array(L.type_flags or [0] * len(L.params), np.int32)
             self._lems_desc = N.LemsSynapsesDesc(len(L.params), self._lems_arrays[3], N.ptr(self._lems_arrays[0]),
                                                  N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]),
-                                                 N.ptr(peer))
+                                                 N.ptr(peer), N.ptr(self._lems_flags))
             d.lems = C.pointer(self._lems_desc)
         if self.lems_cells is not None:
             LC = self.lems_cells

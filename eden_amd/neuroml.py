@@ -38,7 +38,11 @@ set covers:
                     (eden/NeuroML.cpp:6506-6829); user-defined
                     <ComponentType>s that extend a base synapse type, from their LEMS
                     Dynamics (eden/Eden.cpp:2339-2995)
-  inputs            pulseGenerator, poissonFiringSynapse (eden/NeuroML.cpp:6968-7107)
+  inputs            pulseGenerator, poissonFiringSynapse natively (eden/NeuroML.cpp:6968-7107);
+                    pulseGeneratorDL, sine and ramp generators (and their DL forms), voltageClamp,
+                    voltageClampTriple and user <ComponentType>s extending a base current input
+                    as generic LEMS types whose instances sit after a compartment's synapses
+                    (eden/Eden.cpp:3914-3931)
   networks          population (size or instances), projection with connection /
                     connectionWD, synapticConnection, electricalProjection (gapJunction:
                     an instance on each end, each with the other end's voltage as its peer),
@@ -148,12 +152,11 @@ _CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Sy
 # element names of the synapse and input types this reader instantiates, by role; a LEMS
 # <Component type="..."> of these types is read as the element itself
 _SYNAPSES = set(lems_mod.SYNAPSE_TYPES) | {"blockingPlasticSynapse"}
-_INPUTS = {"pulseGenerator", "poissonFiringSynapse"}
+_INPUTS = {"pulseGenerator", "poissonFiringSynapse"} | set(lems_mod.INPUT_TYPES)
 # synapse types that may sit on either end of a continuous connection (no spike input)
 _GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction"}
 _UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
-_UNSUPPORTED_INPUTS = {"pulseGeneratorDL", "sineGenerator", "sineGeneratorDL", "rampGenerator", "rampGeneratorDL",
-                       "voltageClamp", "voltageClampTriple", "transientPoissonFiringSynapse"}
+_UNSUPPORTED_INPUTS = {"transientPoissonFiringSynapse"}
 _UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell", "SpikeSourcePoisson", "spikeGeneratorRandom", "spikeGeneratorPoisson",
                       "spikeGeneratorRefPoisson", "timedSynapticInput"}
 
@@ -250,7 +253,7 @@ class _Doc:
                 self.component_types[el.get("name")] = el
             elif t in _SYNAPSES or lems_mod.is_synapse_component_type(t, self.component_types):
                 self._add("synapse", el)
-            elif t in _INPUTS:
+            elif t in _INPUTS or lems_mod.is_input_component_type(t, self.component_types):
                 self._add("input", el)
             elif t in cells_mod.CELL_TYPES or t == "spikeArray" or cells_mod.base_cell_of(t, self.component_types):
                 self._add("acell", el)
@@ -971,10 +974,15 @@ def read(path, rng_seed=None):
             used |= {c.get("synapse") for c in el if c.get("synapse")}
         elif _tag(el) == "continuousProjection":
             used |= {c.get(k) for c in el for k in ("preComponent", "postComponent") if c.get(k)}
-    # (point neurons take their synapses through the generic path)
+    # (point neurons take their synapses through the generic path, and so do LEMS inputs)
     points = any(p.cell.point is not None for p in pops.values())
-    generic = any(doc.items.get(u, (None, None))[0] == "synapse" and
-                  (_tag(doc.items[u][1]) != "expTwoSynapse" or points) for u in used)
+    input_elems = [(i, doc.items[i][1]) for role, i in doc.order
+                   if role == "input" and _tag(doc.items[i][1]) not in ("pulseGenerator", "poissonFiringSynapse")]
+    used_inputs = {el.get("input") for el in net if _tag(el) == "explicitInput"} | \
+                  {el.get("component") for el in net if _tag(el) == "inputList"}
+    lems_inputs = any(i in used_inputs for i, _ in input_elems)
+    generic = lems_inputs or any(doc.items.get(u, (None, None))[0] == "synapse" and
+                                 (_tag(doc.items[u][1]) != "expTwoSynapse" or points) for u in used)
     generic_index, generic_types, custom = {}, [], {}
     if generic:
         for i, el in syn_elems:
@@ -1013,6 +1021,22 @@ def read(path, rng_seed=None):
                 st, weight = custom[t]
             vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
             vals.update(extra)
+            generic_index[i] = len(generic_types)
+            generic_types.append((t, vals, weight))
+        # LEMS current inputs after the synapse types (their instances come last in a
+        # compartment's list)
+        for i, el in input_elems:
+            t = _tag(el)
+            if t in lems_mod.INPUT_TYPES:
+                st, weight = lems_mod.INPUT_TYPES[t], 1.0
+            else:
+                if t not in custom:
+                    try:
+                        custom[t] = lems_mod.synapse_type_from_lems(t, doc.component_types, convert)
+                    except lems_mod.LemsError as e:
+                        raise NeuroMLError(str(e)) from None
+                st, weight = custom[t]
+            vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
             generic_index[i] = len(generic_types)
             generic_types.append((t, vals, weight))
 
@@ -1157,9 +1181,13 @@ def read(path, rng_seed=None):
     for (iid, g, w) in inputs:
         role, el = doc.items.get(iid, (None, None))
         if role != "input":
-            raise NeuroMLError(f"input '{iid}' not found or not supported (pulseGenerator, poissonFiringSynapse)")
+            raise NeuroMLError(f"input '{iid}' not found or not supported (pulseGenerator, poissonFiringSynapse, "
+                               f"{', '.join(sorted(lems_mod.INPUT_TYPES))}, user input ComponentTypes)")
         t = _tag(el)
-        if t == "pulseGenerator":
+        if iid in generic_index:
+            # a LEMS input: an instance in the compartment's synapse list
+            conns.append((-1, g, generic_index[iid], _f32(w), 0.0, -1))
+        elif t == "pulseGenerator":
             pulses[g].append((_f32(convert(el.get("delay"), "time", "delay")),
                               _f32(convert(el.get("duration"), "time", "duration")),
                               _f32(convert(el.get("amplitude"), "current", "amplitude")), _f32(w)))

@@ -87,18 +87,22 @@ typedef struct edenhip_exptwo_type {
 } edenhip_exptwo_type;
 
 /* Generic LEMS synapse types (alphaSynapse, expThreeSynapse, the PyNN synapses, gap
- * junctions, ...). The reference emits C for each lemsified ComponentType and compiles it
- * with gcc (eden/Eden.cpp:2339-2995, 6992-7216); here the caller supplies HIP source defining
+ * junctions, ...) and LEMS current inputs (sine and ramp generators, voltage clamps, ...). The
+ * reference emits C for each lemsified ComponentType and compiles it with gcc
+ * (eden/Eden.cpp:2339-2995, 6992-7216); here the caller supplies HIP source defining
  *   __device__ void lems_syn(int type, const float *P, const float *S, float *Sn, long stride,
- *                            float v, float vpeer, int init, int event, float dt, float *i,
- *                            float *g)
+ *                            float v, float vpeer, float t, int init, int event, float dt,
+ *                            float *i, float *g)
  * (one instance's step: P its type's parameters, S / Sn its present / next state with
  * variable k at k * stride, v the compartment voltage, vpeer the voltage at the start of the
- * step of the instance's peer compartment (gap junctions; v when it has none), event whether
- * it receives its event this step; i, g its current and conductance exposures), which
- * edenhip_create() compiles with hiprtc into the library's synapse kernel template
- * (eden_amd.lems emits it). When present, every synapse instance's syn_type indexes these
- * types, and syn_types serves only the Poisson inputs. */
+ * step of the instance's peer compartment (gap junctions; v when it has none), t the step's
+ * time, event whether it receives its event this step; i, g its current and conductance
+ * exposures), which edenhip_create() compiles with hiprtc into the library's synapse kernel
+ * template (eden_amd.lems emits it). When present, every synapse instance's syn_type indexes
+ * these types, and syn_types serves only the Poisson inputs. An instance of an input type
+ * (type_flags bit 0) sits in its compartment's synapse list after the synapse types (it never
+ * receives spikes); its current times its weight joins the compartment's input total, after
+ * the native pulse and Poisson inputs (eden/Eden.cpp:3914-3931). */
 typedef struct edenhip_lems_synapses {
     int32_t n_types;
     const char *source;           /* HIP C++ source, NUL-terminated */
@@ -108,6 +112,7 @@ typedef struct edenhip_lems_synapses {
     const int64_t *peer;          /* (abi_version >= 3) [n_syn] the global compartment whose voltage
                                      is the instance's vpeer (the other end of a gap junction,
                                      eden/Eden.cpp:9225-9248), or -1; NULL = none */
+    const int32_t *type_flags;    /* (abi_version >= 4) [n_types] bit 0: an input type; NULL = none */
 } edenhip_lems_synapses;
 
 /* LEMS point neurons (the reference's artificial cells: iafCell, izhikevich2007Cell,

@@ -170,7 +170,7 @@ static void log_poi_event(int64_t p, int64_t step) {
  * C dialect of the source the model carries (eden_amd.lems emits both dialects from one
  * expression tree), compiled by the test harness and handed over here. fp32 flavours only. */
 typedef void (*lems_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float v, float vpeer,
-                          int init, int event, float dt, float *i_out, float *g_out);
+                          float t, int init, int event, float dt, float *i, float *g);
 static lems_fn_t g_lems = NULL;
 /* LEMS point neurons (the C dialect of the model's lems_cell source, compiled by the caller) */
 typedef void (*lems_cell_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float t, float dt,
@@ -327,12 +327,17 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
         }
         /* synapses: per type (ascending type order in the list), aggregated per type */
         real I_synapses_total = 0, G_synapses_total = 0;
+        /* LEMS inputs in the synapse list: i * weight per instance, joined to the input total
+           after the native inputs (eden/Eden.cpp:3914-3931) */
+        const int32_t *tflags = (d->abi_version >= 4 && d->lems && d->lems->type_flags) ? d->lems->type_flags : NULL;
+        real I_lems_input = 0, G_lems_input = 0;
         {
             real I_agg = 0, G_agg = 0;
             int32_t cur_type = -1;
             for (int64_t sy = d->comp_syn_begin[g]; sy < d->comp_syn_begin[g + 1]; sy++) {
                 int32_t ty = d->syn_type[sy];
-                if (ty != cur_type) {
+                const int input = tflags && (tflags[ty] & 1);
+                if (ty != cur_type && !input) {
                     if (cur_type >= 0) { I_synapses_total += I_agg; G_synapses_total += G_agg; }
                     I_agg = 0; G_agg = 0; cur_type = ty;
                 }
@@ -367,9 +372,14 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                     const int64_t pc = (d->abi_version >= 3 && L->peer) ? L->peer[sy] : -1;
                     const real Vpeer = pc >= 0 ? V[pc] : Vcomp;
                     g_lems(ty, L->params + L->param_begin[ty], s->lS[now] + sy, s->lS[nxt] + sy, (long)NSYN,
-                           (float)Vcomp, (float)Vpeer, initial_state, spike_in_flag, (float)dt, &iv, &gv);
-                    I_agg += (real)iv * weight;
-                    G_agg += (real)gv * weight;
+                           (float)Vcomp, (float)Vpeer, (float)time_r, initial_state, spike_in_flag, (float)dt, &iv, &gv);
+                    if (input) {
+                        I_lems_input += (real)iv * weight;
+                        G_lems_input += (real)gv * weight;
+                    } else {
+                        I_agg += (real)iv * weight;
+                        G_agg += (real)gv * weight;
+                    }
                     continue;
                 }
                 /* expTwoSynapse */
@@ -469,6 +479,10 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
             }
             I_input_total += I_agg;
             G_input_total += G_agg;
+        }
+        if (tflags && d->comp_syn_begin[g + 1] > d->comp_syn_begin[g]) {
+            I_input_total += I_lems_input;
+            G_input_total += G_lems_input;
         }
         if (s->cslot && s->cslot[g] >= 0) {
             /* a LEMS point neuron (eden/Eden.cpp:6855-6900): its external current is the synapse

@@ -113,9 +113,13 @@ _LEMS_PRELUDE = {
     "exact": "#include <math.h>\nstatic inline float EXP(float x) { return (float)exp((double)x); }\n"
              "static inline float LOG(float x) { return (float)log((double)x); }\n"
              "static inline float POW(float x, float y) { return (float)pow((double)x, (double)y); }\n"
-             "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
+             + "".join(f"static inline float {f.upper()}(float x) {{ return (float){f}((double)x); }}\n"
+                       for f in ("sin", "cos", "tan", "sinh", "cosh", "tanh"))
+             + "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
     # glibc expf/logf/powf (the f32libm and f32ofast flavours)
     "libm": "#include <math.h>\n#define EXP expf\n#define LOG logf\n#define POW powf\n"
+            "#define SIN sinf\n#define COS cosf\n#define TAN tanf\n#define SINH sinhf\n#define COSH coshf\n"
+            "#define TANH tanhf\n"
             "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
 }
 _LEMS_FLAGS = {

@@ -9,9 +9,12 @@ of the same text: traces, spike raster and every synaptic delivery bit for bit.
 * a network of tree and point cells with generic synapses on dendrites and somata;
 * gap junctions (a pair of membranes; the reference's DomainDecomposition model, whose
   electrical projections join compartments of different tree cells);
-* user-defined synapse ComponentTypes read from LEMS.
+* user-defined synapse ComponentTypes read from LEMS;
+* LEMS current inputs (the reference's CoreInputs and EdenTest4 models; generators and a
+  voltage clamp beside native inputs on point and tree compartments).
 """
 import dataclasses
+import os
 import sys
 
 import numpy as np
@@ -24,6 +27,7 @@ from eden_text import assert_same_text, trajectory_text
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def _say(*a):
@@ -121,3 +125,69 @@ def test_user_defined_component_types(tmp_path):
     tr, er, ost = _check(m)
     assert len(ost) > 2
     assert np.array_equal(tr[:, 1], tr[:, 2]) and np.array_equal(tr[:, 3:5], tr[:, 5:7])
+
+
+def test_reference_core_inputs_files(tmp_path):
+    """The reference's CoreInputs model (tests/coreinputs.py): LEMS current inputs (sine, ramp,
+    voltage clamps, dimensionless generators on Izhikevich cells) and spike sources; output
+    files byte-equal to the oracle's, trace and raster equal."""
+    from coreinputs import write_model
+    path = write_model(str(tmp_path))
+    res = run_lems(path, out_dir=str(tmp_path))
+    m, outs = res["model"], res["outputs"]
+    _say(f"[lems] CoreInputs: {m.n_cells} cells, types {m.lems.type_names}, {res['rows']} rows")
+    tr_r, ec, er, _ = oracle.run(m)
+    t_us = output.row_times_us(m.dt, m.total_rows() + 1)
+    f = outs.trajectories[0]
+    assert_same_text((tmp_path / f.file_name).read_text(), trajectory_text(t_us, tr_r[:, [c[2] for c in f.columns]]))
+    tr, ev, _, _ = _gpu(m)
+    assert np.array_equal(tr, tr_r)
+    assert np.array_equal(ev[0], ec) and np.array_equal(ev[1], er)
+
+
+def test_reference_custom_input_model_on_tree_cells():
+    """The reference's EdenTest4 (two-compartment cells with a user-defined pulse input, user
+    gap junction and synapse types beside the core ones): the GPU equals the oracle."""
+    m, _ = neuroml.read(os.path.join(HERE, "golden", "neuroml", "LEMS_EdenTest4.xml"))
+    tr, er, ost = _check(m)
+    assert float(np.ptp(tr[:, 4])) > 0.5
+
+
+@pytest.mark.parametrize("where", ["point", "tree"])
+def test_lems_inputs_beside_native_inputs(tmp_path, where):
+    """A sine and a ramp generator and a voltage clamp beside native pulse and Poisson inputs,
+    on one-compartment cells and on the compartments of tree cells: bit for bit."""
+    m = configs.mixed_network(tree_sizes=(40, 90), n_point=6, n_conn=40, t_final_ms=40.0, seed=21)
+    types = [("expTwoSynapse", dict(gbase=st.gbase, erev=st.erev, tauRise=st.tau_rise, tauDecay=st.tau_decay),
+              st.weight_prop) for st in m.syn_types]
+    types += [("sineGenerator", dict(phase=0.1, delay=5000.0, duration=20000.0, amplitude=0.05, period=3000.0), 1.0),
+              ("rampGenerator", dict(delay=2000.0, duration=30000.0, startAmplitude=0.0, finishAmplitude=0.08,
+                                     baselineAmplitude=0.01), 1.0),
+              ("voltageClamp", dict(delay=10000.0, duration=10000.0, simpleSeriesResistance=300.0,
+                                    targetVoltage=-30.0), 1.0)]
+    n_syn_types = len(m.syn_types)
+    ls = lems.build(types)
+    # three input instances on chosen compartments, appended after each compartment's synapses
+    cb = m.cell_comp_begin
+    comps = [int(cb[2]), int(cb[3]), int(cb[4])] if where == "point" else [int(cb[0]) + 5, int(cb[1]) + 17,
+                                                                               int(cb[0])]
+    per = [list(range(m.comp_syn_begin[g], m.comp_syn_begin[g + 1])) for g in range(m.n_comps)]
+    old_type, old_w, old_d = m.syn_type.copy(), m.syn_weight.copy(), m.syn_delay.copy()
+    new_type, new_w, new_d, remap = [], [], [], {}
+    extra = {g: [(n_syn_types + k, w) for k, (gg, w) in enumerate(zip(comps, (1.0, 0.7, 1.0))) if gg == g]
+             for g in comps}
+    for g in range(m.n_comps):
+        for s in per[g]:
+            remap[s] = len(new_type)
+            new_type.append(old_type[s]); new_w.append(old_w[s]); new_d.append(old_d[s])
+        for ty, w in extra.get(g, []):
+            new_type.append(ty); new_w.append(w); new_d.append(0.0)
+    counts = [len(per[g]) + len(extra.get(g, [])) for g in range(m.n_comps)]
+    begin = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    g2 = dataclasses.replace(m, comp_syn_begin=begin, syn_type=np.array(new_type, np.int32),
+                             syn_weight=np.array(new_w, np.float32), syn_delay=np.array(new_d, np.float32),
+                             spike_target=np.array([remap[int(s)] for s in m.spike_target], np.int64),
+                             lems=ls, name=f"lems_inputs_{where}")
+    tr, er, ost = _check(g2)
+    base = _check(dataclasses.replace(m, lems=lems.build(types[:n_syn_types]), name="no_inputs"))[0]
+    assert not np.array_equal(tr, base)

@@ -51,14 +51,14 @@ def _stepper(tname, vals, weight=1.0):
     lib = oracle.lems_library(ls.source_c)
     fn = lib.lems_syn
     fn.restype = None
-    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_float, C.c_int, C.c_int,
-                   C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_float, C.c_float, C.c_int,
+                   C.c_int, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     P = ls.params[0]
     S = np.zeros(ls.n_state[0], np.float32)
 
     def step(v, init, event, dt):
         i, g = C.c_float(), C.c_float()
-        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, v, init, event, dt, C.byref(i), C.byref(g))
+        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, v, 0.0, init, event, dt, C.byref(i), C.byref(g))
         return i.value, g.value
     return step
 
@@ -142,7 +142,8 @@ def test_reference_core_synapses_model(tmp_path):
                                  "expThreeSynapse",
                                  "blockingPlasticSynapse/voltageConcDepBlockMechanism/tsodyksMarkramDepFacMechanism",
                                  "silentSynapse", "gapJunction", "linearGradedSynapse", "gradedSynapse",
-                                 "expCondSynapse", "alphaCondSynapse", "expCurrSynapse", "alphaCurrSynapse"]
+                                 "expCondSynapse", "alphaCondSynapse", "expCurrSynapse", "alphaCurrSynapse",
+                                 "sineGenerator"]       # an input type, unused by the network
     # ChemPost cells 1..10 and 21 receive one synapse each, of the projections' types in
     # order; then the graded and electrical ends on ContPre 1-4 and ContPost 1-4
     comp_of = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
@@ -304,14 +305,14 @@ def _stepper_peer(tname, vals, custom=None, weight=1.0):
     lib = oracle.lems_library(ls.source_c)
     fn = lib.lems_syn
     fn.restype = None
-    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_float, C.c_int, C.c_int,
-                   C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float, C.c_float, C.c_float, C.c_int,
+                   C.c_int, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     P = ls.params[0]
     S = np.zeros(max(1, ls.n_state[0]), np.float32)
 
-    def step(v, vpeer, init, event, dt=DT):
+    def step(v, vpeer, init, event, dt=DT, t=0.0):
         i, g = C.c_float(), C.c_float()
-        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, vpeer, init, event, dt, C.byref(i), C.byref(g))
+        fn(0, P.ctypes.data, S.ctypes.data, S.ctypes.data, 1, v, vpeer, t, init, event, dt, C.byref(i), C.byref(g))
         return i.value, g.value, S.copy()
     return step
 
@@ -406,3 +407,99 @@ def test_tsodyks_markram_release_follows_the_event_maps():
     r_ss = (1.0 - a ** m) / (1.0 - (1.0 - p) * a ** m)
     assert q[-1] / q[0] == pytest.approx(r_ss, rel=1e-4)
     assert q[1] / q[0] == pytest.approx(1.0 - p * a ** m, rel=1e-4)     # 1 - R recovers as a^m
+
+
+def _input_stepper(tname, vals, custom=None):
+    return _stepper_peer(tname, vals, custom)
+
+
+def test_input_generators_follow_their_definitions():
+    """The LEMS inputs' current at time t (their state is assigned by the OnCondition of the
+    window t falls in): sine amplitude sin(phase + 2 pi (t - delay) / period) on
+    [delay, delay + duration), 0 outside; ramp from start to finish amplitude over the window,
+    baseline outside (and at initialisation); pulseGeneratorDL the amplitude on the window;
+    voltage clamps (target - v) / R while active, with the conditioning, testing and return
+    voltages for the triple clamp."""
+    sine = _input_stepper("sineGenerator", dict(phase=0.3, delay=20000.0, duration=50000.0, amplitude=1.0,
+                                                period=5000.0))
+    pi = 3.14159265                # the definition's literal
+    for t, want in ((10000.0, 0.0), (20000.0, math.sin(0.3)), (21250.0, math.sin(0.3 + pi / 2)),
+                    (69999.0, math.sin(0.3 + 2 * pi * 49999.0 / 5000.0)), (70000.0, 0.0)):
+        i, g, _ = sine(-65.0, -65.0, 0, 0, t=t)
+        # the float argument near 63 carries an ulp of 4e-6
+        assert i == pytest.approx(want, abs=1e-5), t
+    ramp = _input_stepper("rampGenerator", dict(delay=30000.0, duration=170000.0, startAmplitude=-0.032,
+                                                finishAmplitude=0.05, baselineAmplitude=-0.032))
+    assert ramp(-65.0, -65.0, 1, 0, t=0.0)[2][0] == np.float32(-0.032)          # OnStart
+    for t, want in ((0.0, -0.032), (30000.0, -0.032), (115000.0, -0.032 + 0.082 * 0.5), (200000.0, -0.032)):
+        assert ramp(-65.0, -65.0, 0, 0, t=t)[0] == pytest.approx(want, abs=1e-7), t
+    dl = _input_stepper("pulseGeneratorDL", dict(delay=10000.0, duration=20000.0, amplitude=14.0))
+    assert [dl(-65.0, -65.0, 0, 0, t=t)[0] for t in (9999.0, 10000.0, 29999.0, 30000.0)] == [0.0, 14.0, 14.0, 0.0]
+    vc = _input_stepper("voltageClamp", dict(delay=10000.0, duration=30000.0, simpleSeriesResistance=100.0,
+                                             targetVoltage=-20.0))
+    assert vc(-65.0, -65.0, 0, 0, t=5000.0)[0] == 0.0
+    assert vc(-65.0, -65.0, 0, 0, t=10000.0)[0] == np.float32(np.float32(45.0) / np.float32(100.0))
+    assert vc(-65.0, -65.0, 0, 0, t=40000.0)[0] != 0.0                          # t = delay + duration: still on
+    assert vc(-65.0, -65.0, 0, 0, t=40025.0)[0] == 0.0
+    tc = _input_stepper("voltageClampTriple", dict(delay=10000.0, duration=30000.0, simpleSeriesResistance=100.0,
+                                                   active=1.0, conditioningVoltage=-63.77, testingVoltage=0.0,
+                                                   returnVoltage=-40.77))
+    for t, target in ((0.0, -63.77), (10000.0, 0.0), (40025.0, -40.77)):
+        assert tc(-65.0, -65.0, 0, 0, t=t)[0] == pytest.approx((target + 65.0) / 100.0, rel=1e-6), t
+
+
+def test_voltage_clamp_holds_a_passive_cell_near_its_target(tmp_path):
+    """A passive compartment under a voltage clamp with series resistance R settles where the
+    leak and clamp currents cancel: V = (g E + V_t / R) / (g + 1 / R)."""
+    p = tmp_path / "clamp.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml><ionChannelHH id="pas" conductance="10pS"/>'
+        '<voltageClamp id="vc" delay="5ms" duration="200ms" targetVoltage="-20mV" simpleSeriesResistance="1e8ohm"/>'
+        '<cell id="c"><morphology id="m"><segment id="0"><proximal x="0" y="0" z="0" diameter="17.841242"/>'
+        '<distal x="0" y="0" z="0" diameter="17.841242"/></segment></morphology><biophysicalProperties id="b">'
+        '<membraneProperties><channelDensity id="leak" ionChannel="pas" condDensity="3.0 S_per_m2" erev="-54.3mV"/>'
+        '<specificCapacitance value="1.0 uF_per_cm2"/><initMembPotential value="-65mV"/></membraneProperties>'
+        '<intracellularProperties><resistivity value="0.03 kohm_cm"/></intracellularProperties>'
+        '</biophysicalProperties></cell><network id="n"><population id="p" component="c" size="1"/>'
+        '<inputList id="il" population="p" component="vc"><input id="0" target="../p/0/c" destination="synapses"/>'
+        '</inputList></network></neuroml>'
+        '<Simulation id="s" length="150ms" step="0.025ms" target="n" seed="1"><OutputFile id="o" fileName="o.txt">'
+        '<OutputColumn id="v" quantity="p/0/c/0/v"/></OutputFile></Simulation></Lems>')
+    m, _ = neuroml.read(str(p))
+    assert m.lems.type_flags == [1]
+    tr, _, _, _ = oracle.run(m)
+    g = 0.3 * 1e-5 * float(m.area[0])        # 0.3 mS/cm^2 (3 S/m^2) over the area (um^2), in uS
+    want = (g * -54.3 + -20.0 / 100.0) / (g + 1.0 / 100.0)
+    assert float(tr[-1, 0]) == pytest.approx(want, abs=2e-3)
+    # before the clamp's delay the membrane relaxes to the leak reversal by the engine's
+    # implicit step: V_n = E + (V_0 - E) / (1 + dt g / C)^n
+    C = float(m.capacitance[0])
+    n = np.arange(150)
+    want0 = -54.3 + (-65.0 + 54.3) / (1.0 + 25.0 * g / C) ** n
+    np.testing.assert_allclose(tr[:150, 0], want0, atol=1e-3)   # float32 parameters and steps
+
+
+def test_reference_core_inputs_and_custom_input_models_read(tmp_path):
+    """The reference's CoreInputs model (without its timedSynapticInput): one LEMS input type
+    per generator element, flagged as inputs, each instance after the synapse types in its
+    compartment's list; EdenTest4's user-defined myPulseGenerator reads as an input type and
+    its tree cells carry the instances; both run in the oracle."""
+    from coreinputs import write_model
+    m, _ = neuroml.read(write_model(str(tmp_path)))
+    assert m.lems.type_names == ["expOneSynapse", "expTwoSynapse", "pulseGeneratorDL", "sineGenerator",
+                                 "sineGeneratorDL", "rampGenerator", "rampGeneratorDL", "voltageClamp",
+                                 "voltageClampTriple"]
+    assert m.lems.type_flags == [0, 0, 1, 1, 1, 1, 1, 1, 1]
+    comp_of = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
+    # InputDemos/02-05 (compartments 2-5): sine, ramp, clamp, triple clamp; 11, 12: the spike
+    # sources' synapses; InputDemosDL/01-03: the dimensionless generators
+    assert list(comp_of) == [2, 3, 4, 5, 11, 12, 18, 19, 20]
+    assert list(m.syn_type) == [3, 5, 7, 8, 0, 0, 2, 4, 6]
+    tr, ec, er, _ = oracle.run(m)
+    assert np.bincount(ec).tolist() == [5, 16]
+    m4, _ = neuroml.read(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "neuroml",
+                                      "LEMS_EdenTest4.xml"))
+    assert m4.lems.type_names[-2:] == ["myPulseGenerator", "myPulseGenerator"] and m4.lems.type_flags[-2:] == [1, 1]
+    tr4, _, _, _ = oracle.run(m4)
+    assert np.array_equal(tr4[:, 0], tr4[:, 1])                 # a native pulse and ... the same pulse
+    assert float(np.ptp(tr4[:, 4])) > 0.5
