@@ -2,8 +2,8 @@
 into lems_cell_kernel at context creation. Against the oracle running the C dialect of the
 same text: traces, spike rasters and synaptic deliveries bit for bit.
 
-* the reference's ArtificialCells model (tests/artcells.py), through the NeuroML reader and
-  the output files;
+* the reference's ArtificialCells model in full (tests/artcells.py), through the NeuroML
+  reader and the output files;
 * a network of point neurons and physical cells: an HH cell driving integrate-and-fire,
   Izhikevich and adaptive exponential cells through synapses, point neurons driving passive
   cells, spike sources (spikeArray, spikeGenerator) driving both kinds, a gap junction
