@@ -41,12 +41,14 @@ _P = C.c_void_p
 
 class LemsSynapsesDesc(C.Structure):
     _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
-                ("n_state", _P), ("peer", _P), ("type_flags", _P)]
+                ("n_state", _P), ("peer", _P), ("type_flags", _P), ("instance_param", _P),
+                ("n_params", C.c_int64)]
 
 
 class LemsCellsDesc(C.Structure):
     _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
-                ("n_state", _P), ("v_state", _P), ("cell_type", _P)]
+                ("n_state", _P), ("v_state", _P), ("cell_type", _P), ("cell_param", _P),
+                ("n_params", C.c_int64)]
 
 
 class ModelDesc(C.Structure):

@@ -112,6 +112,7 @@ struct LemsArgs {
     float2 *syn_ext;              // [NC] per-compartment (I_syn, G_syn) of the step
     const int32_t *peer;          // per synapse slot: the gap junction's peer compartment, or -1
     const int32_t *flags;         // per type: bit 0 an input type (null: none)
+    const int32_t *poff;          // per slot: offset of its parameter vector in params (null: its type's)
     float2 *inp_ext;              // [NC] per-compartment (I, G) of the LEMS inputs of the step, or null
 };
 
@@ -125,6 +126,7 @@ struct LemsCellArgs {
     const int32_t *param_begin;   // [n_types + 1]
     const int32_t *v_state;       // [n_types] membrane-voltage state variable, or -1
     float *state;                 // [max state][n]: variable k of slot i at k * n + i
+    const int32_t *poff;          // [n] offset of the cell's parameter vector in params (null: its type's)
 };
 
 // Correctly rounded fp32 division whose expansion does not go through VCC.

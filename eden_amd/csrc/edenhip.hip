@@ -150,6 +150,8 @@ struct edenhip_ctx {
     int32_t *d_lems_peer = nullptr;      // per synapse slot: gap junction peer compartment, or -1
     float2 *d_syn_ext = nullptr;
     int32_t *d_lems_flags = nullptr;     // per generic type: bit 0 an input type
+    int32_t *d_lems_poff = nullptr;      // per synapse slot: offset of its parameter vector (per-instance settings)
+    int32_t *d_acell_poff = nullptr;     // per point neuron: offset of its parameter vector (per-instance settings)
     float2 *d_inp_ext = nullptr;         // per compartment: the LEMS inputs' (I, G) of the step
     // LEMS point neurons: the run-time compiled cell kernel and its tables (one slot per cell)
     hipFunction_t lems_cell_fn = nullptr;
@@ -288,6 +290,11 @@ static int pool_stream(edenhip_ctx *ctx, int slot, hipStream_t *out) {
     return 0;
 }
 
+// floats in a LEMS parameter table: the types' vectors, then any per-instance vectors
+static int64_t lems_param_count(const int32_t *param_begin, int32_t n_types, int64_t n_params) {
+    return std::max<int64_t>(param_begin[n_types], n_params);
+}
+
 static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     if (d->abi_version < 1 || d->abi_version > EDENHIP_ABI_VERSION) return fail(ctx, EDENHIP_E_INVALID, "abi_version mismatch");
     const edenhip_lems_synapses *LS = d->abi_version >= 2 ? d->lems : nullptr;
@@ -298,6 +305,17 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         for (int t = 0; t < LS->n_types; t++) {
             if (LS->n_state[t] < 0 || LS->n_state[t] > 8) return fail(ctx, EDENHIP_E_LIMIT, "lems: more than 8 state variables");
             if (LS->param_begin[t + 1] < LS->param_begin[t]) return fail(ctx, EDENHIP_E_INVALID, "lems: param_begin decreasing");
+        }
+        if (d->abi_version >= 4 && LS->instance_param) {
+            const int64_t nsyn = d->comp_syn_begin[d->n_comps], np = lems_param_count(LS->param_begin, LS->n_types,
+                                                                                          LS->n_params);
+            for (int64_t k = 0; k < nsyn; k++) {
+                const int64_t o = LS->instance_param[k];
+                const int32_t ty = d->syn_type[k];
+                if (ty < 0 || ty >= LS->n_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
+                if (o < -1 || (o >= 0 && o + (LS->param_begin[ty + 1] - LS->param_begin[ty]) > np))
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: instance_param out of range");
+            }
         }
         if (d->abi_version >= 3 && LS->peer) {
             const int64_t nsyn = d->comp_syn_begin[d->n_comps];
@@ -318,6 +336,16 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                 return fail(ctx, EDENHIP_E_INVALID, "lems_cells: v_state out of range");
             if (LC->param_begin[t + 1] < LC->param_begin[t])
                 return fail(ctx, EDENHIP_E_INVALID, "lems_cells: param_begin decreasing");
+        }
+        if (LC->cell_param) {
+            const int64_t np = lems_param_count(LC->param_begin, LC->n_types, LC->n_params);
+            for (int c = 0; c < d->n_cells; c++) {
+                const int64_t o = LC->cell_param[c];
+                const int32_t ty = LC->cell_type[c];
+                if (o == -1) continue;
+                if (ty < 0 || ty >= LC->n_types || o < 0 || o + (LC->param_begin[ty + 1] - LC->param_begin[ty]) > np)
+                    return fail(ctx, EDENHIP_E_INVALID, "lems_cells: cell_param out of range");
+            }
         }
     } else {
         LC = nullptr;
@@ -830,8 +858,21 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         ctx->n_lems_comps = (int32_t)lc.size();
         int32_t smax = 1;
         for (int t = 0; t < LS->n_types; t++) smax = std::max(smax, LS->n_state[t]);
-        std::vector<float> prm(LS->params, LS->params + LS->param_begin[LS->n_types]);
+        const int64_t np = lems_param_count(LS->param_begin, LS->n_types, d->abi_version >= 4 ? LS->n_params : 0);
+        if (np >= (int64_t)1 << 31) return fail(ctx, EDENHIP_E_LIMIT, "lems: more than 2^31 parameters");
+        std::vector<float> prm(LS->params, LS->params + np);
         std::vector<int32_t> pb(LS->param_begin, LS->param_begin + LS->n_types + 1);
+        if (d->abi_version >= 4 && LS->instance_param) {
+            // each slot's parameter vector: its instance's own, else its type's
+            std::vector<int32_t> poff((size_t)n_slots, 0);
+            for (int64_t k = 0; k < n_slots; k++) {
+                const int32_t s = syn_orig[(size_t)k];
+                if (s < 0) continue;
+                const int64_t o = LS->instance_param[s];
+                poff[(size_t)k] = o >= 0 ? (int32_t)o : pb[(size_t)d->syn_type[s]];
+            }
+            TRY(upload(ctx, &ctx->d_lems_poff, poff));
+        }
         TRY(upload(ctx, &ctx->d_lems_comps, lc));
         TRY(upload(ctx, &ctx->d_lems_params, prm));
         TRY(upload(ctx, &ctx->d_lems_pbegin, pb));
@@ -874,9 +915,21 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // point-neuron tables: compartments, types, parameters, state [variable][slot]
         int32_t smax = 1;
         for (int t = 0; t < LC->n_types; t++) smax = std::max(smax, LC->n_state[t]);
-        std::vector<float> prm(LC->params, LC->params + LC->param_begin[LC->n_types]);
+        const int64_t np = lems_param_count(LC->param_begin, LC->n_types, LC->n_params);
+        if (np >= (int64_t)1 << 31) return fail(ctx, EDENHIP_E_LIMIT, "lems_cells: more than 2^31 parameters");
+        std::vector<float> prm(LC->params, LC->params + np);
         std::vector<int32_t> pb(LC->param_begin, LC->param_begin + LC->n_types + 1);
         std::vector<int32_t> vs(LC->v_state, LC->v_state + LC->n_types);
+        if (LC->cell_param) {
+            // each point neuron's parameter vector: its own, else its type's
+            std::vector<int32_t> poff;
+            for (int c = 0; c < d->n_cells; c++) {
+                const int32_t ty = LC->cell_type[c];
+                if (ty < 0) continue;
+                poff.push_back(LC->cell_param[c] >= 0 ? (int32_t)LC->cell_param[c] : pb[(size_t)ty]);
+            }
+            TRY(upload(ctx, &ctx->d_acell_poff, poff));
+        }
         TRY(upload(ctx, &ctx->d_acell_comps, acell_comp));
         TRY(upload(ctx, &ctx->d_acell_type, acell_type));
         TRY(upload(ctx, &ctx->d_acell_params, prm));
@@ -1285,7 +1338,8 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             if (ctx->lems_fn && ctx->n_lems_comps > 0) {
                 // generic synapses of step a: per-compartment currents for the kernels below
                 LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,
-                     ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer, ctx->d_lems_flags, ctx->d_inp_ext};
+                     ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer, ctx->d_lems_flags, ctx->d_lems_poff,
+                     ctx->d_inp_ext};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &la, &st, &dt};
@@ -1296,7 +1350,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             if (ctx->lems_cell_fn && ctx->n_acells > 0) {
                 // point neurons of step a, after the synapse currents of the step
                 LemsCellArgs ca{ctx->d_acell_comps, ctx->d_acell_type, ctx->n_acells, ctx->d_acell_params,
-                                ctx->d_acell_pbegin, ctx->d_acell_vstate, ctx->d_acell_state};
+                                ctx->d_acell_pbegin, ctx->d_acell_vstate, ctx->d_acell_state, ctx->d_acell_poff};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &ca, &st, &dt};

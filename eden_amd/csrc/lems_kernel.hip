@@ -88,8 +88,8 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
         float *S = L.state + s;
         const int32_t pc = L.peer[s];
         const float vpeer = pc >= 0 ? T.V[pc] : V;
-        lems_syn(ty, L.params + L.param_begin[ty], S, S, (long)L.stride, V, vpeer, t, init ? 1 : 0, spike_now ? 1 : 0,
-                 dt, &i, &gx);
+        const float *P = L.params + (L.poff ? L.poff[s] : L.param_begin[ty]);
+        lems_syn(ty, P, S, S, (long)L.stride, V, vpeer, t, init ? 1 : 0, spike_now ? 1 : 0, dt, &i, &gx);
         if (input) {
             I_inp += i * weight;
             G_inp += gx * weight;
@@ -129,7 +129,8 @@ lems_cell_kernel(edenhip::DevTables T, edenhip::LemsCellArgs A, int64_t step, fl
     const float ext = I_syn + I_in;
     float *S = A.state + idx;
     int spike = 0;
-    lems_cell(ty, A.params + A.param_begin[ty], S, S, (long)A.n, t, dt, init ? 1 : 0, ext, &spike);
+    const float *P = A.params + (A.poff ? A.poff[idx] : A.param_begin[ty]);
+    lems_cell(ty, P, S, S, (long)A.n, t, dt, init ? 1 : 0, ext, &spike);
     const int32_t vs = A.v_state[ty];
     if (vs >= 0) T.V[g] = S[(int64_t)vs * A.n];
     if (spike) {

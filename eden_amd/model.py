@@ -110,6 +110,13 @@ class Model:
                                        # (a point neuron's j-th state variable)
     lems_cells: object = None          # eden_amd.lems_cells.LemsCells, or None
     cell_lems_type: np.ndarray = None  # per cell: its point-neuron type, or -1
+    # per-instance parameter vectors (the reference's EdenCustomSetup statements): per generic
+    # synapse instance / per point-neuron cell an index into the list of vectors, -1 for its
+    # type's; each vector has its type's layout
+    syn_param_index: np.ndarray = None
+    syn_param_vecs: list = None
+    cell_param_index: np.ndarray = None
+    cell_param_vecs: list = None
     name: str = "model"
 
     def __post_init__(self):
@@ -249,9 +256,11 @@ class Model:
                 peer = np.ascontiguousarray(self.syn_peer, np.int64)
                 self._lems_peer = peer
             self._lems_flags = np.array(L.type_flags or [0] * len(L.params), np.int32)
+            prm, off = _with_instance_vectors(L.params, self.syn_param_vecs, self.syn_param_index, self.n_synapses)
+            self._lems_arrays = self._lems_arrays[:1] + (prm,) + self._lems_arrays[2:] + (off,)
             self._lems_desc = N.LemsSynapsesDesc(len(L.params), self._lems_arrays[3], N.ptr(self._lems_arrays[0]),
                                                  N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]),
-                                                 N.ptr(peer), N.ptr(self._lems_flags))
+                                                 N.ptr(peer), N.ptr(self._lems_flags), N.ptr(off), len(prm))
             d.lems = C.pointer(self._lems_desc)
         if self.lems_cells is not None:
             LC = self.lems_cells
@@ -261,13 +270,31 @@ class Model:
             self._lc_arrays = (pb, np.concatenate(LC.params + [np.zeros(1, np.float32)]).astype(np.float32),
                                np.array(LC.n_state, np.int32), np.array(LC.v_state, np.int32), ctype,
                                LC.source_hip.encode())
+            prm, off = _with_instance_vectors(LC.params, self.cell_param_vecs, self.cell_param_index, self.n_cells)
+            self._lc_arrays = self._lc_arrays[:1] + (prm,) + self._lc_arrays[2:] + (off,)
             a = self._lc_arrays
             self._lc_desc = N.LemsCellsDesc(len(LC.params), a[5], N.ptr(a[0]), N.ptr(a[1]), N.ptr(a[2]), N.ptr(a[3]),
-                                            N.ptr(a[4]))
+                                            N.ptr(a[4]), N.ptr(off), len(prm))
             d.lems_cells = C.pointer(self._lc_desc)
         d.n_record = len(self.record_comp)
         d.n_event_cols = len(self.event_comp)
         return d
+
+
+def _with_instance_vectors(type_params, vecs, index, n):
+    """The parameter table of the types' vectors followed by the per-instance vectors, and
+    each instance's offset into it (-1: its type's), or None when no instance has its own."""
+    base = [np.asarray(p, np.float32) for p in type_params]
+    total = sum(len(p) for p in base)
+    if not vecs or index is None:
+        return np.concatenate(base + [np.zeros(1, np.float32)]).astype(np.float32), None
+    index = np.asarray(index, np.int64)
+    if index.shape != (n,):
+        raise ValueError("per-instance parameter index: one entry per instance")
+    starts = total + np.concatenate([[0], np.cumsum([len(v) for v in vecs])[:-1]]).astype(np.int64)
+    off = np.where(index >= 0, starts[np.maximum(index, 0)], -1).astype(np.int64)
+    prm = np.concatenate(base + [np.asarray(v, np.float32) for v in vecs]).astype(np.float32)
+    return prm, off
 
 
 class EdenHipError(RuntimeError):

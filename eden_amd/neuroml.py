@@ -52,6 +52,9 @@ set covers:
                     (eden/NeuroML.cpp:6136-6420, eden/Eden.cpp:9225-9270)
   simulation        <Simulation length step seed target> with OutputFile / OutputColumn and
                     EventOutputFile / EventSelection, <Target> (eden/NeuroML.cpp:7979-8311)
+  custom setup      <EdenCustomSetup filename>: per-instance parameters of point neurons,
+                    input-list inputs and projection connections (eden_amd.custom_setup;
+                    eden/NeuroML.cpp:7270-7980, eden/Eden.cpp:9437-9660)
 
 Instantiation follows the reference's order: cells numbered over populations in order (the
 work-item / RNG id, eden/Eden.cpp:7857-7911); per compartment, channel distributions in the
@@ -77,6 +80,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as N
+from . import custom_setup as setup_mod
 from . import lems as lems_mod
 from . import lems_cells as cells_mod
 from .model import Model, cell_rng_seeds, channel_type
@@ -191,6 +195,7 @@ class _Doc:
         self.order = []                  # (role, id) in document order
         self.networks = OrderedDict()
         self.simulations = OrderedDict()
+        self.sim_base = {}               # simulation id -> directory of the file holding it
         self.morphologies = {}
         self.biophysics = {}
         self.component_types = {}        # user <ComponentType> elements by name
@@ -261,6 +266,7 @@ class _Doc:
                 self.networks[el.get("id")] = el
             elif t == "Simulation":
                 self.simulations[el.get("id")] = el
+                self.sim_base[el.get("id")] = base
             elif t in _UNSUPPORTED_SYNAPSES or t in _UNSUPPORTED_CELLS or t in _UNSUPPORTED_INPUTS:
                 # read, and refused only when the model uses it
                 self._add("unsupported", el)
@@ -864,6 +870,141 @@ class _Pop:
             raise NeuroMLError(f"{what}: instance {inst_id} not in population {self.id}") from None
 
 
+def _custom_setup(doc, sim):
+    """The statements of the simulation's EdenCustomSetup file, if it names one (its file name
+    relative to the file holding the Simulation, eden/NeuroML.cpp:8302-8313, 7281-7283)."""
+    els = [el for el in sim if _tag(el) == "EdenCustomSetup"]
+    if not els:
+        return []
+    if len(els) > 1:
+        raise NeuroMLError("EdenCustomSetup already defined previously")
+    name = els[0].get("filename")
+    if not name:
+        raise NeuroMLError("EdenCustomSetup: attribute filename missing")
+    p = os.path.join(doc.sim_base.get(sim.get("id"), "."), name)
+    try:
+        with open(p) as f:
+            text = f.read()
+    except OSError as e:
+        raise NeuroMLError(f"EdenCustomSetup: could not open file \"{p}\": {e.strerror}") from None
+    try:
+        return setup_mod.parse(text)
+    except setup_mod.CustomSetupError as e:
+        raise NeuroMLError(str(e)) from None
+
+
+def _names(items):
+    return [x if isinstance(x, str) else x[0] for x in items]
+
+
+def _value_converter(dim, what):
+    """convert_one(number text, unit) for a parameter of the given engine dimension (None:
+    dimensionless, any unit token ignored as the reference does)."""
+    def one(v, unit):
+        if dim is None:
+            return float(v)
+        if unit is None:
+            raise NeuroMLError(f"{what}: statement needs units for a property of dimension {dim}")
+        return convert(f"{v} {unit}", dim, what)
+    return one
+
+
+def _apply_setting(st, doc, pops, input_lists, projections, input_where, pulses, poisson, generic_index, generic_st,
+                   syn_over, cell_over):
+    """One EdenCustomSetup statement (eden/Eden.cpp:9501-9650): point-neuron parameters per
+    cell, input parameters per input of a list (native pulse and Poisson tables directly,
+    generic inputs per instance), synapse parameters per connection end."""
+    what = st.what()
+
+    def seqs_of(ids, n, label):
+        if ids is None:
+            return list(range(n))
+        pos = {x: i for i, x in enumerate(label)}
+        out = []
+        for x in ids:
+            if x not in pos:
+                raise NeuroMLError(f"{what}: could not resolve id {x}")
+            out.append(pos[x])
+        return out
+
+    def settable(names_dims, state_names, owner):
+        if st.param not in names_dims:
+            if st.param in state_names:
+                raise NeuroMLError(f"{what}: {st.param} is a state variable of {owner}; only parameters can be set")
+            raise NeuroMLError(f"{what}: {owner} has no parameter {st.param}")
+        return names_dims[st.param]
+
+    try:
+        if st.kind == "cell":
+            p = pops.get(st.group)
+            if p is None:
+                raise NeuroMLError(f"{what}: population {st.group} not found")
+            if p.cell.point is None:
+                raise NeuroMLError(f"{what}: population {st.group} holds physical cells; 'set cell' sets point-neuron "
+                                   "parameters")
+            if st.where != "all":
+                raise NeuroMLError(f"{what}: segment specifier '{st.where}' must be 'all' for point neurons")
+            ct = p.cell.point[2]
+            if isinstance(ct, cells_mod.SpikeList):
+                raise NeuroMLError(f"{what}: a spikeArray has no settable parameters")
+            ldim = settable(dict(ct.params), [s for s, _ in ct.state], f"cell type {p.cell.id}")
+            dim = lems_mod.LEMS_DIMS.get(ldim, ldim if ldim in _DIMS else "unsupported")
+            seqs = seqs_of(st.targets, len(p.inst_ids), p.inst_ids)
+            vals = setup_mod.values_of(st, len(seqs), _value_converter(dim, what))
+            for s, v in zip(seqs, vals):
+                cell_over.setdefault(p.first_cell + s, {})[st.param] = v
+        elif st.kind == "input":
+            if st.group not in input_lists:
+                raise NeuroMLError(f"{what}: input list {st.group} not found")
+            comp_id, items = input_lists[st.group]
+            seqs = seqs_of(st.targets, len(items), [x[0] for x in items])
+            el = doc.items[comp_id][1]
+            if comp_id in generic_index:
+                sty = generic_st[generic_index[comp_id]]
+                dim = settable(dict(sty.params), _names(sty.state), f"input {comp_id}")
+            elif _tag(el) == "pulseGenerator":
+                dim = settable({"delay": "time", "duration": "time", "amplitude": "current"}, [], f"input {comp_id}")
+            else:
+                dim = settable({"averageRate": "frequency"}, [], f"input {comp_id}")
+            vals = setup_mod.values_of(st, len(seqs), _value_converter(dim, what))
+            for s, v in zip(seqs, vals):
+                w = input_where[items[s][1]]
+                if w[0] == "conn":
+                    syn_over.setdefault(w[1], {})[st.param] = v
+                elif w[0] == "pulse":
+                    g, j = w[1], w[2]
+                    k = ("delay", "duration", "amplitude").index(st.param)
+                    rec = list(pulses[g][j])
+                    rec[k] = _f32(v)
+                    pulses[g][j] = tuple(rec)
+                else:
+                    g, j = w[1], w[2]
+                    rec = list(poisson[g][j])
+                    rec[0] = _f32(v)
+                    poisson[g][j] = tuple(rec)
+        else:
+            if st.group not in projections:
+                raise NeuroMLError(f"{what}: projection {st.group} not found")
+            kind, items = projections[st.group]
+            if st.where == "pre" and kind == "spiking":
+                raise NeuroMLError(f"{what}: 'pre' specifier can only be used for graded synapses")
+            seqs = seqs_of(st.targets, len(items), [x[0] for x in items])
+            if not seqs:
+                raise NeuroMLError(f"{what}: item set is empty")
+            end = 1 if st.where == "post" else 2
+            comps = {(items[s][3], items[s][4]) for s in seqs}
+            if len(comps) > 1:
+                raise NeuroMLError(f"{what}: target set contains different connection types")
+            syn_id = items[seqs[0]][end + 2]
+            sty = generic_st[generic_index[syn_id]]
+            dim = settable(dict(sty.params), _names(sty.state), f"synapse {syn_id}")
+            vals = setup_mod.values_of(st, len(seqs), _value_converter(dim, what))
+            for s, v in zip(seqs, vals):
+                syn_over.setdefault(items[s][end], {})[st.param] = v
+    except setup_mod.CustomSetupError as e:
+        raise NeuroMLError(str(e)) from None
+
+
 def read(path, rng_seed=None):
     """Read a LEMS or NeuroML file; return (Model, Outputs). rng_seed overrides the
     simulation's seed (EDEN's rng_seed token)."""
@@ -888,6 +1029,7 @@ def read(path, rng_seed=None):
         s = sim.get("seed")
         seed = int(s) if s is not None else int(time.time())
     temperature = convert(net.get("temperature"), "temperature", "temperature") if net.get("temperature") else None
+    statements = _custom_setup(doc, sim)
 
     # channel types in document order, then the cells the network uses
     channel_index, channel_types, gate_ids = {}, [], {}
@@ -981,9 +1123,11 @@ def read(path, rng_seed=None):
     used_inputs = {el.get("input") for el in net if _tag(el) == "explicitInput"} | \
                   {el.get("component") for el in net if _tag(el) == "inputList"}
     lems_inputs = any(i in used_inputs for i, _ in input_elems)
-    generic = lems_inputs or any(doc.items.get(u, (None, None))[0] == "synapse" and
-                                 (_tag(doc.items[u][1]) != "expTwoSynapse" or points) for u in used)
-    generic_index, generic_types, custom = {}, [], {}
+    # (per-instance synapse settings take the generic path too: its parameters are per instance)
+    generic = lems_inputs or any(s.kind == "synapse" for s in statements) or \
+        any(doc.items.get(u, (None, None))[0] == "synapse" and
+            (_tag(doc.items[u][1]) != "expTwoSynapse" or points) for u in used)
+    generic_index, generic_types, generic_st, custom = {}, [], [], {}
     if generic:
         for i, el in syn_elems:
             t = _tag(el)
@@ -1023,6 +1167,7 @@ def read(path, rng_seed=None):
             vals.update(extra)
             generic_index[i] = len(generic_types)
             generic_types.append((t, vals, weight))
+            generic_st.append(st)
         # LEMS current inputs after the synapse types (their instances come last in a
         # compartment's list)
         for i, el in input_elems:
@@ -1039,6 +1184,7 @@ def read(path, rng_seed=None):
             vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
             generic_index[i] = len(generic_types)
             generic_types.append((t, vals, weight))
+            generic_st.append(st)
 
     def synapse(sid, what, native=False):
         role, el = doc.items.get(sid, (None, None))
@@ -1056,6 +1202,10 @@ def read(path, rng_seed=None):
     # network inputs in order (explicitInput and inputList entries as they appear)
     inputs = []    # (component id, compartment, weight)
     conns = []     # (pre compartment, post compartment, synapse type, weight, delay us)
+    # what per-instance settings address: input lists' inputs (id, index into inputs) and
+    # projections' connections (id, post end's and pre end's index into conns or -1, the
+    # synapse component on each end)
+    input_lists, projections = {}, {}
     for el in net:
         t = _tag(el)
         if t == "explicitInput":
@@ -1064,10 +1214,12 @@ def read(path, rng_seed=None):
                                                  frac(el, "fractionAlong")), 1.0))
         elif t == "inputList":
             pop = el.get("population")
+            items = input_lists.setdefault(el.get("id"), (el.get("component"), []))[1]
             for i in el:
                 it = _tag(i)
                 if it not in ("input", "inputW"):
                     continue
+                items.append((int(i.get("id", len(items))), len(inputs)))
                 _, inst = _cell_ref(i.get("target"))
                 w = convert(i.get("weight"), None, "inputW weight") if it == "inputW" else 1.0
                 inputs.append((el.get("component"), comp(pop, inst, int(i.get("segmentId", "0")), "input",
@@ -1075,10 +1227,12 @@ def read(path, rng_seed=None):
         elif t == "projection":
             pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
             st = synapse(el.get("synapse"), f"projection {el.get('id')}")
+            items = projections.setdefault(el.get("id"), ("spiking", []))[1]
             for c in el:
                 ct = _tag(c)
                 if ct not in ("connection", "connectionWD"):
                     continue
+                items.append((int(c.get("id", len(items))), len(conns), -1, el.get("synapse"), None))
                 _, pi = _cell_ref(c.get("preCellId"))
                 _, qi = _cell_ref(c.get("postCellId"))
                 a = comp(pre, pi, int(c.get("preSegmentId", "0")), "connection", frac(c, "preFractionAlong"))
@@ -1096,10 +1250,12 @@ def read(path, rng_seed=None):
             # gap junctions: an instance on each end, the postsynaptic end's first, each with
             # the other end as its peer (eden/Eden.cpp:9225-9248)
             pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
+            items = projections.setdefault(el.get("id"), ("electrical", []))[1]
             for c in el:
                 ct = _tag(c)
                 if ct not in ("electricalConnection", "electricalConnectionInstance", "electricalConnectionInstanceW"):
                     continue
+                items.append((int(c.get("id", len(items))), len(conns), len(conns) + 1, c.get("synapse"), c.get("synapse")))
                 what = f"electricalProjection {el.get('id')}"
                 st = synapse(c.get("synapse"), what)
                 if ct == "electricalConnection":
@@ -1117,10 +1273,13 @@ def read(path, rng_seed=None):
             # presynaptic one on the pre end, each with the other end as its peer
             # (eden/Eden.cpp:9254-9270)
             pre, post = el.get("presynapticPopulation"), el.get("postsynapticPopulation")
+            items = projections.setdefault(el.get("id"), ("continuous", []))[1]
             for c in el:
                 ct = _tag(c)
                 if ct not in ("continuousConnection", "continuousConnectionInstance", "continuousConnectionInstanceW"):
                     continue
+                items.append((int(c.get("id", len(items))), len(conns), len(conns) + 1, c.get("postComponent"),
+                              c.get("preComponent")))
                 what = f"continuousProjection {el.get('id')}"
                 s_pre = synapse(c.get("preComponent"), what)
                 s_post = synapse(c.get("postComponent"), what)
@@ -1178,6 +1337,7 @@ def read(path, rng_seed=None):
     pulses = [[] for _ in range(nc)]
     poisson = [[] for _ in range(nc)]
     poi_count = {}    # (cell, input component) -> instances so far (the RNG stamp's instance)
+    input_where = []  # per input: ("conn", index into conns), ("pulse", g, j) or ("poisson", g, j)
     for (iid, g, w) in inputs:
         role, el = doc.items.get(iid, (None, None))
         if role != "input":
@@ -1186,8 +1346,10 @@ def read(path, rng_seed=None):
         t = _tag(el)
         if iid in generic_index:
             # a LEMS input: an instance in the compartment's synapse list
+            input_where.append(("conn", len(conns)))
             conns.append((-1, g, generic_index[iid], _f32(w), 0.0, -1))
         elif t == "pulseGenerator":
+            input_where.append(("pulse", g, len(pulses[g])))
             pulses[g].append((_f32(convert(el.get("delay"), "time", "delay")),
                               _f32(convert(el.get("duration"), "time", "duration")),
                               _f32(convert(el.get("amplitude"), "current", "amplitude")), _f32(w)))
@@ -1196,7 +1358,15 @@ def read(path, rng_seed=None):
             k = poi_count.get((cell, iid), 0)
             poi_count[(cell, iid)] = k + 1
             rate = _f32(convert(el.get("averageRate"), "frequency", "averageRate"))
+            input_where.append(("poisson", g, len(poisson[g])))
             poisson[g].append((rate, synapse(el.get("synapse"), f"poissonFiringSynapse {iid}", native=True), _f32(w), k))
+    # per-instance settings, in statement order (native inputs' tables directly; generic
+    # synapse instances and point neurons get their own parameter vectors below)
+    syn_over, cell_over = {}, {}
+    for st_ in statements:
+        _apply_setting(st_, doc, pops, input_lists, projections, input_where, pulses, poisson, generic_index,
+                       generic_st, syn_over, cell_over)
+
     pu_b = np.zeros(nc + 1, np.int64)
     np.cumsum([len(x) for x in pulses], out=pu_b[1:])
     pu = [x for lst in pulses for x in lst]
@@ -1296,6 +1466,31 @@ def read(path, rng_seed=None):
     used[np.diff(spk_b) > 0] = True
     vthr[used] = vthr_all[used]
 
+    lems_tables = lems_mod.build(generic_types, {k: v[0] for k, v in custom.items()}) if generic else None
+    syn_pidx, syn_vecs = None, None
+    if syn_over:
+        syn_pidx, syn_vecs = np.full(len(syn_list), -1, np.int64), []
+        for i, k in enumerate(syn_list):
+            if k in syn_over:
+                ty = conns[k][2]
+                names = [p for p, _ in generic_st[ty].params]
+                vec = np.array(lems_tables.params[ty], np.float32)
+                for p, v in syn_over[k].items():
+                    vec[names.index(p)] = _f32(v)
+                syn_pidx[i] = len(syn_vecs)
+                syn_vecs.append(vec)
+    cell_pidx, cell_vecs = None, None
+    if cell_over:
+        cell_pidx, cell_vecs = np.full(n_cells, -1, np.int64), []
+        for c in sorted(cell_over):
+            ty = int(cell_point_type[c])
+            names = [p for p, _ in pop_of_cell[c].cell.point[2].params]
+            vec = np.array(point_types.params[ty], np.float32)
+            for p, v in cell_over[c].items():
+                vec[names.index(p)] = _f32(v)
+            cell_pidx[c] = len(cell_vecs)
+            cell_vecs.append(vec)
+
     f32 = lambda xs, i: np.array([x[i] for x in xs], np.float32)
     model = Model(
         dt=_f32(dt), t_final=float(length), cell_comp_begin=cell_begin,
@@ -1316,8 +1511,8 @@ def read(path, rng_seed=None):
         poisson_rng_offset=np.zeros(len(po), np.int32),
         record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
         record_gate=np.array(rec_gate, np.int32) if any(j >= 0 for j in rec_gate) else None,
-        lems=lems_mod.build(generic_types, {k: v[0] for k, v in custom.items()}) if generic else None,
-        lems_cells=point_types, cell_lems_type=cell_point_type,
+        lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
+        syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
         name=os.path.splitext(os.path.basename(path))[0])
     model.sim_seed = seed
     return model, Outputs(trajectories, events, base_dir)

@@ -113,6 +113,12 @@ typedef struct edenhip_lems_synapses {
                                      is the instance's vpeer (the other end of a gap junction,
                                      eden/Eden.cpp:9225-9248), or -1; NULL = none */
     const int32_t *type_flags;    /* (abi_version >= 4) [n_types] bit 0: an input type; NULL = none */
+    const int64_t *instance_param;/* (abi_version >= 4) [n_syn] offset into params of the instance's own
+                                     parameter vector (per-instance settings: the reference's
+                                     EdenCustomSetup, eden/NeuroML.cpp:7362-7980), or -1 for its
+                                     type's; NULL = every instance its type's */
+    int64_t n_params;             /* (abi_version >= 4) floats in params: the types' vectors, then any
+                                     per-instance vectors; 0 = param_begin[n_types] */
 } edenhip_lems_synapses;
 
 /* LEMS point neurons (the reference's artificial cells: iafCell, izhikevich2007Cell,
@@ -136,6 +142,10 @@ typedef struct edenhip_lems_cells {
     const int32_t *n_state;       /* [n_types] state variables per cell (at most 16) */
     const int32_t *v_state;       /* [n_types] the state variable exposed as the membrane voltage, or -1 */
     const int32_t *cell_type;     /* [n_cells] each cell's type, -1 for a physical cell */
+    const int64_t *cell_param;    /* [n_cells] offset into params of the cell's own parameter vector
+                                     (per-instance settings), or -1 for its type's; NULL = types' */
+    int64_t n_params;             /* floats in params: the types' vectors, then any per-instance
+                                     vectors; 0 = param_begin[n_types] */
 } edenhip_lems_cells;
 
 /*

@@ -371,7 +371,11 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                        (eden/Eden.cpp:3689-3703) */
                     const int64_t pc = (d->abi_version >= 3 && L->peer) ? L->peer[sy] : -1;
                     const real Vpeer = pc >= 0 ? V[pc] : Vcomp;
-                    g_lems(ty, L->params + L->param_begin[ty], s->lS[now] + sy, s->lS[nxt] + sy, (long)NSYN,
+                    /* the instance's own parameter vector where a per-instance setting gave it one
+                       (eden/NeuroML.cpp:7362-7980), else its type's */
+                    const int64_t po = (d->abi_version >= 4 && L->instance_param && L->instance_param[sy] >= 0)
+                                           ? L->instance_param[sy] : L->param_begin[ty];
+                    g_lems(ty, L->params + po, s->lS[now] + sy, s->lS[nxt] + sy, (long)NSYN,
                            (float)Vcomp, (float)Vpeer, (float)time_r, initial_state, spike_in_flag, (float)dt, &iv, &gv);
                     if (input) {
                         I_lems_input += (real)iv * weight;
@@ -492,7 +496,8 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
             const int32_t ty = LC->cell_type[cell];
             const float ext = (float)I_synapses_total + (float)I_input_total;
             int spike = 0;
-            g_lems_cell(ty, LC->params + LC->param_begin[ty], s->cS[now] + slot, s->cS[nxt] + slot, (long)ns,
+            const int64_t po = (LC->cell_param && LC->cell_param[cell] >= 0) ? LC->cell_param[cell] : LC->param_begin[ty];
+            g_lems_cell(ty, LC->params + po, s->cS[now] + slot, s->cS[nxt] + slot, (long)ns,
                         (float)time_r, (float)dt, initial_state, ext, &spike);
             const int32_t vs = LC->v_state[ty];
             Vn[g] = vs >= 0 ? (real)s->cS[nxt][(int64_t)vs * ns + slot] : V[g];
