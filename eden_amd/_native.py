@@ -42,7 +42,7 @@ _P = C.c_void_p
 class LemsSynapsesDesc(C.Structure):
     _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
                 ("n_state", _P), ("peer", _P), ("type_flags", _P), ("instance_param", _P),
-                ("n_params", C.c_int64)]
+                ("n_params", C.c_int64), ("spike_list", _P), ("spike_times", _P), ("n_spike_times", C.c_int64)]
 
 
 class LemsCellsDesc(C.Structure):

@@ -113,6 +113,8 @@ struct LemsArgs {
     const int32_t *peer;          // per synapse slot: the gap junction's peer compartment, or -1
     const int32_t *flags;         // per type: bit 0 an input type (null: none)
     const int32_t *poff;          // per slot: offset of its parameter vector in params (null: its type's)
+    int32_t *tpos;                // per slot: a firing-synapse input's position in ttimes, or -1 (null: none)
+    const float *ttimes;          // spike lists, each ended by a value >= FLT_MAX
     float2 *inp_ext;              // [NC] per-compartment (I, G) of the LEMS inputs of the step, or null
 };
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <atomic>
 #include <mutex>
@@ -151,6 +152,8 @@ struct edenhip_ctx {
     float2 *d_syn_ext = nullptr;
     int32_t *d_lems_flags = nullptr;     // per generic type: bit 0 an input type
     int32_t *d_lems_poff = nullptr;      // per synapse slot: offset of its parameter vector (per-instance settings)
+    int32_t *d_lems_tpos = nullptr;      // per synapse slot: a firing-synapse input's spike-list position, or -1
+    float *d_lems_ttimes = nullptr;      // firing-synapse inputs' spike lists
     int32_t *d_acell_poff = nullptr;     // per point neuron: offset of its parameter vector (per-instance settings)
     float2 *d_inp_ext = nullptr;         // per compartment: the LEMS inputs' (I, G) of the step
     // LEMS point neurons: the run-time compiled cell kernel and its tables (one slot per cell)
@@ -315,6 +318,27 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                 if (ty < 0 || ty >= LS->n_types) return fail(ctx, EDENHIP_E_INVALID, "synapse type index out of range");
                 if (o < -1 || (o >= 0 && o + (LS->param_begin[ty + 1] - LS->param_begin[ty]) > np))
                     return fail(ctx, EDENHIP_E_INVALID, "lems: instance_param out of range");
+            }
+        }
+        if (d->abi_version >= 4 && LS->spike_list) {
+            // firing-synapse inputs: lists inside spike_times, which ends with a sentinel, on
+            // instances of input types that take no incoming connections
+            const int64_t nsyn = d->comp_syn_begin[d->n_comps], nt = LS->n_spike_times;
+            if (!LS->spike_times || nt < 1 || !(LS->spike_times[nt - 1] >= FLT_MAX))
+                return fail(ctx, EDENHIP_E_INVALID, "lems: spike_times must end with a value >= FLT_MAX");
+            if (nt >= (int64_t)1 << 31) return fail(ctx, EDENHIP_E_LIMIT, "lems: more than 2^31 spike times");
+            for (int64_t k = 0; k < nsyn; k++) {
+                const int64_t o = LS->spike_list[k];
+                if (o < -1 || o >= nt) return fail(ctx, EDENHIP_E_INVALID, "lems: spike_list out of range");
+                const int32_t ty = d->syn_type[k];
+                if (o >= 0 && (ty < 0 || ty >= LS->n_types || !LS->type_flags || !(LS->type_flags[ty] & 1)))
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: a firing-synapse input's type must be an input type");
+            }
+            const int64_t nspk = d->comp_spike_begin[d->n_comps];
+            for (int64_t k = 0; k < nspk; k++) {
+                const int64_t tgt = d->spike_target[k];
+                if (tgt >= 0 && tgt < nsyn && LS->spike_list[tgt] >= 0)
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: a firing-synapse input takes no incoming connections");
             }
         }
         if (d->abi_version >= 3 && LS->peer) {
@@ -873,6 +897,22 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             }
             TRY(upload(ctx, &ctx->d_lems_poff, poff));
         }
+        if (d->abi_version >= 4 && LS->spike_list) {
+            std::vector<int32_t> tpos((size_t)n_slots, -1);
+            bool any = false;
+            for (int64_t k = 0; k < n_slots; k++) {
+                const int32_t s = syn_orig[(size_t)k];
+                if (s >= 0 && LS->spike_list[s] >= 0) {
+                    tpos[(size_t)k] = (int32_t)LS->spike_list[s];
+                    any = true;
+                }
+            }
+            if (any) {
+                std::vector<float> tt(LS->spike_times, LS->spike_times + LS->n_spike_times);
+                TRY(upload(ctx, &ctx->d_lems_tpos, tpos));
+                TRY(upload(ctx, &ctx->d_lems_ttimes, tt));
+            }
+        }
         TRY(upload(ctx, &ctx->d_lems_comps, lc));
         TRY(upload(ctx, &ctx->d_lems_params, prm));
         TRY(upload(ctx, &ctx->d_lems_pbegin, pb));
@@ -983,6 +1023,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         for (int t = 0; t < LS->n_types; t++) smax = std::max(smax, LS->n_state[t]);
         TRY(snap(ctx->d_lems_state, (size_t)(smax * n_slots) * sizeof(float)));
     }
+    if (ctx->d_lems_tpos) TRY(snap(ctx->d_lems_tpos, (size_t)n_slots * sizeof(int32_t)));
     if (ctx->d_acell_state) {
         int32_t smax = 1;
         for (int t = 0; t < LC->n_types; t++) smax = std::max(smax, LC->n_state[t]);
@@ -1339,7 +1380,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 // generic synapses of step a: per-compartment currents for the kernels below
                 LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,
                      ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer, ctx->d_lems_flags, ctx->d_lems_poff,
-                     ctx->d_inp_ext};
+                     ctx->d_lems_tpos, ctx->d_lems_ttimes, ctx->d_inp_ext};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &la, &st, &dt};

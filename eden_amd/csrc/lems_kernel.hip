@@ -62,24 +62,38 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
             I_agg = 0.0f; G_agg = 0.0f; cur = ty;
         }
         const float weight = T.syn_weight[s];
-        bool flag = false;
-        if (!init) {
-            flag = trig_now[s] != 0;
-            if (flag) trig_now[s] = 0;
-        }
-        const float delay = T.syn_delay[s];
-        const float next_spike = T.syn_next[s];
-        float nn = next_spike;
         bool spike_now = false;
-        if (!init) {
-            if (t <= next_spike && next_spike < t + dt) spike_now = true;
-            if (flag) {
-                const float fresh = t + delay;
-                if (t <= fresh && fresh < t + dt) spike_now = true;
-                if (nn < t + dt && nn < fresh) nn = fresh;
+        const int32_t tp = L.tpos ? L.tpos[s] : -1;
+        if (tp >= 0) {
+            // a firing-synapse input: its spike list's times before the end of the step fire it
+            // in the same step (eden/Eden.cpp:3780-3788); the list ends with a sentinel
+            // (events of the initial steps are ignored by the generated code, so not raised)
+            int32_t pos = tp;
+            bool fired = false;
+            while (L.ttimes[pos] < t + dt) { fired = true; pos++; }
+            if (!init) {
+                L.tpos[s] = pos;
+                spike_now = fired;
             }
+        } else {
+            bool flag = false;
+            if (!init) {
+                flag = trig_now[s] != 0;
+                if (flag) trig_now[s] = 0;
+            }
+            const float delay = T.syn_delay[s];
+            const float next_spike = T.syn_next[s];
+            float nn = next_spike;
+            if (!init) {
+                if (t <= next_spike && next_spike < t + dt) spike_now = true;
+                if (flag) {
+                    const float fresh = t + delay;
+                    if (t <= fresh && fresh < t + dt) spike_now = true;
+                    if (nn < t + dt && nn < fresh) nn = fresh;
+                }
+            }
+            if (nn != next_spike) T.syn_next[s] = nn;
         }
-        if (nn != next_spike) T.syn_next[s] = nn;
         if (spike_now && T.log_syn) {
             const uint32_t slot = atomicAdd(T.n_syn_events, 1u);
             if (slot < T.max_syn_events) T.syn_events[slot] = (step << 32) | (int64_t)T.syn_orig[s];

@@ -117,6 +117,10 @@ class Model:
     syn_param_vecs: list = None
     cell_param_index: np.ndarray = None
     cell_param_vecs: list = None
+    # firing-synapse inputs (timedSynapticInput): per synapse instance the offset of its spike
+    # list in spike_times, or -1; the lists' times (us), each ended by +inf
+    syn_spike_list: np.ndarray = None
+    spike_times: np.ndarray = None
     name: str = "model"
 
     def __post_init__(self):
@@ -258,9 +262,17 @@ class Model:
             self._lems_flags = np.array(L.type_flags or [0] * len(L.params), np.int32)
             prm, off = _with_instance_vectors(L.params, self.syn_param_vecs, self.syn_param_index, self.n_synapses)
             self._lems_arrays = self._lems_arrays[:1] + (prm,) + self._lems_arrays[2:] + (off,)
+            sl, stimes = None, None
+            if self.syn_spike_list is not None:
+                sl = np.ascontiguousarray(self.syn_spike_list, np.int64)
+                stimes = np.ascontiguousarray(self.spike_times, np.float32)
+                if sl.shape != (self.n_synapses,):
+                    raise ValueError("syn_spike_list: one entry per synapse instance")
+            self._lems_spikes = (sl, stimes)
             self._lems_desc = N.LemsSynapsesDesc(len(L.params), self._lems_arrays[3], N.ptr(self._lems_arrays[0]),
                                                  N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]),
-                                                 N.ptr(peer), N.ptr(self._lems_flags), N.ptr(off), len(prm))
+                                                 N.ptr(peer), N.ptr(self._lems_flags), N.ptr(off), len(prm),
+                                                 N.ptr(sl), N.ptr(stimes), 0 if stimes is None else len(stimes))
             d.lems = C.pointer(self._lems_desc)
         if self.lems_cells is not None:
             LC = self.lems_cells

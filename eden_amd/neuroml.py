@@ -39,6 +39,8 @@ set covers:
                     <ComponentType>s that extend a base synapse type, from their LEMS
                     Dynamics (eden/Eden.cpp:2339-2995)
   inputs            pulseGenerator, poissonFiringSynapse natively (eden/NeuroML.cpp:6968-7107);
+                    timedSynapticInput as an input instance of its synapse's generic type with a
+                    spike list (eden/Eden.cpp:3759-3791, 3859-3912);
                     pulseGeneratorDL, sine and ramp generators (and their DL forms), voltageClamp,
                     voltageClampTriple and user <ComponentType>s extending a base current input
                     as generic LEMS types whose instances sit after a compartment's synapses
@@ -70,6 +72,7 @@ stored as float (eden/NeuroML.cpp:3046-3077, eden/NeuroML.h:530-532).
 Constructs outside this set (other channel and synapse types, artificial cells, other
 inputs) raise NeuroMLError naming the element.
 """
+import dataclasses
 import os
 import re
 import time
@@ -156,13 +159,13 @@ _CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Sy
 # element names of the synapse and input types this reader instantiates, by role; a LEMS
 # <Component type="..."> of these types is read as the element itself
 _SYNAPSES = set(lems_mod.SYNAPSE_TYPES) | {"blockingPlasticSynapse"}
-_INPUTS = {"pulseGenerator", "poissonFiringSynapse"} | set(lems_mod.INPUT_TYPES)
+_INPUTS = {"pulseGenerator", "poissonFiringSynapse", "timedSynapticInput"} | set(lems_mod.INPUT_TYPES)
 # synapse types that may sit on either end of a continuous connection (no spike input)
 _GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction"}
 _UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
 _UNSUPPORTED_INPUTS = {"transientPoissonFiringSynapse"}
 _UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell", "SpikeSourcePoisson", "spikeGeneratorRandom", "spikeGeneratorPoisson",
-                      "spikeGeneratorRefPoisson", "timedSynapticInput"}
+                      "spikeGeneratorRefPoisson"}
 
 
 @dataclass
@@ -959,6 +962,9 @@ def _apply_setting(st, doc, pops, input_lists, projections, input_where, pulses,
             comp_id, items = input_lists[st.group]
             seqs = seqs_of(st.targets, len(items), [x[0] for x in items])
             el = doc.items[comp_id][1]
+            if _tag(el) == "timedSynapticInput":
+                raise NeuroMLError(f"{what}: timedSynapticInput paths are not supported (nor by the reference, "
+                                   "eden/NeuroML.cpp:1746-1749)")
             if comp_id in generic_index:
                 sty = generic_st[generic_index[comp_id]]
                 dim = settable(dict(sty.params), _names(sty.state), f"input {comp_id}")
@@ -1128,43 +1134,50 @@ def read(path, rng_seed=None):
         any(doc.items.get(u, (None, None))[0] == "synapse" and
             (_tag(doc.items[u][1]) != "expTwoSynapse" or points) for u in used)
     generic_index, generic_types, generic_st, custom = {}, [], [], {}
+
+    def generic_synapse(i, el):
+        """(type name, SynapseType, engine values, weight property) of a synapse element."""
+        t = _tag(el)
+        weight = 1.0
+        extra = {}
+        if t == "blockingPlasticSynapse":
+            # composed with its block and plasticity mechanisms (their parameters are the
+            # child elements' attributes), one type per combination
+            mech = {}
+            for ch in el:
+                ct = _tag(ch)
+                if ct not in ("blockMechanism", "plasticityMechanism"):
+                    continue
+                table = lems_mod.BLOCK_MECHANISMS if ct == "blockMechanism" else lems_mod.PLASTICITY_MECHANISMS
+                if ch.get("type") not in table:
+                    raise NeuroMLError(f"synapse {i}: {ct} type '{ch.get('type')}' is not supported")
+                mech[ct] = ch.get("type")
+                for p_, d_ in table[ch.get("type")]["params"]:
+                    if ch.get(p_) is None:
+                        raise NeuroMLError(f"synapse {i}: {ct} parameter {p_} missing")
+                    extra[p_] = convert(ch.get(p_), d_, f"{i} {p_}")
+            t = f"blockingPlasticSynapse/{mech.get('blockMechanism', '')}/{mech.get('plasticityMechanism', '')}"
+            if t not in custom:
+                custom[t] = (lems_mod.blocking_plastic_type(mech.get("blockMechanism"),
+                                                            mech.get("plasticityMechanism")), 1.0)
+            st, weight = custom[t]
+        elif t in lems_mod.SYNAPSE_TYPES:
+            st = lems_mod.SYNAPSE_TYPES[t]
+        else:
+            if t not in custom:
+                try:
+                    custom[t] = lems_mod.synapse_type_from_lems(t, doc.component_types, convert)
+                except lems_mod.LemsError as e:
+                    raise NeuroMLError(str(e)) from None
+            st, weight = custom[t]
+        vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
+        vals.update(extra)
+        return t, st, vals, weight
+
+    timed_times = {}   # timedSynapticInput id -> its spike times (us, listed order)
     if generic:
         for i, el in syn_elems:
-            t = _tag(el)
-            weight = 1.0
-            extra = {}
-            if t == "blockingPlasticSynapse":
-                # composed with its block and plasticity mechanisms (their parameters are the
-                # child elements' attributes), one type per combination
-                mech = {}
-                for ch in el:
-                    ct = _tag(ch)
-                    if ct not in ("blockMechanism", "plasticityMechanism"):
-                        continue
-                    table = lems_mod.BLOCK_MECHANISMS if ct == "blockMechanism" else lems_mod.PLASTICITY_MECHANISMS
-                    if ch.get("type") not in table:
-                        raise NeuroMLError(f"synapse {i}: {ct} type '{ch.get('type')}' is not supported")
-                    mech[ct] = ch.get("type")
-                    for p_, d_ in table[ch.get("type")]["params"]:
-                        if ch.get(p_) is None:
-                            raise NeuroMLError(f"synapse {i}: {ct} parameter {p_} missing")
-                        extra[p_] = convert(ch.get(p_), d_, f"{i} {p_}")
-                t = f"blockingPlasticSynapse/{mech.get('blockMechanism', '')}/{mech.get('plasticityMechanism', '')}"
-                if t not in custom:
-                    custom[t] = (lems_mod.blocking_plastic_type(mech.get("blockMechanism"),
-                                                                mech.get("plasticityMechanism")), 1.0)
-                st, weight = custom[t]
-            elif t in lems_mod.SYNAPSE_TYPES:
-                st = lems_mod.SYNAPSE_TYPES[t]
-            else:
-                if t not in custom:
-                    try:
-                        custom[t] = lems_mod.synapse_type_from_lems(t, doc.component_types, convert)
-                    except lems_mod.LemsError as e:
-                        raise NeuroMLError(str(e)) from None
-                st, weight = custom[t]
-            vals = {p: convert(el.get(p), d, f"{i} {p}") for p, d in st.params if el.get(p) is not None}
-            vals.update(extra)
+            t, st, vals, weight = generic_synapse(i, el)
             generic_index[i] = len(generic_types)
             generic_types.append((t, vals, weight))
             generic_st.append(st)
@@ -1172,6 +1185,23 @@ def read(path, rng_seed=None):
         # compartment's list)
         for i, el in input_elems:
             t = _tag(el)
+            if t == "timedSynapticInput":
+                # a spike list driving its own instance of a synapse type, whose current joins the
+                # input total (eden/Eden.cpp:3859-3912): the synapse's type as an input type
+                sid = el.get("synapse")
+                role, sel = doc.items.get(sid, (None, None))
+                if role != "synapse":
+                    raise NeuroMLError(f"timedSynapticInput {i}: synapse '{sid}' not found or not supported")
+                st_name, st, vals, weight = generic_synapse(sid, sel)
+                key = f"timedSynapticInput/{st_name}"
+                if key not in custom:
+                    custom[key] = (dataclasses.replace(st, is_input=True), weight)
+                timed_times[i] = [_f32(convert(sp.get("time"), "time", f"timedSynapticInput {i} spike"))
+                                  for sp in el if _tag(sp) == "spike"]
+                generic_index[i] = len(generic_types)
+                generic_types.append((key, vals, weight))
+                generic_st.append(custom[key][0])
+                continue
             if t in lems_mod.INPUT_TYPES:
                 st, weight = lems_mod.INPUT_TYPES[t], 1.0
             else:
@@ -1338,6 +1368,7 @@ def read(path, rng_seed=None):
     poisson = [[] for _ in range(nc)]
     poi_count = {}    # (cell, input component) -> instances so far (the RNG stamp's instance)
     input_where = []  # per input: ("conn", index into conns), ("pulse", g, j) or ("poisson", g, j)
+    timed_conn = {}   # index into conns of a timedSynapticInput instance -> its component
     for (iid, g, w) in inputs:
         role, el = doc.items.get(iid, (None, None))
         if role != "input":
@@ -1346,6 +1377,8 @@ def read(path, rng_seed=None):
         t = _tag(el)
         if iid in generic_index:
             # a LEMS input: an instance in the compartment's synapse list
+            if iid in timed_times:
+                timed_conn[len(conns)] = iid
             input_where.append(("conn", len(conns)))
             conns.append((-1, g, generic_index[iid], _f32(w), 0.0, -1))
         elif t == "pulseGenerator":
@@ -1467,6 +1500,18 @@ def read(path, rng_seed=None):
     vthr[used] = vthr_all[used]
 
     lems_tables = lems_mod.build(generic_types, {k: v[0] for k, v in custom.items()}) if generic else None
+    syn_spike_list, spike_times = None, None
+    if timed_conn:
+        # each timedSynapticInput's spike list once, ended by +inf; its instances start there
+        start, times = {}, []
+        for iid in sorted(set(timed_conn.values()), key=list(timed_times).index):
+            start[iid] = len(times)
+            times += timed_times[iid] + [np.inf]
+        syn_spike_list = np.full(len(syn_list), -1, np.int64)
+        for i, k in enumerate(syn_list):
+            if k in timed_conn:
+                syn_spike_list[i] = start[timed_conn[k]]
+        spike_times = np.array(times, np.float32)
     syn_pidx, syn_vecs = None, None
     if syn_over:
         syn_pidx, syn_vecs = np.full(len(syn_list), -1, np.int64), []
@@ -1513,6 +1558,7 @@ def read(path, rng_seed=None):
         record_gate=np.array(rec_gate, np.int32) if any(j >= 0 for j in rec_gate) else None,
         lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
         syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
+        syn_spike_list=syn_spike_list, spike_times=spike_times,
         name=os.path.splitext(os.path.basename(path))[0])
     model.sim_seed = seed
     return model, Outputs(trajectories, events, base_dir)

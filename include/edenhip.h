@@ -119,6 +119,16 @@ typedef struct edenhip_lems_synapses {
                                      type's; NULL = every instance its type's */
     int64_t n_params;             /* (abi_version >= 4) floats in params: the types' vectors, then any
                                      per-instance vectors; 0 = param_begin[n_types] */
+    const int64_t *spike_list;    /* (abi_version >= 4) [n_syn] for a firing-synapse input (the
+                                     reference's timedSynapticInput, eden/Eden.cpp:3759-3791,
+                                     3859-3912, 8840-8852) the offset into spike_times of its spike
+                                     list, else -1; NULL = none. Such an instance takes no incoming
+                                     connections: on each step it receives an event when its next
+                                     listed times fall before t + dt (all of them consumed, the
+                                     position advancing after the initial steps), and its type is an
+                                     input type (type_flags bit 0) */
+    const float *spike_times;     /* spike lists (us, in listed order), each ended by a value >= FLT_MAX */
+    int64_t n_spike_times;
 } edenhip_lems_synapses;
 
 /* LEMS point neurons (the reference's artificial cells: iafCell, izhikevich2007Cell,

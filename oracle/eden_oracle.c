@@ -52,6 +52,7 @@
  *     tests check (integrate-and-fire relaxation, reset and refractory period, one Izhikevich
  *     step by hand, the synapse charges): no reference output fixes their float results.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -240,6 +241,7 @@ typedef struct {
     real *pT[2], *pI[2];    /* Poisson tsince, isi */
     real *pA[2], *pB[2];    /* Poisson synapse A, B */
     float *lS[2];           /* generic LEMS synapse state [k][synapse] */
+    int64_t *tpos[2];       /* firing-synapse inputs: position in the spike list, or -1 */
     float *cS[2];           /* LEMS point-neuron state [k][slot] */
     int32_t *cslot;         /* per compartment: its point-neuron slot, or -1 */
     int64_t n_cslots;
@@ -343,25 +345,40 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                 }
                 real weight = (real)d->syn_weight[sy];
                 int spike_in_flag = 0;
-                if (!initial_state) {
-                    spike_in_flag = !!s->trig[now][sy];
-                    s->trig[now][sy] = 0;
-                }
-                real delay = (real)d->syn_delay[sy];
-                real next_spike = s->nspk[now][sy];
-                real next_next_spike = next_spike;
                 int spike_now = 0;
-                if (!initial_state) {
-                    if (time_r <= next_spike && next_spike < time_r + dt) spike_now = 1;
-                    if (spike_in_flag) {
-                        real fresh_spike = time_r + delay;
-                        if (time_r <= fresh_spike && fresh_spike < time_r + dt) spike_now = 1;
-                        if (next_next_spike < time_r + dt && next_next_spike < fresh_spike) next_next_spike = fresh_spike;
+                if (s->tpos[0] && s->tpos[now][sy] >= 0) {
+                    /* a firing-synapse input (timedSynapticInput): the spiker fires when its next
+                       listed times fall before the end of the step, consuming them all; its event
+                       reaches the synapse in the same step; the position advances only after
+                       the initial steps (eden/Eden.cpp:3759-3791, 3859-3912) */
+                    const float *times = d->lems->spike_times;
+                    int64_t pos = s->tpos[now][sy];
+                    int fired = 0;
+                    while (times[pos] < (float)time_r + (float)dt) { fired = 1; pos++; }
+                    s->tpos[nxt][sy] = initial_state ? s->tpos[now][sy] : pos;
+                    /* (the generated step ignores events during the initial steps) */
+                    spike_now = initial_state ? 0 : fired;
+                    s->nspk[nxt][sy] = s->nspk[now][sy];
+                } else {
+                    if (!initial_state) {
+                        spike_in_flag = !!s->trig[now][sy];
+                        s->trig[now][sy] = 0;
                     }
+                    real delay = (real)d->syn_delay[sy];
+                    real next_spike = s->nspk[now][sy];
+                    real next_next_spike = next_spike;
+                    if (!initial_state) {
+                        if (time_r <= next_spike && next_spike < time_r + dt) spike_now = 1;
+                        if (spike_in_flag) {
+                            real fresh_spike = time_r + delay;
+                            if (time_r <= fresh_spike && fresh_spike < time_r + dt) spike_now = 1;
+                            if (next_next_spike < time_r + dt && next_next_spike < fresh_spike) next_next_spike = fresh_spike;
+                        }
+                    }
+                    s->nspk[nxt][sy] = next_next_spike;
                 }
                 spike_in_flag = spike_now;
                 if (spike_now) log_syn_event(sy, step);
-                s->nspk[nxt][sy] = next_next_spike;
                 if (s->lS[0]) {
                     /* a generic LEMS synapse: its type's step function */
                     const edenhip_lems_synapses *L = d->lems;
@@ -637,6 +654,17 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
         int32_t smax = 1;
         for (int32_t t = 0; t < d->lems->n_types; t++) smax = d->lems->n_state[t] > smax ? d->lems->n_state[t] : smax;
         for (int b = 0; b < 2; b++) ALLOC(s.lS[b], (int64_t)smax * (NSY > 0 ? NSY : 1), float);
+        if (d->abi_version >= 4 && d->lems->spike_list) {
+            const int64_t nt = d->lems->n_spike_times;
+            if (!d->lems->spike_times || nt < 1 || !(d->lems->spike_times[nt - 1] >= FLT_MAX)) return -1;
+            for (int b = 0; b < 2; b++) {
+                ALLOC(s.tpos[b], NSY, int64_t);
+                for (int64_t k = 0; k < NSY; k++) {
+                    if (d->lems->spike_list[k] < -1 || d->lems->spike_list[k] >= nt) return -1;
+                    s.tpos[b][k] = d->lems->spike_list[k];
+                }
+            }
+        }
     }
     for (int32_t c = 0; c < d->n_cells; c++) {
         int64_t c0 = d->cell_comp_begin[c];
@@ -716,6 +744,7 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     }
     free(s.D); free(s.order); free(s.gate_begin); free(s.comp_ev_begin); free(s.comp_ev);
     free(s.lS[0]); free(s.lS[1]);
+    free(s.tpos[0]); free(s.tpos[1]);
     free(s.cS[0]); free(s.cS[1]); free(s.cslot);
     return rows;
 }

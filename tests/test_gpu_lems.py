@@ -129,8 +129,8 @@ def test_user_defined_component_types(tmp_path):
 
 def test_reference_core_inputs_files(tmp_path):
     """The reference's CoreInputs model (tests/coreinputs.py): LEMS current inputs (sine, ramp,
-    voltage clamps, dimensionless generators on Izhikevich cells) and spike sources; output
-    files byte-equal to the oracle's, trace and raster equal."""
+    voltage clamps, dimensionless generators on Izhikevich cells), its timedSynapticInput and
+    spike sources; output files byte-equal to the oracle's, trace and raster equal."""
     from coreinputs import write_model
     path = write_model(str(tmp_path))
     res = run_lems(path, out_dir=str(tmp_path))
@@ -191,3 +191,23 @@ def test_lems_inputs_beside_native_inputs(tmp_path, where):
     tr, er, ost = _check(g2)
     base = _check(dataclasses.replace(m, lems=lems.build(types[:n_syn_types]), name="no_inputs"))[0]
     assert not np.array_equal(tr, base)
+
+
+@pytest.mark.parametrize("opts", [{}, dict(use_graph=True)])
+def test_timed_synaptic_input(tmp_path, opts):
+    """timedSynapticInput: spike lists firing their synapse instances in the same step, the
+    position state advancing per step (two cells, one list with two times in one step):
+    traces and the synaptic event log bit for bit, in the plain and graph launch modes, and
+    after a reset the same run again."""
+    from test_lems_cpu import _timed_model
+    m = _timed_model(tmp_path, [0.003, 0.0102, 1.0, 2.001, 2.004, 3.5], both=True)
+    _say(f"[lems] timed input, opts {opts}")
+    (rt, ec, er, _), (ost, osy) = oracle.run_with_synapse_log(m)
+    with Simulation(m, log_synapse_events=1 << 16, **opts) as sim:
+        tr = sim.run_all()
+        st, sy = sim.read_synapse_events()
+        sim.reset()
+        tr2 = sim.run_all()
+    assert np.array_equal(tr, rt) and np.array_equal(tr2, rt)
+    assert np.array_equal(st, ost) and np.array_equal(sy, osy)
+    assert sorted(st[sy == 0].tolist()) == [1, 2, 101, 201, 351]

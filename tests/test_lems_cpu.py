@@ -480,21 +480,24 @@ def test_voltage_clamp_holds_a_passive_cell_near_its_target(tmp_path):
 
 
 def test_reference_core_inputs_and_custom_input_models_read(tmp_path):
-    """The reference's CoreInputs model (without its timedSynapticInput): one LEMS input type
-    per generator element, flagged as inputs, each instance after the synapse types in its
-    compartment's list; EdenTest4's user-defined myPulseGenerator reads as an input type and
-    its tree cells carry the instances; both run in the oracle."""
+    """The reference's CoreInputs model: one LEMS input type per generator element, flagged as
+    inputs, each instance after the synapse types in its compartment's list, and the
+    timedSynapticInput as an input instance of its expOneSynapse with the spike list;
+    EdenTest4's user-defined myPulseGenerator reads as an input type and its tree cells carry
+    the instances; both run in the oracle."""
     from coreinputs import write_model
     m, _ = neuroml.read(write_model(str(tmp_path)))
     assert m.lems.type_names == ["expOneSynapse", "expTwoSynapse", "pulseGeneratorDL", "sineGenerator",
                                  "sineGeneratorDL", "rampGenerator", "rampGeneratorDL", "voltageClamp",
-                                 "voltageClampTriple"]
-    assert m.lems.type_flags == [0, 0, 1, 1, 1, 1, 1, 1, 1]
+                                 "voltageClampTriple", "timedSynapticInput/expOneSynapse"]
+    assert m.lems.type_flags == [0, 0, 1, 1, 1, 1, 1, 1, 1, 1]
     comp_of = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
-    # InputDemos/02-05 (compartments 2-5): sine, ramp, clamp, triple clamp; 11, 12: the spike
-    # sources' synapses; InputDemosDL/01-03: the dimensionless generators
-    assert list(comp_of) == [2, 3, 4, 5, 11, 12, 18, 19, 20]
-    assert list(m.syn_type) == [3, 5, 7, 8, 0, 0, 2, 4, 6]
+    # InputDemos/02-06 (compartments 2-6): sine, ramp, clamp, triple clamp, the timed synaptic
+    # input; 11, 12: the spike sources' synapses; InputDemosDL/01-03: the dimensionless generators
+    assert list(comp_of) == [2, 3, 4, 5, 6, 11, 12, 18, 19, 20]
+    assert list(m.syn_type) == [3, 5, 7, 8, 9, 0, 0, 2, 4, 6]
+    assert list(m.syn_spike_list) == [-1, -1, -1, -1, 0, -1, -1, -1, -1, -1]
+    assert m.spike_times[-1] == np.inf and len(m.spike_times) == 17
     tr, ec, er, _ = oracle.run(m)
     assert np.bincount(ec).tolist() == [5, 16]
     m4, _ = neuroml.read(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "neuroml",
@@ -503,3 +506,58 @@ def test_reference_core_inputs_and_custom_input_models_read(tmp_path):
     tr4, _, _, _ = oracle.run(m4)
     assert np.array_equal(tr4[:, 0], tr4[:, 1])                 # a native pulse and ... the same pulse
     assert float(np.ptp(tr4[:, 4])) > 0.5
+
+
+def _timed_model(tmp_path, times_ms, dt_ms=0.01, length_ms=5.0, both=False):
+    """One passive cell driven by a timedSynapticInput of an expOneSynapse (and, with both,
+    a second cell whose pulse current equals nothing, to check the input total's order)."""
+    spikes = "".join(f'<spike id="{k}" time="{t} ms"/>' for k, t in enumerate(times_ms))
+    pas = ('<cell id="P"><morphology id="m"><segment id="0"><proximal x="0" y="0" z="0" diameter="10"/>'
+           '<distal x="0" y="0" z="0" diameter="10"/></segment></morphology><biophysicalProperties id="b">'
+           '<membraneProperties><channelDensity id="leak" ionChannel="passiveChan" condDensity="0.3 mS_per_cm2" '
+           'erev="-65mV"/><specificCapacitance value="1.0 uF_per_cm2"/><initMembPotential value="-65mV"/>'
+           '</membraneProperties><intracellularProperties><resistivity value="0.1 kohm_cm"/>'
+           '</intracellularProperties></biophysicalProperties></cell><ionChannelHH id="passiveChan" conductance="10pS"/>')
+    p = tmp_path / "timed.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml>' + pas +
+        '<expOneSynapse id="e1" gbase="2nS" erev="0mV" tauDecay="1ms"/>'
+        f'<timedSynapticInput id="ti" synapse="e1" spikeTarget="./e1">{spikes}</timedSynapticInput>'
+        '<network id="n"><population id="pp" component="P" size="2"/>'
+        '<inputList id="L" component="ti" population="pp"><input id="0" target="../pp/0/P" destination="synapses"/>'
+        + ('<input id="1" target="../pp/1/P" destination="synapses"/>' if both else '') +
+        '</inputList></network></neuroml>'
+        f'<Simulation id="s" length="{length_ms}ms" step="{dt_ms}ms" target="n" seed="1">'
+        '<OutputFile id="o" fileName="o.txt"><OutputColumn id="a" quantity="pp/0/P/0/v"/>'
+        '<OutputColumn id="b" quantity="pp/1/P/0/v"/></OutputFile></Simulation></Lems>')
+    return neuroml.read(str(p))[0]
+
+
+def test_timed_synaptic_input_fires_on_the_listed_steps(tmp_path):
+    """The spike list fires on every step whose end passes its next listed times, all of them
+    consumed at once (eden/Eden.cpp:3780-3788), and the event reaches the synapse in that step:
+    time T on step floor(T / dt) + 1 (t = (step - 1) dt, fired when t + dt > T); two times in
+    one step give one event; a time equal to a step's end fires on the next step."""
+    times = [0.003, 0.0102, 1.0, 2.001, 2.004, 3.5]
+    m = _timed_model(tmp_path, times, both=True)
+    assert list(m.syn_spike_list) == [0, 0] and list(m.spike_times[:6]) == [np.float32(t * 1000) for t in times]
+    (tr, _, _, _), (st, sy) = oracle.run_with_synapse_log(m)
+    for inst in (0, 1):
+        steps = sorted(int(x) for x in st[sy == inst])
+        assert steps == [1, 2, 101, 201, 351], (inst, steps)
+    # expOneSynapse exposes its pre-event conductance: the event step (row 1) adds no current,
+    # the next step depolarises the cell from rest
+    assert tr[1, 0] == np.float32(-65.0) and tr[2, 0] > np.float32(-65.0)
+    assert np.array_equal(tr[:, 0], tr[:, 1])
+
+
+def test_timed_synaptic_input_refuses_custom_settings(tmp_path):
+    """The reference has no per-instance paths into a timedSynapticInput
+    (eden/NeuroML.cpp:1746-1749); neither does the reader."""
+    m = _timed_model(tmp_path, [1.0])
+    src = (tmp_path / "timed.xml").read_text().replace("</Simulation>", '<EdenCustomSetup filename="c.txt"/></Simulation>')
+    (tmp_path / "timed.xml").write_text(src)
+    (tmp_path / "c.txt").write_text("set input L all gbase 1 nS\n")
+    with pytest.raises(neuroml.NeuroMLError, match="timedSynapticInput paths"):
+        neuroml.read(str(tmp_path / "timed.xml"))
+    assert m.lems.type_flags == [0, 1]
