@@ -42,7 +42,10 @@ _P = C.c_void_p
 class LemsSynapsesDesc(C.Structure):
     _fields_ = [("n_types", C.c_int32), ("source", C.c_char_p), ("param_begin", _P), ("params", _P),
                 ("n_state", _P), ("peer", _P), ("type_flags", _P), ("instance_param", _P),
-                ("n_params", C.c_int64), ("spike_list", _P), ("spike_times", _P), ("n_spike_times", C.c_int64)]
+                ("n_params", C.c_int64), ("spike_list", _P), ("spike_times", _P), ("n_spike_times", C.c_int64),
+                ("poisson_spiker", _P), ("n_poisson_spikers", C.c_int64), ("spiker_rate", _P),
+                ("spiker_transient", _P), ("spiker_delay", _P), ("spiker_duration", _P), ("spiker_instance", _P),
+                ("spiker_rng_offset", _P)]
 
 
 class LemsCellsDesc(C.Structure):

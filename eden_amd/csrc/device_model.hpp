@@ -115,6 +115,13 @@ struct LemsArgs {
     const int32_t *poff;          // per slot: offset of its parameter vector in params (null: its type's)
     int32_t *tpos;                // per slot: a firing-synapse input's position in ttimes, or -1 (null: none)
     const float *ttimes;          // spike lists, each ended by a value >= FLT_MAX
+    const int32_t *pspk;          // per slot: a Poisson spiker's index, or -1 (null: none)
+    float2 *pk_state;             // per spiker: (tsince, isi), updated in place
+    const float *pk_isi;          // per spiker: averageIsi = 1 / averageRate (us)
+    const float2 *pk_window;      // per spiker: (delay, duration) of a transient spiker
+    const int32_t *pk_transient;  // per spiker: 1 transient
+    const int32_t *pk_inst;       // per spiker: randof instance
+    const int32_t *pk_roff;       // per spiker: randof invocation offset
     float2 *inp_ext;              // [NC] per-compartment (I, G) of the LEMS inputs of the step, or null
 };
 

@@ -154,6 +154,12 @@ struct edenhip_ctx {
     int32_t *d_lems_poff = nullptr;      // per synapse slot: offset of its parameter vector (per-instance settings)
     int32_t *d_lems_tpos = nullptr;      // per synapse slot: a firing-synapse input's spike-list position, or -1
     float *d_lems_ttimes = nullptr;      // firing-synapse inputs' spike lists
+    int32_t *d_pk_slot = nullptr;        // per synapse slot: its Poisson spiker, or -1
+    float2 *d_pk_state = nullptr;        // per Poisson spiker: (tsince, isi)
+    float *d_pk_isi = nullptr;
+    float2 *d_pk_window = nullptr;
+    int32_t *d_pk_transient = nullptr, *d_pk_inst = nullptr, *d_pk_roff = nullptr;
+    int64_t n_pk = 0;
     int32_t *d_acell_poff = nullptr;     // per point neuron: offset of its parameter vector (per-instance settings)
     float2 *d_inp_ext = nullptr;         // per compartment: the LEMS inputs' (I, G) of the step
     // LEMS point neurons: the run-time compiled cell kernel and its tables (one slot per cell)
@@ -339,6 +345,31 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                 const int64_t tgt = d->spike_target[k];
                 if (tgt >= 0 && tgt < nsyn && LS->spike_list[tgt] >= 0)
                     return fail(ctx, EDENHIP_E_INVALID, "lems: a firing-synapse input takes no incoming connections");
+            }
+        }
+        if (d->abi_version >= 4 && LS->poisson_spiker) {
+            // Poisson spikers: tables present, indices in range, on instances of input types
+            // without a spike list that take no incoming connections
+            const int64_t nsyn = d->comp_syn_begin[d->n_comps], n = LS->n_poisson_spikers;
+            if (n < 0 || n >= (int64_t)1 << 31) return fail(ctx, EDENHIP_E_LIMIT, "lems: poisson spiker count");
+            if (n > 0 && (!LS->spiker_rate || !LS->spiker_transient || !LS->spiker_delay || !LS->spiker_duration ||
+                          !LS->spiker_instance || !LS->spiker_rng_offset))
+                return fail(ctx, EDENHIP_E_INVALID, "lems: poisson spiker table missing");
+            for (int64_t k = 0; k < nsyn; k++) {
+                const int64_t o = LS->poisson_spiker[k];
+                if (o < -1 || o >= n) return fail(ctx, EDENHIP_E_INVALID, "lems: poisson_spiker out of range");
+                if (o < 0) continue;
+                const int32_t ty = d->syn_type[k];
+                if (ty < 0 || ty >= LS->n_types || !LS->type_flags || !(LS->type_flags[ty] & 1))
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: a Poisson spiker's instance must be of an input type");
+                if (LS->spike_list && LS->spike_list[k] >= 0)
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: an instance has both a spike list and a Poisson spiker");
+            }
+            const int64_t nspk = d->comp_spike_begin[d->n_comps];
+            for (int64_t k = 0; k < nspk; k++) {
+                const int64_t tgt = d->spike_target[k];
+                if (tgt >= 0 && tgt < nsyn && LS->poisson_spiker[tgt] >= 0)
+                    return fail(ctx, EDENHIP_E_INVALID, "lems: a Poisson spiker's instance takes no incoming connections");
             }
         }
         if (d->abi_version >= 3 && LS->peer) {
@@ -913,6 +944,28 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                 TRY(upload(ctx, &ctx->d_lems_ttimes, tt));
             }
         }
+        if (d->abi_version >= 4 && LS->poisson_spiker && LS->n_poisson_spikers > 0) {
+            const int64_t n = LS->n_poisson_spikers;
+            std::vector<int32_t> slot_pk((size_t)n_slots, -1);
+            for (int64_t k = 0; k < n_slots; k++) {
+                const int32_t s = syn_orig[(size_t)k];
+                if (s >= 0 && LS->poisson_spiker[s] >= 0) slot_pk[(size_t)k] = (int32_t)LS->poisson_spiker[s];
+            }
+            std::vector<float> isi((size_t)n);
+            std::vector<float2> win((size_t)n);
+            for (int64_t k = 0; k < n; k++) {
+                isi[(size_t)k] = 1.0f / LS->spiker_rate[k];   // averageIsi = 1 / averageRate
+                win[(size_t)k] = make_float2(LS->spiker_delay[k], LS->spiker_duration[k]);
+            }
+            TRY(upload(ctx, &ctx->d_pk_slot, slot_pk));
+            TRY(upload(ctx, &ctx->d_pk_isi, isi));
+            TRY(upload(ctx, &ctx->d_pk_window, win));
+            TRY(upload(ctx, &ctx->d_pk_transient, std::vector<int32_t>(LS->spiker_transient, LS->spiker_transient + n)));
+            TRY(upload(ctx, &ctx->d_pk_inst, std::vector<int32_t>(LS->spiker_instance, LS->spiker_instance + n)));
+            TRY(upload(ctx, &ctx->d_pk_roff, std::vector<int32_t>(LS->spiker_rng_offset, LS->spiker_rng_offset + n)));
+            TRY(alloc_zero(ctx, &ctx->d_pk_state, (size_t)n));
+            ctx->n_pk = n;
+        }
         TRY(upload(ctx, &ctx->d_lems_comps, lc));
         TRY(upload(ctx, &ctx->d_lems_params, prm));
         TRY(upload(ctx, &ctx->d_lems_pbegin, pb));
@@ -1024,6 +1077,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(snap(ctx->d_lems_state, (size_t)(smax * n_slots) * sizeof(float)));
     }
     if (ctx->d_lems_tpos) TRY(snap(ctx->d_lems_tpos, (size_t)n_slots * sizeof(int32_t)));
+    if (ctx->d_pk_state) TRY(snap(ctx->d_pk_state, (size_t)ctx->n_pk * sizeof(float2)));
     if (ctx->d_acell_state) {
         int32_t smax = 1;
         for (int t = 0; t < LC->n_types; t++) smax = std::max(smax, LC->n_state[t]);
@@ -1380,7 +1434,8 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 // generic synapses of step a: per-compartment currents for the kernels below
                 LemsArgs la{ctx->d_lems_comps, ctx->n_lems_comps, ctx->d_lems_params, ctx->d_lems_pbegin, ctx->d_lems_state,
                      ctx->n_slots, ctx->d_syn_ext, ctx->d_lems_peer, ctx->d_lems_flags, ctx->d_lems_poff,
-                     ctx->d_lems_tpos, ctx->d_lems_ttimes, ctx->d_inp_ext};
+                     ctx->d_lems_tpos, ctx->d_lems_ttimes, ctx->d_pk_slot, ctx->d_pk_state, ctx->d_pk_isi,
+                     ctx->d_pk_window, ctx->d_pk_transient, ctx->d_pk_inst, ctx->d_pk_roff, ctx->d_inp_ext};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &la, &st, &dt};

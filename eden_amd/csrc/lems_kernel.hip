@@ -75,6 +75,33 @@ lems_syn_kernel(edenhip::DevTables T, edenhip::LemsArgs L, int64_t step, float d
                 L.tpos[s] = pos;
                 spike_now = fired;
             }
+        } else if (L.pspk && L.pspk[s] >= 0) {
+            // a Poisson firing-synapse input's spiker (poissonFiringSynapse,
+            // transientPoissonFiringSynapse): tsince' = 1; when tsince > isi, tsince = 0, a new
+            // interval and a spike, the synapse's event of the same step
+            const int32_t k = L.pspk[s];
+            const float averageIsi = L.pk_isi[k];
+            const int32_t seed = (int32_t)T.cell_seed[T.comp_cell[g]], inst = L.pk_inst[k], roff = L.pk_roff[k];
+            const bool transient = L.pk_transient[k] != 0;
+            const float2 win = L.pk_window[k];
+            const float2 st = L.pk_state[k];
+            float tn, in;
+            if (init) {
+                tn = 0.0f;
+                const float nextIsi = (-averageIsi) * log_f(1.0f - randof(1.0f, seed, inst, step, roff + 0));
+                in = transient ? nextIsi + win.x : nextIsi;
+            } else {
+                tn = st.x + dt * 1.0f;
+                in = st.y;
+                if (st.x > st.y) {
+                    tn = 0.0f;
+                    const float nextIsi = (-averageIsi) * log_f(1.0f - randof(1.0f, seed, inst, step, roff + 1));
+                    // after the window, the next interval is pushed 1e9 hours (3.6e18 us) away
+                    in = transient ? nextIsi + stepf(((t + nextIsi) - (win.x + win.y)) / win.y) * 3.6e18f : nextIsi;
+                    spike_now = true;
+                }
+            }
+            L.pk_state[k] = make_float2(tn, in);
         } else {
             bool flag = false;
             if (!init) {

@@ -121,6 +121,11 @@ class Model:
     # list in spike_times, or -1; the lists' times (us), each ended by +inf
     syn_spike_list: np.ndarray = None
     spike_times: np.ndarray = None
+    # Poisson firing-synapse inputs on the generic path: per synapse instance an index into the
+    # spiker table, or -1; the table: dict of equal-length arrays rate (1/us), transient (0/1),
+    # delay, duration (us), instance, rng_offset
+    syn_poisson_spiker: np.ndarray = None
+    poisson_spikers: dict = None
     name: str = "model"
 
     def __post_init__(self):
@@ -268,11 +273,26 @@ class Model:
                 stimes = np.ascontiguousarray(self.spike_times, np.float32)
                 if sl.shape != (self.n_synapses,):
                     raise ValueError("syn_spike_list: one entry per synapse instance")
-            self._lems_spikes = (sl, stimes)
+            ps, pk = None, {}
+            if self.syn_poisson_spiker is not None:
+                ps = np.ascontiguousarray(self.syn_poisson_spiker, np.int64)
+                if ps.shape != (self.n_synapses,):
+                    raise ValueError("syn_poisson_spiker: one entry per synapse instance")
+                P = self.poisson_spikers
+                pk = {k: np.ascontiguousarray(P[k], dt) for k, dt in
+                      (("rate", np.float32), ("transient", np.int32), ("delay", np.float32), ("duration", np.float32),
+                       ("instance", np.int32), ("rng_offset", np.int32))}
+                if len({len(v) for v in pk.values()}) != 1:
+                    raise ValueError("poisson_spikers: arrays of equal length")
+            self._lems_spikes = (sl, stimes, ps, pk)
             self._lems_desc = N.LemsSynapsesDesc(len(L.params), self._lems_arrays[3], N.ptr(self._lems_arrays[0]),
                                                  N.ptr(self._lems_arrays[1]), N.ptr(self._lems_arrays[2]),
                                                  N.ptr(peer), N.ptr(self._lems_flags), N.ptr(off), len(prm),
-                                                 N.ptr(sl), N.ptr(stimes), 0 if stimes is None else len(stimes))
+                                                 N.ptr(sl), N.ptr(stimes), 0 if stimes is None else len(stimes),
+                                                 N.ptr(ps), len(pk.get("rate", ())), N.ptr(pk.get("rate")),
+                                                 N.ptr(pk.get("transient")), N.ptr(pk.get("delay")),
+                                                 N.ptr(pk.get("duration")), N.ptr(pk.get("instance")),
+                                                 N.ptr(pk.get("rng_offset")))
             d.lems = C.pointer(self._lems_desc)
         if self.lems_cells is not None:
             LC = self.lems_cells

@@ -39,8 +39,10 @@ set covers:
                     <ComponentType>s that extend a base synapse type, from their LEMS
                     Dynamics (eden/Eden.cpp:2339-2995)
   inputs            pulseGenerator, poissonFiringSynapse natively (eden/NeuroML.cpp:6968-7107);
-                    timedSynapticInput as an input instance of its synapse's generic type with a
-                    spike list (eden/Eden.cpp:3759-3791, 3859-3912);
+                    timedSynapticInput, poissonFiringSynapse of a synapse other than
+                    expTwoSynapse and transientPoissonFiringSynapse as input instances of their
+                    synapse's generic type with a spike list or a Poisson spiker
+                    (eden/Eden.cpp:3759-3791, 3859-3912);
                     pulseGeneratorDL, sine and ramp generators (and their DL forms), voltageClamp,
                     voltageClampTriple and user <ComponentType>s extending a base current input
                     as generic LEMS types whose instances sit after a compartment's synapses
@@ -159,11 +161,14 @@ _CORE_INCLUDES = {"Cells.xml", "Networks.xml", "Simulation.xml", "PyNN.xml", "Sy
 # element names of the synapse and input types this reader instantiates, by role; a LEMS
 # <Component type="..."> of these types is read as the element itself
 _SYNAPSES = set(lems_mod.SYNAPSE_TYPES) | {"blockingPlasticSynapse"}
-_INPUTS = {"pulseGenerator", "poissonFiringSynapse", "timedSynapticInput"} | set(lems_mod.INPUT_TYPES)
+_INPUTS = {"pulseGenerator", "poissonFiringSynapse", "transientPoissonFiringSynapse", "timedSynapticInput"} | \
+    set(lems_mod.INPUT_TYPES)
+# inputs that drive their own instance of a synapse (eden/Eden.cpp:3859-3912)
+_FIRING_INPUTS = {"poissonFiringSynapse", "transientPoissonFiringSynapse", "timedSynapticInput"}
 # synapse types that may sit on either end of a continuous connection (no spike input)
 _GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction"}
 _UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
-_UNSUPPORTED_INPUTS = {"transientPoissonFiringSynapse"}
+_UNSUPPORTED_INPUTS = set()
 _UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell", "SpikeSourcePoisson", "spikeGeneratorRandom", "spikeGeneratorPoisson",
                       "spikeGeneratorRefPoisson"}
 
@@ -913,7 +918,7 @@ def _value_converter(dim, what):
 
 
 def _apply_setting(st, doc, pops, input_lists, projections, input_where, pulses, poisson, generic_index, generic_st,
-                   syn_over, cell_over):
+                   syn_over, cell_over, spiker_over):
     """One EdenCustomSetup statement (eden/Eden.cpp:9501-9650): point-neuron parameters per
     cell, input parameters per input of a list (native pulse and Poisson tables directly,
     generic inputs per instance), synapse parameters per connection end."""
@@ -930,12 +935,13 @@ def _apply_setting(st, doc, pops, input_lists, projections, input_where, pulses,
             out.append(pos[x])
         return out
 
-    def settable(names_dims, state_names, owner):
-        if st.param not in names_dims:
-            if st.param in state_names:
-                raise NeuroMLError(f"{what}: {st.param} is a state variable of {owner}; only parameters can be set")
-            raise NeuroMLError(f"{what}: {owner} has no parameter {st.param}")
-        return names_dims[st.param]
+    def settable(names_dims, state_names, owner, name=None):
+        name = st.param if name is None else name
+        if name not in names_dims:
+            if name in state_names:
+                raise NeuroMLError(f"{what}: {name} is a state variable of {owner}; only parameters can be set")
+            raise NeuroMLError(f"{what}: {owner} has no parameter {name}")
+        return names_dims[name]
 
     try:
         if st.kind == "cell":
@@ -962,21 +968,41 @@ def _apply_setting(st, doc, pops, input_lists, projections, input_where, pulses,
             comp_id, items = input_lists[st.group]
             seqs = seqs_of(st.targets, len(items), [x[0] for x in items])
             el = doc.items[comp_id][1]
-            if _tag(el) == "timedSynapticInput":
-                raise NeuroMLError(f"{what}: timedSynapticInput paths are not supported (nor by the reference, "
-                                   "eden/NeuroML.cpp:1746-1749)")
-            if comp_id in generic_index:
+            tag = _tag(el)
+            # paths into an input (eden/NeuroML.cpp:1700-1752): "synapse/<parameter>" for the
+            # inputs that drive a synapse, the spiker's parameters, a LEMS input's parameters,
+            # a pulse's delay, duration and amplitude; none into a spike list
+            key = st.param
+            if tag in _FIRING_INPUTS and st.param.startswith("synapse/"):
+                key = st.param[len("synapse/"):]
+                if comp_id not in generic_index:
+                    raise NeuroMLError(f"{what}: the synapse of a natively run poissonFiringSynapse (an expTwoSynapse) "
+                                       "has no per-instance parameters here")
+                sty = generic_st[generic_index[comp_id]]
+                dim = settable(dict(sty.params), _names(sty.state), f"the synapse of input {comp_id}", key)
+                where = "syn"
+            elif tag == "timedSynapticInput":
+                raise NeuroMLError(f"{what}: timedSynapticInput paths other than synapse/... are not supported (nor "
+                                   "by the reference, eden/NeuroML.cpp:1746-1749)")
+            elif tag in _FIRING_INPUTS and comp_id in generic_index:
+                table = {"averageRate": "frequency"}
+                if tag == "transientPoissonFiringSynapse":
+                    table.update(delay="time", duration="time")
+                dim = settable(table, ["tsince", "isi", "nextIsi"], f"input {comp_id}")
+                where = "spiker"
+            elif comp_id in generic_index:
                 sty = generic_st[generic_index[comp_id]]
                 dim = settable(dict(sty.params), _names(sty.state), f"input {comp_id}")
-            elif _tag(el) == "pulseGenerator":
+                where = "syn"
+            elif tag == "pulseGenerator":
                 dim = settable({"delay": "time", "duration": "time", "amplitude": "current"}, [], f"input {comp_id}")
             else:
-                dim = settable({"averageRate": "frequency"}, [], f"input {comp_id}")
+                dim = settable({"averageRate": "frequency"}, ["tsince", "isi"], f"input {comp_id}")
             vals = setup_mod.values_of(st, len(seqs), _value_converter(dim, what))
             for s, v in zip(seqs, vals):
                 w = input_where[items[s][1]]
                 if w[0] == "conn":
-                    syn_over.setdefault(w[1], {})[st.param] = v
+                    (syn_over if where == "syn" else spiker_over).setdefault(w[1], {})[key] = v
                 elif w[0] == "pulse":
                     g, j = w[1], w[2]
                     k = ("delay", "duration", "amplitude").index(st.param)
@@ -1124,8 +1150,15 @@ def read(path, rng_seed=None):
             used |= {c.get(k) for c in el for k in ("preComponent", "postComponent") if c.get(k)}
     # (point neurons take their synapses through the generic path, and so do LEMS inputs)
     points = any(p.cell.point is not None for p in pops.values())
+    def native_poisson(el):
+        """A poissonFiringSynapse of an expTwoSynapse runs natively; others take the generic path."""
+        if _tag(el) != "poissonFiringSynapse":
+            return False
+        return _tag(doc.items.get(el.get("synapse"), (None, ET.Element("none")))[1]) == "expTwoSynapse"
+
     input_elems = [(i, doc.items[i][1]) for role, i in doc.order
-                   if role == "input" and _tag(doc.items[i][1]) not in ("pulseGenerator", "poissonFiringSynapse")]
+                   if role == "input" and _tag(doc.items[i][1]) != "pulseGenerator" and
+                   not native_poisson(doc.items[i][1])]
     used_inputs = {el.get("input") for el in net if _tag(el) == "explicitInput"} | \
                   {el.get("component") for el in net if _tag(el) == "inputList"}
     lems_inputs = any(i in used_inputs for i, _ in input_elems)
@@ -1175,6 +1208,7 @@ def read(path, rng_seed=None):
         return t, st, vals, weight
 
     timed_times = {}   # timedSynapticInput id -> its spike times (us, listed order)
+    spikers = {}       # generic Poisson input id -> (averageRate 1/us, transient, delay us, duration us)
     if generic:
         for i, el in syn_elems:
             t, st, vals, weight = generic_synapse(i, el)
@@ -1185,19 +1219,25 @@ def read(path, rng_seed=None):
         # compartment's list)
         for i, el in input_elems:
             t = _tag(el)
-            if t == "timedSynapticInput":
-                # a spike list driving its own instance of a synapse type, whose current joins the
+            if t in _FIRING_INPUTS:
+                # a spiker driving its own instance of a synapse type, whose current joins the
                 # input total (eden/Eden.cpp:3859-3912): the synapse's type as an input type
                 sid = el.get("synapse")
                 role, sel = doc.items.get(sid, (None, None))
                 if role != "synapse":
-                    raise NeuroMLError(f"timedSynapticInput {i}: synapse '{sid}' not found or not supported")
+                    raise NeuroMLError(f"{t} {i}: synapse '{sid}' not found or not supported")
                 st_name, st, vals, weight = generic_synapse(sid, sel)
-                key = f"timedSynapticInput/{st_name}"
+                key = f"{t}/{st_name}"
                 if key not in custom:
                     custom[key] = (dataclasses.replace(st, is_input=True), weight)
-                timed_times[i] = [_f32(convert(sp.get("time"), "time", f"timedSynapticInput {i} spike"))
-                                  for sp in el if _tag(sp) == "spike"]
+                if t == "timedSynapticInput":
+                    timed_times[i] = [_f32(convert(sp.get("time"), "time", f"timedSynapticInput {i} spike"))
+                                      for sp in el if _tag(sp) == "spike"]
+                else:
+                    tr_ = t == "transientPoissonFiringSynapse"
+                    spikers[i] = (_f32(convert(el.get("averageRate"), "frequency", f"{i} averageRate")), int(tr_),
+                                  _f32(convert(el.get("delay"), "time", f"{i} delay")) if tr_ else 0.0,
+                                  _f32(convert(el.get("duration"), "time", f"{i} duration")) if tr_ else 0.0)
                 generic_index[i] = len(generic_types)
                 generic_types.append((key, vals, weight))
                 generic_st.append(custom[key][0])
@@ -1369,6 +1409,7 @@ def read(path, rng_seed=None):
     poi_count = {}    # (cell, input component) -> instances so far (the RNG stamp's instance)
     input_where = []  # per input: ("conn", index into conns), ("pulse", g, j) or ("poisson", g, j)
     timed_conn = {}   # index into conns of a timedSynapticInput instance -> its component
+    spiker_conn = {}  # index into conns of a generic Poisson input instance -> (component, RNG instance)
     for (iid, g, w) in inputs:
         role, el = doc.items.get(iid, (None, None))
         if role != "input":
@@ -1379,6 +1420,12 @@ def read(path, rng_seed=None):
             # a LEMS input: an instance in the compartment's synapse list
             if iid in timed_times:
                 timed_conn[len(conns)] = iid
+            if iid in spikers:
+                # its RNG instance: the earlier instances of the same input on the cell
+                cell = int(np.searchsorted(cell_begin, g, side="right") - 1)
+                k = poi_count.get((cell, iid), 0)
+                poi_count[(cell, iid)] = k + 1
+                spiker_conn[len(conns)] = (iid, k)
             input_where.append(("conn", len(conns)))
             conns.append((-1, g, generic_index[iid], _f32(w), 0.0, -1))
         elif t == "pulseGenerator":
@@ -1395,10 +1442,10 @@ def read(path, rng_seed=None):
             poisson[g].append((rate, synapse(el.get("synapse"), f"poissonFiringSynapse {iid}", native=True), _f32(w), k))
     # per-instance settings, in statement order (native inputs' tables directly; generic
     # synapse instances and point neurons get their own parameter vectors below)
-    syn_over, cell_over = {}, {}
+    syn_over, cell_over, spiker_over = {}, {}, {}
     for st_ in statements:
         _apply_setting(st_, doc, pops, input_lists, projections, input_where, pulses, poisson, generic_index,
-                       generic_st, syn_over, cell_over)
+                       generic_st, syn_over, cell_over, spiker_over)
 
     pu_b = np.zeros(nc + 1, np.int64)
     np.cumsum([len(x) for x in pulses], out=pu_b[1:])
@@ -1512,6 +1559,23 @@ def read(path, rng_seed=None):
             if k in timed_conn:
                 syn_spike_list[i] = start[timed_conn[k]]
         spike_times = np.array(times, np.float32)
+    syn_poisson_spiker, poisson_spikers = None, None
+    if spiker_conn:
+        syn_poisson_spiker = np.full(len(syn_list), -1, np.int64)
+        rows = []
+        for i, k in enumerate(syn_list):
+            if k in spiker_conn:
+                iid, inst = spiker_conn[k]
+                rate, tr_, delay, duration = spikers[iid]
+                over = spiker_over.get(k, {})
+                rate = _f32(over.get("averageRate", rate))
+                delay = _f32(over.get("delay", delay))
+                duration = _f32(over.get("duration", duration))
+                syn_poisson_spiker[i] = len(rows)
+                rows.append((rate, tr_, delay, duration, inst, 0))
+        poisson_spikers = {name: np.array([r[j] for r in rows], dt) for j, (name, dt) in enumerate(
+            (("rate", np.float32), ("transient", np.int32), ("delay", np.float32), ("duration", np.float32),
+             ("instance", np.int32), ("rng_offset", np.int32)))}
     syn_pidx, syn_vecs = None, None
     if syn_over:
         syn_pidx, syn_vecs = np.full(len(syn_list), -1, np.int64), []
@@ -1559,6 +1623,7 @@ def read(path, rng_seed=None):
         lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
         syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
         syn_spike_list=syn_spike_list, spike_times=spike_times,
+        syn_poisson_spiker=syn_poisson_spiker, poisson_spikers=poisson_spikers,
         name=os.path.splitext(os.path.basename(path))[0])
     model.sim_seed = seed
     return model, Outputs(trajectories, events, base_dir)

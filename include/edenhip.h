@@ -129,6 +129,21 @@ typedef struct edenhip_lems_synapses {
                                      input type (type_flags bit 0) */
     const float *spike_times;     /* spike lists (us, in listed order), each ended by a value >= FLT_MAX */
     int64_t n_spike_times;
+    /* (abi_version >= 4) Poisson firing-synapse inputs on the generic path (poissonFiringSynapse
+     * with a synapse other than expTwoSynapse, transientPoissonFiringSynapse; their spiker's
+     * LEMS dynamics at eden/neuroml/LEMS_CoreComponents.inc.xml:623-735): per instance an index
+     * into the spiker tables, or -1 (NULL = none). The spiker draws its intervals with randof
+     * (work item = the cell's RNG seed, the given instance, the step, invocation rng_offset + 0
+     * at the initial steps and + 1 on each spike); its spike is the instance's event in the
+     * same step. Such an instance takes no incoming connections and its type is an input type. */
+    const int64_t *poisson_spiker;
+    int64_t n_poisson_spikers;
+    const float *spiker_rate;       /* [n] averageRate, 1/us */
+    const int32_t *spiker_transient;/* [n] 1: transientPoissonFiringSynapse (delay and duration apply) */
+    const float *spiker_delay;      /* [n] us */
+    const float *spiker_duration;   /* [n] us */
+    const int32_t *spiker_instance; /* [n] randof instance (earlier instances of the same input on the cell) */
+    const int32_t *spiker_rng_offset; /* [n] */
 } edenhip_lems_synapses;
 
 /* LEMS point neurons (the reference's artificial cells: iafCell, izhikevich2007Cell,

@@ -242,6 +242,7 @@ typedef struct {
     real *pA[2], *pB[2];    /* Poisson synapse A, B */
     float *lS[2];           /* generic LEMS synapse state [k][synapse] */
     int64_t *tpos[2];       /* firing-synapse inputs: position in the spike list, or -1 */
+    float *pkT[2], *pkI[2]; /* Poisson spikers of generic instances: tsince, isi */
     float *cS[2];           /* LEMS point-neuron state [k][slot] */
     int32_t *cslot;         /* per compartment: its point-neuron slot, or -1 */
     int64_t n_cslots;
@@ -358,6 +359,39 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                     s->tpos[nxt][sy] = initial_state ? s->tpos[now][sy] : pos;
                     /* (the generated step ignores events during the initial steps) */
                     spike_now = initial_state ? 0 : fired;
+                    s->nspk[nxt][sy] = s->nspk[now][sy];
+                } else if (d->abi_version >= 4 && d->lems && d->lems->poisson_spiker && d->lems->poisson_spiker[sy] >= 0) {
+                    /* a Poisson firing-synapse input's spiker (poissonFiringSynapse,
+                       transientPoissonFiringSynapse; eden/neuroml/LEMS_CoreComponents.inc.xml:
+                       623-735): OnStart tsince = 0, isi = -averageIsi log(1 - random(1)) (+ delay);
+                       tsince' = 1; OnCondition tsince > isi: tsince = 0, a new interval (pushed
+                       past the window by 1e9 hours for the transient one), a spike that is the
+                       synapse's event of the same step */
+                    const edenhip_lems_synapses *L = d->lems;
+                    const int64_t k = L->poisson_spiker[sy];
+                    const float averageIsi = 1.0f / L->spiker_rate[k];
+                    const int32_t inst = L->spiker_instance[k], roff = L->spiker_rng_offset[k];
+                    const int transient = L->spiker_transient[k] != 0;
+                    const float delay = L->spiker_delay[k], duration = L->spiker_duration[k];
+                    const float tsince = s->pkT[now][k], isi = s->pkI[now][k];
+                    float tn, in;
+                    if (initial_state) {
+                        tn = 0.0f;
+                        const float nextIsi = (-averageIsi) * (float)LOG((real)(1.0f - randof_f(1.0f, rng_object_id, inst, step, roff + 0)));
+                        in = transient ? nextIsi + delay : nextIsi;
+                    } else {
+                        tn = tsince + (float)dt * 1.0f;
+                        in = isi;
+                        if (tsince > isi) {
+                            tn = 0.0f;
+                            const float nextIsi = (-averageIsi) * (float)LOG((real)(1.0f - randof_f(1.0f, rng_object_id, inst, step, roff + 1)));
+                            in = transient ? nextIsi + ((((float)time_r + nextIsi) - (delay + duration)) / duration < 0.0f ? 0.0f : 1.0f) * 3.6e18f
+                                           : nextIsi;
+                            spike_now = 1;
+                        }
+                    }
+                    s->pkT[nxt][k] = tn;
+                    s->pkI[nxt][k] = in;
                     s->nspk[nxt][sy] = s->nspk[now][sy];
                 } else {
                     if (!initial_state) {
@@ -654,6 +688,12 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
         int32_t smax = 1;
         for (int32_t t = 0; t < d->lems->n_types; t++) smax = d->lems->n_state[t] > smax ? d->lems->n_state[t] : smax;
         for (int b = 0; b < 2; b++) ALLOC(s.lS[b], (int64_t)smax * (NSY > 0 ? NSY : 1), float);
+        if (d->abi_version >= 4 && d->lems->poisson_spiker) {
+            const int64_t n = d->lems->n_poisson_spikers;
+            for (int64_t k = 0; k < NSY; k++)
+                if (d->lems->poisson_spiker[k] < -1 || d->lems->poisson_spiker[k] >= n) return -1;
+            for (int b = 0; b < 2; b++) { ALLOC(s.pkT[b], n, float); ALLOC(s.pkI[b], n, float); }
+        }
         if (d->abi_version >= 4 && d->lems->spike_list) {
             const int64_t nt = d->lems->n_spike_times;
             if (!d->lems->spike_times || nt < 1 || !(d->lems->spike_times[nt - 1] >= FLT_MAX)) return -1;
@@ -745,6 +785,7 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     free(s.D); free(s.order); free(s.gate_begin); free(s.comp_ev_begin); free(s.comp_ev);
     free(s.lS[0]); free(s.lS[1]);
     free(s.tpos[0]); free(s.tpos[1]);
+    free(s.pkT[0]); free(s.pkT[1]); free(s.pkI[0]); free(s.pkI[1]);
     free(s.cS[0]); free(s.cS[1]); free(s.cslot);
     return rows;
 }

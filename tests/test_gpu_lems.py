@@ -211,3 +211,31 @@ def test_timed_synaptic_input(tmp_path, opts):
     assert np.array_equal(tr, rt) and np.array_equal(tr2, rt)
     assert np.array_equal(st, ost) and np.array_equal(sy, osy)
     assert sorted(st[sy == 0].tolist()) == [1, 2, 101, 201, 351]
+
+
+@pytest.mark.parametrize("opts", [{}, dict(use_graph=True)])
+def test_poisson_firing_inputs_on_the_generic_path(tmp_path, opts):
+    """A native poissonFiringSynapse beside a generic one (expOneSynapse), a
+    transientPoissonFiringSynapse on two cells, and a timedSynapticInput: the spikers' RNG
+    draws, the synaptic events and the traces bit for bit, in the plain and graph launch
+    modes, and again after a reset."""
+    from test_firing_inputs_cpu import _list, _model
+    m = _model(tmp_path,
+               '<poissonFiringSynapse id="pn" averageRate="200 Hz" synapse="e2" spikeTarget="./e2"/>'
+               '<poissonFiringSynapse id="pg" averageRate="150 Hz" synapse="e1" spikeTarget="./e1"/>'
+               '<transientPoissonFiringSynapse id="tp" averageRate="300 Hz" delay="50ms" duration="100ms" '
+               'synapse="e2" spikeTarget="./e2"/>'
+               '<timedSynapticInput id="ti" synapse="e1" spikeTarget="./e1"><spike id="0" time="5 ms"/>'
+               '<spike id="1" time="7.5 ms"/></timedSynapticInput>',
+               _list("N", "pn", [0]) + _list("G", "pg", [0, 1]) + _list("T", "tp", [0, 1]) + _list("S", "ti", [1]),
+               length_ms=200.0)
+    _say(f"[lems] firing inputs, opts {opts}: spikers {len(m.poisson_spikers['rate'])}")
+    (rt, ec, er, _), (ost, osy) = oracle.run_with_synapse_log(m)
+    with Simulation(m, log_synapse_events=1 << 16, **opts) as sim:
+        tr = sim.run_all()
+        st, sy = sim.read_synapse_events()
+        sim.reset()
+        tr2 = sim.run_all()
+    assert np.array_equal(tr, rt) and np.array_equal(tr2, rt)
+    assert np.array_equal(st, ost) and np.array_equal(sy, osy)
+    assert len(ost) > 50

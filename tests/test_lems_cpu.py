@@ -482,15 +482,17 @@ def test_voltage_clamp_holds_a_passive_cell_near_its_target(tmp_path):
 def test_reference_core_inputs_and_custom_input_models_read(tmp_path):
     """The reference's CoreInputs model: one LEMS input type per generator element, flagged as
     inputs, each instance after the synapse types in its compartment's list, and the
-    timedSynapticInput as an input instance of its expOneSynapse with the spike list;
+    timedSynapticInput as an input instance of its expOneSynapse with the spike list (the
+    model's unused Poisson inputs give their types too);
     EdenTest4's user-defined myPulseGenerator reads as an input type and its tree cells carry
     the instances; both run in the oracle."""
     from coreinputs import write_model
     m, _ = neuroml.read(write_model(str(tmp_path)))
     assert m.lems.type_names == ["expOneSynapse", "expTwoSynapse", "pulseGeneratorDL", "sineGenerator",
                                  "sineGeneratorDL", "rampGenerator", "rampGeneratorDL", "voltageClamp",
-                                 "voltageClampTriple", "timedSynapticInput/expOneSynapse"]
-    assert m.lems.type_flags == [0, 0, 1, 1, 1, 1, 1, 1, 1, 1]
+                                 "voltageClampTriple", "timedSynapticInput/expOneSynapse",
+                                 "poissonFiringSynapse/expOneSynapse", "transientPoissonFiringSynapse/expTwoSynapse"]
+    assert m.lems.type_flags == [0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1]
     comp_of = np.repeat(np.arange(m.n_comps), np.diff(m.comp_syn_begin))
     # InputDemos/02-06 (compartments 2-6): sine, ramp, clamp, triple clamp, the timed synaptic
     # input; 11, 12: the spike sources' synapses; InputDemosDL/01-03: the dimensionless generators
