@@ -997,7 +997,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         gen += "\n";
         gen += LC ? std::string(LC->source)
                   : std::string("__device__ __forceinline__ void lems_cell(int, const float *, const float *, float *, "
-                                "long, float, float, int, float, int *spike) { *spike = 0; }\n");
+                                "long, float, float, int, float, int, long long, int *spike) { *spike = 0; }\n");
         std::string err;
         hipFunction_t syn_fn = nullptr, cell_fn = nullptr;
         if (rtc_lems_kernel(gen, &syn_fn, &cell_fn, err) != 0) return fail(ctx, EDENHIP_E_INVALID, err);

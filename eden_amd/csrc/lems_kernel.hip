@@ -34,6 +34,12 @@
 #define COSH(x) ((float)cosh((double)(x)))
 #define TANH(x) ((float)tanh((double)(x)))
 __device__ __forceinline__ float stepf(float x) { return x < 0.0f ? 0.0f : 1.0f; }
+// random(x) of a point neuron's generated step: randof stamped with the cell's RNG seed, the
+// step, the call's number and, as the instance, the reference's NOT_AN_INSTANCE marker of
+// code outside a table loop (~0xFee1600dLL, eden/Eden.cpp:3255-3256)
+__device__ __forceinline__ float lems_randof(float x, int seed, long long step, int n) {
+    return edenhip::randof(x, seed, (int64_t)~0xFee1600dLL, (int64_t)step, n);
+}
 
 #include "lems_synapse_functions.h"
 
@@ -171,7 +177,8 @@ lems_cell_kernel(edenhip::DevTables T, edenhip::LemsCellArgs A, int64_t step, fl
     float *S = A.state + idx;
     int spike = 0;
     const float *P = A.params + (A.poff ? A.poff[idx] : A.param_begin[ty]);
-    lems_cell(ty, P, S, S, (long)A.n, t, dt, init ? 1 : 0, ext, &spike);
+    lems_cell(ty, P, S, S, (long)A.n, t, dt, init ? 1 : 0, ext, (int)T.cell_seed[T.comp_cell[g]], (long long)step,
+              &spike);
     const int32_t vs = A.v_state[ty];
     if (vs >= 0) T.V[g] = S[(int64_t)vs * A.n];
     if (spike) {

@@ -419,7 +419,7 @@ _REL = {".gt.": ">", ".lt.": "<", ".geq.": ">=", ".ge.": ">=", ".leq.": "<=", ".
         ".neq.": "!=", ".ne.": "!="}
 _FUNCS = {"exp": "EXP", "log": "LOG", "ln": "LOG", "sqrt": "sqrtf", "abs": "fabsf", "sin": "SIN", "cos": "COS",
           "tan": "TAN", "sinh": "SINH", "cosh": "COSH", "tanh": "TANH", "ceil": "ceilf", "floor": "floorf",
-          "H": "stepf"}
+          "H": "stepf", "random": "random"}
 
 
 def _tokens(text):
@@ -552,6 +552,8 @@ def infix(tree, names):
     if k == "neg":
         return "( -" + infix(tree[1], names) + " )"
     if k == "fn":
+        if tree[1] == "random":
+            raise LemsError("random() in synapse and input types is not supported")
         return _FUNCS[tree[1]] + "( " + infix(tree[2], names) + " )"
     if k == "pow":
         return "POW( " + infix(tree[1], names) + " , " + infix(tree[2], names) + " )"
@@ -585,6 +587,8 @@ def evaluate(tree, values):
     if k == "neg":
         return F(-evaluate(tree[1], values))
     if k == "fn":
+        if tree[1] not in _NP_FUNCS:
+            raise LemsError(f"{tree[1]}() cannot be evaluated at setup")
         return F(_NP_FUNCS[tree[1]](evaluate(tree[2], values)))
     if k == "cmp":
         a, b = evaluate(tree[2], values), evaluate(tree[3], values)

@@ -100,6 +100,15 @@ def infix_dim(tree, names, dims):
     if k == "neg":
         t, d = infix_dim(tree[1], names, dims)
         return "( -" + t + " )", d
+    if k == "fn" and tree[1] == "random":
+        # random(x): the reference's randof(x, rng_object_id, instance, step, rng_offset + n)
+        # with n counting the calls in emission order (eden/Eden.cpp:2450-2459)
+        if "__rng__" not in names:
+            raise LemsError("random() is not supported here")
+        t, d = infix_dim(tree[2], names, dims)
+        n = names["__rng__"][0]
+        names["__rng__"][0] += 1
+        return f"lems_randof( {t} , rng_seed, rng_step, {n} )", d
     if k == "fn":
         t, d = infix_dim(tree[2], names, dims)
         fn = "stepf" if tree[1] == "H" else _FUNCS[tree[1]]
@@ -193,6 +202,11 @@ def _cell(name, base, params=(), state=(), derived=(), ders=None, on_start=(), o
 
 
 _REFRACT = "t .gt. lastSpikeTime + refract"
+# the Poisson spike sources' state and the time their next spike is used at: the ideal time,
+# or just after the present time when that has passed (dt permitting)
+_POISSON_SOURCE_STATE = [("tsince", "time"), ("tnextIdeal", "time"), ("tnextUsed", "time"), ("isi", "time")]
+_POISSON_SOURCE_DERS = {"tsince": "1", "tnextUsed": "0", "tnextIdeal": "0"}
+_TNEXT_USED = "tnextIdeal*H( (tnextIdeal-t)/t ) + (t+SMALL_TIME)*H( (t-tnextIdeal)/t )"
 _PYNN_REFRACT = "t .gt. lastSpikeTime + (tau_refrac*MSEC)"
 _PYNN_RATE = "(MVOLT * ((i_offset/cm) +  ((v_rest - (v/MVOLT)) / tau_m))/MSEC) + (iSyn / (cm * NFARAD))"
 
@@ -307,6 +321,41 @@ CELL_TYPES = {
         state=[("tsince", "time"), ("tnext", "time")], ders={"tsince": "1", "tnext": "0"},
         on_start=[("tsince", "0"), ("tnext", "period")],
         on_cond=[("tnext-t .lt. SMALL_TIME", [("tsince", "0"), ("tnext", "tnext+period")], True)]),
+    # random spike sources (eden/neuroml/LEMS_CoreComponents.inc.xml:778-937, 1944-2000)
+    "spikeGeneratorRandom": _cell(
+        "spikeGeneratorRandom", "baseSpikeSource", params=[("maxISI", "time"), ("minISI", "time")],
+        consts=[("MSEC", "time", 1000.0)], state=[("tsince", "time"), ("tnext", "time"), ("isi", "time")],
+        ders={"tsince": "1", "tnext": "0"},
+        on_start=[("tsince", "0"), ("isi", "minISI + MSEC * random((maxISI - minISI) / MSEC)"), ("tnext", "isi")],
+        on_cond=[("t .gt. tnext", [("isi", "minISI + MSEC * random((maxISI - minISI) / MSEC)"), ("tsince", "0"),
+                                   ("tnext", "tnext+isi")], True)]),
+    "spikeGeneratorPoisson": _cell(
+        "spikeGeneratorPoisson", "baseSpikeSource", params=[("averageRate", "per_time")],
+        consts=[("SMALL_TIME", "time", 1e-6)], state=_POISSON_SOURCE_STATE, ders=_POISSON_SOURCE_DERS,
+        on_start=[("tsince", "0"), ("isi", "-1 * log(random(1)) / averageRate"), ("tnextIdeal", "isi"),
+                  ("tnextUsed", "isi")],
+        on_cond=[("t .gt. tnextUsed", [("tsince", "0"), ("isi", "-1 * log(random(1)) / averageRate"),
+                                       ("tnextIdeal", "(tnextIdeal+isi)"), ("tnextUsed", _TNEXT_USED)], True)]),
+    "spikeGeneratorRefPoisson": _cell(
+        "spikeGeneratorRefPoisson", "baseSpikeSource", params=[("averageRate", "per_time"), ("minimumISI", "time")],
+        consts=[("SMALL_TIME", "time", 1e-6)], dparams=[("averageIsi", "1 / averageRate")],
+        state=_POISSON_SOURCE_STATE, ders=_POISSON_SOURCE_DERS,
+        on_start=[("tsince", "0"), ("isi", "minimumISI - (averageIsi-minimumISI) * log(random(1))"),
+                  ("tnextIdeal", "isi"), ("tnextUsed", "isi")],
+        on_cond=[("t .gt. tnextUsed", [("tsince", "0"),
+                                       ("isi", "minimumISI - (averageIsi-minimumISI) * log(random(1))"),
+                                       ("tnextIdeal", "(tnextIdeal+isi)"), ("tnextUsed", _TNEXT_USED)], True)]),
+    "SpikeSourcePoisson": _cell(
+        "SpikeSourcePoisson", "baseSpikeSource", params=[("start", "time"), ("duration", "time"), ("rate", "per_time")],
+        consts=[("LONG_TIME", "time", 3.6e18), ("SMALL_TIME", "time", 1e-6)],
+        dparams=[("end", "start + duration")], state=_POISSON_SOURCE_STATE, ders=_POISSON_SOURCE_DERS,
+        on_start=[("isi", "start - log(random(1))/rate"), ("tsince", "0"),
+                  ("tnextIdeal", "isi + H(((isi) - (start+duration))/duration)*LONG_TIME"),
+                  ("tnextUsed", "tnextIdeal")],
+        on_cond=[("t .gt. tnextUsed", [
+            ("isi", "-1 * log(random(1))/rate"),
+            ("tnextIdeal", "(tnextIdeal+isi) + H(((tnextIdeal+isi) - (start+duration))/duration)*LONG_TIME"),
+            ("tnextUsed", _TNEXT_USED), ("tsince", "0")], True)]),
     "IF_curr_alpha": _pynn_iaf("IF_curr_alpha", "basePyNNIaFCell"),
     "IF_curr_exp": _pynn_iaf("IF_curr_exp", "basePyNNIaFCell"),
     "IF_cond_alpha": _pynn_iaf("IF_cond_alpha", "basePyNNIaFCondCell"),
@@ -476,9 +525,9 @@ def emit_spike_list(fn_name, dialect):
     q = "__device__ __forceinline__ " if dialect == "hip" else "static "
     return "\n".join([
         f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float t, float dt, int init,"
-        " float isyn, int *spike) {",
+        " float isyn, int rng_seed, long long rng_step, int *spike) {",
         "    /* spikeArray */",
-        "    (void)stride; (void)isyn;",
+        "    (void)stride; (void)isyn; (void)rng_seed; (void)rng_step;",
         "    int pos = (int)S[0];",
         "    int ev_spike = 0;",
         "    if (!init) {",
@@ -500,17 +549,20 @@ def emit_cell(ct, fn_name, dialect):
     """C / HIP text of one cell type's step function:
 
         void fn(const float *P, const float *S, float *Sn, long stride, float t, float dt,
-                int init, float isyn, int *spike)
+                int init, float isyn, int rng_seed, long long rng_step, int *spike)
 
     P: the type's parameters in table order; S / Sn: the cell's present / next state, variable
     k at k * stride (they may alias); t: the step's time (time_f32); isyn: the external
-    current (synapses then inputs); *spike: set when an OnCondition with an EventOut fires."""
+    current (synapses then inputs); rng_seed, rng_step: the cell's RNG seed and the step, the
+    stamp of random() (lems_randof, invocations numbered in emission order); *spike: set when
+    an OnCondition with an EventOut fires."""
     q = "__device__ __forceinline__ " if dialect == "hip" else "static "
     L = [f"{q}void {fn_name}(const float *P, const float *S, float *Sn, long stride, float t, float dt, int init,"
-         " float isyn, int *spike) {",
+         " float isyn, int rng_seed, long long rng_step, int *spike) {",
          f"    /* {ct.name} */",
-         "    (void)S; (void)Sn; (void)stride; (void)t; (void)dt; (void)init; (void)isyn;"]
-    names, dims = {"t": "t"}, {"t": dim("time")}
+         "    (void)S; (void)Sn; (void)stride; (void)t; (void)dt; (void)init; (void)isyn; (void)rng_seed;"
+         " (void)rng_step;"]
+    names, dims = {"t": "t", "__rng__": [0]}, {"t": dim("time")}
     for r, d in ct.requirements.items():
         if r in ("iSyn", "ISyn"):
             names[r], dims[r] = "isyn", dim(d)
@@ -620,9 +672,10 @@ def build(types, custom=None):
 
     def dispatcher(q):
         L = [f"{q}void lems_cell(int type, const float *P, const float *S, float *Sn, long stride, float t, float dt,"
-             " int init, float isyn, int *spike) {", "    switch (type) {"]
+             " int init, float isyn, int rng_seed, long long rng_step, int *spike) {", "    switch (type) {"]
         for k in range(len(types)):
-            L.append(f"    case {k}: lems_cell_{k}(P, S, Sn, stride, t, dt, init, isyn, spike); break;")
+            L.append(f"    case {k}: lems_cell_{k}(P, S, Sn, stride, t, dt, init, isyn, rng_seed, rng_step, spike);"
+                     " break;")
         L.append("    default: *spike = 0; break;")
         L.append("    }")
         L.append("}")

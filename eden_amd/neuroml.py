@@ -20,7 +20,9 @@ set covers:
   point neurons     the NeuroML core artificial cells (iafTauCell, iafTauRefCell, iafCell,
                     iafRefCell, izhikevichCell, izhikevich2007Cell, adExIaFCell,
                     fitzHughNagumoCell, fitzHughNagumo1969Cell, hindmarshRose1984Cell, the PyNN
-                    IF_* / EIF_* / HH_cond_exp cells) and user <ComponentType>s extending a base
+                    IF_* / EIF_* / HH_cond_exp cells, the spike sources spikeArray,
+                    spikeGenerator, spikeGeneratorRandom, spikeGeneratorPoisson,
+                    spikeGeneratorRefPoisson, SpikeSourcePoisson) and user <ComponentType>s extending a base
                     cell type, each a one-compartment cell run by its generated step
                     (eden_amd.lems_cells; eden/Eden.cpp:6603-6910), recorded by state variable
   discretisation    a cable segment group (neuroLexId sao864921383) becomes
@@ -169,8 +171,7 @@ _FIRING_INPUTS = {"poissonFiringSynapse", "transientPoissonFiringSynapse", "time
 _GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction"}
 _UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
 _UNSUPPORTED_INPUTS = set()
-_UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell", "SpikeSourcePoisson", "spikeGeneratorRandom", "spikeGeneratorPoisson",
-                      "spikeGeneratorRefPoisson"}
+_UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell"}
 
 
 @dataclass

@@ -151,10 +151,13 @@ typedef struct edenhip_lems_synapses {
  * reference compiles per cell type (eden/Eden.cpp:6603-6910). The caller supplies HIP source
  * defining
  *   __device__ void lems_cell(int type, const float *P, const float *S, float *Sn, long stride,
- *                             float t, float dt, int init, float isyn, int *spike)
+ *                             float t, float dt, int init, float isyn, int rng_seed,
+ *                             long long rng_step, int *spike)
  * (one cell's step: P its type's parameters, S / Sn its present / next state with variable k
  * at k * stride, t the step's time, isyn its external current (its synapses' currents, then
- * its inputs'), *spike set when it emits a spike), which edenhip_create() compiles with hiprtc
+ * its inputs'), rng_seed and rng_step the cell's RNG seed and the step (the stamp of the
+ * type's random() calls, eden/Eden.cpp:2450-2459, 6628-6637), *spike set when it emits a
+ * spike), which edenhip_create() compiles with hiprtc
  * beside lems_syn (eden_amd.lems_cells emits it). A point-neuron cell has one compartment,
  * without channels or Poisson inputs; its synapses take the generic path (lems), and the
  * compartment's V mirrors the type's membrane-voltage state variable (its synapses' and gap

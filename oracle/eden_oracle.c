@@ -175,7 +175,7 @@ typedef void (*lems_fn_t)(int type, const float *P, const float *S, float *Sn, l
 static lems_fn_t g_lems = NULL;
 /* LEMS point neurons (the C dialect of the model's lems_cell source, compiled by the caller) */
 typedef void (*lems_cell_fn_t)(int type, const float *P, const float *S, float *Sn, long stride, float t, float dt,
-                               int init, float isyn, int *spike);
+                               int init, float isyn, int rng_seed, long long rng_step, int *spike);
 static lems_cell_fn_t g_lems_cell = NULL;
 
 void FN(set_lems)(void *fn) { g_lems = (lems_fn_t)fn; }
@@ -549,7 +549,7 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
             int spike = 0;
             const int64_t po = (LC->cell_param && LC->cell_param[cell] >= 0) ? LC->cell_param[cell] : LC->param_begin[ty];
             g_lems_cell(ty, LC->params + po, s->cS[now] + slot, s->cS[nxt] + slot, (long)ns,
-                        (float)time_r, (float)dt, initial_state, ext, &spike);
+                        (float)time_r, (float)dt, initial_state, ext, rng_object_id, (long long)step, &spike);
             const int32_t vs = LC->v_state[ty];
             Vn[g] = vs >= 0 ? (real)s->cS[nxt][(int64_t)vs * ns + slot] : V[g];
             if (spike) {

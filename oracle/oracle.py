@@ -108,6 +108,14 @@ def load_variant(path):
     return L
 
 
+# random(x) of a point neuron's step: the oracle's own randof (set after loading), stamped with
+# the cell's seed, the step, the call's number and the reference's NOT_AN_INSTANCE marker
+# (~0xFee1600dLL, eden/Eden.cpp:3255-3256) as the instance
+_RANDOF_PRELUDE = (
+    "float (*lems_randof_impl)(float, int, long long, long long, int) = 0;\n"
+    "static float lems_randof(float x, int seed, long long step, int n) {\n"
+    "    return lems_randof_impl(x, seed, (long long)~0xFee1600dLL, step, n);\n}\n")
+
 _LEMS_PRELUDE = {
     # the oracle's own exp/log: rounded once from a double evaluation (as the f32 flavour)
     "exact": "#include <math.h>\nstatic inline float EXP(float x) { return (float)exp((double)x); }\n"
@@ -115,12 +123,12 @@ _LEMS_PRELUDE = {
              "static inline float POW(float x, float y) { return (float)pow((double)x, (double)y); }\n"
              + "".join(f"static inline float {f.upper()}(float x) {{ return (float){f}((double)x); }}\n"
                        for f in ("sin", "cos", "tan", "sinh", "cosh", "tanh"))
-             + "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
+             + "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n" + _RANDOF_PRELUDE,
     # glibc expf/logf/powf (the f32libm and f32ofast flavours)
     "libm": "#include <math.h>\n#define EXP expf\n#define LOG logf\n#define POW powf\n"
             "#define SIN sinf\n#define COS cosf\n#define TAN tanf\n#define SINH sinhf\n#define COSH coshf\n"
             "#define TANH tanhf\n"
-            "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n",
+            "static float stepf(float x) { if (x < 0) return 0; else return 1; }\n" + _RANDOF_PRELUDE,
 }
 _LEMS_FLAGS = {
     "exact": ["-O2", "-ffp-contract=off"],
@@ -136,7 +144,8 @@ def lems_library(source_c, flavour="exact"):
     import hashlib
     import subprocess
     import tempfile
-    key = hashlib.sha1((flavour + "\n" + source_c).encode()).hexdigest()
+    text = _LEMS_PRELUDE["exact" if flavour == "exact" else "libm"] + source_c
+    key = hashlib.sha1((flavour + "\n" + text).encode()).hexdigest()
     if key in _lems_libs:
         return _lems_libs[key]
     d = os.path.join(tempfile.gettempdir(), "eden_oracle_lems")
@@ -144,11 +153,12 @@ def lems_library(source_c, flavour="exact"):
     src, so = os.path.join(d, key + ".c"), os.path.join(d, key + ".so")
     if not os.path.exists(so):
         with open(src, "w") as f:
-            f.write(_LEMS_PRELUDE["exact" if flavour == "exact" else "libm"] + source_c)
+            f.write(text)
         tmp = so + f".{os.getpid()}.tmp"
         subprocess.run(["gcc", "-std=c11", "-fPIC", "-shared", *_LEMS_FLAGS[flavour], src, "-o", tmp, "-lm"], check=True)
         os.replace(tmp, so)
     lib_ = C.CDLL(so)
+    C.c_void_p.in_dll(lib_, "lems_randof_impl").value = C.cast(lib().oracle_f32_randof, C.c_void_p).value
     _lems_libs[key] = lib_
     return lib_
 

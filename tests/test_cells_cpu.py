@@ -153,3 +153,64 @@ def test_spike_sources_fire_at_their_times(tmp_path):
     _, ec, er, _ = oracle.run(m)
     assert list(er[ec == 0]) == [401, 801, 1401, 2001]          # 50.01 ms: the step from 50.000 to 50.025
     assert list(er[ec == 1]) == [601, 1201, 1801]
+
+
+# NOT_AN_INSTANCE, the instance the reference's generated code stamps random() with outside a
+# table loop, as a point neuron's own code is (~0xFee1600dLL, eden/Eden.cpp:3255-3256)
+NOT_AN_INSTANCE = ~0xFee1600d
+
+
+def _sources(tmp_path, elems, pops, length_ms, record=()):
+    cols = "".join(f'<OutputColumn id="c{k}" quantity="{q}"/>' for k, q in enumerate(record))
+    evs = "".join(f'<EventSelection id="{k}" select="{p}[0]" eventPort="spike"/>' for k, (p, _) in enumerate(pops))
+    p = tmp_path / "rs.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml>' + elems + '<network id="n">' +
+        "".join(f'<population id="{p}" component="{c}" size="1"/>' for p, c in pops) +
+        f'</network></neuroml><Simulation id="s" length="{length_ms}ms" step="0.025ms" target="n" seed="5">'
+        + (f'<OutputFile id="o" fileName="o.txt">{cols}</OutputFile>' if record else "") +
+        f'<EventOutputFile id="e" fileName="e.txt" format="TIME_ID">{evs}</EventOutputFile></Simulation></Lems>')
+    m, _ = neuroml.read(str(p))
+    tr, ec, er, _ = oracle.run(m)
+    return m, tr, ec, er
+
+
+def test_random_spike_source_first_interval_by_hand(tmp_path):
+    """spikeGeneratorPoisson's OnStart draws isi = -1 * log(random(1)) / averageRate; the value
+    left by the last initial step (step 0) is randof(1, the cell's seed, NOT_AN_INSTANCE,
+    step 0, invocation 0). Its recorded isi and first spike follow from that one number."""
+    m, tr, ec, er = _sources(tmp_path, '<spikeGeneratorPoisson id="g" averageRate="50 Hz"/>', [("a", "g")], 200.0,
+                             record=("a[0]/isi", "a[0]/tnextUsed"))
+    seed = int(np.int32(np.uint32(m.cell_rng_seed[0])))
+    r = np.float32(oracle.randof(1.0, seed, NOT_AN_INSTANCE, 0, 0))
+    log_r = np.float32(math.log(float(r)))
+    isi = np.float32(np.float32(-1.0) * log_r) / np.float32(50e-6)
+    assert tr[0, 0] == isi and tr[0, 1] == isi
+    # the first spike: the first step whose start passes isi (t = (step - 1) dt > isi)
+    first = int(np.floor(float(isi) / 25.0)) + 2
+    assert int(er[0]) == first
+
+
+def test_random_spike_sources_follow_their_distributions(tmp_path):
+    """spikeGeneratorRandom: intervals within [minISI, maxISI] up to a step either way, mean
+    near the middle; spikeGeneratorPoisson: a count inside a 4-sigma band; spikeGeneratorRefPoisson:
+    no interval below minimumISI (a step aside), mean interval averageIsi; SpikeSourcePoisson:
+    spikes only within [start, start + duration]."""
+    T = 4000.0
+    m, _, ec, er = _sources(
+        tmp_path,
+        '<spikeGeneratorRandom id="r" minISI="10ms" maxISI="30ms"/>'
+        '<spikeGeneratorPoisson id="p" averageRate="40 Hz"/>'
+        '<spikeGeneratorRefPoisson id="q" averageRate="40 Hz" minimumISI="15ms"/>'
+        '<SpikeSourcePoisson id="s" start="1000ms" duration="1500ms" rate="60 Hz"/>',
+        [("a", "r"), ("b", "p"), ("c", "q"), ("d", "s")], T)
+    dt = 0.025
+    isi = np.diff(er[ec == 0]) * dt
+    assert isi.min() >= 10.0 - dt and isi.max() <= 30.0 + dt and abs(isi.mean() - 20.0) < 1.0
+    n = int(np.sum(ec == 1))
+    assert abs(n - 40 * T / 1000) < 4 * np.sqrt(40 * T / 1000)
+    isi = np.diff(er[ec == 2]) * dt
+    assert isi.min() >= 15.0 - dt and abs(isi.mean() - 25.0) < 3.0
+    t = (er[ec == 3] - 1) * dt
+    assert t.min() >= 1000.0 and t.max() <= 2500.0 + dt
+    assert abs(len(t) - 90) < 4 * np.sqrt(90)

@@ -149,3 +149,43 @@ def test_point_neuron_network_bit_exact(tmp_path, opts):
     assert counts[7] == 3 and counts[8] == 21                 # the spike sources' listed and periodic spikes
     assert float(np.ptp(tr[:, 5])) > 0.01                      # iz's u moves
     assert float(np.abs(tr[:, 11] + 65.0).max()) > 0.05        # the gap junction moves P2
+
+
+@pytest.mark.parametrize("opts", [{}, dict(use_graph=True)])
+def test_random_spike_sources_bit_exact(tmp_path, opts):
+    """spikeGeneratorRandom, spikeGeneratorPoisson, spikeGeneratorRefPoisson and
+    SpikeSourcePoisson (random() stamped with each cell's seed and the step), driving an
+    integrate-and-fire cell through a synapse: spike rasters, recorded state and the target's
+    trace bit for bit."""
+    p = tmp_path / "rs.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml>'
+        '<spikeGeneratorRandom id="r" minISI="10ms" maxISI="30ms"/>'
+        '<spikeGeneratorPoisson id="p" averageRate="40 Hz"/>'
+        '<spikeGeneratorRefPoisson id="q" averageRate="40 Hz" minimumISI="15ms"/>'
+        '<SpikeSourcePoisson id="s" start="100ms" duration="150ms" rate="60 Hz"/>'
+        '<iafCell id="iaf" leakReversal="-65mV" thresh="-52mV" reset="-70mV" C="0.2nF" leakConductance="0.01uS"/>'
+        '<expOneSynapse id="e1" gbase="2nS" erev="0mV" tauDecay="3ms"/>'
+        '<network id="n"><population id="a" component="r" size="2"/><population id="b" component="p" size="2"/>'
+        '<population id="c" component="q" size="1"/><population id="d" component="s" size="1"/>'
+        '<population id="x" component="iaf" size="1"/>'
+        + "".join(f'<projection id="j{k}" presynapticPopulation="{pp}" postsynapticPopulation="x" synapse="e1">'
+                  f'<connectionWD id="0" preCellId="../{pp}/0/{c}" postCellId="../x/0/iaf" weight="3" delay="1ms"/>'
+                  '</projection>' for k, (pp, c) in enumerate((("a", "r"), ("b", "p"), ("c", "q"), ("d", "s"))))
+        + '</network></neuroml><Simulation id="s" length="400ms" step="0.025ms" target="n" seed="9">'
+        '<OutputFile id="o" fileName="o.txt"><OutputColumn id="v" quantity="x[0]/v"/>'
+        '<OutputColumn id="i0" quantity="a[1]/isi"/><OutputColumn id="i1" quantity="b[1]/tnextUsed"/></OutputFile>'
+        '<EventOutputFile id="e" fileName="e.txt" format="TIME_ID">'
+        + "".join(f'<EventSelection id="{k}" select="{s}" eventPort="spike"/>'
+                  for k, s in enumerate(("a[0]", "a[1]", "b[0]", "b[1]", "c[0]", "d[0]", "x[0]")))
+        + '</EventOutputFile></Simulation></Lems>')
+    m, _ = neuroml.read(str(p))
+    _say(f"[cells] random sources, opts {opts}: types {m.lems_cells.type_names}")
+    (tr_r, ec, er, _), (ost, osy) = oracle.run_with_synapse_log(m)
+    tr, ev, (st, sy) = _gpu(m, **opts)
+    assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))
+    assert np.array_equal(ev[0], ec) and np.array_equal(ev[1], er)
+    assert np.array_equal(st, ost) and np.array_equal(sy, osy)
+    counts = np.bincount(ec, minlength=7)
+    # every source fires (the SpikeSourcePoisson, about 9 spikes in its 150 ms, at least once)
+    assert counts[:6].min() > 0 and not np.array_equal(er[ec == 0], er[ec == 1])
