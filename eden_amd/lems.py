@@ -268,6 +268,7 @@ LEMS_DIMS = {"time": "time", "per_time": "frequency", "voltage": "voltage", "cur
              "concentration": "concentration", "conductance_per_voltage": "conductance_per_voltage",
              "voltage_per_time": "voltage_per_time", "currentDensity": "currentDensity", "per_voltage": "per_voltage",
              "resistance": "resistance", "conductanceDensity": "conductivity", "temperature": "temperature",
+             "specificCapacitance": "specific_capacitance",
              "none": None, "": None}
 
 

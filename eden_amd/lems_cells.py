@@ -321,6 +321,53 @@ CELL_TYPES = {
         state=[("tsince", "time"), ("tnext", "time")], ders={"tsince": "1", "tnext": "0"},
         on_start=[("tsince", "0"), ("tnext", "period")],
         on_cond=[("tnext-t .lt. SMALL_TIME", [("tsince", "0"), ("tnext", "tnext+period")], True)]),
+    # Pinsky and Rinzel's two-compartment CA3 cell in one component
+    # (eden/neuroml/LEMS_CoreComponents.inc.xml:1306-1442); its v is a derived variable, so the
+    # reference gives it no membrane-voltage state (its Vs and Vd are recorded by name)
+    "pinskyRinzelCA3Cell": _cell(
+        "pinskyRinzelCA3Cell", "baseCellMembPot",
+        params=[("iSoma", "currentDensity"), ("iDend", "currentDensity"), ("gLs", "conductanceDensity"),
+                ("gLd", "conductanceDensity"), ("gNa", "conductanceDensity"), ("gKdr", "conductanceDensity"),
+                ("gCa", "conductanceDensity"), ("gKahp", "conductanceDensity"), ("gKC", "conductanceDensity"),
+                ("gc", "conductanceDensity"), ("eNa", _V), ("eCa", _V), ("eK", _V), ("eL", _V), ("pp", "none"),
+                ("cm", "specificCapacitance"), ("alphac", "none"), ("betac", "none"), ("gNmda", "conductanceDensity"),
+                ("gAmpa", "conductanceDensity"), ("qd0", "none")],
+        consts=[("MSEC", "time", 1000.0), ("MVOLT", _V, 1.0), ("UAMP_PER_CM2", "currentDensity", 1.0),
+                ("Smax", "none", 125.0), ("Vsyn", _V, 60.0), ("betaqd", "none", 0.001)],
+        state=[("Vs", _V), ("Vd", _V), ("Cad", "none"), ("hs", "none"), ("ns", "none"), ("sd", "none"),
+               ("cd", "none"), ("qd", "none"), ("Si", "none"), ("Wi", "none")],
+        derived=[
+            ("ICad", "currentDensity", "gCa*sd*sd*(Vd-eCa)"),
+            ("alphams_Vs", "none", "0.32*(-46.9-Vs/MVOLT)/(exp((-46.9-Vs/MVOLT)/4.0)-1.0)"),
+            ("betams_Vs", "none", "0.28*(Vs/MVOLT+19.9)/(exp((Vs/MVOLT+19.9)/5.0)-1.0)"),
+            ("Minfs_Vs", "none", "alphams_Vs/(alphams_Vs+betams_Vs)"),
+            ("alphans_Vs", "none", "0.016*(-24.9-Vs/MVOLT)/(exp((-24.9-Vs/MVOLT)/5.0)-1.0)"),
+            ("betans_Vs", "none", "0.25*exp(-1.0-0.025*Vs/MVOLT)"),
+            ("alphahs_Vs", "none", "0.128*exp((-43.0-Vs/MVOLT)/18.0)"),
+            ("betahs_Vs", "none", "4.0/(1.0+exp((-20.0-Vs/MVOLT)/5.0))"),
+            ("alphasd_Vd", "none", "1.6/(1.0+exp(-0.072*(Vd/MVOLT-5.0)))"),
+            ("betasd_Vd", "none", "0.02*(Vd/MVOLT+8.9)/(exp((Vd/MVOLT+8.9)/5.0)-1.0)"),
+            ("Iampa", "currentDensity", "gAmpa*Wi*(Vd-Vsyn)"),
+            ("Sisat", "none", [("Si .gt. Smax", "Smax"), (None, "Si")]),
+            ("Inmda", "currentDensity", "gNmda*Sisat*(Vd-Vsyn)/(1.0+0.28*exp(-0.062*(Vd/MVOLT-60.0)))"),
+            ("Isyn", "currentDensity", "Iampa+Inmda"),
+            ("alphaqd", "none", [("0.00002*Cad .gt. 0.01", "0.01"), (None, "0.00002*Cad")]),
+            ("chid", "none", [("Cad/250 .gt. 1", "1"), (None, "Cad/250")]),
+            ("alphacd_Vd", "none", [("Vd .lt. -10*MVOLT", "exp((Vd/MVOLT+50.0)/11-(Vd/MVOLT+53.5)/27)/18.975"),
+                                    (None, "2.0*exp((-53.5-Vd/MVOLT)/27.0)")]),
+            ("betacd_Vd", "none", [("Vd .lt. -10*MVOLT", "(2.0*exp((-53.5-Vd/MVOLT)/27.0)-alphacd_Vd)"),
+                                   (None, "0")])],
+        ders={"Vs": "(-gLs*(Vs-eL)-gNa*(Minfs_Vs^2)*hs*(Vs-eNa)-gKdr*ns*(Vs-eK)+(gc/pp)*(Vd-Vs)+iSoma/pp) / cm",
+              "Vd": "(iDend/(1.0-pp)-Isyn/(1.0-pp)-gLd*(Vd-eL)-ICad-gKahp*qd*(Vd-eK)-gKC*cd*chid*(Vd-eK)"
+                    "+(gc*(Vs-Vd))/(1.0-pp)) / cm",
+              "Cad": "(-0.13*ICad/UAMP_PER_CM2-0.075*Cad) / MSEC",
+              "hs": "(alphahs_Vs-(alphahs_Vs+betahs_Vs)*hs) / MSEC",
+              "ns": "(alphans_Vs-(alphans_Vs+betans_Vs)*ns) / MSEC",
+              "sd": "(alphasd_Vd-(alphasd_Vd+betasd_Vd)*sd) / MSEC",
+              "cd": "(alphacd_Vd-(alphacd_Vd+betacd_Vd)*cd) / MSEC",
+              "qd": "(alphaqd-(alphaqd+betaqd)*qd) / MSEC",
+              "Si": "-Si/150.0", "Wi": "-Wi/2.0"},
+        on_start=[("Vs", "eL"), ("Vd", "eL"), ("qd", "qd0")]),
     # random spike sources (eden/neuroml/LEMS_CoreComponents.inc.xml:778-937, 1944-2000)
     "spikeGeneratorRandom": _cell(
         "spikeGeneratorRandom", "baseSpikeSource", params=[("maxISI", "time"), ("minISI", "time")],

@@ -19,7 +19,8 @@ set covers:
                     ones (eden/NeuroML.cpp:2530-3566)
   point neurons     the NeuroML core artificial cells (iafTauCell, iafTauRefCell, iafCell,
                     iafRefCell, izhikevichCell, izhikevich2007Cell, adExIaFCell,
-                    fitzHughNagumoCell, fitzHughNagumo1969Cell, hindmarshRose1984Cell, the PyNN
+                    fitzHughNagumoCell, fitzHughNagumo1969Cell, hindmarshRose1984Cell,
+                    pinskyRinzelCA3Cell, the PyNN
                     IF_* / EIF_* / HH_cond_exp cells, the spike sources spikeArray,
                     spikeGenerator, spikeGeneratorRandom, spikeGeneratorPoisson,
                     spikeGeneratorRefPoisson, SpikeSourcePoisson) and user <ComponentType>s extending a base
@@ -171,7 +172,7 @@ _FIRING_INPUTS = {"poissonFiringSynapse", "transientPoissonFiringSynapse", "time
 _GRADED = {"silentSynapse", "linearGradedSynapse", "gradedSynapse", "gapJunction"}
 _UNSUPPORTED_SYNAPSES = {"doubleSynapse"}
 _UNSUPPORTED_INPUTS = set()
-_UNSUPPORTED_CELLS = {"pinskyRinzelCA3Cell"}
+_UNSUPPORTED_CELLS = set()
 
 
 @dataclass

@@ -214,3 +214,36 @@ def test_random_spike_sources_follow_their_distributions(tmp_path):
     t = (er[ec == 3] - 1) * dt
     assert t.min() >= 1000.0 and t.max() <= 2500.0 + dt
     assert abs(len(t) - 90) < 4 * np.sqrt(90)
+
+
+def pinsky_rinzel_model(tmp_path, n=1, length_ms=500.0):
+    """n pinskyRinzelCA3Cells with the reference's test parameters recording Vs, Vd, hs of the first."""
+    cell = ('<pinskyRinzelCA3Cell id="c" iSoma="0.75 uA_per_cm2" iDend="0 uA_per_cm2" gc="2.1 mS_per_cm2" qd0="0" '
+            'gLs="0.1 mS_per_cm2" gLd="0.1 mS_per_cm2" gNa="30 mS_per_cm2" gKdr="15 mS_per_cm2" gCa="10 mS_per_cm2" '
+            'gKahp="0.8 mS_per_cm2" gKC="15 mS_per_cm2" eNa="60 mV" eCa="80 mV" eK="-75 mV" eL="-60 mV" pp="0.5" '
+            'cm="3 uF_per_cm2" alphac="2" betac="0.1" gNmda="0 mS_per_cm2" gAmpa="0 mS_per_cm2"/>')
+    p = tmp_path / "pr.xml"
+    p.write_text(
+        f'<Lems><Target component="s"/><neuroml>{cell}<network id="n"><population id="pop" component="c" size="{n}"/>'
+        f'</network></neuroml><Simulation id="s" length="{length_ms}ms" step="0.025ms" target="n" seed="1">'
+        '<OutputFile id="o" fileName="o.txt"><OutputColumn id="vs" quantity="pop[0]/Vs"/>'
+        '<OutputColumn id="vd" quantity="pop[0]/Vd"/><OutputColumn id="hs" quantity="pop[0]/hs"/></OutputFile>'
+        '</Simulation></Lems>')
+    return str(p)
+
+
+def test_pinsky_rinzel_first_step_by_hand(tmp_path):
+    """pinskyRinzelCA3Cell with the reference's test parameters (EdenTest_ArtificialCells.nml,
+    pr2A): from Vs = Vd = eL with the gates at 0, the first step moves Vs by dt iSoma / (pp cm)
+    (0.75 uA/cm2 / (0.5 * 3 uF/cm2) = 0.5 mV/ms, so 0.0125 mV in 25 us) and hs by
+    dt alpha_hs(eL), alpha_hs = 0.128 exp((-43 - V) / 18) per ms; the cell then bursts."""
+    m, _ = neuroml.read(pinsky_rinzel_model(tmp_path))
+    assert m.lems_cells.v_state == [-1]
+    tr, _, _, _ = oracle.run(m)
+    assert tr[0, 0] == np.float32(-60.0) and tr[0, 1] == np.float32(-60.0)
+    assert tr[1, 0] == pytest.approx(-60.0 + 0.0125, abs=2e-6)
+    assert tr[1, 1] == pytest.approx(-60.0, abs=1e-6)
+    assert tr[1, 2] == pytest.approx(0.025 * 0.128 * math.exp((-43.0 + 60.0) / 18.0), rel=1e-5)
+    vs = tr[:, 0].astype(np.float64)
+    ups = np.flatnonzero((vs[:-1] < 0.0) & (vs[1:] >= 0.0))
+    assert len(ups) >= 3 and float(vs.max()) > 20.0        # somatic spikes

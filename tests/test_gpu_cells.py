@@ -189,3 +189,16 @@ def test_random_spike_sources_bit_exact(tmp_path, opts):
     counts = np.bincount(ec, minlength=7)
     # every source fires (the SpikeSourcePoisson, about 9 spikes in its 150 ms, at least once)
     assert counts[:6].min() > 0 and not np.array_equal(er[ec == 0], er[ec == 1])
+
+
+def test_pinsky_rinzel_bit_exact(tmp_path):
+    """pinskyRinzelCA3Cell (conditional derived variables, powers, dimension factors between
+    current densities and specific capacitance): Vs, Vd and hs bit for bit over 500 ms of
+    bursting."""
+    from test_cells_cpu import pinsky_rinzel_model
+    m, _ = neuroml.read(pinsky_rinzel_model(tmp_path, n=3))
+    _say("[cells] pinskyRinzelCA3Cell x3")
+    tr_r, _, _, _ = oracle.run(m)
+    tr, _, _ = _gpu(m)
+    assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))
+    assert float(tr[:, 0].max()) > 20.0
