@@ -15,6 +15,7 @@ ABI_VERSION = 4
 MAX_GATES = 4
 
 RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED = 0, 1, 2, 3
+GATE_RATES, GATE_TAU_INF = 0, 1
 
 E_OK, E_INVALID, E_DEVICE, E_LIMIT, E_STATE = 0, -1, -2, -3, -4
 
@@ -73,6 +74,7 @@ class ModelDesc(C.Structure):
         ("lems", C.POINTER(LemsSynapsesDesc)),
         ("record_gate", _P),
         ("lems_cells", C.POINTER(LemsCellsDesc)),
+        ("gate_kind", _P),
     ]
 
 

@@ -957,6 +957,10 @@ int build_adj(edenhip_adj *a, const edenhip_model_desc *d) {
         type_shape[(size_t)ty] = shape;
     }
     // recorded compartments
+    if (d->abi_version >= 4 && d->gate_kind)
+        for (int64_t k = 0; k < (int64_t)d->n_channel_types * EDENHIP_MAX_GATES; k++)
+            if (d->gate_kind[k] != EDENHIP_GATE_RATES)
+                return afail(a, EDENHIP_E_INVALID, "the adjoint model takes alpha/beta rate gates only");
     a->n_record = d->n_record;
     a->T.n_record = (int32_t)d->n_record;
     std::vector<int32_t> rec(NC, -1);

@@ -28,7 +28,7 @@ enum RateForm : int32_t { RATE_EXP = 0, RATE_EXPLINEAR = 1, RATE_SIGMOID = 2, RA
 
 // One gate of a channel type, 32 bytes (two ds_read_b128 from LDS).
 struct alignas(16) DGate {
-    int32_t forms;   // alpha form | beta form << 8 | instances << 16 | gates of the type << 24
+    int32_t forms;   // alpha form | kind << 4 | beta form << 8 | instances << 16 | gates of the type << 24
     float q10;
     float ar, am;
     float as, br;
@@ -299,13 +299,16 @@ __device__ __forceinline__ float rate_eval_t(Div &dv, int form, float r, float m
 }
 
 // One gate's next value from the membrane voltage: rates, tau, inf, cnexp with clipping
-// (eden/Eden.cpp:5236-5259, 5336-5375); initialisation steps set the gate to inf.
+// (eden/Eden.cpp:5236-5259, 5282-5383); initialisation steps set the gate to inf. A tau/inf
+// gate (kind 1) holds its time course in the alpha slots and its steady state in the beta
+// slots, and takes them as they are.
 template <class Div>
 __device__ __forceinline__ float gate_next_t(Div &dv, const DGate &G, float V, float q, bool init, float dt) {
-    const float alpha = rate_eval_t(dv, G.forms & 0xFF, G.ar, G.am, G.as, V);
+    const float alpha = rate_eval_t(dv, G.forms & 0xF, G.ar, G.am, G.as, V);
     const float beta = rate_eval_t(dv, (G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
-    const float tau = dv(1.0f, alpha + beta);
-    const float inf = dv(alpha, alpha + beta);
+    const bool tau_inf = (G.forms & 0xF0) != 0;
+    const float tau = tau_inf ? alpha : dv(1.0f, alpha + beta);
+    const float inf = tau_inf ? beta : dv(alpha, alpha + beta);
     if (init) return inf;
     const float tau_factor = dv(dt * G.q10, tau);
     const float blend = exp_f(-tau_factor);

@@ -447,8 +447,12 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             if (g.instances < 0 || g.instances > 255) return fail(ctx, EDENHIP_E_INVALID, "gate instances out of range");
             for (const edenhip_rate *r : {&g.alpha, &g.beta})
                 if (r->form < 0 || r->form > EDENHIP_RATE_FIXED) return fail(ctx, EDENHIP_E_INVALID, "unknown rate form");
+            const int32_t kind = (d->abi_version >= 4 && d->gate_kind) ? d->gate_kind[t * EDENHIP_MAX_GATES + gi]
+                                                                       : EDENHIP_GATE_RATES;
+            if (kind != EDENHIP_GATE_RATES && kind != EDENHIP_GATE_TAU_INF)
+                return fail(ctx, EDENHIP_E_INVALID, "unknown gate kind");
             DGate &G = gates[(size_t)t * GATES_PER_TYPE + gi];
-            G.forms = g.alpha.form | (g.beta.form << 8) | (g.instances << 16) | (ct.n_gates << 24);
+            G.forms = g.alpha.form | (kind << 4) | (g.beta.form << 8) | (g.instances << 16) | (ct.n_gates << 24);
             G.q10 = g.q10;
             G.ar = g.alpha.rate; G.am = g.alpha.midpoint; G.as = g.alpha.scale;
             G.br = g.beta.rate; G.bm = g.beta.midpoint; G.bs = g.beta.scale;

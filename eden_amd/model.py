@@ -126,6 +126,9 @@ class Model:
     # delay, duration (us), instance, rng_offset
     syn_poisson_spiker: np.ndarray = None
     poisson_spikers: dict = None
+    # per channel type, per gate slot (EDENHIP_MAX_GATES of them): its kind (0 alpha/beta
+    # rates, 1 tau/inf), or None for rates everywhere
+    gate_kind: np.ndarray = None
     name: str = "model"
 
     def __post_init__(self):
@@ -167,6 +170,10 @@ class Model:
         for dt, names in conv.items():
             for n in names:
                 setattr(self, n, np.ascontiguousarray(getattr(self, n), dtype=dt))
+        if self.gate_kind is not None:
+            self.gate_kind = np.ascontiguousarray(self.gate_kind, dtype=np.int32).reshape(-1)
+            if self.gate_kind.shape != (len(self.channel_types) * N.MAX_GATES,):
+                raise ValueError("gate_kind: MAX_GATES entries per channel type")
         if self.record_gate is not None:
             self.record_gate = np.ascontiguousarray(self.record_gate, dtype=np.int32)
             if self.record_gate.shape != self.record_comp.shape:
@@ -221,7 +228,7 @@ class Model:
             pulse_duration=self.pulse_duration[pu], pulse_amplitude=self.pulse_amplitude[pu],
             pulse_weight=self.pulse_weight[pu], syn_types=self.syn_types,
             record_comp=new_of[self.record_comp[rec_cols]], event_comp=new_of[self.event_comp[ev_cols]],
-            record_gate=None if self.record_gate is None else self.record_gate[rec_cols],
+            record_gate=None if self.record_gate is None else self.record_gate[rec_cols], gate_kind=self.gate_kind,
             name=f"{self.name}_subset{len(cells)}")
         return m, rec_cols, ev_cols
 

@@ -9,7 +9,9 @@ set covers:
                     the including file; NeuroML's core type files are skipped
                     (eden/NeuroML.cpp:9813-9891)
   ion channels      ionChannelHH / ionChannel with gateHHrates (HHExpRate, HHExpLinearRate,
-                    HHSigmoidRate), q10Settings (q10Fixed, q10ExpTemp); channels without gates
+                    HHSigmoidRate), gateHHtauInf and gateHHratesTauInf (fixedTimeCourse,
+                    HHExpVariable, HHExpLinearVariable, HHSigmoidVariable), q10Settings
+                    (q10Fixed, q10ExpTemp); channels without gates
                     are passive (eden/NeuroML.cpp:4380-4598)
   cells             <cell>: morphology (segments, parent / fractionAlong, implicit proximal
                     points, segmentGroups with member / include, inhomogeneousParameter),
@@ -306,24 +308,58 @@ def _rate(el, what):
     return N.Rate(_RATE_FORMS[t], _f32(r), _f32(mid), _f32(sc))
 
 
+_VARIABLE_FORMS = {"HHExpVariable": N.RATE_EXP, "HHExpLinearVariable": N.RATE_EXPLINEAR,
+                   "HHSigmoidVariable": N.RATE_SIGMOID}
+# gate elements by kind: alpha/beta rates, or tau and inf given (a gateHHratesTauInf's rates
+# play no part in its dynamics, eden/Eden.cpp:5282-5383)
+_GATE_KINDS = {"gateHHrates": N.GATE_RATES, "gateHHtauInf": N.GATE_TAU_INF, "gateHHratesTauInf": N.GATE_TAU_INF}
+
+
+def _steady_state(el, what):
+    """A steadyState element: HHExpVariable / HHExpLinearVariable / HHSigmoidVariable, its
+    rate a plain number (eden/NeuroML.cpp:4028-4057)."""
+    t = el.get("type")
+    if t not in _VARIABLE_FORMS:
+        raise NeuroMLError(f"{what}: steadyState type '{t}' is not supported ({', '.join(_VARIABLE_FORMS)})")
+    r = convert(el.get("rate"), None, what + " rate")
+    mid = convert(el.get("midpoint"), "voltage", what + " midpoint")
+    sc = convert(el.get("scale"), "voltage", what + " scale")
+    return N.Rate(_VARIABLE_FORMS[t], _f32(r), _f32(mid), _f32(sc))
+
+
+def _time_course(el, what):
+    """A timeCourse element: fixedTimeCourse, tau in us (eden/NeuroML.cpp:4087-4108)."""
+    if el.get("type") != "fixedTimeCourse":
+        raise NeuroMLError(f"{what}: timeCourse type '{el.get('type')}' is not supported (fixedTimeCourse)")
+    return N.Rate(N.RATE_FIXED, _f32(convert(el.get("tau"), "time", what + " tau")), 0.0, 0.0)
+
+
 def _channel(el, temperature):
+    """(channel type, gate kinds) of an ion channel element."""
     cid = el.get("id")
-    gates = []
+    gates, kinds = [], []
     for g in el:
         t = _tag(g)
         if t in ("notes", "annotation"):
             continue
-        if t != "gateHHrates" and not (t == "gate" and g.get("type") == "gateHHrates"):
-            raise NeuroMLError(f"channel {cid}: <{t}> gates are not supported (gateHHrates only)")
+        gtype = g.get("type") if t == "gate" else t
+        if gtype not in _GATE_KINDS:
+            raise NeuroMLError(f"channel {cid}: <{t}> gates of type {gtype} are not supported "
+                               f"({', '.join(_GATE_KINDS)})")
+        kind = _GATE_KINDS[gtype]
         inst = int(g.get("instances", "1"))
-        fw = rv = None
+        fw = rv = tau = inf = None
         q10 = 1.0
         for c in g:
             ct = _tag(c)
-            if ct == "forwardRate":
+            if ct == "forwardRate" and kind == N.GATE_RATES:
                 fw = _rate(c, f"channel {cid} gate {g.get('id')} forward")
-            elif ct == "reverseRate":
+            elif ct == "reverseRate" and kind == N.GATE_RATES:
                 rv = _rate(c, f"channel {cid} gate {g.get('id')} reverse")
+            elif ct == "timeCourse":
+                tau = _time_course(c, f"channel {cid} gate {g.get('id')} timeCourse")
+            elif ct == "steadyState":
+                inf = _steady_state(c, f"channel {cid} gate {g.get('id')} steadyState")
             elif ct == "q10Settings":
                 qt = c.get("type")
                 if qt == "q10Fixed":
@@ -336,12 +372,18 @@ def _channel(el, temperature):
                     q10 = f ** ((temperature - t0) / 10.0)
                 else:
                     raise NeuroMLError(f"channel {cid}: q10Settings type '{qt}' is not supported")
-        if fw is None or rv is None:
-            raise NeuroMLError(f"channel {cid} gate {g.get('id')}: forwardRate and reverseRate are required")
-        gates.append((inst, fw, rv, _f32(q10)))
+        if kind == N.GATE_RATES:
+            if fw is None or rv is None:
+                raise NeuroMLError(f"channel {cid} gate {g.get('id')}: forwardRate and reverseRate are required")
+            gates.append((inst, fw, rv, _f32(q10)))
+        else:
+            if tau is None or inf is None:
+                raise NeuroMLError(f"channel {cid} gate {g.get('id')}: timeCourse and steadyState are required")
+            gates.append((inst, tau, inf, _f32(q10)))
+        kinds.append(kind)
     if len(gates) > N.MAX_GATES:
         raise NeuroMLError(f"channel {cid}: more than {N.MAX_GATES} gates")
-    return channel_type(gates)
+    return channel_type(gates), kinds + [N.GATE_RATES] * (N.MAX_GATES - len(kinds))
 
 
 # ---------------------------------------------------------------- cells
@@ -1066,11 +1108,13 @@ def read(path, rng_seed=None):
     statements = _custom_setup(doc, sim)
 
     # channel types in document order, then the cells the network uses
-    channel_index, channel_types, gate_ids = {}, [], {}
+    channel_index, channel_types, gate_ids, gate_kinds = {}, [], {}, []
     for role, i in doc.order:
         if role == "channel":
             channel_index[i] = len(channel_types)
-            channel_types.append(_channel(doc.items[i][1], temperature))
+            ct_, kinds_ = _channel(doc.items[i][1], temperature)
+            channel_types.append(ct_)
+            gate_kinds += kinds_
             gate_ids[i] = [g.get("id") for g in doc.items[i][1] if _tag(g) not in ("notes", "annotation")]
     cells = {}
 
@@ -1623,6 +1667,7 @@ def read(path, rng_seed=None):
         record_comp=np.array(rec, np.int64), event_comp=np.array(evc, np.int64),
         record_gate=np.array(rec_gate, np.int32) if any(j >= 0 for j in rec_gate) else None,
         lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
+        gate_kind=np.array(gate_kinds, np.int32) if any(gate_kinds) else None,
         syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
         syn_spike_list=syn_spike_list, spike_times=spike_times,
         syn_poisson_spiker=syn_poisson_spiker, poisson_spikers=poisson_spikers,

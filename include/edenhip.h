@@ -54,6 +54,18 @@ enum {
 
 #define EDENHIP_MAX_GATES 4
 
+/* How a gate's tau and inf arise (eden/Eden.cpp:5282-5383). EDENHIP_GATE_RATES: from the
+ * forward and reverse rates alpha, beta (gateHHrates): tau = 1 / (alpha + beta), inf =
+ * alpha / (alpha + beta). EDENHIP_GATE_TAU_INF: given directly (gateHHtauInf, and
+ * gateHHratesTauInf, whose rates then play no part): the gate's `alpha` holds the time course
+ * (EDENHIP_RATE_FIXED with rate = tau in us: fixedTimeCourse) and its `beta` the steady state
+ * (EXP / EXPLINEAR / SIGMOID forms with a dimensionless rate: HHExpVariable,
+ * HHExpLinearVariable, HHSigmoidVariable). Both then take the same cnexp update. */
+enum {
+    EDENHIP_GATE_RATES = 0,
+    EDENHIP_GATE_TAU_INF = 1
+};
+
 typedef struct edenhip_rate {
     int32_t form;        /* EDENHIP_RATE_* */
     float rate;          /* 1/us */
@@ -260,6 +272,11 @@ typedef struct edenhip_model_desc {
 
     /* (abi_version >= 4) LEMS point neurons, or NULL */
     const edenhip_lems_cells *lems_cells;
+
+    /* (abi_version >= 4) [n_channel_types * EDENHIP_MAX_GATES] each gate's EDENHIP_GATE_* kind
+     * (gate k of type t at t * EDENHIP_MAX_GATES + k), or NULL: every gate EDENHIP_GATE_RATES.
+     * The adjoint takes EDENHIP_GATE_RATES gates only. */
+    const int32_t *gate_kind;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

@@ -124,6 +124,11 @@ float FN(randof)(float x, int32_t work_item, int64_t instance, int64_t step, int
     return randof_f(x, work_item, instance, step, invocation);
 }
 
+/* A gate's EDENHIP_GATE_* kind (EDENHIP_GATE_RATES without a kind table). */
+static int gate_kind(const edenhip_model_desc *d, int32_t type, int gi) {
+    return (d->abi_version >= 4 && d->gate_kind) ? d->gate_kind[type * EDENHIP_MAX_GATES + gi] : EDENHIP_GATE_RATES;
+}
+
 static real rate_value(const edenhip_rate *r, real V) {
     real rate = (real)r->rate, mid = (real)r->midpoint, scale = (real)r->scale;
     switch (r->form) {
@@ -301,10 +306,14 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                 const edenhip_gate *gt = &ct->gates[gi];
                 real q = s->q[now][qb + gi];
                 real q10 = (real)gt->q10;
+                /* rates give tau and inf; a tau/inf gate (gateHHtauInf) gives them directly, its
+                   time course in the alpha slot, its steady state in the beta slot
+                   (eden/Eden.cpp:5282-5383) */
+                const int tau_inf = gate_kind(d, d->chan_type[ch], gi) == EDENHIP_GATE_TAU_INF;
                 real alpha = rate_value(&gt->alpha, Vcomp);
                 real beta = rate_value(&gt->beta, Vcomp);
-                real tau = (real)1 / (alpha + beta);
-                real inf = alpha / (alpha + beta);
+                real tau = tau_inf ? alpha : (real)1 / (alpha + beta);
+                real inf = tau_inf ? beta : alpha / (alpha + beta);
                 real qn;
                 if (initial_state) {
                     qn = inf;
@@ -731,7 +740,8 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
             for (int gi = 0; gi < ct->n_gates; gi++) {
                 real a = rate_value(&ct->gates[gi].alpha, (real)d->v0[g]);
                 real b = rate_value(&ct->gates[gi].beta, (real)d->v0[g]);
-                s.q[0][s.gate_begin[ch] + gi] = a / (a + b);
+                s.q[0][s.gate_begin[ch] + gi] =
+                    gate_kind(d, d->chan_type[ch], gi) == EDENHIP_GATE_TAU_INF ? b : a / (a + b);
             }
         }
     }

@@ -1,0 +1,90 @@
+"""Gates given by their time course and steady state (gateHHtauInf, gateHHratesTauInf) on the
+CPU: the reader's tables and the oracle's dynamics.
+
+* A passive cell with a tau/inf potassium-like channel, depolarised by a pulse: the recorded
+  gate follows the recurrence q' = clip(b q + (1 - b) inf(V)) with b = exp(-dt q10 / tau), the
+  fixed time course tau taken as it is (not 1 / (alpha + beta)) and inf the sigmoid
+  steady state rate / (1 + exp((mid - V) / scale)) of the step's start voltage, evaluated here
+  in float32 from the recorded voltage trace (eden/Eden.cpp:5282-5383, 5236-5259).
+* gateHHratesTauInf reads like gateHHtauInf (its rates play no part); gateHHratesTau,
+  gateHHratesInf and gateHHInstantaneous are refused with their names.
+"""
+import math
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from eden_amd import _native as N
+from eden_amd import neuroml
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _model(tmp_path, gate_xml, q10="", length_ms=60.0):
+    p = tmp_path / "ti.xml"
+    p.write_text(
+        '<Lems><Target component="s"/><neuroml>'
+        '<ionChannelHH id="leak" conductance="10pS"/>'
+        f'<ionChannelHH id="kti" conductance="10pS" species="k">{gate_xml}</ionChannelHH>'
+        '<cell id="P"><morphology id="m"><segment id="0"><proximal x="0" y="0" z="0" diameter="10"/>'
+        '<distal x="0" y="0" z="0" diameter="10"/></segment></morphology><biophysicalProperties id="b">'
+        '<membraneProperties><channelDensity id="l" ionChannel="leak" condDensity="0.3 mS_per_cm2" erev="-65mV"/>'
+        '<channelDensity id="k" ionChannel="kti" condDensity="2 mS_per_cm2" erev="-80mV"/>'
+        '<specificCapacitance value="1.0 uF_per_cm2"/><initMembPotential value="-65mV"/></membraneProperties>'
+        '<intracellularProperties><resistivity value="0.1 kohm_cm"/></intracellularProperties>'
+        '</biophysicalProperties></cell>'
+        '<pulseGenerator id="pg" delay="5ms" duration="30ms" amplitude="0.05nA"/>'
+        '<network id="n" temperature="6.3 degC"><population id="pp" component="P" size="1"/>'
+        '<explicitInput target="pp[0]" input="pg"/></network></neuroml>'
+        f'<Simulation id="s" length="{length_ms}ms" step="0.025ms" target="n" seed="1">'
+        '<OutputFile id="o" fileName="o.txt"><OutputColumn id="v" quantity="pp/0/P/0/v"/>'
+        '<OutputColumn id="q" quantity="pp/0/P/0/b/membraneProperties/k/kti/n/q"/></OutputFile>'
+        '</Simulation></Lems>')
+    return neuroml.read(str(p))[0]
+
+
+TAU_INF = ('<gateHHtauInf id="n" instances="2">{q10}<timeCourse type="fixedTimeCourse" tau="4ms"/>'
+           '<steadyState type="HHSigmoidVariable" rate="1" scale="5mV" midpoint="-55mV"/></gateHHtauInf>')
+
+
+def test_tau_inf_gate_recurrence(tmp_path):
+    q10 = '<q10Settings type="q10Fixed" fixedQ10="1.5"/>'
+    m = _model(tmp_path, TAU_INF.format(q10=q10))
+    assert m.gate_kind.tolist() == [0] * 4 + [N.GATE_TAU_INF, 0, 0, 0]
+    g = m.channel_types[1].gates[0]
+    assert (g.alpha.form, g.alpha.rate) == (N.RATE_FIXED, 4000.0)
+    assert (g.beta.form, g.beta.rate, g.beta.midpoint, g.beta.scale) == (N.RATE_SIGMOID, 1.0, -55.0, 5.0)
+    tr, _, _, _ = oracle.run(m)
+    V, q = tr[:, 0], tr[:, 1]
+    assert float(np.ptp(V)) > 3.0 and float(np.ptp(q)) > 0.05
+    f32 = np.float32
+    blend = f32(math.exp(-float(f32(f32(25.0) * f32(1.5)) / f32(4000.0))))
+    want = [q[0]]
+    for r in range(len(q) - 1):
+        inf = f32(1.0) / (f32(1.0) + f32(math.exp(float(f32(f32(-55.0) - V[r]) / f32(5.0)))))
+        x = blend * want[-1] + (f32(1.0) - blend) * inf
+        want.append(min(max(x, f32(1e-6)), f32(1 - 1e-6)))
+    np.testing.assert_allclose(q, np.array(want, np.float32), rtol=0, atol=2e-7)
+    # the first row: the initial steps leave the gate at inf of the initial voltage
+    assert q[0] == pytest.approx(1.0 / (1.0 + math.exp((-55.0 + 65.0) / 5.0)), abs=1e-7)
+
+
+def test_rates_tau_inf_reads_as_tau_inf(tmp_path):
+    m1 = _model(tmp_path, TAU_INF.format(q10=""))
+    tr1, _, _, _ = oracle.run(m1)
+    rti = ('<gateHHratesTauInf id="n" instances="2"><forwardRate type="HHExpRate" rate="1per_ms" midpoint="0mV" '
+           'scale="10mV"/><reverseRate type="HHExpRate" rate="1per_ms" midpoint="0mV" scale="-10mV"/>'
+           '<timeCourse type="fixedTimeCourse" tau="4ms"/>'
+           '<steadyState type="HHSigmoidVariable" rate="1" scale="5mV" midpoint="-55mV"/></gateHHratesTauInf>')
+    m2 = _model(tmp_path, rti)
+    tr2, _, _, _ = oracle.run(m2)
+    assert np.array_equal(tr1, tr2)
+
+
+@pytest.mark.parametrize("gate", ["gateHHratesTau", "gateHHratesInf", "gateHHInstantaneous"])
+def test_other_gate_kinds_refused(tmp_path, gate):
+    with pytest.raises(neuroml.NeuroMLError, match=gate):
+        _model(tmp_path, f'<{gate} id="n" instances="1"/>')
