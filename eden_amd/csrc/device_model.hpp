@@ -98,6 +98,8 @@ struct DevTables {
     // LEMS inputs: per compartment (I, G) of the present step, summed after the native inputs
     // (null: none)
     const float2 *inp_ext;
+    const float2 *chx_ext;  // [NC] (I, G) of the channels of a compartment whose channels run in
+                            // ext_channels_kernel (chmeta == CHM_EXTERNAL), or null
 };
 
 // Arguments of lems_syn_kernel (the generic LEMS synapse kernel, compiled at run time from
@@ -357,6 +359,23 @@ __device__ __forceinline__ float randof(float x, int32_t work_item, int64_t inst
 __device__ __forceinline__ float &trace_at(const DevTables &T, int32_t col, int64_t row) {
     return T.trace[(int64_t)col * T.trace_rows + row];
 }
+
+// chmeta of a compartment with more channel distributions (NCH) or gate variables (NQ) than
+// the compartment kernels hold: ext_channels_kernel computes its channels each step, in
+// distribution order, and the kernels take the totals from DevTables::chx_ext.
+constexpr uint32_t CHM_EXTERNAL = 0xFFFFFFFEu;
+
+// Arguments of ext_channels_kernel: one lane per such compartment.
+struct ExtChanArgs {
+    const int32_t *comps;      // [n] compartments
+    const int32_t *ch_begin;   // [n + 1] into the channel arrays
+    const int32_t *ch_type;    // per channel: its type
+    const float *ch_g, *ch_e;  // per channel: Gbase, Erev
+    const int32_t *q_begin;    // per channel: its first gate variable in q
+    float *q;                  // gate variables, updated in place
+    float2 *out;               // [NC] the compartment's (I, G) of the step
+    int32_t n;
+};
 
 // Per-compartment constants cached in registers for the whole time loop.
 // gmeta bit 27: the compartment has pulse inputs (the first one is cached here); bit 31: it

@@ -42,6 +42,8 @@ bool const_below_root(const edenhip_model_desc *d, int64_t c0, int32_t n) {
         if (d->comp_syn_begin[g + 1] != d->comp_syn_begin[g]) return false;
         for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++)
             if (d->channel_types[d->chan_type[ch]].n_gates != 0) return false;
+        // (more channel distributions than the kernels' slots: channels of their own pass)
+        if (d->comp_chan_begin[g + 1] - d->comp_chan_begin[g] > NCH) return false;
     }
     return true;
 }
@@ -155,6 +157,12 @@ struct edenhip_ctx {
     int32_t *d_lems_tpos = nullptr;      // per synapse slot: a firing-synapse input's spike-list position, or -1
     float *d_lems_ttimes = nullptr;      // firing-synapse inputs' spike lists
     int32_t *d_pk_slot = nullptr;        // per synapse slot: its Poisson spiker, or -1
+    // compartments whose channels run in ext_channels_kernel
+    int32_t n_xcomp = 0;
+    int64_t n_xq = 0;
+    int32_t *d_x_comp = nullptr, *d_x_chb = nullptr, *d_x_type = nullptr, *d_x_qb = nullptr;
+    float *d_x_g = nullptr, *d_x_e = nullptr, *d_x_q = nullptr;
+    float2 *d_chx_ext = nullptr;
     float2 *d_pk_state = nullptr;        // per Poisson spiker: (tsince, isi)
     float *d_pk_isi = nullptr;
     float2 *d_pk_window = nullptr;
@@ -487,11 +495,47 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     const int64_t n_pulse = d->comp_pulse_begin[NC];
     if (n_syn >= (int64_t)1 << 31 || n_spk >= (int64_t)1 << 31 || n_poi >= (int64_t)1 << 31)
         return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 synapses or spike targets");
+    // compartments with more channel distributions or gate variables than the compartment
+    // kernels hold run their channels in ext_channels_kernel
+    std::vector<int32_t> x_comp, x_chb{0}, x_type, x_qb;
+    std::vector<float> x_g, x_e, x_q;
+    std::vector<char> ext_comp(NC, 0);
     for (int64_t g = 0; g < NC; g++) {
         V[g] = d->v0[g]; C[g] = d->capacitance[g]; A[g] = d->area[g]; invrc[g] = d->inv_rc_axial[g];
         vthr[g] = d->v_threshold ? d->v_threshold[g] : NAN;
         const int64_t cb = d->comp_chan_begin[g], ce = d->comp_chan_begin[g + 1];
-        if (ce - cb > NCH) return fail(ctx, EDENHIP_E_LIMIT, "compartment with more than 4 channel distributions");
+        int64_t n_gvars = 0;
+        for (int64_t ch = cb; ch < ce; ch++) {
+            const int ty = d->chan_type[ch];
+            if (ty < 0 || ty >= d->n_channel_types) return fail(ctx, EDENHIP_E_INVALID, "channel type index out of range");
+            n_gvars += d->channel_types[ty].n_gates;
+        }
+        if (ce - cb > NCH || n_gvars > NQ) {
+            if (acell_slot[g] >= 0) return fail(ctx, EDENHIP_E_INVALID, "a point neuron takes no channel distributions");
+            ext_comp[g] = 1;
+            x_comp.push_back((int32_t)g);
+            for (int64_t ch = cb; ch < ce; ch++) {
+                const int ty = d->chan_type[ch];
+                const edenhip_channel_type &ct = d->channel_types[ty];
+                x_type.push_back(ty);
+                x_g.push_back(d->chan_gbase[ch]);
+                x_e.push_back(d->chan_erev[ch]);
+                x_qb.push_back((int32_t)x_q.size());
+                for (int gi = 0; gi < ct.n_gates; gi++) {
+                    // setup value alpha/(alpha+beta) at V0; the initialisation steps overwrite it
+                    const double a = edenhip_rate_host(ct.gates[gi].alpha, d->v0[g]);
+                    const double b = edenhip_rate_host(ct.gates[gi].beta, d->v0[g]);
+                    x_q.push_back((float)(a / (a + b)));
+                }
+            }
+            if (x_q.size() >= ((size_t)1 << 31)) return fail(ctx, EDENHIP_E_LIMIT, "more than 2^31 gate variables");
+            x_chb.push_back((int32_t)x_type.size());
+            chmeta[g] = CHM_EXTERNAL; gmeta[g] = 0;
+            pulse_begin[g + 1] = (int32_t)d->comp_pulse_begin[g + 1];
+            spk_begin[g + 1] = (int32_t)d->comp_spike_begin[g + 1];
+            poi_begin[g + 1] = (int32_t)d->comp_poisson_begin[g + 1];
+            continue;
+        }
         uint32_t cm = 0xFFFFFFFFu, gm = 0;
         int nq = 0;
         for (int64_t ch = cb; ch < ce; ch++) {
@@ -570,6 +614,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             rec_var[r] = v;
             continue;
         }
+        if (ext_comp[g] && j >= 0)
+            return fail(ctx, EDENHIP_E_INVALID, "record_gate: gate variables of a compartment with more than 4 channel "
+                                                "distributions or gate variables are not recorded");
         if (j < -1 || j >= (int32_t)((gmeta[g] >> 24) & 7u))
             return fail(ctx, EDENHIP_E_INVALID, "record_gate: -1 or a gate variable of the compartment");
         rec_var[r] = j;
@@ -672,7 +719,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             gq_first[g] = (int64_t)gqv.size();
             const int32_t first_task = (int32_t)(ctask.size() - tc.ctask_off);
             uint32_t mask = 0;
-            for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
+            for (int64_t ch = d->comp_chan_begin[g]; !ext_comp[g] && ch < d->comp_chan_begin[g + 1]; ch++) {
                 const int slot = (int)(ch - d->comp_chan_begin[g]);
                 const edenhip_channel_type &ct = d->channel_types[d->chan_type[ch]];
                 if (ct.n_gates == 0) continue;
@@ -687,9 +734,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
                     gqv.push_back((float)(a / (a + b)));
                 }
             }
-            const int nch = (int)(d->comp_chan_begin[g + 1] - d->comp_chan_begin[g]);
+            const int nch = ext_comp[g] ? 0 : (int)(d->comp_chan_begin[g + 1] - d->comp_chan_begin[g]);
             if (first_task > 0xFFFF) return fail(ctx, EDENHIP_E_LIMIT, "more than 65535 gated channels in a cell");
-            tcm[g] = (uint32_t)nch | (mask << 3) | ((uint32_t)first_task << 16);
+            tcm[g] = (uint32_t)nch | (mask << 3) | ((uint32_t)first_task << 16) | (ext_comp[g] ? TCM_XCH : 0u);
         }
         tc.n_ctask = (int32_t)(ctask.size() - tc.ctask_off);
         tc.n_gate = (int32_t)(gqv.size() - tc.gate_off);
@@ -759,7 +806,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         // compartment, so their Hines diagonals are the same every step
         {
             bool ok = true;
-            for (int32_t i = 1; i < n && ok; i++) ok = (tcm[c0 + i] & (0xF8u | TCM_SYN)) == 0u;
+            for (int32_t i = 1; i < n && ok; i++) ok = (tcm[c0 + i] & (0xF8u | TCM_SYN | TCM_XCH)) == 0u;
             tc.const_tree = ok ? 1 : 0;
             if (ok != P.merged) return fail(ctx, EDENHIP_E_INVALID, "internal: schedule format of a constant tree");
         }
@@ -843,9 +890,23 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     ctx->n_slots = n_slots;
     // generic synapses run in their own kernel before the compartments of each step: one
     // launch per step, like a network
-    ctx->network = any_targets || (lems && n_syn > 0) || ctx->n_acells > 0;
+    ctx->network = any_targets || (lems && n_syn > 0) || ctx->n_acells > 0 || !x_comp.empty();
     ctx->tree_net = ctx->tree_net || ctx->network;
-    ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0;
+    ctx->point_quad = n_syn == 0 && n_poi == 0 && n_spk == 0 && x_comp.empty();
+    if (!x_comp.empty()) {
+        // channels beyond the compartment kernels' slots: their own pre-pass each step
+        ctx->n_xcomp = (int32_t)x_comp.size();
+        ctx->n_xq = (int64_t)x_q.size();
+        TRY(upload(ctx, &ctx->d_x_comp, x_comp));
+        TRY(upload(ctx, &ctx->d_x_chb, x_chb));
+        TRY(upload(ctx, &ctx->d_x_type, x_type));
+        TRY(upload(ctx, &ctx->d_x_g, x_g));
+        TRY(upload(ctx, &ctx->d_x_e, x_e));
+        TRY(upload(ctx, &ctx->d_x_qb, x_qb));
+        TRY(upload(ctx, &ctx->d_x_q, x_q.empty() ? std::vector<float>(1, 0.0f) : x_q));
+        TRY(alloc_zero(ctx, &ctx->d_chx_ext, (size_t)NC));
+        ctx->T.chx_ext = ctx->d_chx_ext;
+    }
 
     // recorded gate variables: point cells keep theirs in T.q (NQ slots per compartment in
     // distribution / gate order), tree cells in the gate table gqv
@@ -1081,6 +1142,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(snap(ctx->d_lems_state, (size_t)(smax * n_slots) * sizeof(float)));
     }
     if (ctx->d_lems_tpos) TRY(snap(ctx->d_lems_tpos, (size_t)n_slots * sizeof(int32_t)));
+    if (ctx->d_x_q) TRY(snap(ctx->d_x_q, (size_t)std::max<int64_t>(ctx->n_xq, 1) * sizeof(float)));
     if (ctx->d_pk_state) TRY(snap(ctx->d_pk_state, (size_t)ctx->n_pk * sizeof(float2)));
     if (ctx->d_acell_state) {
         int32_t smax = 1;
@@ -1187,7 +1249,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
 // Windowed network path: eligibility, per-compartment type runs, spike-target ring
 // addresses, second state copies. Models it does not cover keep the per-step path.
 static int build_netwin(edenhip_ctx *ctx, const edenhip_model_desc *d, const std::vector<int32_t> &point) {
-    if (!ctx->network || !ctx->buckets.empty() || ctx->opt.network_window == -1 || ctx->lems_fn) return 0;
+    if (!ctx->network || !ctx->buckets.empty() || ctx->opt.network_window == -1 || ctx->lems_fn || ctx->n_xcomp > 0)
+        return 0;
     const int64_t NC = d->n_comps;
     if (d->n_syn_types > 64 || (int64_t)point.size() != NC) return 0;
     const double dt = (double)d->dt;
@@ -1456,6 +1519,17 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 void *args[] = {&T, &ca, &st, &dt};
                 HIPCHECK(ctx, hipModuleLaunchKernel(ctx->lems_cell_fn, (unsigned)((ctx->n_acells + 63) / 64), 1, 1, 64, 1,
                                                     1, 0, ctx->stream, args, nullptr));
+                nl++;
+            }
+            if (ctx->n_xcomp > 0) {
+                // channels of the compartments beyond the kernels' slots, from this step's V
+                ExtChanArgs xa{ctx->d_x_comp, ctx->d_x_chb, ctx->d_x_type, ctx->d_x_g, ctx->d_x_e, ctx->d_x_qb,
+                               ctx->d_x_q, ctx->d_chx_ext, ctx->n_xcomp};
+                int64_t st = a;
+                float dt = ctx->dt;
+                void *args[] = {&T, &ctx->d_gates, &xa, &st, &dt};
+                HIPCHECK(ctx, hipLaunchKernel(ext_channels_kernel_ptr(), dim3((ctx->n_xcomp + 63) / 64), dim3(64), args, 0,
+                                              ctx->stream));
                 nl++;
             }
             if (ctx->n_point > 0) {

@@ -50,6 +50,11 @@ point_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const 
         if (!init && k.rec >= 0) trace_at(T, k.rec, s - 1 - row0) = V;
         float I_ch, G_ch;
         channels_step(gates, k, V, q, init, dt, I_ch, G_ch);
+        if (k.chmeta == CHM_EXTERNAL) {
+            const float2 e = T.chx_ext[g];   // channels of this step (ext_channels_kernel)
+            I_ch = e.x;
+            G_ch = e.y;
+        }
         float I_syn = 0.0f, G_syn = 0.0f;
         if (has_syn) {
             if (T.syn_ext) {
@@ -541,6 +546,11 @@ tree_kernel(DevTables T, const DGate *__restrict__ gates_g, int n_gates, const D
             const CompConst k = cx.k;
             float I_ch, G_ch;
             tree_channels(k, cx.tcm, sF, V[j], I_ch, G_ch);
+            if (k.chmeta == CHM_EXTERNAL) {
+                const float2 e = T.chx_ext[c0 + c];   // channels of this step (ext_channels_kernel)
+                I_ch = e.x;
+                G_ch = e.y;
+            }
             float I_in = pulses_current(T, k, c0 + c, t), G_in = 0.0f;
             if (cx.tcm & TCM_INPUT) {
                 const float2 in = sIn[c];
@@ -874,6 +884,42 @@ const void *tree_kernel_ptr(int tpb, bool split, bool net) {
     }
 }
 
+// Channels of compartments beyond the compartment kernels' slots (more than NCH channel
+// distributions or NQ gate variables), one lane per compartment, before the step's
+// compartment kernels: each channel in distribution order takes its open fraction from its
+// gates' present values (each gate `instances` times, conductance_scaling = 1), the gates move
+// to their next values, and the channel's current and conductance join the totals
+// (eden/Eden.cpp:5679-5898), as channels_step does for the others.
+__global__ void __launch_bounds__(64) ext_channels_kernel(DevTables T, const DGate *__restrict__ gates, ExtChanArgs X,
+                                                          int64_t step, float dt) {
+    const int32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= X.n) return;
+    const int32_t g = X.comps[idx];
+    const bool init = step <= 0;
+    const float V = T.V[g], A = T.A[g];
+    float I_ch = 0.0f, G_ch = 0.0f;
+    for (int32_t c = X.ch_begin[idx]; c < X.ch_begin[idx + 1]; c++) {
+        const int32_t type = X.ch_type[c];
+        const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
+        float fopen = 1.0f;
+        for (int gi = 0; gi < ng; gi++) {
+            const DGate G = gates[type * GATES_PER_TYPE + gi];
+            float *qp = X.q + X.q_begin[c] + gi;
+            const float qs = *qp;
+            const int inst = (G.forms >> 16) & 0xFF;
+            for (int i = 0; i < inst; i++) fopen = fopen * qs;
+            *qp = gate_next(G, V, qs, init, dt);
+        }
+        const float Gscaled = X.ch_g[c] * fopen;
+        const float iDensity = Gscaled * (X.ch_e[c] - V) * 1e-5f;
+        const float gDensity = (Gscaled) * 1e-5f;
+        I_ch += (iDensity * A);
+        G_ch += (gDensity * A);
+    }
+    X.out[g] = make_float2(I_ch, G_ch);
+}
+
+const void *ext_channels_kernel_ptr() { return reinterpret_cast<const void *>(&ext_channels_kernel); }
 const void *point_kernel_ptr() { return reinterpret_cast<const void *>(&point_kernel); }
 const void *point_quad_kernel_ptr() { return reinterpret_cast<const void *>(&point_quad_kernel); }
 

@@ -36,6 +36,7 @@ constexpr uint32_t TCX_COMPLEX = 1u << 31;
 // (bits 16-31).
 constexpr uint32_t TCM_INPUT = 1u << 7;
 constexpr uint32_t TCM_SYN = 1u << 8;
+constexpr uint32_t TCM_XCH = 1u << 9;    // channels run in ext_channels_kernel (chmeta CHM_EXTERNAL)
 
 constexpr int TREE_SLOTS = 4;        // compartments per thread
 
@@ -106,6 +107,7 @@ const void *tree_kernel_ptr(int tpb, bool split, bool net = false);
 const void *net_window_kernel_ptr();
 const void *net_deliver_kernel_ptr();
 const void *point_kernel_ptr();
+const void *ext_channels_kernel_ptr();
 const void *point_quad_kernel_ptr();
 
 }  // namespace edenhip
