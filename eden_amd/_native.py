@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(ROOT, "libedenhip.so")
 ABI_VERSION = 4
 MAX_GATES = 4
 
-RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED = 0, 1, 2, 3
+RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED, RATE_LEMS = 0, 1, 2, 3, 4
 GATE_RATES, GATE_TAU_INF = 0, 1
 
 E_OK, E_INVALID, E_DEVICE, E_LIMIT, E_STATE = 0, -1, -2, -3, -4
@@ -75,6 +75,7 @@ class ModelDesc(C.Structure):
         ("record_gate", _P),
         ("lems_cells", C.POINTER(LemsCellsDesc)),
         ("gate_kind", _P),
+        ("lems_rates_source", C.c_char_p),
     ]
 
 

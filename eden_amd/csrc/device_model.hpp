@@ -17,6 +17,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifdef EDENHIP_LEMS_RATES
+// a user rate ComponentType's instance k (generated code compiled with hiprtc, defined at
+// global scope); only the run-time compiled channel kernel sees gates of this form
+__device__ float lems_rate(int k, float v);
+#endif
+
 namespace edenhip {
 
 constexpr int NCH = 4;      // channel instances per compartment (register slots)
@@ -24,7 +30,7 @@ constexpr int NQ = 4;       // gate state variables per compartment (register sl
 constexpr int GATES_PER_TYPE = 4;
 constexpr int MAX_CHAN_TYPES = 64;
 
-enum RateForm : int32_t { RATE_EXP = 0, RATE_EXPLINEAR = 1, RATE_SIGMOID = 2, RATE_FIXED = 3 };
+enum RateForm : int32_t { RATE_EXP = 0, RATE_EXPLINEAR = 1, RATE_SIGMOID = 2, RATE_FIXED = 3, RATE_LEMS = 4 };
 
 // One gate of a channel type, 32 bytes (two ds_read_b128 from LDS).
 struct alignas(16) DGate {
@@ -289,6 +295,9 @@ struct DivIeee {
 
 template <class Div>
 __device__ __forceinline__ float rate_eval_t(Div &dv, int form, float r, float mid, float scale, float V) {
+#ifdef EDENHIP_LEMS_RATES
+    if (form == RATE_LEMS) return ::lems_rate((int)r, V);
+#endif
     const bool sig = form == RATE_SIGMOID, lin = form == RATE_EXPLINEAR;
     const float x = dv(sig ? (mid - V) : (V - mid), scale);
     const float e = exp_f(lin ? -x : x);
@@ -376,6 +385,39 @@ struct ExtChanArgs {
     float2 *out;               // [NC] the compartment's (I, G) of the step
     int32_t n;
 };
+
+// One lane of ext_channels_kernel: a compartment's channels in distribution order, each from
+// its gates' present values (each gate `instances` times, conductance_scaling = 1), the gates
+// moving to their next values, currents and conductances summed from zero
+// (eden/Eden.cpp:5679-5898), as channels_step does for the compartments it holds.
+__device__ __forceinline__ void ext_channels(const DevTables &T, const DGate *__restrict__ gates,
+                                             const ExtChanArgs &X, int64_t step, float dt) {
+    const int32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= X.n) return;
+    const int32_t g = X.comps[idx];
+    const bool init = step <= 0;
+    const float V = T.V[g], A = T.A[g];
+    float I_ch = 0.0f, G_ch = 0.0f;
+    for (int32_t c = X.ch_begin[idx]; c < X.ch_begin[idx + 1]; c++) {
+        const int32_t type = X.ch_type[c];
+        const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
+        float fopen = 1.0f;
+        for (int gi = 0; gi < ng; gi++) {
+            const DGate G = gates[type * GATES_PER_TYPE + gi];
+            float *qp = X.q + X.q_begin[c] + gi;
+            const float qs = *qp;
+            const int inst = (G.forms >> 16) & 0xFF;
+            for (int i = 0; i < inst; i++) fopen = fopen * qs;
+            *qp = gate_next(G, V, qs, init, dt);
+        }
+        const float Gscaled = X.ch_g[c] * fopen;
+        const float iDensity = Gscaled * (X.ch_e[c] - V) * 1e-5f;
+        const float gDensity = (Gscaled) * 1e-5f;
+        I_ch += (iDensity * A);
+        G_ch += (gDensity * A);
+    }
+    X.out[g] = make_float2(I_ch, G_ch);
+}
 
 // Per-compartment constants cached in registers for the whole time loop.
 // gmeta bit 27: the compartment has pulse inputs (the first one is cached here); bit 31: it

@@ -163,6 +163,7 @@ struct edenhip_ctx {
     int32_t *d_x_comp = nullptr, *d_x_chb = nullptr, *d_x_type = nullptr, *d_x_qb = nullptr;
     float *d_x_g = nullptr, *d_x_e = nullptr, *d_x_q = nullptr;
     float2 *d_chx_ext = nullptr;
+    hipFunction_t ext_rtc_fn = nullptr;  // the pre-pass compiled with the model's user rate types
     float2 *d_pk_state = nullptr;        // per Poisson spiker: (tsince, isi)
     float *d_pk_isi = nullptr;
     float2 *d_pk_window = nullptr;
@@ -444,7 +445,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     }
     ctx->n_acells = (int32_t)acell_comp.size();
 
-    // channel type table
+    // channel type table (types with user rate ComponentTypes run in the run-time compiled
+    // channel pre-pass)
+    const char *rates_src = d->abi_version >= 4 ? d->lems_rates_source : nullptr;
+    std::vector<char> lems_rate_type((size_t)std::max(d->n_channel_types, 1), 0);
     std::vector<DGate> gates((size_t)std::max(d->n_channel_types, 1) * GATES_PER_TYPE);
     std::memset(gates.data(), 0, gates.size() * sizeof(DGate));
     for (int t = 0; t < d->n_channel_types; t++) {
@@ -453,8 +457,12 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         for (int gi = 0; gi < ct.n_gates; gi++) {
             const edenhip_gate &g = ct.gates[gi];
             if (g.instances < 0 || g.instances > 255) return fail(ctx, EDENHIP_E_INVALID, "gate instances out of range");
-            for (const edenhip_rate *r : {&g.alpha, &g.beta})
-                if (r->form < 0 || r->form > EDENHIP_RATE_FIXED) return fail(ctx, EDENHIP_E_INVALID, "unknown rate form");
+            for (const edenhip_rate *r : {&g.alpha, &g.beta}) {
+                if (r->form < 0 || r->form > EDENHIP_RATE_LEMS) return fail(ctx, EDENHIP_E_INVALID, "unknown rate form");
+                if (r->form == EDENHIP_RATE_LEMS && !rates_src)
+                    return fail(ctx, EDENHIP_E_INVALID, "a LEMS rate form needs lems_rates_source");
+                if (r->form == EDENHIP_RATE_LEMS) lems_rate_type[(size_t)t] = 1;
+            }
             const int32_t kind = (d->abi_version >= 4 && d->gate_kind) ? d->gate_kind[t * EDENHIP_MAX_GATES + gi]
                                                                        : EDENHIP_GATE_RATES;
             if (kind != EDENHIP_GATE_RATES && kind != EDENHIP_GATE_TAU_INF)
@@ -505,12 +513,14 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         vthr[g] = d->v_threshold ? d->v_threshold[g] : NAN;
         const int64_t cb = d->comp_chan_begin[g], ce = d->comp_chan_begin[g + 1];
         int64_t n_gvars = 0;
+        bool lems_rates_here = false;
         for (int64_t ch = cb; ch < ce; ch++) {
             const int ty = d->chan_type[ch];
             if (ty < 0 || ty >= d->n_channel_types) return fail(ctx, EDENHIP_E_INVALID, "channel type index out of range");
             n_gvars += d->channel_types[ty].n_gates;
+            lems_rates_here |= lems_rate_type[(size_t)ty] != 0;
         }
-        if (ce - cb > NCH || n_gvars > NQ) {
+        if (ce - cb > NCH || n_gvars > NQ || lems_rates_here) {
             if (acell_slot[g] >= 0) return fail(ctx, EDENHIP_E_INVALID, "a point neuron takes no channel distributions");
             ext_comp[g] = 1;
             x_comp.push_back((int32_t)g);
@@ -1053,8 +1063,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             }
         }
     }
-    if ((lems && n_syn > 0) || ctx->n_acells > 0) {
-        // the generated synapse and point-neuron functions, each side stubbed when absent
+    const bool rates_rtc = rates_src && ctx->n_xcomp > 0;
+    if ((lems && n_syn > 0) || ctx->n_acells > 0 || rates_rtc) {
+        // the generated synapse, point-neuron and rate functions, each stubbed when absent
         std::string gen = lems ? std::string(LS->source)
                                : std::string("__device__ __forceinline__ void lems_syn(int, const float *, const float *, "
                                              "float *, long, float, float, float, int, int, float, float *i, float *g) "
@@ -1063,11 +1074,15 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         gen += LC ? std::string(LC->source)
                   : std::string("__device__ __forceinline__ void lems_cell(int, const float *, const float *, float *, "
                                 "long, float, float, int, float, int, long long, int *spike) { *spike = 0; }\n");
+        gen += "\n";
+        gen += rates_src ? std::string(rates_src)
+                         : std::string("__device__ float lems_rate(int, float) { return 0.0f; }\n");
         std::string err;
-        hipFunction_t syn_fn = nullptr, cell_fn = nullptr;
-        if (rtc_lems_kernel(gen, &syn_fn, &cell_fn, err) != 0) return fail(ctx, EDENHIP_E_INVALID, err);
+        hipFunction_t syn_fn = nullptr, cell_fn = nullptr, ext_fn = nullptr;
+        if (rtc_lems_kernel(gen, &syn_fn, &cell_fn, &ext_fn, err) != 0) return fail(ctx, EDENHIP_E_INVALID, err);
         if (lems && n_syn > 0) ctx->lems_fn = syn_fn;
         if (ctx->n_acells > 0) ctx->lems_cell_fn = cell_fn;
+        if (rates_rtc) ctx->ext_rtc_fn = ext_fn;
     }
     if (ctx->n_acells > 0) {
         // point-neuron tables: compartments, types, parameters, state [variable][slot]
@@ -1528,8 +1543,12 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &ctx->d_gates, &xa, &st, &dt};
-                HIPCHECK(ctx, hipLaunchKernel(ext_channels_kernel_ptr(), dim3((ctx->n_xcomp + 63) / 64), dim3(64), args, 0,
-                                              ctx->stream));
+                if (ctx->ext_rtc_fn)
+                    HIPCHECK(ctx, hipModuleLaunchKernel(ctx->ext_rtc_fn, (unsigned)((ctx->n_xcomp + 63) / 64), 1, 1, 64, 1, 1,
+                                                        0, ctx->stream, args, nullptr));
+                else
+                    HIPCHECK(ctx, hipLaunchKernel(ext_channels_kernel_ptr(), dim3((ctx->n_xcomp + 63) / 64), dim3(64), args,
+                                                  0, ctx->stream));
                 nl++;
             }
             if (ctx->n_point > 0) {

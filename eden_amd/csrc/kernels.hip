@@ -892,31 +892,7 @@ const void *tree_kernel_ptr(int tpb, bool split, bool net) {
 // (eden/Eden.cpp:5679-5898), as channels_step does for the others.
 __global__ void __launch_bounds__(64) ext_channels_kernel(DevTables T, const DGate *__restrict__ gates, ExtChanArgs X,
                                                           int64_t step, float dt) {
-    const int32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= X.n) return;
-    const int32_t g = X.comps[idx];
-    const bool init = step <= 0;
-    const float V = T.V[g], A = T.A[g];
-    float I_ch = 0.0f, G_ch = 0.0f;
-    for (int32_t c = X.ch_begin[idx]; c < X.ch_begin[idx + 1]; c++) {
-        const int32_t type = X.ch_type[c];
-        const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
-        float fopen = 1.0f;
-        for (int gi = 0; gi < ng; gi++) {
-            const DGate G = gates[type * GATES_PER_TYPE + gi];
-            float *qp = X.q + X.q_begin[c] + gi;
-            const float qs = *qp;
-            const int inst = (G.forms >> 16) & 0xFF;
-            for (int i = 0; i < inst; i++) fopen = fopen * qs;
-            *qp = gate_next(G, V, qs, init, dt);
-        }
-        const float Gscaled = X.ch_g[c] * fopen;
-        const float iDensity = Gscaled * (X.ch_e[c] - V) * 1e-5f;
-        const float gDensity = (Gscaled) * 1e-5f;
-        I_ch += (iDensity * A);
-        G_ch += (gDensity * A);
-    }
-    X.out[g] = make_float2(I_ch, G_ch);
+    ext_channels(T, gates, X, step, dt);
 }
 
 const void *ext_channels_kernel_ptr() { return reinterpret_cast<const void *>(&ext_channels_kernel); }

@@ -22,6 +22,7 @@
 //
 // The generated source (`lems_syn` and `lems_cell`) is inserted where
 // lems_synapse_functions.h is included.
+#define EDENHIP_LEMS_RATES 1
 #include "device_model.hpp"
 
 #define EXP edenhip::exp_f
@@ -185,4 +186,13 @@ lems_cell_kernel(edenhip::DevTables T, edenhip::LemsCellArgs A, int64_t step, fl
         send_spike(T, g, nxt);
         log_event(T, T.ev_col[g], step);
     }
+}
+
+// Channels of the compartments run outside the compartment kernels, here with the user rate
+// ComponentTypes' generated lems_rate (gates of form RATE_LEMS); the statically compiled
+// ext_channels_kernel runs the same code for models without them.
+extern "C" __global__ void __launch_bounds__(64)
+ext_channels_kernel_rtc(edenhip::DevTables T, const edenhip::DGate *__restrict__ gates, edenhip::ExtChanArgs X,
+                        int64_t step, float dt) {
+    edenhip::ext_channels(T, gates, X, step, dt);
 }

@@ -20,12 +20,13 @@ namespace {
 
 struct Compiled {
     hipModule_t module = nullptr;
-    hipFunction_t fn = nullptr, cell_fn = nullptr;
+    hipFunction_t fn = nullptr, cell_fn = nullptr, ext_fn = nullptr;
 };
 
 }  // namespace
 
-int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, hipFunction_t *cell_fn, std::string &err) {
+int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, hipFunction_t *cell_fn, hipFunction_t *ext_fn,
+                    std::string &err) {
     static std::mutex mu;
     static std::map<std::string, Compiled> cache;   // key: device arch + '\n' + source
     int dev = 0;
@@ -47,6 +48,7 @@ int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, hipFunction
     if (it != cache.end()) {
         *fn = it->second.fn;
         *cell_fn = it->second.cell_fn;
+        *ext_fn = it->second.ext_fn;
         return 0;
     }
     hiprtcProgram prog;
@@ -91,13 +93,15 @@ int rtc_lems_kernel(const std::string &generated, hipFunction_t *fn, hipFunction
         return -1;
     }
     if (hipModuleGetFunction(&c.fn, c.module, "lems_syn_kernel") != hipSuccess ||
-        hipModuleGetFunction(&c.cell_fn, c.module, "lems_cell_kernel") != hipSuccess) {
-        err = "lems_syn_kernel / lems_cell_kernel not found in the compiled module";
+        hipModuleGetFunction(&c.cell_fn, c.module, "lems_cell_kernel") != hipSuccess ||
+        hipModuleGetFunction(&c.ext_fn, c.module, "ext_channels_kernel_rtc") != hipSuccess) {
+        err = "lems_syn_kernel / lems_cell_kernel / ext_channels_kernel_rtc not found in the compiled module";
         return -1;
     }
     cache[key] = c;
     *fn = c.fn;
     *cell_fn = c.cell_fn;
+    *ext_fn = c.ext_fn;
     return 0;
 }
 

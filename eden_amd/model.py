@@ -129,6 +129,7 @@ class Model:
     # per channel type, per gate slot (EDENHIP_MAX_GATES of them): its kind (0 alpha/beta
     # rates, 1 tau/inf), or None for rates everywhere
     gate_kind: np.ndarray = None
+    lems_rates: object = None          # eden_amd.lems_rates.LemsRates: user gate rate ComponentTypes, or None
     name: str = "model"
 
     def __post_init__(self):
@@ -258,7 +259,8 @@ class Model:
         d.syn_types = C.cast(self._st, C.c_void_p)
         for name, _ in N.ModelDesc._fields_:
             if name in ("abi_version", "dt", "t_final", "n_cells", "n_comps", "n_channel_types", "channel_types",
-                        "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems", "lems_cells"):
+                        "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems", "lems_cells",
+                        "lems_rates_source"):
                 continue
             setattr(d, name, N.ptr(getattr(self, name)))
         if self.lems is not None:
@@ -315,6 +317,9 @@ class Model:
             self._lc_desc = N.LemsCellsDesc(len(LC.params), a[5], N.ptr(a[0]), N.ptr(a[1]), N.ptr(a[2]), N.ptr(a[3]),
                                             N.ptr(a[4]), N.ptr(off), len(prm))
             d.lems_cells = C.pointer(self._lc_desc)
+        if self.lems_rates is not None:
+            self._rates_src = self.lems_rates.source_hip.encode()
+            d.lems_rates_source = self._rates_src
         d.n_record = len(self.record_comp)
         d.n_event_cols = len(self.event_comp)
         return d

@@ -10,7 +10,8 @@ set covers:
                     (eden/NeuroML.cpp:9813-9891)
   ion channels      ionChannelHH / ionChannel with gateHHrates (HHExpRate, HHExpLinearRate,
                     HHSigmoidRate), gateHHtauInf and gateHHratesTauInf (fixedTimeCourse,
-                    HHExpVariable, HHExpLinearVariable, HHSigmoidVariable), q10Settings
+                    HHExpVariable, HHExpLinearVariable, HHSigmoidVariable), user rate, time
+                    course and steady state ComponentTypes (eden_amd.lems_rates), q10Settings
                     (q10Fixed, q10ExpTemp); channels without gates
                     are passive (eden/NeuroML.cpp:4380-4598)
   cells             <cell>: morphology (segments, parent / fractionAlong, implicit proximal
@@ -93,6 +94,7 @@ from . import _native as N
 from . import custom_setup as setup_mod
 from . import lems as lems_mod
 from . import lems_cells as cells_mod
+from . import lems_rates as rates_mod
 from .model import Model, cell_rng_seeds, channel_type
 
 
@@ -298,8 +300,10 @@ def _as_element(comp):
 _RATE_FORMS = {"HHExpRate": N.RATE_EXP, "HHExpLinearRate": N.RATE_EXPLINEAR, "HHSigmoidRate": N.RATE_SIGMOID}
 
 
-def _rate(el, what):
+def _rate(el, what, user=None):
     t = el.get("type")
+    if t not in _RATE_FORMS and user is not None:
+        return user(el, "baseVoltageDepRate", what)
     if t not in _RATE_FORMS:
         raise NeuroMLError(f"{what}: rate type '{t}' is not supported (HHExpRate, HHExpLinearRate, HHSigmoidRate)")
     r = convert(el.get("rate"), "frequency", what + " rate")
@@ -315,10 +319,12 @@ _VARIABLE_FORMS = {"HHExpVariable": N.RATE_EXP, "HHExpLinearVariable": N.RATE_EX
 _GATE_KINDS = {"gateHHrates": N.GATE_RATES, "gateHHtauInf": N.GATE_TAU_INF, "gateHHratesTauInf": N.GATE_TAU_INF}
 
 
-def _steady_state(el, what):
+def _steady_state(el, what, user=None):
     """A steadyState element: HHExpVariable / HHExpLinearVariable / HHSigmoidVariable, its
-    rate a plain number (eden/NeuroML.cpp:4028-4057)."""
+    rate a plain number (eden/NeuroML.cpp:4028-4057), or a user ComponentType."""
     t = el.get("type")
+    if t not in _VARIABLE_FORMS and user is not None:
+        return user(el, "baseVoltageDepVariable", what)
     if t not in _VARIABLE_FORMS:
         raise NeuroMLError(f"{what}: steadyState type '{t}' is not supported ({', '.join(_VARIABLE_FORMS)})")
     r = convert(el.get("rate"), None, what + " rate")
@@ -327,15 +333,19 @@ def _steady_state(el, what):
     return N.Rate(_VARIABLE_FORMS[t], _f32(r), _f32(mid), _f32(sc))
 
 
-def _time_course(el, what):
-    """A timeCourse element: fixedTimeCourse, tau in us (eden/NeuroML.cpp:4087-4108)."""
+def _time_course(el, what, user=None):
+    """A timeCourse element: fixedTimeCourse, tau in us (eden/NeuroML.cpp:4087-4108), or a
+    user ComponentType."""
+    if el.get("type") != "fixedTimeCourse" and user is not None:
+        return user(el, "baseVoltageDepTime", what)
     if el.get("type") != "fixedTimeCourse":
         raise NeuroMLError(f"{what}: timeCourse type '{el.get('type')}' is not supported (fixedTimeCourse)")
     return N.Rate(N.RATE_FIXED, _f32(convert(el.get("tau"), "time", what + " tau")), 0.0, 0.0)
 
 
-def _channel(el, temperature):
-    """(channel type, gate kinds) of an ion channel element."""
+def _channel(el, temperature, user=None):
+    """(channel type, gate kinds) of an ion channel element; `user(element, base type, what)`
+    turns a user rate ComponentType into an EDENHIP_RATE_LEMS rate."""
     cid = el.get("id")
     gates, kinds = [], []
     for g in el:
@@ -353,13 +363,13 @@ def _channel(el, temperature):
         for c in g:
             ct = _tag(c)
             if ct == "forwardRate" and kind == N.GATE_RATES:
-                fw = _rate(c, f"channel {cid} gate {g.get('id')} forward")
+                fw = _rate(c, f"channel {cid} gate {g.get('id')} forward", user)
             elif ct == "reverseRate" and kind == N.GATE_RATES:
-                rv = _rate(c, f"channel {cid} gate {g.get('id')} reverse")
+                rv = _rate(c, f"channel {cid} gate {g.get('id')} reverse", user)
             elif ct == "timeCourse":
-                tau = _time_course(c, f"channel {cid} gate {g.get('id')} timeCourse")
+                tau = _time_course(c, f"channel {cid} gate {g.get('id')} timeCourse", user)
             elif ct == "steadyState":
-                inf = _steady_state(c, f"channel {cid} gate {g.get('id')} steadyState")
+                inf = _steady_state(c, f"channel {cid} gate {g.get('id')} steadyState", user)
             elif ct == "q10Settings":
                 qt = c.get("type")
                 if qt == "q10Fixed":
@@ -922,6 +932,15 @@ class _Pop:
             raise NeuroMLError(f"{what}: instance {inst_id} not in population {self.id}") from None
 
 
+def _rates_or_none(entries, temperature):
+    if not entries:
+        return None
+    try:
+        return rates_mod.build(entries, temperature)
+    except lems_mod.LemsError as e:
+        raise NeuroMLError(str(e)) from None
+
+
 def _custom_setup(doc, sim):
     """The statements of the simulation's EdenCustomSetup file, if it names one (its file name
     relative to the file holding the Simulation, eden/NeuroML.cpp:8302-8313, 7281-7283)."""
@@ -1109,10 +1128,34 @@ def read(path, rng_seed=None):
 
     # channel types in document order, then the cells the network uses
     channel_index, channel_types, gate_ids, gate_kinds = {}, [], {}, []
+    rate_entries, rate_types = [], {}
+
+    def user_rate(el, base, what):
+        """A user rate / time course / steady state ComponentType instance: its rate of form
+        EDENHIP_RATE_LEMS (eden/NeuroML.cpp:4028-4108)."""
+        t = el.get("type")
+        if rates_mod.base_rate_of(t, doc.component_types) != base:
+            raise NeuroMLError(f"{what}: type '{t}' is not supported (a built-in form or a user ComponentType "
+                               f"extending {base})")
+        if t not in rate_types:
+            try:
+                rate_types[t] = rates_mod.rate_type_from_lems(
+                    t, doc.component_types, lambda text, d: _lems_value(text, d, f"ComponentType {t} constant"))
+            except lems_mod.LemsError as e:
+                raise NeuroMLError(str(e)) from None
+        rt = rate_types[t]
+        vals = {}
+        for p, d in rt.params:
+            if el.get(p) is None:
+                raise NeuroMLError(f"{what}: parameter {p} of {t} missing")
+            vals[p] = _lems_value(el.get(p), d, f"{what} {p}")
+        rate_entries.append((rt, vals))
+        return N.Rate(N.RATE_LEMS, float(len(rate_entries) - 1), 0.0, 0.0)
+
     for role, i in doc.order:
         if role == "channel":
             channel_index[i] = len(channel_types)
-            ct_, kinds_ = _channel(doc.items[i][1], temperature)
+            ct_, kinds_ = _channel(doc.items[i][1], temperature, user_rate)
             channel_types.append(ct_)
             gate_kinds += kinds_
             gate_ids[i] = [g.get("id") for g in doc.items[i][1] if _tag(g) not in ("notes", "annotation")]
@@ -1668,6 +1711,7 @@ def read(path, rng_seed=None):
         record_gate=np.array(rec_gate, np.int32) if any(j >= 0 for j in rec_gate) else None,
         lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
         gate_kind=np.array(gate_kinds, np.int32) if any(gate_kinds) else None,
+        lems_rates=_rates_or_none(rate_entries, temperature),
         syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
         syn_spike_list=syn_spike_list, spike_times=spike_times,
         syn_poisson_spiker=syn_poisson_spiker, poisson_spikers=poisson_spikers,

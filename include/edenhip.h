@@ -49,7 +49,10 @@ enum {
     EDENHIP_RATE_EXP = 0,        /* r * exp((V - mid) / scale) */
     EDENHIP_RATE_EXPLINEAR = 1,  /* r * (V == mid ? 1 : x / (1 - exp(-x))), x = (V - mid) / scale */
     EDENHIP_RATE_SIGMOID = 2,    /* r / (1 + exp((mid - V) / scale)) */
-    EDENHIP_RATE_FIXED = 3       /* r */
+    EDENHIP_RATE_FIXED = 3,      /* r */
+    EDENHIP_RATE_LEMS = 4        /* (abi_version >= 4) lems_rate((int)r, V): instance r of a user
+                                    rate, time-course or steady-state ComponentType, from
+                                    edenhip_model_desc.lems_rates_source */
 };
 
 #define EDENHIP_MAX_GATES 4
@@ -277,6 +280,14 @@ typedef struct edenhip_model_desc {
      * (gate k of type t at t * EDENHIP_MAX_GATES + k), or NULL: every gate EDENHIP_GATE_RATES.
      * The adjoint takes EDENHIP_GATE_RATES gates only. */
     const int32_t *gate_kind;
+
+    /* (abi_version >= 4) HIP source defining
+     *   __device__ float lems_rate(int k, float v)
+     * (instance k of a user gate rate / time course / steady state ComponentType at membrane
+     * voltage v; eden_amd.lems_rates emits it), for gates with EDENHIP_RATE_LEMS forms, or
+     * NULL. edenhip_create() compiles it with hiprtc into the channel pre-pass; compartments
+     * with such gates run their channels there (one launch per step). */
+    const char *lems_rates_source;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

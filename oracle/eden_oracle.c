@@ -124,6 +124,12 @@ float FN(randof)(float x, int32_t work_item, int64_t instance, int64_t step, int
     return randof_f(x, work_item, instance, step, invocation);
 }
 
+/* User rate / time course / steady state ComponentTypes (EDENHIP_RATE_LEMS): the C dialect of
+ * the model's lems_rate source, compiled by the caller */
+typedef float (*lems_rate_fn_t)(int k, float v);
+static lems_rate_fn_t g_lems_rate = NULL;
+void FN(set_lems_rates)(void *fn) { g_lems_rate = (lems_rate_fn_t)fn; }
+
 /* A gate's EDENHIP_GATE_* kind (EDENHIP_GATE_RATES without a kind table). */
 static int gate_kind(const edenhip_model_desc *d, int32_t type, int gi) {
     return (d->abi_version >= 4 && d->gate_kind) ? d->gate_kind[type * EDENHIP_MAX_GATES + gi] : EDENHIP_GATE_RATES;
@@ -138,6 +144,8 @@ static real rate_value(const edenhip_rate *r, real V) {
         return rate * ((V == mid) ? (real)1 : (((V - mid) / scale) / ((real)1 - EXP(-(V - mid) / scale))));
     case EDENHIP_RATE_SIGMOID:
         return rate / ((real)1 + EXP((mid - V) / scale));
+    case EDENHIP_RATE_LEMS:
+        return (real)g_lems_rate((int)r->rate, (float)V);
     default:
         return rate;
     }
@@ -647,6 +655,18 @@ int64_t FN(run)(const edenhip_model_desc *d, int64_t n_rows, int32_t substeps, i
     if (LC) return -1; /* point neurons: fp32 flavours only */
 #endif
     if (LC && !g_lems_cell) return -1;
+    {
+        /* user rate ComponentTypes: fp32 flavours, with their compiled functions */
+        int lems_rates = 0;
+        for (int32_t t = 0; t < d->n_channel_types; t++)
+            for (int gi = 0; gi < d->channel_types[t].n_gates; gi++)
+                lems_rates |= d->channel_types[t].gates[gi].alpha.form == EDENHIP_RATE_LEMS ||
+                              d->channel_types[t].gates[gi].beta.form == EDENHIP_RATE_LEMS;
+#if defined(ORACLE_F64)
+        if (lems_rates) return -1;
+#endif
+        if (lems_rates && !g_lems_rate) return -1;
+    }
     const int64_t NC = d->n_comps;
     const int64_t NCH = d->comp_chan_begin[NC];
     const int64_t NSY = d->comp_syn_begin[NC];

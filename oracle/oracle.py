@@ -49,7 +49,7 @@ def lib():
             pc = getattr(L, f"oracle_{sfx}_poisson_log_count")
             pc.restype = C.c_int64
             pc.argtypes = []
-            for nm in ("set_lems", "set_lems_cells"):
+            for nm in ("set_lems", "set_lems_cells", "set_lems_rates"):
                 sl = getattr(L, f"oracle_{sfx}_{nm}")
                 sl.restype = None
                 sl.argtypes = [C.c_void_p]
@@ -182,6 +182,12 @@ def run(model, n_rows=-1, precision="f32", substeps=1, threads=0, max_events=10_
             raise RuntimeError("LEMS point neurons: the oracle runs them in its fp32 flavours only")
         fn = C.cast(lems_library(model.lems_cells.source_c, flav).lems_cell, C.c_void_p)
         getattr(L, f"oracle_{precision}_set_lems_cells")(fn)
+    if getattr(model, "lems_rates", None) is not None:
+        flav = {"f32": "exact", "f32libm": "libm", "f32ofast": "ofast"}.get(precision)
+        if flav is None:
+            raise RuntimeError("user rate ComponentTypes: the oracle runs them in its fp32 flavours only")
+        fn = C.cast(lems_library(model.lems_rates.source_c, flav).lems_rate, C.c_void_p)
+        getattr(L, f"oracle_{precision}_set_lems_rates")(fn)
     total = model.total_rows() if n_rows < 0 else n_rows
     nrec = len(model.record_comp)
     trace = np.zeros((max(total, 1), max(nrec, 1)), np.float32) if want_trace else None
