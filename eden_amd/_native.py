@@ -15,7 +15,7 @@ ABI_VERSION = 4
 MAX_GATES = 4
 
 RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED, RATE_LEMS = 0, 1, 2, 3, 4
-GATE_RATES, GATE_TAU_INF = 0, 1
+GATE_RATES, GATE_TAU_INF, GATE_INSTANTANEOUS = 0, 1, 2
 
 E_OK, E_INVALID, E_DEVICE, E_LIMIT, E_STATE = 0, -1, -2, -3, -4
 

@@ -405,10 +405,13 @@ __device__ __forceinline__ void ext_channels(const DevTables &T, const DGate *__
         for (int gi = 0; gi < ng; gi++) {
             const DGate G = gates[type * GATES_PER_TYPE + gi];
             float *qp = X.q + X.q_begin[c] + gi;
-            const float qs = *qp;
+            // an instantaneous gate (kind 2) is its steady state at this step's voltage
+            const bool instantaneous = ((G.forms >> 4) & 0xF) == 2;
+            DivIeee dv;
+            const float qs = instantaneous ? rate_eval_t(dv, (G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V) : *qp;
             const int inst = (G.forms >> 16) & 0xFF;
             for (int i = 0; i < inst; i++) fopen = fopen * qs;
-            *qp = gate_next(G, V, qs, init, dt);
+            *qp = instantaneous ? qs : gate_next(G, V, qs, init, dt);
         }
         const float Gscaled = X.ch_g[c] * fopen;
         const float iDensity = Gscaled * (X.ch_e[c] - V) * 1e-5f;

@@ -465,8 +465,10 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             }
             const int32_t kind = (d->abi_version >= 4 && d->gate_kind) ? d->gate_kind[t * EDENHIP_MAX_GATES + gi]
                                                                        : EDENHIP_GATE_RATES;
-            if (kind != EDENHIP_GATE_RATES && kind != EDENHIP_GATE_TAU_INF)
+            if (kind != EDENHIP_GATE_RATES && kind != EDENHIP_GATE_TAU_INF && kind != EDENHIP_GATE_INSTANTANEOUS)
                 return fail(ctx, EDENHIP_E_INVALID, "unknown gate kind");
+            // instantaneous gates take the step's voltage: their channels run in the pre-pass
+            if (kind == EDENHIP_GATE_INSTANTANEOUS) lems_rate_type[(size_t)t] = 1;
             DGate &G = gates[(size_t)t * GATES_PER_TYPE + gi];
             G.forms = g.alpha.form | (kind << 4) | (g.beta.form << 8) | (g.instances << 16) | (ct.n_gates << 24);
             G.q10 = g.q10;

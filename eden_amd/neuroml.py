@@ -9,7 +9,7 @@ set covers:
                     the including file; NeuroML's core type files are skipped
                     (eden/NeuroML.cpp:9813-9891)
   ion channels      ionChannelHH / ionChannel with gateHHrates (HHExpRate, HHExpLinearRate,
-                    HHSigmoidRate), gateHHtauInf and gateHHratesTauInf (fixedTimeCourse,
+                    HHSigmoidRate), gateHHtauInf, gateHHratesTauInf, gateHHInstantaneous (fixedTimeCourse,
                     HHExpVariable, HHExpLinearVariable, HHSigmoidVariable), user rate, time
                     course and steady state ComponentTypes (eden_amd.lems_rates), q10Settings
                     (q10Fixed, q10ExpTemp); channels without gates
@@ -316,7 +316,8 @@ _VARIABLE_FORMS = {"HHExpVariable": N.RATE_EXP, "HHExpLinearVariable": N.RATE_EX
                    "HHSigmoidVariable": N.RATE_SIGMOID}
 # gate elements by kind: alpha/beta rates, or tau and inf given (a gateHHratesTauInf's rates
 # play no part in its dynamics, eden/Eden.cpp:5282-5383)
-_GATE_KINDS = {"gateHHrates": N.GATE_RATES, "gateHHtauInf": N.GATE_TAU_INF, "gateHHratesTauInf": N.GATE_TAU_INF}
+_GATE_KINDS = {"gateHHrates": N.GATE_RATES, "gateHHtauInf": N.GATE_TAU_INF, "gateHHratesTauInf": N.GATE_TAU_INF,
+               "gateHHInstantaneous": N.GATE_INSTANTANEOUS}
 
 
 def _steady_state(el, what, user=None):
@@ -386,6 +387,10 @@ def _channel(el, temperature, user=None):
             if fw is None or rv is None:
                 raise NeuroMLError(f"channel {cid} gate {g.get('id')}: forwardRate and reverseRate are required")
             gates.append((inst, fw, rv, _f32(q10)))
+        elif kind == N.GATE_INSTANTANEOUS:
+            if inf is None:
+                raise NeuroMLError(f"channel {cid} gate {g.get('id')}: steadyState is required")
+            gates.append((inst, N.Rate(N.RATE_FIXED, 0.0, 0.0, 0.0), inf, _f32(q10)))
         else:
             if tau is None or inf is None:
                 raise NeuroMLError(f"channel {cid} gate {g.get('id')}: timeCourse and steadyState are required")

@@ -66,7 +66,11 @@ enum {
  * HHExpLinearVariable, HHSigmoidVariable). Both then take the same cnexp update. */
 enum {
     EDENHIP_GATE_RATES = 0,
-    EDENHIP_GATE_TAU_INF = 1
+    EDENHIP_GATE_TAU_INF = 1,
+    /* (gateHHInstantaneous) no state: the gate's value in a step is its steady state, held in
+     * its `beta`, at the step's voltage (eden/Eden.cpp:5269-5281); compartments with such
+     * gates run their channels in the channel pre-pass (one launch per step) */
+    EDENHIP_GATE_INSTANTANEOUS = 2
 };
 
 typedef struct edenhip_rate {

@@ -317,12 +317,20 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                 /* rates give tau and inf; a tau/inf gate (gateHHtauInf) gives them directly, its
                    time course in the alpha slot, its steady state in the beta slot
                    (eden/Eden.cpp:5282-5383) */
-                const int tau_inf = gate_kind(d, d->chan_type[ch], gi) == EDENHIP_GATE_TAU_INF;
+                const int kind = gate_kind(d, d->chan_type[ch], gi);
+                const int tau_inf = kind == EDENHIP_GATE_TAU_INF;
                 real alpha = rate_value(&gt->alpha, Vcomp);
                 real beta = rate_value(&gt->beta, Vcomp);
                 real tau = tau_inf ? alpha : (real)1 / (alpha + beta);
                 real inf = tau_inf ? beta : alpha / (alpha + beta);
                 real qn;
+                if (kind == EDENHIP_GATE_INSTANTANEOUS) {
+                    /* no state: the steady state at this step's voltage (eden/Eden.cpp:5269-5281) */
+                    q = beta;
+                    s->q[nxt][qb + gi] = q;
+                    for (int k = 0; k < gt->instances; k++) fopen = fopen * q;
+                    continue;
+                }
                 if (initial_state) {
                     qn = inf;
                 } else {

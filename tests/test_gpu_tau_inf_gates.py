@@ -74,3 +74,19 @@ def test_tau_inf_gates_bit_exact(tmp_path, opts):
     assert np.array_equal(ev[0], ec) and np.array_equal(ev[1], er)
     assert float(np.ptp(tr[:, 4])) > 0.05 and float(np.ptp(tr[:, 5])) > 0.05
     assert float(np.ptp(tr[:, 2])) > 5.0
+
+
+@pytest.mark.parametrize("opts", [{}, dict(use_graph=True)])
+def test_instantaneous_gates_bit_exact(tmp_path, opts):
+    """gateHHInstantaneous: its channel runs in the channel pre-pass with the step's own
+    voltage; traces bit for bit against the oracle."""
+    from test_tau_inf_gates_cpu import _model as ti_model
+    gate = ('<gateHHInstantaneous id="n" instances="2"><steadyState type="HHSigmoidVariable" rate="1" '
+            'scale="4mV" midpoint="-60mV"/></gateHHInstantaneous>')
+    m = ti_model(tmp_path, gate, record_q=False, n_cells=3)
+    print(f"[tau-inf] instantaneous gate, opts {opts}", flush=True)
+    tr_r, _, _, _ = oracle.run(m)
+    with Simulation(m, **opts) as sim:
+        tr = sim.run_all()
+    assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))
+    assert float(np.ptp(tr[:, 0])) > 1.0

@@ -6,8 +6,11 @@ CPU: the reader's tables and the oracle's dynamics.
   fixed time course tau taken as it is (not 1 / (alpha + beta)) and inf the sigmoid
   steady state rate / (1 + exp((mid - V) / scale)) of the step's start voltage, evaluated here
   in float32 from the recorded voltage trace (eden/Eden.cpp:5282-5383, 5236-5259).
-* gateHHratesTauInf reads like gateHHtauInf (its rates play no part); gateHHratesTau,
-  gateHHratesInf and gateHHInstantaneous are refused with their names.
+* gateHHratesTauInf reads like gateHHtauInf (its rates play no part); gateHHratesTau and
+  gateHHratesInf are refused with their names.
+* gateHHInstantaneous (eden/Eden.cpp:5269-5281): no state; in each step the channel's open
+  fraction is the steady state at that step's own voltage. The first step from rest, by hand:
+  V1 = (V0 + dt (I + V0 G) / C) / (1 + dt G / C) with the channel's I = g x(V0)^k (E - V0) 1e-5 A.
 """
 import math
 import os
@@ -23,7 +26,7 @@ from oracle import oracle
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def _model(tmp_path, gate_xml, q10="", length_ms=60.0):
+def _model(tmp_path, gate_xml, q10="", length_ms=60.0, record_q=True, n_cells=1):
     p = tmp_path / "ti.xml"
     p.write_text(
         '<Lems><Target component="s"/><neuroml>'
@@ -37,11 +40,12 @@ def _model(tmp_path, gate_xml, q10="", length_ms=60.0):
         '<intracellularProperties><resistivity value="0.1 kohm_cm"/></intracellularProperties>'
         '</biophysicalProperties></cell>'
         '<pulseGenerator id="pg" delay="5ms" duration="30ms" amplitude="0.05nA"/>'
-        '<network id="n" temperature="6.3 degC"><population id="pp" component="P" size="1"/>'
+        f'<network id="n" temperature="6.3 degC"><population id="pp" component="P" size="{n_cells}"/>'
         '<explicitInput target="pp[0]" input="pg"/></network></neuroml>'
         f'<Simulation id="s" length="{length_ms}ms" step="0.025ms" target="n" seed="1">'
         '<OutputFile id="o" fileName="o.txt"><OutputColumn id="v" quantity="pp/0/P/0/v"/>'
-        '<OutputColumn id="q" quantity="pp/0/P/0/b/membraneProperties/k/kti/n/q"/></OutputFile>'
+        + ('<OutputColumn id="q" quantity="pp/0/P/0/b/membraneProperties/k/kti/n/q"/>' if record_q else '') +
+        '</OutputFile>'
         '</Simulation></Lems>')
     return neuroml.read(str(p))[0]
 
@@ -84,7 +88,49 @@ def test_rates_tau_inf_reads_as_tau_inf(tmp_path):
     assert np.array_equal(tr1, tr2)
 
 
-@pytest.mark.parametrize("gate", ["gateHHratesTau", "gateHHratesInf", "gateHHInstantaneous"])
+@pytest.mark.parametrize("gate", ["gateHHratesTau", "gateHHratesInf"])
 def test_other_gate_kinds_refused(tmp_path, gate):
     with pytest.raises(neuroml.NeuroMLError, match=gate):
         _model(tmp_path, f'<{gate} id="n" instances="1"/>')
+
+
+def test_instantaneous_gate_first_step_by_hand(tmp_path):
+    gate = ('<gateHHInstantaneous id="n" instances="2"><steadyState type="HHSigmoidVariable" rate="1" '
+            'scale="4mV" midpoint="-60mV"/></gateHHInstantaneous>')
+    m = _model(tmp_path, gate, length_ms=2.0)
+    assert m.gate_kind.tolist()[4] == N.GATE_INSTANTANEOUS
+    tr, _, _, _ = oracle.run(m)
+    V0 = -65.0
+    x = 1.0 / (1.0 + math.exp((-60.0 - V0) / 4.0))
+    A = float(m.area[0])
+    C = float(m.capacitance[0])
+    g_leak, g_k = 0.3 * 1e-5 * A, 2.0 * x * x * 1e-5 * A     # condDensity in mS/cm2 units of the engine
+    I = g_leak * (-65.0 - V0) + g_k * (-80.0 - V0)
+    G = g_leak + g_k
+    # the reference's semi-implicit step for a one-compartment cell (eden/Eden.cpp:6057-6107,
+    # 6114-6225 with N = 1): V' = V + dt (I + V G) / C, then V' / (1 + dt G / C)
+    V1 = (V0 + 25.0 * (I + V0 * G) / C) / (1.0 + 25.0 * G / C)
+    assert tr[1, 0] == pytest.approx(V1, abs=2e-5)
+    assert float(np.ptp(tr[:, 0])) > 1.0
+
+
+def test_instantaneous_gate_takes_the_step_s_own_voltage(tmp_path):
+    """Step 2 from the recorded V1: the open fraction uses x(V1), not the previous step's x(V0)
+    (which a gate updated after its use would give)."""
+    gate = ('<gateHHInstantaneous id="n" instances="2"><steadyState type="HHSigmoidVariable" rate="1" '
+            'scale="4mV" midpoint="-60mV"/></gateHHInstantaneous>')
+    m = _model(tmp_path, gate, length_ms=2.0)
+    tr, _, _, _ = oracle.run(m)
+    A, C = float(m.area[0]), float(m.capacitance[0])
+
+    def step(V, xv):
+        g_leak, g_k = 0.3 * 1e-5 * A, 2.0 * xv * xv * 1e-5 * A
+        I = g_leak * (-65.0 - V) + g_k * (-80.0 - V)
+        G = g_leak + g_k
+        return (V + 25.0 * (I + V * G) / C) / (1.0 + 25.0 * G / C)
+
+    x = lambda V: 1.0 / (1.0 + math.exp((-60.0 - V) / 4.0))
+    V1 = float(tr[1, 0])
+    want, lagged = step(V1, x(V1)), step(V1, x(-65.0))
+    assert tr[2, 0] == pytest.approx(want, abs=2e-5)
+    assert abs(lagged - want) > 1e-4
