@@ -15,7 +15,7 @@ ABI_VERSION = 4
 MAX_GATES = 4
 
 RATE_EXP, RATE_EXPLINEAR, RATE_SIGMOID, RATE_FIXED, RATE_LEMS = 0, 1, 2, 3, 4
-GATE_RATES, GATE_TAU_INF, GATE_INSTANTANEOUS = 0, 1, 2
+GATE_RATES, GATE_TAU_INF, GATE_INSTANTANEOUS, GATE_RATES_TAU, GATE_RATES_INF = 0, 1, 2, 3, 4
 
 E_OK, E_INVALID, E_DEVICE, E_LIMIT, E_STATE = 0, -1, -2, -3, -4
 
@@ -76,6 +76,7 @@ class ModelDesc(C.Structure):
         ("lems_cells", C.POINTER(LemsCellsDesc)),
         ("gate_kind", _P),
         ("lems_rates_source", C.c_char_p),
+        ("gate_extra_rate", _P),
     ]
 
 

@@ -329,6 +329,17 @@ __device__ __forceinline__ float gate_next_t(Div &dv, const DGate &G, float V, f
     return nv;
 }
 
+// The cnexp update from a given tau and inf (the tail of gate_next_t, IEEE divisions).
+__device__ __forceinline__ float gate_cnexp(float q10, float tau, float inf, float q, bool init, float dt) {
+    if (init) return inf;
+    const float tau_factor = (dt * q10) / tau;
+    const float blend = exp_f(-tau_factor);
+    float nv = (blend) * q + (1.0f - blend) * inf;
+    if (!(nv > 1e-6f)) nv = 1e-6f;
+    if (!(nv < (float)(1 - 1e-6))) nv = (float)(1 - 1e-6);
+    return nv;
+}
+
 __device__ __forceinline__ float gate_next(const DGate &G, float V, float q, bool init, float dt) {
     DivFast f;
     float nv = gate_next_t(f, G, V, q, init, dt);
@@ -384,6 +395,9 @@ struct ExtChanArgs {
     float *q;                  // gate variables, updated in place
     float2 *out;               // [NC] the compartment's (I, G) of the step
     int32_t n;
+    const float4 *gate_extra;  // per gate table entry: (form, rate, midpoint, scale) of the time
+                               // course of a rates+tau gate or the steady state of a rates+inf
+                               // gate, or null
 };
 
 // One lane of ext_channels_kernel: a compartment's channels in distribution order, each from
@@ -406,12 +420,24 @@ __device__ __forceinline__ void ext_channels(const DevTables &T, const DGate *__
             const DGate G = gates[type * GATES_PER_TYPE + gi];
             float *qp = X.q + X.q_begin[c] + gi;
             // an instantaneous gate (kind 2) is its steady state at this step's voltage
-            const bool instantaneous = ((G.forms >> 4) & 0xF) == 2;
+            const int kind = (G.forms >> 4) & 0xF;
+            const bool instantaneous = kind == 2;
             DivIeee dv;
             const float qs = instantaneous ? rate_eval_t(dv, (G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V) : *qp;
             const int inst = (G.forms >> 16) & 0xFF;
             for (int i = 0; i < inst; i++) fopen = fopen * qs;
-            *qp = instantaneous ? qs : gate_next(G, V, qs, init, dt);
+            if (kind == 3 || kind == 4) {
+                // rates with a time course (3) or a steady state (4) of their own
+                const float4 xr = X.gate_extra[type * GATES_PER_TYPE + gi];
+                const float a = rate_eval_t(dv, G.forms & 0xF, G.ar, G.am, G.as, V);
+                const float b = rate_eval_t(dv, (G.forms >> 8) & 0xFF, G.br, G.bm, G.bs, V);
+                const float x = rate_eval_t(dv, (int)xr.x, xr.y, xr.z, xr.w, V);
+                const float tau = kind == 3 ? x : 1.0f / (a + b);
+                const float inf = kind == 4 ? x : a / (a + b);
+                *qp = gate_cnexp(G.q10, tau, inf, qs, init, dt);
+            } else {
+                *qp = instantaneous ? qs : gate_next(G, V, qs, init, dt);
+            }
         }
         const float Gscaled = X.ch_g[c] * fopen;
         const float iDensity = Gscaled * (X.ch_e[c] - V) * 1e-5f;

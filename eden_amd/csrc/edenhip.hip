@@ -164,6 +164,7 @@ struct edenhip_ctx {
     float *d_x_g = nullptr, *d_x_e = nullptr, *d_x_q = nullptr;
     float2 *d_chx_ext = nullptr;
     hipFunction_t ext_rtc_fn = nullptr;  // the pre-pass compiled with the model's user rate types
+    float4 *d_gate_extra = nullptr;      // per gate table entry: a rates+tau/inf gate's third rate
     float2 *d_pk_state = nullptr;        // per Poisson spiker: (tsince, isi)
     float *d_pk_isi = nullptr;
     float2 *d_pk_window = nullptr;
@@ -449,6 +450,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     // channel pre-pass)
     const char *rates_src = d->abi_version >= 4 ? d->lems_rates_source : nullptr;
     std::vector<char> lems_rate_type((size_t)std::max(d->n_channel_types, 1), 0);
+    std::vector<float4> gate_extra((size_t)std::max(d->n_channel_types, 1) * GATES_PER_TYPE, make_float4(3.f, 0.f, 0.f, 0.f));
     std::vector<DGate> gates((size_t)std::max(d->n_channel_types, 1) * GATES_PER_TYPE);
     std::memset(gates.data(), 0, gates.size() * sizeof(DGate));
     for (int t = 0; t < d->n_channel_types; t++) {
@@ -465,10 +467,18 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             }
             const int32_t kind = (d->abi_version >= 4 && d->gate_kind) ? d->gate_kind[t * EDENHIP_MAX_GATES + gi]
                                                                        : EDENHIP_GATE_RATES;
-            if (kind != EDENHIP_GATE_RATES && kind != EDENHIP_GATE_TAU_INF && kind != EDENHIP_GATE_INSTANTANEOUS)
+            if (kind < EDENHIP_GATE_RATES || kind > EDENHIP_GATE_RATES_INF)
                 return fail(ctx, EDENHIP_E_INVALID, "unknown gate kind");
-            // instantaneous gates take the step's voltage: their channels run in the pre-pass
-            if (kind == EDENHIP_GATE_INSTANTANEOUS) lems_rate_type[(size_t)t] = 1;
+            // instantaneous gates take the step's voltage, rates with a third rate take it from
+            // the extra table: their channels run in the pre-pass
+            if (kind >= EDENHIP_GATE_INSTANTANEOUS) lems_rate_type[(size_t)t] = 1;
+            if (kind == EDENHIP_GATE_RATES_TAU || kind == EDENHIP_GATE_RATES_INF) {
+                if (!d->gate_extra_rate) return fail(ctx, EDENHIP_E_INVALID, "a rates+tau/inf gate needs gate_extra_rate");
+                const edenhip_rate &x = d->gate_extra_rate[t * EDENHIP_MAX_GATES + gi];
+                if (x.form < 0 || x.form > EDENHIP_RATE_LEMS || (x.form == EDENHIP_RATE_LEMS && !rates_src))
+                    return fail(ctx, EDENHIP_E_INVALID, "unknown extra rate form");
+                gate_extra[(size_t)t * GATES_PER_TYPE + gi] = make_float4((float)x.form, x.rate, x.midpoint, x.scale);
+            }
             DGate &G = gates[(size_t)t * GATES_PER_TYPE + gi];
             G.forms = g.alpha.form | (kind << 4) | (g.beta.form << 8) | (g.instances << 16) | (ct.n_gates << 24);
             G.q10 = g.q10;
@@ -918,6 +928,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(upload(ctx, &ctx->d_x_q, x_q.empty() ? std::vector<float>(1, 0.0f) : x_q));
         TRY(alloc_zero(ctx, &ctx->d_chx_ext, (size_t)NC));
         ctx->T.chx_ext = ctx->d_chx_ext;
+        TRY(upload(ctx, &ctx->d_gate_extra, gate_extra));
     }
 
     // recorded gate variables: point cells keep theirs in T.q (NQ slots per compartment in
@@ -1541,7 +1552,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             if (ctx->n_xcomp > 0) {
                 // channels of the compartments beyond the kernels' slots, from this step's V
                 ExtChanArgs xa{ctx->d_x_comp, ctx->d_x_chb, ctx->d_x_type, ctx->d_x_g, ctx->d_x_e, ctx->d_x_qb,
-                               ctx->d_x_q, ctx->d_chx_ext, ctx->n_xcomp};
+                               ctx->d_x_q, ctx->d_chx_ext, ctx->n_xcomp, ctx->d_gate_extra};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &ctx->d_gates, &xa, &st, &dt};

@@ -130,6 +130,8 @@ class Model:
     # rates, 1 tau/inf), or None for rates everywhere
     gate_kind: np.ndarray = None
     lems_rates: object = None          # eden_amd.lems_rates.LemsRates: user gate rate ComponentTypes, or None
+    gate_extra_rate: list = None       # per channel type, per gate slot: the third rate (N.Rate) of a
+                                       # rates+tau / rates+inf gate, or None
     name: str = "model"
 
     def __post_init__(self):
@@ -260,7 +262,7 @@ class Model:
         for name, _ in N.ModelDesc._fields_:
             if name in ("abi_version", "dt", "t_final", "n_cells", "n_comps", "n_channel_types", "channel_types",
                         "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems", "lems_cells",
-                        "lems_rates_source"):
+                        "lems_rates_source", "gate_extra_rate"):
                 continue
             setattr(d, name, N.ptr(getattr(self, name)))
         if self.lems is not None:
@@ -317,6 +319,12 @@ class Model:
             self._lc_desc = N.LemsCellsDesc(len(LC.params), a[5], N.ptr(a[0]), N.ptr(a[1]), N.ptr(a[2]), N.ptr(a[3]),
                                             N.ptr(a[4]), N.ptr(off), len(prm))
             d.lems_cells = C.pointer(self._lc_desc)
+        if self.gate_extra_rate is not None:
+            if len(self.gate_extra_rate) != len(self.channel_types) * N.MAX_GATES:
+                raise ValueError("gate_extra_rate: MAX_GATES entries per channel type")
+            self._gx = (N.Rate * len(self.gate_extra_rate))(
+                *[r if r is not None else N.Rate(N.RATE_FIXED, 0.0, 0.0, 0.0) for r in self.gate_extra_rate])
+            d.gate_extra_rate = C.cast(self._gx, C.c_void_p)
         if self.lems_rates is not None:
             self._rates_src = self.lems_rates.source_hip.encode()
             d.lems_rates_source = self._rates_src

@@ -9,7 +9,8 @@ set covers:
                     the including file; NeuroML's core type files are skipped
                     (eden/NeuroML.cpp:9813-9891)
   ion channels      ionChannelHH / ionChannel with gateHHrates (HHExpRate, HHExpLinearRate,
-                    HHSigmoidRate), gateHHtauInf, gateHHratesTauInf, gateHHInstantaneous (fixedTimeCourse,
+                    HHSigmoidRate), gateHHtauInf, gateHHratesTauInf, gateHHratesTau,
+                    gateHHratesInf, gateHHInstantaneous (fixedTimeCourse,
                     HHExpVariable, HHExpLinearVariable, HHSigmoidVariable), user rate, time
                     course and steady state ComponentTypes (eden_amd.lems_rates), q10Settings
                     (q10Fixed, q10ExpTemp); channels without gates
@@ -317,7 +318,8 @@ _VARIABLE_FORMS = {"HHExpVariable": N.RATE_EXP, "HHExpLinearVariable": N.RATE_EX
 # gate elements by kind: alpha/beta rates, or tau and inf given (a gateHHratesTauInf's rates
 # play no part in its dynamics, eden/Eden.cpp:5282-5383)
 _GATE_KINDS = {"gateHHrates": N.GATE_RATES, "gateHHtauInf": N.GATE_TAU_INF, "gateHHratesTauInf": N.GATE_TAU_INF,
-               "gateHHInstantaneous": N.GATE_INSTANTANEOUS}
+               "gateHHInstantaneous": N.GATE_INSTANTANEOUS, "gateHHratesTau": N.GATE_RATES_TAU,
+               "gateHHratesInf": N.GATE_RATES_INF}
 
 
 def _steady_state(el, what, user=None):
@@ -348,7 +350,7 @@ def _channel(el, temperature, user=None):
     """(channel type, gate kinds) of an ion channel element; `user(element, base type, what)`
     turns a user rate ComponentType into an EDENHIP_RATE_LEMS rate."""
     cid = el.get("id")
-    gates, kinds = [], []
+    gates, kinds, extras = [], [], []
     for g in el:
         t = _tag(g)
         if t in ("notes", "annotation"):
@@ -363,9 +365,10 @@ def _channel(el, temperature, user=None):
         q10 = 1.0
         for c in g:
             ct = _tag(c)
-            if ct == "forwardRate" and kind == N.GATE_RATES:
+            with_rates = kind in (N.GATE_RATES, N.GATE_RATES_TAU, N.GATE_RATES_INF)
+            if ct == "forwardRate" and with_rates:
                 fw = _rate(c, f"channel {cid} gate {g.get('id')} forward", user)
-            elif ct == "reverseRate" and kind == N.GATE_RATES:
+            elif ct == "reverseRate" and with_rates:
                 rv = _rate(c, f"channel {cid} gate {g.get('id')} reverse", user)
             elif ct == "timeCourse":
                 tau = _time_course(c, f"channel {cid} gate {g.get('id')} timeCourse", user)
@@ -383,10 +386,15 @@ def _channel(el, temperature, user=None):
                     q10 = f ** ((temperature - t0) / 10.0)
                 else:
                     raise NeuroMLError(f"channel {cid}: q10Settings type '{qt}' is not supported")
-        if kind == N.GATE_RATES:
+        extra = None
+        if kind in (N.GATE_RATES, N.GATE_RATES_TAU, N.GATE_RATES_INF):
             if fw is None or rv is None:
                 raise NeuroMLError(f"channel {cid} gate {g.get('id')}: forwardRate and reverseRate are required")
             gates.append((inst, fw, rv, _f32(q10)))
+            extra = tau if kind == N.GATE_RATES_TAU else inf if kind == N.GATE_RATES_INF else None
+            if kind != N.GATE_RATES and extra is None:
+                raise NeuroMLError(f"channel {cid} gate {g.get('id')}: "
+                                   f"{'timeCourse' if kind == N.GATE_RATES_TAU else 'steadyState'} is required")
         elif kind == N.GATE_INSTANTANEOUS:
             if inf is None:
                 raise NeuroMLError(f"channel {cid} gate {g.get('id')}: steadyState is required")
@@ -396,9 +404,11 @@ def _channel(el, temperature, user=None):
                 raise NeuroMLError(f"channel {cid} gate {g.get('id')}: timeCourse and steadyState are required")
             gates.append((inst, tau, inf, _f32(q10)))
         kinds.append(kind)
+        extras.append(extra)
     if len(gates) > N.MAX_GATES:
         raise NeuroMLError(f"channel {cid}: more than {N.MAX_GATES} gates")
-    return channel_type(gates), kinds + [N.GATE_RATES] * (N.MAX_GATES - len(kinds))
+    pad = N.MAX_GATES - len(kinds)
+    return channel_type(gates), kinds + [N.GATE_RATES] * pad, extras + [None] * pad
 
 
 # ---------------------------------------------------------------- cells
@@ -1132,7 +1142,7 @@ def read(path, rng_seed=None):
     statements = _custom_setup(doc, sim)
 
     # channel types in document order, then the cells the network uses
-    channel_index, channel_types, gate_ids, gate_kinds = {}, [], {}, []
+    channel_index, channel_types, gate_ids, gate_kinds, gate_extras = {}, [], {}, [], []
     rate_entries, rate_types = [], {}
 
     def user_rate(el, base, what):
@@ -1160,9 +1170,10 @@ def read(path, rng_seed=None):
     for role, i in doc.order:
         if role == "channel":
             channel_index[i] = len(channel_types)
-            ct_, kinds_ = _channel(doc.items[i][1], temperature, user_rate)
+            ct_, kinds_, extras_ = _channel(doc.items[i][1], temperature, user_rate)
             channel_types.append(ct_)
             gate_kinds += kinds_
+            gate_extras += extras_
             gate_ids[i] = [g.get("id") for g in doc.items[i][1] if _tag(g) not in ("notes", "annotation")]
     cells = {}
 
@@ -1716,6 +1727,7 @@ def read(path, rng_seed=None):
         record_gate=np.array(rec_gate, np.int32) if any(j >= 0 for j in rec_gate) else None,
         lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
         gate_kind=np.array(gate_kinds, np.int32) if any(gate_kinds) else None,
+        gate_extra_rate=gate_extras if any(x is not None for x in gate_extras) else None,
         lems_rates=_rates_or_none(rate_entries, temperature),
         syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
         syn_spike_list=syn_spike_list, spike_times=spike_times,

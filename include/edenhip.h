@@ -70,7 +70,12 @@ enum {
     /* (gateHHInstantaneous) no state: the gate's value in a step is its steady state, held in
      * its `beta`, at the step's voltage (eden/Eden.cpp:5269-5281); compartments with such
      * gates run their channels in the channel pre-pass (one launch per step) */
-    EDENHIP_GATE_INSTANTANEOUS = 2
+    EDENHIP_GATE_INSTANTANEOUS = 2,
+    /* (gateHHratesTau, gateHHratesInf) the rates give inf (resp. tau) and the third rate, in
+     * edenhip_model_desc.gate_extra_rate, gives tau (resp. inf); channels with such gates run
+     * in the channel pre-pass */
+    EDENHIP_GATE_RATES_TAU = 3,
+    EDENHIP_GATE_RATES_INF = 4
 };
 
 typedef struct edenhip_rate {
@@ -292,6 +297,11 @@ typedef struct edenhip_model_desc {
      * NULL. edenhip_create() compiles it with hiprtc into the channel pre-pass; compartments
      * with such gates run their channels there (one launch per step). */
     const char *lems_rates_source;
+
+    /* (abi_version >= 4) [n_channel_types * EDENHIP_MAX_GATES] the time course of an
+     * EDENHIP_GATE_RATES_TAU gate or the steady state of an EDENHIP_GATE_RATES_INF gate, or
+     * NULL when there are none */
+    const edenhip_rate *gate_extra_rate;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

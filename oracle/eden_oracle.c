@@ -323,6 +323,11 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
                 real beta = rate_value(&gt->beta, Vcomp);
                 real tau = tau_inf ? alpha : (real)1 / (alpha + beta);
                 real inf = tau_inf ? beta : alpha / (alpha + beta);
+                if (kind == EDENHIP_GATE_RATES_TAU || kind == EDENHIP_GATE_RATES_INF) {
+                    /* rates with a time course or a steady state of their own */
+                    const real x = rate_value(&d->gate_extra_rate[d->chan_type[ch] * EDENHIP_MAX_GATES + gi], Vcomp);
+                    if (kind == EDENHIP_GATE_RATES_TAU) tau = x; else inf = x;
+                }
                 real qn;
                 if (kind == EDENHIP_GATE_INSTANTANEOUS) {
                     /* no state: the steady state at this step's voltage (eden/Eden.cpp:5269-5281) */

@@ -90,3 +90,18 @@ def test_instantaneous_gates_bit_exact(tmp_path, opts):
         tr = sim.run_all()
     assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))
     assert float(np.ptp(tr[:, 0])) > 1.0
+
+
+@pytest.mark.parametrize("kind", ["gateHHratesTau", "gateHHratesInf"])
+def test_rates_with_a_third_rate_bit_exact(tmp_path, kind):
+    from test_tau_inf_gates_cpu import _model as ti_model
+    rates = ('<forwardRate type="HHSigmoidRate" rate="0.5per_ms" midpoint="-50mV" scale="5mV"/>'
+             '<reverseRate type="HHExpLinearRate" rate="0.1per_ms" midpoint="-55mV" scale="10mV"/>')
+    third = ('<timeCourse type="fixedTimeCourse" tau="3ms"/>' if kind == "gateHHratesTau" else
+             '<steadyState type="HHSigmoidVariable" rate="1" midpoint="-50mV" scale="5mV"/>')
+    m = ti_model(tmp_path, f'<{kind} id="n" instances="2">{rates}{third}</{kind}>', record_q=False, n_cells=2)
+    print(f"[tau-inf] {kind}", flush=True)
+    tr_r, _, _, _ = oracle.run(m)
+    with Simulation(m) as sim:
+        tr = sim.run_all()
+    assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))

@@ -7,7 +7,7 @@ CPU: the reader's tables and the oracle's dynamics.
   steady state rate / (1 + exp((mid - V) / scale)) of the step's start voltage, evaluated here
   in float32 from the recorded voltage trace (eden/Eden.cpp:5282-5383, 5236-5259).
 * gateHHratesTauInf reads like gateHHtauInf (its rates play no part); gateHHratesTau and
-  gateHHratesInf are refused with their names.
+  gateHHratesInf take their third rate in place of the one their rates would give.
 * gateHHInstantaneous (eden/Eden.cpp:5269-5281): no state; in each step the channel's open
   fraction is the steady state at that step's own voltage. The first step from rest, by hand:
   V1 = (V0 + dt (I + V0 G) / C) / (1 + dt G / C) with the channel's I = g x(V0)^k (E - V0) 1e-5 A.
@@ -88,10 +88,31 @@ def test_rates_tau_inf_reads_as_tau_inf(tmp_path):
     assert np.array_equal(tr1, tr2)
 
 
-@pytest.mark.parametrize("gate", ["gateHHratesTau", "gateHHratesInf"])
-def test_other_gate_kinds_refused(tmp_path, gate):
-    with pytest.raises(neuroml.NeuroMLError, match=gate):
-        _model(tmp_path, f'<{gate} id="n" instances="1"/>')
+def test_rates_with_a_third_rate(tmp_path):
+    """gateHHratesTau takes tau from its time course and inf from its rates, gateHHratesInf inf
+    from its steady state and tau from its rates (eden/Eden.cpp:5282-5383). Two sigmoid rates
+    summing to 0.5 per ms at every voltage give tau = 2 ms and inf = the sigmoid of rate 1: a
+    rates+tau gate with a 2 ms time course and a rates+inf gate with that sigmoid as its steady
+    state reproduce the plain rates gate to float rounding; a 20 ms time course changes the
+    trace."""
+    rates = ('<forwardRate type="HHSigmoidRate" rate="0.5per_ms" midpoint="-50mV" scale="5mV"/>'
+             '<reverseRate type="HHSigmoidRate" rate="0.5per_ms" midpoint="-50mV" scale="-5mV"/>')
+    # alpha + beta = 0.5 per ms at every V (the two sigmoids sum to 1), so tau = 2 ms, and
+    # inf = alpha / 0.5 = the sigmoid of rate 1
+    plain = _model(tmp_path, f'<gateHHrates id="n" instances="2">{rates}</gateHHrates>')
+    with_tau = _model(tmp_path, f'<gateHHratesTau id="n" instances="2">{rates}'
+                                '<timeCourse type="fixedTimeCourse" tau="2ms"/></gateHHratesTau>')
+    with_inf = _model(tmp_path, f'<gateHHratesInf id="n" instances="2">{rates}'
+                                '<steadyState type="HHSigmoidVariable" rate="1" midpoint="-50mV" scale="5mV"/>'
+                                '</gateHHratesInf>')
+    slow = _model(tmp_path, f'<gateHHratesTau id="n" instances="2">{rates}'
+                            '<timeCourse type="fixedTimeCourse" tau="20ms"/></gateHHratesTau>')
+    assert with_tau.gate_kind.tolist()[4] == N.GATE_RATES_TAU and with_inf.gate_kind.tolist()[4] == N.GATE_RATES_INF
+    assert with_tau.gate_extra_rate[4].rate == 2000.0
+    t0 = oracle.run(plain)[0]
+    for m in (with_tau, with_inf):
+        np.testing.assert_allclose(oracle.run(m)[0], t0, rtol=0, atol=2e-4)
+    assert float(np.abs(oracle.run(slow)[0][:, 1] - t0[:, 1]).max()) > 1e-2
 
 
 def test_instantaneous_gate_first_step_by_hand(tmp_path):
