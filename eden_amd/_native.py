@@ -77,6 +77,7 @@ class ModelDesc(C.Structure):
         ("gate_kind", _P),
         ("lems_rates_source", C.c_char_p),
         ("gate_extra_rate", _P),
+        ("chan_scaling", _P),
     ]
 
 

@@ -395,6 +395,7 @@ struct ExtChanArgs {
     float *q;                  // gate variables, updated in place
     float2 *out;               // [NC] the compartment's (I, G) of the step
     int32_t n;
+    const float *scaling;      // per channel type: conductance_scaling, or null (1)
     const float4 *gate_extra;  // per gate table entry: (form, rate, midpoint, scale) of the time
                                // course of a rates+tau gate or the steady state of a rates+inf
                                // gate, or null
@@ -415,7 +416,7 @@ __device__ __forceinline__ void ext_channels(const DevTables &T, const DGate *__
     for (int32_t c = X.ch_begin[idx]; c < X.ch_begin[idx + 1]; c++) {
         const int32_t type = X.ch_type[c];
         const int ng = gates[type * GATES_PER_TYPE].forms >> 24;
-        float fopen = 1.0f;
+        float fopen = X.scaling ? X.scaling[type] : 1.0f;   // conductance_scaling
         for (int gi = 0; gi < ng; gi++) {
             const DGate G = gates[type * GATES_PER_TYPE + gi];
             float *qp = X.q + X.q_begin[c] + gi;

@@ -165,6 +165,7 @@ struct edenhip_ctx {
     float2 *d_chx_ext = nullptr;
     hipFunction_t ext_rtc_fn = nullptr;  // the pre-pass compiled with the model's user rate types
     float4 *d_gate_extra = nullptr;      // per gate table entry: a rates+tau/inf gate's third rate
+    float *d_chan_scaling = nullptr;     // per channel type: conductance_scaling
     float2 *d_pk_state = nullptr;        // per Poisson spiker: (tsince, isi)
     float *d_pk_isi = nullptr;
     float2 *d_pk_window = nullptr;
@@ -456,6 +457,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     for (int t = 0; t < d->n_channel_types; t++) {
         const edenhip_channel_type &ct = d->channel_types[t];
         if (ct.n_gates < 0 || ct.n_gates > GATES_PER_TYPE) return fail(ctx, EDENHIP_E_LIMIT, "channel type with more than 4 gates");
+        // a conductance scaling other than 1: the channel runs in the pre-pass
+        if (d->abi_version >= 4 && d->chan_scaling && d->chan_scaling[t] != 1.0f) lems_rate_type[(size_t)t] = 1;
         for (int gi = 0; gi < ct.n_gates; gi++) {
             const edenhip_gate &g = ct.gates[gi];
             if (g.instances < 0 || g.instances > 255) return fail(ctx, EDENHIP_E_INVALID, "gate instances out of range");
@@ -929,6 +932,9 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         TRY(alloc_zero(ctx, &ctx->d_chx_ext, (size_t)NC));
         ctx->T.chx_ext = ctx->d_chx_ext;
         TRY(upload(ctx, &ctx->d_gate_extra, gate_extra));
+        if (d->abi_version >= 4 && d->chan_scaling)
+            TRY(upload(ctx, &ctx->d_chan_scaling,
+                       std::vector<float>(d->chan_scaling, d->chan_scaling + std::max(d->n_channel_types, 1))));
     }
 
     // recorded gate variables: point cells keep theirs in T.q (NQ slots per compartment in
@@ -1552,7 +1558,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
             if (ctx->n_xcomp > 0) {
                 // channels of the compartments beyond the kernels' slots, from this step's V
                 ExtChanArgs xa{ctx->d_x_comp, ctx->d_x_chb, ctx->d_x_type, ctx->d_x_g, ctx->d_x_e, ctx->d_x_qb,
-                               ctx->d_x_q, ctx->d_chx_ext, ctx->n_xcomp, ctx->d_gate_extra};
+                               ctx->d_x_q, ctx->d_chx_ext, ctx->n_xcomp, ctx->d_chan_scaling, ctx->d_gate_extra};
                 int64_t st = a;
                 float dt = ctx->dt;
                 void *args[] = {&T, &ctx->d_gates, &xa, &st, &dt};

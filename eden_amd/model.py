@@ -130,6 +130,7 @@ class Model:
     # rates, 1 tau/inf), or None for rates everywhere
     gate_kind: np.ndarray = None
     lems_rates: object = None          # eden_amd.lems_rates.LemsRates: user gate rate ComponentTypes, or None
+    chan_scaling: np.ndarray = None    # per channel type: conductance_scaling (q10ConductanceScaling), or None
     gate_extra_rate: list = None       # per channel type, per gate slot: the third rate (N.Rate) of a
                                        # rates+tau / rates+inf gate, or None
     name: str = "model"
@@ -262,7 +263,7 @@ class Model:
         for name, _ in N.ModelDesc._fields_:
             if name in ("abi_version", "dt", "t_final", "n_cells", "n_comps", "n_channel_types", "channel_types",
                         "n_syn_types", "syn_types", "n_record", "n_event_cols", "lems", "lems_cells",
-                        "lems_rates_source", "gate_extra_rate"):
+                        "lems_rates_source", "gate_extra_rate", "chan_scaling"):
                 continue
             setattr(d, name, N.ptr(getattr(self, name)))
         if self.lems is not None:
@@ -319,6 +320,11 @@ class Model:
             self._lc_desc = N.LemsCellsDesc(len(LC.params), a[5], N.ptr(a[0]), N.ptr(a[1]), N.ptr(a[2]), N.ptr(a[3]),
                                             N.ptr(a[4]), N.ptr(off), len(prm))
             d.lems_cells = C.pointer(self._lc_desc)
+        if self.chan_scaling is not None:
+            self._cs = np.ascontiguousarray(self.chan_scaling, np.float32)
+            if self._cs.shape != (len(self.channel_types),):
+                raise ValueError("chan_scaling: one entry per channel type")
+            d.chan_scaling = N.ptr(self._cs)
         if self.gate_extra_rate is not None:
             if len(self.gate_extra_rate) != len(self.channel_types) * N.MAX_GATES:
                 raise ValueError("gate_extra_rate: MAX_GATES entries per channel type")

@@ -351,9 +351,19 @@ def _channel(el, temperature, user=None):
     turns a user rate ComponentType into an EDENHIP_RATE_LEMS rate."""
     cid = el.get("id")
     gates, kinds, extras = [], [], []
+    scaling = 1.0
     for g in el:
         t = _tag(g)
         if t in ("notes", "annotation"):
+            continue
+        if t == "q10ConductanceScaling":
+            # conductance_scaling = q10Factor^((T - experimentalTemp) / 10), the start of the
+            # open fraction (eden/Eden.cpp:5700-5721)
+            if temperature is None:
+                raise NeuroMLError(f"channel {cid}: q10ConductanceScaling needs the network's temperature")
+            f = convert(g.get("q10Factor"), None, "q10Factor")
+            t0 = convert(g.get("experimentalTemp"), "temperature", "experimentalTemp")
+            scaling = f ** ((temperature - t0) / 10.0)
             continue
         gtype = g.get("type") if t == "gate" else t
         if gtype not in _GATE_KINDS:
@@ -408,7 +418,7 @@ def _channel(el, temperature, user=None):
     if len(gates) > N.MAX_GATES:
         raise NeuroMLError(f"channel {cid}: more than {N.MAX_GATES} gates")
     pad = N.MAX_GATES - len(kinds)
-    return channel_type(gates), kinds + [N.GATE_RATES] * pad, extras + [None] * pad
+    return channel_type(gates), kinds + [N.GATE_RATES] * pad, extras + [None] * pad, _f32(scaling)
 
 
 # ---------------------------------------------------------------- cells
@@ -1142,7 +1152,7 @@ def read(path, rng_seed=None):
     statements = _custom_setup(doc, sim)
 
     # channel types in document order, then the cells the network uses
-    channel_index, channel_types, gate_ids, gate_kinds, gate_extras = {}, [], {}, [], []
+    channel_index, channel_types, gate_ids, gate_kinds, gate_extras, chan_scaling = {}, [], {}, [], [], []
     rate_entries, rate_types = [], {}
 
     def user_rate(el, base, what):
@@ -1170,11 +1180,12 @@ def read(path, rng_seed=None):
     for role, i in doc.order:
         if role == "channel":
             channel_index[i] = len(channel_types)
-            ct_, kinds_, extras_ = _channel(doc.items[i][1], temperature, user_rate)
+            ct_, kinds_, extras_, scaling_ = _channel(doc.items[i][1], temperature, user_rate)
+            chan_scaling.append(scaling_)
             channel_types.append(ct_)
             gate_kinds += kinds_
             gate_extras += extras_
-            gate_ids[i] = [g.get("id") for g in doc.items[i][1] if _tag(g) not in ("notes", "annotation")]
+            gate_ids[i] = [g.get("id") for g in doc.items[i][1] if _tag(g) not in ("notes", "annotation", "q10ConductanceScaling")]
     cells = {}
 
     def get_cell(cid, what):
@@ -1728,6 +1739,7 @@ def read(path, rng_seed=None):
         lems=lems_tables, lems_cells=point_types, cell_lems_type=cell_point_type,
         gate_kind=np.array(gate_kinds, np.int32) if any(gate_kinds) else None,
         gate_extra_rate=gate_extras if any(x is not None for x in gate_extras) else None,
+        chan_scaling=np.array(chan_scaling, np.float32) if any(x != 1.0 for x in chan_scaling) else None,
         lems_rates=_rates_or_none(rate_entries, temperature),
         syn_param_index=syn_pidx, syn_param_vecs=syn_vecs, cell_param_index=cell_pidx, cell_param_vecs=cell_vecs,
         syn_spike_list=syn_spike_list, spike_times=spike_times,

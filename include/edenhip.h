@@ -302,6 +302,11 @@ typedef struct edenhip_model_desc {
      * EDENHIP_GATE_RATES_TAU gate or the steady state of an EDENHIP_GATE_RATES_INF gate, or
      * NULL when there are none */
     const edenhip_rate *gate_extra_rate;
+
+    /* (abi_version >= 4) [n_channel_types] each channel type's conductance_scaling, the start
+     * of its open fraction (q10ConductanceScaling, eden/Eden.cpp:5700-5721), or NULL: 1
+     * everywhere. Channels whose scaling is not 1 run in the channel pre-pass. */
+    const float *chan_scaling;
 } edenhip_model_desc;
 
 /* Launch-shape choices. Zero-initialise for defaults. */

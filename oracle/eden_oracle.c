@@ -308,7 +308,8 @@ static void do_cell(const edenhip_model_desc *d, ostate *s, int32_t cell, int no
         /* ion channel distributions, in distribution order */
         for (int64_t ch = d->comp_chan_begin[g]; ch < d->comp_chan_begin[g + 1]; ch++) {
             const edenhip_channel_type *ct = &d->channel_types[d->chan_type[ch]];
-            real fopen = 1; /* conductance_scaling */
+            /* conductance_scaling (eden/Eden.cpp:5700-5721) */
+            real fopen = (d->abi_version >= 4 && d->chan_scaling) ? (real)d->chan_scaling[d->chan_type[ch]] : (real)1;
             int64_t qb = s->gate_begin[ch];
             for (int gi = 0; gi < ct->n_gates; gi++) {
                 const edenhip_gate *gt = &ct->gates[gi];

@@ -105,3 +105,14 @@ def test_rates_with_a_third_rate_bit_exact(tmp_path, kind):
     with Simulation(m) as sim:
         tr = sim.run_all()
     assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))
+
+
+def test_conductance_scaling_bit_exact(tmp_path):
+    from test_tau_inf_gates_cpu import TAU_INF, _model as ti_model
+    sc = '<q10ConductanceScaling q10Factor="3" experimentalTemp="-3.7 degC"/>'
+    m = ti_model(tmp_path, sc + TAU_INF.format(q10=""), record_q=False, n_cells=2)
+    print("[tau-inf] q10ConductanceScaling", flush=True)
+    tr_r, _, _, _ = oracle.run(m)
+    with Simulation(m) as sim:
+        tr = sim.run_all()
+    assert np.array_equal(tr, tr_r), float(np.nanmax(np.abs(tr - tr_r)))

@@ -155,3 +155,19 @@ def test_instantaneous_gate_takes_the_step_s_own_voltage(tmp_path):
     want, lagged = step(V1, x(V1)), step(V1, x(-65.0))
     assert tr[2, 0] == pytest.approx(want, abs=2e-5)
     assert abs(lagged - want) > 1e-4
+
+
+def test_q10_conductance_scaling(tmp_path):
+    """q10ConductanceScaling starts the channel's open fraction at q10Factor^((T - T0) / 10)
+    (eden/Eden.cpp:5700-5721): with T - T0 = 10 K and a factor 3 the channel acts as one of
+    three times its density (to float rounding of the two products)."""
+    sc = '<q10ConductanceScaling q10Factor="3" experimentalTemp="-3.7 degC"/>'
+    m = _model(tmp_path, sc + TAU_INF.format(q10=""), record_q=False)
+    assert m.chan_scaling.tolist() == [1.0, pytest.approx(3.0, rel=1e-6)]
+    tr = oracle.run(m)[0]
+    src = (tmp_path / "ti.xml").read_text().replace(sc, "").replace('condDensity="2 mS_per_cm2"', 'condDensity="6 mS_per_cm2"')
+    (tmp_path / "ti3.xml").write_text(src)
+    tr3 = oracle.run(neuroml.read(str(tmp_path / "ti3.xml"))[0])[0]
+    np.testing.assert_allclose(tr, tr3, rtol=0, atol=1e-3)
+    tr1 = oracle.run(_model(tmp_path, TAU_INF.format(q10=""), record_q=False))[0]
+    assert float(np.abs(tr1 - tr).max()) > 0.1
