@@ -273,17 +273,18 @@ __global__ void zero_i32_kernel(int32_t *p, int n) {
 // Sources of recorded gate and state variables: the kind in the top bits, the index below.
 constexpr int64_t REC_TREE_GATE = (int64_t)1 << 60;   // the tree cells' gate table
 constexpr int64_t REC_CELL_STATE = (int64_t)2 << 60;  // the point neurons' state table
+constexpr int64_t REC_EXT_GATE = (int64_t)3 << 60;    // the channel pre-pass's gate variables
 constexpr int64_t REC_INDEX = ((int64_t)1 << 60) - 1; // (kind 0: the point cells' gates, T.q)
 
 // Recorded gate and state variables into trace row `row`: column col[i] takes the value src[i]
 // names.
 __global__ void record_gates_kernel(DevTables T, const float *__restrict__ gq, const float *__restrict__ cs,
-                                    const int32_t *__restrict__ col, const int64_t *__restrict__ src, int32_t n,
-                                    int64_t row) {
+                                    const float *__restrict__ xq, const int32_t *__restrict__ col,
+                                    const int64_t *__restrict__ src, int32_t n, int64_t row) {
     const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
     const int64_t k = src[i] & REC_INDEX, kind = src[i] >> 60;
-    trace_at(T, col[i], row) = kind == 0 ? T.q[k] : kind == 1 ? gq[k] : cs[k];
+    trace_at(T, col[i], row) = kind == 0 ? T.q[k] : kind == 1 ? gq[k] : kind == 2 ? cs[k] : xq[k];
 }
 
 // Streams come from a per-device pool created once per process and kept until it ends:
@@ -523,6 +524,7 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
     std::vector<int32_t> x_comp, x_chb{0}, x_type, x_qb;
     std::vector<float> x_g, x_e, x_q;
     std::vector<char> ext_comp(NC, 0);
+    std::vector<int64_t> x_first_q(NC, -1), x_nq(NC, 0);   // per such compartment: its gate variables
     for (int64_t g = 0; g < NC; g++) {
         V[g] = d->v0[g]; C[g] = d->capacitance[g]; A[g] = d->area[g]; invrc[g] = d->inv_rc_axial[g];
         vthr[g] = d->v_threshold ? d->v_threshold[g] : NAN;
@@ -539,6 +541,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             if (acell_slot[g] >= 0) return fail(ctx, EDENHIP_E_INVALID, "a point neuron takes no channel distributions");
             ext_comp[g] = 1;
             x_comp.push_back((int32_t)g);
+            x_first_q[g] = (int64_t)x_q.size();
+            x_nq[g] = n_gvars;
             for (int64_t ch = cb; ch < ce; ch++) {
                 const int ty = d->chan_type[ch];
                 const edenhip_channel_type &ct = d->channel_types[ty];
@@ -639,10 +643,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
             rec_var[r] = v;
             continue;
         }
-        if (ext_comp[g] && j >= 0)
-            return fail(ctx, EDENHIP_E_INVALID, "record_gate: gate variables of a compartment with more than 4 channel "
-                                                "distributions or gate variables are not recorded");
-        if (j < -1 || j >= (int32_t)((gmeta[g] >> 24) & 7u))
+        const int64_t nq_here = ext_comp[g] ? x_nq[g] : (int64_t)((gmeta[g] >> 24) & 7u);
+        if (j < -1 || j >= nq_here)
             return fail(ctx, EDENHIP_E_INVALID, "record_gate: -1 or a gate variable of the compartment");
         rec_var[r] = j;
         if (j >= 0) continue;
@@ -947,6 +949,8 @@ static int build(edenhip_ctx *ctx, const edenhip_model_desc *d) {
         grec_col[k] = ctx->n_dev_cols - (int32_t)grec.size() + (int32_t)k;
         if (acell_slot[g] >= 0)
             grec_src[k] = REC_CELL_STATE | (j * ctx->n_acells + acell_slot[g]);
+        else if (ext_comp[g])
+            grec_src[k] = REC_EXT_GATE | (x_first_q[g] + j);
         else if (gq_first[g] >= 0)
             grec_src[k] = REC_TREE_GATE | (gq_first[g] + j);
         else
@@ -1526,7 +1530,7 @@ static int launch_chunk(edenhip_ctx *ctx, int64_t s0, int64_t s1, int64_t row0) 
                 const int64_t row = a - 1 - row0;
                 hipLaunchKernelGGL(record_gates_kernel, dim3((ctx->n_grec + 255) / 256), dim3(256), 0, ctx->stream, T,
                                    (const float *)ctx->d_gq, (const float *)ctx->d_acell_state,
-                                   (const int32_t *)ctx->d_grec_col, (const int64_t *)ctx->d_grec_src, ctx->n_grec,
+                                   (const float *)ctx->d_x_q, (const int32_t *)ctx->d_grec_col, (const int64_t *)ctx->d_grec_src, ctx->n_grec,
                                    row);
                 HIPCHECK(ctx, hipGetLastError());
                 nl++;
